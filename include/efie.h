@@ -1,0 +1,149 @@
+/*
+ * efie.h -- C ABI of libefie_hip.so, the MI355X (gfx950) implementation of
+ * the method-of-moments EFIE hot path of mahmut26/Matlab-cuda-efief-vs-cpu:
+ * RWG impedance-matrix fill Z, port excitation V, per-frequency complex LU
+ * solve and the Zin/S11 epilogue of the microstrip-patch sweep.
+ *
+ * Plain C types only.  Conventions are the reference MEX conventions:
+ *   - index tables are 1-based int32 (0 in plusTri/minusTri means "no
+ *     triangle", patch_sweep_fastquad_cuda.cu:375,379),
+ *   - matrices are column-major ([Nv x 3] V, [Nt x 3] F, [Ne x 2] edge,
+ *     [Ne x 3] rp/rm, [Ne x Ne] Z),
+ *   - complex numbers are interleaved (re, im) doubles.
+ * Every entry point returns an efie_status; on failure efie_last_error()
+ * and efie_last_error_id() give the message and the reference MEX error
+ * identifier ("mex:port", "mex:rhs", ...), and no output is written.
+ * A context is bound to one HIP device and one stream; calls on one context
+ * are not re-entrant across threads.
+ */
+#ifndef EFIE_H
+#define EFIE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EFIE_ABI_VERSION 1
+
+typedef enum {
+    EFIE_OK = 0,
+    EFIE_E_ARGS = 1,     /* bad argument (sizes, ranges, options)            */
+    EFIE_E_MESH = 2,     /* mesh / RWG tables inconsistent                   */
+    EFIE_E_PORT = 3,     /* port edge out of range                           */
+    EFIE_E_RHS = 4,      /* spread radius leaves a layer without weight      */
+    EFIE_E_SINGULAR = 5, /* exactly zero pivot in the LU                     */
+    EFIE_E_HIP = 6,      /* HIP runtime error                                */
+    EFIE_E_NOMEM = 7,    /* device or host allocation failed                 */
+    EFIE_E_STATE = 8     /* mesh or RWG tables not set on the context        */
+} efie_status;
+
+enum { EFIE_MODEL_QUADRATURE = 0, EFIE_MODEL_CENTROID = 1 };
+enum { EFIE_PREC_FP64 = 0, EFIE_PREC_MIXED = 1 };
+
+/* Numerical options.  The first seven fields carry the names and meaning of
+ * the reference `opts` struct (main_patch_efie_rwg.m:281-289); quad_order 7
+ * selects a 7-point degree-5 rule that the reference does not have.
+ * model CENTROID reproduces the reference kernel_Z_centroid
+ * (patch_sweep_cuda.cu:94-153) and its Iport = x[p]-x[g] epilogue. */
+typedef struct {
+    int quad_order;
+    double near_sing_eps;
+    int self_subdiv;
+    double self_alpha;
+    int near_subdiv;
+    double near_alpha;
+    int near_order;
+    int model;          /* EFIE_MODEL_*                                       */
+    int precision;      /* EFIE_PREC_*                                        */
+    int batch;          /* frequencies factored together (0 = automatic)      */
+    double ir_tol;      /* mixed: stop refinement at ||V-Zx||/||V|| <= ir_tol  */
+    int ir_max_iter;    /* mixed: refinement iteration cap                    */
+} efie_opts;
+
+/* Per-call timings (HIP events on the context stream) and counters. */
+typedef struct {
+    double ms_total;        /* whole call, device work only                  */
+    double ms_fill;         /* Z fill (all fill kernels)                     */
+    double ms_factor;       /* LU factorisation                              */
+    double ms_solve;        /* triangular solves + refinement + epilogue     */
+    double ms_fill_kernel;  /* near-pair record kernel, summed over calls    */
+    int n_fill_kernel;      /* launches of that kernel                       */
+    long long n_near_pairs; /* unordered near triangle pairs                 */
+    long long n_green;      /* Green-function evaluations per frequency      */
+    int ir_iters_max;       /* mixed: most refinement iterations used         */
+    double ir_resid_max;    /* mixed: largest final ||V-Zx||/||V||            */
+} efie_stats;
+
+/* "accurate" preset (main_patch_efie_rwg.m:281-289), FP64, quadrature. */
+void efie_opts_default(efie_opts* o);
+/* name: "accurate", "fast" (main_patch_gpu_mex_sweep_l.m:50-56),
+ * "centroid" (the reference MEX centroid model). */
+efie_status efie_opts_preset(efie_opts* o, const char* name);
+
+const char* efie_last_error(void);
+const char* efie_last_error_id(void);
+
+/* ---- host-side helpers: the MATLAB functions that build the MEX inputs -- */
+
+/* make_patch_with_ground_mesh.m:1-46 */
+efie_status efie_mesh_counts(int npx, int npy, int ngx, int ngy, int* Nv, int* Nt);
+efie_status efie_make_mesh(double Lp, double Wp, double Lg, double Wg, double h,
+                           int npx, int npy, int ngx, int ngy,
+                           double* V, int* F);
+/* build_rwg_interior.m:1-113 (two calls: count, then fill) */
+efie_status efie_rwg_count(const int* F, int Nt, int Nv, int* Ne);
+efie_status efie_build_rwg(const double* V, int Nv, const int* F, int Nt, int Ne,
+                           int* edge, int* plusTri, int* minusTri,
+                           int* plusSign, int* minusSign, double* rp,
+                           double* rm, double* Ap, double* Am, double* len,
+                           double* center, int* zSign);
+/* pick_port_edge_by_point_and_layer.m:1-11; *edge is 1-based */
+efie_status efie_pick_port(int Ne, const double* center, const int* zSign,
+                           const double p[3], int layerSign, int* edge);
+
+/* ---- device context ------------------------------------------------------ */
+
+typedef struct efie_ctx efie_ctx;
+
+efie_status efie_create(efie_ctx** ctx, int device);
+void efie_destroy(efie_ctx* ctx);
+
+/* Mesh and RWG tables: arguments 1-12 of both reference MEX functions
+ * (patch_sweep_cuda.cu:4-16, gassemble_ZV_EFIE_RWG_DIFF_GPU_FAST.cu:127-139).
+ * Copied to HBM once; frequency-independent geometry (quadrature points,
+ * near-pair lists) is derived lazily and cached until the tables or the
+ * geometry options change. */
+efie_status efie_set_mesh(efie_ctx* ctx, const double* V, int Nv, const int* F, int Nt);
+efie_status efie_set_rwg(efie_ctx* ctx, const int* edge, const int* plusTri,
+                         const int* minusTri, const int* plusSign,
+                         const int* minusSign, const double* rp,
+                         const double* rm, const double* Ap, const double* Am,
+                         const double* len, int Ne);
+
+/* [Z, Vrhs] of gassemble_ZV_EFIE_RWG_DIFF_GPU_FAST (arguments 13-17,
+ * gassemble_ZV_EFIE_RWG_DIFF_GPU_FAST.cu:140-148): k, omega, mu, eps and
+ * exc = [patchEdge, groundEdge, V0, spreadRadius, selfAlpha].  Z is
+ * [Ne x Ne] column-major interleaved complex, Vrhs [Ne] interleaved complex
+ * (imaginary parts 0).  Z is computed with the quadrature scheme of
+ * assemble_ZV_EFIE_RWG_DIFF_CPU.m (opts), or the centroid model when
+ * opts->model is CENTROID (then exc[4] is the self-softening factor). */
+efie_status efie_assemble_zv(efie_ctx* ctx, double k_re, double k_im,
+                             double omega, double mu, double eps_re,
+                             double eps_im, const double* exc, int n_exc,
+                             const efie_opts* opts, double* Z, double* Vrhs);
+
+/* [Zin, S11] of patch_sweep_cuda (arguments 13-20, patch_sweep_cuda.cu:17-27):
+ * per frequency k = omega*sqrt(mu*eps), fill Z, LU-solve Z x = V, then
+ * Iport = sum((wPatch - wGnd) .* x .* len) (main_patch_efie_rwg.m:336-338),
+ * Zin = V0/Iport, S11 = (Zin - Z0)/(Zin + Z0).  Zin, S11: [Nf] interleaved
+ * complex.  stats may be NULL. */
+efie_status efie_sweep(efie_ctx* ctx, const double* freqs, int Nf, double mu,
+                       double eps_re, double eps_im, int patchEdge,
+                       int groundEdge, double V0, double spread, double Z0,
+                       const efie_opts* opts, double* Zin, double* S11,
+                       efie_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,945 @@
+// efie_api.cpp -- C ABI of libefie_hip.so: argument checking, host-side
+// table builders, device context and the frequency-sweep driver.
+#include "../../include/efie.h"
+#include "efie_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace efie;
+
+namespace {
+
+constexpr double kPi = 3.141592653589793238462643383279502884;
+
+thread_local std::string g_err = "";
+thread_local std::string g_err_id = "";
+
+efie_status fail(efie_status st, const char* id, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
+efie_status fail(efie_status st, const char* id, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    g_err_id = id;
+    return st;
+}
+
+efie_status ok() {
+    g_err.clear();
+    g_err_id.clear();
+    return EFIE_OK;
+}
+
+#define HIPCHK(call)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(EFIE_E_HIP, "efie:hip", "%s: %s", #call, hipGetErrorString(e_));     \
+    } while (0)
+
+// Device buffer that frees itself and only grows.
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t ensure(size_t n) {
+        if (n <= bytes && p) return hipSuccess;
+        release();
+        if (n == 0) n = 16;
+        hipError_t e = hipMalloc(&p, n);
+        if (e == hipSuccess) bytes = n;
+        else p = nullptr;
+        return e;
+    }
+    template <typename T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+// ------------------------------------------------------ quadrature rules --
+// tri_quad_rule.m:3-22 (order 1, 2, otherwise the 6-point rule with the
+// reference's 15-digit literals) plus Radon's 7-point degree-5 rule for 7.
+int quad_rule(int order, double* u, double* v, double* w) {
+    if (order == 1) {
+        u[0] = 1.0 / 3; v[0] = 1.0 / 3; w[0] = 1.0 / 2;
+        return 1;
+    }
+    if (order == 2) {
+        const double uu[3] = {1.0 / 6, 2.0 / 3, 1.0 / 6}, vv[3] = {1.0 / 6, 1.0 / 6, 2.0 / 3};
+        for (int i = 0; i < 3; ++i) { u[i] = uu[i]; v[i] = vv[i]; w[i] = (1.0 / 6) * 1.0; }
+        return 3;
+    }
+    if (order == 7) {
+        const double s15 = std::sqrt(15.0);
+        const double a1 = (6.0 - s15) / 21.0, a2 = (6.0 + s15) / 21.0;
+        const double w0 = 9.0 / 80.0, w1 = (155.0 - s15) / 2400.0, w2 = (155.0 + s15) / 2400.0;
+        const double uu[7] = {1.0 / 3, a1, 1 - 2 * a1, a1, a2, 1 - 2 * a2, a2};
+        const double vv[7] = {1.0 / 3, a1, a1, 1 - 2 * a1, a2, a2, 1 - 2 * a2};
+        const double ww[7] = {w0, w1, w1, w1, w2, w2, w2};
+        for (int i = 0; i < 7; ++i) { u[i] = uu[i]; v[i] = vv[i]; w[i] = ww[i]; }
+        return 7;
+    }
+    const double a = 0.445948490915965, b = 0.108103018168070;
+    const double w1 = 0.111690794839005, w2 = 0.054975871827661;
+    const double uu[6] = {a, a, 1 - 2 * a, b, b, 1 - 2 * b};
+    const double vv[6] = {a, 1 - 2 * a, a, b, 1 - 2 * b, b};
+    const double ww[6] = {w1, w1, w1, w2, w2, w2};
+    for (int i = 0; i < 6; ++i) { u[i] = uu[i]; v[i] = vv[i]; w[i] = ww[i]; }
+    return 6;
+}
+
+struct V3 {
+    double x, y, z;
+};
+inline V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline double norm(V3 a) { return std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+inline V3 cross(V3 a, V3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+inline V3 mid(V3 a, V3 b) { return {0.5 * (a.x + b.x), 0.5 * (a.y + b.y), 0.5 * (a.z + b.z)}; }
+
+// A + u(B-A) + v(C-A)  (map_ref_to_vertices.m)
+inline V3 map_ref(V3 A, V3 B, V3 C, double u, double v) {
+    return {A.x + u * (B.x - A.x) + v * (C.x - A.x), A.y + u * (B.y - A.y) + v * (C.y - A.y),
+            A.z + u * (B.z - A.z) + v * (C.z - A.z)};
+}
+
+// 4^depth midpoint children in the order of subdivide_triangles.m:13-16.
+void subdivide(V3 A, V3 B, V3 C, int depth, std::vector<V3>& out) {
+    out.assign({A, B, C});
+    for (int d = 0; d < depth; ++d) {
+        std::vector<V3> nw;
+        nw.reserve(out.size() * 4);
+        for (size_t s = 0; s < out.size(); s += 3) {
+            V3 a = out[s], b = out[s + 1], c = out[s + 2];
+            V3 ab = mid(a, b), bc = mid(b, c), ca = mid(c, a);
+            const V3 kids[12] = {a, ab, ca, ab, b, bc, ca, bc, c, ab, bc, ca};
+            nw.insert(nw.end(), kids, kids + 12);
+        }
+        out.swap(nw);
+    }
+}
+
+struct GeoKey {
+    int quad_order = -1, near_order = -1, near_subdiv = -1, self_subdiv = -1;
+    double near_sing_eps = -1, self_alpha = -1;
+    bool operator==(const GeoKey& o) const {
+        return quad_order == o.quad_order && near_order == o.near_order &&
+               near_subdiv == o.near_subdiv && self_subdiv == o.self_subdiv &&
+               near_sing_eps == o.near_sing_eps && self_alpha == o.self_alpha;
+    }
+};
+
+}  // namespace
+
+namespace {
+struct RawE {
+    int a, b, raw;
+};
+std::vector<RawE> sorted_raw(const int* F, int Nt) {
+    std::vector<RawE> r((size_t)Nt * 3);
+    for (int t = 0; t < Nt; ++t) {
+        const int tri[3] = {F[t], F[t + Nt], F[t + 2 * Nt]};
+        for (int le = 0; le < 3; ++le) {
+            int a = tri[le], b = tri[(le + 1) % 3];
+            r[3 * t + le] = {std::min(a, b), std::max(a, b), 3 * t + le};
+        }
+    }
+    std::stable_sort(r.begin(), r.end(), [](const RawE& x, const RawE& y) {
+        return x.a != y.a ? x.a < y.a : x.b < y.b;
+    });
+    return r;
+}
+}  // namespace
+
+struct efie_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[8] = {};
+
+    // host copies of the tables (column-major, 1-based as given)
+    int Nv = 0, Nt = 0, Ne = 0;
+    std::vector<double> V;
+    std::vector<int> F;
+    std::vector<int> edge, plusTri, minusTri, plusSign, minusSign;
+    std::vector<double> rp, rm, Ap, Am, len;
+    bool haveMesh = false, haveRwg = false, midValid = false;
+
+    // geometry cache (quadrature tables etc.)
+    bool geoValid = false;
+    GeoKey key;
+    int Nq = 0, Nq2n = 0, NpNear = 0, Nq2s = 0, NsSelf = 0;
+    long long nNear = 0;
+
+    DBuf dTri, dReg, dNearPts, dSelfPts, dASelf, dAeq, dNearMat, dRowCnt, dList;
+    DBuf dSideTri, dSide, dMid, dLen;
+    // centroid model tables
+    DBuf dCPos, dCF, dCDv, dCW, dCT, dCAsoft;
+    double cAlpha = -1;
+    // per-call buffers
+    DBuf dRecs, dFreq, dZ, dZ32, dWp, dWg, dSums, dV, dWl, dOut;
+    DBuf dCand, dCnt, dIpiv, dInfo;
+    size_t cntBytes = 0;
+};
+
+extern "C" {
+
+const char* efie_last_error(void) { return g_err.c_str(); }
+const char* efie_last_error_id(void) { return g_err_id.c_str(); }
+
+void efie_opts_default(efie_opts* o) {
+    // accurate preset (main_patch_efie_rwg.m:281-289)
+    o->quad_order = 4;
+    o->near_sing_eps = 2.5;
+    o->self_subdiv = 2;
+    o->self_alpha = 0.20;
+    o->near_subdiv = 2;
+    o->near_alpha = 0.15;
+    o->near_order = 4;
+    o->model = EFIE_MODEL_QUADRATURE;
+    o->precision = EFIE_PREC_FP64;
+    o->batch = 0;
+    o->ir_tol = 1e-12;
+    o->ir_max_iter = 10;
+}
+
+efie_status efie_opts_preset(efie_opts* o, const char* name) {
+    if (!o || !name) return fail(EFIE_E_ARGS, "efie:opts", "null argument");
+    efie_opts_default(o);
+    if (!std::strcmp(name, "accurate")) return ok();
+    if (!std::strcmp(name, "fast")) {
+        // main_patch_gpu_mex_sweep_l.m:50-56
+        o->quad_order = 2;
+        o->near_sing_eps = 1.2;
+        o->self_subdiv = 2;
+        o->self_alpha = 0.2;
+        o->near_subdiv = 1;
+        o->near_alpha = 0.15;
+        o->near_order = 2;
+        return ok();
+    }
+    if (!std::strcmp(name, "centroid")) {
+        o->model = EFIE_MODEL_CENTROID;
+        return ok();
+    }
+    return fail(EFIE_E_ARGS, "efie:opts", "unknown preset '%s'", name);
+}
+
+// ------------------------------------------------------------- mesh ------
+
+efie_status efie_mesh_counts(int npx, int npy, int ngx, int ngy, int* Nv, int* Nt) {
+    if (npx < 1 || npy < 1 || ngx < 1 || ngy < 1 || !Nv || !Nt)
+        return fail(EFIE_E_ARGS, "efie:mesh", "cell counts must be >= 1");
+    *Nv = (npx + 1) * (npy + 1) + (ngx + 1) * (ngy + 1);
+    *Nt = 2 * (npx * npy + ngx * ngy);
+    return ok();
+}
+
+// make_patch_with_ground_mesh.m:1-46.  Vertices in meshgrid(x,y)(:) order
+// (y fastest), faces with the (N_x+1) stride exactly as the reference
+// writes them; for N_x != N_y that stride does not match the vertex order,
+// which the reference does as well.
+efie_status efie_make_mesh(double Lp, double Wp, double Lg, double Wg, double h, int npx, int npy,
+                           int ngx, int ngy, double* V, int* F) {
+    int Nv, Nt;
+    efie_status st = efie_mesh_counts(npx, npy, ngx, ngy, &Nv, &Nt);
+    if (st) return st;
+    if (!V || !F) return fail(EFIE_E_ARGS, "efie:mesh", "null output");
+    auto lin = [](double d1, double d2, int n) {
+        std::vector<double> y(n);
+        const int n1 = n - 1;
+        for (int i = 0; i <= n1; ++i) y[i] = d1 + ((double)i * (d2 - d1)) / (double)n1;
+        y[n1] = d2;
+        return y;
+    };
+    int v = 0, t = 0;
+    auto layer = [&](double W, double L, int nx, int ny, double z) {
+        const int off = v;
+        std::vector<double> xs = lin(-W / 2, W / 2, nx + 1), ys = lin(-L / 2, L / 2, ny + 1);
+        for (int c = 0; c <= nx; ++c)
+            for (int r = 0; r <= ny; ++r, ++v) {
+                V[v] = xs[c];
+                V[v + Nv] = ys[r];
+                V[v + 2 * Nv] = z * 1.0;
+            }
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                const int v1 = off + (j - 1) * (nx + 1) + i, v2 = v1 + 1, v3 = v1 + (nx + 1), v4 = v3 + 1;
+                F[t] = v1; F[t + Nt] = v2; F[t + 2 * Nt] = v4; ++t;
+                F[t] = v1; F[t + Nt] = v4; F[t + 2 * Nt] = v3; ++t;
+            }
+    };
+    layer(Wp, Lp, npx, npy, +h / 2);
+    layer(Wg, Lg, ngx, ngy, -h / 2);
+    return ok();
+}
+
+// --------------------------------------------------------------- RWG -----
+// build_rwg_interior.m: unique undirected edges in lexicographic order of
+// the sorted vertex pair (:25); an edge is interior when exactly two
+// triangle edges map to it (:29-31); its first incidence in raw order is
+// the plus side (:47-63).
+
+
+efie_status efie_rwg_count(const int* F, int Nt, int Nv, int* Ne) {
+    if (!F || !Ne || Nt < 1 || Nv < 3) return fail(EFIE_E_ARGS, "efie:rwg", "bad mesh sizes");
+    for (long long i = 0; i < 3LL * Nt; ++i)
+        if (F[i] < 1 || F[i] > Nv) return fail(EFIE_E_MESH, "mex:F", "F has invalid indices.");
+    std::vector<RawE> r = sorted_raw(F, Nt);
+    int ne = 0;
+    for (size_t i = 0; i < r.size();) {
+        size_t j = i;
+        while (j < r.size() && r[j].a == r[i].a && r[j].b == r[i].b) ++j;
+        if (j - i == 2) ++ne;
+        i = j;
+    }
+    *Ne = ne;
+    return ok();
+}
+
+efie_status efie_build_rwg(const double* V, int Nv, const int* F, int Nt, int Ne, int* edge,
+                           int* plusTri, int* minusTri, int* plusSign, int* minusSign, double* rp,
+                           double* rm, double* Ap, double* Am, double* len, double* center,
+                           int* zSign) {
+    int ne = 0;
+    efie_status st = efie_rwg_count(F, Nt, Nv, &ne);
+    if (st) return st;
+    if (ne != Ne) return fail(EFIE_E_ARGS, "efie:rwg", "Ne=%d but the mesh has %d interior edges", Ne, ne);
+    std::vector<RawE> r = sorted_raw(F, Nt);
+    auto vx = [&](int vi) { return V3{V[vi - 1], V[vi - 1 + Nv], V[vi - 1 + 2 * Nv]}; };
+    int e = 0;
+    for (size_t i = 0; i < r.size();) {
+        size_t j = i;
+        while (j < r.size() && r[j].a == r[i].a && r[j].b == r[i].b) ++j;
+        if (j - i == 2) {
+            const int a = r[i].a, b = r[i].b;
+            edge[e] = a;
+            edge[e + Ne] = b;
+            for (int s = 0; s < 2; ++s) {
+                const int raw = r[i + s].raw, t = raw / 3, le = raw % 3;
+                const int tri[3] = {F[t], F[t + Nt], F[t + 2 * Nt]};
+                const int sg = (tri[le] == a && tri[(le + 1) % 3] == b) ? 1 : -1;
+                (s == 0 ? plusTri : minusTri)[e] = t + 1;
+                (s == 0 ? plusSign : minusSign)[e] = sg;
+            }
+            ++e;
+        }
+        i = j;
+    }
+    double zmax = -INFINITY, zmin = INFINITY;
+    for (int v = 0; v < Nv; ++v) {
+        zmax = std::max(zmax, V[v + 2 * Nv]);
+        zmin = std::min(zmin, V[v + 2 * Nv]);
+    }
+    const double zmid = 0.5 * (zmax + zmin);
+    for (e = 0; e < Ne; ++e) {
+        const int va = edge[e], vb = edge[e + Ne];
+        const V3 ra = vx(va), rb = vx(vb);
+        len[e] = norm(rb - ra);
+        const V3 c = mid(ra, rb);
+        center[e] = c.x; center[e + Ne] = c.y; center[e + 2 * Ne] = c.z;
+        for (int s = 0; s < 2; ++s) {
+            const int t = (s == 0 ? plusTri : minusTri)[e] - 1;
+            const int ids[3] = {F[t], F[t + Nt], F[t + 2 * Nt]};
+            int opp = 0, cnt = 0;
+            for (int k = 0; k < 3; ++k)
+                if (ids[k] != va && ids[k] != vb) { opp = ids[k]; ++cnt; }
+            if (cnt != 1) return fail(EFIE_E_MESH, "efie:rwg", "opposite vertex not found for edge %d", e + 1);
+            const V3 o = vx(opp);
+            double* rr = s == 0 ? rp : rm;
+            rr[e] = o.x; rr[e + Ne] = o.y; rr[e + 2 * Ne] = o.z;
+            (s == 0 ? Ap : Am)[e] = 0.5 * norm(cross(vx(ids[1]) - vx(ids[0]), vx(ids[2]) - vx(ids[0])));
+        }
+        zSign[e] = c.z > zmid ? 1 : -1;
+    }
+    return ok();
+}
+
+efie_status efie_pick_port(int Ne, const double* center, const int* zSign, const double p[3],
+                           int layerSign, int* edgeOut) {
+    if (!center || !zSign || !p || !edgeOut || Ne < 1) return fail(EFIE_E_ARGS, "efie:port", "bad arguments");
+    int best = -1;
+    double bd = INFINITY;
+    for (int e = 0; e < Ne; ++e) {
+        if (zSign[e] != layerSign) continue;
+        const double d = norm(V3{center[e] - p[0], center[e + Ne] - p[1], center[e + 2 * Ne] - p[2]});
+        if (best < 0 || d < bd) { bd = d; best = e + 1; }
+    }
+    if (best < 0) return fail(EFIE_E_PORT, "efie:port", "no edge on layer %d", layerSign);
+    *edgeOut = best;
+    return ok();
+}
+
+// ------------------------------------------------------------ context ----
+
+efie_status efie_create(efie_ctx** out, int device) {
+    if (!out) return fail(EFIE_E_ARGS, "efie:args", "null context pointer");
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(EFIE_E_ARGS, "efie:args", "device %d of %d", device, n);
+    HIPCHK(hipSetDevice(device));
+    efie_ctx* c = new efie_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreate(&c->stream);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(EFIE_E_HIP, "efie:hip", "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    for (auto& ev : c->ev) {
+        e = hipEventCreate(&ev);
+        if (e != hipSuccess) {
+            efie_destroy(c);
+            return fail(EFIE_E_HIP, "efie:hip", "hipEventCreate: %s", hipGetErrorString(e));
+        }
+    }
+    *out = c;
+    return ok();
+}
+
+void efie_destroy(efie_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& ev : c->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;  // DBuf destructors free device memory
+}
+
+efie_status efie_set_mesh(efie_ctx* c, const double* V, int Nv, const int* F, int Nt) {
+    if (!c || !V || !F) return fail(EFIE_E_ARGS, "efie:args", "null argument");
+    if (Nv < 3 || Nt < 1) return fail(EFIE_E_ARGS, "mex:V", "V must be [Nv x 3] double.");
+    for (long long i = 0; i < 3LL * Nt; ++i)
+        if (F[i] < 1 || F[i] > Nv) return fail(EFIE_E_MESH, "mex:F", "F has invalid indices.");
+    c->Nv = Nv;
+    c->Nt = Nt;
+    c->V.assign(V, V + 3 * (size_t)Nv);
+    c->F.assign(F, F + 3 * (size_t)Nt);
+    c->haveMesh = true;
+    c->haveRwg = false;
+    c->midValid = false;
+    c->geoValid = false;
+    c->cAlpha = -1;
+    return ok();
+}
+
+efie_status efie_set_rwg(efie_ctx* c, const int* edge, const int* plusTri, const int* minusTri,
+                         const int* plusSign, const int* minusSign, const double* rp,
+                         const double* rm, const double* Ap, const double* Am, const double* len,
+                         int Ne) {
+    if (!c) return fail(EFIE_E_ARGS, "efie:args", "null context");
+    if (!c->haveMesh) return fail(EFIE_E_STATE, "efie:state", "efie_set_mesh must come first");
+    if (!edge || !plusTri || !minusTri || !plusSign || !minusSign || !rp || !rm || !Ap || !Am || !len)
+        return fail(EFIE_E_ARGS, "efie:args", "null table");
+    if (Ne < 1) return fail(EFIE_E_ARGS, "mex:edge", "edge must be [Ne x 2] int32.");
+    for (int e = 0; e < Ne; ++e) {
+        if (plusTri[e] < 0 || plusTri[e] > c->Nt)
+            return fail(EFIE_E_MESH, "mex:plusTri", "plusTri must be int32 [Ne x 1].");
+        if (minusTri[e] < 0 || minusTri[e] > c->Nt)
+            return fail(EFIE_E_MESH, "mex:minusTri", "minusTri must be int32 [Ne x 1].");
+        if (edge[e] < 1 || edge[e] > c->Nv || edge[e + Ne] < 1 || edge[e + Ne] > c->Nv)
+            return fail(EFIE_E_MESH, "mex:edge", "edge has invalid vertex indices.");
+    }
+    c->Ne = Ne;
+    c->edge.assign(edge, edge + 2 * (size_t)Ne);
+    c->plusTri.assign(plusTri, plusTri + Ne);
+    c->minusTri.assign(minusTri, minusTri + Ne);
+    c->plusSign.assign(plusSign, plusSign + Ne);
+    c->minusSign.assign(minusSign, minusSign + Ne);
+    c->rp.assign(rp, rp + 3 * (size_t)Ne);
+    c->rm.assign(rm, rm + 3 * (size_t)Ne);
+    c->Ap.assign(Ap, Ap + Ne);
+    c->Am.assign(Am, Am + Ne);
+    c->len.assign(len, len + Ne);
+    c->haveRwg = true;
+    c->midValid = false;
+    c->geoValid = false;
+    c->cAlpha = -1;
+    return ok();
+}
+
+}  // extern "C"
+
+namespace {
+
+// Triangle geometry of precompute_tri_geom.m:11-17 for triangle t (0-based).
+void tri_vertices(const efie_ctx* c, int t, V3& A, V3& B, V3& C) {
+    const int Nv = c->Nv, Nt = c->Nt;
+    auto vx = [&](int vi) { return V3{c->V[vi - 1], c->V[vi - 1 + Nv], c->V[vi - 1 + 2 * Nv]}; };
+    A = vx(c->F[t]);
+    B = vx(c->F[t + Nt]);
+    C = vx(c->F[t + 2 * Nt]);
+}
+
+efie_status check_opts(const efie_opts* o) {
+    if (o->model == EFIE_MODEL_CENTROID) return ok();
+    if (o->model != EFIE_MODEL_QUADRATURE) return fail(EFIE_E_ARGS, "efie:opts", "unknown model");
+    if (o->near_subdiv < 0 || o->near_subdiv > 3 || o->self_subdiv < 0 || o->self_subdiv > 3)
+        return fail(EFIE_E_ARGS, "efie:opts", "subdivision depth must be 0..3");
+    if (o->precision != EFIE_PREC_FP64 && o->precision != EFIE_PREC_MIXED)
+        return fail(EFIE_E_ARGS, "efie:opts", "unknown precision");
+    return ok();
+}
+
+efie_status ensure_edge_mid(efie_ctx* c);
+
+// Frequency-independent data of the quadrature model, cached per options.
+efie_status ensure_geometry(efie_ctx* c, const efie_opts* o) {
+    GeoKey k;
+    k.quad_order = o->quad_order;
+    k.near_order = o->near_order;
+    k.near_subdiv = o->near_subdiv;
+    k.self_subdiv = o->self_subdiv;
+    k.near_sing_eps = o->near_sing_eps;
+    k.self_alpha = o->self_alpha;
+    if (c->geoValid && c->key == k) return ok();
+    const int Nt = c->Nt, Ne = c->Ne;
+    hipStream_t s = c->stream;
+
+    std::vector<TriGeo> tg(Nt);
+    std::vector<V3> tv(3 * (size_t)Nt);
+    for (int t = 0; t < Nt; ++t) {
+        V3 A, B, C;
+        tri_vertices(c, t, A, B, C);
+        tv[3 * t] = A; tv[3 * t + 1] = B; tv[3 * t + 2] = C;
+        tg[t].cx = (A.x + B.x + C.x) / 3.0;
+        tg[t].cy = (A.y + B.y + C.y) / 3.0;
+        tg[t].cz = (A.z + B.z + C.z) / 3.0;
+        tg[t].area2 = norm(cross(B - A, C - A));
+    }
+    double qu[16], qv[16], qw[16];
+    // regular rule: points relative to the centroid, weight qw * area2
+    const int Nq = quad_rule(o->quad_order, qu, qv, qw);
+    std::vector<Pt> reg((size_t)Nt * Nq);
+    for (int t = 0; t < Nt; ++t)
+        for (int i = 0; i < Nq; ++i) {
+            V3 r = map_ref(tv[3 * t], tv[3 * t + 1], tv[3 * t + 2], qu[i], qv[i]);
+            reg[(size_t)t * Nq + i] = {r.x - tg[t].cx, r.y - tg[t].cy, r.z - tg[t].cz, qw[i] * tg[t].area2};
+        }
+    // subdivided rules for near and self pairs
+    double su[16], sv[16], sw[16];
+    const int Nq2 = quad_rule(o->near_order, su, sv, sw);
+    auto subdiv_pts = [&](int depth, std::vector<Pt>& pts, std::vector<double>* aSelf) {
+        const int Ns = 1 << (2 * depth);
+        pts.resize((size_t)Nt * Ns * Nq2);
+        if (aSelf) aSelf->resize((size_t)Nt * Ns);
+        std::vector<V3> sub;
+        for (int t = 0; t < Nt; ++t) {
+            subdivide(tv[3 * t], tv[3 * t + 1], tv[3 * t + 2], depth, sub);
+            for (int p = 0; p < Ns; ++p) {
+                V3 A = sub[3 * p], B = sub[3 * p + 1], C = sub[3 * p + 2];
+                const double a2p = norm(cross(B - A, C - A));
+                if (aSelf) (*aSelf)[(size_t)t * Ns + p] = o->self_alpha * std::sqrt((0.5 * a2p) / kPi);
+                for (int i = 0; i < Nq2; ++i) {
+                    V3 r = map_ref(A, B, C, su[i], sv[i]);
+                    pts[((size_t)t * Ns + p) * Nq2 + i] = {r.x - tg[t].cx, r.y - tg[t].cy, r.z - tg[t].cz,
+                                                           sw[i] * a2p};
+                }
+            }
+        }
+        return Ns;
+    };
+    std::vector<Pt> nearPts, selfPts;
+    std::vector<double> aSelf;
+    const int NsN = subdiv_pts(o->near_subdiv, nearPts, nullptr);
+    const int NsS = subdiv_pts(o->self_subdiv, selfPts, &aSelf);
+    std::vector<double> aeq(Nt);
+    for (int t = 0; t < Nt; ++t) aeq[t] = std::sqrt((0.5 * tg[t].area2) / kPi);
+
+    // RWG sides: f = c (r - v); rwg_eval_on_triangle.m tests t == plusTri first
+    std::vector<int> sideTri(2 * (size_t)Ne);
+    std::vector<Side> side(2 * (size_t)Ne);
+    for (int e = 0; e < Ne; ++e) {
+        for (int sd = 0; sd < 2; ++sd) {
+            const int t1 = (sd == 0 ? c->plusTri[e] : c->minusTri[e]);
+            if (t1 == 0) {
+                sideTri[2 * e + sd] = -1;
+                side[2 * e + sd] = {0, 0, 0, 0};
+                continue;
+            }
+            const bool plus = (t1 == c->plusTri[e]);
+            const double sg = plus ? c->plusSign[e] : c->minusSign[e];
+            const double l = c->len[e], A = plus ? c->Ap[e] : c->Am[e];
+            const double* vv = plus ? c->rp.data() : c->rm.data();
+            const double cc = plus ? sg * (l / (2 * A)) : -(sg * (l / (2 * A)));
+            const TriGeo g = tg[t1 - 1];
+            sideTri[2 * e + sd] = t1 - 1;
+            side[2 * e + sd] = {vv[e] - g.cx, vv[e + Ne] - g.cy, vv[e + 2 * Ne] - g.cz, cc};
+        }
+    }
+    HIPCHK(c->dTri.ensure(sizeof(TriGeo) * Nt));
+    HIPCHK(c->dReg.ensure(sizeof(Pt) * reg.size()));
+    HIPCHK(c->dNearPts.ensure(sizeof(Pt) * nearPts.size()));
+    HIPCHK(c->dSelfPts.ensure(sizeof(Pt) * selfPts.size()));
+    HIPCHK(c->dASelf.ensure(sizeof(double) * aSelf.size()));
+    HIPCHK(c->dAeq.ensure(sizeof(double) * Nt));
+    HIPCHK(c->dSideTri.ensure(sizeof(int) * sideTri.size()));
+    HIPCHK(c->dSide.ensure(sizeof(Side) * side.size()));
+    HIPCHK(c->dNearMat.ensure((size_t)Nt * Nt));
+    HIPCHK(c->dRowCnt.ensure(sizeof(int) * Nt));
+    HIPCHK(hipMemcpyAsync(c->dTri.p, tg.data(), sizeof(TriGeo) * Nt, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->dReg.p, reg.data(), sizeof(Pt) * reg.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->dNearPts.p, nearPts.data(), sizeof(Pt) * nearPts.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->dSelfPts.p, selfPts.data(), sizeof(Pt) * selfPts.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->dASelf.p, aSelf.data(), sizeof(double) * aSelf.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->dAeq.p, aeq.data(), sizeof(double) * Nt, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->dSideTri.p, sideTri.data(), sizeof(int) * sideTri.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->dSide.p, side.data(), sizeof(Side) * side.size(), hipMemcpyHostToDevice, s));
+
+    // near pairs (triangles_are_near.m) on the device: map, row counts, list
+    HIPCHK(launch_near_map(c->dTri.as<TriGeo>(), Nt, o->near_sing_eps, c->dNearMat.as<uint8_t>(),
+                           c->dRowCnt.as<int>(), s));
+    std::vector<int> cnt(Nt);
+    HIPCHK(hipMemcpyAsync(cnt.data(), c->dRowCnt.p, sizeof(int) * Nt, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    std::vector<int> start(Nt);
+    long long tot = 0;
+    for (int t = 0; t < Nt; ++t) {
+        start[t] = (int)tot;
+        tot += cnt[t];
+    }
+    HIPCHK(c->dList.ensure(sizeof(int2) * (size_t)std::max(1LL, tot)));
+    HIPCHK(hipMemcpyAsync(c->dRowCnt.p, start.data(), sizeof(int) * Nt, hipMemcpyHostToDevice, s));
+    HIPCHK(launch_near_list(c->dNearMat.as<uint8_t>(), c->dRowCnt.as<int>(), Nt, c->dList.as<int2>(), s));
+    HIPCHK(hipStreamSynchronize(s));
+
+    c->Nq = Nq;
+    c->Nq2n = Nq2;
+    c->NpNear = NsN * Nq2;
+    c->Nq2s = Nq2;
+    c->NsSelf = NsS;
+    c->nNear = tot;
+    c->key = k;
+    c->geoValid = true;
+    return ok();
+}
+
+// Host precompute of the centroid model (patch_sweep_cuda.cu:244-326).
+efie_status ensure_centroid(efie_ctx* c, double selfAlpha) {
+    if (c->cAlpha == selfAlpha && c->dCPos.p) return ok();
+    const int Nt = c->Nt, Ne = c->Ne;
+    std::vector<V3> cent(Nt);
+    std::vector<double> asoft(Nt);
+    for (int t = 0; t < Nt; ++t) {
+        V3 A, B, C;
+        tri_vertices(c, t, A, B, C);
+        cent[t] = {(A.x + B.x + C.x) / 3.0, (A.y + B.y + C.y) / 3.0, (A.z + B.z + C.z) / 3.0};
+        const double area2 = norm(cross(B - A, C - A));
+        asoft[t] = selfAlpha * std::sqrt((0.5 * area2) / kPi);
+    }
+    std::vector<double> pos(6 * (size_t)Ne, 0.0), f(6 * (size_t)Ne, 0.0), dv(2 * (size_t)Ne, 0.0),
+        w(2 * (size_t)Ne, 0.0);
+    std::vector<int> tsd(2 * (size_t)Ne);
+    for (int e = 0; e < Ne; ++e) {
+        const int tp = c->plusTri[e], tm = c->minusTri[e];
+        tsd[2 * e] = tp;
+        tsd[2 * e + 1] = tm;
+        const double l = c->len[e];
+        if (tp > 0) { pos[6 * e] = cent[tp - 1].x; pos[6 * e + 1] = cent[tp - 1].y; pos[6 * e + 2] = cent[tp - 1].z; }
+        if (tm > 0) { pos[6 * e + 3] = cent[tm - 1].x; pos[6 * e + 4] = cent[tm - 1].y; pos[6 * e + 5] = cent[tm - 1].z; }
+        w[2 * e] = tp > 0 ? c->Ap[e] : 0.0;
+        w[2 * e + 1] = tm > 0 ? c->Am[e] : 0.0;
+        if (tp > 0 && c->Ap[e] > 0) {
+            const double sc = (double)c->plusSign[e] * (l / (2.0 * c->Ap[e]));
+            for (int d = 0; d < 3; ++d) f[6 * e + d] = sc * (pos[6 * e + d] - c->rp[e + d * (size_t)Ne]);
+            dv[2 * e] = (double)c->plusSign[e] * (l / c->Ap[e]);
+        }
+        if (tm > 0 && c->Am[e] > 0) {
+            const double sc = (double)c->minusSign[e] * (l / (2.0 * c->Am[e]));
+            for (int d = 0; d < 3; ++d) f[6 * e + 3 + d] = sc * (c->rm[e + d * (size_t)Ne] - pos[6 * e + 3 + d]);
+            dv[2 * e + 1] = (double)c->minusSign[e] * (-(l / c->Am[e]));
+        }
+    }
+    hipStream_t s = c->stream;
+    HIPCHK(c->dCPos.ensure(sizeof(double) * pos.size()));
+    HIPCHK(c->dCF.ensure(sizeof(double) * f.size()));
+    HIPCHK(c->dCDv.ensure(sizeof(double) * dv.size()));
+    HIPCHK(c->dCW.ensure(sizeof(double) * w.size()));
+    HIPCHK(c->dCT.ensure(sizeof(int) * tsd.size()));
+    HIPCHK(c->dCAsoft.ensure(sizeof(double) * Nt));
+    HIPCHK(hipMemcpyAsync(c->dCPos.p, pos.data(), sizeof(double) * pos.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->dCF.p, f.data(), sizeof(double) * f.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->dCDv.p, dv.data(), sizeof(double) * dv.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->dCW.p, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->dCT.p, tsd.data(), sizeof(int) * tsd.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->dCAsoft.p, asoft.data(), sizeof(double) * Nt, hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    c->cAlpha = selfAlpha;
+    return ok();
+}
+
+Freq make_freq(std::complex<double> k, double omega, double mu, std::complex<double> eps) {
+    Freq f;
+    f.kr = k.real();
+    f.ki = k.imag();
+    // alpha = 1j*omega*mu, beta = 1/(1j*omega*eps)  (assemble_ZV_EFIE_RWG_DIFF_CPU.m:30)
+    f.ar = 0.0;
+    f.ai = omega * mu;
+    std::complex<double> jwe = std::complex<double>(0.0, omega) * eps;
+    double d = jwe.real() * jwe.real() + jwe.imag() * jwe.imag();
+    f.br = jwe.real() / d;
+    f.bi = -jwe.imag() / d;
+    return f;
+}
+
+// Fill one Z (interleaved complex, element (m,n) at Z[2*(m*ldr + n*ldc)]).
+efie_status fill_z(efie_ctx* c, const efie_opts* o, const Freq* dfr, double* Z, size_t ldr,
+                   size_t ldc) {
+    hipStream_t s = c->stream;
+    if (o->model == EFIE_MODEL_CENTROID) {
+        HIPCHK(launch_centroid(c->dCPos.as<double>(), c->dCF.as<double>(), c->dCDv.as<double>(),
+                               c->dCW.as<double>(), c->dCT.as<int>(), c->dCAsoft.as<double>(), c->Ne, dfr,
+                               Z, ldr, ldc, s));
+        return ok();
+    }
+    const int Nt = c->Nt;
+    HIPCHK(c->dRecs.ensure(sizeof(Rec) * rec_index(0, Nt)));
+    Rec* recs = c->dRecs.as<Rec>();
+    HIPCHK(launch_rec_self(c->dSelfPts.as<Pt>(), c->Nq2s, c->NsSelf, c->dASelf.as<double>(), Nt, dfr, recs, s));
+    HIPCHK(launch_rec_near(c->dNearPts.as<Pt>(), c->NpNear, c->dTri.as<TriGeo>(), c->dAeq.as<double>(),
+                           o->near_alpha, c->dList.as<int2>(), (int)c->nNear, dfr, recs, s));
+    HIPCHK(launch_rec_regular(c->dReg.as<Pt>(), c->Nq, c->dTri.as<TriGeo>(), Nt, c->dNearMat.as<uint8_t>(), dfr,
+                              recs, s));
+    HIPCHK(launch_combine(recs, c->dSideTri.as<int>(), c->dSide.as<Side>(), c->Ne, dfr, Z, ldr, ldc, s));
+    return ok();
+}
+
+// Port weights and V on the device; fails with mex:rhs like the reference
+// when a layer gets no weight.
+efie_status make_rhs(efie_ctx* c, int pe, int ge, double V0, double spread) {
+    hipStream_t s = c->stream;
+    const int Ne = c->Ne;
+    HIPCHK(c->dWp.ensure(sizeof(double) * Ne));
+    HIPCHK(c->dWg.ensure(sizeof(double) * Ne));
+    HIPCHK(c->dSums.ensure(sizeof(double) * 2));
+    HIPCHK(c->dV.ensure(sizeof(double) * 2 * Ne));
+    HIPCHK(c->dWl.ensure(sizeof(double) * 2 * Ne));
+    HIPCHK(launch_port_weights(c->dMid.as<double>(), Ne, pe - 1, spread, c->dWp.as<double>(), c->dSums.as<double>(), 0, s));
+    HIPCHK(launch_port_weights(c->dMid.as<double>(), Ne, ge - 1, spread, c->dWg.as<double>(), c->dSums.as<double>(), 1, s));
+    double sums[2];
+    HIPCHK(hipMemcpyAsync(sums, c->dSums.p, sizeof sums, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (!(sums[0] > 1e-30) || !(sums[1] > 1e-30))
+        return fail(EFIE_E_RHS, "mex:rhs", "spreadRadius too small (no weights).");
+    HIPCHK(launch_rhs(c->dWp.as<double>(), c->dWg.as<double>(), c->dLen.as<double>(), Ne, V0, c->dV.as<double>(),
+                      c->dWl.as<double>(), s));
+    return ok();
+}
+
+// Edge midpoints (port_weights_gaussian.m:7-11) and lengths for the RHS
+// and the Zin epilogue, needed by both models.
+efie_status ensure_edge_mid(efie_ctx* c) {
+    if (c->midValid) return ok();
+    const int Ne = c->Ne;
+    std::vector<double> midp(3 * (size_t)Ne);
+    for (int e = 0; e < Ne; ++e) {
+        const int a = c->edge[e] - 1, b = c->edge[e + Ne] - 1;
+        for (int d = 0; d < 3; ++d)
+            midp[3 * e + d] = 0.5 * (c->V[a + d * (size_t)c->Nv] + c->V[b + d * (size_t)c->Nv]);
+    }
+    HIPCHK(c->dMid.ensure(sizeof(double) * midp.size()));
+    HIPCHK(c->dLen.ensure(sizeof(double) * Ne));
+    HIPCHK(hipMemcpyAsync(c->dMid.p, midp.data(), sizeof(double) * midp.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->dLen.p, c->len.data(), sizeof(double) * Ne, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->midValid = true;
+    return ok();
+}
+
+efie_status prepare(efie_ctx* c, const efie_opts* o, double selfAlphaCentroid) {
+    if (!c->haveMesh || !c->haveRwg)
+        return fail(EFIE_E_STATE, "efie:state", "mesh and RWG tables must be set first");
+    HIPCHK(hipSetDevice(c->device));
+    efie_status st = check_opts(o);
+    if (st) return st;
+    st = ensure_edge_mid(c);
+    if (st) return st;
+    if (o->model == EFIE_MODEL_CENTROID) return ensure_centroid(c, selfAlphaCentroid);
+    return ensure_geometry(c, o);
+}
+
+}  // namespace
+
+extern "C" {
+
+efie_status efie_assemble_zv(efie_ctx* c, double k_re, double k_im, double omega, double mu,
+                             double eps_re, double eps_im, const double* exc, int n_exc,
+                             const efie_opts* opts, double* Z, double* Vrhs) {
+    if (!c || !exc || !Z || !Vrhs) return fail(EFIE_E_ARGS, "efie:args", "null argument");
+    if (n_exc < 5)
+        return fail(EFIE_E_ARGS, "mex:exc",
+                    "exc must be double vector len>=5: [patchEdge, groundEdge, V0, spreadRadius, selfAlpha].");
+    efie_opts o;
+    if (opts) o = *opts;
+    else efie_opts_default(&o);
+    const int pe = (int)exc[0], ge = (int)exc[1];
+    if (!c->haveRwg) return fail(EFIE_E_STATE, "efie:state", "mesh and RWG tables must be set first");
+    if (pe < 1 || pe > c->Ne) return fail(EFIE_E_PORT, "mex:port", "patchEdge out of range.");
+    if (ge < 1 || ge > c->Ne) return fail(EFIE_E_PORT, "mex:port", "groundEdge out of range.");
+    efie_status st = prepare(c, &o, exc[4]);
+    if (st) return st;
+    st = make_rhs(c, pe, ge, exc[2], exc[3]);
+    if (st) return st;
+    const int Ne = c->Ne;
+    hipStream_t s = c->stream;
+    Freq fr;
+    fr.kr = k_re;
+    fr.ki = k_im;
+    // alpha = j omega mu, beta = 1/(j omega eps) (gassemble_ZV_EFIE_RWG_DIFF_GPU_FAST.cu:341-351)
+    fr.ar = 0.0;
+    fr.ai = omega * mu;
+    {
+        double dr = -omega * eps_im, di = omega * eps_re, d2 = dr * dr + di * di;
+        fr.br = dr / d2;
+        fr.bi = -di / d2;
+    }
+    HIPCHK(c->dFreq.ensure(sizeof(Freq)));
+    HIPCHK(hipMemcpyAsync(c->dFreq.p, &fr, sizeof fr, hipMemcpyHostToDevice, s));
+    HIPCHK(c->dZ.ensure(sizeof(double) * 2 * (size_t)Ne * Ne));
+    st = fill_z(c, &o, c->dFreq.as<Freq>(), c->dZ.as<double>(), 1, (size_t)Ne);
+    if (st) return st;
+    HIPCHK(hipMemcpyAsync(Z, c->dZ.p, sizeof(double) * 2 * (size_t)Ne * Ne, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(Vrhs, c->dV.p, sizeof(double) * 2 * (size_t)Ne, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return ok();
+}
+
+efie_status efie_sweep(efie_ctx* c, const double* freqs, int Nf, double mu, double eps_re,
+                       double eps_im, int patchEdge, int groundEdge, double V0, double spread,
+                       double Z0, const efie_opts* opts, double* Zin, double* S11,
+                       efie_stats* stats) {
+    if (!c || !freqs || !Zin || !S11) return fail(EFIE_E_ARGS, "efie:args", "null argument");
+    if (Nf < 0) return fail(EFIE_E_ARGS, "mex:freqs", "freqs must be double vector.");
+    efie_opts o;
+    if (opts) o = *opts;
+    else efie_opts_default(&o);
+    if (!c->haveRwg) return fail(EFIE_E_STATE, "efie:state", "mesh and RWG tables must be set first");
+    if (patchEdge < 1 || patchEdge > c->Ne) return fail(EFIE_E_PORT, "mex:port", "patchEdge out of range.");
+    if (groundEdge < 1 || groundEdge > c->Ne) return fail(EFIE_E_PORT, "mex:port", "groundEdge out of range.");
+    // the centroid sweep MEX hard-codes self_alpha = 0.20 (patch_sweep_cuda.cu:269)
+    efie_status st = prepare(c, &o, 0.20);
+    if (st) return st;
+    st = make_rhs(c, patchEdge, groundEdge, V0, spread);
+    if (st) return st;
+    if (stats) std::memset(stats, 0, sizeof *stats);
+    if (Nf == 0) return ok();
+
+    const int n = c->Ne;
+    const int lda = ((n + 1) + 31) & ~31;  // rhs in column n
+    const size_t zElems = (size_t)n * lda;
+    int B = o.batch > 0 ? o.batch : 8;
+    {
+        // keep the batch within a fixed share of device memory
+        size_t freeB = 0, totB = 0;
+        if (hipMemGetInfo(&freeB, &totB) == hipSuccess) {
+            size_t per = zElems * 16;
+            size_t cap = (size_t)(0.5 * (double)freeB) / std::max<size_t>(per, 1);
+            if (cap < (size_t)B) B = (int)std::max<size_t>(1, cap);
+        }
+    }
+    if (B > Nf) B = Nf;
+    hipStream_t s = c->stream;
+    HIPCHK(c->dZ.ensure(sizeof(double) * 2 * zElems * B));
+    HIPCHK(c->dFreq.ensure(sizeof(Freq) * B));
+    HIPCHK(c->dOut.ensure(sizeof(double) * 4 * (size_t)Nf));
+    const int nodes = lu_tree_nodes(n);
+    HIPCHK(c->dCand.ensure(sizeof(int) * (size_t)B * nodes * lu_cand_stride()));
+    size_t cntBytes = sizeof(int) * (size_t)B * nodes;
+    if (c->dCnt.bytes < cntBytes || !c->dCnt.p) {
+        HIPCHK(c->dCnt.ensure(cntBytes));
+        HIPCHK(hipMemsetAsync(c->dCnt.p, 0, c->dCnt.bytes, s));
+    }
+    HIPCHK(c->dIpiv.ensure(sizeof(int) * (size_t)B * n));
+    HIPCHK(c->dInfo.ensure(sizeof(int) * B));
+    LuWork wk{c->dCand.as<int>(), c->dCnt.as<int>(), c->dIpiv.as<int>(), c->dInfo.as<int>()};
+
+    std::vector<Freq> fr(B);
+    std::complex<double> eps(eps_re, eps_im);
+    float msFill = 0, msFac = 0, msSolve = 0;
+    HIPCHK(hipEventRecord(c->ev[0], s));
+    for (int f0 = 0; f0 < Nf; f0 += B) {
+        const int nb = std::min(B, Nf - f0);
+        for (int i = 0; i < nb; ++i) {
+            const double omega = 2.0 * kPi * freqs[f0 + i];
+            std::complex<double> k = omega * std::sqrt(std::complex<double>(mu, 0.0) * eps);
+            fr[i] = make_freq(k, omega, mu, eps);
+        }
+        HIPCHK(hipMemcpyAsync(c->dFreq.p, fr.data(), sizeof(Freq) * nb, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemsetAsync(c->dInfo.p, 0, sizeof(int) * nb, s));
+        HIPCHK(hipEventRecord(c->ev[1], s));
+        double* Zb = c->dZ.as<double>();
+        for (int i = 0; i < nb; ++i) {
+            double* Zi = Zb + 2 * zElems * i;
+            st = fill_z(c, &o, c->dFreq.as<Freq>() + i, Zi, (size_t)lda, 1);
+            if (st) return st;
+            // rhs into column n
+            HIPCHK(hipMemcpy2DAsync(Zi + 2 * n, sizeof(double) * 2 * lda, c->dV.p, sizeof(double) * 2,
+                                    sizeof(double) * 2, n, hipMemcpyDeviceToDevice, s));
+        }
+        HIPCHK(hipEventRecord(c->ev[2], s));
+        HIPCHK(lu_factor<double>(Zb, 2 * zElems, n, lda, nb, wk, s));
+        HIPCHK(hipEventRecord(c->ev[3], s));
+        HIPCHK(lu_backsolve<double>(Zb, 2 * zElems, n, lda, nb, s));
+        // x is column n of each matrix (element stride 2*lda doubles)
+        HIPCHK(launch_epilogue(Zb + 2 * n, 2 * zElems, 2 * (size_t)lda, c->dWl.as<double>(), n,
+                               patchEdge - 1, groundEdge - 1, o.model == EFIE_MODEL_CENTROID, V0, Z0, nb,
+                               c->dOut.as<double>() + 4 * (size_t)f0, s));
+        HIPCHK(hipEventRecord(c->ev[4], s));
+        std::vector<int> info(nb);
+        HIPCHK(hipMemcpyAsync(info.data(), c->dInfo.p, sizeof(int) * nb, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        for (int i = 0; i < nb; ++i)
+            if (info[i] != 0)
+                return fail(EFIE_E_SINGULAR, "efie:singular",
+                            "Z is singular at f=%g Hz (zero pivot in column %d)", freqs[f0 + i], info[i]);
+        float t1, t2, t3;
+        HIPCHK(hipEventElapsedTime(&t1, c->ev[1], c->ev[2]));
+        HIPCHK(hipEventElapsedTime(&t2, c->ev[2], c->ev[3]));
+        HIPCHK(hipEventElapsedTime(&t3, c->ev[3], c->ev[4]));
+        msFill += t1;
+        msFac += t2;
+        msSolve += t3;
+    }
+    HIPCHK(hipEventRecord(c->ev[5], s));
+    std::vector<double> out(4 * (size_t)Nf);
+    HIPCHK(hipMemcpyAsync(out.data(), c->dOut.p, sizeof(double) * out.size(), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (int f = 0; f < Nf; ++f) {
+        Zin[2 * f] = out[4 * f];
+        Zin[2 * f + 1] = out[4 * f + 1];
+        S11[2 * f] = out[4 * f + 2];
+        S11[2 * f + 1] = out[4 * f + 3];
+    }
+    if (stats) {
+        float tot = 0;
+        HIPCHK(hipEventElapsedTime(&tot, c->ev[0], c->ev[5]));
+        stats->ms_total = tot;
+        stats->ms_fill = msFill;
+        stats->ms_factor = msFac;
+        stats->ms_solve = msSolve;
+        stats->n_near_pairs = c->nNear;
+        if (o.model == EFIE_MODEL_QUADRATURE) {
+            const long long Nt = c->Nt, np = (long long)c->NpNear, ns = (long long)c->NsSelf * c->Nq2s;
+            const long long reg = Nt * (Nt - 1) / 2 - c->nNear;
+            stats->n_green = reg * c->Nq * c->Nq + c->nNear * np * np + Nt * ns * ns;
+        }
+    }
+    return ok();
+}
+
+}  // extern "C"

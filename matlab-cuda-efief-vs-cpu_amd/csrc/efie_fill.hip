@@ -1,0 +1,636 @@
+// efie_fill.hip -- impedance-matrix fill, excitation and epilogue kernels
+// for gfx950.
+//
+// The reference fills Z one RWG pair at a time and recomputes every
+// triangle-pair integral for each of the up to nine RWG pairs that share it
+// (assemble_ZV_EFIE_RWG_DIFF_CPU.m:9-36, patch_sweep_fastquad_cuda.cu:347-412).
+// Here the fill is split in two:
+//   1. triangle-pair moments ("records"): for every unordered pair t1 <= t2
+//      the four Green-weighted moments S0, S1, S1', S2 of the quadrature
+//      rule the reference would use for that pair (regular, subdivided near
+//      or subdivided self), computed once per frequency;
+//   2. combine: Z(m,n) = sum over the four side pairs of alpha*Ivec +
+//      beta*Isca, where with f = c (r - v) on each side
+//        Ivec = cm cn [S2 - S1.un - S1'.um + S0 (um.un)],  Isca = 4 cm cn S0,
+//      which is the reference integrand sum regrouped (rwg_eval_on_triangle.m
+//      gives f and div f = 2c on both sides).
+// Coordinates inside a pair are relative to each triangle's centroid so the
+// regrouping does not cancel digits.
+#include "efie_internal.h"
+
+#include <math.h>
+
+namespace efie {
+
+static constexpr double kPi = 3.141592653589793238462643383279502884;
+
+// G = exp(-j k R) / (4 pi R) for complex k = kr + j ki
+// (rwg_pair_integrals_regular.m:22; exp(ki R)(cos(kr R) - j sin(kr R))).
+__device__ __forceinline__ void green64(double kr, double ki, double R, double& gr,
+                                        double& gi) {
+    double s, c;
+    sincos(kr * R, &s, &c);
+    double amp = (ki == 0.0) ? 1.0 : exp(ki * R);
+    double d = 4.0 * kPi * R;
+    gr = (amp * c) / d;
+    gi = -(amp * s) / d;
+}
+
+// Near test of triangles_are_near.m:1-12 in the reference's operation
+// order, with contraction disabled so every rounding matches the C oracle.
+__device__ bool near_test(const TriGeo a, const TriGeo b, double eps) {
+#pragma clang fp contract(off)
+    double dx = a.cx - b.cx, dy = a.cy - b.cy, dz = a.cz - b.cz;
+    double d = sqrt(dx * dx + dy * dy + dz * dz);
+    double A1 = 0.5 * a.area2, A2 = 0.5 * b.area2;
+    double s3 = sqrt(3.0);
+    double l1 = sqrt(4 * A1 / s3), l2 = sqrt(4 * A2 / s3);
+    double h = 0.5 * (l1 + l2);
+    return d < eps * h;
+}
+
+// ------------------------------------------------------------ near map --
+
+__global__ __launch_bounds__(256) void k_near_map(const TriGeo* __restrict__ tg, int Nt,
+                                                  double eps, uint8_t* __restrict__ nearMat,
+                                                  int* __restrict__ rowCount) {
+    __shared__ int cnt[256];
+    int t1 = blockIdx.x;
+    TriGeo a = tg[t1];
+    int c = 0;
+    for (int t2 = threadIdx.x; t2 < Nt; t2 += blockDim.x) {
+        bool nr = (t1 != t2) && near_test(a, tg[t2], eps);
+        nearMat[(size_t)t1 * Nt + t2] = nr ? 1 : 0;
+        c += (nr && t2 > t1) ? 1 : 0;
+    }
+    cnt[threadIdx.x] = c;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) cnt[threadIdx.x] += cnt[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) rowCount[t1] = cnt[0];
+}
+
+// Ordered compaction of the near pairs (t1 < t2) of one row.
+__global__ __launch_bounds__(256) void k_near_list(const uint8_t* __restrict__ nearMat,
+                                                   const int* __restrict__ rowStart, int Nt,
+                                                   int2* __restrict__ list) {
+    __shared__ int scan[256];
+    int t1 = blockIdx.x;
+    int out = rowStart[t1];
+    for (int base = t1 + 1; base < Nt; base += 256) {
+        int t2 = base + (int)threadIdx.x;
+        int f = (t2 < Nt) ? nearMat[(size_t)t1 * Nt + t2] : 0;
+        scan[threadIdx.x] = f;
+        __syncthreads();
+        for (int o = 1; o < 256; o <<= 1) {
+            int v = ((int)threadIdx.x >= o) ? scan[threadIdx.x - o] : 0;
+            __syncthreads();
+            scan[threadIdx.x] += v;
+            __syncthreads();
+        }
+        if (f) list[out + scan[threadIdx.x] - 1] = make_int2(t1, t2);
+        int tot = scan[255];
+        __syncthreads();
+        out += tot;
+    }
+}
+
+// ---------------------------------------------------- record helpers ----
+
+struct Acc {
+    double s0r, s0i, s1xr, s1xi, s1yr, s1yi, s1zr, s1zi;
+    double s1pxr, s1pxi, s1pyr, s1pyi, s1pzr, s1pzi, s2r, s2i;
+};
+
+__device__ __forceinline__ void acc_zero(Acc& a) {
+    a.s0r = a.s0i = a.s1xr = a.s1xi = a.s1yr = a.s1yi = a.s1zr = a.s1zi = 0.0;
+    a.s1pxr = a.s1pxi = a.s1pyr = a.s1pyi = a.s1pzr = a.s1pzi = a.s2r = a.s2i = 0.0;
+}
+
+// Fold the inner sums of one test point i into the pair moments:
+// g = sum_j W_j G_ij, h = sum_j W_j G_ij rho'_j.
+__device__ __forceinline__ void acc_fold(Acc& a, double w, double x, double y, double z,
+                                         double gr, double gi, double hxr, double hxi,
+                                         double hyr, double hyi, double hzr, double hzi) {
+    double wgr = w * gr, wgi = w * gi;
+    a.s0r += wgr;
+    a.s0i += wgi;
+    a.s1xr += wgr * x; a.s1xi += wgi * x;
+    a.s1yr += wgr * y; a.s1yi += wgi * y;
+    a.s1zr += wgr * z; a.s1zi += wgi * z;
+    a.s1pxr += w * hxr; a.s1pxi += w * hxi;
+    a.s1pyr += w * hyr; a.s1pyi += w * hyi;
+    a.s1pzr += w * hzr; a.s1pzi += w * hzi;
+    a.s2r += w * (x * hxr + y * hyr + z * hzr);
+    a.s2i += w * (x * hxi + y * hyi + z * hzi);
+}
+
+__device__ __forceinline__ void acc_store(Rec* r, const Acc& a) {
+    double2* p = reinterpret_cast<double2*>(r->v);
+    p[0] = make_double2(a.s0r, a.s0i);
+    p[1] = make_double2(a.s1xr, a.s1xi);
+    p[2] = make_double2(a.s1yr, a.s1yi);
+    p[3] = make_double2(a.s1zr, a.s1zi);
+    p[4] = make_double2(a.s1pxr, a.s1pxi);
+    p[5] = make_double2(a.s1pyr, a.s1pyi);
+    p[6] = make_double2(a.s1pzr, a.s1pzi);
+    p[7] = make_double2(a.s2r, a.s2i);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// ------------------------------------------------------ regular pairs ---
+// Tile of 64 test triangles (one per lane) by 64 source triangles (16 per
+// wave); one block per upper-triangular tile pair.  The test points stay in
+// registers, the source points are wave-uniform loads.
+
+constexpr int kRegTile = 64;
+
+template <int NQ>
+__global__ __launch_bounds__(256) void k_rec_regular(const Pt* __restrict__ pts,
+                                                     const TriGeo* __restrict__ tg, int Nt,
+                                                     const uint8_t* __restrict__ nearMat,
+                                                     const Freq* __restrict__ fr,
+                                                     Rec* __restrict__ recs) {
+    // linear block id -> tile pair (ti <= tj)
+    long long b = blockIdx.x;
+    int tj = (int)((sqrt(8.0 * (double)b + 1.0) - 1.0) * 0.5);
+    while ((long long)(tj + 1) * (tj + 2) / 2 <= b) ++tj;
+    while ((long long)tj * (tj + 1) / 2 > b) --tj;
+    int ti = (int)(b - (long long)tj * (tj + 1) / 2);
+
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int t1 = ti * kRegTile + lane;
+    const double kr = fr->kr, ki = fr->ki;
+
+    Pt p1[NQ];
+    TriGeo g1 = {0, 0, 0, 0};
+    if (t1 < Nt) {
+        g1 = tg[t1];
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) p1[i] = pts[(size_t)t1 * NQ + i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) p1[i] = Pt{0, 0, 0, 0};
+    }
+
+    constexpr int perWave = kRegTile / 4;
+    for (int q = 0; q < perWave; ++q) {
+        const int t2 = tj * kRegTile + wave * perWave + q;
+        if (t2 >= Nt) break;
+        bool active = (t1 < t2) && (nearMat[(size_t)t2 * Nt + t1] == 0);
+        if (!active) continue;
+        const TriGeo g2 = tg[t2];
+        const Pt* p2 = pts + (size_t)t2 * NQ;
+        const double ox = g1.cx - g2.cx, oy = g1.cy - g2.cy, oz = g1.cz - g2.cz;
+        Acc a;
+        acc_zero(a);
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            const double xi = p1[i].x + ox, yi = p1[i].y + oy, zi = p1[i].z + oz;
+            double gr = 0, gi = 0, hxr = 0, hxi = 0, hyr = 0, hyi = 0, hzr = 0, hzi = 0;
+#pragma unroll
+            for (int j = 0; j < NQ; ++j) {
+                const Pt q2 = p2[j];
+                double dx = xi - q2.x, dy = yi - q2.y, dz = zi - q2.z;
+                double R = sqrt(dx * dx + dy * dy + dz * dz);
+                R = R < 1e-15 ? 1e-15 : R;
+                double Gr, Gi;
+                green64(kr, ki, R, Gr, Gi);
+                double wr = q2.w * Gr, wi = q2.w * Gi;
+                gr += wr; gi += wi;
+                hxr += wr * q2.x; hxi += wi * q2.x;
+                hyr += wr * q2.y; hyi += wi * q2.y;
+                hzr += wr * q2.z; hzi += wi * q2.z;
+            }
+            acc_fold(a, p1[i].w, p1[i].x, p1[i].y, p1[i].z, gr, gi, hxr, hxi, hyr, hyi, hzr, hzi);
+        }
+        acc_store(recs + rec_index(t1, t2), a);
+    }
+}
+
+// ------------------------------------------- subdivided (near / self) ---
+// One wave per triangle pair.  The Np x Np point pairs are covered by lanes
+// that each own one test point i and a contiguous range of source points j:
+// rows are taken 64 at a time; when r < 64 rows remain, floor(64/r) lanes
+// share a row and split its j range.  Partial moments are reduced across
+// the wave at the end.
+
+template <bool SELF>
+__device__ void subdiv_pair(const Pt* __restrict__ P1, const Pt* __restrict__ P2, int Np,
+                            int Nq2, double ox, double oy, double oz, double aNear2,
+                            const double* __restrict__ aSelf, double kr, double ki,
+                            Rec* __restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    Acc a;
+    acc_zero(a);
+    for (int base = 0; base < Np; base += 64) {
+        const int r = min(64, Np - base);
+        const int parts = 64 / r;
+        const int part = lane / r;
+        const int i = base + lane % r;
+        if (part < parts) {
+            const int j0 = (Np * part) / parts, j1 = (Np * (part + 1)) / parts;
+            const Pt pi = P1[i];
+            const double xi = pi.x + ox, yi = pi.y + oy, zi = pi.z + oz;
+            const int psub = i / Nq2;
+            double aS2 = 0.0;
+            if (SELF) {
+                double as = aSelf[psub];
+                aS2 = as * as;
+            }
+            double gr = 0, gi = 0, hxr = 0, hxi = 0, hyr = 0, hyi = 0, hzr = 0, hzi = 0;
+            for (int j = j0; j < j1; ++j) {
+                const Pt q2 = P2[j];
+                double dx = xi - q2.x, dy = yi - q2.y, dz = zi - q2.z;
+                double R2 = dx * dx + dy * dy + dz * dz;
+                double R;
+                if (!SELF) {
+                    R = sqrt(R2 + aNear2);       // near_singular.m:41-42
+                } else if (j / Nq2 == psub) {
+                    R = sqrt(R2 + aS2);          // singular.m:46-47
+                } else {
+                    R = sqrt(R2);
+                    R = R > 1e-15 ? R : 1e-15;   // singular.m:48-49
+                }
+                double Gr, Gi;
+                green64(kr, ki, R, Gr, Gi);
+                double wr = q2.w * Gr, wi = q2.w * Gi;
+                gr += wr; gi += wi;
+                hxr += wr * q2.x; hxi += wi * q2.x;
+                hyr += wr * q2.y; hyi += wi * q2.y;
+                hzr += wr * q2.z; hzi += wi * q2.z;
+            }
+            acc_fold(a, pi.w, pi.x, pi.y, pi.z, gr, gi, hxr, hxi, hyr, hyi, hzr, hzi);
+        }
+    }
+    double* v = &a.s0r;
+    double res[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) res[k] = wave_sum(v[k]);
+    if (lane == 0) {
+        double2* p = reinterpret_cast<double2*>(out->v);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) p[k] = make_double2(res[2 * k], res[2 * k + 1]);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_rec_near(const Pt* __restrict__ subPts, int Np, int Nq2,
+                                                  const TriGeo* __restrict__ tg,
+                                                  const double* __restrict__ aeq,
+                                                  double near_alpha,
+                                                  const int2* __restrict__ list, int nPairs,
+                                                  const Freq* __restrict__ fr,
+                                                  Rec* __restrict__ recs) {
+    const int pair = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (pair >= nPairs) return;
+    const int2 tp = list[pair];
+    const TriGeo g1 = tg[tp.x], g2 = tg[tp.y];
+    // a = near_alpha * min(sqrt(A1/pi), sqrt(A2/pi))  (near_singular.m:16-19)
+    double a1 = aeq[tp.x], a2 = aeq[tp.y];
+    double as = near_alpha * (a1 < a2 ? a1 : a2);
+    subdiv_pair<false>(subPts + (size_t)tp.x * Np, subPts + (size_t)tp.y * Np, Np, Nq2,
+                       g1.cx - g2.cx, g1.cy - g2.cy, g1.cz - g2.cz, as * as, nullptr,
+                       fr->kr, fr->ki, recs + rec_index(tp.x, tp.y));
+}
+
+__global__ __launch_bounds__(256) void k_rec_self(const Pt* __restrict__ selfPts, int Nq2, int Ns,
+                                                  const double* __restrict__ aSelf, int Nt,
+                                                  const Freq* __restrict__ fr,
+                                                  Rec* __restrict__ recs) {
+    const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= Nt) return;
+    const int Np = Ns * Nq2;
+    const Pt* P = selfPts + (size_t)t * Np;
+    subdiv_pair<true>(P, P, Np, Nq2, 0.0, 0.0, 0.0, 0.0, aSelf + (size_t)t * Ns, fr->kr, fr->ki,
+                      recs + rec_index(t, t));
+}
+
+// ------------------------------------------------------------ combine ---
+
+template <bool ROWFAST>
+__global__ __launch_bounds__(256) void k_combine(const Rec* __restrict__ recs,
+                                                 const int* __restrict__ sideTri,
+                                                 const Side* __restrict__ side, int Ne,
+                                                 const Freq* __restrict__ fr,
+                                                 double* __restrict__ Z, size_t ldr, size_t ldc) {
+    // 16 x 16 tile; the fast thread index runs along the unit-stride output axis
+    int a = threadIdx.x & 15, b = threadIdx.x >> 4;
+    int m = blockIdx.x * 16 + (ROWFAST ? a : b);
+    int n = blockIdx.y * 16 + (ROWFAST ? b : a);
+    if (m >= Ne || n >= Ne) return;
+    const double ar = fr->ar, ai = fr->ai, br = fr->br, bi = fr->bi;
+    double zr = 0.0, zi = 0.0;
+#pragma unroll
+    for (int sa = 0; sa < 2; ++sa) {
+        const int t1 = sideTri[2 * m + sa];
+        if (t1 < 0) continue;
+        const Side sm = side[2 * m + sa];
+#pragma unroll
+        for (int sb = 0; sb < 2; ++sb) {
+            const int t2 = sideTri[2 * n + sb];
+            if (t2 < 0) continue;
+            const Side sn = side[2 * n + sb];
+            const bool sw = t1 > t2;
+            const double2* rp =
+                reinterpret_cast<const double2*>(recs[sw ? rec_index(t2, t1) : rec_index(t1, t2)].v);
+            const double2 s0 = rp[0], s2 = rp[7];
+            double2 s1x = rp[1], s1y = rp[2], s1z = rp[3];
+            double2 s1px = rp[4], s1py = rp[5], s1pz = rp[6];
+            if (sw) {
+                double2 t;
+                t = s1x; s1x = s1px; s1px = t;
+                t = s1y; s1y = s1py; s1py = t;
+                t = s1z; s1z = s1pz; s1pz = t;
+            }
+            const double uu = sm.ux * sn.ux + sm.uy * sn.uy + sm.uz * sn.uz;
+            // S2 - S1.un - S1'.um + S0 (um.un)
+            double ivr = s2.x - (s1x.x * sn.ux + s1y.x * sn.uy + s1z.x * sn.uz) -
+                         (s1px.x * sm.ux + s1py.x * sm.uy + s1pz.x * sm.uz) + s0.x * uu;
+            double ivi = s2.y - (s1x.y * sn.ux + s1y.y * sn.uy + s1z.y * sn.uz) -
+                         (s1px.y * sm.ux + s1py.y * sm.uy + s1pz.y * sm.uz) + s0.y * uu;
+            const double cc = sm.c * sn.c;
+            ivr *= cc;
+            ivi *= cc;
+            const double isr = 4.0 * cc * s0.x, isi = 4.0 * cc * s0.y;
+            // Zmn + 1j*omega*mu*Ivec + (1/(1j*omega*eps))*Isca
+            zr += ar * ivr - ai * ivi;
+            zi += ar * ivi + ai * ivr;
+            zr += br * isr - bi * isi;
+            zi += br * isi + bi * isr;
+        }
+    }
+    double* o = Z + 2 * ((size_t)m * ldr + (size_t)n * ldc);
+    *reinterpret_cast<double2*>(o) = make_double2(zr, zi);
+}
+
+// ----------------------------------------------- reference centroid fill --
+// kernel_Z_centroid (patch_sweep_cuda.cu:94-153): one point per triangle.
+
+template <bool ROWFAST>
+__global__ __launch_bounds__(256) void k_centroid(const double* __restrict__ pos,
+                                                  const double* __restrict__ f,
+                                                  const double* __restrict__ dv,
+                                                  const double* __restrict__ w,
+                                                  const int* __restrict__ tsd,
+                                                  const double* __restrict__ asoft, int Ne,
+                                                  const Freq* __restrict__ fr,
+                                                  double* __restrict__ Z, size_t ldr, size_t ldc) {
+    int a = threadIdx.x & 15, b = threadIdx.x >> 4;
+    int m = blockIdx.x * 16 + (ROWFAST ? a : b);
+    int n = blockIdx.y * 16 + (ROWFAST ? b : a);
+    if (m >= Ne || n >= Ne) return;
+    const double kr = fr->kr, ki = fr->ki;
+    double zr = 0.0, zi = 0.0;
+    for (int sa = 0; sa < 2; ++sa) {
+        const double wm = w[2 * m + sa];
+        const int tm = tsd[2 * m + sa];
+        if (wm == 0.0 || tm == 0) continue;
+        for (int sb = 0; sb < 2; ++sb) {
+            const double wn = w[2 * n + sb];
+            const int tn = tsd[2 * n + sb];
+            if (wn == 0.0 || tn == 0) continue;
+            const double* pm = pos + 6 * m + 3 * sa;
+            const double* pn = pos + 6 * n + 3 * sb;
+            double dx = pm[0] - pn[0], dy = pm[1] - pn[1], dz = pm[2] - pn[2];
+            double R = sqrt(dx * dx + dy * dy + dz * dz);
+            if (tm == tn) {
+                double as = asoft[tm - 1];
+                R = sqrt(R * R + as * as);
+            }
+            if (R < 1e-15) R = 1e-15;
+            double s, c;
+            sincos(kr * R, &s, &c);
+            double amp = exp(ki * R);
+            double inv = 1.0 / (4.0 * kPi * R);
+            double Gr = amp * c * inv, Gi = -amp * s * inv;
+            const double* fm = f + 6 * m + 3 * sa;
+            const double* fn = f + 6 * n + 3 * sb;
+            double dotff = fm[0] * fn[0] + fm[1] * fn[1] + fm[2] * fn[2];
+            double divprod = dv[2 * m + sa] * dv[2 * n + sb];
+            double ww = wm * wn;
+            double tvr = Gr * (dotff * ww), tvi = Gi * (dotff * ww);
+            double tsr = Gr * (divprod * ww), tsi = Gi * (divprod * ww);
+            double ur = fr->ar * tvr - fr->ai * tvi, ui = fr->ar * tvi + fr->ai * tvr;
+            double vr = fr->br * tsr - fr->bi * tsi, vi = fr->br * tsi + fr->bi * tsr;
+            zr += ur + vr;
+            zi += ui + vi;
+        }
+    }
+    double* o = Z + 2 * ((size_t)m * ldr + (size_t)n * ldc);
+    *reinterpret_cast<double2*>(o) = make_double2(zr, zi);
+}
+
+// ------------------------------------------------- excitation / port ----
+
+__device__ double block_sum_1024(double v, double* sh) {
+    v = wave_sum(v);
+    int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) sh[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) t += sh[k];
+        sh[0] = t;
+    }
+    __syncthreads();
+    t = sh[0];
+    __syncthreads();
+    return t;
+}
+
+// port_weights_gaussian.m:5-28: layer of every edge midpoint from the
+// midpoint z range, Gaussian in the distance to edge0, normalised.
+__global__ __launch_bounds__(1024) void k_port_weights(const double* __restrict__ mid, int Ne,
+                                                       int edge0, double spread,
+                                                       double* __restrict__ out,
+                                                       double* __restrict__ sums, int which) {
+    __shared__ double sh[32];
+    __shared__ double zr[2];
+    double zmin = INFINITY, zmax = -INFINITY;
+    for (int e = threadIdx.x; e < Ne; e += blockDim.x) {
+        double z = mid[3 * e + 2];
+        zmin = fmin(zmin, z);
+        zmax = fmax(zmax, z);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        zmin = fmin(zmin, __shfl_xor(zmin, o, 64));
+        zmax = fmax(zmax, __shfl_xor(zmax, o, 64));
+    }
+    int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) { sh[w] = zmin; sh[16 + w] = zmax; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = INFINITY, b = -INFINITY;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) { a = fmin(a, sh[k]); b = fmax(b, sh[16 + k]); }
+        zr[0] = a;
+        zr[1] = b;
+    }
+    __syncthreads();
+    const double zmid = 0.5 * (zr[0] + zr[1]);
+    const double x0 = mid[3 * edge0], y0 = mid[3 * edge0 + 1], z0 = mid[3 * edge0 + 2];
+    const int lay0 = z0 > zmid ? 1 : -1;
+    double part = 0.0;
+    for (int e = threadIdx.x; e < Ne; e += blockDim.x) {
+        double dx = mid[3 * e] - x0, dy = mid[3 * e + 1] - y0, dz = mid[3 * e + 2] - z0;
+        double t = sqrt(dx * dx + dy * dy + dz * dz) / spread;
+        int lay = mid[3 * e + 2] > zmid ? 1 : -1;
+        double wt = exp(-(t * t)) * (lay == lay0 ? 1.0 : 0.0);
+        out[e] = wt;
+        part += wt;
+    }
+    double tot = block_sum_1024(part, sh);
+    for (int e = threadIdx.x; e < Ne; e += blockDim.x) out[e] = out[e] / tot;
+    if (threadIdx.x == 0) sums[which] = tot;
+}
+
+__global__ void k_rhs(const double* __restrict__ wP, const double* __restrict__ wG,
+                      const double* __restrict__ len, int Ne, double V0,
+                      double* __restrict__ Vrhs, double* __restrict__ wdiff) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= Ne) return;
+    // V = (0 + (+V0/2) wP) + (-V0/2) wG   (assemble_ZV_EFIE_RWG_DIFF_CPU.m:43-51)
+    double v = (0.0 + (V0 / 2) * wP[e]) + (-V0 / 2) * wG[e];
+    Vrhs[2 * e] = v;
+    Vrhs[2 * e + 1] = 0.0;
+    wdiff[2 * e] = wP[e] * len[e];
+    wdiff[2 * e + 1] = wG[e] * len[e];
+}
+
+// Zin epilogue (main_patch_efie_rwg.m:336-340, patch_sweep_fastquad_cuda.cu:
+// 787-799).  One block per batch element; X holds the solution column.
+__global__ __launch_bounds__(1024) void k_epilogue(const double* __restrict__ X, size_t strideX,
+                                                   size_t xinc, const double* __restrict__ wl, int Ne,
+                                                   int pe, int ge, int centroidMode, double V0,
+                                                   double Z0, double* __restrict__ out) {
+    __shared__ double sh[32];
+    const double* x = X + (size_t)blockIdx.x * strideX;
+    double ipr, ipi;
+    if (centroidMode) {
+        // Iport = x[p] - x[g]  (patch_sweep_cuda.cu:489-491)
+        ipr = x[xinc * pe] - x[xinc * ge];
+        ipi = x[xinc * pe + 1] - x[xinc * ge + 1];
+    } else {
+        double pr = 0, pim = 0, gr = 0, gim = 0;
+        for (int e = threadIdx.x; e < Ne; e += blockDim.x) {
+            double xr = x[xinc * e], xi = x[xinc * e + 1];
+            pr += wl[2 * e] * xr; pim += wl[2 * e] * xi;
+            gr += wl[2 * e + 1] * xr; gim += wl[2 * e + 1] * xi;
+        }
+        pr = block_sum_1024(pr, sh);
+        pim = block_sum_1024(pim, sh);
+        gr = block_sum_1024(gr, sh);
+        gim = block_sum_1024(gim, sh);
+        ipr = pr - gr;
+        ipi = pim - gim;
+    }
+    if (threadIdx.x == 0) {
+        double den = ipr * ipr + ipi * ipi;
+        double zr = V0 * (ipr / den), zi = -V0 * (ipi / den);
+        double a_r = zr - Z0, a_i = zi, b_r = zr + Z0, b_i = zi;
+        double bden = b_r * b_r + b_i * b_i;
+        double* o = out + 4 * blockIdx.x;
+        o[0] = zr;
+        o[1] = zi;
+        o[2] = (a_r * b_r + a_i * b_i) / bden;
+        o[3] = (a_i * b_r - a_r * b_i) / bden;
+    }
+}
+
+// ------------------------------------------------------------ launchers --
+
+hipError_t launch_near_map(const TriGeo* tg, int Nt, double eps, uint8_t* nearMat, int* rowCount,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(k_near_map, dim3(Nt), dim3(256), 0, s, tg, Nt, eps, nearMat, rowCount);
+    return hipGetLastError();
+}
+
+hipError_t launch_near_list(const uint8_t* nearMat, const int* rowStart, int Nt, int2* list,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(k_near_list, dim3(Nt), dim3(256), 0, s, nearMat, rowStart, Nt, list);
+    return hipGetLastError();
+}
+
+hipError_t launch_rec_regular(const Pt* regPts, int Nq, const TriGeo* tg, int Nt,
+                              const uint8_t* nearMat, const Freq* fr, Rec* recs, hipStream_t s) {
+    long long T = (Nt + kRegTile - 1) / kRegTile;
+    long long nb = T * (T + 1) / 2;
+    dim3 g((unsigned)nb), blk(256);
+    switch (Nq) {
+        case 1: hipLaunchKernelGGL(k_rec_regular<1>, g, blk, 0, s, regPts, tg, Nt, nearMat, fr, recs); break;
+        case 3: hipLaunchKernelGGL(k_rec_regular<3>, g, blk, 0, s, regPts, tg, Nt, nearMat, fr, recs); break;
+        case 6: hipLaunchKernelGGL(k_rec_regular<6>, g, blk, 0, s, regPts, tg, Nt, nearMat, fr, recs); break;
+        case 7: hipLaunchKernelGGL(k_rec_regular<7>, g, blk, 0, s, regPts, tg, Nt, nearMat, fr, recs); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_rec_near(const Pt* subPts, int Np, const TriGeo* tg, const double* aeq,
+                           double near_alpha, const int2* list, int nPairs, const Freq* fr,
+                           Rec* recs, hipStream_t s) {
+    if (nPairs <= 0) return hipSuccess;
+    // Nq2 only matters for the self kernel; pass Np so psub is unused
+    hipLaunchKernelGGL(k_rec_near, dim3((nPairs + 3) / 4), dim3(256), 0, s, subPts, Np, Np, tg, aeq,
+                       near_alpha, list, nPairs, fr, recs);
+    return hipGetLastError();
+}
+
+hipError_t launch_rec_self(const Pt* selfPts, int Nq2, int Ns, const double* aSelf, int Nt,
+                           const Freq* fr, Rec* recs, hipStream_t s) {
+    hipLaunchKernelGGL(k_rec_self, dim3((Nt + 3) / 4), dim3(256), 0, s, selfPts, Nq2, Ns, aSelf, Nt,
+                       fr, recs);
+    return hipGetLastError();
+}
+
+hipError_t launch_combine(const Rec* recs, const int* sideTri, const Side* side, int Ne,
+                          const Freq* fr, double* Z, size_t ldr, size_t ldc, hipStream_t s) {
+    dim3 g((Ne + 15) / 16, (Ne + 15) / 16), blk(256);
+    if (ldr == 1)
+        hipLaunchKernelGGL(k_combine<true>, g, blk, 0, s, recs, sideTri, side, Ne, fr, Z, ldr, ldc);
+    else
+        hipLaunchKernelGGL(k_combine<false>, g, blk, 0, s, recs, sideTri, side, Ne, fr, Z, ldr, ldc);
+    return hipGetLastError();
+}
+
+hipError_t launch_centroid(const double* pos, const double* f, const double* dv, const double* w,
+                           const int* tsd, const double* asoft, int Ne, const Freq* fr, double* Z,
+                           size_t ldr, size_t ldc, hipStream_t s) {
+    dim3 g((Ne + 15) / 16, (Ne + 15) / 16), blk(256);
+    if (ldr == 1)
+        hipLaunchKernelGGL(k_centroid<true>, g, blk, 0, s, pos, f, dv, w, tsd, asoft, Ne, fr, Z, ldr, ldc);
+    else
+        hipLaunchKernelGGL(k_centroid<false>, g, blk, 0, s, pos, f, dv, w, tsd, asoft, Ne, fr, Z, ldr, ldc);
+    return hipGetLastError();
+}
+
+hipError_t launch_port_weights(const double* mid, int Ne, int edge0, double spread, double* out,
+                               double* sums, int which, hipStream_t s) {
+    hipLaunchKernelGGL(k_port_weights, dim3(1), dim3(1024), 0, s, mid, Ne, edge0, spread, out, sums,
+                       which);
+    return hipGetLastError();
+}
+
+hipError_t launch_rhs(const double* wP, const double* wG, const double* len, int Ne, double V0,
+                      double* Vrhs, double* wdiff, hipStream_t s) {
+    hipLaunchKernelGGL(k_rhs, dim3((Ne + 255) / 256), dim3(256), 0, s, wP, wG, len, Ne, V0, Vrhs, wdiff);
+    return hipGetLastError();
+}
+
+hipError_t launch_epilogue(const double* X, size_t strideX, size_t xinc, const double* wdiff,
+                           int Ne, int patchEdge0, int groundEdge0, int centroidMode, double V0,
+                           double Z0, int batch, double* ZinS11, hipStream_t s) {
+    hipLaunchKernelGGL(k_epilogue, dim3(batch), dim3(1024), 0, s, X, strideX, xinc, wdiff, Ne, patchEdge0,
+                       groundEdge0, centroidMode, V0, Z0, ZinS11);
+    return hipGetLastError();
+}
+
+}  // namespace efie

@@ -1,0 +1,125 @@
+// efie_internal.h -- device data layout shared by the host orchestration
+// (efie_api.cpp) and the kernels (efie_fill.hip, efie_lu.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace efie {
+
+// Quadrature point relative to its triangle's centroid, with its full
+// weight (rule weight times twice the (sub-)triangle area).
+struct Pt {
+    double x, y, z, w;
+};
+
+// Per-triangle data: centroid and twice the area (precompute_tri_geom.m).
+struct TriGeo {
+    double cx, cy, cz, area2;
+};
+
+// One side of an RWG basis function restricted to one triangle:
+// f(r) = c * (r - v) and div f = 2c with c = +s*l/(2A) on the plus
+// triangle and -s*l/(2A) on the minus triangle (rwg_eval_on_triangle.m).
+// u = v - centroid(tri).
+struct Side {
+    double ux, uy, uz, c;
+};
+
+// Moments of one unordered triangle pair {t1 <= t2} for one frequency,
+// with rho = r - centroid(t1), rho' = r' - centroid(t2):
+//   S0 = sum W G,  S1 = sum W G rho,  S1p = sum W G rho',
+//   S2 = sum W G (rho . rho').
+// Stored as 16 doubles in the order S0, S1x, S1y, S1z, S1px, S1py, S1pz, S2
+// (re, im interleaved); 128 bytes, one cache line.
+struct Rec {
+    double v[16];
+};
+
+// Packed upper-triangular index of the record of pair (t1 <= t2),
+// column-major over t2 so that consecutive t1 are consecutive records.
+__host__ __device__ inline size_t rec_index(int t1, int t2) {
+    return (size_t)t2 * (size_t)(t2 + 1) / 2 + (size_t)t1;
+}
+
+// Scalars of one frequency.
+struct Freq {
+    double kr, ki;       // k = kr + j ki
+    double ar, ai;       // alpha = j omega mu
+    double br, bi;       // beta = 1/(j omega eps)
+};
+
+// --------------------------------------------------------- launchers ----
+// All launchers enqueue on `s` and return hipGetLastError().
+
+hipError_t launch_near_map(const TriGeo* tg, int Nt, double eps, uint8_t* nearMat,
+                           int* rowCount, hipStream_t s);
+hipError_t launch_near_list(const uint8_t* nearMat, const int* rowStart, int Nt,
+                            int2* list, hipStream_t s);
+
+// Regular (non-near, t1 < t2) records; nearMat excludes near pairs.
+hipError_t launch_rec_regular(const Pt* regPts, int Nq, const TriGeo* tg, int Nt,
+                              const uint8_t* nearMat, const Freq* fr, Rec* recs,
+                              hipStream_t s);
+// Subdivided near-pair records (rwg_pair_integrals_near_singular.m).
+hipError_t launch_rec_near(const Pt* subPts, int Np, const TriGeo* tg, const double* aeq,
+                           double near_alpha, const int2* list, int nPairs,
+                           const Freq* fr, Rec* recs, hipStream_t s);
+// Self records (rwg_pair_integrals_singular.m): Np points per triangle in
+// Ns sub-triangles of Nq2 points; aSelf[t*Ns + p] is the softening length.
+hipError_t launch_rec_self(const Pt* selfPts, int Nq2, int Ns, const double* aSelf,
+                           int Nt, const Freq* fr, Rec* recs, hipStream_t s);
+
+// Z(m,n) = sum over the four side pairs of alpha*Ivec + beta*Isca.
+// Output element (m,n) goes to Z[m*ldr + n*ldc] (interleaved complex).
+hipError_t launch_combine(const Rec* recs, const int* sideTri, const Side* side, int Ne,
+                          const Freq* fr, double* Z, size_t ldr, size_t ldc, hipStream_t s);
+
+// Reference centroid model (kernel_Z_centroid).
+hipError_t launch_centroid(const double* pos, const double* f, const double* dv,
+                           const double* w, const int* tsd, const double* asoft, int Ne,
+                           const Freq* fr, double* Z, size_t ldr, size_t ldc, hipStream_t s);
+
+// Gaussian edge-strip weights on the edge midpoints of one layer,
+// normalised (port_weights_gaussian.m): out[e] = w_e / sum(w), and
+// sums[which] = sum(w).
+hipError_t launch_port_weights(const double* mid, int Ne, int edge0, double spread,
+                               double* out, double* sums, int which, hipStream_t s);
+// V = +V0/2 wP - V0/2 wG (interleaved complex, imag 0) and
+// wdiff = (wP - wG) .* len for the Zin epilogue.
+hipError_t launch_rhs(const double* wP, const double* wG, const double* len, int Ne,
+                      double V0, double* Vrhs, double* wdiff, hipStream_t s);
+
+// --------------------------------------------------------------- LU -----
+// Batched in-place LU with tournament (CALU) pivoting of nb batch matrices
+// stored row-major with leading dimension lda (A[b*strideA + i*lda + j]),
+// interleaved complex of type T (double or float).  Column n (< lda) holds
+// the right-hand side and is carried through all row operations, so after
+// the factorisation it holds L^{-1} P b.
+struct LuWork {
+    int* cand;      // tournament candidates
+    int* counters;  // tree arrival counters
+    int* ipiv;      // [batch][n] 1-based LAPACK-style pivots
+    int* info;      // [batch] first zero pivot (1-based) or 0
+};
+
+// Tournament tree nodes of an n-row panel; cand needs
+// batch * nodes * lu_cand_stride() ints, counters batch * nodes (zeroed once).
+int lu_tree_nodes(int n);
+int lu_cand_stride();
+template <typename T>
+hipError_t lu_factor(T* A, size_t strideA, int n, int lda, int batch, LuWork w,
+                     hipStream_t s);
+// Back substitution U x = y in place on column n of A.
+template <typename T>
+hipError_t lu_backsolve(T* A, size_t strideA, int n, int lda, int batch, hipStream_t s);
+
+// Zin epilogue of batch element b: x[e] = X[b*strideX + e*xinc] (complex),
+// Iport = sum(wP.*len.*x) - sum(wG.*len.*x) (or x[p]-x[g] in the centroid
+// model), Zin = V0/Iport, S11 = (Zin-Z0)/(Zin+Z0); ZinS11[4b..4b+3].
+hipError_t launch_epilogue(const double* X, size_t strideX, size_t xinc, const double* wdiff,
+                           int Ne, int patchEdge0, int groundEdge0, int centroidMode, double V0,
+                           double Z0, int batch, double* ZinS11, hipStream_t s);
+
+}  // namespace efie

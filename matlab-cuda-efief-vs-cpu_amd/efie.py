@@ -1,0 +1,292 @@
+"""Python front end of libefie_hip.so (the MI355X EFIE hot path).
+
+Mirrors the reference's host-side MATLAB functions and its two MEX entry
+points with the same names, argument order and meaning:
+
+  make_patch_with_ground_mesh   make_patch_with_ground_mesh.m:1-46
+  build_rwg_interior            build_rwg_interior.m:1-113
+  pick_port_edge_by_point_and_layer  pick_port_edge_by_point_and_layer.m:1-11
+  patch_sweep_cuda              patch_sweep_cuda.cu:4-27      -> (Zin, S11)
+  gassemble_ZV_EFIE_RWG_DIFF_GPU_FAST  gassemble_ZV_EFIE_RWG_DIFF_GPU_FAST.cu:127-148 -> (Z, Vrhs)
+
+All compute goes through the C ABI declared in include/efie.h; there is no
+fallback: if the shared library is missing this module raises.  Index
+tables are 1-based int32 and matrices column-major, as in MATLAB.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libefie_hip.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+EFIE_OK = 0
+MODEL_QUADRATURE, MODEL_CENTROID = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
+
+# physics constants of the reference drivers (main_patch_efie_rwg.m:218-220)
+C0 = 299792458.0
+MU0 = 4 * np.pi * 1e-7
+EPS0 = 1.0 / (MU0 * C0 ** 2)
+
+
+class EfieError(RuntimeError):
+    """Error raised from the C ABI; `ident` is the reference MEX error id."""
+
+    def __init__(self, status, ident, msg):
+        super().__init__(f"{ident}: {msg}")
+        self.status = status
+        self.ident = ident
+
+
+class Opts(C.Structure):
+    _fields_ = [
+        ("quad_order", C.c_int), ("near_sing_eps", C.c_double), ("self_subdiv", C.c_int),
+        ("self_alpha", C.c_double), ("near_subdiv", C.c_int), ("near_alpha", C.c_double),
+        ("near_order", C.c_int), ("model", C.c_int), ("precision", C.c_int), ("batch", C.c_int),
+        ("ir_tol", C.c_double), ("ir_max_iter", C.c_int),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("ms_total", C.c_double), ("ms_fill", C.c_double), ("ms_factor", C.c_double),
+        ("ms_solve", C.c_double), ("ms_fill_kernel", C.c_double), ("n_fill_kernel", C.c_int),
+        ("n_near_pairs", C.c_longlong), ("n_green", C.c_longlong), ("ir_iters_max", C.c_int),
+        ("ir_resid_max", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+EXPORTS = [
+    "efie_opts_default", "efie_opts_preset", "efie_last_error", "efie_last_error_id",
+    "efie_mesh_counts", "efie_make_mesh", "efie_rwg_count", "efie_build_rwg", "efie_pick_port",
+    "efie_create", "efie_destroy", "efie_set_mesh", "efie_set_rwg", "efie_assemble_zv",
+    "efie_sweep",
+]
+
+_lib = None
+
+
+def lib():
+    """Load libefie_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(
+                f"{LIB_PATH} is missing: build it with python matlab-cuda-efief-vs-cpu_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        L.efie_opts_default.argtypes = [C.POINTER(Opts)]
+        L.efie_opts_default.restype = None
+        L.efie_opts_preset.argtypes = [C.POINTER(Opts), C.c_char_p]
+        L.efie_last_error.restype = C.c_char_p
+        L.efie_last_error_id.restype = C.c_char_p
+        L.efie_mesh_counts.argtypes = [C.c_int] * 4 + [_ip, _ip]
+        L.efie_make_mesh.argtypes = [C.c_double] * 5 + [C.c_int] * 4 + [_dp, _ip]
+        L.efie_rwg_count.argtypes = [_ip, C.c_int, C.c_int, _ip]
+        L.efie_build_rwg.argtypes = [_dp, C.c_int, _ip, C.c_int, C.c_int] + [_ip] * 5 + [_dp] * 6 + [_ip]
+        L.efie_pick_port.argtypes = [C.c_int, _dp, _ip, _dp, C.c_int, _ip]
+        L.efie_create.argtypes = [C.POINTER(C.c_void_p), C.c_int]
+        L.efie_destroy.argtypes = [C.c_void_p]
+        L.efie_destroy.restype = None
+        L.efie_set_mesh.argtypes = [C.c_void_p, _dp, C.c_int, _ip, C.c_int]
+        L.efie_set_rwg.argtypes = [C.c_void_p] + [_ip] * 5 + [_dp] * 5 + [C.c_int]
+        L.efie_assemble_zv.argtypes = [C.c_void_p] + [C.c_double] * 6 + [_dp, C.c_int, C.POINTER(Opts), _dp, _dp]
+        L.efie_sweep.argtypes = [C.c_void_p, _dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                 C.c_double, C.c_double, C.c_double, C.POINTER(Opts), _dp, _dp, C.POINTER(Stats)]
+        _lib = L
+    return _lib
+
+
+def _check(st):
+    if st != EFIE_OK:
+        L = lib()
+        raise EfieError(st, L.efie_last_error_id().decode(), L.efie_last_error().decode())
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def opts(preset: str = "accurate", **kw) -> Opts:
+    o = Opts()
+    _check(lib().efie_opts_preset(C.byref(o), preset.encode()))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise AttributeError(k)
+        setattr(o, k, v)
+    return o
+
+
+# ----------------------------------------------------------- host tables --
+
+def make_patch_with_ground_mesh(Lp, Wp, Lg, Wg, h, Np_x, Np_y, Ng_x, Ng_y):
+    """[V, F] of make_patch_with_ground_mesh.m (F 1-based int32)."""
+    Nv, Nt = C.c_int(), C.c_int()
+    _check(lib().efie_mesh_counts(Np_x, Np_y, Ng_x, Ng_y, C.byref(Nv), C.byref(Nt)))
+    V = np.zeros((Nv.value, 3), order="F")
+    F = np.zeros((Nt.value, 3), dtype=np.int32, order="F")
+    _check(lib().efie_make_mesh(Lp, Wp, Lg, Wg, h, Np_x, Np_y, Ng_x, Ng_y, _d(V), _i(F)))
+    return V, F
+
+
+@dataclass
+class Rwg:
+    """The rwg struct of build_rwg_interior.m."""
+    edge: np.ndarray
+    plusTri: np.ndarray
+    minusTri: np.ndarray
+    plusSign: np.ndarray
+    minusSign: np.ndarray
+    rp: np.ndarray
+    rm: np.ndarray
+    Ap: np.ndarray
+    Am: np.ndarray
+    len: np.ndarray
+    center: np.ndarray
+    zSign: np.ndarray
+
+    @property
+    def Ne(self) -> int:
+        return int(self.plusTri.shape[0])
+
+
+def build_rwg_interior(V, F) -> Rwg:
+    V = np.asfortranarray(V, dtype=np.float64)
+    F = np.asfortranarray(F, dtype=np.int32)
+    Nv, Nt = V.shape[0], F.shape[0]
+    ne = C.c_int()
+    _check(lib().efie_rwg_count(_i(F), Nt, Nv, C.byref(ne)))
+    Ne = ne.value
+    r = Rwg(edge=np.zeros((Ne, 2), np.int32, order="F"), plusTri=np.zeros(Ne, np.int32),
+            minusTri=np.zeros(Ne, np.int32), plusSign=np.zeros(Ne, np.int32),
+            minusSign=np.zeros(Ne, np.int32), rp=np.zeros((Ne, 3), order="F"),
+            rm=np.zeros((Ne, 3), order="F"), Ap=np.zeros(Ne), Am=np.zeros(Ne), len=np.zeros(Ne),
+            center=np.zeros((Ne, 3), order="F"), zSign=np.zeros(Ne, np.int32))
+    _check(lib().efie_build_rwg(_d(V), Nv, _i(F), Nt, Ne, _i(r.edge), _i(r.plusTri), _i(r.minusTri),
+                                _i(r.plusSign), _i(r.minusSign), _d(r.rp), _d(r.rm), _d(r.Ap), _d(r.Am),
+                                _d(r.len), _d(r.center), _i(r.zSign)))
+    return r
+
+
+def pick_port_edge_by_point_and_layer(rwg: Rwg, p, layerSign) -> int:
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    e = C.c_int()
+    _check(lib().efie_pick_port(rwg.Ne, _d(np.asfortranarray(rwg.center)), _i(rwg.zSign), _d(p),
+                                int(layerSign), C.byref(e)))
+    return e.value
+
+
+# ---------------------------------------------------------------- device --
+
+class Context:
+    """A device context holding one mesh and its RWG tables in HBM."""
+
+    def __init__(self, device: int = 0):
+        self._c = C.c_void_p()
+        _check(lib().efie_create(C.byref(self._c), device))
+        self._keep = []
+
+    def close(self):
+        if self._c:
+            lib().efie_destroy(self._c)
+            self._c = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_mesh(self, V, F):
+        V = np.asfortranarray(V, dtype=np.float64)
+        F = np.asfortranarray(F, dtype=np.int32)
+        _check(lib().efie_set_mesh(self._c, _d(V), V.shape[0], _i(F), F.shape[0]))
+        return self
+
+    def set_rwg(self, edge, plusTri, minusTri, plusSign, minusSign, rp, rm, Ap, Am, len_):
+        a = [np.asfortranarray(x, dtype=np.int32) for x in (edge, plusTri, minusTri, plusSign, minusSign)]
+        b = [np.asfortranarray(x, dtype=np.float64) for x in (rp, rm, Ap, Am, len_)]
+        Ne = a[1].shape[0]
+        _check(lib().efie_set_rwg(self._c, *[_i(x) for x in a], *[_d(x) for x in b], Ne))
+        self.Ne = Ne
+        return self
+
+    def set_problem(self, V, F, rwg: Rwg):
+        self.set_mesh(V, F)
+        return self.set_rwg(rwg.edge, rwg.plusTri, rwg.minusTri, rwg.plusSign, rwg.minusSign,
+                            rwg.rp, rwg.rm, rwg.Ap, rwg.Am, rwg.len)
+
+    def assemble_zv(self, k, omega, mu, eps, exc, o: Optional[Opts] = None):
+        k, eps = complex(k), complex(eps)
+        exc = np.ascontiguousarray(exc, dtype=np.float64)
+        Ne = self.Ne
+        Z = np.zeros(2 * Ne * Ne)
+        Vr = np.zeros(2 * Ne)
+        o = o or opts()
+        _check(lib().efie_assemble_zv(self._c, k.real, k.imag, omega, mu, eps.real, eps.imag, _d(exc),
+                                      exc.shape[0], C.byref(o), _d(Z), _d(Vr)))
+        return Z.view(np.complex128).reshape((Ne, Ne), order="F"), Vr.view(np.complex128).copy()
+
+    def sweep(self, freqs, mu, eps, patchEdge, groundEdge, V0, spreadRadius, Z0,
+              o: Optional[Opts] = None, stats: bool = False):
+        freqs = np.ascontiguousarray(np.ravel(freqs), dtype=np.float64)
+        Nf = freqs.shape[0]
+        eps = complex(eps)
+        zin = np.zeros(2 * Nf)
+        s11 = np.zeros(2 * Nf)
+        st = Stats()
+        o = o or opts()
+        _check(lib().efie_sweep(self._c, _d(freqs), Nf, mu, eps.real, eps.imag, int(patchEdge), int(groundEdge),
+                                V0, spreadRadius, Z0, C.byref(o), _d(zin), _d(s11), C.byref(st)))
+        out = (zin.view(np.complex128).copy(), s11.view(np.complex128).copy())
+        return (*out, st.as_dict()) if stats else out
+
+
+# ------------------------------------------------- reference MEX mirrors --
+
+def patch_sweep_cuda(V, F, edge, plusTri, minusTri, plusSign, minusSign, rp, rm, Ap, Am, len_,
+                     freqs, mu, eps, patchEdge, groundEdge, V0, spreadRadius, Z0,
+                     o: Optional[Opts] = None, device: int = 0):
+    """[Zin, S11] = patch_sweep_cuda(...) with the reference's 20 arguments
+    (patch_sweep_cuda.cu:4-27); `o` is the optional extra options argument."""
+    with Context(device) as ctx:
+        ctx.set_mesh(V, F)
+        ctx.set_rwg(edge, plusTri, minusTri, plusSign, minusSign, rp, rm, Ap, Am, len_)
+        return ctx.sweep(freqs, mu, eps, patchEdge, groundEdge, V0, spreadRadius, Z0, o)
+
+
+def gassemble_ZV_EFIE_RWG_DIFF_GPU_FAST(V, F, E, plusTri, minusTri, plusSign, minusSign, rp, rm,
+                                        Ap, Am, len_, k, omega, mu, eps, exc,
+                                        o: Optional[Opts] = None, device: int = 0):
+    """[Z, Vrhs] with the reference's 17 arguments
+    (gassemble_ZV_EFIE_RWG_DIFF_GPU_FAST.cu:127-148)."""
+    with Context(device) as ctx:
+        ctx.set_mesh(V, F)
+        ctx.set_rwg(E, plusTri, minusTri, plusSign, minusSign, rp, rm, Ap, Am, len_)
+        return ctx.assemble_zv(k, omega, mu, eps, exc, o)
+
+
+def wavenumber(omega, mu, eps):
+    """k = omega*sqrt(mu*eps), principal root (main_patch_efie_rwg.m:319)."""
+    return omega * np.sqrt(complex(mu) * complex(eps))
