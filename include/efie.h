@@ -56,9 +56,35 @@ typedef struct {
     int model;          /* EFIE_MODEL_*                                       */
     int precision;      /* EFIE_PREC_*                                        */
     int batch;          /* frequencies factored together (0 = automatic)      */
-    double ir_tol;      /* mixed: stop refinement at ||V-Zx||/||V|| <= ir_tol  */
+    double ir_tol;      /* mixed: stop refinement at ||V-Zx||/||V|| <= ir_tol  *
+                         * or when a correction no longer halves the residual */
     int ir_max_iter;    /* mixed: refinement iteration cap                    */
+    int near_mode;      /* EFIE_NEAR_*: how near and self pairs are integrated */
+    int profile;        /* 1: time every kernel class into efie_stats          */
 } efie_opts;
+
+/* near_mode: DIRECT evaluates the subdivided near/self quadrature at every
+ * frequency as the reference does; MOMENTS expands the Green's function in
+ * powers of k R, integrates the frequency-independent moments once per mesh
+ * and sums the series per frequency (tail below 1e-17 relative; falls back
+ * to DIRECT when k R is too large for that). */
+enum { EFIE_NEAR_DIRECT = 0, EFIE_NEAR_MOMENTS = 1 };
+
+/* Kernel classes timed when efie_opts.profile is set. */
+enum {
+    EFIE_K_REGULAR = 0,  /* regular triangle-pair records                     */
+    EFIE_K_NEAR = 1,     /* near-pair records (direct quadrature)             */
+    EFIE_K_SELF = 2,     /* self-pair records (direct quadrature)             */
+    EFIE_K_POLY = 3,     /* near+self records from moments                    */
+    EFIE_K_COMBINE = 4,  /* Z(m,n) from records                               */
+    EFIE_K_TSLU = 5,     /* LU panel pivot selection                          */
+    EFIE_K_LASWP = 6,    /* LU row interchanges                               */
+    EFIE_K_PANEL = 7,    /* LU panel triangular solves                        */
+    EFIE_K_GEMM = 8,     /* LU trailing update (matrix cores)                 */
+    EFIE_K_TRSV = 9,     /* triangular solves                                 */
+    EFIE_K_OTHER = 10,   /* conversions, residuals, epilogue                  */
+    EFIE_NKCLASS = 11
+};
 
 /* Per-call timings (HIP events on the context stream) and counters. */
 typedef struct {
@@ -66,12 +92,15 @@ typedef struct {
     double ms_fill;         /* Z fill (all fill kernels)                     */
     double ms_factor;       /* LU factorisation                              */
     double ms_solve;        /* triangular solves + refinement + epilogue     */
-    double ms_fill_kernel;  /* near-pair record kernel, summed over calls    */
-    int n_fill_kernel;      /* launches of that kernel                       */
     long long n_near_pairs; /* unordered near triangle pairs                 */
     long long n_green;      /* Green-function evaluations per frequency      */
     int ir_iters_max;       /* mixed: most refinement iterations used         */
     double ir_resid_max;    /* mixed: largest final ||V-Zx||/||V||            */
+    int moment_terms;       /* series length P+1 used by MOMENTS (0: direct)  */
+    double ms_moments;      /* one-off moment integration (when it ran)       */
+    /* profile == 1: summed device time and launch count per kernel class  */
+    double ms_kernel[EFIE_NKCLASS];
+    int n_kernel[EFIE_NKCLASS];
 } efie_stats;
 
 /* "accurate" preset (main_patch_efie_rwg.m:281-289), FP64, quadrature. */
@@ -141,6 +170,16 @@ efie_status efie_sweep(efie_ctx* ctx, const double* freqs, int Nf, double mu,
                        double eps_re, double eps_im, int patchEdge,
                        int groundEdge, double V0, double spread, double Z0,
                        const efie_opts* opts, double* Zin, double* S11,
+                       efie_stats* stats);
+
+/* Dense complex solve A x = b for `batch` independent systems (A: n x n
+ * column-major interleaved complex, batch after batch; b, x: n each), the
+ * cusolverDnZgetrf + cusolverDnZgetrs pair of the reference
+ * (patch_sweep_cuda.cu:473-480).  precision FP64: blocked LU with
+ * tournament pivoting on the matrix cores; MIXED: complex FP32 LU plus FP64
+ * iterative refinement (ir_tol 1e-14, at most 10 corrections). */
+efie_status efie_zgesv(efie_ctx* ctx, int n, int batch, const double* A,
+                       const double* b, int precision, double* x,
                        efie_stats* stats);
 
 #ifdef __cplusplus

@@ -183,18 +183,29 @@ struct efie_ctx {
     // geometry cache (quadrature tables etc.)
     bool geoValid = false;
     GeoKey key;
-    int Nq = 0, Nq2n = 0, NpNear = 0, Nq2s = 0, NsSelf = 0;
+    int Nq = 0, NpNear = 0, Nq2s = 0, NsSelf = 0;
     long long nNear = 0;
+    double rNearMax = 0, rSelfMax = 0, lMax = 0, aNearMax = 0, aSelfMax = 0;
+    // moment cache (near_mode MOMENTS), valid for series length momP + 1
+    int momP = -1;
+    double nearAlphaMom = -1;
 
     DBuf dTri, dReg, dNearPts, dSelfPts, dASelf, dAeq, dNearMat, dRowCnt, dList;
-    DBuf dSideTri, dSide, dMid, dLen;
+    DBuf dSideTri, dSide, dMid, dLen, dQn, dQs;
     // centroid model tables
     DBuf dCPos, dCF, dCDv, dCW, dCT, dCAsoft;
     double cAlpha = -1;
     // per-call buffers
     DBuf dRecs, dFreq, dZ, dZ32, dWp, dWg, dSums, dV, dWl, dOut;
-    DBuf dCand, dCnt, dIpiv, dInfo;
-    size_t cntBytes = 0;
+    DBuf dCand, dCnt, dIpiv, dInfo, dPerm, dX, dR, dR32, dRn2, dBn2, dDone, dIters, dResid, dB;
+
+    // profiling (opts.profile): event pairs per kernel class
+    bool prof = false;
+    std::vector<hipEvent_t> pool;
+    size_t poolUsed = 0;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
+    int openCls = -1;
+    hipEvent_t openEv = nullptr;
 };
 
 extern "C" {
@@ -214,8 +225,10 @@ void efie_opts_default(efie_opts* o) {
     o->model = EFIE_MODEL_QUADRATURE;
     o->precision = EFIE_PREC_FP64;
     o->batch = 0;
-    o->ir_tol = 1e-12;
+    o->ir_tol = 1e-14;
     o->ir_max_iter = 10;
+    o->near_mode = EFIE_NEAR_DIRECT;
+    o->profile = 0;
 }
 
 efie_status efie_opts_preset(efie_opts* o, const char* name) {
@@ -417,6 +430,7 @@ void efie_destroy(efie_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto& ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
+    for (auto& ev : c->pool) (void)hipEventDestroy(ev);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;  // DBuf destructors free device memory
 }
@@ -477,6 +491,53 @@ efie_status efie_set_rwg(efie_ctx* c, const int* edge, const int* plusTri, const
 
 namespace {
 
+// ------------------------------------------------------------ profiling --
+
+void prof_mark(void* user, int cls, bool start) {
+    efie_ctx* c = static_cast<efie_ctx*>(user);
+    auto next = [c]() {
+        if (c->poolUsed == c->pool.size()) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return (hipEvent_t) nullptr;
+            c->pool.push_back(e);
+        }
+        return c->pool[c->poolUsed++];
+    };
+    hipEvent_t e = next();
+    if (!e) return;
+    (void)hipEventRecord(e, c->stream);
+    if (start) {
+        c->openCls = cls;
+        c->openEv = e;
+    } else if (c->openCls == cls && c->openEv) {
+        c->marks.push_back({cls, {c->openEv, e}});
+        c->openEv = nullptr;
+        c->openCls = -1;
+    }
+}
+
+Prof make_prof(efie_ctx* c) {
+    Prof p;
+    if (c->prof) {
+        p.user = c;
+        p.mark = prof_mark;
+    }
+    return p;
+}
+
+// Sum the recorded event pairs into stats (after a stream synchronize).
+void prof_collect(efie_ctx* c, efie_stats* st) {
+    for (auto& m : c->marks) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, m.second.first, m.second.second) == hipSuccess && st) {
+            st->ms_kernel[m.first] += ms;
+            st->n_kernel[m.first] += 1;
+        }
+    }
+    c->marks.clear();
+    c->poolUsed = 0;
+}
+
 // Triangle geometry of precompute_tri_geom.m:11-17 for triangle t (0-based).
 void tri_vertices(const efie_ctx* c, int t, V3& A, V3& B, V3& C) {
     const int Nv = c->Nv, Nt = c->Nt;
@@ -487,16 +548,36 @@ void tri_vertices(const efie_ctx* c, int t, V3& A, V3& B, V3& C) {
 }
 
 efie_status check_opts(const efie_opts* o) {
+    if (o->precision != EFIE_PREC_FP64 && o->precision != EFIE_PREC_MIXED)
+        return fail(EFIE_E_ARGS, "efie:opts", "unknown precision");
     if (o->model == EFIE_MODEL_CENTROID) return ok();
     if (o->model != EFIE_MODEL_QUADRATURE) return fail(EFIE_E_ARGS, "efie:opts", "unknown model");
     if (o->near_subdiv < 0 || o->near_subdiv > 3 || o->self_subdiv < 0 || o->self_subdiv > 3)
         return fail(EFIE_E_ARGS, "efie:opts", "subdivision depth must be 0..3");
-    if (o->precision != EFIE_PREC_FP64 && o->precision != EFIE_PREC_MIXED)
-        return fail(EFIE_E_ARGS, "efie:opts", "unknown precision");
+    if (o->near_mode != EFIE_NEAR_DIRECT && o->near_mode != EFIE_NEAR_MOMENTS)
+        return fail(EFIE_E_ARGS, "efie:opts", "unknown near_mode");
     return ok();
 }
 
-efie_status ensure_edge_mid(efie_ctx* c);
+// Edge midpoints (port_weights_gaussian.m:7-11) and lengths for the RHS
+// and the Zin epilogue, needed by both models.
+efie_status ensure_edge_mid(efie_ctx* c) {
+    if (c->midValid) return ok();
+    const int Ne = c->Ne;
+    std::vector<double> midp(3 * (size_t)Ne);
+    for (int e = 0; e < Ne; ++e) {
+        const int a = c->edge[e] - 1, b = c->edge[e + Ne] - 1;
+        for (int d = 0; d < 3; ++d)
+            midp[3 * e + d] = 0.5 * (c->V[a + d * (size_t)c->Nv] + c->V[b + d * (size_t)c->Nv]);
+    }
+    HIPCHK(c->dMid.ensure(sizeof(double) * midp.size()));
+    HIPCHK(c->dLen.ensure(sizeof(double) * Ne));
+    HIPCHK(hipMemcpyAsync(c->dMid.p, midp.data(), sizeof(double) * midp.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->dLen.p, c->len.data(), sizeof(double) * Ne, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->midValid = true;
+    return ok();
+}
 
 // Frequency-independent data of the quadrature model, cached per options.
 efie_status ensure_geometry(efie_ctx* c, const efie_opts* o) {
@@ -513,6 +594,7 @@ efie_status ensure_geometry(efie_ctx* c, const efie_opts* o) {
 
     std::vector<TriGeo> tg(Nt);
     std::vector<V3> tv(3 * (size_t)Nt);
+    double lmax = 0, aeqMax = 0;
     for (int t = 0; t < Nt; ++t) {
         V3 A, B, C;
         tri_vertices(c, t, A, B, C);
@@ -521,6 +603,8 @@ efie_status ensure_geometry(efie_ctx* c, const efie_opts* o) {
         tg[t].cy = (A.y + B.y + C.y) / 3.0;
         tg[t].cz = (A.z + B.z + C.z) / 3.0;
         tg[t].area2 = norm(cross(B - A, C - A));
+        lmax = std::max(lmax, std::sqrt(4 * (0.5 * tg[t].area2) / std::sqrt(3.0)));
+        aeqMax = std::max(aeqMax, std::sqrt((0.5 * tg[t].area2) / kPi));
     }
     double qu[16], qv[16], qw[16];
     // regular rule: points relative to the centroid, weight qw * area2
@@ -534,7 +618,8 @@ efie_status ensure_geometry(efie_ctx* c, const efie_opts* o) {
     // subdivided rules for near and self pairs
     double su[16], sv[16], sw[16];
     const int Nq2 = quad_rule(o->near_order, su, sv, sw);
-    auto subdiv_pts = [&](int depth, std::vector<Pt>& pts, std::vector<double>* aSelf) {
+    double rmaxN = 0, rmaxS = 0, aSelfMax = 0;
+    auto subdiv_pts = [&](int depth, std::vector<Pt>& pts, std::vector<double>* aSelf, double& rmax) {
         const int Ns = 1 << (2 * depth);
         pts.resize((size_t)Nt * Ns * Nq2);
         if (aSelf) aSelf->resize((size_t)Nt * Ns);
@@ -544,11 +629,16 @@ efie_status ensure_geometry(efie_ctx* c, const efie_opts* o) {
             for (int p = 0; p < Ns; ++p) {
                 V3 A = sub[3 * p], B = sub[3 * p + 1], C = sub[3 * p + 2];
                 const double a2p = norm(cross(B - A, C - A));
-                if (aSelf) (*aSelf)[(size_t)t * Ns + p] = o->self_alpha * std::sqrt((0.5 * a2p) / kPi);
+                if (aSelf) {
+                    const double as = o->self_alpha * std::sqrt((0.5 * a2p) / kPi);
+                    (*aSelf)[(size_t)t * Ns + p] = as;
+                    aSelfMax = std::max(aSelfMax, as);
+                }
                 for (int i = 0; i < Nq2; ++i) {
                     V3 r = map_ref(A, B, C, su[i], sv[i]);
-                    pts[((size_t)t * Ns + p) * Nq2 + i] = {r.x - tg[t].cx, r.y - tg[t].cy, r.z - tg[t].cz,
-                                                           sw[i] * a2p};
+                    Pt q = {r.x - tg[t].cx, r.y - tg[t].cy, r.z - tg[t].cz, sw[i] * a2p};
+                    pts[((size_t)t * Ns + p) * Nq2 + i] = q;
+                    rmax = std::max(rmax, std::sqrt(q.x * q.x + q.y * q.y + q.z * q.z));
                 }
             }
         }
@@ -556,8 +646,8 @@ efie_status ensure_geometry(efie_ctx* c, const efie_opts* o) {
     };
     std::vector<Pt> nearPts, selfPts;
     std::vector<double> aSelf;
-    const int NsN = subdiv_pts(o->near_subdiv, nearPts, nullptr);
-    const int NsS = subdiv_pts(o->self_subdiv, selfPts, &aSelf);
+    const int NsN = subdiv_pts(o->near_subdiv, nearPts, nullptr, rmaxN);
+    const int NsS = subdiv_pts(o->self_subdiv, selfPts, &aSelf, rmaxS);
     std::vector<double> aeq(Nt);
     for (int t = 0; t < Nt; ++t) aeq[t] = std::sqrt((0.5 * tg[t].area2) / kPi);
 
@@ -582,6 +672,7 @@ efie_status ensure_geometry(efie_ctx* c, const efie_opts* o) {
             side[2 * e + sd] = {vv[e] - g.cx, vv[e + Ne] - g.cy, vv[e + 2 * Ne] - g.cz, cc};
         }
     }
+
     HIPCHK(c->dTri.ensure(sizeof(TriGeo) * Nt));
     HIPCHK(c->dReg.ensure(sizeof(Pt) * reg.size()));
     HIPCHK(c->dNearPts.ensure(sizeof(Pt) * nearPts.size()));
@@ -619,13 +710,57 @@ efie_status ensure_geometry(efie_ctx* c, const efie_opts* o) {
     HIPCHK(hipStreamSynchronize(s));
 
     c->Nq = Nq;
-    c->Nq2n = Nq2;
     c->NpNear = NsN * Nq2;
     c->Nq2s = Nq2;
     c->NsSelf = NsS;
     c->nNear = tot;
+    c->rNearMax = rmaxN;
+    c->rSelfMax = rmaxS;
+    c->lMax = lmax;
+    c->aNearMax = aeqMax;
+    c->aSelfMax = aSelfMax;
     c->key = k;
     c->geoValid = true;
+    c->momP = -1;  // moments belong to the old geometry
+    return ok();
+}
+
+// Series length for the moment expansion: the smallest P with
+// (x^(P+1)/(P+1)!) e^x <= 1e-17, x = |k|max * Rmax, where Rmax bounds the
+// (softened) distance of every near and self point pair.  -1 if P > 60.
+int moment_terms(const efie_ctx* c, const efie_opts* o, double kmax) {
+    const double dN = o->near_sing_eps * c->lMax;  // centroid distance of a near pair
+    const double aN = o->near_alpha * c->aNearMax;
+    const double rN = std::sqrt((dN + 2 * c->rNearMax) * (dN + 2 * c->rNearMax) + aN * aN);
+    const double rS = std::sqrt(4 * c->rSelfMax * c->rSelfMax + c->aSelfMax * c->aSelfMax);
+    const double x = kmax * std::max(rN, rS) * 1.0001;
+    double term = std::exp(x);  // e^x x^n / n!
+    for (int n = 1; n <= 61; ++n) {
+        term *= x / n;
+        if (term <= 1e-17) return n - 1;
+    }
+    return -1;
+}
+
+// Moments of the near and self pairs for series length P+1 (cached; a
+// longer series than cached recomputes them).
+efie_status ensure_moments(efie_ctx* c, const efie_opts* o, int P, efie_stats* st) {
+    if (c->momP >= P && c->nearAlphaMom == o->near_alpha) return ok();
+    hipStream_t s = c->stream;
+    const size_t per = (size_t)(P + 1) * 8;
+    HIPCHK(c->dQn.ensure(sizeof(double) * per * (size_t)std::max(1LL, c->nNear)));
+    HIPCHK(c->dQs.ensure(sizeof(double) * per * (size_t)c->Nt));
+    HIPCHK(hipEventRecord(c->ev[6], s));
+    HIPCHK(launch_moments(c->dNearPts.as<Pt>(), c->NpNear, c->dTri.as<TriGeo>(), c->dAeq.as<double>(), o->near_alpha,
+                          c->dList.as<int2>(), (int)c->nNear, c->dSelfPts.as<Pt>(), c->Nq2s, c->NsSelf,
+                          c->dASelf.as<double>(), c->Nt, P, c->dQn.as<double>(), c->dQs.as<double>(), s));
+    HIPCHK(hipEventRecord(c->ev[7], s));
+    HIPCHK(hipStreamSynchronize(s));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev[6], c->ev[7]));
+    if (st) st->ms_moments += ms;
+    c->momP = P;
+    c->nearAlphaMom = o->near_alpha;
     return ok();
 }
 
@@ -698,24 +833,43 @@ Freq make_freq(std::complex<double> k, double omega, double mu, std::complex<dou
 }
 
 // Fill one Z (interleaved complex, element (m,n) at Z[2*(m*ldr + n*ldc)]).
-efie_status fill_z(efie_ctx* c, const efie_opts* o, const Freq* dfr, double* Z, size_t ldr,
+// useMoments: near/self records from the cached moments (momP terms).
+efie_status fill_z(efie_ctx* c, const efie_opts* o, bool useMoments, const Freq* dfr, double* Z, size_t ldr,
                    size_t ldc) {
     hipStream_t s = c->stream;
+    Prof pf = make_prof(c);
     if (o->model == EFIE_MODEL_CENTROID) {
+        pf.begin(EFIE_K_OTHER);
         HIPCHK(launch_centroid(c->dCPos.as<double>(), c->dCF.as<double>(), c->dCDv.as<double>(),
                                c->dCW.as<double>(), c->dCT.as<int>(), c->dCAsoft.as<double>(), c->Ne, dfr,
                                Z, ldr, ldc, s));
+        pf.end(EFIE_K_OTHER);
         return ok();
     }
     const int Nt = c->Nt;
     HIPCHK(c->dRecs.ensure(sizeof(Rec) * rec_index(0, Nt)));
     Rec* recs = c->dRecs.as<Rec>();
-    HIPCHK(launch_rec_self(c->dSelfPts.as<Pt>(), c->Nq2s, c->NsSelf, c->dASelf.as<double>(), Nt, dfr, recs, s));
-    HIPCHK(launch_rec_near(c->dNearPts.as<Pt>(), c->NpNear, c->dTri.as<TriGeo>(), c->dAeq.as<double>(),
-                           o->near_alpha, c->dList.as<int2>(), (int)c->nNear, dfr, recs, s));
+    if (useMoments) {
+        pf.begin(EFIE_K_POLY);
+        HIPCHK(launch_poly_records(c->dQn.as<double>(), c->dList.as<int2>(), (int)c->nNear, c->dQs.as<double>(), Nt,
+                                   c->momP, dfr, recs, s));
+        pf.end(EFIE_K_POLY);
+    } else {
+        pf.begin(EFIE_K_SELF);
+        HIPCHK(launch_rec_self(c->dSelfPts.as<Pt>(), c->Nq2s, c->NsSelf, c->dASelf.as<double>(), Nt, dfr, recs, s));
+        pf.end(EFIE_K_SELF);
+        pf.begin(EFIE_K_NEAR);
+        HIPCHK(launch_rec_near(c->dNearPts.as<Pt>(), c->NpNear, c->dTri.as<TriGeo>(), c->dAeq.as<double>(),
+                               o->near_alpha, c->dList.as<int2>(), (int)c->nNear, dfr, recs, s));
+        pf.end(EFIE_K_NEAR);
+    }
+    pf.begin(EFIE_K_REGULAR);
     HIPCHK(launch_rec_regular(c->dReg.as<Pt>(), c->Nq, c->dTri.as<TriGeo>(), Nt, c->dNearMat.as<uint8_t>(), dfr,
                               recs, s));
+    pf.end(EFIE_K_REGULAR);
+    pf.begin(EFIE_K_COMBINE);
     HIPCHK(launch_combine(recs, c->dSideTri.as<int>(), c->dSide.as<Side>(), c->Ne, dfr, Z, ldr, ldc, s));
+    pf.end(EFIE_K_COMBINE);
     return ok();
 }
 
@@ -741,26 +895,6 @@ efie_status make_rhs(efie_ctx* c, int pe, int ge, double V0, double spread) {
     return ok();
 }
 
-// Edge midpoints (port_weights_gaussian.m:7-11) and lengths for the RHS
-// and the Zin epilogue, needed by both models.
-efie_status ensure_edge_mid(efie_ctx* c) {
-    if (c->midValid) return ok();
-    const int Ne = c->Ne;
-    std::vector<double> midp(3 * (size_t)Ne);
-    for (int e = 0; e < Ne; ++e) {
-        const int a = c->edge[e] - 1, b = c->edge[e + Ne] - 1;
-        for (int d = 0; d < 3; ++d)
-            midp[3 * e + d] = 0.5 * (c->V[a + d * (size_t)c->Nv] + c->V[b + d * (size_t)c->Nv]);
-    }
-    HIPCHK(c->dMid.ensure(sizeof(double) * midp.size()));
-    HIPCHK(c->dLen.ensure(sizeof(double) * Ne));
-    HIPCHK(hipMemcpyAsync(c->dMid.p, midp.data(), sizeof(double) * midp.size(), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->dLen.p, c->len.data(), sizeof(double) * Ne, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    c->midValid = true;
-    return ok();
-}
-
 efie_status prepare(efie_ctx* c, const efie_opts* o, double selfAlphaCentroid) {
     if (!c->haveMesh || !c->haveRwg)
         return fail(EFIE_E_STATE, "efie:state", "mesh and RWG tables must be set first");
@@ -771,6 +905,157 @@ efie_status prepare(efie_ctx* c, const efie_opts* o, double selfAlphaCentroid) {
     if (st) return st;
     if (o->model == EFIE_MODEL_CENTROID) return ensure_centroid(c, selfAlphaCentroid);
     return ensure_geometry(c, o);
+}
+
+// Decide the near/self integration for a set of wavenumbers: the moment
+// series when asked for and short enough, otherwise direct quadrature.
+efie_status choose_near(efie_ctx* c, const efie_opts* o, double kmax, efie_stats* st, bool* useMoments) {
+    *useMoments = false;
+    if (o->model != EFIE_MODEL_QUADRATURE || o->near_mode != EFIE_NEAR_MOMENTS) return ok();
+    const int P = moment_terms(c, o, kmax);
+    if (P < 0) return ok();
+    efie_status e = ensure_moments(c, o, P, st);
+    if (e) return e;
+    *useMoments = true;
+    if (st) st->moment_terms = c->momP + 1;
+    return ok();
+}
+
+
+int batch_size(const efie_opts& o, size_t zElems) {
+    int B = o.batch > 0 ? o.batch : 8;
+    size_t freeB = 0, totB = 0;
+    if (hipMemGetInfo(&freeB, &totB) == hipSuccess) {
+        const size_t per = zElems * (o.precision == EFIE_PREC_MIXED ? 24 : 16);
+        const size_t cap = (size_t)(0.5 * (double)freeB) / std::max<size_t>(per, 1);
+        if (cap < (size_t)B) B = (int)std::max<size_t>(1, cap);
+    }
+    return B;
+}
+
+// Device buffers of the batched solver for n unknowns and B matrices.
+efie_status ensure_solver(efie_ctx* c, const efie_opts& o, int n, int lda, int B) {
+    const size_t zElems = (size_t)n * lda;
+    hipStream_t s = c->stream;
+    HIPCHK(c->dZ.ensure(sizeof(double) * 2 * zElems * B));
+    if (o.precision == EFIE_PREC_MIXED) {
+        HIPCHK(c->dZ32.ensure(sizeof(float) * 2 * zElems * B));
+        HIPCHK(c->dPerm.ensure(sizeof(int) * (size_t)B * n));
+        HIPCHK(c->dX.ensure(sizeof(double) * 2 * (size_t)B * n));
+        HIPCHK(c->dR.ensure(sizeof(double) * 2 * (size_t)B * n));
+        HIPCHK(c->dR32.ensure(sizeof(float) * 2 * (size_t)B * n));
+        HIPCHK(c->dRn2.ensure(sizeof(double) * B));
+        HIPCHK(c->dBn2.ensure(sizeof(double) * B));
+        HIPCHK(c->dDone.ensure(sizeof(int) * B));
+        HIPCHK(c->dIters.ensure(sizeof(int) * B));
+        HIPCHK(c->dResid.ensure(sizeof(double) * B));
+    }
+    const int nodes = lu_tree_nodes(n);
+    HIPCHK(c->dCand.ensure(sizeof(int) * (size_t)B * nodes * lu_cand_stride()));
+    const size_t cntBytes = sizeof(int) * (size_t)B * nodes;
+    if (c->dCnt.bytes < cntBytes || !c->dCnt.p) {
+        HIPCHK(c->dCnt.ensure(cntBytes));
+        HIPCHK(launch_fill_int(c->dCnt.as<int>(), 0, (int)(c->dCnt.bytes / sizeof(int)), s));
+    }
+    HIPCHK(c->dIpiv.ensure(sizeof(int) * (size_t)B * n));
+    HIPCHK(c->dInfo.ensure(sizeof(int) * B));
+    return ok();
+}
+
+// Factor and solve the nb matrices in c->dZ (row-major, rhs in column n).
+// FP64: LU in place, back substitution on column n.  MIXED: LU of an FP32
+// copy, then FP64 iterative refinement against the FP64 matrix with
+// right-hand sides V + b*vStride.  Returns where the FP64 solutions are.
+efie_status solve_batch(efie_ctx* c, const efie_opts& o, int n, int lda, int nb, const double* V, size_t vStride,
+                        const Prof& pf, const double** X, size_t* xStride, size_t* xinc) {
+    hipStream_t s = c->stream;
+    const size_t zElems = (size_t)n * lda;
+    double* Zb = c->dZ.as<double>();
+    LuWork wk{c->dCand.as<int>(), c->dCnt.as<int>(), c->dIpiv.as<int>(), c->dInfo.as<int>()};
+    HIPCHK(launch_fill_int(c->dInfo.as<int>(), 0, nb, s));
+    if (o.precision != EFIE_PREC_MIXED) {
+        HIPCHK(lu_factor<double>(Zb, 2 * zElems, n, lda, nb, wk, s, &pf));
+        HIPCHK(hipEventRecord(c->ev[3], s));
+        pf.begin(EFIE_K_TRSV);
+        HIPCHK(lu_backsolve<double>(Zb, 2 * zElems, n, lda, nb, s));
+        pf.end(EFIE_K_TRSV);
+        *X = Zb + 2 * n;
+        *xStride = 2 * zElems;
+        *xinc = 2 * (size_t)lda;
+        return ok();
+    }
+    float* Z32 = c->dZ32.as<float>();
+    pf.begin(EFIE_K_OTHER);
+    HIPCHK(launch_to_f32(Zb, Z32, zElems * nb, s));
+    pf.end(EFIE_K_OTHER);
+    HIPCHK(lu_factor<float>(Z32, 2 * zElems, n, lda, nb, wk, s, &pf));
+    HIPCHK(hipEventRecord(c->ev[3], s));
+    pf.begin(EFIE_K_TRSV);
+    HIPCHK(lu_backsolve<float>(Z32, 2 * zElems, n, lda, nb, s));
+    pf.end(EFIE_K_TRSV);
+    pf.begin(EFIE_K_OTHER);
+    HIPCHK(lu_make_perm(c->dIpiv.as<int>(), n, nb, c->dPerm.as<int>(), s));
+    HIPCHK(launch_col_to_vec(Z32, 2 * zElems, n, lda, c->dX.as<double>(), nb, s));
+    HIPCHK(launch_fill_int(c->dDone.as<int>(), 0, nb, s));
+    HIPCHK(launch_fill_int(c->dIters.as<int>(), 0, nb, s));
+    HIPCHK(launch_fill_dbl(c->dRn2.as<double>(), 0.0, nb, s));
+    pf.end(EFIE_K_OTHER);
+    for (int it = 0; it <= o.ir_max_iter; ++it) {
+        pf.begin(EFIE_K_OTHER);
+        HIPCHK(launch_residual(Zb, 2 * zElems, n, lda, V, vStride, c->dX.as<double>(), c->dR.as<double>(),
+                               c->dRn2.as<double>(), c->dDone.as<int>(), nb, s));
+        HIPCHK(launch_ir_check(c->dRn2.as<double>(), c->dBn2.as<double>(), o.ir_tol, nb, c->dDone.as<int>(),
+                               c->dIters.as<int>(), c->dResid.as<double>(), it == 0, it == o.ir_max_iter, s));
+        pf.end(EFIE_K_OTHER);
+        if (it == o.ir_max_iter) break;
+        pf.begin(EFIE_K_OTHER);
+        HIPCHK(launch_perm_f32(c->dR.as<double>(), c->dPerm.as<int>(), n, nb, c->dR32.as<float>(),
+                               c->dDone.as<int>(), s));
+        pf.end(EFIE_K_OTHER);
+        pf.begin(EFIE_K_TRSV);
+        HIPCHK(lu_trsv<float>(Z32, 2 * zElems, n, lda, c->dR32.as<float>(), 2 * (size_t)n, 1, nb, true,
+                              c->dDone.as<int>(), s));
+        HIPCHK(lu_trsv<float>(Z32, 2 * zElems, n, lda, c->dR32.as<float>(), 2 * (size_t)n, 1, nb, false,
+                              c->dDone.as<int>(), s));
+        pf.end(EFIE_K_TRSV);
+        pf.begin(EFIE_K_OTHER);
+        HIPCHK(launch_add_f32(c->dX.as<double>(), c->dR32.as<float>(), n, nb, c->dDone.as<int>(), s));
+        pf.end(EFIE_K_OTHER);
+    }
+    *X = c->dX.as<double>();
+    *xStride = 2 * (size_t)n;
+    *xinc = 2;
+    return ok();
+}
+
+// Synchronise after a batch: zero pivots become efie:singular, refinement
+// counters go into the stats, profile marks are summed.
+efie_status finish_batch(efie_ctx* c, const efie_opts& o, int nb, efie_stats* S, const double* freqs) {
+    hipStream_t s = c->stream;
+    std::vector<int> info(nb), iters(nb);
+    std::vector<double> resid(nb);
+    HIPCHK(hipMemcpyAsync(info.data(), c->dInfo.p, sizeof(int) * nb, hipMemcpyDeviceToHost, s));
+    const bool mixed = o.precision == EFIE_PREC_MIXED;
+    if (mixed) {
+        HIPCHK(hipMemcpyAsync(iters.data(), c->dIters.p, sizeof(int) * nb, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(resid.data(), c->dResid.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    prof_collect(c, S);
+    for (int i = 0; i < nb; ++i) {
+        if (info[i] != 0) {
+            if (freqs)
+                return fail(EFIE_E_SINGULAR, "efie:singular", "Z is singular at f=%g Hz (zero pivot in column %d)",
+                            freqs[i], info[i]);
+            return fail(EFIE_E_SINGULAR, "efie:singular", "matrix %d is singular (zero pivot in column %d)", i,
+                        info[i]);
+        }
+        if (mixed) {
+            S->ir_iters_max = std::max(S->ir_iters_max, iters[i]);
+            S->ir_resid_max = std::max(S->ir_resid_max, resid[i]);
+        }
+    }
+    return ok();
 }
 
 }  // namespace
@@ -793,7 +1078,11 @@ efie_status efie_assemble_zv(efie_ctx* c, double k_re, double k_im, double omega
     if (ge < 1 || ge > c->Ne) return fail(EFIE_E_PORT, "mex:port", "groundEdge out of range.");
     efie_status st = prepare(c, &o, exc[4]);
     if (st) return st;
+    c->prof = false;
     st = make_rhs(c, pe, ge, exc[2], exc[3]);
+    if (st) return st;
+    bool mom = false;
+    st = choose_near(c, &o, std::hypot(k_re, k_im), nullptr, &mom);
     if (st) return st;
     const int Ne = c->Ne;
     hipStream_t s = c->stream;
@@ -811,7 +1100,7 @@ efie_status efie_assemble_zv(efie_ctx* c, double k_re, double k_im, double omega
     HIPCHK(c->dFreq.ensure(sizeof(Freq)));
     HIPCHK(hipMemcpyAsync(c->dFreq.p, &fr, sizeof fr, hipMemcpyHostToDevice, s));
     HIPCHK(c->dZ.ensure(sizeof(double) * 2 * (size_t)Ne * Ne));
-    st = fill_z(c, &o, c->dFreq.as<Freq>(), c->dZ.as<double>(), 1, (size_t)Ne);
+    st = fill_z(c, &o, mom, c->dFreq.as<Freq>(), c->dZ.as<double>(), 1, (size_t)Ne);
     if (st) return st;
     HIPCHK(hipMemcpyAsync(Z, c->dZ.p, sizeof(double) * 2 * (size_t)Ne * Ne, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(Vrhs, c->dV.p, sizeof(double) * 2 * (size_t)Ne, hipMemcpyDeviceToHost, s));
@@ -823,7 +1112,8 @@ efie_status efie_sweep(efie_ctx* c, const double* freqs, int Nf, double mu, doub
                        double eps_im, int patchEdge, int groundEdge, double V0, double spread,
                        double Z0, const efie_opts* opts, double* Zin, double* S11,
                        efie_stats* stats) {
-    if (!c || !freqs || !Zin || !S11) return fail(EFIE_E_ARGS, "efie:args", "null argument");
+    if (!c || (!freqs && Nf > 0) || (!Zin && Nf > 0) || (!S11 && Nf > 0))
+        return fail(EFIE_E_ARGS, "efie:args", "null argument");
     if (Nf < 0) return fail(EFIE_E_ARGS, "mex:freqs", "freqs must be double vector.");
     efie_opts o;
     if (opts) o = *opts;
@@ -834,79 +1124,77 @@ efie_status efie_sweep(efie_ctx* c, const double* freqs, int Nf, double mu, doub
     // the centroid sweep MEX hard-codes self_alpha = 0.20 (patch_sweep_cuda.cu:269)
     efie_status st = prepare(c, &o, 0.20);
     if (st) return st;
+    efie_stats local;
+    efie_stats* S = stats ? stats : &local;
+    std::memset(S, 0, sizeof *S);
+    c->prof = o.profile != 0;
+    c->marks.clear();
+    c->poolUsed = 0;
     st = make_rhs(c, patchEdge, groundEdge, V0, spread);
     if (st) return st;
-    if (stats) std::memset(stats, 0, sizeof *stats);
     if (Nf == 0) return ok();
+
+    std::complex<double> eps(eps_re, eps_im);
+    std::vector<std::complex<double>> ks(Nf);
+    double kmax = 0;
+    for (int f = 0; f < Nf; ++f) {
+        const double omega = 2.0 * kPi * freqs[f];
+        ks[f] = omega * std::sqrt(std::complex<double>(mu, 0.0) * eps);
+        kmax = std::max(kmax, std::abs(ks[f]));
+    }
+    bool mom = false;
+    st = choose_near(c, &o, kmax, S, &mom);
+    if (st) return st;
 
     const int n = c->Ne;
     const int lda = ((n + 1) + 31) & ~31;  // rhs in column n
     const size_t zElems = (size_t)n * lda;
-    int B = o.batch > 0 ? o.batch : 8;
-    {
-        // keep the batch within a fixed share of device memory
-        size_t freeB = 0, totB = 0;
-        if (hipMemGetInfo(&freeB, &totB) == hipSuccess) {
-            size_t per = zElems * 16;
-            size_t cap = (size_t)(0.5 * (double)freeB) / std::max<size_t>(per, 1);
-            if (cap < (size_t)B) B = (int)std::max<size_t>(1, cap);
-        }
-    }
-    if (B > Nf) B = Nf;
+    const int B = std::min(batch_size(o, zElems), Nf);
     hipStream_t s = c->stream;
-    HIPCHK(c->dZ.ensure(sizeof(double) * 2 * zElems * B));
+    st = ensure_solver(c, o, n, lda, B);
+    if (st) return st;
     HIPCHK(c->dFreq.ensure(sizeof(Freq) * B));
     HIPCHK(c->dOut.ensure(sizeof(double) * 4 * (size_t)Nf));
-    const int nodes = lu_tree_nodes(n);
-    HIPCHK(c->dCand.ensure(sizeof(int) * (size_t)B * nodes * lu_cand_stride()));
-    size_t cntBytes = sizeof(int) * (size_t)B * nodes;
-    if (c->dCnt.bytes < cntBytes || !c->dCnt.p) {
-        HIPCHK(c->dCnt.ensure(cntBytes));
-        HIPCHK(hipMemsetAsync(c->dCnt.p, 0, c->dCnt.bytes, s));
+    Prof pf = make_prof(c);
+    if (o.precision == EFIE_PREC_MIXED) {
+        // ||V||^2 of the shared right-hand side, for every batch element
+        std::vector<double> hv(2 * (size_t)n);
+        HIPCHK(hipMemcpyAsync(hv.data(), c->dV.p, sizeof(double) * hv.size(), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        double vn2 = 0;
+        for (double v : hv) vn2 += v * v;
+        HIPCHK(launch_fill_dbl(c->dBn2.as<double>(), vn2, B, s));
     }
-    HIPCHK(c->dIpiv.ensure(sizeof(int) * (size_t)B * n));
-    HIPCHK(c->dInfo.ensure(sizeof(int) * B));
-    LuWork wk{c->dCand.as<int>(), c->dCnt.as<int>(), c->dIpiv.as<int>(), c->dInfo.as<int>()};
 
     std::vector<Freq> fr(B);
-    std::complex<double> eps(eps_re, eps_im);
-    float msFill = 0, msFac = 0, msSolve = 0;
+    double msFill = 0, msFac = 0, msSolve = 0;
     HIPCHK(hipEventRecord(c->ev[0], s));
     for (int f0 = 0; f0 < Nf; f0 += B) {
         const int nb = std::min(B, Nf - f0);
-        for (int i = 0; i < nb; ++i) {
-            const double omega = 2.0 * kPi * freqs[f0 + i];
-            std::complex<double> k = omega * std::sqrt(std::complex<double>(mu, 0.0) * eps);
-            fr[i] = make_freq(k, omega, mu, eps);
-        }
+        for (int i = 0; i < nb; ++i) fr[i] = make_freq(ks[f0 + i], 2.0 * kPi * freqs[f0 + i], mu, eps);
         HIPCHK(hipMemcpyAsync(c->dFreq.p, fr.data(), sizeof(Freq) * nb, hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemsetAsync(c->dInfo.p, 0, sizeof(int) * nb, s));
         HIPCHK(hipEventRecord(c->ev[1], s));
         double* Zb = c->dZ.as<double>();
         for (int i = 0; i < nb; ++i) {
             double* Zi = Zb + 2 * zElems * i;
-            st = fill_z(c, &o, c->dFreq.as<Freq>() + i, Zi, (size_t)lda, 1);
+            st = fill_z(c, &o, mom, c->dFreq.as<Freq>() + i, Zi, (size_t)lda, 1);
             if (st) return st;
             // rhs into column n
             HIPCHK(hipMemcpy2DAsync(Zi + 2 * n, sizeof(double) * 2 * lda, c->dV.p, sizeof(double) * 2,
                                     sizeof(double) * 2, n, hipMemcpyDeviceToDevice, s));
         }
         HIPCHK(hipEventRecord(c->ev[2], s));
-        HIPCHK(lu_factor<double>(Zb, 2 * zElems, n, lda, nb, wk, s));
-        HIPCHK(hipEventRecord(c->ev[3], s));
-        HIPCHK(lu_backsolve<double>(Zb, 2 * zElems, n, lda, nb, s));
-        // x is column n of each matrix (element stride 2*lda doubles)
-        HIPCHK(launch_epilogue(Zb + 2 * n, 2 * zElems, 2 * (size_t)lda, c->dWl.as<double>(), n,
-                               patchEdge - 1, groundEdge - 1, o.model == EFIE_MODEL_CENTROID, V0, Z0, nb,
-                               c->dOut.as<double>() + 4 * (size_t)f0, s));
+        const double* X = nullptr;
+        size_t xStride = 0, xinc = 0;
+        st = solve_batch(c, o, n, lda, nb, c->dV.as<double>(), 0, pf, &X, &xStride, &xinc);
+        if (st) return st;
+        pf.begin(EFIE_K_OTHER);
+        HIPCHK(launch_epilogue(X, xStride, xinc, c->dWl.as<double>(), n, patchEdge - 1, groundEdge - 1,
+                               o.model == EFIE_MODEL_CENTROID, V0, Z0, nb, c->dOut.as<double>() + 4 * (size_t)f0, s));
+        pf.end(EFIE_K_OTHER);
         HIPCHK(hipEventRecord(c->ev[4], s));
-        std::vector<int> info(nb);
-        HIPCHK(hipMemcpyAsync(info.data(), c->dInfo.p, sizeof(int) * nb, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        for (int i = 0; i < nb; ++i)
-            if (info[i] != 0)
-                return fail(EFIE_E_SINGULAR, "efie:singular",
-                            "Z is singular at f=%g Hz (zero pivot in column %d)", freqs[f0 + i], info[i]);
+        st = finish_batch(c, o, nb, S, freqs + f0);
+        if (st) return st;
         float t1, t2, t3;
         HIPCHK(hipEventElapsedTime(&t1, c->ev[1], c->ev[2]));
         HIPCHK(hipEventElapsedTime(&t2, c->ev[2], c->ev[3]));
@@ -925,20 +1213,82 @@ efie_status efie_sweep(efie_ctx* c, const double* freqs, int Nf, double mu, doub
         S11[2 * f] = out[4 * f + 2];
         S11[2 * f + 1] = out[4 * f + 3];
     }
-    if (stats) {
-        float tot = 0;
-        HIPCHK(hipEventElapsedTime(&tot, c->ev[0], c->ev[5]));
-        stats->ms_total = tot;
-        stats->ms_fill = msFill;
-        stats->ms_factor = msFac;
-        stats->ms_solve = msSolve;
-        stats->n_near_pairs = c->nNear;
-        if (o.model == EFIE_MODEL_QUADRATURE) {
-            const long long Nt = c->Nt, np = (long long)c->NpNear, ns = (long long)c->NsSelf * c->Nq2s;
-            const long long reg = Nt * (Nt - 1) / 2 - c->nNear;
-            stats->n_green = reg * c->Nq * c->Nq + c->nNear * np * np + Nt * ns * ns;
-        }
+    float tot = 0;
+    HIPCHK(hipEventElapsedTime(&tot, c->ev[0], c->ev[5]));
+    S->ms_total = tot;
+    S->ms_fill = msFill;
+    S->ms_factor = msFac;
+    S->ms_solve = msSolve;
+    S->n_near_pairs = c->nNear;
+    if (o.model == EFIE_MODEL_QUADRATURE) {
+        const long long Nt = c->Nt, np = (long long)c->NpNear, ns = (long long)c->NsSelf * c->Nq2s;
+        const long long reg = Nt * (Nt - 1) / 2 - c->nNear;
+        S->n_green = reg * c->Nq * c->Nq + (mom ? 0 : c->nNear * np * np + Nt * ns * ns);
     }
+    c->prof = false;
+    return ok();
+}
+
+efie_status efie_zgesv(efie_ctx* c, int n, int batch, const double* A, const double* b, int precision,
+                       double* x, efie_stats* stats) {
+    if (!c || !A || !b || !x) return fail(EFIE_E_ARGS, "efie:args", "null argument");
+    if (n < 1 || batch < 1) return fail(EFIE_E_ARGS, "efie:args", "n and batch must be >= 1");
+    if (precision != EFIE_PREC_FP64 && precision != EFIE_PREC_MIXED)
+        return fail(EFIE_E_ARGS, "efie:opts", "unknown precision");
+    HIPCHK(hipSetDevice(c->device));
+    efie_opts o;
+    efie_opts_default(&o);
+    o.precision = precision;
+    efie_stats local;
+    efie_stats* S = stats ? stats : &local;
+    std::memset(S, 0, sizeof *S);
+    const int lda = ((n + 1) + 31) & ~31;
+    const size_t zElems = (size_t)n * lda;
+    hipStream_t s = c->stream;
+    efie_status st = ensure_solver(c, o, n, lda, batch);
+    if (st) return st;
+    // row-major copies with the right-hand side in column n
+    std::vector<double> h(2 * zElems, 0.0);
+    std::vector<double> bn2(batch, 0.0);
+    HIPCHK(c->dB.ensure(sizeof(double) * 2 * (size_t)n * batch));
+    HIPCHK(hipMemcpyAsync(c->dB.p, b, sizeof(double) * 2 * (size_t)n * batch, hipMemcpyHostToDevice, s));
+    for (int q = 0; q < batch; ++q) {
+        const double* Aq = A + 2 * (size_t)n * n * q;
+        const double* bq = b + 2 * (size_t)n * q;
+        for (int i = 0; i < n; ++i) {
+            for (int j = 0; j < n; ++j) {
+                h[2 * ((size_t)i * lda + j)] = Aq[2 * ((size_t)i + (size_t)j * n)];
+                h[2 * ((size_t)i * lda + j) + 1] = Aq[2 * ((size_t)i + (size_t)j * n) + 1];
+            }
+            h[2 * ((size_t)i * lda + n)] = bq[2 * i];
+            h[2 * ((size_t)i * lda + n) + 1] = bq[2 * i + 1];
+            bn2[q] += bq[2 * i] * bq[2 * i] + bq[2 * i + 1] * bq[2 * i + 1];
+        }
+        HIPCHK(hipMemcpyAsync(c->dZ.as<double>() + 2 * zElems * q, h.data(), sizeof(double) * h.size(),
+                              hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    if (precision == EFIE_PREC_MIXED)
+        HIPCHK(hipMemcpyAsync(c->dBn2.p, bn2.data(), sizeof(double) * batch, hipMemcpyHostToDevice, s));
+    Prof pf = make_prof(c);
+    HIPCHK(hipEventRecord(c->ev[2], s));
+    const double* X = nullptr;
+    size_t xStride = 0, xinc = 0;
+    st = solve_batch(c, o, n, lda, batch, c->dB.as<double>(), 2 * (size_t)n, pf, &X, &xStride, &xinc);
+    if (st) return st;
+    HIPCHK(hipEventRecord(c->ev[4], s));
+    st = finish_batch(c, o, batch, S, nullptr);
+    if (st) return st;
+    for (int q = 0; q < batch; ++q)
+        HIPCHK(hipMemcpy2DAsync(x + 2 * (size_t)n * q, sizeof(double) * 2, X + xStride * q, sizeof(double) * xinc,
+                                sizeof(double) * 2, n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    float t2 = 0, t3 = 0;
+    HIPCHK(hipEventElapsedTime(&t2, c->ev[2], c->ev[3]));
+    HIPCHK(hipEventElapsedTime(&t3, c->ev[3], c->ev[4]));
+    S->ms_factor = t2;
+    S->ms_solve = t3;
+    S->ms_total = t2 + t3;
     return ok();
 }
 

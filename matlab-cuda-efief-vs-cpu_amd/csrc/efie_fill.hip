@@ -313,6 +313,167 @@ __global__ __launch_bounds__(256) void k_rec_self(const Pt* __restrict__ selfPts
                       recs + rec_index(t, t));
 }
 
+// ------------------------------------ frequency-independent moments ---
+// With G(R) = (1/4pi) sum_n (-jk)^n R^(n-1) / n!, every record of a near or
+// self pair is S_X(k) = (1/4pi) sum_{n=0..P} ((-jk)^n/n!) Q_X[n] where
+//   Q_X[q] = sum_ij W_ij X_ij R_ij^(q-1),   X in {1, rho, rho', rho.rho'},
+// R being the (softened) distance of the reference scheme.  The Q depend
+// only on the mesh, so the 9216 Green evaluations of a near pair are done
+// once per mesh, and each frequency costs (P+1) complex multiply-adds per
+// moment.  P is chosen on the host so that the series tail is below 1e-17.
+// Layout: Q[pair][q][8] with X order 1, rho x/y/z, rho' x/y/z, rho.rho'.
+
+constexpr int kMomChunk = 4;
+
+template <bool SELF>
+__device__ void moment_pair(const Pt* __restrict__ P1, const Pt* __restrict__ P2, int Np, int Nq2,
+                            double ox, double oy, double oz, double aNear2,
+                            const double* __restrict__ aSelf, int P, double* __restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    for (int q0 = 0; q0 <= P; q0 += kMomChunk) {
+        double acc[kMomChunk][8];
+#pragma unroll
+        for (int c = 0; c < kMomChunk; ++c)
+#pragma unroll
+            for (int x = 0; x < 8; ++x) acc[c][x] = 0.0;
+        for (int base = 0; base < Np; base += 64) {
+            const int r = min(64, Np - base);
+            const int parts = 64 / r;
+            const int part = lane / r;
+            const int i = base + lane % r;
+            if (part >= parts) continue;
+            const int j0 = (Np * part) / parts, j1 = (Np * (part + 1)) / parts;
+            const Pt pi = P1[i];
+            const double xi = pi.x + ox, yi = pi.y + oy, zi = pi.z + oz;
+            const int psub = i / Nq2;
+            double aS2 = 0.0;
+            if (SELF) {
+                double as = aSelf[psub];
+                aS2 = as * as;
+            }
+            double g[kMomChunk], hx[kMomChunk], hy[kMomChunk], hz[kMomChunk];
+#pragma unroll
+            for (int c = 0; c < kMomChunk; ++c) g[c] = hx[c] = hy[c] = hz[c] = 0.0;
+            for (int j = j0; j < j1; ++j) {
+                const Pt q2 = P2[j];
+                double dx = xi - q2.x, dy = yi - q2.y, dz = zi - q2.z;
+                double R2 = dx * dx + dy * dy + dz * dz;
+                double R;
+                if (!SELF) {
+                    R = sqrt(R2 + aNear2);
+                } else if (j / Nq2 == psub) {
+                    R = sqrt(R2 + aS2);
+                } else {
+                    R = sqrt(R2);
+                    R = R > 1e-15 ? R : 1e-15;
+                }
+                // R^(q0-1)
+                double rp = 1.0 / R;
+                for (int e = 0; e < q0; ++e) rp *= R;
+#pragma unroll
+                for (int c = 0; c < kMomChunk; ++c) {
+                    const double t = q2.w * rp;
+                    g[c] += t;
+                    hx[c] += t * q2.x;
+                    hy[c] += t * q2.y;
+                    hz[c] += t * q2.z;
+                    rp *= R;
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < kMomChunk; ++c) {
+                const double w = pi.w;
+                acc[c][0] += w * g[c];
+                acc[c][1] += w * g[c] * pi.x;
+                acc[c][2] += w * g[c] * pi.y;
+                acc[c][3] += w * g[c] * pi.z;
+                acc[c][4] += w * hx[c];
+                acc[c][5] += w * hy[c];
+                acc[c][6] += w * hz[c];
+                acc[c][7] += w * (pi.x * hx[c] + pi.y * hy[c] + pi.z * hz[c]);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < kMomChunk; ++c)
+#pragma unroll
+            for (int x = 0; x < 8; ++x) {
+                double v = wave_sum(acc[c][x]);
+                if (lane == 0 && q0 + c <= P) out[(size_t)(q0 + c) * 8 + x] = v;
+            }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_moment_near(const Pt* __restrict__ subPts, int Np,
+                                                     const TriGeo* __restrict__ tg,
+                                                     const double* __restrict__ aeq,
+                                                     double near_alpha,
+                                                     const int2* __restrict__ list, int nPairs,
+                                                     int P, double* __restrict__ Q) {
+    const int pair = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (pair >= nPairs) return;
+    const int2 tp = list[pair];
+    const TriGeo g1 = tg[tp.x], g2 = tg[tp.y];
+    double a1 = aeq[tp.x], a2 = aeq[tp.y];
+    double as = near_alpha * (a1 < a2 ? a1 : a2);
+    moment_pair<false>(subPts + (size_t)tp.x * Np, subPts + (size_t)tp.y * Np, Np, Np,
+                       g1.cx - g2.cx, g1.cy - g2.cy, g1.cz - g2.cz, as * as, nullptr, P,
+                       Q + (size_t)pair * (P + 1) * 8);
+}
+
+__global__ __launch_bounds__(256) void k_moment_self(const Pt* __restrict__ selfPts, int Nq2,
+                                                     int Ns, const double* __restrict__ aSelf,
+                                                     int Nt, int P, double* __restrict__ Q) {
+    const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= Nt) return;
+    const int Np = Ns * Nq2;
+    const Pt* Pp = selfPts + (size_t)t * Np;
+    moment_pair<true>(Pp, Pp, Np, Nq2, 0.0, 0.0, 0.0, 0.0, aSelf + (size_t)t * Ns, P,
+                      Q + (size_t)t * (P + 1) * 8);
+}
+
+// Records of near pairs (threads 0..nNear-1) and self pairs (the next Nt)
+// at one frequency from the moments.
+__global__ __launch_bounds__(256) void k_poly_records(const double* __restrict__ Qn,
+                                                      const int2* __restrict__ list, int nNear,
+                                                      const double* __restrict__ Qs, int Nt,
+                                                      int P, const Freq* __restrict__ fr,
+                                                      Rec* __restrict__ recs) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= nNear + Nt) return;
+    const double* q;
+    size_t ri;
+    if (id < nNear) {
+        const int2 tp = list[id];
+        q = Qn + (size_t)id * (P + 1) * 8;
+        ri = rec_index(tp.x, tp.y);
+    } else {
+        const int t = id - nNear;
+        q = Qs + (size_t)t * (P + 1) * 8;
+        ri = rec_index(t, t);
+    }
+    // c_n = (-jk)^n / n! / (4 pi);  -jk = ki - j kr
+    const double mr = fr->ki, mi = -fr->kr;
+    double cr = 1.0 / (4.0 * kPi), ci = 0.0;
+    double sr[8], si[8];
+#pragma unroll
+    for (int x = 0; x < 8; ++x) { sr[x] = 0.0; si[x] = 0.0; }
+    for (int n = 0; n <= P; ++n) {
+#pragma unroll
+        for (int x = 0; x < 8; ++x) {
+            const double v = q[(size_t)n * 8 + x];
+            sr[x] += cr * v;
+            si[x] += ci * v;
+        }
+        const double inv = 1.0 / (double)(n + 1);
+        const double tr = (cr * mr - ci * mi) * inv, ti = (cr * mi + ci * mr) * inv;
+        cr = tr;
+        ci = ti;
+    }
+    double2* o = reinterpret_cast<double2*>(recs[ri].v);
+#pragma unroll
+    for (int x = 0; x < 8; ++x) o[x] = make_double2(sr[x], si[x]);
+}
+
 // ------------------------------------------------------------ combine ---
 
 template <bool ROWFAST>
@@ -588,6 +749,25 @@ hipError_t launch_rec_self(const Pt* selfPts, int Nq2, int Ns, const double* aSe
                            const Freq* fr, Rec* recs, hipStream_t s) {
     hipLaunchKernelGGL(k_rec_self, dim3((Nt + 3) / 4), dim3(256), 0, s, selfPts, Nq2, Ns, aSelf, Nt,
                        fr, recs);
+    return hipGetLastError();
+}
+
+hipError_t launch_moments(const Pt* subPts, int Np, const TriGeo* tg, const double* aeq,
+                          double near_alpha, const int2* list, int nPairs, const Pt* selfPts,
+                          int Nq2, int Ns, const double* aSelf, int Nt, int P, double* Qn,
+                          double* Qs, hipStream_t s) {
+    if (nPairs > 0)
+        hipLaunchKernelGGL(k_moment_near, dim3((nPairs + 3) / 4), dim3(256), 0, s, subPts, Np, tg, aeq,
+                           near_alpha, list, nPairs, P, Qn);
+    hipLaunchKernelGGL(k_moment_self, dim3((Nt + 3) / 4), dim3(256), 0, s, selfPts, Nq2, Ns, aSelf, Nt, P, Qs);
+    return hipGetLastError();
+}
+
+hipError_t launch_poly_records(const double* Qn, const int2* list, int nNear, const double* Qs, int Nt,
+                               int P, const Freq* fr, Rec* recs, hipStream_t s) {
+    const int n = nNear + Nt;
+    hipLaunchKernelGGL(k_poly_records, dim3((n + 255) / 256), dim3(256), 0, s, Qn, list, nNear, Qs, Nt, P, fr,
+                       recs);
     return hipGetLastError();
 }
 

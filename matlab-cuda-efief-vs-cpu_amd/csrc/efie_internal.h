@@ -71,6 +71,16 @@ hipError_t launch_rec_near(const Pt* subPts, int Np, const TriGeo* tg, const dou
 hipError_t launch_rec_self(const Pt* selfPts, int Nq2, int Ns, const double* aSelf,
                            int Nt, const Freq* fr, Rec* recs, hipStream_t s);
 
+// Frequency-independent moments Q[pair][q][8], q = 0..P, of the near pairs
+// (Qn) and the self pairs (Qs), and the per-frequency records built from
+// them (near pairs and self pairs).
+hipError_t launch_moments(const Pt* subPts, int Np, const TriGeo* tg, const double* aeq,
+                          double near_alpha, const int2* list, int nPairs, const Pt* selfPts,
+                          int Nq2, int Ns, const double* aSelf, int Nt, int P, double* Qn,
+                          double* Qs, hipStream_t s);
+hipError_t launch_poly_records(const double* Qn, const int2* list, int nNear, const double* Qs,
+                               int Nt, int P, const Freq* fr, Rec* recs, hipStream_t s);
+
 // Z(m,n) = sum over the four side pairs of alpha*Ivec + beta*Isca.
 // Output element (m,n) goes to Z[m*ldr + n*ldc] (interleaved complex).
 hipError_t launch_combine(const Rec* recs, const int* sideTri, const Side* side, int Ne,
@@ -110,10 +120,47 @@ int lu_tree_nodes(int n);
 int lu_cand_stride();
 template <typename T>
 hipError_t lu_factor(T* A, size_t strideA, int n, int lda, int batch, LuWork w,
-                     hipStream_t s);
+                     hipStream_t s, const struct Prof* prof);
 // Back substitution U x = y in place on column n of A.
 template <typename T>
 hipError_t lu_backsolve(T* A, size_t strideA, int n, int lda, int batch, hipStream_t s);
+// Triangular solve with the factors in A on a vector x (element e of batch
+// b at X[b*strideX + 2*e*xs]): unit lower L (lower) or upper U; batch
+// elements with skip[b] != 0 (skip may be null) are not touched.
+template <typename T>
+hipError_t lu_trsv(const T* A, size_t strideA, int n, int lda, T* X, size_t strideX, size_t xs,
+                   int batch, bool lower, const int* skip, hipStream_t s);
+// Row permutation equivalent to the sequential interchanges of ipiv.
+hipError_t lu_make_perm(const int* ipiv, int n, int batch, int* perm, hipStream_t s);
+
+// ------------------------------------------------ mixed precision -------
+hipError_t launch_to_f32(const double* A64, float* A32, size_t complexElems, hipStream_t s);
+hipError_t launch_col_to_vec(const float* A32, size_t strideA, int n, int lda, double* X, int batch,
+                             hipStream_t s);
+// R = V - A64 X per batch element (V[b] at V + b*vStride), rn2[b] += ||R||^2
+hipError_t launch_residual(const double* A64, size_t strideA, int n, int lda, const double* V,
+                           size_t vStride, const double* X, double* R, double* rn2, const int* done,
+                           int batch, hipStream_t s);
+hipError_t launch_ir_check(double* rn2, const double* vn2, double tol, int batch, int* done,
+                           int* iters, double* resid, int firstPass, int lastPass, hipStream_t s);
+hipError_t launch_perm_f32(const double* R, const int* perm, int n, int batch, float* R32,
+                           const int* done, hipStream_t s);
+hipError_t launch_add_f32(double* X, const float* D32, int n, int batch, const int* done,
+                          hipStream_t s);
+hipError_t launch_fill_int(int* p, int v, int n, hipStream_t s);
+hipError_t launch_fill_dbl(double* p, double v, int n, hipStream_t s);
+
+// Optional per-kernel-class timing hook (efie_opts.profile).
+struct Prof {
+    void* user = nullptr;
+    void (*mark)(void* user, int cls, bool start) = nullptr;
+    void begin(int cls) const {
+        if (mark) mark(user, cls, true);
+    }
+    void end(int cls) const {
+        if (mark) mark(user, cls, false);
+    }
+};
 
 // Zin epilogue of batch element b: x[e] = X[b*strideX + e*xinc] (complex),
 // Iport = sum(wP.*len.*x) - sum(wG.*len.*x) (or x[p]-x[g] in the centroid

@@ -291,12 +291,53 @@ __global__ __launch_bounds__(256) void k_laswp(T* A, size_t strideA, int n, int 
 }
 
 // ----------------------------------------------------- panel finish ------
-// Every block factors A11 (w x w) in LDS without pivoting, then either
-// solves a block of 256 rows of L21 = A21 U11^-1 or a block of 256 columns
-// of U12 = L11^-1 A12 (columns up to and including the rhs column n).
+// k_panel_diag: A11 (w x w, after the interchanges) = L11 U11 in place,
+// without pivoting (the tournament already ordered the pivot rows).
+template <typename T>
+__global__ __launch_bounds__(256) void k_panel_diag(T* A, size_t strideA, int lda, int j0, int w, int* info) {
+    __shared__ cpx<T> a11[NB * NB];
+    const int b = blockIdx.x;
+    T* Ab = A + (size_t)b * strideA;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < w * w; e += 256) {
+        int i = e / w, k = e % w;
+        a11[i * NB + k] = ld(Ab + 2 * ((size_t)(j0 + i) * lda + j0 + k));
+    }
+    __syncthreads();
+    for (int c = 0; c < w; ++c) {
+        const cpx<T> d = a11[c * NB + c];
+        const bool zp = (d.r == (T)0 && d.i == (T)0);
+        if (!zp) {
+            const cpx<T> inv = cinv(d);
+            for (int i = c + 1 + tid; i < w; i += 256) a11[i * NB + c] = cmul(a11[i * NB + c], inv);
+        } else if (tid == 0) {
+            atomicCAS(info + b, 0, j0 + c + 1);
+        }
+        __syncthreads();
+        if (!zp) {
+            const int rr = w - c - 1;
+            for (int e = tid; e < rr * rr; e += 256) {
+                int i = c + 1 + e / rr, k = c + 1 + e % rr;
+                cpx<T> l = a11[i * NB + c], u = a11[c * NB + k], v = a11[i * NB + k];
+                v.r -= l.r * u.r - l.i * u.i;
+                v.i -= l.r * u.i + l.i * u.r;
+                a11[i * NB + k] = v;
+            }
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < w * w; e += 256) {
+        int i = e / w, k = e % w;
+        st(Ab + 2 * ((size_t)(j0 + i) * lda + j0 + k), a11[i * NB + k]);
+    }
+}
+
+// k_panel: with L11 U11 in place, blocks [0, nRowBlocks) solve 256 rows of
+// L21 = A21 U11^-1 each and the others 256 columns of U12 = L11^-1 A12
+// (columns up to and including the rhs column n).
 template <typename T>
 __global__ __launch_bounds__(256) void k_panel(T* A, size_t strideA, int n, int lda, int j0, int w,
-                                               int nRowBlocks, int* info) {
+                                               int nRowBlocks) {
     __shared__ cpx<T> a11[NB * NB];
     const int b = blockIdx.y;
     T* Ab = A + (size_t)b * strideA;
@@ -306,34 +347,6 @@ __global__ __launch_bounds__(256) void k_panel(T* A, size_t strideA, int n, int 
         a11[i * NB + k] = ld(Ab + 2 * ((size_t)(j0 + i) * lda + j0 + k));
     }
     __syncthreads();
-    for (int c = 0; c < w; ++c) {
-        cpx<T> d = a11[c * NB + c];
-        bool zp = (d.r == (T)0 && d.i == (T)0);
-        if (!zp) {
-            cpx<T> inv = cinv(d);
-            for (int i = c + 1 + tid; i < w; i += 256) a11[i * NB + c] = cmul(a11[i * NB + c], inv);
-        }
-        __syncthreads();
-        if (!zp) {
-            int rr = w - c - 1;
-            for (int e = tid; e < rr * rr; e += 256) {
-                int i = c + 1 + e / rr, k = c + 1 + e % rr;
-                cpx<T> l = a11[i * NB + c], u = a11[c * NB + k], v = a11[i * NB + k];
-                v.r -= l.r * u.r - l.i * u.i;
-                v.i -= l.r * u.i + l.i * u.r;
-                a11[i * NB + k] = v;
-            }
-        } else if (tid == 0 && blockIdx.x == 0) {
-            atomicCAS(info + b, 0, j0 + c + 1);
-        }
-        __syncthreads();
-    }
-    if (blockIdx.x == 0) {
-        for (int e = tid; e < w * w; e += 256) {
-            int i = e / w, k = e % w;
-            st(Ab + 2 * ((size_t)(j0 + i) * lda + j0 + k), a11[i * NB + k]);
-        }
-    }
     if ((int)blockIdx.x < nRowBlocks) {
         // rows of L21: x U11 = a
         int row = j0 + w + blockIdx.x * 256 + tid;
@@ -388,7 +401,9 @@ __global__ __launch_bounds__(256) void k_panel(T* A, size_t strideA, int n, int 
 // j0..j0+w, cols c0..).  Block tile 64 x 64 complex, four waves of 32 x 32,
 // each 2 x 2 MFMA 16x16 tiles with separate real and imaginary accumulators.
 // MFMA 16x16x4 operand layout: A[i][k] in lane i + 16k, B[k][j] in lane
-// j + 16k, D[i][j] in lane j + 16(i/4), element i%4.
+// j + 16k.  The 16x16 result: lane l element v holds D[4(l/16)+v][l%16]
+// for the f32 instruction and D[4v + l/16][l%16] for the f64 instruction
+// (both measured on gfx950).
 
 template <typename T>
 struct mf;
@@ -398,6 +413,7 @@ struct mf<double> {
     static __device__ __forceinline__ v4 mfma(double a, double b, v4 c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lk, int v) { return 4 * v + lk; }
 };
 template <>
 struct mf<float> {
@@ -405,6 +421,7 @@ struct mf<float> {
     static __device__ __forceinline__ v4 mfma(float a, float b, v4 c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lk, int v) { return 4 * lk + v; }
 };
 
 template <typename T>
@@ -468,7 +485,7 @@ __global__ __launch_bounds__(256) void k_gemm_update(T* A, size_t strideA, int n
         for (int y = 0; y < 2; ++y)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                int row = r0 + wr + 16 * x + 4 * lk + v;
+                int row = r0 + wr + 16 * x + mf<T>::row(lk, v);
                 int col = c0 + wc + 16 * y + li;
                 if (row < n && col <= n) {
                     T* p = Ab + 2 * ((size_t)row * lda + col);
@@ -480,36 +497,41 @@ __global__ __launch_bounds__(256) void k_gemm_update(T* A, size_t strideA, int n
             }
 }
 
-// ------------------------------------------------- back substitution -----
-// Blocks of 64 rows from the bottom: solve the diagonal block in LDS, then
-// subtract its contribution from the rows above.
-template <typename T>
-__global__ __launch_bounds__(256) void k_trsv_diag(T* A, size_t strideA, int n, int lda, int r0,
-                                                   int bs) {
+// ------------------------------------------------- triangular solves -----
+// Blocked triangular solve on a vector x (element e of batch b at
+// X[b*strideX + 2*e*xs]), upper (U x = y, non-unit diagonal, bottom block
+// first) or unit lower (L y = r, top block first): the diagonal block is
+// solved by one wave in LDS, then the rows not yet solved are updated by a
+// GEMV.  Batch elements with skip[b] != 0 are left untouched.
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(256) void k_trsv_diag(const T* A, size_t strideA, int lda, T* X,
+                                                   size_t strideX, size_t xs, int r0, int bs,
+                                                   const int* skip) {
     __shared__ cpx<T> u[64][65];
     __shared__ cpx<T> x[64];
     const int b = blockIdx.x;
-    T* Ab = A + (size_t)b * strideA;
+    if (skip && skip[b]) return;
+    const T* Ab = A + (size_t)b * strideA;
+    T* Xb = X + (size_t)b * strideX;
     const int tid = threadIdx.x;
     for (int e = tid; e < bs * bs; e += 256) {
         int i = e / bs, k = e % bs;
         u[i][k] = ld(Ab + 2 * ((size_t)(r0 + i) * lda + r0 + k));
     }
-    for (int i = tid; i < bs; i += 256) x[i] = ld(Ab + 2 * ((size_t)(r0 + i) * lda + n));
+    for (int i = tid; i < bs; i += 256) x[i] = ld(Xb + 2 * (size_t)(r0 + i) * xs);
     __syncthreads();
     if (tid < 64) {
-        // one wave: column-oriented backward sweep
-        for (int j = bs - 1; j >= 0; --j) {
-            cpx<T> xj;
-            if (tid == 0) {
+        for (int jj = 0; jj < bs; ++jj) {
+            const int j = LOWER ? jj : bs - 1 - jj;
+            if (!LOWER && tid == 0) {
                 cpx<T> d = u[j][j];
-                xj = (d.r == (T)0 && d.i == (T)0) ? x[j] : cmul(x[j], cinv(d));
-                x[j] = xj;
+                x[j] = (d.r == (T)0 && d.i == (T)0) ? x[j] : cmul(x[j], cinv(d));
             }
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-            xj = x[j];
-            if (tid < j) {
+            const cpx<T> xj = x[j];
+            const bool act = LOWER ? (tid > j && tid < bs) : (tid < j);
+            if (act) {
                 cpx<T> l = u[tid][j];
                 cpx<T> v = x[tid];
                 v.r -= l.r * xj.r - l.i * xj.i;
@@ -521,32 +543,60 @@ __global__ __launch_bounds__(256) void k_trsv_diag(T* A, size_t strideA, int n, 
         }
     }
     __syncthreads();
-    for (int i = tid; i < bs; i += 256) st(Ab + 2 * ((size_t)(r0 + i) * lda + n), x[i]);
+    for (int i = tid; i < bs; i += 256) st(Xb + 2 * (size_t)(r0 + i) * xs, x[i]);
 }
 
+// rows [rlo, rhi) -= A[rows, r0:r0+bs] * x[r0:r0+bs]; one wave per row
 template <typename T>
-__global__ __launch_bounds__(256) void k_trsv_update(T* A, size_t strideA, int n, int lda, int r0,
-                                                     int bs) {
-    __shared__ cpx<T> x[64];
+__global__ __launch_bounds__(256) void k_trsv_update(const T* A, size_t strideA, int lda, T* X,
+                                                     size_t strideX, size_t xs, int r0, int bs,
+                                                     int rlo, int rhi, const int* skip) {
     const int b = blockIdx.y;
-    T* Ab = A + (size_t)b * strideA;
-    const int tid = threadIdx.x;
-    for (int i = tid; i < bs; i += 256) x[i] = ld(Ab + 2 * ((size_t)(r0 + i) * lda + n));
-    __syncthreads();
-    int row = blockIdx.x * 256 + tid;
-    if (row >= r0) return;
-    const T* p = Ab + 2 * ((size_t)row * lda + r0);
+    if (skip && skip[b]) return;
+    const T* Ab = A + (size_t)b * strideA;
+    T* Xb = X + (size_t)b * strideX;
+    const int lane = threadIdx.x & 63;
+    const int row = rlo + blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= rhi) return;
     cpx<T> s = {0, 0};
-    for (int k = 0; k < bs; ++k) {
-        cpx<T> u = ld(p + 2 * k);
-        s.r += u.r * x[k].r - u.i * x[k].i;
-        s.i += u.r * x[k].i + u.i * x[k].r;
+    if (lane < bs) {
+        cpx<T> u = ld(Ab + 2 * ((size_t)row * lda + r0 + lane));
+        cpx<T> xv = ld(Xb + 2 * (size_t)(r0 + lane) * xs);
+        s.r = u.r * xv.r - u.i * xv.i;
+        s.i = u.r * xv.i + u.i * xv.r;
     }
-    T* y = Ab + 2 * ((size_t)row * lda + n);
-    cpx<T> v = ld(y);
-    v.r -= s.r;
-    v.i -= s.i;
-    st(y, v);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        s.r += __shfl_xor(s.r, o, 64);
+        s.i += __shfl_xor(s.i, o, 64);
+    }
+    if (lane == 0) {
+        T* y = Xb + 2 * (size_t)row * xs;
+        cpx<T> v = ld(y);
+        v.r -= s.r;
+        v.i -= s.i;
+        st(y, v);
+    }
+}
+
+// perm[b][i] = source row of position i after the sequential swaps of ipiv
+// (applying ipiv to a vector v gives v_new[i] = v[perm[i]]).
+__global__ __launch_bounds__(256) void k_make_perm(const int* ipiv, int n, int* perm) {
+    extern __shared__ int sh[];
+    const int b = blockIdx.x;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) sh[i] = i;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int* ip = ipiv + (size_t)b * n;
+        for (int i = 0; i < n; ++i) {
+            int q = ip[i] - 1;
+            int t = sh[i];
+            sh[i] = sh[q];
+            sh[q] = t;
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) perm[(size_t)b * n + i] = sh[i];
 }
 
 // ---------------------------------------------------------- drivers ------
@@ -555,47 +605,81 @@ int lu_tree_nodes(int n) { return tree_geom(n).nNodes; }
 int lu_cand_stride() { return NB + 1; }
 
 template <typename T>
-hipError_t lu_factor(T* A, size_t strideA, int n, int lda, int batch, LuWork wk, hipStream_t s) {
+hipError_t lu_factor(T* A, size_t strideA, int n, int lda, int batch, LuWork wk, hipStream_t s,
+                     const Prof* pf) {
+    Prof none;
+    if (!pf) pf = &none;
     // candidate and counter slots per batch element are sized for the first
     // (largest) panel; every counter is back at zero when a launch ends
     const int nodeStride = tree_geom(n).nNodes;
     for (int j0 = 0; j0 < n; j0 += NB) {
         const int w = n - j0 < NB ? n - j0 : NB;
         TreeGeom g = tree_geom(n - j0);
+        pf->begin(5);
         hipLaunchKernelGGL(k_tslu<T>, dim3(g.levelCount[0], batch), dim3(256), 0, s, A, strideA, n,
                            lda, j0, w, g, nodeStride, wk.cand, wk.counters, wk.ipiv, wk.info);
+        pf->end(5);
+        pf->begin(6);
         hipLaunchKernelGGL(k_laswp<T>, dim3((n + 1 + 255) / 256, batch), dim3(256), 0, s, A,
                            strideA, n, lda, j0, w, wk.ipiv);
+        pf->end(6);
         const int rem = n - j0 - w;
         const int nRowBlocks = (rem + 255) / 256;
         const int nColBlocks = (rem + 1 + 255) / 256;
+        pf->begin(7);
+        hipLaunchKernelGGL(k_panel_diag<T>, dim3(batch), dim3(256), 0, s, A, strideA, lda, j0, w, wk.info);
         hipLaunchKernelGGL(k_panel<T>, dim3(nRowBlocks + nColBlocks, batch), dim3(256), 0, s, A,
-                           strideA, n, lda, j0, w, nRowBlocks, wk.info);
+                           strideA, n, lda, j0, w, nRowBlocks);
+        pf->end(7);
         if (rem > 0) {
             dim3 gg((rem + 1 + 63) / 64, (rem + 63) / 64, batch);
+            pf->begin(8);
             hipLaunchKernelGGL(k_gemm_update<T>, gg, dim3(256), 0, s, A, strideA, n, lda, j0, w);
+            pf->end(8);
         }
     }
     return hipGetLastError();
 }
 
 template <typename T>
-hipError_t lu_backsolve(T* A, size_t strideA, int n, int lda, int batch, hipStream_t s) {
-    int nblk = (n + 63) / 64;
-    for (int bk = nblk - 1; bk >= 0; --bk) {
-        int r0 = bk * 64;
-        int bs = n - r0 < 64 ? n - r0 : 64;
-        hipLaunchKernelGGL(k_trsv_diag<T>, dim3(batch), dim3(256), 0, s, A, strideA, n, lda, r0, bs);
-        if (r0 > 0)
-            hipLaunchKernelGGL(k_trsv_update<T>, dim3((r0 + 255) / 256, batch), dim3(256), 0, s, A,
-                               strideA, n, lda, r0, bs);
+hipError_t lu_trsv(const T* A, size_t strideA, int n, int lda, T* X, size_t strideX, size_t xs,
+                   int batch, bool lower, const int* skip, hipStream_t s) {
+    const int nblk = (n + 63) / 64;
+    for (int q = 0; q < nblk; ++q) {
+        const int bk = lower ? q : nblk - 1 - q;
+        const int r0 = bk * 64;
+        const int bs = n - r0 < 64 ? n - r0 : 64;
+        if (lower)
+            hipLaunchKernelGGL((k_trsv_diag<T, true>), dim3(batch), dim3(256), 0, s, A, strideA, lda, X, strideX,
+                               xs, r0, bs, skip);
+        else
+            hipLaunchKernelGGL((k_trsv_diag<T, false>), dim3(batch), dim3(256), 0, s, A, strideA, lda, X, strideX,
+                               xs, r0, bs, skip);
+        const int rlo = lower ? r0 + bs : 0, rhi = lower ? n : r0;
+        if (rhi > rlo)
+            hipLaunchKernelGGL(k_trsv_update<T>, dim3((rhi - rlo + 3) / 4, batch), dim3(256), 0, s, A, strideA,
+                               lda, X, strideX, xs, r0, bs, rlo, rhi, skip);
     }
     return hipGetLastError();
 }
 
-template hipError_t lu_factor<double>(double*, size_t, int, int, int, LuWork, hipStream_t);
-template hipError_t lu_factor<float>(float*, size_t, int, int, int, LuWork, hipStream_t);
+template <typename T>
+hipError_t lu_backsolve(T* A, size_t strideA, int n, int lda, int batch, hipStream_t s) {
+    return lu_trsv<T>(A, strideA, n, lda, A + 2 * (size_t)n, strideA, (size_t)lda, batch, false, nullptr, s);
+}
+
+hipError_t lu_make_perm(const int* ipiv, int n, int batch, int* perm, hipStream_t s) {
+    hipLaunchKernelGGL(k_make_perm, dim3(batch), dim3(256), sizeof(int) * (size_t)n, s, ipiv, n, perm);
+    return hipGetLastError();
+}
+
+template hipError_t lu_factor<double>(double*, size_t, int, int, int, LuWork, hipStream_t, const Prof*);
+template hipError_t lu_factor<float>(float*, size_t, int, int, int, LuWork, hipStream_t, const Prof*);
 template hipError_t lu_backsolve<double>(double*, size_t, int, int, int, hipStream_t);
 template hipError_t lu_backsolve<float>(float*, size_t, int, int, int, hipStream_t);
+template hipError_t lu_trsv<double>(const double*, size_t, int, int, double*, size_t, size_t, int, bool,
+                                    const int*, hipStream_t);
+template hipError_t lu_trsv<float>(const float*, size_t, int, int, float*, size_t, size_t, int, bool,
+                                   const int*, hipStream_t);
 
 }  // namespace efie

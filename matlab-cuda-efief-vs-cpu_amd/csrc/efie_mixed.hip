@@ -1,0 +1,161 @@
+// efie_mixed.hip -- kernels of the mixed-precision solve: complex FP32 LU
+// of the FP64-filled Z, then FP64 iterative refinement
+//   r = V - Z x (FP64),  d = (LU_32)^-1 r,  x += d
+// until ||r|| <= ir_tol ||V|| (per frequency, decided on the device).
+#include "efie_internal.h"
+
+namespace efie {
+
+// A32 = (float) A64 for an n x (n+1) row-major batch with leading dimension lda.
+__global__ void k_to_f32(const double* __restrict__ A64, float* __restrict__ A32, size_t total) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < total; i += stride) {
+        double2 v = reinterpret_cast<const double2*>(A64)[i];
+        reinterpret_cast<float2*>(A32)[i] = make_float2((float)v.x, (float)v.y);
+    }
+}
+
+// X64[b][i] = (double) A32[b][i][n] (the solved right-hand-side column).
+__global__ void k_col_to_vec(const float* __restrict__ A32, size_t strideA, int n, int lda,
+                             double* __restrict__ X, int batch) {
+    const int b = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || b >= batch) return;
+    float2 v = reinterpret_cast<const float2*>(A32 + (size_t)b * strideA)[(size_t)i * lda + n];
+    reinterpret_cast<double2*>(X + 2 * (size_t)b * n)[i] = make_double2(v.x, v.y);
+}
+
+// R = V - A64 X (one wave per row); rn2[b] += |R_i|^2.  rn2 must be zero.
+__global__ __launch_bounds__(256) void k_residual(const double* __restrict__ A64, size_t strideA,
+                                                  int n, int lda, const double* __restrict__ V,
+                                                  size_t vStride, const double* __restrict__ X,
+                                                  double* __restrict__ R, double* __restrict__ rn2,
+                                                  const int* __restrict__ done) {
+    const int b = blockIdx.y;
+    if (done[b]) return;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const double2* a = reinterpret_cast<const double2*>(A64 + (size_t)b * strideA) + (size_t)row * lda;
+    const double2* x = reinterpret_cast<const double2*>(X) + (size_t)b * n;
+    double sr = 0.0, si = 0.0;
+    for (int j = lane; j < n; j += 64) {
+        double2 av = a[j], xv = x[j];
+        sr += av.x * xv.x - av.y * xv.y;
+        si += av.x * xv.y + av.y * xv.x;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        sr += __shfl_xor(sr, o, 64);
+        si += __shfl_xor(si, o, 64);
+    }
+    if (lane == 0) {
+        const double2 v = reinterpret_cast<const double2*>(V + (size_t)b * vStride)[row];
+        const double rr = v.x - sr, ri = v.y - si;
+        reinterpret_cast<double2*>(R)[(size_t)b * n + row] = make_double2(rr, ri);
+        atomicAdd(rn2 + b, rr * rr + ri * ri);
+    }
+}
+
+// done[b] once ||R|| <= tol ||V|| or the residual stopped shrinking (less
+// than a factor 2 per correction: the FP64 evaluation floor of V - Z x);
+// keeps the last residual ratio and counts corrections; clears rn2.
+__global__ void k_ir_check(double* __restrict__ rn2, const double* __restrict__ vn2, double tol, int batch,
+                           int* __restrict__ done, int* __restrict__ iters, double* __restrict__ resid,
+                           int firstPass, int lastPass) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= batch) return;
+    if (!done[b]) {
+        const double ratio = sqrt(rn2[b] / vn2[b]);
+        const bool stalled = !firstPass && ratio > 0.5 * resid[b];
+        resid[b] = firstPass ? ratio : fmin(ratio, resid[b]);
+        if (ratio <= tol || stalled) done[b] = 1;
+        else if (!lastPass) iters[b] += 1;
+    }
+    rn2[b] = 0.0;
+}
+
+// R32[b][i] = (float) R[b][perm[b][i]]   (row interchanges of the LU)
+__global__ void k_perm_f32(const double* __restrict__ R, const int* __restrict__ perm, int n, int batch,
+                           float* __restrict__ R32, const int* __restrict__ done) {
+    const int b = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || b >= batch || done[b]) return;
+    const double2 v = reinterpret_cast<const double2*>(R)[(size_t)b * n + perm[(size_t)b * n + i]];
+    reinterpret_cast<float2*>(R32)[(size_t)b * n + i] = make_float2((float)v.x, (float)v.y);
+}
+
+// X += (double) D32
+__global__ void k_add_f32(double* __restrict__ X, const float* __restrict__ D32, int n, int batch,
+                          const int* __restrict__ done) {
+    const int b = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || b >= batch || done[b]) return;
+    double2* x = reinterpret_cast<double2*>(X) + (size_t)b * n + i;
+    const float2 d = reinterpret_cast<const float2*>(D32)[(size_t)b * n + i];
+    double2 v = *x;
+    v.x += (double)d.x;
+    v.y += (double)d.y;
+    *x = v;
+}
+
+__global__ void k_fill_int(int* p, int v, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_fill_dbl(double* p, double v, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+hipError_t launch_to_f32(const double* A64, float* A32, size_t complexElems, hipStream_t s) {
+    const size_t blocks = std::min<size_t>((complexElems + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)blocks), dim3(256), 0, s, A64, A32, complexElems);
+    return hipGetLastError();
+}
+
+hipError_t launch_col_to_vec(const float* A32, size_t strideA, int n, int lda, double* X, int batch,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(k_col_to_vec, dim3((n + 255) / 256, batch), dim3(256), 0, s, A32, strideA, n, lda, X,
+                       batch);
+    return hipGetLastError();
+}
+
+hipError_t launch_residual(const double* A64, size_t strideA, int n, int lda, const double* V,
+                           size_t vStride, const double* X, double* R, double* rn2, const int* done,
+                           int batch, hipStream_t s) {
+    hipLaunchKernelGGL(k_residual, dim3((n + 3) / 4, batch), dim3(256), 0, s, A64, strideA, n, lda, V, vStride,
+                       X, R, rn2, done);
+    return hipGetLastError();
+}
+
+hipError_t launch_ir_check(double* rn2, const double* vn2, double tol, int batch, int* done, int* iters,
+                           double* resid, int firstPass, int lastPass, hipStream_t s) {
+    hipLaunchKernelGGL(k_ir_check, dim3((batch + 63) / 64), dim3(64), 0, s, rn2, vn2, tol, batch, done, iters,
+                       resid, firstPass, lastPass);
+    return hipGetLastError();
+}
+
+hipError_t launch_perm_f32(const double* R, const int* perm, int n, int batch, float* R32, const int* done,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(k_perm_f32, dim3((n + 255) / 256, batch), dim3(256), 0, s, R, perm, n, batch, R32, done);
+    return hipGetLastError();
+}
+
+hipError_t launch_add_f32(double* X, const float* D32, int n, int batch, const int* done, hipStream_t s) {
+    hipLaunchKernelGGL(k_add_f32, dim3((n + 255) / 256, batch), dim3(256), 0, s, X, D32, n, batch, done);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_int(int* p, int v, int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_int, dim3((n + 255) / 256), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_dbl(double* p, double v, int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_dbl, dim3((n + 255) / 256), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+}  // namespace efie

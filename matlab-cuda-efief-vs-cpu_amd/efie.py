@@ -31,6 +31,7 @@ _ip = C.POINTER(C.c_int)
 EFIE_OK = 0
 MODEL_QUADRATURE, MODEL_CENTROID = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
+NEAR_DIRECT, NEAR_MOMENTS = 0, 1
 
 # physics constants of the reference drivers (main_patch_efie_rwg.m:218-220)
 C0 = 299792458.0
@@ -52,27 +53,35 @@ class Opts(C.Structure):
         ("quad_order", C.c_int), ("near_sing_eps", C.c_double), ("self_subdiv", C.c_int),
         ("self_alpha", C.c_double), ("near_subdiv", C.c_int), ("near_alpha", C.c_double),
         ("near_order", C.c_int), ("model", C.c_int), ("precision", C.c_int), ("batch", C.c_int),
-        ("ir_tol", C.c_double), ("ir_max_iter", C.c_int),
+        ("ir_tol", C.c_double), ("ir_max_iter", C.c_int), ("near_mode", C.c_int), ("profile", C.c_int),
     ]
+
+
+KERNEL_CLASSES = ["regular", "near", "self", "poly", "combine", "tslu", "laswp", "panel", "gemm",
+                  "trsv", "other"]
 
 
 class Stats(C.Structure):
     _fields_ = [
         ("ms_total", C.c_double), ("ms_fill", C.c_double), ("ms_factor", C.c_double),
-        ("ms_solve", C.c_double), ("ms_fill_kernel", C.c_double), ("n_fill_kernel", C.c_int),
-        ("n_near_pairs", C.c_longlong), ("n_green", C.c_longlong), ("ir_iters_max", C.c_int),
-        ("ir_resid_max", C.c_double),
+        ("ms_solve", C.c_double), ("n_near_pairs", C.c_longlong), ("n_green", C.c_longlong),
+        ("ir_iters_max", C.c_int), ("ir_resid_max", C.c_double), ("moment_terms", C.c_int),
+        ("ms_moments", C.c_double), ("ms_kernel", C.c_double * len(KERNEL_CLASSES)),
+        ("n_kernel", C.c_int * len(KERNEL_CLASSES)),
     ]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("ms_kernel", "n_kernel")}
+        d["kernels"] = {name: {"ms": self.ms_kernel[i], "n": self.n_kernel[i]}
+                        for i, name in enumerate(KERNEL_CLASSES) if self.n_kernel[i]}
+        return d
 
 
 EXPORTS = [
     "efie_opts_default", "efie_opts_preset", "efie_last_error", "efie_last_error_id",
     "efie_mesh_counts", "efie_make_mesh", "efie_rwg_count", "efie_build_rwg", "efie_pick_port",
     "efie_create", "efie_destroy", "efie_set_mesh", "efie_set_rwg", "efie_assemble_zv",
-    "efie_sweep",
+    "efie_sweep", "efie_zgesv",
 ]
 
 _lib = None
@@ -104,6 +113,7 @@ def lib():
         L.efie_assemble_zv.argtypes = [C.c_void_p] + [C.c_double] * 6 + [_dp, C.c_int, C.POINTER(Opts), _dp, _dp]
         L.efie_sweep.argtypes = [C.c_void_p, _dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
                                  C.c_double, C.c_double, C.c_double, C.POINTER(Opts), _dp, _dp, C.POINTER(Stats)]
+        L.efie_zgesv.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, C.POINTER(Stats)]
         _lib = L
     return _lib
 
@@ -261,6 +271,25 @@ class Context:
                                 V0, spreadRadius, Z0, C.byref(o), _d(zin), _d(s11), C.byref(st)))
         out = (zin.view(np.complex128).copy(), s11.view(np.complex128).copy())
         return (*out, st.as_dict()) if stats else out
+
+
+    def zgesv(self, A, b, precision=PREC_FP64, stats=False):
+        """x = A \\ b for a batch: A [batch, n, n] (or [n, n]), b [batch, n]."""
+        A = np.asarray(A, dtype=np.complex128)
+        single = A.ndim == 2
+        if single:
+            A = A[None]
+            b = np.asarray(b)[None]
+        nb, n = A.shape[0], A.shape[1]
+        Af = np.ascontiguousarray(np.transpose(A, (0, 2, 1)))  # column-major per matrix
+        bf = np.ascontiguousarray(b, dtype=np.complex128)
+        x = np.zeros((nb, n), dtype=np.complex128)
+        st = Stats()
+        _check(lib().efie_zgesv(self._c, n, nb, Af.view(np.float64).ctypes.data_as(_dp),
+                                bf.view(np.float64).ctypes.data_as(_dp), precision,
+                                x.view(np.float64).ctypes.data_as(_dp), C.byref(st)))
+        x = x[0] if single else x
+        return (x, st.as_dict()) if stats else x
 
 
 # ------------------------------------------------- reference MEX mirrors --

@@ -1,0 +1,207 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Tolerances (FP64 path): Z entries within 1e-10 of the oracle relative to the
+entry (observed ~1e-15), Zin within 1e-10 relative.  The mixed-precision
+path (complex FP32 LU + FP64 refinement) is held to the same 1e-10 on Zin.
+"""
+import numpy as np
+import pytest
+
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+ZTOL = 1e-10
+ZIN_TOL = 1e-10
+
+
+@pytest.fixture(scope="module")
+def ctx(efie_mod):
+    c = efie_mod.Context(0)
+    yield c
+    c.close()
+
+
+def _setup(efie_mod, orc, name):
+    V, F = efie_mod.make_patch_with_ground_mesh(*W.mesh_args(name))
+    r = efie_mod.build_rwg_interior(V, F)
+    pP, pG = W.feed_points(name)
+    pe = efie_mod.pick_port_edge_by_point_and_layer(r, pP, 1)
+    ge = efie_mod.pick_port_edge_by_point_and_layer(r, pG, -1)
+    P = orc.Problem(V, F, orc.build_rwg(V, F))
+    return V, F, r, pe, ge, 1.5 * r.len.mean(), P
+
+
+def _entry_rel(A, B):
+    return float(np.max(np.abs(A - B) / np.abs(B)))
+
+
+# ------------------------------------------------------------ dense solve --
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("n", [1, 7, 32, 33, 64, 65, 200, 517])
+def test_zgesv_random(efie_mod, ctx, prec, n):
+    rng = np.random.default_rng(n)
+    nb = 3
+    A = rng.standard_normal((nb, n, n)) + 1j * rng.standard_normal((nb, n, n))
+    b = rng.standard_normal((nb, n)) + 1j * rng.standard_normal((nb, n))
+    x = ctx.zgesv(A, b, prec)
+    xr = np.linalg.solve(A, b[..., None])[..., 0]
+    cond = max(np.linalg.cond(a) for a in A)
+    assert np.abs(x - xr).max() <= 1e-14 * cond * np.abs(xr).max() * max(n, 10)
+
+
+def test_zgesv_known_answers(efie_mod, ctx):
+    n = 70
+    I = np.eye(n, dtype=complex)
+    b = np.arange(n) + 1j
+    assert np.array_equal(ctx.zgesv(I, b), b)
+    Pm = I[np.random.default_rng(3).permutation(n)]
+    assert np.array_equal(ctx.zgesv(Pm, b), Pm.T @ b)
+    D = np.diag(np.arange(1, n + 1) * (1 + 1j))
+    assert np.allclose(ctx.zgesv(D, b), b / np.diag(D), rtol=1e-15, atol=0)
+    S = np.ones((n, n), dtype=complex)
+    with pytest.raises(efie_mod.EfieError) as e:
+        ctx.zgesv(S, b)
+    assert e.value.ident == "efie:singular"
+
+
+# ---------------------------------------------------------------- the fill --
+
+OPTSETS = [
+    dict(quad_order=4, near_sing_eps=1.2, near_subdiv=1, near_order=2),
+    dict(quad_order=7, near_sing_eps=1.2, near_subdiv=2, near_order=4),
+    dict(quad_order=2, near_sing_eps=2.5, near_subdiv=2, near_order=4),
+    dict(quad_order=1, near_sing_eps=0.5, near_subdiv=1, near_order=1, self_subdiv=1),
+    dict(quad_order=7, near_sing_eps=2.5, near_subdiv=3, near_order=7, self_subdiv=3),
+]
+
+
+@pytest.mark.parametrize("near_mode", [0, 1])
+@pytest.mark.parametrize("ok", range(len(OPTSETS)))
+def test_full_z_tiny(efie_mod, orc, ctx, ok, near_mode):
+    V, F, r, pe, ge, spread, P = _setup(efie_mod, orc, "tiny")
+    ctx.set_problem(V, F, r)
+    o = dict(W.OPTS_C1)
+    o.update(OPTSETS[ok])
+    w = 2 * np.pi * 2.5e9
+    eps = W.EPS0 * (2.2 - 0.01j)
+    k = orc.wavenumber(w, W.MU0, eps)
+    Zg, Vg = ctx.assemble_zv(k, w, W.MU0, eps, [pe, ge, 1.0, spread, 0.2],
+                             efie_mod.opts(**o, near_mode=near_mode))
+    Zo = P.z_full(k, w, W.MU0, eps, orc.opts(**o))
+    assert _entry_rel(Zg, Zo) <= ZTOL
+    assert np.abs(Vg - P.rhs(pe, ge, 1.0, spread)).max() <= 1e-15
+
+
+def test_centroid_model(efie_mod, orc, ctx):
+    V, F, r, pe, ge, spread, P = _setup(efie_mod, orc, "tiny")
+    ctx.set_problem(V, F, r)
+    w = 2 * np.pi * 2e9
+    eps = W.EPS0 * (1 - 1e-3j)
+    k = orc.wavenumber(w, W.MU0, eps)
+    for alpha in (0.2, 0.35):
+        Zg, _ = ctx.assemble_zv(k, w, W.MU0, eps, [pe, ge, 1.0, spread, alpha], efie_mod.opts("centroid"))
+        Zo = P.z_centroid(k, 1j * w * W.MU0, 1 / (1j * w * eps), alpha)
+        assert _entry_rel(Zg, Zo) <= ZTOL
+
+
+@pytest.mark.parametrize("near_mode", [0, 1])
+def test_c1_rows_vs_golden(efie_mod, orc, ctx, near_mode):
+    g = np.load(W_GOLDEN("c1.npz"))
+    V, F, r, pe, ge, spread, P = _setup(efie_mod, orc, "c1")
+    assert r.Ne == int(g["Ne"]) and (pe, ge) == tuple(g["ports"])
+    for k in ("edge", "plusTri", "minusTri", "plusSign", "minusSign"):
+        assert np.array_equal(getattr(r, k), g[k])
+    ctx.set_problem(V, F, r)
+    w = 2 * np.pi * g["freqs"][0]
+    k = orc.wavenumber(w, W.MU0, W.EPS0)
+    Zg, Vg = ctx.assemble_zv(k, w, W.MU0, W.EPS0, [pe, ge, 1.0, spread, 0.2],
+                             efie_mod.opts(**W.OPTS_C1, near_mode=near_mode))
+    rows = g["rows"] - 1
+    assert _entry_rel(Zg[rows], g["zrows"]) <= ZTOL
+    assert np.abs(Vg - g["vrhs"]).max() <= 1e-15
+    # complex symmetry of the whole matrix
+    assert np.abs(Zg - Zg.T).max() <= 1e-13 * np.abs(Zg).max()
+
+
+# ---------------------------------------------------------------- the sweep --
+
+def W_GOLDEN(name):
+    import os
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", name)
+
+
+@pytest.mark.parametrize("prec,near_mode,batch", [(0, 0, 0), (0, 1, 2), (1, 1, 0), (1, 0, 5)])
+def test_c1_sweep_vs_golden(efie_mod, orc, ctx, prec, near_mode, batch):
+    g = np.load(W_GOLDEN("c1.npz"))
+    V, F, r, pe, ge, spread, _ = _setup(efie_mod, orc, "c1")
+    ctx.set_problem(V, F, r)
+    o = efie_mod.opts(**W.OPTS_C1, precision=prec, near_mode=near_mode, batch=batch)
+    zin, s11, st = ctx.sweep(g["freqs"], W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0, o, stats=True)
+    assert np.max(np.abs(zin - g["zin"]) / np.abs(g["zin"])) <= ZIN_TOL
+    assert np.max(np.abs(s11 - g["s11"])) <= ZIN_TOL
+    if prec == 1:
+        # refinement reaches the FP64 floor of ||V - Z x|| / ||V|| (~1e-12 here)
+        assert st["ir_resid_max"] <= 1e-11 and st["ir_iters_max"] <= 8
+
+
+def test_c1_centroid_sweep_vs_golden(efie_mod, orc, ctx):
+    g = np.load(W_GOLDEN("c1.npz"))
+    V, F, r, pe, ge, spread, _ = _setup(efie_mod, orc, "c1")
+    ctx.set_problem(V, F, r)
+    zin, s11 = ctx.sweep(g["freqs"], W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0, efie_mod.opts("centroid"))
+    assert np.max(np.abs(zin - g["zin_centroid"]) / np.abs(g["zin_centroid"])) <= ZIN_TOL
+    assert np.max(np.abs(s11 - g["s11_centroid"])) <= ZIN_TOL
+
+
+def test_lossy_sweep_tiny_vs_oracle(efie_mod, orc, ctx):
+    V, F, r, pe, ge, spread, P = _setup(efie_mod, orc, "tiny")
+    ctx.set_problem(V, F, r)
+    fr = W.freqs("tiny")
+    eps = W.EPS0 * 2.59 * (1 - 1e-2j)
+    for prec, nm in ((0, 0), (1, 1)):
+        zin, s11 = ctx.sweep(fr, W.MU0, eps, pe, ge, 1.0, spread, 50.0,
+                             efie_mod.opts(**W.OPTS_C1, precision=prec, near_mode=nm))
+        zo, so = P.sweep(fr, W.MU0, eps, pe, ge, 1.0, spread, 50.0, orc.opts(**W.OPTS_C1))
+        assert np.max(np.abs(zin - zo) / np.abs(zo)) <= ZIN_TOL
+
+
+def test_sweep_errors(efie_mod, orc, ctx):
+    V, F, r, pe, ge, spread, _ = _setup(efie_mod, orc, "micro")
+    ctx.set_problem(V, F, r)
+    fr = W.freqs("micro")
+    with pytest.raises(efie_mod.EfieError) as e:
+        ctx.sweep(fr, W.MU0, W.EPS0, r.Ne + 1, ge, 1.0, spread, 50.0)
+    assert e.value.ident == "mex:port"
+    with pytest.raises(efie_mod.EfieError) as e:
+        ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, 0.0, 50.0)
+    assert e.value.ident == "mex:rhs"
+    zin, s11 = ctx.sweep(np.array([]), W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0)
+    assert zin.size == 0
+
+
+# ---------------------------------------------------------- the MEX layer --
+
+def test_mex_entry_points_on_device(efie_mod, orc):
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "mxshim"))
+    import mx
+    g = np.load(W_GOLDEN("c1.npz"))
+    V, F, r, pe, ge, spread, P = _setup(efie_mod, orc, "c1")
+    tables = [V, F, r.edge, r.plusTri, r.minusTri, r.plusSign, r.minusSign, r.rp, r.rm, r.Ap, r.Am, r.len]
+    opts = {k: float(v) for k, v in W.OPTS_C1.items()}
+    zin, s11 = mx.call("patch_sweep_cuda", 2, *tables, g["freqs"], W.MU0, complex(W.EPS0, 0), np.int32(pe),
+                       np.int32(ge), 1.0, spread, 50.0, opts)
+    assert np.max(np.abs(zin[:, 0] - g["zin"]) / np.abs(g["zin"])) <= ZIN_TOL
+    # centroid model of the reference MEX when opts.model = 1 (20-argument call uses quadrature)
+    zc, _ = mx.call("patch_sweep_cuda", 2, *tables, g["freqs"], W.MU0, W.EPS0, np.int32(pe), np.int32(ge), 1.0,
+                    spread, 50.0, {"model": 1.0})
+    assert np.max(np.abs(zc[:, 0] - g["zin_centroid"]) / np.abs(g["zin_centroid"])) <= ZIN_TOL
+    w = 2 * np.pi * g["freqs"][0]
+    k = w * np.sqrt(W.MU0 * W.EPS0)
+    Z, Vr = mx.call("gassemble_ZV_EFIE_RWG_DIFF_GPU_FAST", 2, *tables, complex(k, 0), w, W.MU0, W.EPS0,
+                    np.array([pe, ge, 1.0, spread, 0.2]), opts)
+    assert _entry_rel(Z[g["rows"] - 1], g["zrows"]) <= ZTOL
+    assert np.abs(Vr[:, 0] - g["vrhs"]).max() <= 1e-15
