@@ -3,12 +3,17 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2_6k]
 
 One step is one full frequency sweep of the configuration (fill Z, LU
-solve, Zin/S11 for every frequency) with the mesh and RWG tables already
-resident in HBM.  The printed JSON line carries:
-  value        frequency points per second of the headline path,
-  roofline     the dominant kernel's achieved rate against the MI355X peak,
-  cpu_baseline the CPU oracle (this project's C restatement of the
-               reference assembler) timed on a bounded sample on the host.
+solve, Zin/S11 at every frequency) with the mesh and RWG tables already
+resident in HBM.  Two paths are timed:
+  fp64  the plain method: per-frequency quadrature of every triangle pair
+        (as the reference does) and an FP64 LU;
+  fast  the headline path: near and self pairs from frequency-independent
+        moments, complex FP32 LU on the matrix cores and FP64 iterative
+        refinement.
+`value` is the fast path's frequency points per second; the JSON line also
+carries the fp64 path's rate, the largest |dZin|/|Zin| between the two, a
+roofline entry for the dominant kernel (from a profiled extra sweep) and the
+CPU baseline (the oracle on a bounded sample).
 """
 from __future__ import annotations
 
@@ -26,10 +31,10 @@ sys.path.insert(0, os.path.join(ROOT, "matlab-cuda-efief-vs-cpu_amd"))
 import efie  # noqa: E402
 import workloads as W  # noqa: E402
 
-PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 (matrix and vector)
-PEAK_FP32_TFLOPS = 157.3  # MI355X dense FP32 matrix
+PEAK = {"fp64": 78.6, "fp32": 157.3}  # MI355X dense TFLOP/s (FP64 matrix = vector)
 PEAK_HBM_GBS = 8000.0
-FLOP_PER_GREEN = 48       # flop count convention per Green-function evaluation
+FLOP_PER_GREEN = 48   # per Green-function evaluation incl. its moment updates
+NB = 32               # LU panel width of the library
 
 
 def log(*a):
@@ -45,31 +50,74 @@ def setup(config):
     return V, F, r, pe, ge, 1.5 * r.len.mean(), W.freqs(config)
 
 
-def run_sweeps(ctx, args, o, fr, pe, ge, spread, steps, warmup):
+def timed(ctx, o, fr, pe, ge, spread, steps, warmup, name):
     for i in range(warmup):
         ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0, o)
-        log(f"warmup {i + 1}/{warmup} done")
+        log(f"[{name}] warmup {i + 1}/{warmup} done")
     t0 = time.perf_counter()
     out = None
     for i in range(steps):
         out = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0, o, stats=True)
-        log(f"step {i + 1}/{steps} done")
+        log(f"[{name}] step {i + 1}/{steps} done")
     dt = time.perf_counter() - t0  # each sweep returns after its stream has drained
     return dt, out
 
 
-def cpu_baseline(V, F, r, Ne, fr, budget_s=20.0):
-    """Oracle fill of a few rows of Z at one frequency plus an oracle LU at
-    n=1024, both on all host threads, extrapolated to one frequency point."""
+def lu_gemm_flops(n):
+    f = 0.0
+    for j0 in range(0, n, NB):
+        w = min(NB, n - j0)
+        rem = n - j0 - w
+        if rem > 0:
+            f += 8.0 * rem * (rem + 1) * w
+    return f
+
+
+def roofline(st, n, nf, fast):
+    """Dominant kernel class of a profiled sweep and its achieved rate."""
+    ks = st["kernels"]
+    name = max(ks, key=lambda k: ks[k]["ms"])
+    ms, cnt = ks[name]["ms"], ks[name]["n"]
+    if name == "gemm":
+        work = lu_gemm_flops(n) * nf
+        prec = "fp32" if fast else "fp64"
+        ach = work / (ms * 1e-3) / 1e12
+        return {"kernel": "k_gemm_update", "bound": "mfma", "achieved": ach, "peak": PEAK[prec],
+                "unit": "TFLOP/s", "frac": ach / PEAK[prec], "traffic": None,
+                "work": f"8*rem*(rem+1)*w flop per panel update, {work/cnt:.3e} flop/launch avg, "
+                        f"{cnt} launches, {ms/cnt*1e3:.1f} us/launch avg"}
+    if name in ("regular", "near", "self"):
+        evals = st["n_green_" + name]
+        work = evals * FLOP_PER_GREEN * nf
+        ach = work / (ms * 1e-3) / 1e12
+        return {"kernel": "k_rec_" + name, "bound": "mfma", "achieved": ach, "peak": PEAK["fp64"],
+                "unit": "TFLOP/s", "frac": ach / PEAK["fp64"], "traffic": None,
+                "work": f"{evals:.3e} Green evaluations x {FLOP_PER_GREEN} flop per launch, {cnt} launches, "
+                        f"{ms/cnt*1e3:.1f} us/launch avg (FP64 vector peak = FP64 matrix peak)"}
+    if name == "combine":
+        byts = 16.0 * n * n  # Z written once per frequency
+        ach = byts * nf / (ms * 1e-3) / 1e9
+        return {"kernel": "k_combine", "bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": ach / PEAK_HBM_GBS, "traffic": None,
+                "work": f"16*N^2 B of Z written per launch, {cnt} launches, {ms/cnt*1e3:.1f} us/launch avg"}
+    return {"kernel": name, "bound": "mfma", "achieved": None, "peak": None, "unit": None, "frac": None,
+            "traffic": None, "work": f"{ms:.1f} ms over {cnt} launches"}
+
+
+def cpu_baseline(V, F, Ne, fr):
+    """Oracle fill of a few rows of Z at one frequency (all host threads)
+    plus an oracle LU at n=1024 (one thread), extrapolated to one frequency
+    point of the full problem."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     threads = min(os.cpu_count() or 1, 16)
     ro = O.build_rwg(V, F)
     P = O.Problem(V, F, ro)
     o = O.opts(**W.OPTS_C1)
-    w = 2 * np.pi * fr[len(fr) // 2]
+    fmid = fr[len(fr) // 2]
+    w = 2 * np.pi * fmid
     k = O.wavenumber(w, W.MU0, W.EPS0)
-    rows = max(threads, 8)
+    rows = max(2 * threads, 16)
     t = time.perf_counter()
     P.z_rows(np.linspace(1, Ne, rows).astype(np.int32), k, w, W.MU0, W.EPS0, o, threads)
     t_rows = time.perf_counter() - t
@@ -79,13 +127,13 @@ def cpu_baseline(V, F, r, Ne, fr, budget_s=20.0):
     t = time.perf_counter()
     O.zgetrf(A)
     t_lu = time.perf_counter() - t
-    per_freq = t_rows * Ne / rows + t_lu * (Ne / n_lu) ** 3
-    return {"value": 1.0 / per_freq, "unit": "freq points/s", "cores": 1,
-            "kind": "port",
-            "sample": f"oracle fill of {rows} rows of Z at f={fr[len(fr)//2]/1e9:.3f} GHz "
-                      f"({t_rows:.1f}s, scaled x{Ne/rows:.0f}) + oracle zgetrf at n={n_lu} "
-                      f"({t_lu:.1f}s, scaled x{(Ne/n_lu)**3:.0f}); fill on {threads} threads, LU on 1",
-            "fill_threads": threads}
+    fill_s = t_rows * Ne / rows
+    lu_s = t_lu * (Ne / n_lu) ** 3
+    return {"value": 1.0 / (fill_s + lu_s), "unit": "freq points/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (C restatement of assemble_ZV_EFIE_RWG_DIFF_CPU) fill of {rows} rows of Z "
+                      f"at {fmid/1e9:.2f} GHz on {threads} threads ({t_rows:.1f} s, x{Ne/rows:.0f} -> "
+                      f"{fill_s:.0f} s per frequency) + oracle zgetrf at n={n_lu} on 1 thread "
+                      f"({t_lu:.1f} s, x{(Ne/n_lu)**3:.0f} -> {lu_s:.0f} s)"}
 
 
 def main():
@@ -95,31 +143,52 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2_6k")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {a.gpus}); the frequency sweep would shard "
-              f"by frequency across GPUs, which this build does not run.", file=sys.stderr)
+        print(f"bench.py measures one GPU (asked for {a.gpus}): the sweep would shard by frequency "
+              f"across GPUs, which this build does not run.", file=sys.stderr)
         sys.exit(2)
     V, F, r, pe, ge, spread, fr = setup(a.config)
-    log(f"config {a.config}: Ne={r.Ne} Nt={F.shape[0]} Nf={len(fr)}")
-    o64 = efie.opts(**W.OPTS_C1)
+    n = r.Ne
+    log(f"config {a.config}: Ne={n} Nt={F.shape[0]} Nf={len(fr)}")
+    o64 = efie.opts(**W.OPTS_C1, precision=efie.PREC_FP64, near_mode=efie.NEAR_DIRECT)
+    ofast = efie.opts(**W.OPTS_C1, precision=efie.PREC_MIXED, near_mode=efie.NEAR_MOMENTS)
     with efie.Context(0) as ctx:
         ctx.set_problem(V, F, r)
-        dt, (zin, s11, st) = run_sweeps(ctx, a, o64, fr, pe, ge, spread, a.steps, a.warmup)
-    nf = len(fr) * a.steps
-    value = nf / dt
+        dt_fast, (zin_f, s11_f, st_f) = timed(ctx, ofast, fr, pe, ge, spread, a.steps, a.warmup, "fast")
+        res = {}
+        if not a.no_fp64:
+            dt_64, (zin_64, s11_64, st_64) = timed(ctx, o64, fr, pe, ge, spread, a.steps, a.warmup, "fp64")
+            res["fp64"] = {"value": len(fr) * a.steps / dt_64, "ms_per_step": 1e3 * dt_64 / a.steps,
+                           "stats": st_64}
+            res["max_rel_err_zin_fast_vs_fp64"] = float(np.max(np.abs(zin_f - zin_64) / np.abs(zin_64)))
+        # profiled pass of the headline path for the roofline
+        oprof = efie.opts(**W.OPTS_C1, precision=efie.PREC_MIXED, near_mode=efie.NEAR_MOMENTS, profile=1)
+        _, _, st_p = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0, oprof, stats=True)
+        log("profiled pass done")
+    Nt = F.shape[0]
+    reg_pairs = Nt * (Nt - 1) // 2 - st_p["n_near_pairs"]
+    nq = {4: 6, 7: 7, 2: 3, 1: 1}.get(W.OPTS_C1["quad_order"], 6)
+    st_p["n_green_regular"] = reg_pairs * nq * nq
+    value = len(fr) * a.steps / dt_fast
     line = {
         "metric": "freq points/s (Z fill+LU) at N~6k, mixed vs FP64; max rel err Zin vs FP64",
         "value": value, "unit": "freq points/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": 1e3 * dt / a.steps, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "c128", "data": "synthetic",
-        "config": {"workload": f"{a.config}: patch {W.CONFIGS[a.config]['cells'][:2]} + ground "
-                               f"{W.CONFIGS[a.config]['cells'][2:]} cells, Ne={r.Ne}, Nf={len(fr)}, "
-                               f"FP64 fill + FP64 LU"},
-        "stats": st,
+        "ms_per_step": 1e3 * dt_fast / a.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c128 fill, c64 LU + c128 refinement", "data": "synthetic",
+        "config": {"workload": f"{a.config}: patch {W.CONFIGS[a.config]['cells'][0]}x"
+                               f"{W.CONFIGS[a.config]['cells'][1]} + ground {W.CONFIGS[a.config]['cells'][2]}x"
+                               f"{W.CONFIGS[a.config]['cells'][3]} cells, Ne={n}, Nt={Nt}, Nf={len(fr)} "
+                               f"({fr[0]/1e9:.1f}-{fr[-1]/1e9:.1f} GHz), 7-pt regular rule, 6-pt x16 near/self"},
+        "roofline": roofline(st_p, n, len(fr), True),
+        "fast": {"stats": st_f},
+        **res,
+        "profile": st_p["kernels"],
     }
     if not a.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(V, F, r, r.Ne, fr)
+        line["cpu_baseline"] = cpu_baseline(V, F, n, fr)
+        log("cpu baseline done")
     print(json.dumps(line))
 
 
