@@ -153,7 +153,8 @@ def main():
     n = r.Ne
     log(f"config {a.config}: Ne={n} Nt={F.shape[0]} Nf={len(fr)}")
     o64 = efie.opts(**W.OPTS_C1, precision=efie.PREC_FP64, near_mode=efie.NEAR_DIRECT)
-    ofast = efie.opts(**W.OPTS_C1, precision=efie.PREC_MIXED, near_mode=efie.NEAR_MOMENTS)
+    # refinement to ||V - Zx|| <= 1e-10 ||V||
+    ofast = efie.opts(**W.OPTS_C1, precision=efie.PREC_MIXED, near_mode=efie.NEAR_MOMENTS, ir_tol=1e-10)
     with efie.Context(0) as ctx:
         ctx.set_problem(V, F, r)
         dt_fast, (zin_f, s11_f, st_f) = timed(ctx, ofast, fr, pe, ge, spread, a.steps, a.warmup, "fast")
@@ -164,7 +165,8 @@ def main():
                            "stats": st_64}
             res["max_rel_err_zin_fast_vs_fp64"] = float(np.max(np.abs(zin_f - zin_64) / np.abs(zin_64)))
         # profiled pass of the headline path for the roofline
-        oprof = efie.opts(**W.OPTS_C1, precision=efie.PREC_MIXED, near_mode=efie.NEAR_MOMENTS, profile=1)
+        oprof = efie.opts(**W.OPTS_C1, precision=efie.PREC_MIXED, near_mode=efie.NEAR_MOMENTS, ir_tol=1e-10,
+                      profile=1)
         _, _, st_p = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0, oprof, stats=True)
         log("profiled pass done")
     Nt = F.shape[0]

@@ -55,7 +55,8 @@ typedef struct {
     int near_order;
     int model;          /* EFIE_MODEL_*                                       */
     int precision;      /* EFIE_PREC_*                                        */
-    int batch;          /* frequencies factored together (0 = automatic)      */
+    int batch;          /* frequencies factored together (0 = automatic: up   *
+                         * to 32, within 60% of the free device memory)       */
     double ir_tol;      /* mixed: stop refinement at ||V-Zx||/||V|| <= ir_tol  *
                          * or when a correction no longer halves the residual */
     int ir_max_iter;    /* mixed: refinement iteration cap                    */

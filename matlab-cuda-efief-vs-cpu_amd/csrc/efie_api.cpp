@@ -191,13 +191,15 @@ struct efie_ctx {
     double nearAlphaMom = -1;
 
     DBuf dTri, dReg, dNearPts, dSelfPts, dASelf, dAeq, dNearMat, dRowCnt, dList;
-    DBuf dSideTri, dSide, dMid, dLen, dQn, dQs;
+    DBuf dSideTri, dSide, dMid, dLen, dQn, dQs, dRad;
     // centroid model tables
     DBuf dCPos, dCF, dCDv, dCW, dCT, dCAsoft;
     double cAlpha = -1;
     // per-call buffers
     DBuf dRecs, dFreq, dZ, dZ32, dWp, dWg, dSums, dV, dWl, dOut;
     DBuf dCand, dCnt, dIpiv, dInfo, dPerm, dX, dR, dR32, dRn2, dBn2, dDone, dIters, dResid, dB;
+    DBuf dFlags, dTickets, dErr;
+    TrsvWork tw{nullptr, nullptr, nullptr, 0};
 
     // profiling (opts.profile): event pairs per kernel class
     bool prof = false;
@@ -648,8 +650,12 @@ efie_status ensure_geometry(efie_ctx* c, const efie_opts* o) {
     std::vector<double> aSelf;
     const int NsN = subdiv_pts(o->near_subdiv, nearPts, nullptr, rmaxN);
     const int NsS = subdiv_pts(o->self_subdiv, selfPts, &aSelf, rmaxS);
-    std::vector<double> aeq(Nt);
-    for (int t = 0; t < Nt; ++t) aeq[t] = std::sqrt((0.5 * tg[t].area2) / kPi);
+    std::vector<double> aeq(Nt), rad(Nt);
+    for (int t = 0; t < Nt; ++t) {
+        aeq[t] = std::sqrt((0.5 * tg[t].area2) / kPi);
+        const V3 cc{tg[t].cx, tg[t].cy, tg[t].cz};
+        rad[t] = std::max(norm(tv[3 * t] - cc), std::max(norm(tv[3 * t + 1] - cc), norm(tv[3 * t + 2] - cc)));
+    }
 
     // RWG sides: f = c (r - v); rwg_eval_on_triangle.m tests t == plusTri first
     std::vector<int> sideTri(2 * (size_t)Ne);
@@ -679,6 +685,7 @@ efie_status ensure_geometry(efie_ctx* c, const efie_opts* o) {
     HIPCHK(c->dSelfPts.ensure(sizeof(Pt) * selfPts.size()));
     HIPCHK(c->dASelf.ensure(sizeof(double) * aSelf.size()));
     HIPCHK(c->dAeq.ensure(sizeof(double) * Nt));
+    HIPCHK(c->dRad.ensure(sizeof(double) * Nt));
     HIPCHK(c->dSideTri.ensure(sizeof(int) * sideTri.size()));
     HIPCHK(c->dSide.ensure(sizeof(Side) * side.size()));
     HIPCHK(c->dNearMat.ensure((size_t)Nt * Nt));
@@ -689,6 +696,7 @@ efie_status ensure_geometry(efie_ctx* c, const efie_opts* o) {
     HIPCHK(hipMemcpyAsync(c->dSelfPts.p, selfPts.data(), sizeof(Pt) * selfPts.size(), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(c->dASelf.p, aSelf.data(), sizeof(double) * aSelf.size(), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(c->dAeq.p, aeq.data(), sizeof(double) * Nt, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->dRad.p, rad.data(), sizeof(double) * Nt, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(c->dSideTri.p, sideTri.data(), sizeof(int) * sideTri.size(), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(c->dSide.p, side.data(), sizeof(Side) * side.size(), hipMemcpyHostToDevice, s));
 
@@ -856,16 +864,18 @@ efie_status fill_z(efie_ctx* c, const efie_opts* o, bool useMoments, const Freq*
         pf.end(EFIE_K_POLY);
     } else {
         pf.begin(EFIE_K_SELF);
-        HIPCHK(launch_rec_self(c->dSelfPts.as<Pt>(), c->Nq2s, c->NsSelf, c->dASelf.as<double>(), Nt, dfr, recs, s));
+        HIPCHK(launch_rec_self(c->dSelfPts.as<Pt>(), c->Nq2s, c->NsSelf, c->dASelf.as<double>(), c->dRad.as<double>(),
+                               c->aSelfMax, Nt, dfr, recs, s));
         pf.end(EFIE_K_SELF);
         pf.begin(EFIE_K_NEAR);
-        HIPCHK(launch_rec_near(c->dNearPts.as<Pt>(), c->NpNear, c->dTri.as<TriGeo>(), c->dAeq.as<double>(),
+        HIPCHK(launch_rec_near(c->dNearPts.as<Pt>(), c->NpNear, c->dTri.as<TriGeo>(), c->dRad.as<double>(),
+                               c->dAeq.as<double>(),
                                o->near_alpha, c->dList.as<int2>(), (int)c->nNear, dfr, recs, s));
         pf.end(EFIE_K_NEAR);
     }
     pf.begin(EFIE_K_REGULAR);
-    HIPCHK(launch_rec_regular(c->dReg.as<Pt>(), c->Nq, c->dTri.as<TriGeo>(), Nt, c->dNearMat.as<uint8_t>(), dfr,
-                              recs, s));
+    HIPCHK(launch_rec_regular(c->dReg.as<Pt>(), c->Nq, c->dTri.as<TriGeo>(), c->dRad.as<double>(), Nt,
+                              c->dNearMat.as<uint8_t>(), dfr, recs, s));
     pf.end(EFIE_K_REGULAR);
     pf.begin(EFIE_K_COMBINE);
     HIPCHK(launch_combine(recs, c->dSideTri.as<int>(), c->dSide.as<Side>(), c->Ne, dfr, Z, ldr, ldc, s));
@@ -922,15 +932,23 @@ efie_status choose_near(efie_ctx* c, const efie_opts* o, double kmax, efie_stats
 }
 
 
-int batch_size(const efie_opts& o, size_t zElems) {
-    int B = o.batch > 0 ? o.batch : 8;
+// Frequencies factored together: latency-bound steps of the LU (pivot
+// selection, panel solves, triangular solves) cost per launch, so a batch
+// holds up to 32 matrices, limited to 60% of the free device memory and
+// split evenly over the sweep (201 frequencies -> 7 batches of 29).
+int batch_size(const efie_opts& o, size_t zElems, int Nf) {
+    int B = o.batch > 0 ? o.batch : 32;
     size_t freeB = 0, totB = 0;
     if (hipMemGetInfo(&freeB, &totB) == hipSuccess) {
         const size_t per = zElems * (o.precision == EFIE_PREC_MIXED ? 24 : 16);
-        const size_t cap = (size_t)(0.5 * (double)freeB) / std::max<size_t>(per, 1);
+        const size_t cap = (size_t)(0.6 * (double)freeB) / std::max<size_t>(per, 1);
         if (cap < (size_t)B) B = (int)std::max<size_t>(1, cap);
     }
-    return B;
+    if (Nf > 0 && o.batch <= 0) {
+        const int nbatch = (Nf + B - 1) / B;
+        B = (Nf + nbatch - 1) / nbatch;
+    }
+    return std::max(1, std::min(B, std::max(Nf, 1)));
 }
 
 // Device buffers of the batched solver for n unknowns and B matrices.
@@ -959,6 +977,22 @@ efie_status ensure_solver(efie_ctx* c, const efie_opts& o, int n, int lda, int B
     }
     HIPCHK(c->dIpiv.ensure(sizeof(int) * (size_t)B * n));
     HIPCHK(c->dInfo.ensure(sizeof(int) * B));
+    // single-launch triangular solve: flags hold the epoch of the last
+    // launch that finished each block, so they are zeroed only when new
+    const size_t flagBytes = sizeof(int) * (size_t)B * ((n + 63) / 64);
+    if (c->dFlags.bytes < flagBytes || !c->dFlags.p) {
+        HIPCHK(c->dFlags.ensure(flagBytes));
+        HIPCHK(launch_fill_int(c->dFlags.as<int>(), 0, (int)(c->dFlags.bytes / sizeof(int)), s));
+        c->tw.epoch = 0;
+    }
+    HIPCHK(c->dTickets.ensure(sizeof(int) * B));
+    if (!c->dErr.p) {
+        HIPCHK(c->dErr.ensure(sizeof(int)));
+        HIPCHK(launch_fill_int(c->dErr.as<int>(), 0, 1, s));
+    }
+    c->tw.flags = c->dFlags.as<int>();
+    c->tw.tickets = c->dTickets.as<int>();
+    c->tw.err = c->dErr.as<int>();
     return ok();
 }
 
@@ -977,7 +1011,7 @@ efie_status solve_batch(efie_ctx* c, const efie_opts& o, int n, int lda, int nb,
         HIPCHK(lu_factor<double>(Zb, 2 * zElems, n, lda, nb, wk, s, &pf));
         HIPCHK(hipEventRecord(c->ev[3], s));
         pf.begin(EFIE_K_TRSV);
-        HIPCHK(lu_backsolve<double>(Zb, 2 * zElems, n, lda, nb, s));
+        HIPCHK(lu_backsolve<double>(Zb, 2 * zElems, n, lda, nb, s, &c->tw));
         pf.end(EFIE_K_TRSV);
         *X = Zb + 2 * n;
         *xStride = 2 * zElems;
@@ -991,7 +1025,7 @@ efie_status solve_batch(efie_ctx* c, const efie_opts& o, int n, int lda, int nb,
     HIPCHK(lu_factor<float>(Z32, 2 * zElems, n, lda, nb, wk, s, &pf));
     HIPCHK(hipEventRecord(c->ev[3], s));
     pf.begin(EFIE_K_TRSV);
-    HIPCHK(lu_backsolve<float>(Z32, 2 * zElems, n, lda, nb, s));
+    HIPCHK(lu_backsolve<float>(Z32, 2 * zElems, n, lda, nb, s, &c->tw));
     pf.end(EFIE_K_TRSV);
     pf.begin(EFIE_K_OTHER);
     HIPCHK(lu_make_perm(c->dIpiv.as<int>(), n, nb, c->dPerm.as<int>(), s));
@@ -1014,9 +1048,9 @@ efie_status solve_batch(efie_ctx* c, const efie_opts& o, int n, int lda, int nb,
         pf.end(EFIE_K_OTHER);
         pf.begin(EFIE_K_TRSV);
         HIPCHK(lu_trsv<float>(Z32, 2 * zElems, n, lda, c->dR32.as<float>(), 2 * (size_t)n, 1, nb, true,
-                              c->dDone.as<int>(), s));
+                              c->dDone.as<int>(), s, &c->tw));
         HIPCHK(lu_trsv<float>(Z32, 2 * zElems, n, lda, c->dR32.as<float>(), 2 * (size_t)n, 1, nb, false,
-                              c->dDone.as<int>(), s));
+                              c->dDone.as<int>(), s, &c->tw));
         pf.end(EFIE_K_TRSV);
         pf.begin(EFIE_K_OTHER);
         HIPCHK(launch_add_f32(c->dX.as<double>(), c->dR32.as<float>(), n, nb, c->dDone.as<int>(), s));
@@ -1034,7 +1068,9 @@ efie_status finish_batch(efie_ctx* c, const efie_opts& o, int nb, efie_stats* S,
     hipStream_t s = c->stream;
     std::vector<int> info(nb), iters(nb);
     std::vector<double> resid(nb);
+    int werr = 0;
     HIPCHK(hipMemcpyAsync(info.data(), c->dInfo.p, sizeof(int) * nb, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&werr, c->dErr.p, sizeof(int), hipMemcpyDeviceToHost, s));
     const bool mixed = o.precision == EFIE_PREC_MIXED;
     if (mixed) {
         HIPCHK(hipMemcpyAsync(iters.data(), c->dIters.p, sizeof(int) * nb, hipMemcpyDeviceToHost, s));
@@ -1042,6 +1078,10 @@ efie_status finish_batch(efie_ctx* c, const efie_opts& o, int nb, efie_stats* S,
     }
     HIPCHK(hipStreamSynchronize(s));
     prof_collect(c, S);
+    if (werr) {
+        HIPCHK(launch_fill_int(c->dErr.as<int>(), 0, 1, s));
+        return fail(EFIE_E_HIP, "efie:hip", "triangular solve: a block wait exceeded its limit");
+    }
     for (int i = 0; i < nb; ++i) {
         if (info[i] != 0) {
             if (freqs)
@@ -1149,7 +1189,7 @@ efie_status efie_sweep(efie_ctx* c, const double* freqs, int Nf, double mu, doub
     const int n = c->Ne;
     const int lda = ((n + 1) + 31) & ~31;  // rhs in column n
     const size_t zElems = (size_t)n * lda;
-    const int B = std::min(batch_size(o, zElems), Nf);
+    const int B = batch_size(o, zElems, Nf);
     hipStream_t s = c->stream;
     st = ensure_solver(c, o, n, lda, B);
     if (st) return st;

@@ -17,6 +17,7 @@
 // Coordinates inside a pair are relative to each triangle's centroid so the
 // regrouping does not cancel digits.
 #include "efie_internal.h"
+#include "efie_math.h"
 
 #include <math.h>
 
@@ -25,15 +26,67 @@ namespace efie {
 static constexpr double kPi = 3.141592653589793238462643383279502884;
 
 // G = exp(-j k R) / (4 pi R) for complex k = kr + j ki
-// (rwg_pair_integrals_regular.m:22; exp(ki R)(cos(kr R) - j sin(kr R))).
-__device__ __forceinline__ void green64(double kr, double ki, double R, double& gr,
-                                        double& gi) {
+// (rwg_pair_integrals_regular.m:22; exp(ki R)(cos(kr R) - j sin(kr R))) is
+// evaluated per triangle pair by PairGreen / green_eval below.
+
+// 1/sqrt(x) for x > 0: hardware estimate refined by two Newton steps.
+__device__ __forceinline__ double rsqrt64(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+    y = y * fma(-h * y, y, 1.5);
+    y = y * fma(-h * y, y, 1.5);
+    return y;
+}
+
+// Green's function of one triangle pair: e^{-jkR} = e^{-jkR0} e^{-jk(R-R0)}
+// with R0 the centroid distance, so each point pair needs only cos/sin of
+// the small phase kr (R - R0) when its bound dmax (largest |R - R0| of the
+// pair) keeps kr dmax <= 0.75; otherwise the full reduction is used.
+struct PairGreen {
+    double R0, e0r, e0i, kr, ki;
+    bool small;
+};
+
+__device__ __forceinline__ PairGreen pair_green(double kr, double ki, double R0, double dmax) {
+    PairGreen g;
+    g.R0 = R0;
+    g.kr = kr;
+    g.ki = ki;
+    g.small = fabs(kr) * dmax <= 0.75 && fabs(ki) * dmax <= 1e-3;
     double s, c;
-    sincos(kr * R, &s, &c);
-    double amp = (ki == 0.0) ? 1.0 : exp(ki * R);
-    double d = 4.0 * kPi * R;
-    gr = (amp * c) / d;
-    gi = -(amp * s) / d;
+    sincos_cw(kr * R0, &s, &c);
+    const double amp = (ki == 0.0) ? 1.0 : exp(ki * R0);
+    g.e0r = amp * c;
+    g.e0i = -amp * s;
+    return g;
+}
+
+// G(R) for R^2 = R2 (R2 > 0).
+__device__ __forceinline__ void green_eval(const PairGreen& g, double R2, double& gr, double& gi) {
+    const double invR = rsqrt64(R2);
+    const double R = R2 * invR;
+    const double f = invR * (1.0 / (4.0 * kPi));
+    if (g.small) {
+        const double d = R - g.R0;
+        double s, c;
+        sincos_small(g.kr * d, &s, &c);
+        // e^{ki d} = 1 + ki d + (ki d)^2/2 + (ki d)^3/6 for |ki d| <= 1e-3
+        double a = 1.0;
+        if (g.ki != 0.0) {
+            const double z = g.ki * d;
+            a = 1.0 + z * (1.0 + z * (0.5 + z * (1.0 / 6.0 + z * (1.0 / 24.0))));
+        }
+        // (e0r + j e0i)(c - j s) a f
+        const double af = a * f;
+        gr = (g.e0r * c + g.e0i * s) * af;
+        gi = (g.e0i * c - g.e0r * s) * af;
+    } else {
+        double s, c;
+        sincos_cw(g.kr * R, &s, &c);
+        const double amp = (g.ki == 0.0) ? 1.0 : exp(g.ki * R);
+        gr = c * amp * f;
+        gi = -s * amp * f;
+    }
 }
 
 // Near test of triangles_are_near.m:1-12 in the reference's operation
@@ -154,7 +207,8 @@ constexpr int kRegTile = 64;
 
 template <int NQ>
 __global__ __launch_bounds__(256) void k_rec_regular(const Pt* __restrict__ pts,
-                                                     const TriGeo* __restrict__ tg, int Nt,
+                                                     const TriGeo* __restrict__ tg,
+                                                     const double* __restrict__ rad, int Nt,
                                                      const uint8_t* __restrict__ nearMat,
                                                      const Freq* __restrict__ fr,
                                                      Rec* __restrict__ recs) {
@@ -190,21 +244,22 @@ __global__ __launch_bounds__(256) void k_rec_regular(const Pt* __restrict__ pts,
         const TriGeo g2 = tg[t2];
         const Pt* p2 = pts + (size_t)t2 * NQ;
         const double ox = g1.cx - g2.cx, oy = g1.cy - g2.cy, oz = g1.cz - g2.cz;
+        const PairGreen pg = pair_green(kr, ki, sqrt(ox * ox + oy * oy + oz * oz), rad[t1] + rad[t2]);
         Acc a;
         acc_zero(a);
-#pragma unroll
+#pragma unroll 1
         for (int i = 0; i < NQ; ++i) {
             const double xi = p1[i].x + ox, yi = p1[i].y + oy, zi = p1[i].z + oz;
             double gr = 0, gi = 0, hxr = 0, hxi = 0, hyr = 0, hyi = 0, hzr = 0, hzi = 0;
-#pragma unroll
+#pragma unroll 1
             for (int j = 0; j < NQ; ++j) {
                 const Pt q2 = p2[j];
-                double dx = xi - q2.x, dy = yi - q2.y, dz = zi - q2.z;
-                double R = sqrt(dx * dx + dy * dy + dz * dz);
-                R = R < 1e-15 ? 1e-15 : R;
+                const double dx = xi - q2.x, dy = yi - q2.y, dz = zi - q2.z;
+                double R2 = dx * dx + dy * dy + dz * dz;
+                R2 = R2 < 1e-30 ? 1e-30 : R2;  // R < 1e-15 -> 1e-15
                 double Gr, Gi;
-                green64(kr, ki, R, Gr, Gi);
-                double wr = q2.w * Gr, wi = q2.w * Gi;
+                green_eval(pg, R2, Gr, Gi);
+                const double wr = q2.w * Gr, wi = q2.w * Gi;
                 gr += wr; gi += wi;
                 hxr += wr * q2.x; hxi += wi * q2.x;
                 hyr += wr * q2.y; hyi += wi * q2.y;
@@ -226,9 +281,11 @@ __global__ __launch_bounds__(256) void k_rec_regular(const Pt* __restrict__ pts,
 template <bool SELF>
 __device__ void subdiv_pair(const Pt* __restrict__ P1, const Pt* __restrict__ P2, int Np,
                             int Nq2, double ox, double oy, double oz, double aNear2,
-                            const double* __restrict__ aSelf, double kr, double ki,
+                            const double* __restrict__ aSelf, double kr, double ki, double dmax,
                             Rec* __restrict__ out) {
     const int lane = threadIdx.x & 63;
+    // the softened distance stays within dmax of sqrt(|o|^2 + a^2)
+    const PairGreen pg = pair_green(kr, ki, sqrt(ox * ox + oy * oy + oz * oz + aNear2), dmax);
     Acc a;
     acc_zero(a);
     for (int base = 0; base < Np; base += 64) {
@@ -251,17 +308,15 @@ __device__ void subdiv_pair(const Pt* __restrict__ P1, const Pt* __restrict__ P2
                 const Pt q2 = P2[j];
                 double dx = xi - q2.x, dy = yi - q2.y, dz = zi - q2.z;
                 double R2 = dx * dx + dy * dy + dz * dz;
-                double R;
                 if (!SELF) {
-                    R = sqrt(R2 + aNear2);       // near_singular.m:41-42
+                    R2 = R2 + aNear2;            // near_singular.m:41-42
                 } else if (j / Nq2 == psub) {
-                    R = sqrt(R2 + aS2);          // singular.m:46-47
+                    R2 = R2 + aS2;               // singular.m:46-47
                 } else {
-                    R = sqrt(R2);
-                    R = R > 1e-15 ? R : 1e-15;   // singular.m:48-49
+                    R2 = R2 > 1e-30 ? R2 : 1e-30;  // max(R, 1e-15), singular.m:48-49
                 }
                 double Gr, Gi;
-                green64(kr, ki, R, Gr, Gi);
+                green_eval(pg, R2, Gr, Gi);
                 double wr = q2.w * Gr, wi = q2.w * Gi;
                 gr += wr; gi += wi;
                 hxr += wr * q2.x; hxi += wi * q2.x;
@@ -284,6 +339,7 @@ __device__ void subdiv_pair(const Pt* __restrict__ P1, const Pt* __restrict__ P2
 
 __global__ __launch_bounds__(256) void k_rec_near(const Pt* __restrict__ subPts, int Np, int Nq2,
                                                   const TriGeo* __restrict__ tg,
+                                                  const double* __restrict__ rad,
                                                   const double* __restrict__ aeq,
                                                   double near_alpha,
                                                   const int2* __restrict__ list, int nPairs,
@@ -298,19 +354,21 @@ __global__ __launch_bounds__(256) void k_rec_near(const Pt* __restrict__ subPts,
     double as = near_alpha * (a1 < a2 ? a1 : a2);
     subdiv_pair<false>(subPts + (size_t)tp.x * Np, subPts + (size_t)tp.y * Np, Np, Nq2,
                        g1.cx - g2.cx, g1.cy - g2.cy, g1.cz - g2.cz, as * as, nullptr,
-                       fr->kr, fr->ki, recs + rec_index(tp.x, tp.y));
+                       fr->kr, fr->ki, rad[tp.x] + rad[tp.y], recs + rec_index(tp.x, tp.y));
 }
 
 __global__ __launch_bounds__(256) void k_rec_self(const Pt* __restrict__ selfPts, int Nq2, int Ns,
-                                                  const double* __restrict__ aSelf, int Nt,
+                                                  const double* __restrict__ aSelf,
+                                                  const double* __restrict__ rad, double aSelfMax, int Nt,
                                                   const Freq* __restrict__ fr,
                                                   Rec* __restrict__ recs) {
     const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (t >= Nt) return;
     const int Np = Ns * Nq2;
     const Pt* P = selfPts + (size_t)t * Np;
+    // self pair: R0 = 0 and |R - R0| <= 2 rad + a
     subdiv_pair<true>(P, P, Np, Nq2, 0.0, 0.0, 0.0, 0.0, aSelf + (size_t)t * Ns, fr->kr, fr->ki,
-                      recs + rec_index(t, t));
+                      2.0 * rad[t] + aSelfMax, recs + rec_index(t, t));
 }
 
 // ------------------------------------ frequency-independent moments ---
@@ -720,35 +778,35 @@ hipError_t launch_near_list(const uint8_t* nearMat, const int* rowStart, int Nt,
     return hipGetLastError();
 }
 
-hipError_t launch_rec_regular(const Pt* regPts, int Nq, const TriGeo* tg, int Nt,
+hipError_t launch_rec_regular(const Pt* regPts, int Nq, const TriGeo* tg, const double* rad, int Nt,
                               const uint8_t* nearMat, const Freq* fr, Rec* recs, hipStream_t s) {
     long long T = (Nt + kRegTile - 1) / kRegTile;
     long long nb = T * (T + 1) / 2;
     dim3 g((unsigned)nb), blk(256);
     switch (Nq) {
-        case 1: hipLaunchKernelGGL(k_rec_regular<1>, g, blk, 0, s, regPts, tg, Nt, nearMat, fr, recs); break;
-        case 3: hipLaunchKernelGGL(k_rec_regular<3>, g, blk, 0, s, regPts, tg, Nt, nearMat, fr, recs); break;
-        case 6: hipLaunchKernelGGL(k_rec_regular<6>, g, blk, 0, s, regPts, tg, Nt, nearMat, fr, recs); break;
-        case 7: hipLaunchKernelGGL(k_rec_regular<7>, g, blk, 0, s, regPts, tg, Nt, nearMat, fr, recs); break;
+        case 1: hipLaunchKernelGGL(k_rec_regular<1>, g, blk, 0, s, regPts, tg, rad, Nt, nearMat, fr, recs); break;
+        case 3: hipLaunchKernelGGL(k_rec_regular<3>, g, blk, 0, s, regPts, tg, rad, Nt, nearMat, fr, recs); break;
+        case 6: hipLaunchKernelGGL(k_rec_regular<6>, g, blk, 0, s, regPts, tg, rad, Nt, nearMat, fr, recs); break;
+        case 7: hipLaunchKernelGGL(k_rec_regular<7>, g, blk, 0, s, regPts, tg, rad, Nt, nearMat, fr, recs); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_rec_near(const Pt* subPts, int Np, const TriGeo* tg, const double* aeq,
+hipError_t launch_rec_near(const Pt* subPts, int Np, const TriGeo* tg, const double* rad, const double* aeq,
                            double near_alpha, const int2* list, int nPairs, const Freq* fr,
                            Rec* recs, hipStream_t s) {
     if (nPairs <= 0) return hipSuccess;
     // Nq2 only matters for the self kernel; pass Np so psub is unused
-    hipLaunchKernelGGL(k_rec_near, dim3((nPairs + 3) / 4), dim3(256), 0, s, subPts, Np, Np, tg, aeq,
+    hipLaunchKernelGGL(k_rec_near, dim3((nPairs + 3) / 4), dim3(256), 0, s, subPts, Np, Np, tg, rad, aeq,
                        near_alpha, list, nPairs, fr, recs);
     return hipGetLastError();
 }
 
-hipError_t launch_rec_self(const Pt* selfPts, int Nq2, int Ns, const double* aSelf, int Nt,
-                           const Freq* fr, Rec* recs, hipStream_t s) {
-    hipLaunchKernelGGL(k_rec_self, dim3((Nt + 3) / 4), dim3(256), 0, s, selfPts, Nq2, Ns, aSelf, Nt,
-                       fr, recs);
+hipError_t launch_rec_self(const Pt* selfPts, int Nq2, int Ns, const double* aSelf, const double* rad,
+                           double aSelfMax, int Nt, const Freq* fr, Rec* recs, hipStream_t s) {
+    hipLaunchKernelGGL(k_rec_self, dim3((Nt + 3) / 4), dim3(256), 0, s, selfPts, Nq2, Ns, aSelf, rad, aSelfMax,
+                       Nt, fr, recs);
     return hipGetLastError();
 }
 

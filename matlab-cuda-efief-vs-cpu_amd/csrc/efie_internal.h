@@ -59,17 +59,20 @@ hipError_t launch_near_list(const uint8_t* nearMat, const int* rowStart, int Nt,
                             int2* list, hipStream_t s);
 
 // Regular (non-near, t1 < t2) records; nearMat excludes near pairs.
-hipError_t launch_rec_regular(const Pt* regPts, int Nq, const TriGeo* tg, int Nt,
+// rad[t]: largest distance from the centroid of t to its vertices (bounds
+// every quadrature point of t).
+hipError_t launch_rec_regular(const Pt* regPts, int Nq, const TriGeo* tg, const double* rad, int Nt,
                               const uint8_t* nearMat, const Freq* fr, Rec* recs,
                               hipStream_t s);
 // Subdivided near-pair records (rwg_pair_integrals_near_singular.m).
-hipError_t launch_rec_near(const Pt* subPts, int Np, const TriGeo* tg, const double* aeq,
-                           double near_alpha, const int2* list, int nPairs,
+hipError_t launch_rec_near(const Pt* subPts, int Np, const TriGeo* tg, const double* rad,
+                           const double* aeq, double near_alpha, const int2* list, int nPairs,
                            const Freq* fr, Rec* recs, hipStream_t s);
 // Self records (rwg_pair_integrals_singular.m): Np points per triangle in
 // Ns sub-triangles of Nq2 points; aSelf[t*Ns + p] is the softening length.
 hipError_t launch_rec_self(const Pt* selfPts, int Nq2, int Ns, const double* aSelf,
-                           int Nt, const Freq* fr, Rec* recs, hipStream_t s);
+                           const double* rad, double aSelfMax, int Nt, const Freq* fr, Rec* recs,
+                           hipStream_t s);
 
 // Frequency-independent moments Q[pair][q][8], q = 0..P, of the near pairs
 // (Qn) and the self pairs (Qs), and the per-frequency records built from
@@ -121,15 +124,25 @@ int lu_cand_stride();
 template <typename T>
 hipError_t lu_factor(T* A, size_t strideA, int n, int lda, int batch, LuWork w,
                      hipStream_t s, const struct Prof* prof);
-// Back substitution U x = y in place on column n of A.
+// Work buffers of the single-launch triangular solve: flags [batch][nblk]
+// (zeroed once), tickets [batch], err [1]; epoch grows by one per launch.
+struct TrsvWork {
+    int* flags;
+    int* tickets;
+    int* err;
+    int epoch;
+};
+
+// Back substitution U x = y in place on column n of A (tw may be null: one
+// launch pair per 64-row block instead of the single-launch solve).
 template <typename T>
-hipError_t lu_backsolve(T* A, size_t strideA, int n, int lda, int batch, hipStream_t s);
+hipError_t lu_backsolve(T* A, size_t strideA, int n, int lda, int batch, hipStream_t s, TrsvWork* tw);
 // Triangular solve with the factors in A on a vector x (element e of batch
 // b at X[b*strideX + 2*e*xs]): unit lower L (lower) or upper U; batch
 // elements with skip[b] != 0 (skip may be null) are not touched.
 template <typename T>
 hipError_t lu_trsv(const T* A, size_t strideA, int n, int lda, T* X, size_t strideX, size_t xs,
-                   int batch, bool lower, const int* skip, hipStream_t s);
+                   int batch, bool lower, const int* skip, hipStream_t s, TrsvWork* tw);
 // Row permutation equivalent to the sequential interchanges of ipiv.
 hipError_t lu_make_perm(const int* ipiv, int n, int batch, int* perm, hipStream_t s);
 

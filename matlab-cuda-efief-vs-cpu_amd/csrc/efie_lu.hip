@@ -30,7 +30,7 @@
 namespace efie {
 
 constexpr int NB = 32;    // panel width
-constexpr int BR = 128;   // leaf rows of the tournament
+
 
 template <typename T>
 struct cpx {
@@ -64,78 +64,19 @@ __device__ __forceinline__ cpx<T> cinv(cpx<T> a) {
     return {a.r / d, -a.i / d};
 }
 
-// ------------------------------------------------------ GEPP in LDS ------
-// Partial pivoting on an r x w block (row stride w) held in LDS, rows
-// labelled by lab[].  Afterwards lab[0..min(r,w)) are the pivot rows in
-// order.  Returns (block-uniformly) the first column with an exactly zero
-// pivot, or -1.  Requires blockDim.x == 256.
-template <typename T>
-__device__ int gepp_lds(cpx<T>* a, int* lab, int r, int w, T* redv, int* redi) {
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wv = tid >> 6;
-    int zero_col = -1;
-    const int steps = r < w ? r : w;
-    for (int c = 0; c < steps; ++c) {
-        // pivot search: max |re|+|im|, first index on ties
-        T best = (T)-1;
-        int bi = 0x7fffffff;
-        for (int i = c + tid; i < r; i += 256) {
-            cpx<T> v = a[i * w + c];
-            T m = fabs(v.r) + fabs(v.i);
-            if (m > best) { best = m; bi = i; }
-        }
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            T ob = __shfl_xor(best, o, 64);
-            int oi = __shfl_xor(bi, o, 64);
-            if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
-        }
-        if (lane == 0) { redv[wv] = best; redi[wv] = bi; }
-        __syncthreads();
-        if (tid == 0) {
-            T b = redv[0];
-            int ii = redi[0];
-            for (int k = 1; k < 4; ++k)
-                if (redv[k] > b || (redv[k] == b && redi[k] < ii)) { b = redv[k]; ii = redi[k]; }
-            redv[4] = b;
-            redi[4] = ii;
-        }
-        __syncthreads();
-        const int p = redi[4];
-        const bool zp = !(redv[4] > (T)0);
-        // swap rows c and p
-        if (p != c) {
-            for (int k = tid; k < w; k += 256) {
-                cpx<T> t = a[c * w + k];
-                a[c * w + k] = a[p * w + k];
-                a[p * w + k] = t;
-            }
-            if (tid == 0) { int t = lab[c]; lab[c] = lab[p]; lab[p] = t; }
-        }
-        __syncthreads();
-        if (zp) {
-            if (zero_col < 0) zero_col = c;
-            continue;
-        }
-        const cpx<T> inv = cinv(a[c * w + c]);
-        // scale column c, then rank-1 update of the trailing block
-        for (int i = c + 1 + tid; i < r; i += 256) a[i * w + c] = cmul(a[i * w + c], inv);
-        __syncthreads();
-        const int rr = r - c - 1, cc = w - c - 1;
-        for (int e = tid; e < rr * cc; e += 256) {
-            int i = c + 1 + e / cc, k = c + 1 + e % cc;
-            cpx<T> l = a[i * w + c], u = a[c * w + k];
-            cpx<T> v = a[i * w + k];
-            v.r -= l.r * u.r - l.i * u.i;
-            v.i -= l.r * u.i + l.i * u.r;
-            a[i * w + k] = v;
-        }
-        __syncthreads();
-    }
-    return zero_col;
+__global__ void k_zero_int(int* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
 }
 
 // ------------------------------------------------------ tournament -------
+// Tree of candidate sets over the rows of the panel.  Every node is one
+// wave: lane l holds one panel row (w <= NB complex values in registers)
+// and the node runs partial pivoting on its rows (pivot = largest
+// |re|+|im|, first position on ties, as LAPACK's izamax), keeping the
+// pivot rows in pivot order as its candidates.  Leaves hold TL = 64 rows.
+
+constexpr int TL = 64;
 
 struct TreeGeom {
     int nLevels;
@@ -146,7 +87,7 @@ struct TreeGeom {
 
 static TreeGeom tree_geom(int M) {
     TreeGeom g{};
-    int L = (M + BR - 1) / BR;
+    int L = (M + TL - 1) / TL;
     int base = 0, lv = 0;
     while (true) {
         g.levelBase[lv] = base;
@@ -161,111 +102,218 @@ static TreeGeom tree_geom(int M) {
     return g;
 }
 
+// Cross-workgroup exchange of small data: agent-scope relaxed atomics read
+// and write the coherent copy directly, and the writer's release fence
+// orders them before the flag or counter that publishes them.  No
+// agent-scope acquire fence is used on the reading side: on a multi-die
+// device that fence invalidates the whole L2 of the reading die.
 __device__ __forceinline__ int ald(const int* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void ast(int* p, int v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+template <typename T>
+__device__ __forceinline__ cpx<T> ald_c(const T* p) {
+    cpx<T> v;
+    v.r = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    v.i = __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return v;
+}
+template <typename T>
+__device__ __forceinline__ void ast_c(T* p, cpx<T> v) {
+    __hip_atomic_store(p, v.r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p + 1, v.i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// orders this thread's earlier memory operations before later ones
+// (release to the agent; no cache invalidation)
+__device__ __forceinline__ void release_agent() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
+// compiler/wave ordering after an observed flag (the data is read atomically)
+__device__ __forceinline__ void acquire_wg() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup"); }
+
+// Largest 64-bit key of the wave, returned wave-uniformly: DPP steps
+// quad_perm [1,0,3,2] and [2,3,0,1] (groups of 4), row_half_mirror (8),
+// row_mirror (16), then row_bcast:15 and row_bcast:31 leave the maximum in
+// lane 63, which is read back with readlane.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long x) {
+    const int lo = (int)(unsigned)x, hi = (int)(unsigned)(x >> 32);
+    const int l2 = __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROWMASK, 0xF, false);
+    const int h2 = __builtin_amdgcn_update_dpp(hi, hi, CTRL, ROWMASK, 0xF, false);
+    return ((unsigned long long)(unsigned)h2 << 32) | (unsigned)l2;
+}
+__device__ __forceinline__ unsigned long long umax64(unsigned long long a, unsigned long long b) {
+    return a > b ? a : b;
+}
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long k) {
+    k = umax64(k, dpp_u64<0xB1, 0xF>(k));
+    k = umax64(k, dpp_u64<0x4E, 0xF>(k));
+    k = umax64(k, dpp_u64<0x141, 0xF>(k));
+    k = umax64(k, dpp_u64<0x140, 0xF>(k));
+    k = umax64(k, dpp_u64<0x142, 0xA>(k));
+    k = umax64(k, dpp_u64<0x143, 0xC>(k));
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)k, 63);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(k >> 32), 63);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+template <typename T>
+__device__ __forceinline__ T readlane_t(T v, int lane) {
+    if constexpr (std::is_same<T, double>::value) {
+        unsigned long long u = __double_as_longlong(v);
+        const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, lane);
+        const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), lane);
+        return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+    } else {
+        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+    }
+}
+
+// Wave-level partial pivoting of up to 64 rows (one per lane, label myLab,
+// -1 for none) of the w columns j0.. of A.  Returns in `chosen` (lane i)
+// the label of the i-th pivot row; zc = first column with a zero pivot.
+// The pivot of a column is the row of largest |re|+|im| (compared in
+// single precision), the first position among equals.
+template <typename T>
+__device__ void gepp_wave(const T* Ab, int lda, int j0, int w, int r, int myLab, int& chosen, int& zc) {
+    const int lane = threadIdx.x;
+    cpx<T> a[NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        a[k].r = (T)0;
+        a[k].i = (T)0;
+        if (k < w && myLab >= 0) a[k] = ld(Ab + 2 * ((size_t)myLab * lda + j0 + k));
+    }
+    int pos = lane;
+    bool act = lane < r;
+    chosen = -1;
+    zc = -1;
+    const int steps = r < w ? r : w;
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+        if (c < steps) {
+            // key: magnitude bits (order-preserving for values >= 0) over ~position
+            const float mag = (float)(fabs(a[c].r) + fabs(a[c].i));
+            const unsigned long long key =
+                act ? (((unsigned long long)__float_as_uint(mag) << 32) | (unsigned)(0xFFFFFFFFu - (unsigned)pos))
+                    : 0ull;
+            const unsigned long long best = wave_max_u64(key);
+            const int bpos = (int)(0xFFFFFFFFu - (unsigned)best);
+            const bool nonzero = (best >> 32) != 0;
+            const unsigned long long m = __ballot(act && pos == bpos);
+            const int p = __ffsll((long long)m) - 1;
+            const int labp = __builtin_amdgcn_readlane(myLab, p);
+            if (pos == c) pos = bpos;  // the row at position c takes the pivot row's place
+            if (lane == p) {
+                pos = c;
+                act = false;
+            }
+            if (lane == c) chosen = labp;
+            if (!nonzero) {
+                if (zc < 0) zc = c;
+            } else {
+                const cpx<T> d = {readlane_t(a[c].r, p), readlane_t(a[c].i, p)};
+                const cpx<T> inv = cinv(d);
+                // rows already chosen (and the pivot row) get multiplier 0
+                cpx<T> l = cmul(a[c], inv);
+                if (!act) l = {(T)0, (T)0};
+                else a[c] = l;
+#pragma unroll
+                for (int k = c + 1; k < NB; ++k) {
+                    if (k < w) {
+                        const cpx<T> u = {readlane_t(a[k].r, p), readlane_t(a[k].i, p)};
+                        a[k].r -= l.r * u.r - l.i * u.i;
+                        a[k].i -= l.r * u.i + l.i * u.r;
+                    }
+                }
+            }
+        }
+    }
+}
 
 // cand layout per batch: [nNodes][NB + 1] (count followed by labels)
 template <typename T>
-__global__ __launch_bounds__(256) void k_tslu(T* A, size_t strideA, int n, int lda, int j0, int w,
-                                              TreeGeom g, int nodeStride, int* cand, int* counters,
-                                              int* ipiv, int* info) {
-    __shared__ cpx<T> a[(BR > 2 * NB ? BR : 2 * NB) * NB];
-    __shared__ int lab[(BR > 2 * NB ? BR : 2 * NB)];
-    __shared__ T redv[8];
-    __shared__ int redi[8];
+__global__ __launch_bounds__(64) void k_tslu(T* A, size_t strideA, int n, int lda, int j0, int w,
+                                             TreeGeom g, int nodeStride, int* cand, int* counters,
+                                             int* ipiv, int* info) {
     __shared__ int s_go;
     const int b = blockIdx.y;
-    T* Ab = A + (size_t)b * strideA;
+    const T* Ab = A + (size_t)b * strideA;
     int* cb = cand + (size_t)b * nodeStride * (NB + 1);
     int* ctr = counters + (size_t)b * nodeStride;
-    const int tid = threadIdx.x;
+    const int lane = threadIdx.x;
     const int M = n - j0;
 
     int level = 0, node = blockIdx.x;
-    // leaf rows
-    int r = min(BR, M - node * BR);
-    for (int i = tid; i < r; i += 256) lab[i] = j0 + node * BR + i;
-    __syncthreads();
+    int r = min(TL, M - node * TL);
+    int myLab = lane < r ? j0 + node * TL + lane : -1;
     while (true) {
-        // load the labelled rows of the panel into LDS
-        for (int e = tid; e < r * w; e += 256) {
-            int i = e / w, k = e % w;
-            a[i * w + k] = ld(Ab + 2 * ((size_t)lab[i] * lda + j0 + k));
-        }
-        __syncthreads();
-        int zc = gepp_lds<T>(a, lab, r, w, redv, redi);
+        int chosen, zc;
+        gepp_wave<T>(Ab, lda, j0, w, r, myLab, chosen, zc);
         const int keep = r < w ? r : w;
-        const bool root = (g.levelCount[level] == 1);
-        if (root) {
-            if (tid == 0) {
-                if (zc >= 0 && info[b] == 0) info[b] = j0 + zc + 1;
-                // simulate the sequential swaps: ipiv[j0+i] = current position
-                // of the i-th selected row
-                int rowAt[2 * NB], posOf[2 * NB], nt = 0;
-                // rows touched: positions j0..j0+keep-1 and the pivot rows
-                for (int i = 0; i < keep; ++i) {
-                    int target = lab[i];
-                    int q = target;  // current position of target
-                    for (int k = 0; k < nt; ++k)
-                        if (rowAt[k] == target) { q = posOf[k]; break; }
-                    int pos = j0 + i;
-                    ipiv[(size_t)b * n + pos] = q + 1;
-                    // row currently at pos
-                    int cur = pos;
-                    bool found = false;
-                    for (int k = 0; k < nt; ++k)
-                        if (posOf[k] == pos) { cur = rowAt[k]; found = true; break; }
-                    (void)found;
-                    // move cur to q, target to pos
-                    bool fc = false, ft = false;
-                    for (int k = 0; k < nt; ++k) {
-                        if (rowAt[k] == cur && !fc) { posOf[k] = q; fc = true; }
-                        else if (rowAt[k] == target && !ft) { posOf[k] = pos; ft = true; }
-                    }
-                    if (!fc) { rowAt[nt] = cur; posOf[nt] = q; ++nt; }
-                    if (!ft && target != cur) { rowAt[nt] = target; posOf[nt] = pos; ++nt; }
+        if (g.levelCount[level] == 1) {
+            // Root: turn the chosen rows (lane i holds the i-th) into LAPACK
+            // interchanges ipiv[j0+i] = current position of the i-th row.
+            // Lane k < w tracks the row at position j0+k; lane k < nd tracks
+            // the k-th row pushed out of the top block and its position.
+            if (lane == 0 && zc >= 0 && info[b] == 0) info[b] = j0 + zc + 1;
+            int topRow = j0 + lane;
+            int dRow = -1, dPos = -1, nd = 0;
+            for (int i = 0; i < keep; ++i) {
+                const int t = __shfl(chosen, i, 64);
+                const unsigned long long mTop = __ballot(lane < keep && lane >= i && topRow == t);
+                const unsigned long long mDisp = __ballot(lane < nd && dRow == t);
+                int q = t, kd = -1;
+                if (mTop) q = j0 + (__ffsll((long long)mTop) - 1);
+                else if (mDisp) {
+                    kd = __ffsll((long long)mDisp) - 1;
+                    q = __shfl(dPos, kd, 64);
                 }
+                if (lane == 0) ipiv[(size_t)b * n + j0 + i] = q + 1;
+                const int cur = __shfl(topRow, i, 64);  // row now at position j0+i
+                if (q - j0 < keep) {
+                    if (lane == q - j0) topRow = cur;
+                } else if (kd >= 0) {
+                    if (lane == kd) dRow = cur;          // position q now holds cur
+                } else {
+                    if (lane == nd) { dRow = cur; dPos = q; }
+                    ++nd;
+                }
+                if (lane == i) topRow = t;
             }
             return;
         }
         // publish this node's candidates
         int* my = cb + (size_t)(g.levelBase[level] + node) * (NB + 1);
-        for (int i = tid; i < keep; i += 256) ast(my + 1 + i, lab[i]);
-        if (tid == 0) ast(my, keep);
-        __threadfence();
+        if (lane < keep) ast(my + 1 + lane, chosen);
+        if (lane == 0) ast(my, keep);
+        release_agent();
         __syncthreads();
         const int parent = node >> 1;
-        const int sib = node ^ 1;
-        const bool hasSib = sib < g.levelCount[level];
-        if (tid == 0) {
+        const bool hasSib = (node ^ 1) < g.levelCount[level];
+        if (lane == 0) {
             int go = 1;
             if (hasSib) {
                 int* c = ctr + g.levelBase[level + 1] + parent;
-                int old = atomicAdd(c, 1);
+                const int old = atomicAdd(c, 1);
                 if (old == 0) go = 0;
-                else ast(c, 0);  // last arrival: reset for the next panel
+                else ast(c, 0);  // last arrival resets the counter for the next panel
             }
             s_go = go;
         }
         __syncthreads();
         if (!s_go) return;
-        __threadfence();
-        // gather children candidates, left child first
-        int r2 = 0;
-        for (int ch = 0; ch < 2; ++ch) {
-            int cn = 2 * parent + ch;
-            if (cn >= g.levelCount[level]) break;
-            const int* cp = cb + (size_t)(g.levelBase[level] + cn) * (NB + 1);
-            int cnt = ald(cp);
-            for (int i = tid; i < cnt; i += 256) lab[r2 + i] = ald(cp + 1 + i);
-            r2 += cnt;
-        }
-        __syncthreads();
-        r = r2;
+        acquire_wg();
+        // candidates of both children, left child first, one per lane
+        const int* c0 = cb + (size_t)(g.levelBase[level] + 2 * parent) * (NB + 1);
+        const int n0 = ald(c0);
+        int n1 = 0;
+        const int* c1 = c0 + (NB + 1);
+        if (2 * parent + 1 < g.levelCount[level]) n1 = ald(c1);
+        myLab = -1;
+        if (lane < n0) myLab = ald(c0 + 1 + lane);
+        else if (lane < n0 + n1) myLab = ald(c1 + 1 + (lane - n0));
+        r = n0 + n1;
         level += 1;
         node = parent;
     }
@@ -294,95 +342,118 @@ __global__ __launch_bounds__(256) void k_laswp(T* A, size_t strideA, int n, int 
 // k_panel_diag: A11 (w x w, after the interchanges) = L11 U11 in place,
 // without pivoting (the tournament already ordered the pivot rows).
 template <typename T>
-__global__ __launch_bounds__(256) void k_panel_diag(T* A, size_t strideA, int lda, int j0, int w, int* info) {
-    __shared__ cpx<T> a11[NB * NB];
+__global__ __launch_bounds__(64) void k_panel_diag(T* A, size_t strideA, int lda, int j0, int w, int* info) {
+    // one wave, lane k holds row k of the block; row c is broadcast through LDS
+    __shared__ cpx<T> prow[NB];
     const int b = blockIdx.x;
     T* Ab = A + (size_t)b * strideA;
-    const int tid = threadIdx.x;
-    for (int e = tid; e < w * w; e += 256) {
-        int i = e / w, k = e % w;
-        a11[i * NB + k] = ld(Ab + 2 * ((size_t)(j0 + i) * lda + j0 + k));
+    const int lane = threadIdx.x;
+    cpx<T> a[NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        a[k].r = (T)0;
+        a[k].i = (T)0;
+        if (k < w && lane < w) a[k] = ld(Ab + 2 * ((size_t)(j0 + lane) * lda + j0 + k));
     }
-    __syncthreads();
-    for (int c = 0; c < w; ++c) {
-        const cpx<T> d = a11[c * NB + c];
-        const bool zp = (d.r == (T)0 && d.i == (T)0);
-        if (!zp) {
-            const cpx<T> inv = cinv(d);
-            for (int i = c + 1 + tid; i < w; i += 256) a11[i * NB + c] = cmul(a11[i * NB + c], inv);
-        } else if (tid == 0) {
-            atomicCAS(info + b, 0, j0 + c + 1);
-        }
-        __syncthreads();
-        if (!zp) {
-            const int rr = w - c - 1;
-            for (int e = tid; e < rr * rr; e += 256) {
-                int i = c + 1 + e / rr, k = c + 1 + e % rr;
-                cpx<T> l = a11[i * NB + c], u = a11[c * NB + k], v = a11[i * NB + k];
-                v.r -= l.r * u.r - l.i * u.i;
-                v.i -= l.r * u.i + l.i * u.r;
-                a11[i * NB + k] = v;
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+        if (c < w) {
+            if (lane == c) {
+#pragma unroll
+                for (int k = c; k < NB; ++k)
+                    if (k < w) prow[k] = a[k];
             }
+            __syncthreads();
+            const cpx<T> d = prow[c];
+            const bool zp = (d.r == (T)0 && d.i == (T)0);
+            if (zp) {
+                if (lane == 0) atomicCAS(info + b, 0, j0 + c + 1);
+            } else if (lane > c && lane < w) {
+                const cpx<T> l = cmul(a[c], cinv(d));
+                a[c] = l;
+#pragma unroll
+                for (int k = c + 1; k < NB; ++k) {
+                    if (k < w) {
+                        const cpx<T> u = prow[k];
+                        a[k].r -= l.r * u.r - l.i * u.i;
+                        a[k].i -= l.r * u.i + l.i * u.r;
+                    }
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
-    for (int e = tid; e < w * w; e += 256) {
-        int i = e / w, k = e % w;
-        st(Ab + 2 * ((size_t)(j0 + i) * lda + j0 + k), a11[i * NB + k]);
+    if (lane < w) {
+#pragma unroll
+        for (int k = 0; k < NB; ++k)
+            if (k < w) st(Ab + 2 * ((size_t)(j0 + lane) * lda + j0 + k), a[k]);
     }
 }
 
-// k_panel: with L11 U11 in place, blocks [0, nRowBlocks) solve 256 rows of
-// L21 = A21 U11^-1 each and the others 256 columns of U12 = L11^-1 A12
-// (columns up to and including the rhs column n).
+// With L11 U11 of the w x w diagonal block at (d, d) in place:
+// k_panel_rows solves rows [rlo, rhi) of L21 = A21 U11^-1 (columns d..d+w),
+// k_panel_cols solves columns [clo, chi) of U12 = L11^-1 A12 (rows d..d+w).
 template <typename T>
-__global__ __launch_bounds__(256) void k_panel(T* A, size_t strideA, int n, int lda, int j0, int w,
-                                               int nRowBlocks) {
+__global__ __launch_bounds__(256) void k_panel_rows(T* A, size_t strideA, int lda, int d, int w, int rlo,
+                                                    int rhi) {
     __shared__ cpx<T> a11[NB * NB];
     const int b = blockIdx.y;
     T* Ab = A + (size_t)b * strideA;
     const int tid = threadIdx.x;
     for (int e = tid; e < w * w; e += 256) {
         int i = e / w, k = e % w;
-        a11[i * NB + k] = ld(Ab + 2 * ((size_t)(j0 + i) * lda + j0 + k));
+        a11[i * NB + k] = ld(Ab + 2 * ((size_t)(d + i) * lda + d + k));
     }
     __syncthreads();
-    if ((int)blockIdx.x < nRowBlocks) {
-        // rows of L21: x U11 = a
-        int row = j0 + w + blockIdx.x * 256 + tid;
-        if (row >= n) return;
-        T* p = Ab + 2 * ((size_t)row * lda + j0);
-        cpx<T> x[NB];
+    const int row = rlo + blockIdx.x * 256 + tid;
+    if (row >= rhi) return;
+    T* p = Ab + 2 * ((size_t)row * lda + d);
+    cpx<T> x[NB];
 #pragma unroll
-        for (int c = 0; c < NB; ++c)
-            if (c < w) x[c] = ld(p + 2 * c);
+    for (int c = 0; c < NB; ++c)
+        if (c < w) x[c] = ld(p + 2 * c);
 #pragma unroll
-        for (int c = 0; c < NB; ++c) {
-            if (c >= w) break;
+    for (int c = 0; c < NB; ++c) {
+        if (c < w) {
             cpx<T> s = x[c];
+#pragma unroll
             for (int k = 0; k < c; ++k) {
                 cpx<T> u = a11[k * NB + c];
                 s.r -= x[k].r * u.r - x[k].i * u.i;
                 s.i -= x[k].r * u.i + x[k].i * u.r;
             }
-            cpx<T> d = a11[c * NB + c];
-            x[c] = (d.r == (T)0 && d.i == (T)0) ? s : cmul(s, cinv(d));
+            cpx<T> dd = a11[c * NB + c];
+            x[c] = (dd.r == (T)0 && dd.i == (T)0) ? s : cmul(s, cinv(dd));
         }
+    }
 #pragma unroll
-        for (int c = 0; c < NB; ++c)
-            if (c < w) st(p + 2 * c, x[c]);
-    } else {
-        // columns of U12: L11 y = a, unit diagonal
-        int col = j0 + w + (blockIdx.x - nRowBlocks) * 256 + tid;
-        if (col > n) return;
-        cpx<T> y[NB];
+    for (int c = 0; c < NB; ++c)
+        if (c < w) st(p + 2 * c, x[c]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_panel_cols(T* A, size_t strideA, int lda, int d, int w, int clo,
+                                                    int chi) {
+    __shared__ cpx<T> a11[NB * NB];
+    const int b = blockIdx.y;
+    T* Ab = A + (size_t)b * strideA;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < w * w; e += 256) {
+        int i = e / w, k = e % w;
+        a11[i * NB + k] = ld(Ab + 2 * ((size_t)(d + i) * lda + d + k));
+    }
+    __syncthreads();
+    const int col = clo + blockIdx.x * 256 + tid;
+    if (col >= chi) return;
+    cpx<T> y[NB];
 #pragma unroll
-        for (int c = 0; c < NB; ++c)
-            if (c < w) y[c] = ld(Ab + 2 * ((size_t)(j0 + c) * lda + col));
+    for (int c = 0; c < NB; ++c)
+        if (c < w) y[c] = ld(Ab + 2 * ((size_t)(d + c) * lda + col));
 #pragma unroll
-        for (int c = 0; c < NB; ++c) {
-            if (c >= w) break;
+    for (int c = 0; c < NB; ++c) {
+        if (c < w) {
             cpx<T> s = y[c];
+#pragma unroll
             for (int k = 0; k < c; ++k) {
                 cpx<T> l = a11[c * NB + k];
                 s.r -= l.r * y[k].r - l.i * y[k].i;
@@ -390,10 +461,10 @@ __global__ __launch_bounds__(256) void k_panel(T* A, size_t strideA, int n, int 
             }
             y[c] = s;
         }
-#pragma unroll
-        for (int c = 0; c < NB; ++c)
-            if (c < w) st(Ab + 2 * ((size_t)(j0 + c) * lda + col), y[c]);
     }
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+        if (c < w) st(Ab + 2 * ((size_t)(d + c) * lda + col), y[c]);
 }
 
 // -------------------------------------------------- trailing update ------
@@ -424,33 +495,21 @@ struct mf<float> {
     static __device__ __forceinline__ int row(int lk, int v) { return 4 * lk + v; }
 };
 
+// C[r0:r1, c0:c1) -= A[r0:r1, k0:k0+K) * A[k0:k0+K, c0:c1) for every batch
+// matrix; K is consumed in chunks of KC staged in LDS.
+constexpr int KC = 32;
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_gemm_update(T* A, size_t strideA, int n, int lda, int j0,
-                                                     int w) {
+__global__ __launch_bounds__(256) void k_gemm(T* A, size_t strideA, int lda, int r0, int r1, int c0,
+                                              int c1, int k0, int K) {
     typedef typename mf<T>::v4 v4;
-    __shared__ cpx<T> Ls[64][NB + 1];
-    __shared__ cpx<T> Us[NB][64 + 1];
+    __shared__ cpx<T> Ls[64][KC + 1];
+    __shared__ cpx<T> Us[KC][64 + 1];
     const int b = blockIdx.z;
     T* Ab = A + (size_t)b * strideA;
-    const int r0 = j0 + w + blockIdx.y * 64;
-    const int c0 = j0 + w + blockIdx.x * 64;
+    const int rb = r0 + blockIdx.y * 64;
+    const int cb = c0 + blockIdx.x * 64;
     const int tid = threadIdx.x;
-    // stage L (64 x w) and U (w x 64), zero padded
-    for (int e = tid; e < 64 * NB; e += 256) {
-        int i = e / NB, k = e % NB;
-        int row = r0 + i;
-        cpx<T> v = {0, 0};
-        if (row < n && k < w) v = ld(Ab + 2 * ((size_t)row * lda + j0 + k));
-        Ls[i][k] = v;
-    }
-    for (int e = tid; e < NB * 64; e += 256) {
-        int k = e / 64, j = e % 64;
-        int col = c0 + j;
-        cpx<T> v = {0, 0};
-        if (col <= n && k < w) v = ld(Ab + 2 * ((size_t)(j0 + k) * lda + col));
-        Us[k][j] = v;
-    }
-    __syncthreads();
     const int lane = tid & 63, wv = tid >> 6;
     const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
     const int li = lane & 15, lk = lane >> 4;
@@ -462,22 +521,41 @@ __global__ __launch_bounds__(256) void k_gemm_update(T* A, size_t strideA, int n
             accR[x][y] = v4{0, 0, 0, 0};
             accI[x][y] = v4{0, 0, 0, 0};
         }
-    const int kmax = (w + 3) & ~3;
-    for (int k0 = 0; k0 < kmax; k0 += 4) {
-        cpx<T> av[2], bv[2];
+    for (int kc = 0; kc < K; kc += KC) {
+        const int kw = K - kc < KC ? K - kc : KC;
+        __syncthreads();
+        for (int e = tid; e < 64 * KC; e += 256) {
+            const int i = e / KC, k = e % KC;
+            const int row = rb + i;
+            cpx<T> v = {0, 0};
+            if (row < r1 && k < kw) v = ld(Ab + 2 * ((size_t)row * lda + k0 + kc + k));
+            Ls[i][k] = v;
+        }
+        for (int e = tid; e < KC * 64; e += 256) {
+            const int k = e / 64, j = e % 64;
+            const int col = cb + j;
+            cpx<T> v = {0, 0};
+            if (col < c1 && k < kw) v = ld(Ab + 2 * ((size_t)(k0 + kc + k) * lda + col));
+            Us[k][j] = v;
+        }
+        __syncthreads();
+        const int kmax = (kw + 3) & ~3;
+        for (int kk = 0; kk < kmax; kk += 4) {
+            cpx<T> av[2], bv[2];
 #pragma unroll
-        for (int x = 0; x < 2; ++x) av[x] = Ls[wr + 16 * x + li][k0 + lk];
+            for (int x = 0; x < 2; ++x) av[x] = Ls[wr + 16 * x + li][kk + lk];
 #pragma unroll
-        for (int y = 0; y < 2; ++y) bv[y] = Us[k0 + lk][wc + 16 * y + li];
+            for (int y = 0; y < 2; ++y) bv[y] = Us[kk + lk][wc + 16 * y + li];
 #pragma unroll
-        for (int x = 0; x < 2; ++x)
+            for (int x = 0; x < 2; ++x)
 #pragma unroll
-            for (int y = 0; y < 2; ++y) {
-                accR[x][y] = mf<T>::mfma(av[x].r, bv[y].r, accR[x][y]);
-                accR[x][y] = mf<T>::mfma(-av[x].i, bv[y].i, accR[x][y]);
-                accI[x][y] = mf<T>::mfma(av[x].r, bv[y].i, accI[x][y]);
-                accI[x][y] = mf<T>::mfma(av[x].i, bv[y].r, accI[x][y]);
-            }
+                for (int y = 0; y < 2; ++y) {
+                    accR[x][y] = mf<T>::mfma(av[x].r, bv[y].r, accR[x][y]);
+                    accR[x][y] = mf<T>::mfma(-av[x].i, bv[y].i, accR[x][y]);
+                    accI[x][y] = mf<T>::mfma(av[x].r, bv[y].i, accI[x][y]);
+                    accI[x][y] = mf<T>::mfma(av[x].i, bv[y].r, accI[x][y]);
+                }
+        }
     }
 #pragma unroll
     for (int x = 0; x < 2; ++x)
@@ -485,9 +563,9 @@ __global__ __launch_bounds__(256) void k_gemm_update(T* A, size_t strideA, int n
         for (int y = 0; y < 2; ++y)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                int row = r0 + wr + 16 * x + mf<T>::row(lk, v);
-                int col = c0 + wc + 16 * y + li;
-                if (row < n && col <= n) {
+                const int row = rb + wr + 16 * x + mf<T>::row(lk, v);
+                const int col = cb + wc + 16 * y + li;
+                if (row < r1 && col < c1) {
                     T* p = Ab + 2 * ((size_t)row * lda + col);
                     cpx<T> c = ld(p);
                     c.r -= accR[x][y][v];
@@ -544,6 +622,114 @@ __global__ __launch_bounds__(256) void k_trsv_diag(const T* A, size_t strideA, i
     }
     __syncthreads();
     for (int i = tid; i < bs; i += 256) st(Xb + 2 * (size_t)(r0 + i) * xs, x[i]);
+}
+
+// Single-launch blocked triangular solve.  Workgroups take tickets in
+// dispatch order; ticket v solves block v (lower, top first) or block
+// nblk-1-v (upper, bottom first).  A workgroup subtracts the contribution of
+// each earlier block as soon as that block's flag carries this launch's
+// epoch, then solves its diagonal block and publishes its x with a flag.
+// It only waits on workgroups with smaller tickets, which were dispatched
+// before it, so the waits always end; a bounded spin reports an error
+// instead of waiting forever.
+constexpr long long kSpinLimit = 1LL << 28;
+
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(256) void k_trsv_fused(const T* A, size_t strideA, int lda, int n, T* X,
+                                                    size_t strideX, size_t xs, int* flags, int* tickets,
+                                                    int epoch, const int* skip, int* err) {
+    __shared__ cpx<T> u[64][65];
+    __shared__ cpx<T> xb[64];
+    __shared__ cpx<T> xc[64];
+    __shared__ cpx<T> part[4][64];
+    __shared__ int s_vid;
+    const int b = blockIdx.y;
+    if (skip && skip[b]) return;
+    const int tid = threadIdx.x;
+    const int nblk = (n + 63) / 64;
+    if (tid == 0) s_vid = atomicAdd(tickets + b, 1);
+    __syncthreads();
+    const int vid = s_vid;
+    const int bk = LOWER ? vid : nblk - 1 - vid;
+    const int r0 = bk * 64;
+    const int bs = n - r0 < 64 ? n - r0 : 64;
+    const T* Ab = A + (size_t)b * strideA;
+    T* Xb = X + (size_t)b * strideX;
+    int* fl = flags + (size_t)b * nblk;
+    for (int e = tid; e < bs * bs; e += 256) {
+        const int i = e / bs, k = e % bs;
+        u[i][k] = ld(Ab + 2 * ((size_t)(r0 + i) * lda + r0 + k));
+    }
+    for (int i = tid; i < bs; i += 256) xb[i] = ld(Xb + 2 * (size_t)(r0 + i) * xs);
+    // thread (row i, quarter q) accumulates sum_j M[r0+i][c0+j] x_c[j], j in quarter q
+    const int i = tid >> 2, q = tid & 3;
+    cpx<T> acc = {0, 0};
+    for (int step = 0; step < vid; ++step) {
+        const int c = LOWER ? step : nblk - 1 - step;
+        const int c0 = c * 64;
+        const int cs = n - c0 < 64 ? n - c0 : 64;
+        if (tid == 0) {
+            long long spins = 0;
+            while (__hip_atomic_load(fl + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins > kSpinLimit) {
+                    atomicExch(err, 1);
+                    break;
+                }
+            }
+        }
+        __syncthreads();
+        acquire_wg();
+        for (int j = tid; j < cs; j += 256) xc[j] = ald_c(Xb + 2 * (size_t)(c0 + j) * xs);
+        __syncthreads();
+        if (i < bs) {
+            const T* mrow = Ab + 2 * ((size_t)(r0 + i) * lda + c0);
+            for (int j = q * 16; j < q * 16 + 16 && j < cs; ++j) {
+                const cpx<T> m = ld(mrow + 2 * j);
+                acc.r += m.r * xc[j].r - m.i * xc[j].i;
+                acc.i += m.r * xc[j].i + m.i * xc[j].r;
+            }
+        }
+        __syncthreads();
+    }
+    part[q][i] = acc;
+    __syncthreads();
+    if (tid < bs) {
+        cpx<T> v = xb[tid];
+        for (int k = 0; k < 4; ++k) {
+            v.r -= part[k][tid].r;
+            v.i -= part[k][tid].i;
+        }
+        xb[tid] = v;
+    }
+    __syncthreads();
+    if (tid < 64) {
+        for (int jj = 0; jj < bs; ++jj) {
+            const int j = LOWER ? jj : bs - 1 - jj;
+            if (!LOWER && tid == 0) {
+                const cpx<T> d = u[j][j];
+                xb[j] = (d.r == (T)0 && d.i == (T)0) ? xb[j] : cmul(xb[j], cinv(d));
+            }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            const cpx<T> xj = xb[j];
+            const bool act = LOWER ? (tid > j && tid < bs) : (tid < j);
+            if (act) {
+                const cpx<T> l = u[tid][j];
+                cpx<T> v = xb[tid];
+                v.r -= l.r * xj.r - l.i * xj.i;
+                v.i -= l.r * xj.i + l.i * xj.r;
+                xb[tid] = v;
+            }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        }
+    }
+    __syncthreads();
+    for (int k = tid; k < bs; k += 256) ast_c(Xb + 2 * (size_t)(r0 + k) * xs, xb[k]);
+    release_agent();
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(fl + bk, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // rows [rlo, rhi) -= A[rows, r0:r0+bs] * x[r0:r0+bs]; one wave per row
@@ -604,6 +790,28 @@ __global__ __launch_bounds__(256) void k_make_perm(const int* ipiv, int n, int* 
 int lu_tree_nodes(int n) { return tree_geom(n).nNodes; }
 int lu_cand_stride() { return NB + 1; }
 
+constexpr int WB = 4 * NB;  // outer block: the trailing update runs with K = WB
+
+template <typename T>
+static void gemm(T* A, size_t strideA, int lda, int batch, int r0, int r1, int c0, int c1, int k0, int K,
+                 hipStream_t s) {
+    if (r1 <= r0 || c1 <= c0 || K <= 0) return;
+    dim3 g((c1 - c0 + 63) / 64, (r1 - r0 + 63) / 64, batch);
+    hipLaunchKernelGGL(k_gemm<T>, g, dim3(256), 0, s, A, strideA, lda, r0, r1, c0, c1, k0, K);
+}
+
+template <typename T>
+static void panel_cols(T* A, size_t strideA, int lda, int batch, int d, int w, int clo, int chi, hipStream_t s) {
+    if (chi <= clo) return;
+    hipLaunchKernelGGL(k_panel_cols<T>, dim3((chi - clo + 255) / 256, batch), dim3(256), 0, s, A, strideA, lda, d,
+                       w, clo, chi);
+}
+
+// Blocked right-looking LU.  Outer blocks of WB columns are factored as NB
+// wide sub-panels (tournament pivoting, interchanges over the whole row,
+// diagonal block, L21, and a right-looking update restricted to the block);
+// then the block's U12 = L^-1 A12 over the remaining columns (incl. the rhs
+// column n) and one trailing update A22 -= L21 U12 with K = WB.
 template <typename T>
 hipError_t lu_factor(T* A, size_t strideA, int n, int lda, int batch, LuWork wk, hipStream_t s,
                      const Prof* pf) {
@@ -612,39 +820,61 @@ hipError_t lu_factor(T* A, size_t strideA, int n, int lda, int batch, LuWork wk,
     // candidate and counter slots per batch element are sized for the first
     // (largest) panel; every counter is back at zero when a launch ends
     const int nodeStride = tree_geom(n).nNodes;
-    for (int j0 = 0; j0 < n; j0 += NB) {
-        const int w = n - j0 < NB ? n - j0 : NB;
-        TreeGeom g = tree_geom(n - j0);
-        pf->begin(5);
-        hipLaunchKernelGGL(k_tslu<T>, dim3(g.levelCount[0], batch), dim3(256), 0, s, A, strideA, n,
-                           lda, j0, w, g, nodeStride, wk.cand, wk.counters, wk.ipiv, wk.info);
-        pf->end(5);
-        pf->begin(6);
-        hipLaunchKernelGGL(k_laswp<T>, dim3((n + 1 + 255) / 256, batch), dim3(256), 0, s, A,
-                           strideA, n, lda, j0, w, wk.ipiv);
-        pf->end(6);
-        const int rem = n - j0 - w;
-        const int nRowBlocks = (rem + 255) / 256;
-        const int nColBlocks = (rem + 1 + 255) / 256;
-        pf->begin(7);
-        hipLaunchKernelGGL(k_panel_diag<T>, dim3(batch), dim3(256), 0, s, A, strideA, lda, j0, w, wk.info);
-        hipLaunchKernelGGL(k_panel<T>, dim3(nRowBlocks + nColBlocks, batch), dim3(256), 0, s, A,
-                           strideA, n, lda, j0, w, nRowBlocks);
-        pf->end(7);
-        if (rem > 0) {
-            dim3 gg((rem + 1 + 63) / 64, (rem + 63) / 64, batch);
+    for (int j0 = 0; j0 < n; j0 += WB) {
+        const int wb = n - j0 < WB ? n - j0 : WB;
+        const int jend = j0 + wb;
+        for (int c0 = j0; c0 < jend; c0 += NB) {
+            const int w = jend - c0 < NB ? jend - c0 : NB;
+            TreeGeom g = tree_geom(n - c0);
+            pf->begin(5);
+            hipLaunchKernelGGL(k_tslu<T>, dim3(g.levelCount[0], batch), dim3(64), 0, s, A, strideA, n, lda, c0, w,
+                               g, nodeStride, wk.cand, wk.counters, wk.ipiv, wk.info);
+            pf->end(5);
+            pf->begin(6);
+            hipLaunchKernelGGL(k_laswp<T>, dim3((n + 1 + 255) / 256, batch), dim3(256), 0, s, A, strideA, n, lda,
+                               c0, w, wk.ipiv);
+            pf->end(6);
+            pf->begin(7);
+            hipLaunchKernelGGL(k_panel_diag<T>, dim3(batch), dim3(64), 0, s, A, strideA, lda, c0, w, wk.info);
+            if (c0 + w < n)
+                hipLaunchKernelGGL(k_panel_rows<T>, dim3((n - c0 - w + 255) / 256, batch), dim3(256), 0, s, A,
+                                   strideA, lda, c0, w, c0 + w, n);
+            panel_cols<T>(A, strideA, lda, batch, c0, w, c0 + w, jend, s);
+            pf->end(7);
             pf->begin(8);
-            hipLaunchKernelGGL(k_gemm_update<T>, gg, dim3(256), 0, s, A, strideA, n, lda, j0, w);
+            gemm<T>(A, strideA, lda, batch, c0 + w, n, c0 + w, jend, c0, w, s);
             pf->end(8);
         }
+        // U12 of the block rows over columns jend..n (rhs included)
+        pf->begin(7);
+        for (int c0 = j0; c0 < jend; c0 += NB) {
+            const int w = jend - c0 < NB ? jend - c0 : NB;
+            panel_cols<T>(A, strideA, lda, batch, c0, w, jend, n + 1, s);
+            gemm<T>(A, strideA, lda, batch, c0 + w, jend, jend, n + 1, c0, w, s);
+        }
+        pf->end(7);
+        pf->begin(8);
+        gemm<T>(A, strideA, lda, batch, jend, n, jend, n + 1, j0, wb, s);
+        pf->end(8);
     }
     return hipGetLastError();
 }
 
 template <typename T>
 hipError_t lu_trsv(const T* A, size_t strideA, int n, int lda, T* X, size_t strideX, size_t xs,
-                   int batch, bool lower, const int* skip, hipStream_t s) {
+                   int batch, bool lower, const int* skip, hipStream_t s, TrsvWork* tw) {
     const int nblk = (n + 63) / 64;
+    if (tw) {
+        ++tw->epoch;
+        hipLaunchKernelGGL(k_zero_int, dim3((batch + 255) / 256), dim3(256), 0, s, tw->tickets, batch);
+        if (lower)
+            hipLaunchKernelGGL((k_trsv_fused<T, true>), dim3(nblk, batch), dim3(256), 0, s, A, strideA, lda, n, X,
+                               strideX, xs, tw->flags, tw->tickets, tw->epoch, skip, tw->err);
+        else
+            hipLaunchKernelGGL((k_trsv_fused<T, false>), dim3(nblk, batch), dim3(256), 0, s, A, strideA, lda, n, X,
+                               strideX, xs, tw->flags, tw->tickets, tw->epoch, skip, tw->err);
+        return hipGetLastError();
+    }
     for (int q = 0; q < nblk; ++q) {
         const int bk = lower ? q : nblk - 1 - q;
         const int r0 = bk * 64;
@@ -664,9 +894,11 @@ hipError_t lu_trsv(const T* A, size_t strideA, int n, int lda, T* X, size_t stri
 }
 
 template <typename T>
-hipError_t lu_backsolve(T* A, size_t strideA, int n, int lda, int batch, hipStream_t s) {
-    return lu_trsv<T>(A, strideA, n, lda, A + 2 * (size_t)n, strideA, (size_t)lda, batch, false, nullptr, s);
+hipError_t lu_backsolve(T* A, size_t strideA, int n, int lda, int batch, hipStream_t s, TrsvWork* tw) {
+    return lu_trsv<T>(A, strideA, n, lda, A + 2 * (size_t)n, strideA, (size_t)lda, batch, false, nullptr, s, tw);
 }
+
+
 
 hipError_t lu_make_perm(const int* ipiv, int n, int batch, int* perm, hipStream_t s) {
     hipLaunchKernelGGL(k_make_perm, dim3(batch), dim3(256), sizeof(int) * (size_t)n, s, ipiv, n, perm);
@@ -675,11 +907,11 @@ hipError_t lu_make_perm(const int* ipiv, int n, int batch, int* perm, hipStream_
 
 template hipError_t lu_factor<double>(double*, size_t, int, int, int, LuWork, hipStream_t, const Prof*);
 template hipError_t lu_factor<float>(float*, size_t, int, int, int, LuWork, hipStream_t, const Prof*);
-template hipError_t lu_backsolve<double>(double*, size_t, int, int, int, hipStream_t);
-template hipError_t lu_backsolve<float>(float*, size_t, int, int, int, hipStream_t);
+template hipError_t lu_backsolve<double>(double*, size_t, int, int, int, hipStream_t, TrsvWork*);
+template hipError_t lu_backsolve<float>(float*, size_t, int, int, int, hipStream_t, TrsvWork*);
 template hipError_t lu_trsv<double>(const double*, size_t, int, int, double*, size_t, size_t, int, bool,
-                                    const int*, hipStream_t);
+                                    const int*, hipStream_t, TrsvWork*);
 template hipError_t lu_trsv<float>(const float*, size_t, int, int, float*, size_t, size_t, int, bool,
-                                   const int*, hipStream_t);
+                                   const int*, hipStream_t, TrsvWork*);
 
 }  // namespace efie

@@ -1,6 +1,7 @@
-"""Builds the mx API shim and the two MEX entry points against it (g++), for
-the MEX-layer tests:  _build/libmxshim.so, _build/mex_patch_sweep_cuda.so,
-_build/mex_gassemble_ZV_EFIE_RWG_DIFF_GPU_FAST.so (next to libefie_hip.so).
+"""Builds the CPU-side test libraries next to libefie_hip.so (g++): the mx
+API shim and the two MEX entry points against it (_build/libmxshim.so,
+_build/mex_patch_sweep_cuda.so, _build/mex_gassemble_ZV_EFIE_RWG_DIFF_GPU_FAST.so)
+and a host build of the device sine/cosine (_build/libefie_math_host.so).
 """
 import os
 import subprocess
@@ -24,6 +25,10 @@ def build():
                         "-I", os.path.join(ROOT, "include"), os.path.join(PKG, "mex", name + ".cpp"),
                         "-o", so, "-L", OUT, "-lefie_hip", "-lmxshim", "-Wl,-rpath,$ORIGIN"], check=True)
         outs.append(so)
+    mathlib = os.path.join(OUT, "libefie_math_host.so")
+    subprocess.run(["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-ffp-contract=off",
+                    os.path.join(HERE, "efie_math_host.cpp"), "-o", mathlib], check=True)
+    outs.append(mathlib)
     return outs
 
 
