@@ -63,14 +63,39 @@ def timed(ctx, o, fr, pe, ge, spread, steps, warmup, name):
     return dt, out
 
 
+WB = 4 * NB           # outer block of the LU (trailing update with K = WB)
+
+
 def lu_gemm_flops(n):
+    """Flops of every matrix-core update launched by one LU of size n (rhs
+    column included), following the library's blocking: NB-wide sub-panels
+    inside WB-wide outer blocks."""
     f = 0.0
-    for j0 in range(0, n, NB):
-        w = min(NB, n - j0)
-        rem = n - j0 - w
-        if rem > 0:
-            f += 8.0 * rem * (rem + 1) * w
+
+    def g(r0, r1, c0, c1, K):
+        return 8.0 * max(0, r1 - r0) * max(0, c1 - c0) * K if K > 0 else 0.0
+    for j0 in range(0, n, WB):
+        jend = min(n, j0 + WB)
+        for c0 in range(j0, jend, NB):
+            w = min(NB, jend - c0)
+            f += g(c0 + w, n, c0 + w, jend, w)           # update inside the block
+        for c0 in range(j0, jend, NB):
+            w = min(NB, jend - c0)
+            f += g(c0 + w, jend, jend, n + 1, w)         # U12 of the block rows
+        f += g(jend, n, jend, n + 1, jend - j0)          # trailing update
     return f
+
+
+def traffic(kernel):
+    """HBM bytes per launch of `kernel` on this workload, measured with
+    rocprofv3 FETCH_SIZE / WRITE_SIZE and scaled by the same counters on a
+    copy kernel of known size (profiles/traffic.json), or None."""
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(p) as fh:
+            return json.load(fh)[kernel]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def roofline(st, n, nf, fast):
@@ -82,10 +107,10 @@ def roofline(st, n, nf, fast):
         work = lu_gemm_flops(n) * nf
         prec = "fp32" if fast else "fp64"
         ach = work / (ms * 1e-3) / 1e12
-        return {"kernel": "k_gemm_update", "bound": "mfma", "achieved": ach, "peak": PEAK[prec],
-                "unit": "TFLOP/s", "frac": ach / PEAK[prec], "traffic": None,
-                "work": f"8*rem*(rem+1)*w flop per panel update, {work/cnt:.3e} flop/launch avg, "
-                        f"{cnt} launches, {ms/cnt*1e3:.1f} us/launch avg"}
+        return {"kernel": f"k_gemm<{'float' if fast else 'double'}>", "bound": "mfma", "achieved": ach,
+                "peak": PEAK[prec], "unit": "TFLOP/s", "frac": ach / PEAK[prec], "traffic": traffic("k_gemm"),
+                "work": f"8*M*N*K flop per update ({lu_gemm_flops(n):.4e} per LU), {work/cnt:.3e} flop/launch "
+                        f"avg over {cnt} launches, {ms/cnt*1e3:.1f} us/launch avg"}
     if name in ("regular", "near", "self"):
         evals = st["n_green_" + name]
         work = evals * FLOP_PER_GREEN * nf
@@ -144,12 +169,23 @@ def main():
     ap.add_argument("--config", default="c2_6k")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--nf", type=int, default=0, help="use only the first NF frequencies (profiling runs)")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="only run the plain copy kernel on this many MiB (counter calibration)")
     a = ap.parse_args()
+    if a.copy_probe_mb:
+        with efie.Context(0) as ctx:
+            nbytes = a.copy_probe_mb << 20
+            ms = ctx.diag_copy(nbytes, 5)
+        print(json.dumps({"copy_bytes": nbytes, "ms": ms, "GB/s": 2 * nbytes / (ms * 1e-3) / 1e9}))
+        return
     if a.gpus != 1:
         print(f"bench.py measures one GPU (asked for {a.gpus}): the sweep would shard by frequency "
               f"across GPUs, which this build does not run.", file=sys.stderr)
         sys.exit(2)
     V, F, r, pe, ge, spread, fr = setup(a.config)
+    if a.nf:
+        fr = fr[: a.nf]
     n = r.Ne
     log(f"config {a.config}: Ne={n} Nt={F.shape[0]} Nf={len(fr)}")
     o64 = efie.opts(**W.OPTS_C1, precision=efie.PREC_FP64, near_mode=efie.NEAR_DIRECT)

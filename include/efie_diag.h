@@ -1,0 +1,23 @@
+/*
+ * efie_diag.h -- diagnostic entry points of libefie_hip.so (no counterpart
+ * in the reference): a plain device-to-device copy kernel of known size,
+ * used to calibrate hardware byte counters (FETCH_SIZE / WRITE_SIZE) when
+ * profiling the EFIE kernels.
+ */
+#ifndef EFIE_DIAG_H
+#define EFIE_DIAG_H
+
+#include "efie.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Copies `bytes` (multiple of 16) between two device buffers `reps` times
+ * with one streaming kernel per copy; *ms = average kernel time. */
+efie_status efie_diag_copy(efie_ctx* ctx, size_t bytes, int reps, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

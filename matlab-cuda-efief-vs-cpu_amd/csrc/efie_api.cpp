@@ -1,6 +1,7 @@
 // efie_api.cpp -- C ABI of libefie_hip.so: argument checking, host-side
 // table builders, device context and the frequency-sweep driver.
 #include "../../include/efie.h"
+#include "../../include/efie_diag.h"
 #include "efie_internal.h"
 
 #include <hip/hip_runtime.h>
@@ -1329,6 +1330,32 @@ efie_status efie_zgesv(efie_ctx* c, int n, int batch, const double* A, const dou
     S->ms_factor = t2;
     S->ms_solve = t3;
     S->ms_total = t2 + t3;
+    return ok();
+}
+
+efie_status efie_diag_copy(efie_ctx* c, size_t bytes, int reps, double* ms) {
+    if (!c || !ms || bytes == 0 || bytes % 16 || reps < 1) return fail(EFIE_E_ARGS, "efie:args", "bad arguments");
+    HIPCHK(hipSetDevice(c->device));
+    void *a = nullptr, *b = nullptr;
+    HIPCHK(hipMalloc(&a, bytes));
+    hipError_t e = hipMalloc(&b, bytes);
+    if (e != hipSuccess) {
+        (void)hipFree(a);
+        return fail(EFIE_E_NOMEM, "efie:hip", "hipMalloc: %s", hipGetErrorString(e));
+    }
+    hipStream_t s = c->stream;
+    e = hipMemsetAsync(a, 0, bytes, s);
+    if (e == hipSuccess) e = launch_copy16(a, b, bytes, s);
+    if (e == hipSuccess) e = hipEventRecord(c->ev[0], s);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_copy16(a, b, bytes, s);
+    if (e == hipSuccess) e = hipEventRecord(c->ev[1], s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    float t = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, c->ev[0], c->ev[1]);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (e != hipSuccess) return fail(EFIE_E_HIP, "efie:hip", "copy: %s", hipGetErrorString(e));
+    *ms = t / reps;
     return ok();
 }
 

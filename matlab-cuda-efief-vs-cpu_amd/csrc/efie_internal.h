@@ -161,6 +161,7 @@ hipError_t launch_perm_f32(const double* R, const int* perm, int n, int batch, f
 hipError_t launch_add_f32(double* X, const float* D32, int n, int batch, const int* done,
                           hipStream_t s);
 hipError_t launch_fill_int(int* p, int v, int n, hipStream_t s);
+hipError_t launch_copy16(const void* src, void* dst, size_t bytes, hipStream_t s);
 hipError_t launch_fill_dbl(double* p, double v, int n, hipStream_t s);
 
 // Optional per-kernel-class timing hook (efie_opts.profile).

@@ -109,6 +109,20 @@ __global__ void k_fill_dbl(double* p, double v, int n) {
     if (i < n) p[i] = v;
 }
 
+// Plain streaming copy (counter calibration).
+__global__ void k_copy16(const double2* __restrict__ src, double2* __restrict__ dst, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
+hipError_t launch_copy16(const void* src, void* dst, size_t bytes, hipStream_t s) {
+    const size_t n = bytes / 16;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_copy16, dim3((unsigned)blocks), dim3(256), 0, s, (const double2*)src, (double2*)dst, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_to_f32(const double* A64, float* A32, size_t complexElems, hipStream_t s) {
     const size_t blocks = std::min<size_t>((complexElems + 255) / 256, 65536);
     hipLaunchKernelGGL(k_to_f32, dim3((unsigned)blocks), dim3(256), 0, s, A64, A32, complexElems);

@@ -81,7 +81,7 @@ EXPORTS = [
     "efie_opts_default", "efie_opts_preset", "efie_last_error", "efie_last_error_id",
     "efie_mesh_counts", "efie_make_mesh", "efie_rwg_count", "efie_build_rwg", "efie_pick_port",
     "efie_create", "efie_destroy", "efie_set_mesh", "efie_set_rwg", "efie_assemble_zv",
-    "efie_sweep", "efie_zgesv",
+    "efie_sweep", "efie_zgesv", "efie_diag_copy",
 ]
 
 _lib = None
@@ -114,6 +114,7 @@ def lib():
         L.efie_sweep.argtypes = [C.c_void_p, _dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
                                  C.c_double, C.c_double, C.c_double, C.POINTER(Opts), _dp, _dp, C.POINTER(Stats)]
         L.efie_zgesv.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, C.POINTER(Stats)]
+        L.efie_diag_copy.argtypes = [C.c_void_p, C.c_size_t, C.c_int, _dp]
         _lib = L
     return _lib
 
@@ -290,6 +291,13 @@ class Context:
                                 x.view(np.float64).ctypes.data_as(_dp), C.byref(st)))
         x = x[0] if single else x
         return (x, st.as_dict()) if stats else x
+
+
+    def diag_copy(self, nbytes, reps=10):
+        """Average time (ms) of a plain device copy of nbytes (efie_diag.h)."""
+        ms = C.c_double()
+        _check(lib().efie_diag_copy(self._c, int(nbytes), int(reps), C.byref(ms)))
+        return ms.value
 
 
 # ------------------------------------------------- reference MEX mirrors --

@@ -21,11 +21,12 @@ def _declared(header):
 
 def test_library_exports_every_declared_symbol(efie_mod):
     L = ctypes.CDLL(efie_mod.LIB_PATH)
-    names = _declared("efie.h")
-    assert len(names) >= 15
+    names = _declared("efie.h") + _declared("efie_diag.h")
+    assert len(names) >= 17
     for n in names:
         assert hasattr(L, n), n
-    assert sorted(efie_mod.EXPORTS) == sorted(n for n in names if n in efie_mod.EXPORTS)
+    for n in efie_mod.EXPORTS:
+        assert n in names, n
 
 
 @pytest.mark.parametrize("cells", [(1, 1, 1, 1), (3, 2, 2, 4), (8, 8, 12, 12), (26, 26, 37, 37)])
