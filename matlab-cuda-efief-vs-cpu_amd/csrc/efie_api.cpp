@@ -206,7 +206,11 @@ struct efie_ctx {
     bool prof = false;
     std::vector<hipEvent_t> pool;
     size_t poolUsed = 0;
-    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
+    struct Mark {
+        int cls, launches;
+        hipEvent_t a, b;
+    };
+    std::vector<Mark> marks;
     int openCls = -1;
     hipEvent_t openEv = nullptr;
 };
@@ -496,7 +500,7 @@ namespace {
 
 // ------------------------------------------------------------ profiling --
 
-void prof_mark(void* user, int cls, bool start) {
+void prof_mark(void* user, int cls, int launches) {
     efie_ctx* c = static_cast<efie_ctx*>(user);
     auto next = [c]() {
         if (c->poolUsed == c->pool.size()) {
@@ -509,11 +513,11 @@ void prof_mark(void* user, int cls, bool start) {
     hipEvent_t e = next();
     if (!e) return;
     (void)hipEventRecord(e, c->stream);
-    if (start) {
+    if (launches < 0) {
         c->openCls = cls;
         c->openEv = e;
     } else if (c->openCls == cls && c->openEv) {
-        c->marks.push_back({cls, {c->openEv, e}});
+        c->marks.push_back({cls, launches, c->openEv, e});
         c->openEv = nullptr;
         c->openCls = -1;
     }
@@ -532,9 +536,9 @@ Prof make_prof(efie_ctx* c) {
 void prof_collect(efie_ctx* c, efie_stats* st) {
     for (auto& m : c->marks) {
         float ms = 0;
-        if (hipEventElapsedTime(&ms, m.second.first, m.second.second) == hipSuccess && st) {
-            st->ms_kernel[m.first] += ms;
-            st->n_kernel[m.first] += 1;
+        if (hipEventElapsedTime(&ms, m.a, m.b) == hipSuccess && st) {
+            st->ms_kernel[m.cls] += ms;
+            st->n_kernel[m.cls] += m.launches;
         }
     }
     c->marks.clear();

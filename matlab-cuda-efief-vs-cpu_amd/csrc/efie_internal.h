@@ -165,14 +165,16 @@ hipError_t launch_copy16(const void* src, void* dst, size_t bytes, hipStream_t s
 hipError_t launch_fill_dbl(double* p, double v, int n, hipStream_t s);
 
 // Optional per-kernel-class timing hook (efie_opts.profile).
+// Event marks around kernel launches of one class; end() names how many
+// kernels the region launched (-1 marks a region start).
 struct Prof {
     void* user = nullptr;
-    void (*mark)(void* user, int cls, bool start) = nullptr;
+    void (*mark)(void* user, int cls, int launches) = nullptr;
     void begin(int cls) const {
-        if (mark) mark(user, cls, true);
+        if (mark) mark(user, cls, -1);
     }
-    void end(int cls) const {
-        if (mark) mark(user, cls, false);
+    void end(int cls, int launches = 1) const {
+        if (mark) mark(user, cls, launches);
     }
 };
 

@@ -792,19 +792,22 @@ int lu_cand_stride() { return NB + 1; }
 
 constexpr int WB = 4 * NB;  // outer block: the trailing update runs with K = WB
 
+// Returns the number of kernels launched (0 for an empty update).
 template <typename T>
-static void gemm(T* A, size_t strideA, int lda, int batch, int r0, int r1, int c0, int c1, int k0, int K,
-                 hipStream_t s) {
-    if (r1 <= r0 || c1 <= c0 || K <= 0) return;
+static int gemm(T* A, size_t strideA, int lda, int batch, int r0, int r1, int c0, int c1, int k0, int K,
+                hipStream_t s) {
+    if (r1 <= r0 || c1 <= c0 || K <= 0) return 0;
     dim3 g((c1 - c0 + 63) / 64, (r1 - r0 + 63) / 64, batch);
     hipLaunchKernelGGL(k_gemm<T>, g, dim3(256), 0, s, A, strideA, lda, r0, r1, c0, c1, k0, K);
+    return 1;
 }
 
 template <typename T>
-static void panel_cols(T* A, size_t strideA, int lda, int batch, int d, int w, int clo, int chi, hipStream_t s) {
-    if (chi <= clo) return;
+static int panel_cols(T* A, size_t strideA, int lda, int batch, int d, int w, int clo, int chi, hipStream_t s) {
+    if (chi <= clo) return 0;
     hipLaunchKernelGGL(k_panel_cols<T>, dim3((chi - clo + 255) / 256, batch), dim3(256), 0, s, A, strideA, lda, d,
                        w, clo, chi);
+    return 1;
 }
 
 // Blocked right-looking LU.  Outer blocks of WB columns are factored as NB
@@ -835,27 +838,28 @@ hipError_t lu_factor(T* A, size_t strideA, int n, int lda, int batch, LuWork wk,
                                c0, w, wk.ipiv);
             pf->end(6);
             pf->begin(7);
+            int np = 1;
             hipLaunchKernelGGL(k_panel_diag<T>, dim3(batch), dim3(64), 0, s, A, strideA, lda, c0, w, wk.info);
-            if (c0 + w < n)
+            if (c0 + w < n) {
                 hipLaunchKernelGGL(k_panel_rows<T>, dim3((n - c0 - w + 255) / 256, batch), dim3(256), 0, s, A,
                                    strideA, lda, c0, w, c0 + w, n);
-            panel_cols<T>(A, strideA, lda, batch, c0, w, c0 + w, jend, s);
-            pf->end(7);
+                ++np;
+            }
+            np += panel_cols<T>(A, strideA, lda, batch, c0, w, c0 + w, jend, s);
+            pf->end(7, np);
             pf->begin(8);
-            gemm<T>(A, strideA, lda, batch, c0 + w, n, c0 + w, jend, c0, w, s);
-            pf->end(8);
+            pf->end(8, gemm<T>(A, strideA, lda, batch, c0 + w, n, c0 + w, jend, c0, w, s));
         }
         // U12 of the block rows over columns jend..n (rhs included)
-        pf->begin(7);
         for (int c0 = j0; c0 < jend; c0 += NB) {
             const int w = jend - c0 < NB ? jend - c0 : NB;
-            panel_cols<T>(A, strideA, lda, batch, c0, w, jend, n + 1, s);
-            gemm<T>(A, strideA, lda, batch, c0 + w, jend, jend, n + 1, c0, w, s);
+            pf->begin(7);
+            pf->end(7, panel_cols<T>(A, strideA, lda, batch, c0, w, jend, n + 1, s));
+            pf->begin(8);
+            pf->end(8, gemm<T>(A, strideA, lda, batch, c0 + w, jend, jend, n + 1, c0, w, s));
         }
-        pf->end(7);
         pf->begin(8);
-        gemm<T>(A, strideA, lda, batch, jend, n, jend, n + 1, j0, wb, s);
-        pf->end(8);
+        pf->end(8, gemm<T>(A, strideA, lda, batch, jend, n, jend, n + 1, j0, wb, s));
     }
     return hipGetLastError();
 }
