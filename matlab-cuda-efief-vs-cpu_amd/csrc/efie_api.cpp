@@ -198,7 +198,7 @@ struct efie_ctx {
     double cAlpha = -1;
     // per-call buffers
     DBuf dRecs, dFreq, dZ, dZ32, dWp, dWg, dSums, dV, dWl, dOut;
-    DBuf dCand, dCnt, dIpiv, dInfo, dPerm, dX, dR, dR32, dRn2, dBn2, dDone, dIters, dResid, dB;
+    DBuf dCand, dCnt, dIpiv, dInfo, dSwp, dPerm, dX, dR, dR32, dRn2, dBn2, dDone, dIters, dResid, dB;
     DBuf dFlags, dTickets, dErr;
     TrsvWork tw{nullptr, nullptr, nullptr, 0};
 
@@ -982,6 +982,7 @@ efie_status ensure_solver(efie_ctx* c, const efie_opts& o, int n, int lda, int B
     }
     HIPCHK(c->dIpiv.ensure(sizeof(int) * (size_t)B * n));
     HIPCHK(c->dInfo.ensure(sizeof(int) * B));
+    HIPCHK(c->dSwp.ensure(sizeof(int) * (size_t)B * lu_swap_stride()));
     // single-launch triangular solve: flags hold the epoch of the last
     // launch that finished each block, so they are zeroed only when new
     const size_t flagBytes = sizeof(int) * (size_t)B * ((n + 63) / 64);
@@ -1010,7 +1011,8 @@ efie_status solve_batch(efie_ctx* c, const efie_opts& o, int n, int lda, int nb,
     hipStream_t s = c->stream;
     const size_t zElems = (size_t)n * lda;
     double* Zb = c->dZ.as<double>();
-    LuWork wk{c->dCand.as<int>(), c->dCnt.as<int>(), c->dIpiv.as<int>(), c->dInfo.as<int>()};
+    LuWork wk{c->dCand.as<int>(), c->dCnt.as<int>(), c->dIpiv.as<int>(), c->dInfo.as<int>(),
+              c->dSwp.as<int>()};
     HIPCHK(launch_fill_int(c->dInfo.as<int>(), 0, nb, s));
     if (o.precision != EFIE_PREC_MIXED) {
         HIPCHK(lu_factor<double>(Zb, 2 * zElems, n, lda, nb, wk, s, &pf));

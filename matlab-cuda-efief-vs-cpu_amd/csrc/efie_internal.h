@@ -115,7 +115,10 @@ struct LuWork {
     int* counters;  // tree arrival counters
     int* ipiv;      // [batch][n] 1-based LAPACK-style pivots
     int* info;      // [batch] first zero pivot (1-based) or 0
+    int* swaps;     // [batch][lu_swap_stride()]: row moves of the current panel
 };
+// ints per batch element of LuWork::swaps
+int lu_swap_stride();
 
 // Tournament tree nodes of an n-row panel; cand needs
 // batch * nodes * lu_cand_stride() ints, counters batch * nodes (zeroed once).
