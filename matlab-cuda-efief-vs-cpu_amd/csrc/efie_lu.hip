@@ -59,6 +59,11 @@ template <typename T>
 __device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
     return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r};
 }
+// a - l*u as four fused multiply-adds
+template <typename T>
+__device__ __forceinline__ cpx<T> cfms(cpx<T> a, cpx<T> l, cpx<T> u) {
+    return {fma(-l.r, u.r, fma(l.i, u.i, a.r)), fma(-l.r, u.i, fma(-l.i, u.r, a.i))};
+}
 template <typename T>
 __device__ __forceinline__ cpx<T> cinv(cpx<T> a) {
     T d = a.r * a.r + a.i * a.i;
@@ -158,6 +163,38 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long k)
     return ((unsigned long long)hi << 32) | lo;
 }
 
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ int dpp_i32(int v) {
+    return __builtin_amdgcn_update_dpp(v, v, CTRL, ROWMASK, 0xF, false);
+}
+// Largest value of the wave (lanes a DPP pattern cannot reach compare with
+// themselves), broadcast from lane 63.
+__device__ __forceinline__ int wave_max_i32(int v) {
+    v = max(v, dpp_i32<0xB1, 0xF>(v));
+    v = max(v, dpp_i32<0x4E, 0xF>(v));
+    v = max(v, dpp_i32<0x141, 0xF>(v));
+    v = max(v, dpp_i32<0x140, 0xF>(v));
+    v = max(v, dpp_i32<0x142, 0xA>(v));
+    v = max(v, dpp_i32<0x143, 0xC>(v));
+    return __builtin_amdgcn_readlane(v, 63);
+}
+__device__ __forceinline__ int wave_min_i32(int v) {
+    v = min(v, dpp_i32<0xB1, 0xF>(v));
+    v = min(v, dpp_i32<0x4E, 0xF>(v));
+    v = min(v, dpp_i32<0x141, 0xF>(v));
+    v = min(v, dpp_i32<0x140, 0xF>(v));
+    v = min(v, dpp_i32<0x142, 0xA>(v));
+    v = min(v, dpp_i32<0x143, 0xC>(v));
+    return __builtin_amdgcn_readlane(v, 63);
+}
+// 1/x from the hardware reciprocal (one Newton step for double); used only
+// to rank pivot candidates, never for the factors themselves
+__device__ __forceinline__ float rcp_fast(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rcp_fast(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    return r * (2.0 - x * r);
+}
+
 template <typename T>
 __device__ __forceinline__ T readlane_t(T v, int lane) {
     if constexpr (std::is_same<T, double>::value) {
@@ -174,13 +211,21 @@ __device__ __forceinline__ T readlane_t(T v, int lane) {
 // -1 for none) of the w columns j0.. of A.  Returns in `chosen` (lane i)
 // the label of the i-th pivot row; zc = first column with a zero pivot.
 // The pivot of a column is the row of largest |re|+|im| (compared in
-// single precision), the first position among equals.
+// single precision), the first position among equals.  Must be called by
+// all 64 threads of a one-wave block; prow is NB complex values of LDS.
+//
+// The step is a chain of dependent operations, so it is kept short: a
+// float DPP max (positions are compared only when magnitudes tie), the
+// pivot row through LDS, and the multipliers from a hardware reciprocal.
+// The multipliers here only rank the candidates of later columns; the
+// factors themselves are computed by the panel kernels.  Columns past w
+// are zero in every row, so the updates run over all columns unguarded.
 template <typename T>
-__device__ __forceinline__ void gepp_wave(const T* Ab, int lda, int j0, int w, int r, int myLab, int& chosen, int& zc) {
+__device__ __forceinline__ void gepp_wave(const T* Ab, int lda, int j0, int w, int r, int myLab, int& chosen,
+                                          int& zc, cpx<T>* prow) {
     const int lane = threadIdx.x;
     // all NB loads are issued unconditionally (row clamped to j0, columns
-    // past w read in-bounds padding) and masked afterwards, so they are in
-    // flight together instead of one guarded round trip each
+    // past w read in-bounds padding) and masked afterwards
     cpx<T> a[NB];
     const T* rowp = Ab + 2 * ((size_t)(myLab >= 0 ? myLab : j0) * lda + j0);
 #pragma unroll
@@ -199,41 +244,48 @@ __device__ __forceinline__ void gepp_wave(const T* Ab, int lda, int j0, int w, i
 #pragma unroll
     for (int c = 0; c < NB; ++c) {
         if (c < steps) {
-            // key: magnitude bits (order-preserving for values >= 0) over ~position
-            const float mag = (float)(fabs(a[c].r) + fabs(a[c].i));
-            const unsigned long long key =
-                act ? (((unsigned long long)__float_as_uint(mag) << 32) | (unsigned)(0xFFFFFFFFu - (unsigned)pos))
-                    : 0ull;
-            const unsigned long long best = wave_max_u64(key);
-            const int bpos = (int)(0xFFFFFFFFu - (unsigned)best);
-            const bool nonzero = (best >> 32) != 0;
-            const unsigned long long m = __ballot(act && pos == bpos);
-            const int p = __ffsll((long long)m) - 1;
+            // magnitudes compared as the integer bits of non-negative floats;
+            // inactive rows carry -1 and never win (some row is active at
+            // every step)
+            const int mag = act ? __float_as_int((float)(fabs(a[c].r) + fabs(a[c].i))) : -1;
+            const int mx = wave_max_i32(mag);
+            const unsigned long long m = __ballot(act && mag == mx);
+            int p = __ffsll((long long)m) - 1;
+            if (__popcll(m) > 1) {
+                // equal magnitudes: the first position wins
+                const bool cand = (m >> lane) & 1ull;
+                const int pmin = wave_min_i32(cand ? pos : 0x7fffffff);
+                p = __ffsll((long long)__ballot(cand && pos == pmin)) - 1;
+            }
+            const bool nonzero = mx > 0;
+            const int bpos = __builtin_amdgcn_readlane(pos, p);
             const int labp = __builtin_amdgcn_readlane(myLab, p);
+            if (lane == p) {
+#pragma unroll
+                for (int k = c; k < NB; ++k) prow[k] = a[k];
+            }
             if (pos == c) pos = bpos;  // the row at position c takes the pivot row's place
             if (lane == p) {
                 pos = c;
                 act = false;
             }
             if (lane == c) chosen = labp;
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
             if (!nonzero) {
                 if (zc < 0) zc = c;
             } else {
-                const cpx<T> d = {readlane_t(a[c].r, p), readlane_t(a[c].i, p)};
-                const cpx<T> inv = cinv(d);
+                const cpx<T> d = prow[c];
+                const T rn = rcp_fast(d.r * d.r + d.i * d.i);
+                const cpx<T> inv = {d.r * rn, -d.i * rn};
                 // rows already chosen (and the pivot row) get multiplier 0
                 cpx<T> l = cmul(a[c], inv);
                 if (!act) l = {(T)0, (T)0};
-                else a[c] = l;
 #pragma unroll
-                for (int k = c + 1; k < NB; ++k) {
-                    if (k < w) {
-                        const cpx<T> u = {readlane_t(a[k].r, p), readlane_t(a[k].i, p)};
-                        a[k].r -= l.r * u.r - l.i * u.i;
-                        a[k].i -= l.r * u.i + l.i * u.r;
-                    }
-                }
+                for (int k = c + 1; k < NB; ++k) a[k] = cfms(a[k], l, prow[k]);
             }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         }
     }
 }
@@ -244,6 +296,7 @@ __global__ __launch_bounds__(64) void k_tslu(T* A, size_t strideA, int n, int ld
                                              TreeGeom g, int nodeStride, int* cand, int* counters,
                                              int* ipiv, int* info, int* swaps) {
     __shared__ int s_go;
+    __shared__ cpx<T> prow[NB];
     const int b = blockIdx.y;
     const T* Ab = A + (size_t)b * strideA;
     int* cb = cand + (size_t)b * nodeStride * (NB + 1);
@@ -256,7 +309,7 @@ __global__ __launch_bounds__(64) void k_tslu(T* A, size_t strideA, int n, int ld
     int myLab = lane < r ? j0 + node * TL + lane : -1;
     while (true) {
         int chosen, zc;
-        gepp_wave<T>(Ab, lda, j0, w, r, myLab, chosen, zc);
+        gepp_wave<T>(Ab, lda, j0, w, r, myLab, chosen, zc, prow);
         const int keep = r < w ? r : w;
         if (g.levelCount[level] == 1) {
             // Root: turn the chosen rows (lane i holds the i-th) into LAPACK
@@ -318,7 +371,7 @@ __global__ __launch_bounds__(64) void k_tslu(T* A, size_t strideA, int n, int ld
         __syncthreads();
         const int parent = node >> 1;
         const bool hasSib = (node ^ 1) < g.levelCount[level];
-        if (lane == 0) {
+        if (threadIdx.x == 0) {
             int go = 1;
             if (hasSib) {
                 int* c = ctr + g.levelBase[level + 1] + parent;
@@ -409,13 +462,8 @@ __global__ __launch_bounds__(64) void k_panel_diag(T* A, size_t strideA, int lda
                 const cpx<T> l = cmul(a[c], cinv(d));
                 a[c] = l;
 #pragma unroll
-                for (int k = c + 1; k < NB; ++k) {
-                    if (k < w) {
-                        const cpx<T> u = prow[k];
-                        a[k].r -= l.r * u.r - l.i * u.i;
-                        a[k].i -= l.r * u.i + l.i * u.r;
-                    }
-                }
+                for (int k = c + 1; k < NB; ++k)
+                    if (k < w) a[k] = cfms(a[k], l, prow[k]);
             }
             __syncthreads();
         }
