@@ -845,17 +845,21 @@ Freq make_freq(std::complex<double> k, double omega, double mu, std::complex<dou
     return f;
 }
 
-// Fill one Z (interleaved complex, element (m,n) at Z[2*(m*ldr + n*ldc)]).
-// useMoments: near/self records from the cached moments (momP terms).
-efie_status fill_z(efie_ctx* c, const efie_opts* o, bool useMoments, const Freq* dfr, double* Z, size_t ldr,
-                   size_t ldc) {
+// Fill one Z (interleaved complex, element (m,n) at Z[2*(m*ld + n)], which
+// for the symmetric Z is also the column-major matrix).  useMoments:
+// near/self records from the cached moments (momP terms).  With Z32
+// non-null the quadrature model also writes the complex64 copy
+// (Z32[2*(m*ld + n)]) and sets *wrote32.
+efie_status fill_z(efie_ctx* c, const efie_opts* o, bool useMoments, const Freq* dfr, double* Z, size_t ld,
+                   float* Z32 = nullptr, bool* wrote32 = nullptr) {
     hipStream_t s = c->stream;
     Prof pf = make_prof(c);
+    if (wrote32) *wrote32 = false;
     if (o->model == EFIE_MODEL_CENTROID) {
         pf.begin(EFIE_K_OTHER);
         HIPCHK(launch_centroid(c->dCPos.as<double>(), c->dCF.as<double>(), c->dCDv.as<double>(),
                                c->dCW.as<double>(), c->dCT.as<int>(), c->dCAsoft.as<double>(), c->Ne, dfr,
-                               Z, ldr, ldc, s));
+                               Z, ld, 1, s));
         pf.end(EFIE_K_OTHER);
         return ok();
     }
@@ -883,8 +887,9 @@ efie_status fill_z(efie_ctx* c, const efie_opts* o, bool useMoments, const Freq*
                               c->dNearMat.as<uint8_t>(), dfr, recs, s));
     pf.end(EFIE_K_REGULAR);
     pf.begin(EFIE_K_COMBINE);
-    HIPCHK(launch_combine(recs, c->dSideTri.as<int>(), c->dSide.as<Side>(), c->Ne, dfr, Z, ldr, ldc, s));
+    HIPCHK(launch_combine(recs, c->dSideTri.as<int>(), c->dSide.as<Side>(), c->Ne, dfr, Z, ld, Z32, ld, s));
     pf.end(EFIE_K_COMBINE);
+    if (wrote32) *wrote32 = Z32 != nullptr;
     return ok();
 }
 
@@ -1007,7 +1012,7 @@ efie_status ensure_solver(efie_ctx* c, const efie_opts& o, int n, int lda, int B
 // copy, then FP64 iterative refinement against the FP64 matrix with
 // right-hand sides V + b*vStride.  Returns where the FP64 solutions are.
 efie_status solve_batch(efie_ctx* c, const efie_opts& o, int n, int lda, int nb, const double* V, size_t vStride,
-                        const Prof& pf, const double** X, size_t* xStride, size_t* xinc) {
+                        const Prof& pf, const double** X, size_t* xStride, size_t* xinc, bool z32Ready = false) {
     hipStream_t s = c->stream;
     const size_t zElems = (size_t)n * lda;
     double* Zb = c->dZ.as<double>();
@@ -1026,9 +1031,11 @@ efie_status solve_batch(efie_ctx* c, const efie_opts& o, int n, int lda, int nb,
         return ok();
     }
     float* Z32 = c->dZ32.as<float>();
-    pf.begin(EFIE_K_OTHER);
-    HIPCHK(launch_to_f32(Zb, Z32, zElems * nb, s));
-    pf.end(EFIE_K_OTHER);
+    if (!z32Ready) {
+        pf.begin(EFIE_K_OTHER);
+        HIPCHK(launch_to_f32(Zb, Z32, zElems * nb, s));
+        pf.end(EFIE_K_OTHER);
+    }
     HIPCHK(lu_factor<float>(Z32, 2 * zElems, n, lda, nb, wk, s, &pf));
     HIPCHK(hipEventRecord(c->ev[3], s));
     pf.begin(EFIE_K_TRSV);
@@ -1147,7 +1154,7 @@ efie_status efie_assemble_zv(efie_ctx* c, double k_re, double k_im, double omega
     HIPCHK(c->dFreq.ensure(sizeof(Freq)));
     HIPCHK(hipMemcpyAsync(c->dFreq.p, &fr, sizeof fr, hipMemcpyHostToDevice, s));
     HIPCHK(c->dZ.ensure(sizeof(double) * 2 * (size_t)Ne * Ne));
-    st = fill_z(c, &o, mom, c->dFreq.as<Freq>(), c->dZ.as<double>(), 1, (size_t)Ne);
+    st = fill_z(c, &o, mom, c->dFreq.as<Freq>(), c->dZ.as<double>(), (size_t)Ne);
     if (st) return st;
     HIPCHK(hipMemcpyAsync(Z, c->dZ.p, sizeof(double) * 2 * (size_t)Ne * Ne, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(Vrhs, c->dV.p, sizeof(double) * 2 * (size_t)Ne, hipMemcpyDeviceToHost, s));
@@ -1222,18 +1229,21 @@ efie_status efie_sweep(efie_ctx* c, const double* freqs, int Nf, double mu, doub
         HIPCHK(hipMemcpyAsync(c->dFreq.p, fr.data(), sizeof(Freq) * nb, hipMemcpyHostToDevice, s));
         HIPCHK(hipEventRecord(c->ev[1], s));
         double* Zb = c->dZ.as<double>();
+        float* Z32b = o.precision == EFIE_PREC_MIXED ? c->dZ32.as<float>() : nullptr;
+        bool have32 = Z32b != nullptr;
         for (int i = 0; i < nb; ++i) {
-            double* Zi = Zb + 2 * zElems * i;
-            st = fill_z(c, &o, mom, c->dFreq.as<Freq>() + i, Zi, (size_t)lda, 1);
+            bool w32 = false;
+            st = fill_z(c, &o, mom, c->dFreq.as<Freq>() + i, Zb + 2 * zElems * i, (size_t)lda,
+                        Z32b ? Z32b + 2 * zElems * i : nullptr, &w32);
             if (st) return st;
-            // rhs into column n
-            HIPCHK(hipMemcpy2DAsync(Zi + 2 * n, sizeof(double) * 2 * lda, c->dV.p, sizeof(double) * 2,
-                                    sizeof(double) * 2, n, hipMemcpyDeviceToDevice, s));
+            have32 = have32 && w32;
         }
+        // rhs into column n (of the complex64 copy too when the fill wrote it)
+        HIPCHK(launch_set_rhs(c->dV.as<double>(), Zb, have32 ? Z32b : nullptr, n, lda, zElems, nb, s));
         HIPCHK(hipEventRecord(c->ev[2], s));
         const double* X = nullptr;
         size_t xStride = 0, xinc = 0;
-        st = solve_batch(c, o, n, lda, nb, c->dV.as<double>(), 0, pf, &X, &xStride, &xinc);
+        st = solve_batch(c, o, n, lda, nb, c->dV.as<double>(), 0, pf, &X, &xStride, &xinc, have32);
         if (st) return st;
         pf.begin(EFIE_K_OTHER);
         HIPCHK(launch_epilogue(X, xStride, xinc, c->dWl.as<double>(), n, patchEdge - 1, groundEdge - 1,

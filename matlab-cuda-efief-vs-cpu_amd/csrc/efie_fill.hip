@@ -534,60 +534,110 @@ __global__ __launch_bounds__(256) void k_poly_records(const double* __restrict__
 
 // ------------------------------------------------------------ combine ---
 
-template <bool ROWFAST>
-__global__ __launch_bounds__(256) void k_combine(const Rec* __restrict__ recs,
-                                                 const int* __restrict__ sideTri,
-                                                 const Side* __restrict__ side, int Ne,
-                                                 const Freq* __restrict__ fr,
-                                                 double* __restrict__ Z, size_t ldr, size_t ldc) {
-    // 16 x 16 tile; the fast thread index runs along the unit-stride output axis
-    int a = threadIdx.x & 15, b = threadIdx.x >> 4;
-    int m = blockIdx.x * 16 + (ROWFAST ? a : b);
-    int n = blockIdx.y * 16 + (ROWFAST ? b : a);
-    if (m >= Ne || n >= Ne) return;
-    const double ar = fr->ar, ai = fr->ai, br = fr->br, bi = fr->bi;
-    double zr = 0.0, zi = 0.0;
+// Z is complex symmetric (Z(m,n) = Z(n,m): the four side pairs of (n,m)
+// are those of (m,n) with the roles swapped), so one block computes a
+// 16 x 16 tile (I, J) with I <= J and writes it twice, the mirror through
+// an LDS transpose so both stores are row-contiguous.  Thread (r, c) owns
+// entry (16I + r, 16J + c).  All table and record loads are unconditional
+// (indices clamped, missing triangles masked afterwards) so they are in
+// flight together.  With Z32 set, the complex64 copy of the mixed-precision
+// solve is written in the same pass.
+__device__ __forceinline__ void combine_entry(const Rec* __restrict__ recs, const int* __restrict__ sideTri,
+                                              const Side* __restrict__ side, int m, int n, const Freq& f,
+                                              double& zr, double& zi) {
+    int tm[2], tn[2];
+    Side sm[2], sn[2];
 #pragma unroll
-    for (int sa = 0; sa < 2; ++sa) {
-        const int t1 = sideTri[2 * m + sa];
-        if (t1 < 0) continue;
-        const Side sm = side[2 * m + sa];
+    for (int k = 0; k < 2; ++k) {
+        tm[k] = sideTri[2 * m + k];
+        tn[k] = sideTri[2 * n + k];
+        sm[k] = side[2 * m + k];
+        sn[k] = side[2 * n + k];
+    }
+    double2 R[2][2][8];
+#pragma unroll
+    for (int sa = 0; sa < 2; ++sa)
 #pragma unroll
         for (int sb = 0; sb < 2; ++sb) {
-            const int t2 = sideTri[2 * n + sb];
-            if (t2 < 0) continue;
-            const Side sn = side[2 * n + sb];
-            const bool sw = t1 > t2;
-            const double2* rp =
-                reinterpret_cast<const double2*>(recs[sw ? rec_index(t2, t1) : rec_index(t1, t2)].v);
-            const double2 s0 = rp[0], s2 = rp[7];
-            double2 s1x = rp[1], s1y = rp[2], s1z = rp[3];
-            double2 s1px = rp[4], s1py = rp[5], s1pz = rp[6];
+            const int t1 = tm[sa] < 0 ? 0 : tm[sa], t2 = tn[sb] < 0 ? 0 : tn[sb];
+            const double2* rp = reinterpret_cast<const double2*>(
+                recs[t1 > t2 ? rec_index(t2, t1) : rec_index(t1, t2)].v);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) R[sa][sb][q] = rp[q];
+        }
+    zr = 0.0;
+    zi = 0.0;
+#pragma unroll
+    for (int sa = 0; sa < 2; ++sa)
+#pragma unroll
+        for (int sb = 0; sb < 2; ++sb) {
+            const bool sw = tm[sa] > tn[sb];
+            const double2 s0 = R[sa][sb][0], s2 = R[sa][sb][7];
+            double2 s1x = R[sa][sb][1], s1y = R[sa][sb][2], s1z = R[sa][sb][3];
+            double2 s1px = R[sa][sb][4], s1py = R[sa][sb][5], s1pz = R[sa][sb][6];
             if (sw) {
                 double2 t;
                 t = s1x; s1x = s1px; s1px = t;
                 t = s1y; s1y = s1py; s1py = t;
                 t = s1z; s1z = s1pz; s1pz = t;
             }
-            const double uu = sm.ux * sn.ux + sm.uy * sn.uy + sm.uz * sn.uz;
+            const Side a = sm[sa], c = sn[sb];
+            const double uu = a.ux * c.ux + a.uy * c.uy + a.uz * c.uz;
             // S2 - S1.un - S1'.um + S0 (um.un)
-            double ivr = s2.x - (s1x.x * sn.ux + s1y.x * sn.uy + s1z.x * sn.uz) -
-                         (s1px.x * sm.ux + s1py.x * sm.uy + s1pz.x * sm.uz) + s0.x * uu;
-            double ivi = s2.y - (s1x.y * sn.ux + s1y.y * sn.uy + s1z.y * sn.uz) -
-                         (s1px.y * sm.ux + s1py.y * sm.uy + s1pz.y * sm.uz) + s0.y * uu;
-            const double cc = sm.c * sn.c;
+            double ivr = s2.x - (s1x.x * c.ux + s1y.x * c.uy + s1z.x * c.uz) -
+                         (s1px.x * a.ux + s1py.x * a.uy + s1pz.x * a.uz) + s0.x * uu;
+            double ivi = s2.y - (s1x.y * c.ux + s1y.y * c.uy + s1z.y * c.uz) -
+                         (s1px.y * a.ux + s1py.y * a.uy + s1pz.y * a.uz) + s0.y * uu;
+            const double cc = a.c * c.c;
             ivr *= cc;
             ivi *= cc;
             const double isr = 4.0 * cc * s0.x, isi = 4.0 * cc * s0.y;
-            // Zmn + 1j*omega*mu*Ivec + (1/(1j*omega*eps))*Isca
-            zr += ar * ivr - ai * ivi;
-            zi += ar * ivi + ai * ivr;
-            zr += br * isr - bi * isi;
-            zi += br * isi + bi * isr;
+            // Zmn + 1j*omega*mu*Ivec + (1/(1j*omega*eps))*Isca; a missing
+            // triangle (index -1) contributes nothing
+            const bool ok = tm[sa] >= 0 && tn[sb] >= 0;
+            const double dr = (f.ar * ivr - f.ai * ivi) + (f.br * isr - f.bi * isi);
+            const double di = (f.ar * ivi + f.ai * ivr) + (f.br * isi + f.bi * isr);
+            zr += ok ? dr : 0.0;
+            zi += ok ? di : 0.0;
         }
+}
+
+__global__ __launch_bounds__(256) void k_combine(const Rec* __restrict__ recs, const int* __restrict__ sideTri,
+                                                 const Side* __restrict__ side, int Ne,
+                                                 const Freq* __restrict__ fr, double* __restrict__ Z, size_t ld,
+                                                 float* __restrict__ Z32, size_t ld32, int T, int nblk) {
+    __shared__ double2 tile[16][17];
+    // XCD-aware order (each XCD takes a contiguous run of tiles), then the
+    // linear tile index -> (I <= J)
+    long long id = blockIdx.x;
+    {
+        const int q = nblk >> 3, rr = nblk & 7, x = (int)(id & 7), loc = (int)(id >> 3);
+        id = x < rr ? (long long)x * (q + 1) + loc : (long long)rr * (q + 1) + (long long)(x - rr) * q + loc;
     }
-    double* o = Z + 2 * ((size_t)m * ldr + (size_t)n * ldc);
-    *reinterpret_cast<double2*>(o) = make_double2(zr, zi);
+    int J = (int)((sqrt(8.0 * (double)id + 1.0) - 1.0) * 0.5);
+    while ((long long)(J + 1) * (J + 2) / 2 <= id) ++J;
+    while ((long long)J * (J + 1) / 2 > id) --J;
+    const int I = (int)(id - (long long)J * (J + 1) / 2);
+    (void)T;
+    const int r = threadIdx.x >> 4, c = threadIdx.x & 15;
+    const int m = 16 * I + r, n = 16 * J + c;
+    const Freq f = *fr;
+    double zr, zi;
+    combine_entry(recs, sideTri, side, m < Ne ? m : Ne - 1, n < Ne ? n : Ne - 1, f, zr, zi);
+    if (m < Ne && n < Ne) {
+        *reinterpret_cast<double2*>(Z + 2 * ((size_t)m * ld + n)) = make_double2(zr, zi);
+        if (Z32) *reinterpret_cast<float2*>(Z32 + 2 * ((size_t)m * ld32 + n)) = make_float2((float)zr, (float)zi);
+    }
+    if (I == J) return;  // block-uniform: the diagonal tile has no mirror
+    tile[r][c] = make_double2(zr, zi);
+    __syncthreads();
+    // mirror: thread (r, c) writes entry (16J + r, 16I + c) = tile[c][r]
+    const int mm = 16 * J + r, nn = 16 * I + c;
+    if (mm < Ne && nn < Ne) {
+        const double2 v = tile[c][r];
+        *reinterpret_cast<double2*>(Z + 2 * ((size_t)mm * ld + nn)) = v;
+        if (Z32) *reinterpret_cast<float2*>(Z32 + 2 * ((size_t)mm * ld32 + nn)) = make_float2((float)v.x, (float)v.y);
+    }
 }
 
 // ----------------------------------------------- reference centroid fill --
@@ -830,12 +880,11 @@ hipError_t launch_poly_records(const double* Qn, const int2* list, int nNear, co
 }
 
 hipError_t launch_combine(const Rec* recs, const int* sideTri, const Side* side, int Ne,
-                          const Freq* fr, double* Z, size_t ldr, size_t ldc, hipStream_t s) {
-    dim3 g((Ne + 15) / 16, (Ne + 15) / 16), blk(256);
-    if (ldr == 1)
-        hipLaunchKernelGGL(k_combine<true>, g, blk, 0, s, recs, sideTri, side, Ne, fr, Z, ldr, ldc);
-    else
-        hipLaunchKernelGGL(k_combine<false>, g, blk, 0, s, recs, sideTri, side, Ne, fr, Z, ldr, ldc);
+                          const Freq* fr, double* Z, size_t ld, float* Z32, size_t ld32, hipStream_t s) {
+    const int T = (Ne + 15) / 16;
+    const long long nb = (long long)T * (T + 1) / 2;
+    hipLaunchKernelGGL(k_combine, dim3((unsigned)nb), dim3(256), 0, s, recs, sideTri, side, Ne, fr, Z, ld, Z32,
+                       ld32, T, (int)nb);
     return hipGetLastError();
 }
 

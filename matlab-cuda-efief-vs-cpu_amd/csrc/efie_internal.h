@@ -85,9 +85,11 @@ hipError_t launch_poly_records(const double* Qn, const int2* list, int nNear, co
                                int Nt, int P, const Freq* fr, Rec* recs, hipStream_t s);
 
 // Z(m,n) = sum over the four side pairs of alpha*Ivec + beta*Isca.
-// Output element (m,n) goes to Z[m*ldr + n*ldc] (interleaved complex).
+// Output element (m,n) goes to Z[m*ld + n] (interleaved complex; Z is
+// symmetric, so this is also the column-major matrix with leading
+// dimension ld); with Z32 non-null also a complex64 copy at Z32[m*ld32 + n].
 hipError_t launch_combine(const Rec* recs, const int* sideTri, const Side* side, int Ne,
-                          const Freq* fr, double* Z, size_t ldr, size_t ldc, hipStream_t s);
+                          const Freq* fr, double* Z, size_t ld, float* Z32, size_t ld32, hipStream_t s);
 
 // Reference centroid model (kernel_Z_centroid).
 hipError_t launch_centroid(const double* pos, const double* f, const double* dv,
@@ -151,6 +153,10 @@ hipError_t lu_make_perm(const int* ipiv, int n, int batch, int* perm, hipStream_
 
 // ------------------------------------------------ mixed precision -------
 hipError_t launch_to_f32(const double* A64, float* A32, size_t complexElems, hipStream_t s);
+// Column n of the nb matrices (complex elements zElems apart, row-major,
+// leading dimension lda) := V; also in Z32 when non-null.
+hipError_t launch_set_rhs(const double* V, double* Z, float* Z32, int n, int lda, size_t zElems, int nb,
+                          hipStream_t s);
 hipError_t launch_col_to_vec(const float* A32, size_t strideA, int n, int lda, double* X, int batch,
                              hipStream_t s);
 // R = V - A64 X per batch element (V[b] at V + b*vStride), rn2[b] += ||R||^2

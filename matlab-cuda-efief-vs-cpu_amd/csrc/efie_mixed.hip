@@ -172,4 +172,23 @@ hipError_t launch_fill_dbl(double* p, double v, int n, hipStream_t s) {
     return hipGetLastError();
 }
 
+
+// Column n of every batch matrix (FP64, and the complex64 copy when Z32 is
+// non-null) set to the right-hand side V.
+__global__ __launch_bounds__(256) void k_set_rhs(const double* __restrict__ V, double* __restrict__ Z,
+                                                 float* __restrict__ Z32, int n, int lda, size_t zElems) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const size_t e = (size_t)blockIdx.y * zElems + (size_t)i * lda + n;
+    const double2 v = reinterpret_cast<const double2*>(V)[i];
+    reinterpret_cast<double2*>(Z)[e] = v;
+    if (Z32) reinterpret_cast<float2*>(Z32)[e] = make_float2((float)v.x, (float)v.y);
+}
+
+hipError_t launch_set_rhs(const double* V, double* Z, float* Z32, int n, int lda, size_t zElems, int nb,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_set_rhs, dim3((n + 255) / 256, nb), dim3(256), 0, s, V, Z, Z32, n, lda, zElems);
+    return hipGetLastError();
+}
+
 }  // namespace efie
