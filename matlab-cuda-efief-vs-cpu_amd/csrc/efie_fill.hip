@@ -42,9 +42,16 @@ __device__ __forceinline__ double rsqrt64(double x) {
 // with R0 the centroid distance, so each point pair needs only cos/sin of
 // the small phase kr (R - R0) when its bound dmax (largest |R - R0| of the
 // pair) keeps kr dmax <= 0.75; otherwise the full reduction is used.
+// Green's function of one triangle pair, factored about the centroid
+// distance R0: G(R) = e0 * e^{-jk(R - R0)} / R with e0 = e^{-jk R0}/(4 pi).
+// The inner sums accumulate only e^{-jk(R - R0)}/R; the caller multiplies
+// the finished moments by e0 once per pair.  |R - R0| <= dmax (the sum of
+// the two triangles' point radii) bounds the residual phase, which picks
+// the cheapest series that stays below 1e-17 relative (mode 0: |k dmax| <=
+// 0.2, mode 1: <= 0.75) or the full range-reduced sine/cosine (mode 2).
 struct PairGreen {
     double R0, e0r, e0i, kr, ki;
-    bool small;
+    int mode;
 };
 
 __device__ __forceinline__ PairGreen pair_green(double kr, double ki, double R0, double dmax) {
@@ -52,40 +59,39 @@ __device__ __forceinline__ PairGreen pair_green(double kr, double ki, double R0,
     g.R0 = R0;
     g.kr = kr;
     g.ki = ki;
-    g.small = fabs(kr) * dmax <= 0.75 && fabs(ki) * dmax <= 1e-3;
+    const double x = fabs(kr) * dmax;
+    const bool lossOk = fabs(ki) * dmax <= 1e-3;
+    g.mode = (x <= 0.2 && lossOk) ? 0 : (x <= 0.75 && lossOk) ? 1 : 2;
     double s, c;
     sincos_cw(kr * R0, &s, &c);
-    const double amp = (ki == 0.0) ? 1.0 : exp(ki * R0);
+    const double amp = ((ki == 0.0) ? 1.0 : exp(ki * R0)) * (1.0 / (4.0 * kPi));
     g.e0r = amp * c;
     g.e0i = -amp * s;
     return g;
 }
 
-// G(R) for R^2 = R2 (R2 > 0).
-__device__ __forceinline__ void green_eval(const PairGreen& g, double R2, double& gr, double& gi) {
+// e^{-jk(R - R0)} / R for R^2 = R2 (R2 > 0).
+__device__ __forceinline__ void green_rel(const PairGreen& g, double R2, double& gr, double& gi) {
     const double invR = rsqrt64(R2);
-    const double R = R2 * invR;
-    const double f = invR * (1.0 / (4.0 * kPi));
-    if (g.small) {
-        const double d = R - g.R0;
+    const double d = R2 * invR - g.R0;
+    if (g.mode <= 1) {
         double s, c;
-        sincos_small(g.kr * d, &s, &c);
-        // e^{ki d} = 1 + ki d + (ki d)^2/2 + (ki d)^3/6 for |ki d| <= 1e-3
-        double a = 1.0;
+        if (g.mode == 0) sincos_small_short(g.kr * d, &s, &c);
+        else sincos_small(g.kr * d, &s, &c);
+        // e^{ki d} = 1 + ki d + (ki d)^2/2 + ... for |ki d| <= 1e-3
+        double af = invR;
         if (g.ki != 0.0) {
             const double z = g.ki * d;
-            a = 1.0 + z * (1.0 + z * (0.5 + z * (1.0 / 6.0 + z * (1.0 / 24.0))));
+            af *= 1.0 + z * (1.0 + z * (0.5 + z * (1.0 / 6.0 + z * (1.0 / 24.0))));
         }
-        // (e0r + j e0i)(c - j s) a f
-        const double af = a * f;
-        gr = (g.e0r * c + g.e0i * s) * af;
-        gi = (g.e0i * c - g.e0r * s) * af;
+        gr = c * af;
+        gi = -s * af;
     } else {
         double s, c;
-        sincos_cw(g.kr * R, &s, &c);
-        const double amp = (g.ki == 0.0) ? 1.0 : exp(g.ki * R);
-        gr = c * amp * f;
-        gi = -s * amp * f;
+        sincos_cw(g.kr * d, &s, &c);
+        const double af = (g.ki == 0.0) ? invR : exp(g.ki * d) * invR;
+        gr = c * af;
+        gi = -s * af;
     }
 }
 
@@ -180,6 +186,17 @@ __device__ __forceinline__ void acc_fold(Acc& a, double w, double x, double y, d
     a.s2i += w * (x * hxi + y * hyi + z * hzi);
 }
 
+// Multiply the eight complex moments by e0 (the pair's reference factor).
+__device__ __forceinline__ void acc_scale(Acc& a, double er, double ei) {
+    double* v = &a.s0r;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const double x = v[2 * k], y = v[2 * k + 1];
+        v[2 * k] = x * er - y * ei;
+        v[2 * k + 1] = x * ei + y * er;
+    }
+}
+
 __device__ __forceinline__ void acc_store(Rec* r, const Acc& a) {
     double2* p = reinterpret_cast<double2*>(r->v);
     p[0] = make_double2(a.s0r, a.s0i);
@@ -224,16 +241,17 @@ __global__ __launch_bounds__(256) void k_rec_regular(const Pt* __restrict__ pts,
     const int t1 = ti * kRegTile + lane;
     const double kr = fr->kr, ki = fr->ki;
 
-    Pt p1[NQ];
-    TriGeo g1 = {0, 0, 0, 0};
-    if (t1 < Nt) {
-        g1 = tg[t1];
-#pragma unroll
-        for (int i = 0; i < NQ; ++i) p1[i] = pts[(size_t)t1 * NQ + i];
-    } else {
-#pragma unroll
-        for (int i = 0; i < NQ; ++i) p1[i] = Pt{0, 0, 0, 0};
+    // the block's 64 x NQ test points are staged in LDS (a register array
+    // indexed by the rolled loop below would live in scratch)
+    __shared__ Pt sp[kRegTile * NQ];
+    for (int e = threadIdx.x; e < kRegTile * NQ; e += 256) {
+        const int t = ti * kRegTile + e / NQ;
+        sp[e] = pts[(size_t)(t < Nt ? t : Nt - 1) * NQ + e % NQ];
     }
+    __syncthreads();
+    const int t1c = t1 < Nt ? t1 : Nt - 1;
+    const TriGeo g1 = tg[t1c];
+    const Pt* p1 = sp + lane * NQ;
 
     constexpr int perWave = kRegTile / 4;
     for (int q = 0; q < perWave; ++q) {
@@ -249,7 +267,8 @@ __global__ __launch_bounds__(256) void k_rec_regular(const Pt* __restrict__ pts,
         acc_zero(a);
 #pragma unroll 1
         for (int i = 0; i < NQ; ++i) {
-            const double xi = p1[i].x + ox, yi = p1[i].y + oy, zi = p1[i].z + oz;
+            const Pt pi = p1[i];
+            const double xi = pi.x + ox, yi = pi.y + oy, zi = pi.z + oz;
             double gr = 0, gi = 0, hxr = 0, hxi = 0, hyr = 0, hyi = 0, hzr = 0, hzi = 0;
 #pragma unroll 1
             for (int j = 0; j < NQ; ++j) {
@@ -258,15 +277,16 @@ __global__ __launch_bounds__(256) void k_rec_regular(const Pt* __restrict__ pts,
                 double R2 = dx * dx + dy * dy + dz * dz;
                 R2 = R2 < 1e-30 ? 1e-30 : R2;  // R < 1e-15 -> 1e-15
                 double Gr, Gi;
-                green_eval(pg, R2, Gr, Gi);
+                green_rel(pg, R2, Gr, Gi);
                 const double wr = q2.w * Gr, wi = q2.w * Gi;
                 gr += wr; gi += wi;
                 hxr += wr * q2.x; hxi += wi * q2.x;
                 hyr += wr * q2.y; hyi += wi * q2.y;
                 hzr += wr * q2.z; hzi += wi * q2.z;
             }
-            acc_fold(a, p1[i].w, p1[i].x, p1[i].y, p1[i].z, gr, gi, hxr, hxi, hyr, hyi, hzr, hzi);
+            acc_fold(a, pi.w, pi.x, pi.y, pi.z, gr, gi, hxr, hxi, hyr, hyi, hzr, hzi);
         }
+        acc_scale(a, pg.e0r, pg.e0i);
         acc_store(recs + rec_index(t1, t2), a);
     }
 }
@@ -316,7 +336,7 @@ __device__ void subdiv_pair(const Pt* __restrict__ P1, const Pt* __restrict__ P2
                     R2 = R2 > 1e-30 ? R2 : 1e-30;  // max(R, 1e-15), singular.m:48-49
                 }
                 double Gr, Gi;
-                green_eval(pg, R2, Gr, Gi);
+                green_rel(pg, R2, Gr, Gi);
                 double wr = q2.w * Gr, wi = q2.w * Gi;
                 gr += wr; gi += wi;
                 hxr += wr * q2.x; hxi += wi * q2.x;
@@ -326,6 +346,7 @@ __device__ void subdiv_pair(const Pt* __restrict__ P1, const Pt* __restrict__ P2
             acc_fold(a, pi.w, pi.x, pi.y, pi.z, gr, gi, hxr, hxi, hyr, hyi, hzr, hzi);
         }
     }
+    acc_scale(a, pg.e0r, pg.e0i);
     double* v = &a.s0r;
     double res[16];
 #pragma unroll

@@ -83,4 +83,22 @@ EFIE_HD inline void sincos_small(double t, double* s, double* c) {
     *s = t + t * t2 * ps;
 }
 
+// cos t and sin t for |t| <= 0.2 by their Taylor series through t^10 and
+// t^11 (truncation below 1e-17).
+EFIE_HD inline void sincos_small_short(double t, double* s, double* c) {
+    const double t2 = t * t;
+    double pc = -2.755731922398589e-07;
+    pc = pc * t2 + 2.48015873015873e-05;
+    pc = pc * t2 - 0.001388888888888889;
+    pc = pc * t2 + 0.041666666666666664;
+    pc = pc * t2 - 0.5;
+    *c = pc * t2 + 1.0;
+    double ps = -2.505210838544172e-08;
+    ps = ps * t2 + 2.7557319223985893e-06;
+    ps = ps * t2 - 0.0001984126984126984;
+    ps = ps * t2 + 0.008333333333333333;
+    ps = ps * t2 - 0.16666666666666666;
+    *s = t + t * t2 * ps;
+}
+
 }  // namespace efie

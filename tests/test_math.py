@@ -36,3 +36,22 @@ def test_sincos_error_bound():
     s0, c0 = _sincos(np.array([0.0, -0.3, 0.3]))
     assert s0[0] == 0.0 and c0[0] == 1.0
     assert s0[1] == -s0[2] and c0[1] == c0[2]
+
+
+def test_small_argument_series():
+    """The Taylor series of the residual phase (efie_math.h sincos_small on
+    |t| <= 0.75, sincos_small_short on |t| <= 0.2) against numpy.  A series
+    missing its last term would be off by more than 1e-14 at the end of its
+    range, far outside these bounds."""
+    L = C.CDLL(LIB)
+    dp = C.POINTER(C.c_double)
+    rng = np.random.default_rng(1)
+    for which, lim in ((0, 0.75), (1, 0.2)):
+        x = np.concatenate([rng.uniform(-lim, lim, 50000), [-lim, lim, 0.0, 1e-12]])
+        s = np.zeros_like(x)
+        c = np.zeros_like(x)
+        L.efie_math_sincos_small(x.ctypes.data_as(dp), x.size, s.ctypes.data_as(dp), c.ctypes.data_as(dp),
+                                 which)
+        assert np.max(np.abs(c - np.cos(x))) <= 3e-16
+        nz = np.abs(x) > 0
+        assert np.max(np.abs(s[nz] / np.sin(x[nz]) - 1)) <= 4e-16
