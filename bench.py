@@ -63,7 +63,7 @@ def timed(ctx, o, fr, pe, ge, spread, steps, warmup, name):
     return dt, out
 
 
-WB = 4 * NB           # outer block of the LU (trailing update with K = WB)
+WB = 8 * NB           # outer block of the LU (trailing update with K = WB)
 
 
 def lu_gemm_flops(n):

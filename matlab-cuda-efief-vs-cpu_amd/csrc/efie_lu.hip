@@ -614,7 +614,7 @@ struct gemm_vec<double> {
     typedef double2 v;
 };
 
-template <typename T, int BM, int BN>
+template <typename T, int BM, int BN, bool M3 = false>
 __global__ __launch_bounds__(256) void k_gemm(T* A, size_t strideA, int lda, int r0, int r1, int c0,
                                               int c1, int k0, int K, int gx, int gy, int nblk) {
     typedef typename mf<T>::v4 v4;
@@ -649,13 +649,17 @@ __global__ __launch_bounds__(256) void k_gemm(T* A, size_t strideA, int lda, int
     const int wr = (wv >> 1) * (BM / 2), wc = (wv & 1) * (BN / 2);
     const int li = lane & 15, lg = lane >> 4;
 
-    v4 accR[TM][TN], accI[TM][TN];
+    // 4M: accR = sum ar br - ai bi, accI = sum ar bi + ai br.
+    // 3M: accR = sum ar br, accI = sum ai bi, acc3 = sum (ar+ai)(br+bi);
+    // re = accR - accI, im = acc3 - accR - accI.
+    v4 accR[TM][TN], accI[TM][TN], acc3[M3 ? TM : 1][M3 ? TN : 1];
 #pragma unroll
     for (int x = 0; x < TM; ++x)
 #pragma unroll
         for (int y = 0; y < TN; ++y) {
             accR[x][y] = v4{0, 0, 0, 0};
             accI[x][y] = v4{0, 0, 0, 0};
+            if constexpr (M3) acc3[x][y] = v4{0, 0, 0, 0};
         }
 
     // Operand loads are unconditional, from addresses clamped into the
@@ -733,13 +737,23 @@ __global__ __launch_bounds__(256) void k_gemm(T* A, size_t strideA, int lda, int
             for (int j = 0; j < J; ++j) af[j] = As[buf][wr + 16 * x + li][J * lg + j];
 #pragma unroll
             for (int j = 0; j < J; ++j) {
-                const T ni = -af[j].i;
+                if constexpr (M3) {
+                    const T as = af[j].r + af[j].i;
 #pragma unroll
-                for (int y = 0; y < TN; ++y) {
-                    accR[x][y] = mf<T>::mfma(af[j].r, bf[y][j].r, accR[x][y]);
-                    accR[x][y] = mf<T>::mfma(ni, bf[y][j].i, accR[x][y]);
-                    accI[x][y] = mf<T>::mfma(af[j].r, bf[y][j].i, accI[x][y]);
-                    accI[x][y] = mf<T>::mfma(af[j].i, bf[y][j].r, accI[x][y]);
+                    for (int y = 0; y < TN; ++y) {
+                        accR[x][y] = mf<T>::mfma(af[j].r, bf[y][j].r, accR[x][y]);
+                        accI[x][y] = mf<T>::mfma(af[j].i, bf[y][j].i, accI[x][y]);
+                        acc3[x][y] = mf<T>::mfma(as, bf[y][j].r + bf[y][j].i, acc3[x][y]);
+                    }
+                } else {
+                    const T ni = -af[j].i;
+#pragma unroll
+                    for (int y = 0; y < TN; ++y) {
+                        accR[x][y] = mf<T>::mfma(af[j].r, bf[y][j].r, accR[x][y]);
+                        accR[x][y] = mf<T>::mfma(ni, bf[y][j].i, accR[x][y]);
+                        accI[x][y] = mf<T>::mfma(af[j].r, bf[y][j].i, accI[x][y]);
+                        accI[x][y] = mf<T>::mfma(af[j].i, bf[y][j].r, accI[x][y]);
+                    }
                 }
             }
         }
@@ -766,9 +780,14 @@ __global__ __launch_bounds__(256) void k_gemm(T* A, size_t strideA, int lda, int
             for (int v = 0; v < 4; ++v) {
                 const int row = rb + wr + 16 * x + mf<T>::row(lg, v);
                 const int col = cb + wc + 16 * y + li;
+                T pr = accR[x][y][v], pi = accI[x][y][v];
+                if constexpr (M3) {
+                    const T t1 = accR[x][y][v], t2 = accI[x][y][v];
+                    pr = t1 - t2;
+                    pi = acc3[x][y][v] - t1 - t2;
+                }
                 if (row < r1 && col < c1)
-                    st(Ab + 2 * ((size_t)row * lda + col),
-                       cpx<T>{cv[y][v].r - accR[x][y][v], cv[y][v].i - accI[x][y][v]});
+                    st(Ab + 2 * ((size_t)row * lda + col), cpx<T>{cv[y][v].r - pr, cv[y][v].i - pi});
             }
     }
 }
@@ -1010,20 +1029,23 @@ int lu_tree_nodes(int n) { return tree_geom(n).nNodes; }
 int lu_cand_stride() { return NB + 1; }
 int lu_swap_stride() { return lu_swap_stride_c; }
 
-constexpr int WB = 4 * NB;  // outer block: the trailing update runs with K = WB
+constexpr int WB = 8 * NB;  // outer block: the trailing update runs with K = WB
 
 template <typename T, int BM, int BN>
 static void gemm_launch(T* A, size_t strideA, int lda, int batch, int r0, int r1, int c0, int c1, int k0, int K,
                         hipStream_t s) {
     const int gx = (c1 - c0 + BN - 1) / BN, gy = (r1 - r0 + BM - 1) / BM;
     const int nblk = gx * gy * batch;
-    hipLaunchKernelGGL((k_gemm<T, BM, BN>), dim3(nblk), dim3(256), 0, s, A, strideA, lda, r0, r1, c0, c1, k0, K,
-                       gx, gy, nblk);
+    hipLaunchKernelGGL((k_gemm<T, BM, BN, true>), dim3(nblk), dim3(256), 0, s, A, strideA, lda, r0, r1, c0, c1, k0,
+                       K, gx, gy, nblk);
 }
 
 // 64 x 64 tiles: four blocks per CU keep 16 waves resident, which hides
 // the operand and C latency better than larger tiles at lower occupancy
-// (measured at n = 6009: 112 TFLOP/s FP32 against 96 for 128 x 128).
+// (measured at n = 6009, K = 128: 112 TFLOP/s FP32 against 96 for
+// 128 x 128).  The complex product uses three real products (3M:
+// ar br, ai bi, (ar+ai)(br+bi)); with K = 256 the FP32 update runs at
+// 145 TFLOP/s counted as 8 M N K.
 // Returns the number of kernels launched (0 for an empty update).
 template <typename T>
 static int gemm(T* A, size_t strideA, int lda, int batch, int r0, int r1, int c0, int c1, int k0, int K,
