@@ -171,6 +171,13 @@ std::vector<RawE> sorted_raw(const int* F, int Nt) {
 struct efie_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // second stream: the next batch of a sweep is filled on it while the
+    // current batch is factored on `stream`
+    hipStream_t fillStream = nullptr;
+    hipEvent_t evFill[2] = {}, evFillStart[2] = {};
+    // third stream: LU look-ahead (the next block's panels)
+    hipStream_t panelStream = nullptr;
+    hipEvent_t evPanel = nullptr, evAhead = nullptr;
     hipEvent_t ev[8] = {};
 
     // host copies of the tables (column-major, 1-based as given)
@@ -420,8 +427,17 @@ efie_status efie_create(efie_ctx** out, int device) {
         delete c;
         return fail(EFIE_E_HIP, "efie:hip", "hipStreamCreate: %s", hipGetErrorString(e));
     }
-    for (auto& ev : c->ev) {
-        e = hipEventCreate(&ev);
+    for (hipStream_t* st : {&c->fillStream, &c->panelStream}) {
+        e = hipStreamCreate(st);
+        if (e != hipSuccess) {
+            efie_destroy(c);
+            return fail(EFIE_E_HIP, "efie:hip", "hipStreamCreate: %s", hipGetErrorString(e));
+        }
+    }
+    for (hipEvent_t* ev : {&c->ev[0], &c->ev[1], &c->ev[2], &c->ev[3], &c->ev[4], &c->ev[5], &c->ev[6], &c->ev[7],
+                           &c->evFill[0], &c->evFill[1], &c->evFillStart[0], &c->evFillStart[1], &c->evPanel,
+                           &c->evAhead}) {
+        e = hipEventCreate(ev);
         if (e != hipSuccess) {
             efie_destroy(c);
             return fail(EFIE_E_HIP, "efie:hip", "hipEventCreate: %s", hipGetErrorString(e));
@@ -435,9 +451,15 @@ void efie_destroy(efie_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->fillStream) (void)hipStreamSynchronize(c->fillStream);
+    if (c->panelStream) (void)hipStreamSynchronize(c->panelStream);
     for (auto& ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
+    for (hipEvent_t ev : {c->evFill[0], c->evFill[1], c->evFillStart[0], c->evFillStart[1], c->evPanel, c->evAhead})
+        if (ev) (void)hipEventDestroy(ev);
     for (auto& ev : c->pool) (void)hipEventDestroy(ev);
+    if (c->fillStream) (void)hipStreamDestroy(c->fillStream);
+    if (c->panelStream) (void)hipStreamDestroy(c->panelStream);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;  // DBuf destructors free device memory
 }
@@ -851,8 +873,8 @@ Freq make_freq(std::complex<double> k, double omega, double mu, std::complex<dou
 // non-null the quadrature model also writes the complex64 copy
 // (Z32[2*(m*ld + n)]) and sets *wrote32.
 efie_status fill_z(efie_ctx* c, const efie_opts* o, bool useMoments, const Freq* dfr, double* Z, size_t ld,
-                   float* Z32 = nullptr, bool* wrote32 = nullptr) {
-    hipStream_t s = c->stream;
+                   float* Z32 = nullptr, bool* wrote32 = nullptr, hipStream_t s = nullptr) {
+    if (!s) s = c->stream;
     Prof pf = make_prof(c);
     if (wrote32) *wrote32 = false;
     if (o->model == EFIE_MODEL_CENTROID) {
@@ -944,13 +966,16 @@ efie_status choose_near(efie_ctx* c, const efie_opts* o, double kmax, efie_stats
 
 // Frequencies factored together: latency-bound steps of the LU (pivot
 // selection, panel solves, triangular solves) cost per launch, so a batch
-// holds up to 32 matrices, limited to 60% of the free device memory and
-// split evenly over the sweep (201 frequencies -> 7 batches of 29).
+// holds up to 32 matrices, limited so two batches fit in 60% of the free
+// device memory, and split evenly over the sweep (201 frequencies -> 7
+// batches of 29).
 int batch_size(const efie_opts& o, size_t zElems, int Nf) {
     int B = o.batch > 0 ? o.batch : 32;
     size_t freeB = 0, totB = 0;
     if (hipMemGetInfo(&freeB, &totB) == hipSuccess) {
-        const size_t per = zElems * (o.precision == EFIE_PREC_MIXED ? 24 : 16);
+        // two batches of matrices are resident while one is filled and the
+        // other factored
+        const size_t per = 2 * zElems * (o.precision == EFIE_PREC_MIXED ? 24 : 16);
         const size_t cap = (size_t)(0.6 * (double)freeB) / std::max<size_t>(per, 1);
         if (cap < (size_t)B) B = (int)std::max<size_t>(1, cap);
     }
@@ -962,12 +987,12 @@ int batch_size(const efie_opts& o, size_t zElems, int Nf) {
 }
 
 // Device buffers of the batched solver for n unknowns and B matrices.
-efie_status ensure_solver(efie_ctx* c, const efie_opts& o, int n, int lda, int B) {
+efie_status ensure_solver(efie_ctx* c, const efie_opts& o, int n, int lda, int B, int nbuf = 1) {
     const size_t zElems = (size_t)n * lda;
     hipStream_t s = c->stream;
-    HIPCHK(c->dZ.ensure(sizeof(double) * 2 * zElems * B));
+    HIPCHK(c->dZ.ensure(sizeof(double) * 2 * zElems * B * nbuf));
     if (o.precision == EFIE_PREC_MIXED) {
-        HIPCHK(c->dZ32.ensure(sizeof(float) * 2 * zElems * B));
+        HIPCHK(c->dZ32.ensure(sizeof(float) * 2 * zElems * B * nbuf));
         HIPCHK(c->dPerm.ensure(sizeof(int) * (size_t)B * n));
         HIPCHK(c->dX.ensure(sizeof(double) * 2 * (size_t)B * n));
         HIPCHK(c->dR.ensure(sizeof(double) * 2 * (size_t)B * n));
@@ -1012,12 +1037,15 @@ efie_status ensure_solver(efie_ctx* c, const efie_opts& o, int n, int lda, int B
 // copy, then FP64 iterative refinement against the FP64 matrix with
 // right-hand sides V + b*vStride.  Returns where the FP64 solutions are.
 efie_status solve_batch(efie_ctx* c, const efie_opts& o, int n, int lda, int nb, const double* V, size_t vStride,
-                        const Prof& pf, const double** X, size_t* xStride, size_t* xinc, bool z32Ready = false) {
+                        const Prof& pf, const double** X, size_t* xStride, size_t* xinc, bool z32Ready = false,
+                        double* Zbuf = nullptr, float* Z32buf = nullptr) {
     hipStream_t s = c->stream;
     const size_t zElems = (size_t)n * lda;
-    double* Zb = c->dZ.as<double>();
+    double* Zb = Zbuf ? Zbuf : c->dZ.as<double>();
+    // look-ahead on the panel stream, except when per-kernel profiling
+    // needs every kernel on one stream
     LuWork wk{c->dCand.as<int>(), c->dCnt.as<int>(), c->dIpiv.as<int>(), c->dInfo.as<int>(),
-              c->dSwp.as<int>()};
+              c->dSwp.as<int>(), c->prof ? nullptr : c->panelStream, c->evPanel, c->evAhead};
     HIPCHK(launch_fill_int(c->dInfo.as<int>(), 0, nb, s));
     if (o.precision != EFIE_PREC_MIXED) {
         HIPCHK(lu_factor<double>(Zb, 2 * zElems, n, lda, nb, wk, s, &pf));
@@ -1030,7 +1058,7 @@ efie_status solve_batch(efie_ctx* c, const efie_opts& o, int n, int lda, int nb,
         *xinc = 2 * (size_t)lda;
         return ok();
     }
-    float* Z32 = c->dZ32.as<float>();
+    float* Z32 = Z32buf ? Z32buf : c->dZ32.as<float>();
     if (!z32Ready) {
         pf.begin(EFIE_K_OTHER);
         HIPCHK(launch_to_f32(Zb, Z32, zElems * nb, s));
@@ -1204,10 +1232,16 @@ efie_status efie_sweep(efie_ctx* c, const double* freqs, int Nf, double mu, doub
     const int lda = ((n + 1) + 31) & ~31;  // rhs in column n
     const size_t zElems = (size_t)n * lda;
     const int B = batch_size(o, zElems, Nf);
+    const int nBatch = (Nf + B - 1) / B;
+    // batches are pipelined over two streams (fill of the next batch while
+    // this one is factored) unless per-kernel profiling asks for one stream
+    const bool overlap = nBatch > 1 && !c->prof;
+    const int nbuf = overlap ? 2 : 1;
     hipStream_t s = c->stream;
-    st = ensure_solver(c, o, n, lda, B);
+    hipStream_t sf = overlap ? c->fillStream : s;
+    st = ensure_solver(c, o, n, lda, B, nbuf);
     if (st) return st;
-    HIPCHK(c->dFreq.ensure(sizeof(Freq) * B));
+    HIPCHK(c->dFreq.ensure(sizeof(Freq) * B * nbuf));
     HIPCHK(c->dOut.ensure(sizeof(double) * 4 * (size_t)Nf));
     Prof pf = make_prof(c);
     if (o.precision == EFIE_PREC_MIXED) {
@@ -1220,30 +1254,55 @@ efie_status efie_sweep(efie_ctx* c, const double* freqs, int Nf, double mu, doub
         HIPCHK(launch_fill_dbl(c->dBn2.as<double>(), vn2, B, s));
     }
 
-    std::vector<Freq> fr(B);
-    double msFill = 0, msFac = 0, msSolve = 0;
-    HIPCHK(hipEventRecord(c->ev[0], s));
-    for (int f0 = 0; f0 < Nf; f0 += B) {
-        const int nb = std::min(B, Nf - f0);
-        for (int i = 0; i < nb; ++i) fr[i] = make_freq(ks[f0 + i], 2.0 * kPi * freqs[f0 + i], mu, eps);
-        HIPCHK(hipMemcpyAsync(c->dFreq.p, fr.data(), sizeof(Freq) * nb, hipMemcpyHostToDevice, s));
-        HIPCHK(hipEventRecord(c->ev[1], s));
-        double* Zb = c->dZ.as<double>();
-        float* Z32b = o.precision == EFIE_PREC_MIXED ? c->dZ32.as<float>() : nullptr;
-        bool have32 = Z32b != nullptr;
+    std::vector<Freq> fr[2] = {std::vector<Freq>(B), std::vector<Freq>(B)};
+    bool have32[2] = {false, false};
+    const bool mixed = o.precision == EFIE_PREC_MIXED;
+    auto zbuf = [&](int buf) { return c->dZ.as<double>() + 2 * zElems * (size_t)B * buf; };
+    auto z32buf = [&](int buf) { return mixed ? c->dZ32.as<float>() + 2 * zElems * (size_t)B * buf : nullptr; };
+    // fill batch bi into buffer bi % nbuf on stream sf (Z, its complex64
+    // copy and the rhs column); evFill[buf] marks its end
+    auto enqueue_fill = [&](int bi) -> efie_status {
+        const int f0 = bi * B, nb = std::min(B, Nf - f0), buf = bi % nbuf;
+        for (int i = 0; i < nb; ++i) fr[buf][i] = make_freq(ks[f0 + i], 2.0 * kPi * freqs[f0 + i], mu, eps);
+        Freq* dfr = c->dFreq.as<Freq>() + (size_t)B * buf;
+        HIPCHK(hipEventRecord(c->evFillStart[buf], sf));
+        HIPCHK(hipMemcpyAsync(dfr, fr[buf].data(), sizeof(Freq) * nb, hipMemcpyHostToDevice, sf));
+        double* Zb = zbuf(buf);
+        float* Z32b = z32buf(buf);
+        bool h32 = Z32b != nullptr;
         for (int i = 0; i < nb; ++i) {
             bool w32 = false;
-            st = fill_z(c, &o, mom, c->dFreq.as<Freq>() + i, Zb + 2 * zElems * i, (size_t)lda,
-                        Z32b ? Z32b + 2 * zElems * i : nullptr, &w32);
-            if (st) return st;
-            have32 = have32 && w32;
+            efie_status e = fill_z(c, &o, mom, dfr + i, Zb + 2 * zElems * i, (size_t)lda,
+                                   Z32b ? Z32b + 2 * zElems * i : nullptr, &w32, sf);
+            if (e) return e;
+            h32 = h32 && w32;
         }
         // rhs into column n (of the complex64 copy too when the fill wrote it)
-        HIPCHK(launch_set_rhs(c->dV.as<double>(), Zb, have32 ? Z32b : nullptr, n, lda, zElems, nb, s));
+        HIPCHK(launch_set_rhs(c->dV.as<double>(), Zb, h32 ? Z32b : nullptr, n, lda, zElems, nb, sf));
+        HIPCHK(hipEventRecord(c->evFill[buf], sf));
+        have32[buf] = h32;
+        return ok();
+    };
+
+    double msFill = 0, msFac = 0, msSolve = 0;
+    HIPCHK(hipEventRecord(c->ev[0], s));
+    if (overlap) HIPCHK(hipStreamWaitEvent(sf, c->ev[0], 0));
+    st = enqueue_fill(0);
+    if (st) return st;
+    for (int bi = 0; bi < nBatch; ++bi) {
+        const int f0 = bi * B, nb = std::min(B, Nf - f0), buf = bi % nbuf;
+        // the other buffer was last used by batch bi-1, whose solve has
+        // completed (finish_batch synchronised the solve stream)
+        if (overlap && bi + 1 < nBatch) {
+            st = enqueue_fill(bi + 1);
+            if (st) return st;
+        }
+        HIPCHK(hipStreamWaitEvent(s, c->evFill[buf], 0));
         HIPCHK(hipEventRecord(c->ev[2], s));
         const double* X = nullptr;
         size_t xStride = 0, xinc = 0;
-        st = solve_batch(c, o, n, lda, nb, c->dV.as<double>(), 0, pf, &X, &xStride, &xinc, have32);
+        st = solve_batch(c, o, n, lda, nb, c->dV.as<double>(), 0, pf, &X, &xStride, &xinc, have32[buf], zbuf(buf),
+                         z32buf(buf));
         if (st) return st;
         pf.begin(EFIE_K_OTHER);
         HIPCHK(launch_epilogue(X, xStride, xinc, c->dWl.as<double>(), n, patchEdge - 1, groundEdge - 1,
@@ -1253,12 +1312,16 @@ efie_status efie_sweep(efie_ctx* c, const double* freqs, int Nf, double mu, doub
         st = finish_batch(c, o, nb, S, freqs + f0);
         if (st) return st;
         float t1, t2, t3;
-        HIPCHK(hipEventElapsedTime(&t1, c->ev[1], c->ev[2]));
+        HIPCHK(hipEventElapsedTime(&t1, c->evFillStart[buf], c->evFill[buf]));
         HIPCHK(hipEventElapsedTime(&t2, c->ev[2], c->ev[3]));
         HIPCHK(hipEventElapsedTime(&t3, c->ev[3], c->ev[4]));
         msFill += t1;
         msFac += t2;
         msSolve += t3;
+        if (!overlap && bi + 1 < nBatch) {
+            st = enqueue_fill(bi + 1);
+            if (st) return st;
+        }
     }
     HIPCHK(hipEventRecord(c->ev[5], s));
     std::vector<double> out(4 * (size_t)Nf);

@@ -117,7 +117,12 @@ struct LuWork {
     int* counters;  // tree arrival counters
     int* ipiv;      // [batch][n] 1-based LAPACK-style pivots
     int* info;      // [batch] first zero pivot (1-based) or 0
-    int* swaps;     // [batch][lu_swap_stride()]: row moves of the current panel
+    int* swaps;     // [batch][lu_swap_stride()]: row moves of the current block's panels
+    // look-ahead: the next outer block's panels are factored on `panel`
+    // while the trailing update runs on the main stream (panel may be the
+    // main stream itself); the two events order them
+    hipStream_t panel;
+    hipEvent_t evPanel, evAhead;
 };
 // ints per batch element of LuWork::swaps
 int lu_swap_stride();

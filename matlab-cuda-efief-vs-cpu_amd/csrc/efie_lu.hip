@@ -30,6 +30,8 @@
 namespace efie {
 
 constexpr int NB = 32;    // panel width
+constexpr int WB = 8 * NB;  // outer block: the trailing update runs with K = WB
+constexpr int kSwapSlots = WB / NB;  // interchange lists kept per block
 constexpr int lu_swap_stride_c = 1 + 4 * NB;  // count + up to 2 NB (dst, src) pairs
 
 
@@ -294,7 +296,7 @@ __device__ __forceinline__ void gepp_wave(const T* Ab, int lda, int j0, int w, i
 template <typename T>
 __global__ __launch_bounds__(64) void k_tslu(T* A, size_t strideA, int n, int lda, int j0, int w,
                                              TreeGeom g, int nodeStride, int* cand, int* counters,
-                                             int* ipiv, int* info, int* swaps) {
+                                             int* ipiv, int* info, int* swaps, int slot) {
     __shared__ int s_go;
     __shared__ cpx<T> prow[NB];
     const int b = blockIdx.y;
@@ -344,7 +346,7 @@ __global__ __launch_bounds__(64) void k_tslu(T* A, size_t strideA, int n, int ld
             // The same interchanges as a gather list for k_laswp: position
             // j0+lane now holds row topRow (lane < keep), position dPos holds
             // dRow (lane < nd); unchanged positions are left out.
-            int* sw = swaps + (size_t)b * lu_swap_stride_c;
+            int* sw = swaps + ((size_t)b * kSwapSlots + slot) * lu_swap_stride_c;
             const bool e1 = lane < keep && topRow != j0 + lane;
             const bool e2 = lane < nd && dRow != dPos;
             const unsigned long long m1 = __ballot(e1), m2 = __ballot(e2);
@@ -400,22 +402,14 @@ __global__ __launch_bounds__(64) void k_tslu(T* A, size_t strideA, int n, int ld
 }
 
 // ------------------------------------------------------------ laswp -------
-// Applies the panel's interchanges to every column (rhs included) as one
-// gather: the tournament root lists the m <= 2 NB positions whose row
-// changes (destination, source).  Each thread owns one column, loads all
-// m sources first (unconditionally, unused slots read row j0) and then
-// stores, so no round trip waits on another.
+// Row interchanges as gathers: the tournament root lists the m <= 2 NB
+// positions whose row changes (destination, source).  Each thread owns one
+// column, loads all m sources first (unconditionally, unused slots read
+// row j0) and then stores, so no round trip waits on another.
 template <typename T>
-__global__ __launch_bounds__(256) void k_laswp(T* A, size_t strideA, int n, int lda, const int* swaps) {
-    __shared__ int L[lu_swap_stride_c];
-    const int b = blockIdx.y;
-    const int c = blockIdx.x * 256 + threadIdx.x;
-    const int* sw = swaps + (size_t)b * lu_swap_stride_c;
-    if (threadIdx.x < lu_swap_stride_c) L[threadIdx.x] = sw[threadIdx.x];
-    __syncthreads();
+__device__ __forceinline__ void swap_column(T* Ab, int lda, int c, const int* L) {
     const int m = L[0];
-    if (c > n || m == 0) return;
-    T* Ab = A + (size_t)b * strideA;
+    if (m == 0) return;
     const int fallback = L[2];
     cpx<T> v[2 * NB];
 #pragma unroll
@@ -426,6 +420,38 @@ __global__ __launch_bounds__(256) void k_laswp(T* A, size_t strideA, int n, int 
 #pragma unroll
     for (int e = 0; e < 2 * NB; ++e)
         if (e < m) st(Ab + 2 * ((size_t)L[1 + 2 * e] * lda + c), v[e]);
+}
+
+// One panel's interchanges (list `slot`) on columns [clo, chi).
+template <typename T>
+__global__ __launch_bounds__(256) void k_laswp(T* A, size_t strideA, int lda, const int* swaps, int slot, int clo,
+                                               int chi) {
+    __shared__ int L[lu_swap_stride_c];
+    const int b = blockIdx.y;
+    const int* sw = swaps + ((size_t)b * kSwapSlots + slot) * lu_swap_stride_c;
+    if (threadIdx.x < lu_swap_stride_c) L[threadIdx.x] = sw[threadIdx.x];
+    __syncthreads();
+    const int c = clo + blockIdx.x * 256 + threadIdx.x;
+    if (c >= chi) return;
+    swap_column(A + (size_t)b * strideA, lda, c, L);
+}
+
+// The interchanges of nslots panels, in order, on columns [lo1, hi1) and
+// [lo2, hi2) (the columns outside the block that produced them).
+template <typename T>
+__global__ __launch_bounds__(256) void k_laswp_rest(T* A, size_t strideA, int lda, const int* swaps, int nslots,
+                                                    int lo1, int hi1, int lo2, int hi2) {
+    __shared__ int L[kSwapSlots * lu_swap_stride_c];
+    const int b = blockIdx.y;
+    const int* sw = swaps + (size_t)b * kSwapSlots * lu_swap_stride_c;
+    for (int i = threadIdx.x; i < nslots * lu_swap_stride_c; i += 256) L[i] = sw[i];
+    __syncthreads();
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int n1 = hi1 - lo1;
+    const int c = t < n1 ? lo1 + t : lo2 + (t - n1);
+    if (t >= n1 + (hi2 - lo2)) return;
+    T* Ab = A + (size_t)b * strideA;
+    for (int k = 0; k < nslots; ++k) swap_column(Ab, lda, c, L + k * lu_swap_stride_c);
 }
 
 // ----------------------------------------------------- panel finish ------
@@ -1027,9 +1053,8 @@ __global__ __launch_bounds__(256) void k_make_perm(const int* ipiv, int n, int* 
 
 int lu_tree_nodes(int n) { return tree_geom(n).nNodes; }
 int lu_cand_stride() { return NB + 1; }
-int lu_swap_stride() { return lu_swap_stride_c; }
+int lu_swap_stride() { return kSwapSlots * lu_swap_stride_c; }
 
-constexpr int WB = 8 * NB;  // outer block: the trailing update runs with K = WB
 
 template <typename T, int BM, int BN>
 static void gemm_launch(T* A, size_t strideA, int lda, int batch, int r0, int r1, int c0, int c1, int k0, int K,
@@ -1063,45 +1088,79 @@ static int panel_cols(T* A, size_t strideA, int lda, int batch, int d, int w, in
     return 1;
 }
 
-// Blocked right-looking LU.  Outer blocks of WB columns are factored as NB
-// wide sub-panels (tournament pivoting, interchanges over the whole row,
-// diagonal block, L21, and a right-looking update restricted to the block);
-// then the block's U12 = L^-1 A12 over the remaining columns (incl. the rhs
-// column n) and one trailing update A22 -= L21 U12 with K = WB.
+// Panels of the outer block [j0, jend): for every NB-wide sub-panel the
+// tournament, its interchanges on the block's own columns, the diagonal
+// block, L21, the block-local U12 and the update of the rest of the block.
+template <typename T>
+static void panel_phase(T* A, size_t strideA, int n, int lda, int batch, const LuWork& wk, int j0, int jend,
+                        int nodeStride, hipStream_t sp, const Prof* pf) {
+    for (int c0 = j0; c0 < jend; c0 += NB) {
+        const int w = jend - c0 < NB ? jend - c0 : NB;
+        const int slot = (c0 - j0) / NB;
+        TreeGeom g = tree_geom(n - c0);
+        pf->begin(5);
+        hipLaunchKernelGGL(k_tslu<T>, dim3(g.levelCount[0], batch), dim3(64), 0, sp, A, strideA, n, lda, c0, w, g,
+                           nodeStride, wk.cand, wk.counters, wk.ipiv, wk.info, wk.swaps, slot);
+        pf->end(5);
+        pf->begin(6);
+        hipLaunchKernelGGL(k_laswp<T>, dim3((jend - j0 + 255) / 256, batch), dim3(256), 0, sp, A, strideA, lda,
+                           wk.swaps, slot, j0, jend);
+        pf->end(6);
+        pf->begin(7);
+        int np = 1;
+        hipLaunchKernelGGL(k_panel_diag<T>, dim3(batch), dim3(64), 0, sp, A, strideA, lda, c0, w, wk.info);
+        if (c0 + w < n) {
+            hipLaunchKernelGGL(k_panel_rows<T>, dim3((n - c0 - w + 255) / 256, batch), dim3(256), 0, sp, A, strideA,
+                               lda, c0, w, c0 + w, n);
+            ++np;
+        }
+        np += panel_cols<T>(A, strideA, lda, batch, c0, w, c0 + w, jend, sp);
+        pf->end(7, np);
+        pf->begin(8);
+        pf->end(8, gemm<T>(A, strideA, lda, batch, c0 + w, n, c0 + w, jend, c0, w, sp));
+    }
+}
+
+// Blocked right-looking LU with look-ahead.  Outer blocks of WB columns are
+// factored as NB-wide sub-panels (panel_phase) whose interchanges touch
+// only the block's columns at first.  For every block: apply its
+// interchanges to all other columns (incl. the rhs column n), U12 = L^-1
+// A12 over the columns right of it, then the trailing update A22 -= L21
+// U12 with K = WB, split in two: the next block's columns first, after
+// which that block's panels run on the panel stream while the main stream
+// updates the remaining columns.  The two never touch the same columns:
+// the panels stay inside their block, and their interchanges reach the
+// other columns only after the update that reads those rows has finished.
 template <typename T>
 hipError_t lu_factor(T* A, size_t strideA, int n, int lda, int batch, LuWork wk, hipStream_t s,
                      const Prof* pf) {
     Prof none;
     if (!pf) pf = &none;
+    hipStream_t sp = wk.panel ? wk.panel : s;
     // candidate and counter slots per batch element are sized for the first
     // (largest) panel; every counter is back at zero when a launch ends
     const int nodeStride = tree_geom(n).nNodes;
+    if (sp != s) {
+        // the panel stream starts after everything already queued on s
+        hipEventRecord(wk.evAhead, s);
+        hipStreamWaitEvent(sp, wk.evAhead, 0);
+    }
+    panel_phase<T>(A, strideA, n, lda, batch, wk, 0, n < WB ? n : WB, nodeStride, sp, pf);
+    if (sp != s) {
+        hipEventRecord(wk.evPanel, sp);
+    }
     for (int j0 = 0; j0 < n; j0 += WB) {
         const int wb = n - j0 < WB ? n - j0 : WB;
         const int jend = j0 + wb;
-        for (int c0 = j0; c0 < jend; c0 += NB) {
-            const int w = jend - c0 < NB ? jend - c0 : NB;
-            TreeGeom g = tree_geom(n - c0);
-            pf->begin(5);
-            hipLaunchKernelGGL(k_tslu<T>, dim3(g.levelCount[0], batch), dim3(64), 0, s, A, strideA, n, lda, c0, w,
-                               g, nodeStride, wk.cand, wk.counters, wk.ipiv, wk.info, wk.swaps);
-            pf->end(5);
+        const int jnext = n - jend < WB ? n : jend + WB;
+        if (sp != s) hipStreamWaitEvent(s, wk.evPanel, 0);
+        // this block's interchanges on the columns outside it
+        const int nrest = j0 + (n + 1 - jend);
+        if (nrest > 0) {
             pf->begin(6);
-            hipLaunchKernelGGL(k_laswp<T>, dim3((n + 1 + 255) / 256, batch), dim3(256), 0, s, A, strideA, n, lda,
-                               wk.swaps);
+            hipLaunchKernelGGL(k_laswp_rest<T>, dim3((nrest + 255) / 256, batch), dim3(256), 0, s, A, strideA, lda,
+                               wk.swaps, (wb + NB - 1) / NB, 0, j0, jend, n + 1);
             pf->end(6);
-            pf->begin(7);
-            int np = 1;
-            hipLaunchKernelGGL(k_panel_diag<T>, dim3(batch), dim3(64), 0, s, A, strideA, lda, c0, w, wk.info);
-            if (c0 + w < n) {
-                hipLaunchKernelGGL(k_panel_rows<T>, dim3((n - c0 - w + 255) / 256, batch), dim3(256), 0, s, A,
-                                   strideA, lda, c0, w, c0 + w, n);
-                ++np;
-            }
-            np += panel_cols<T>(A, strideA, lda, batch, c0, w, c0 + w, jend, s);
-            pf->end(7, np);
-            pf->begin(8);
-            pf->end(8, gemm<T>(A, strideA, lda, batch, c0 + w, n, c0 + w, jend, c0, w, s));
         }
         // U12 of the block rows over columns jend..n (rhs included)
         for (int c0 = j0; c0 < jend; c0 += NB) {
@@ -1111,8 +1170,19 @@ hipError_t lu_factor(T* A, size_t strideA, int n, int lda, int batch, LuWork wk,
             pf->begin(8);
             pf->end(8, gemm<T>(A, strideA, lda, batch, c0 + w, jend, jend, n + 1, c0, w, s));
         }
+        if (jend >= n) break;
+        // trailing update: the next block's columns, then (while its panels
+        // are factored on the panel stream) the remaining columns
         pf->begin(8);
-        pf->end(8, gemm<T>(A, strideA, lda, batch, jend, n, jend, n + 1, j0, wb, s));
+        pf->end(8, gemm<T>(A, strideA, lda, batch, jend, n, jend, jnext, j0, wb, s));
+        if (sp != s) {
+            hipEventRecord(wk.evAhead, s);
+            hipStreamWaitEvent(sp, wk.evAhead, 0);
+        }
+        panel_phase<T>(A, strideA, n, lda, batch, wk, jend, jnext, nodeStride, sp, pf);
+        if (sp != s) hipEventRecord(wk.evPanel, sp);
+        pf->begin(8);
+        pf->end(8, gemm<T>(A, strideA, lda, batch, jend, n, jnext, n + 1, j0, wb, s));
     }
     return hipGetLastError();
 }
