@@ -178,6 +178,14 @@ struct efie_ctx {
     // third stream: LU look-ahead (the next block's panels)
     hipStream_t panelStream = nullptr;
     hipEvent_t evPanel = nullptr, evAhead = nullptr;
+    // fourth stream: refinement / back substitution of a batch while the
+    // next batch is factored
+    hipStream_t irStream = nullptr;
+    hipEvent_t evLU[2] = {}, evLUStart[2] = {}, evIR[2] = {}, evIRStart[2] = {};
+    // pinned host slots for the per-batch results (info, iterations,
+    // residuals, wait-limit flag), one per buffer
+    void* hStat = nullptr;
+    size_t hStatBytes = 0;
     hipEvent_t ev[8] = {};
 
     // host copies of the tables (column-major, 1-based as given)
@@ -427,7 +435,7 @@ efie_status efie_create(efie_ctx** out, int device) {
         delete c;
         return fail(EFIE_E_HIP, "efie:hip", "hipStreamCreate: %s", hipGetErrorString(e));
     }
-    for (hipStream_t* st : {&c->fillStream, &c->panelStream}) {
+    for (hipStream_t* st : {&c->fillStream, &c->panelStream, &c->irStream}) {
         e = hipStreamCreate(st);
         if (e != hipSuccess) {
             efie_destroy(c);
@@ -436,7 +444,8 @@ efie_status efie_create(efie_ctx** out, int device) {
     }
     for (hipEvent_t* ev : {&c->ev[0], &c->ev[1], &c->ev[2], &c->ev[3], &c->ev[4], &c->ev[5], &c->ev[6], &c->ev[7],
                            &c->evFill[0], &c->evFill[1], &c->evFillStart[0], &c->evFillStart[1], &c->evPanel,
-                           &c->evAhead}) {
+                           &c->evAhead, &c->evLU[0], &c->evLU[1], &c->evLUStart[0], &c->evLUStart[1], &c->evIR[0],
+                           &c->evIR[1], &c->evIRStart[0], &c->evIRStart[1]}) {
         e = hipEventCreate(ev);
         if (e != hipSuccess) {
             efie_destroy(c);
@@ -453,13 +462,18 @@ void efie_destroy(efie_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->fillStream) (void)hipStreamSynchronize(c->fillStream);
     if (c->panelStream) (void)hipStreamSynchronize(c->panelStream);
+    if (c->irStream) (void)hipStreamSynchronize(c->irStream);
     for (auto& ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
-    for (hipEvent_t ev : {c->evFill[0], c->evFill[1], c->evFillStart[0], c->evFillStart[1], c->evPanel, c->evAhead})
+    for (hipEvent_t ev : {c->evFill[0], c->evFill[1], c->evFillStart[0], c->evFillStart[1], c->evPanel, c->evAhead,
+                          c->evLU[0], c->evLU[1], c->evLUStart[0], c->evLUStart[1], c->evIR[0], c->evIR[1],
+                          c->evIRStart[0], c->evIRStart[1]})
         if (ev) (void)hipEventDestroy(ev);
     for (auto& ev : c->pool) (void)hipEventDestroy(ev);
+    if (c->hStat) (void)hipHostFree(c->hStat);
     if (c->fillStream) (void)hipStreamDestroy(c->fillStream);
     if (c->panelStream) (void)hipStreamDestroy(c->panelStream);
+    if (c->irStream) (void)hipStreamDestroy(c->irStream);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;  // DBuf destructors free device memory
 }
@@ -986,6 +1000,10 @@ int batch_size(const efie_opts& o, size_t zElems, int Nf) {
     return std::max(1, std::min(B, std::max(Nf, 1)));
 }
 
+// Host bytes of one batch's results: info and iterations (int), final
+// relative residuals (double), the wait-limit flag (int).
+size_t stat_slot_bytes(int B) { return (size_t)B * (2 * sizeof(int) + sizeof(double)) + 64; }
+
 // Device buffers of the batched solver for n unknowns and B matrices.
 efie_status ensure_solver(efie_ctx* c, const efie_opts& o, int n, int lda, int B, int nbuf = 1) {
     const size_t zElems = (size_t)n * lda;
@@ -993,7 +1011,7 @@ efie_status ensure_solver(efie_ctx* c, const efie_opts& o, int n, int lda, int B
     HIPCHK(c->dZ.ensure(sizeof(double) * 2 * zElems * B * nbuf));
     if (o.precision == EFIE_PREC_MIXED) {
         HIPCHK(c->dZ32.ensure(sizeof(float) * 2 * zElems * B * nbuf));
-        HIPCHK(c->dPerm.ensure(sizeof(int) * (size_t)B * n));
+        HIPCHK(c->dPerm.ensure(sizeof(int) * (size_t)B * n * nbuf));
         HIPCHK(c->dX.ensure(sizeof(double) * 2 * (size_t)B * n));
         HIPCHK(c->dR.ensure(sizeof(double) * 2 * (size_t)B * n));
         HIPCHK(c->dR32.ensure(sizeof(float) * 2 * (size_t)B * n));
@@ -1011,7 +1029,15 @@ efie_status ensure_solver(efie_ctx* c, const efie_opts& o, int n, int lda, int B
         HIPCHK(launch_fill_int(c->dCnt.as<int>(), 0, (int)(c->dCnt.bytes / sizeof(int)), s));
     }
     HIPCHK(c->dIpiv.ensure(sizeof(int) * (size_t)B * n));
-    HIPCHK(c->dInfo.ensure(sizeof(int) * B));
+    HIPCHK(c->dInfo.ensure(sizeof(int) * B * nbuf));
+    const size_t statBytes = 2 * stat_slot_bytes(B);
+    if (c->hStatBytes < statBytes) {
+        if (c->hStat) HIPCHK(hipHostFree(c->hStat));
+        c->hStat = nullptr;
+        c->hStatBytes = 0;
+        HIPCHK(hipHostMalloc(&c->hStat, statBytes));
+        c->hStatBytes = statBytes;
+    }
     HIPCHK(c->dSwp.ensure(sizeof(int) * (size_t)B * lu_swap_stride()));
     // single-launch triangular solve: flags hold the epoch of the last
     // launch that finished each block, so they are zeroed only when new
@@ -1032,70 +1058,100 @@ efie_status ensure_solver(efie_ctx* c, const efie_opts& o, int n, int lda, int B
     return ok();
 }
 
-// Factor and solve the nb matrices in c->dZ (row-major, rhs in column n).
-// FP64: LU in place, back substitution on column n.  MIXED: LU of an FP32
-// copy, then FP64 iterative refinement against the FP64 matrix with
-// right-hand sides V + b*vStride.  Returns where the FP64 solutions are.
-efie_status solve_batch(efie_ctx* c, const efie_opts& o, int n, int lda, int nb, const double* V, size_t vStride,
-                        const Prof& pf, const double** X, size_t* xStride, size_t* xinc, bool z32Ready = false,
-                        double* Zbuf = nullptr, float* Z32buf = nullptr) {
+// Buffers of one batch: matrices (row-major, rhs in column n), the FP32
+// copy (mixed precision), first-zero-pivot flags and row permutations.
+struct BatchBufs {
+    double* Z;
+    float* Z32;
+    int* info;
+    int* perm;
+};
+
+BatchBufs batch_bufs(efie_ctx* c, const efie_opts& o, int n, int lda, int B, int buf) {
+    const size_t zElems = (size_t)n * lda;
+    BatchBufs bb;
+    bb.Z = c->dZ.as<double>() + 2 * zElems * (size_t)B * buf;
+    bb.Z32 = o.precision == EFIE_PREC_MIXED ? c->dZ32.as<float>() + 2 * zElems * (size_t)B * buf : nullptr;
+    bb.info = c->dInfo.as<int>() + (size_t)B * buf;
+    bb.perm = o.precision == EFIE_PREC_MIXED ? c->dPerm.as<int>() + (size_t)B * n * buf : nullptr;
+    return bb;
+}
+
+// LU of the nb matrices of a batch on the context stream (look-ahead
+// panels on the panel stream).  FP64: in place; MIXED: of the FP32 copy
+// (converted here unless the fill wrote it), and the row permutation of
+// its pivots for the refinement.
+efie_status factor_batch(efie_ctx* c, const efie_opts& o, int n, int lda, int nb, const Prof& pf, bool z32Ready,
+                         const BatchBufs& bb) {
     hipStream_t s = c->stream;
     const size_t zElems = (size_t)n * lda;
-    double* Zb = Zbuf ? Zbuf : c->dZ.as<double>();
     // look-ahead on the panel stream, except when per-kernel profiling
     // needs every kernel on one stream
-    LuWork wk{c->dCand.as<int>(), c->dCnt.as<int>(), c->dIpiv.as<int>(), c->dInfo.as<int>(),
+    LuWork wk{c->dCand.as<int>(), c->dCnt.as<int>(), c->dIpiv.as<int>(), bb.info,
               c->dSwp.as<int>(), c->prof ? nullptr : c->panelStream, c->evPanel, c->evAhead};
-    HIPCHK(launch_fill_int(c->dInfo.as<int>(), 0, nb, s));
+    HIPCHK(launch_fill_int(bb.info, 0, nb, s));
     if (o.precision != EFIE_PREC_MIXED) {
-        HIPCHK(lu_factor<double>(Zb, 2 * zElems, n, lda, nb, wk, s, &pf));
-        HIPCHK(hipEventRecord(c->ev[3], s));
+        HIPCHK(lu_factor<double>(bb.Z, 2 * zElems, n, lda, nb, wk, s, &pf));
+        return ok();
+    }
+    if (!z32Ready) {
+        pf.begin(EFIE_K_OTHER);
+        HIPCHK(launch_to_f32(bb.Z, bb.Z32, zElems * nb, s));
+        pf.end(EFIE_K_OTHER);
+    }
+    HIPCHK(lu_factor<float>(bb.Z32, 2 * zElems, n, lda, nb, wk, s, &pf));
+    pf.begin(EFIE_K_OTHER);
+    HIPCHK(lu_make_perm(c->dIpiv.as<int>(), n, nb, bb.perm, s));
+    pf.end(EFIE_K_OTHER);
+    return ok();
+}
+
+// Solution of a factored batch on stream si.  FP64: back substitution on
+// column n.  MIXED: back substitution in FP32, then FP64 iterative
+// refinement against the FP64 matrix with right-hand sides V + b*vStride.
+// Returns where the FP64 solutions are.
+efie_status refine_batch(efie_ctx* c, const efie_opts& o, int n, int lda, int nb, const double* V, size_t vStride,
+                         const Prof& pf, const BatchBufs& bb, hipStream_t si, const double** X, size_t* xStride,
+                         size_t* xinc) {
+    const size_t zElems = (size_t)n * lda;
+    if (o.precision != EFIE_PREC_MIXED) {
         pf.begin(EFIE_K_TRSV);
-        HIPCHK(lu_backsolve<double>(Zb, 2 * zElems, n, lda, nb, s, &c->tw));
+        HIPCHK(lu_backsolve<double>(bb.Z, 2 * zElems, n, lda, nb, si, &c->tw));
         pf.end(EFIE_K_TRSV);
-        *X = Zb + 2 * n;
+        *X = bb.Z + 2 * n;
         *xStride = 2 * zElems;
         *xinc = 2 * (size_t)lda;
         return ok();
     }
-    float* Z32 = Z32buf ? Z32buf : c->dZ32.as<float>();
-    if (!z32Ready) {
-        pf.begin(EFIE_K_OTHER);
-        HIPCHK(launch_to_f32(Zb, Z32, zElems * nb, s));
-        pf.end(EFIE_K_OTHER);
-    }
-    HIPCHK(lu_factor<float>(Z32, 2 * zElems, n, lda, nb, wk, s, &pf));
-    HIPCHK(hipEventRecord(c->ev[3], s));
+    float* Z32 = bb.Z32;
     pf.begin(EFIE_K_TRSV);
-    HIPCHK(lu_backsolve<float>(Z32, 2 * zElems, n, lda, nb, s, &c->tw));
+    HIPCHK(lu_backsolve<float>(Z32, 2 * zElems, n, lda, nb, si, &c->tw));
     pf.end(EFIE_K_TRSV);
     pf.begin(EFIE_K_OTHER);
-    HIPCHK(lu_make_perm(c->dIpiv.as<int>(), n, nb, c->dPerm.as<int>(), s));
-    HIPCHK(launch_col_to_vec(Z32, 2 * zElems, n, lda, c->dX.as<double>(), nb, s));
-    HIPCHK(launch_fill_int(c->dDone.as<int>(), 0, nb, s));
-    HIPCHK(launch_fill_int(c->dIters.as<int>(), 0, nb, s));
-    HIPCHK(launch_fill_dbl(c->dRn2.as<double>(), 0.0, nb, s));
+    HIPCHK(launch_col_to_vec(Z32, 2 * zElems, n, lda, c->dX.as<double>(), nb, si));
+    HIPCHK(launch_fill_int(c->dDone.as<int>(), 0, nb, si));
+    HIPCHK(launch_fill_int(c->dIters.as<int>(), 0, nb, si));
+    HIPCHK(launch_fill_dbl(c->dRn2.as<double>(), 0.0, nb, si));
     pf.end(EFIE_K_OTHER);
     for (int it = 0; it <= o.ir_max_iter; ++it) {
         pf.begin(EFIE_K_OTHER);
-        HIPCHK(launch_residual(Zb, 2 * zElems, n, lda, V, vStride, c->dX.as<double>(), c->dR.as<double>(),
-                               c->dRn2.as<double>(), c->dDone.as<int>(), nb, s));
+        HIPCHK(launch_residual(bb.Z, 2 * zElems, n, lda, V, vStride, c->dX.as<double>(), c->dR.as<double>(),
+                               c->dRn2.as<double>(), c->dDone.as<int>(), nb, si));
         HIPCHK(launch_ir_check(c->dRn2.as<double>(), c->dBn2.as<double>(), o.ir_tol, nb, c->dDone.as<int>(),
-                               c->dIters.as<int>(), c->dResid.as<double>(), it == 0, it == o.ir_max_iter, s));
+                               c->dIters.as<int>(), c->dResid.as<double>(), it == 0, it == o.ir_max_iter, si));
         pf.end(EFIE_K_OTHER);
         if (it == o.ir_max_iter) break;
         pf.begin(EFIE_K_OTHER);
-        HIPCHK(launch_perm_f32(c->dR.as<double>(), c->dPerm.as<int>(), n, nb, c->dR32.as<float>(),
-                               c->dDone.as<int>(), s));
+        HIPCHK(launch_perm_f32(c->dR.as<double>(), bb.perm, n, nb, c->dR32.as<float>(), c->dDone.as<int>(), si));
         pf.end(EFIE_K_OTHER);
         pf.begin(EFIE_K_TRSV);
         HIPCHK(lu_trsv<float>(Z32, 2 * zElems, n, lda, c->dR32.as<float>(), 2 * (size_t)n, 1, nb, true,
-                              c->dDone.as<int>(), s, &c->tw));
+                              c->dDone.as<int>(), si, &c->tw));
         HIPCHK(lu_trsv<float>(Z32, 2 * zElems, n, lda, c->dR32.as<float>(), 2 * (size_t)n, 1, nb, false,
-                              c->dDone.as<int>(), s, &c->tw));
+                              c->dDone.as<int>(), si, &c->tw));
         pf.end(EFIE_K_TRSV);
         pf.begin(EFIE_K_OTHER);
-        HIPCHK(launch_add_f32(c->dX.as<double>(), c->dR32.as<float>(), n, nb, c->dDone.as<int>(), s));
+        HIPCHK(launch_add_f32(c->dX.as<double>(), c->dR32.as<float>(), n, nb, c->dDone.as<int>(), si));
         pf.end(EFIE_K_OTHER);
     }
     *X = c->dX.as<double>();
@@ -1104,24 +1160,39 @@ efie_status solve_batch(efie_ctx* c, const efie_opts& o, int n, int lda, int nb,
     return ok();
 }
 
-// Synchronise after a batch: zero pivots become efie:singular, refinement
-// counters go into the stats, profile marks are summed.
-efie_status finish_batch(efie_ctx* c, const efie_opts& o, int nb, efie_stats* S, const double* freqs) {
-    hipStream_t s = c->stream;
-    std::vector<int> info(nb), iters(nb);
-    std::vector<double> resid(nb);
-    int werr = 0;
-    HIPCHK(hipMemcpyAsync(info.data(), c->dInfo.p, sizeof(int) * nb, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(&werr, c->dErr.p, sizeof(int), hipMemcpyDeviceToHost, s));
-    const bool mixed = o.precision == EFIE_PREC_MIXED;
-    if (mixed) {
-        HIPCHK(hipMemcpyAsync(iters.data(), c->dIters.p, sizeof(int) * nb, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipMemcpyAsync(resid.data(), c->dResid.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+// Queue the copy of a batch's results into host slot `slot` (pinned, so
+// the copy is asynchronous) on stream si.
+efie_status stage_stats(efie_ctx* c, const efie_opts& o, int nb, int slot, int B, const BatchBufs& bb,
+                        hipStream_t si) {
+    char* h = static_cast<char*>(c->hStat) + stat_slot_bytes(B) * slot;
+    int* info = reinterpret_cast<int*>(h);
+    int* iters = info + B;
+    double* resid = reinterpret_cast<double*>(h + 2 * sizeof(int) * (size_t)B);
+    int* werr = reinterpret_cast<int*>(h + (2 * sizeof(int) + sizeof(double)) * (size_t)B);
+    HIPCHK(hipMemcpyAsync(info, bb.info, sizeof(int) * nb, hipMemcpyDeviceToHost, si));
+    HIPCHK(hipMemcpyAsync(werr, c->dErr.p, sizeof(int), hipMemcpyDeviceToHost, si));
+    if (o.precision == EFIE_PREC_MIXED) {
+        HIPCHK(hipMemcpyAsync(iters, c->dIters.p, sizeof(int) * nb, hipMemcpyDeviceToHost, si));
+        HIPCHK(hipMemcpyAsync(resid, c->dResid.p, sizeof(double) * nb, hipMemcpyDeviceToHost, si));
     }
-    HIPCHK(hipStreamSynchronize(s));
+    return ok();
+}
+
+// After a batch's slot event has completed: zero pivots become
+// efie:singular, refinement counters go into the stats, profile marks are
+// summed.
+efie_status finish_batch(efie_ctx* c, const efie_opts& o, int nb, int slot, int B, efie_stats* S,
+                         const double* freqs) {
+    const char* h = static_cast<const char*>(c->hStat) + stat_slot_bytes(B) * slot;
+    const int* info = reinterpret_cast<const int*>(h);
+    const int* iters = info + B;
+    const double* resid = reinterpret_cast<const double*>(h + 2 * sizeof(int) * (size_t)B);
+    const int werr = *reinterpret_cast<const int*>(h + (2 * sizeof(int) + sizeof(double)) * (size_t)B);
+    const bool mixed = o.precision == EFIE_PREC_MIXED;
     prof_collect(c, S);
     if (werr) {
-        HIPCHK(launch_fill_int(c->dErr.as<int>(), 0, 1, s));
+        HIPCHK(launch_fill_int(c->dErr.as<int>(), 0, 1, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
         return fail(EFIE_E_HIP, "efie:hip", "triangular solve: a block wait exceeded its limit");
     }
     for (int i = 0; i < nb; ++i) {
@@ -1284,44 +1355,81 @@ efie_status efie_sweep(efie_ctx* c, const double* freqs, int Nf, double mu, doub
         return ok();
     };
 
+    // Pipeline over batches (overlap): fill on sf, LU on s (panels on the
+    // panel stream), back substitution / refinement / Zin on si.  Batch bi
+    // uses buffer bi % 2.  Once batch bi is queued, the host waits for the
+    // refinement of batch bi-1 (which ran beside the LU of batch bi), reads
+    // its results and queues the fill of batch bi+1 into its buffer.
+    hipStream_t si = overlap ? c->irStream : s;
     double msFill = 0, msFac = 0, msSolve = 0;
     HIPCHK(hipEventRecord(c->ev[0], s));
-    if (overlap) HIPCHK(hipStreamWaitEvent(sf, c->ev[0], 0));
+    if (overlap) {
+        HIPCHK(hipStreamWaitEvent(sf, c->ev[0], 0));
+        HIPCHK(hipStreamWaitEvent(si, c->ev[0], 0));
+    }
+    auto finish = [&](int bj) -> efie_status {
+        const int f0 = bj * B, nb = std::min(B, Nf - f0), buf = bj % nbuf;
+        HIPCHK(hipEventSynchronize(c->evIR[buf]));
+        float t1, t2, t3;
+        HIPCHK(hipEventElapsedTime(&t1, c->evFillStart[buf], c->evFill[buf]));
+        HIPCHK(hipEventElapsedTime(&t2, c->evLUStart[buf], c->evLU[buf]));
+        HIPCHK(hipEventElapsedTime(&t3, c->evIRStart[buf], c->evIR[buf]));
+        msFill += t1;
+        msFac += t2;
+        msSolve += t3;
+        return finish_batch(c, o, nb, buf, B, S, freqs + f0);
+    };
     st = enqueue_fill(0);
     if (st) return st;
     for (int bi = 0; bi < nBatch; ++bi) {
         const int f0 = bi * B, nb = std::min(B, Nf - f0), buf = bi % nbuf;
-        // the other buffer was last used by batch bi-1, whose solve has
-        // completed (finish_batch synchronised the solve stream)
-        if (overlap && bi + 1 < nBatch) {
-            st = enqueue_fill(bi + 1);
-            if (st) return st;
-        }
+        const BatchBufs bb = batch_bufs(c, o, n, lda, B, buf);
         HIPCHK(hipStreamWaitEvent(s, c->evFill[buf], 0));
-        HIPCHK(hipEventRecord(c->ev[2], s));
+        HIPCHK(hipEventRecord(c->evLUStart[buf], s));
+        st = factor_batch(c, o, n, lda, nb, pf, have32[buf], bb);
+        if (st) return st;
+        HIPCHK(hipEventRecord(c->evLU[buf], s));
+        HIPCHK(hipStreamWaitEvent(si, c->evLU[buf], 0));
+        HIPCHK(hipEventRecord(c->evIRStart[buf], si));
         const double* X = nullptr;
         size_t xStride = 0, xinc = 0;
-        st = solve_batch(c, o, n, lda, nb, c->dV.as<double>(), 0, pf, &X, &xStride, &xinc, have32[buf], zbuf(buf),
-                         z32buf(buf));
+        st = refine_batch(c, o, n, lda, nb, c->dV.as<double>(), 0, pf, bb, si, &X, &xStride, &xinc);
         if (st) return st;
         pf.begin(EFIE_K_OTHER);
         HIPCHK(launch_epilogue(X, xStride, xinc, c->dWl.as<double>(), n, patchEdge - 1, groundEdge - 1,
-                               o.model == EFIE_MODEL_CENTROID, V0, Z0, nb, c->dOut.as<double>() + 4 * (size_t)f0, s));
+                               o.model == EFIE_MODEL_CENTROID, V0, Z0, nb, c->dOut.as<double>() + 4 * (size_t)f0,
+                               si));
         pf.end(EFIE_K_OTHER);
-        HIPCHK(hipEventRecord(c->ev[4], s));
-        st = finish_batch(c, o, nb, S, freqs + f0);
+        st = stage_stats(c, o, nb, buf, B, bb, si);
         if (st) return st;
-        float t1, t2, t3;
-        HIPCHK(hipEventElapsedTime(&t1, c->evFillStart[buf], c->evFill[buf]));
-        HIPCHK(hipEventElapsedTime(&t2, c->ev[2], c->ev[3]));
-        HIPCHK(hipEventElapsedTime(&t3, c->ev[3], c->ev[4]));
-        msFill += t1;
-        msFac += t2;
-        msSolve += t3;
-        if (!overlap && bi + 1 < nBatch) {
-            st = enqueue_fill(bi + 1);
+        HIPCHK(hipEventRecord(c->evIR[buf], si));
+        if (overlap) {
+            // batch bi-1's results (its refinement ran beside this LU),
+            // then the next fill into its buffer
+            if (bi >= 1) {
+                st = finish(bi - 1);
+                if (st) return st;
+            }
+            if (bi + 1 < nBatch) {
+                if (bi >= 1) HIPCHK(hipStreamWaitEvent(sf, c->evIR[(bi + 1) % nbuf], 0));
+                st = enqueue_fill(bi + 1);
+                if (st) return st;
+            }
+        } else {
+            st = finish(bi);
             if (st) return st;
+            if (bi + 1 < nBatch) {
+                st = enqueue_fill(bi + 1);
+                if (st) return st;
+            }
         }
+    }
+    if (overlap) {
+        st = finish(nBatch - 1);
+        if (st) return st;
+        // the context stream carries the remaining work of the call
+        HIPCHK(hipEventRecord(c->ev[1], si));
+        HIPCHK(hipStreamWaitEvent(s, c->ev[1], 0));
     }
     HIPCHK(hipEventRecord(c->ev[5], s));
     std::vector<double> out(4 * (size_t)Nf);
@@ -1391,13 +1499,20 @@ efie_status efie_zgesv(efie_ctx* c, int n, int batch, const double* A, const dou
     if (precision == EFIE_PREC_MIXED)
         HIPCHK(hipMemcpyAsync(c->dBn2.p, bn2.data(), sizeof(double) * batch, hipMemcpyHostToDevice, s));
     Prof pf = make_prof(c);
+    const BatchBufs bb = batch_bufs(c, o, n, lda, batch, 0);
     HIPCHK(hipEventRecord(c->ev[2], s));
     const double* X = nullptr;
     size_t xStride = 0, xinc = 0;
-    st = solve_batch(c, o, n, lda, batch, c->dB.as<double>(), 2 * (size_t)n, pf, &X, &xStride, &xinc);
+    st = factor_batch(c, o, n, lda, batch, pf, false, bb);
+    if (st) return st;
+    HIPCHK(hipEventRecord(c->ev[3], s));
+    st = refine_batch(c, o, n, lda, batch, c->dB.as<double>(), 2 * (size_t)n, pf, bb, s, &X, &xStride, &xinc);
+    if (st) return st;
+    st = stage_stats(c, o, batch, 0, batch, bb, s);
     if (st) return st;
     HIPCHK(hipEventRecord(c->ev[4], s));
-    st = finish_batch(c, o, batch, S, nullptr);
+    HIPCHK(hipStreamSynchronize(s));
+    st = finish_batch(c, o, batch, 0, batch, S, nullptr);
     if (st) return st;
     for (int q = 0; q < batch; ++q)
         HIPCHK(hipMemcpy2DAsync(x + 2 * (size_t)n * q, sizeof(double) * 2, X + xStride * q, sizeof(double) * xinc,
