@@ -56,7 +56,7 @@ typedef struct {
     int model;          /* EFIE_MODEL_*                                       */
     int precision;      /* EFIE_PREC_*                                        */
     int batch;          /* frequencies factored together (0 = automatic: up   *
-                         * to 32, within 60% of the free device memory)       */
+                         * to 52, two batches within 60% of free memory)      */
     double ir_tol;      /* mixed: stop refinement at ||V-Zx||/||V|| <= ir_tol  *
                          * or when a correction no longer halves the residual */
     int ir_max_iter;    /* mixed: refinement iteration cap                    */

@@ -980,11 +980,12 @@ efie_status choose_near(efie_ctx* c, const efie_opts* o, double kmax, efie_stats
 
 // Frequencies factored together: latency-bound steps of the LU (pivot
 // selection, panel solves, triangular solves) cost per launch, so a batch
-// holds up to 32 matrices, limited so two batches fit in 60% of the free
-// device memory, and split evenly over the sweep (201 frequencies -> 7
-// batches of 29).
+// holds up to 52 matrices (larger batches leave too little of the sweep
+// to pipeline), limited so two batches fit in 60% of the free device
+// memory, and split evenly over the sweep (201 frequencies at n = 6009 ->
+// 4 batches of 51).
 int batch_size(const efie_opts& o, size_t zElems, int Nf) {
-    int B = o.batch > 0 ? o.batch : 32;
+    int B = o.batch > 0 ? o.batch : 52;
     size_t freeB = 0, totB = 0;
     if (hipMemGetInfo(&freeB, &totB) == hipSuccess) {
         // two batches of matrices are resident while one is filled and the
