@@ -558,7 +558,8 @@ __global__ __launch_bounds__(256) void k_poly_records(const double* __restrict__
 // Z is complex symmetric (Z(m,n) = Z(n,m): the four side pairs of (n,m)
 // are those of (m,n) with the roles swapped), so one block computes a
 // 16 x 16 tile (I, J) with I <= J and writes it twice, the mirror through
-// an LDS transpose so both stores are row-contiguous.  Thread (r, c) owns
+// an LDS transpose so both stores are row-contiguous (on a diagonal tile
+// the entries with m <= n, mirrored to the others).  Thread (r, c) owns
 // entry (16I + r, 16J + c).  All table and record loads are unconditional
 // (indices clamped, missing triangles masked afterwards) so they are in
 // flight together.  With Z32 set, the complex64 copy of the mixed-precision
@@ -645,16 +646,18 @@ __global__ __launch_bounds__(256) void k_combine(const Rec* __restrict__ recs, c
     const Freq f = *fr;
     double zr, zi;
     combine_entry(recs, sideTri, side, m < Ne ? m : Ne - 1, n < Ne ? n : Ne - 1, f, zr, zi);
-    if (m < Ne && n < Ne) {
+    // the upper triangle (m <= n) is computed; on a diagonal tile the lower
+    // half comes from the mirror as on every other tile, so Z is exactly
+    // symmetric whatever discretisation the records carry
+    if (m < Ne && n < Ne && (I != J || r <= c)) {
         *reinterpret_cast<double2*>(Z + 2 * ((size_t)m * ld + n)) = make_double2(zr, zi);
         if (Z32) *reinterpret_cast<float2*>(Z32 + 2 * ((size_t)m * ld32 + n)) = make_float2((float)zr, (float)zi);
     }
-    if (I == J) return;  // block-uniform: the diagonal tile has no mirror
     tile[r][c] = make_double2(zr, zi);
     __syncthreads();
     // mirror: thread (r, c) writes entry (16J + r, 16I + c) = tile[c][r]
     const int mm = 16 * J + r, nn = 16 * I + c;
-    if (mm < Ne && nn < Ne) {
+    if (mm < Ne && nn < Ne && (I != J || r > c)) {
         const double2 v = tile[c][r];
         *reinterpret_cast<double2*>(Z + 2 * ((size_t)mm * ld + nn)) = v;
         if (Z32) *reinterpret_cast<float2*>(Z32 + 2 * ((size_t)mm * ld32 + nn)) = make_float2((float)v.x, (float)v.y);

@@ -499,6 +499,185 @@ static void pair_subdivided(const orc_problem* P, int m, int n, const trigeom* g
     *Isca = is;
 }
 
+/* ------------------------------------------- singularity extraction -- */
+
+static void cross3(const double* a, const double* b, double* c) {
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+static double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+/* Closed-form potentials, edge by edge (edge i from v_i to v_{i+1},
+ * counter-clockwise about the normal n): with l^-, l^+ the in-line
+ * coordinates of the edge ends, P0 the signed distance of the projection
+ * rho from the edge line (positive inside), d the height of r above the
+ * plane, R0^2 = P0^2 + d^2 and R^+- the distances to the edge ends,
+ *   f_i  = ln((R^+ + l^+)/(R^- + l^-))
+ *   b_i  = atan(P0 l^+ / (R0^2 + |d| R^+)) - atan(P0 l^- / (R0^2 + |d| R^-))
+ *   int 1/R          = sum_i  P0_i f_i - |d| b_i
+ *   int (r'-rho)/R   = 1/2 sum_i u_i (R0_i^2 f_i + l^+ R^+ - l^- R^-)
+ * (u_i the outward in-plane edge normal).  f_i uses the algebraically equal
+ * ln((R^- - l^-)/(R^+ - l^+)) when both l are negative, and
+ * ln((R^+ + l^+)(R^- - l^-)/R0^2) when rho projects inside the edge, to
+ * avoid cancellation; an edge whose line passes through r contributes no
+ * P0 f term (P0 f -> 0). */
+int orc_tri_potentials(const double* r, const double* v1, const double* v2, const double* v3,
+                       double* out) {
+    const double* v[3] = {v1, v2, v3};
+    double e1[3], e2[3], nn[3];
+    for (int d = 0; d < 3; ++d) { e1[d] = v2[d] - v1[d]; e2[d] = v3[d] - v1[d]; }
+    cross3(e1, e2, nn);
+    double nl = norm3(nn);
+    for (int d = 0; d < 3; ++d) nn[d] /= nl;
+    double rv[3] = {r[0] - v1[0], r[1] - v1[1], r[2] - v1[2]};
+    double h = dot3(rv, nn);
+    double ah = fabs(h);
+    double rho[3] = {r[0] - h * nn[0], r[1] - h * nn[1], r[2] - h * nn[2]};
+    double scale = norm3(e1) + norm3(e2);
+    double I0 = 0.0, I1[3] = {0, 0, 0};
+    for (int i = 0; i < 3; ++i) {
+        const double *a = v[i], *b = v[(i + 1) % 3];
+        double s[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+        double L = norm3(s);
+        double lh[3] = {s[0] / L, s[1] / L, s[2] / L};
+        double u[3];
+        cross3(lh, nn, u);
+        double am[3] = {a[0] - rho[0], a[1] - rho[1], a[2] - rho[2]};
+        double bm[3] = {b[0] - rho[0], b[1] - rho[1], b[2] - rho[2]};
+        double lm = dot3(am, lh), lp = dot3(bm, lh);
+        double P0 = dot3(am, u);
+        double R02 = P0 * P0 + h * h;
+        double Rm = sqrt(R02 + lm * lm), Rp = sqrt(R02 + lp * lp);
+        double f;
+        if (R02 <= 1e-28 * scale * scale) {
+            f = 0.0; /* r on the edge line: P0 f and R0^2 f vanish */
+        } else if (lm >= 0.0) {
+            f = log((Rp + lp) / (Rm + lm));
+        } else if (lp <= 0.0) {
+            f = log((Rm - lm) / (Rp - lp));
+        } else {
+            /* rho projects inside the edge: R^- + l^- = R0^2 / (R^- - l^-) */
+            f = log((Rp + lp) * (Rm - lm) / R02);
+        }
+        double beta = 0.0;
+        if (ah > 0.0)
+            beta = atan(P0 * lp / (R02 + ah * Rp)) - atan(P0 * lm / (R02 + ah * Rm));
+        I0 += P0 * f - ah * beta;
+        double c = 0.5 * (R02 * f + lp * Rp - lm * Rm);
+        for (int d = 0; d < 3; ++d) I1[d] += c * u[d];
+    }
+    out[0] = I0;
+    out[1] = I1[0]; out[2] = I1[1]; out[3] = I1[2];
+    return 0;
+}
+
+/* (exp(-jkR) - 1)/(4 pi R), with its limit -jk/(4 pi) at R = 0; a Taylor
+ * series for |kR| < 1e-3. */
+static cx green_smooth(double kr, double ki, double R) {
+    double ar = kr * R, ai = ki * R;
+    if (sqrt(ar * ar + ai * ai) < 1e-3) {
+        /* sum_{q>=1} (-jk)^q R^(q-1)/q! with -jk = (ki, -kr) */
+        cx mjk = cmk(ki, -kr);
+        cx term = mjk, acc = mjk;
+        for (int q = 2; q <= 6; ++q) {
+            term = cscale(cmul(term, mjk), R / q);
+            acc = cadd(acc, term);
+        }
+        return cscale(acc, 1.0 / (4.0 * PI_D));
+    }
+    cx g = green(kr, ki, R);
+    return cmk(g.re - 1.0 / (4.0 * PI_D * R), g.im);
+}
+
+/* f_n = c (r' - v) on triangle t of RWG n, div f_n = dv */
+static void rwg_affine(const orc_problem* P, int n, int t, double* c, double* v, double* dv) {
+    int e = n - 1;
+    if (t == P->plusTri[e]) {
+        double s = P->plusSign[e], l = P->len[e], A = P->Ap[e];
+        *c = s * (l / (2 * A));
+        for (int d = 0; d < 3; ++d) v[d] = P->rp[e + d * P->Ne];
+        *dv = s * (l / A);
+    } else if (t == P->minusTri[e]) {
+        double s = P->minusSign[e], l = P->len[e], A = P->Am[e];
+        *c = -s * (l / (2 * A));
+        for (int d = 0; d < 3; ++d) v[d] = P->rm[e + d * P->Ne];
+        *dv = s * (-l / A);
+    } else {
+        *c = 0.0; *dv = 0.0; v[0] = v[1] = v[2] = 0.0;
+    }
+}
+
+/* Outer triangle t1 (RWG m), inner t2 (RWG n), t1 <= t2. */
+static void pair_extract_ordered(const orc_problem* P, int m, int n, const trigeom* g1, const trigeom* g2,
+                                 int t1, int t2, const orc_opts* o, double kr, double ki, cx* Ivec, cx* Isca) {
+    double qp2[32], qw2[16];
+    int Nq2 = orc_quad_rule(o->near_order, qp2, qw2);
+    int depth = t1 == t2 ? o->self_subdiv : o->near_subdiv;
+    int Ns1, Ns2;
+    double* sub1 = subdivide(g1->r1, g1->r2, g1->r3, depth, &Ns1);
+    double* sub2 = subdivide(g2->r1, g2->r2, g2->r3, depth, &Ns2);
+    double cn, vn[3], dn;
+    rwg_affine(P, n, t2, &cn, vn, &dn);
+    double r[48], rq[48], fm[48], fn[48], dm[16], dq[16], wp[16], wq[16];
+    cx iv = cmk(0, 0), is = cmk(0, 0);
+    const double inv4pi = 1.0 / (4.0 * PI_D);
+    for (int p = 0; p < Ns1; ++p) {
+        const double *Ap_ = sub1 + 9 * p, *Bp = Ap_ + 3, *Cp = Ap_ + 6;
+        double a2p = tri_area2(Ap_, Bp, Cp);
+        map_ref(Ap_, Bp, Cp, qp2, Nq2, r);
+        for (int i = 0; i < Nq2; ++i) wp[i] = qw2[i] * a2p;
+        rwg_eval(P, m, t1, r, Nq2, fm, dm);
+        for (int i = 0; i < Nq2; ++i) {
+            const double* ri = r + 3 * i;
+            /* static part: inner integral in closed form */
+            double pot[4];
+            orc_tri_potentials(ri, g2->r1, g2->r2, g2->r3, pot);
+            double e1[3], e2[3], nn[3];
+            for (int d = 0; d < 3; ++d) { e1[d] = g2->r2[d] - g2->r1[d]; e2[d] = g2->r3[d] - g2->r1[d]; }
+            cross3(e1, e2, nn);
+            double nl = norm3(nn);
+            for (int d = 0; d < 3; ++d) nn[d] /= nl;
+            double h = (ri[0] - g2->r1[0]) * nn[0] + (ri[1] - g2->r1[1]) * nn[1] + (ri[2] - g2->r1[2]) * nn[2];
+            double vec[3];
+            for (int d = 0; d < 3; ++d) vec[d] = cn * (pot[1 + d] + (ri[d] - h * nn[d] - vn[d]) * pot[0]);
+            double ffs = fm[3 * i] * vec[0] + fm[3 * i + 1] * vec[1] + fm[3 * i + 2] * vec[2];
+            cx cv = cmk(inv4pi * ffs, 0.0);
+            cx cs = cmk(inv4pi * dn * pot[0], 0.0);
+            /* smooth part by quadrature */
+            for (int q = 0; q < Ns2; ++q) {
+                const double *Aq = sub2 + 9 * q, *Bq = Aq + 3, *Cq = Aq + 6;
+                double a2q = tri_area2(Aq, Bq, Cq);
+                map_ref(Aq, Bq, Cq, qp2, Nq2, rq);
+                for (int j = 0; j < Nq2; ++j) wq[j] = qw2[j] * a2q;
+                rwg_eval(P, n, t2, rq, Nq2, fn, dq);
+                for (int j = 0; j < Nq2; ++j) {
+                    double dx = ri[0] - rq[3 * j], dy = ri[1] - rq[3 * j + 1], dz = ri[2] - rq[3 * j + 2];
+                    cx gs = green_smooth(kr, ki, sqrt(dx * dx + dy * dy + dz * dz));
+                    double ff = fm[3 * i] * fn[3 * j] + fm[3 * i + 1] * fn[3 * j + 1] + fm[3 * i + 2] * fn[3 * j + 2];
+                    cv = cadd(cv, cscale(gs, wq[j] * ff));
+                    cs = cadd(cs, cscale(gs, wq[j] * dq[j]));
+                }
+            }
+            iv = cadd(iv, cscale(cv, wp[i]));
+            is = cadd(is, cscale(cs, wp[i] * dm[i]));
+        }
+    }
+    free(sub1);
+    free(sub2);
+    *Ivec = iv;
+    *Isca = is;
+}
+
+/* The outer triangle is the lower-numbered one, and on a self pair the
+ * outer function is the lower-numbered RWG (the integrals are symmetric
+ * under exchange of the two sides, their discretisation is not). */
+static void pair_extract(const orc_problem* P, int m, int n, const trigeom* g1, const trigeom* g2, int t1,
+                         int t2, const orc_opts* o, double kr, double ki, cx* Ivec, cx* Isca) {
+    if (t1 < t2 || (t1 == t2 && m <= n)) pair_extract_ordered(P, m, n, g1, g2, t1, t2, o, kr, ki, Ivec, Isca);
+    else pair_extract_ordered(P, n, m, g2, g1, t2, t1, o, kr, ki, Ivec, Isca);
+}
+
 int orc_pair_integral(const orc_problem* P, int kind, int m, int n, int t1, int t2,
                       double k_re, double k_im, const orc_opts* o, double* out) {
     trigeom g1, g2;
@@ -512,6 +691,8 @@ int orc_pair_integral(const orc_problem* P, int kind, int m, int n, int t1, int 
         pair_regular(P, m, n, &g1, &g2, t1, t2, qp, qw, Nq, k_re, k_im, &iv, &is);
     } else if (kind == 1) {
         pair_subdivided(P, m, n, &g1, &g2, t1, t2, 0, o, k_re, k_im, &iv, &is);
+    } else if (kind == 3) {
+        pair_extract(P, m, n, &g1, &g2, t1, t2, o, k_re, k_im, &iv, &is);
     } else {
         if (t1 != t2) return -1; /* rwg_pair_integrals_singular.m:3-5 */
         pair_subdivided(P, m, n, &g1, &g2, t1, t2, 1, o, k_re, k_im, &iv, &is);
@@ -553,7 +734,8 @@ int orc_assemble_z_rows(const orc_problem* P, double k_re, double k_im, double o
                     cx iv, is;
                     const trigeom *g1 = &G[t1 - 1], *g2 = &G[t2 - 1];
                     if (t1 == t2) {
-                        pair_subdivided(P, m, n, g1, g2, t1, t2, 1, o, k_re, k_im, &iv, &is);
+                        if (o->extract) pair_extract(P, m, n, g1, g2, t1, t2, o, k_re, k_im, &iv, &is);
+                        else pair_subdivided(P, m, n, g1, g2, t1, t2, 1, o, k_re, k_im, &iv, &is);
                     } else {
                         double d3[3] = {g1->cent[0] - g2->cent[0], g1->cent[1] - g2->cent[1],
                                         g1->cent[2] - g2->cent[2]};
@@ -561,7 +743,9 @@ int orc_assemble_z_rows(const orc_problem* P, double k_re, double k_im, double o
                         double A1 = 0.5 * g1->area2, A2 = 0.5 * g2->area2;
                         double l1 = sqrt(4 * A1 / sqrt(3.0)), l2 = sqrt(4 * A2 / sqrt(3.0));
                         double h = 0.5 * (l1 + l2);
-                        if (d < o->near_sing_eps * h)
+                        if (d < o->near_sing_eps * h && o->extract)
+                            pair_extract(P, m, n, g1, g2, t1, t2, o, k_re, k_im, &iv, &is);
+                        else if (d < o->near_sing_eps * h)
                             pair_subdivided(P, m, n, g1, g2, t1, t2, 0, o, k_re, k_im, &iv, &is);
                         else
                             pair_regular(P, m, n, g1, g2, t1, t2, qp, qw, Nq, k_re, k_im, &iv, &is);

@@ -30,6 +30,10 @@ typedef struct {
     int near_subdiv;
     double near_alpha;
     int near_order;
+    /* 1: self and near pairs by singularity extraction (orc_pair_extract)
+     * instead of the reference's softened subdivision; not in the
+     * reference (its Z is only defined for extract = 0). */
+    int extract;
 } orc_opts;
 
 /* Mesh sizes of make_patch_with_ground_mesh.m:1-46. */
@@ -67,6 +71,23 @@ typedef struct {
     const double* rp; const double* rm;
     const double* Ap; const double* Am; const double* len;
 } orc_problem;
+
+/* Static potentials of a flat triangle (v1, v2, v3) at the point r, in
+ * closed form (Wilton, Rao, Glisson, Schaubert, Al-Bundak and Butler, IEEE
+ * TAP 32(3), 1984; Graglia, IEEE TAP 41(10), 1993):
+ *   out[0]   = int_T 1/|r - r'| dS'
+ *   out[1:4] = int_T (r' - rho)/|r - r'| dS'  (rho = projection of r onto
+ *              the triangle's plane; the vector lies in that plane).
+ * Not in the reference.  Returns 0. */
+int orc_tri_potentials(const double* r, const double* v1, const double* v2, const double* v3,
+                       double* out);
+
+/* kind 3 of orc_pair_integral: singularity extraction for a self or near
+ * pair, G = 1/(4 pi R) + (exp(-jkR) - 1)/(4 pi R).  The outer integral runs
+ * over the lower-numbered triangle, subdivided 4^depth times (self_subdiv
+ * for t1 == t2, near_subdiv otherwise) with the near_order rule; the inner
+ * static part is orc_tri_potentials over the whole other triangle, the
+ * inner smooth part the same subdivided rule.  Not in the reference. */
 
 int orc_pair_integral(const orc_problem* P, int kind, int m, int n, int t1,
                       int t2, double k_re, double k_im, const orc_opts* o,

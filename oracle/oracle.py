@@ -34,6 +34,7 @@ class OrcOpts(C.Structure):
         ("near_subdiv", C.c_int),
         ("near_alpha", C.c_double),
         ("near_order", C.c_int),
+        ("extract", C.c_int),
     ]
 
 
@@ -63,6 +64,7 @@ def lib():
         L.orc_build_rwg.argtypes = [_dp, C.c_int, _ip, C.c_int, C.c_int] + [_ip] * 5 + [_dp] * 6 + [_ip]
         L.orc_quad_rule.argtypes = [C.c_int, _dp, _dp]
         L.orc_triangles_are_near.argtypes = [_dp, C.c_int, _ip, C.c_int, C.c_int, C.c_int, C.c_double]
+        L.orc_tri_potentials.argtypes = [_dp] * 5
         L.orc_pair_integral.argtypes = [C.POINTER(OrcProblem), C.c_int, C.c_int, C.c_int, C.c_int,
                                         C.c_int, C.c_double, C.c_double, C.POINTER(OrcOpts), _dp]
         L.orc_assemble_z_rows.argtypes = [C.POINTER(OrcProblem)] + [C.c_double] * 6 + [
@@ -236,6 +238,14 @@ class Problem:
         if rc:
             raise RuntimeError(f"oracle sweep failed ({rc})")
         return zin.view(np.complex128).copy(), s11.view(np.complex128).copy()
+
+
+def tri_potentials(r, v1, v2, v3):
+    """(int_T 1/R dS', int_T (r' - rho)/R dS') in closed form (orc_tri_potentials)."""
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (r, v1, v2, v3)]
+    out = np.zeros(4)
+    lib().orc_tri_potentials(*[_d(x) for x in a], _d(out))
+    return out[0], out[1:].copy()
 
 
 def pick_port(rwg: Rwg, p, layer):
