@@ -68,8 +68,13 @@ typedef struct {
  * frequency as the reference does; MOMENTS expands the Green's function in
  * powers of k R, integrates the frequency-independent moments once per mesh
  * and sums the series per frequency (tail below 1e-17 relative; falls back
- * to DIRECT when k R is too large for that). */
-enum { EFIE_NEAR_DIRECT = 0, EFIE_NEAR_MOMENTS = 1 };
+ * to DIRECT when k R is too large for that).  EXTRACT replaces the
+ * reference's softened scheme on near and self pairs by singularity
+ * extraction: the static 1/(4 pi R) part of the inner integral in closed
+ * form over the whole source triangle, the smooth remainder
+ * (exp(-jkR) - 1)/(4 pi R) by the subdivided near/self rule (as a moment
+ * series).  It has no counterpart in the reference. */
+enum { EFIE_NEAR_DIRECT = 0, EFIE_NEAR_MOMENTS = 1, EFIE_NEAR_EXTRACT = 2 };
 
 /* Kernel classes timed when efie_opts.profile is set. */
 enum {

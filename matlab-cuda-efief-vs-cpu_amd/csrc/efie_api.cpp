@@ -202,9 +202,12 @@ struct efie_ctx {
     int Nq = 0, NpNear = 0, Nq2s = 0, NsSelf = 0;
     long long nNear = 0;
     double rNearMax = 0, rSelfMax = 0, lMax = 0, aNearMax = 0, aSelfMax = 0;
-    // moment cache (near_mode MOMENTS), valid for series length momP + 1
+    // moment cache (near_mode MOMENTS or EXTRACT), valid for series length
+    // momP + 1 of the scheme momExtract says
     int momP = -1;
     double nearAlphaMom = -1;
+    bool momExtract = false;
+    DBuf dTriV;  // triangle vertices relative to the centroid [Nt][9]
 
     DBuf dTri, dReg, dNearPts, dSelfPts, dASelf, dAeq, dNearMat, dRowCnt, dList;
     DBuf dSideTri, dSide, dMid, dLen, dQn, dQs, dRad;
@@ -597,7 +600,7 @@ efie_status check_opts(const efie_opts* o) {
     if (o->model != EFIE_MODEL_QUADRATURE) return fail(EFIE_E_ARGS, "efie:opts", "unknown model");
     if (o->near_subdiv < 0 || o->near_subdiv > 3 || o->self_subdiv < 0 || o->self_subdiv > 3)
         return fail(EFIE_E_ARGS, "efie:opts", "subdivision depth must be 0..3");
-    if (o->near_mode != EFIE_NEAR_DIRECT && o->near_mode != EFIE_NEAR_MOMENTS)
+    if (o->near_mode != EFIE_NEAR_DIRECT && o->near_mode != EFIE_NEAR_MOMENTS && o->near_mode != EFIE_NEAR_EXTRACT)
         return fail(EFIE_E_ARGS, "efie:opts", "unknown near_mode");
     return ok();
 }
@@ -720,6 +723,16 @@ efie_status ensure_geometry(efie_ctx* c, const efie_opts* o) {
         }
     }
 
+    std::vector<double> triV(9 * (size_t)Nt);
+    for (int t = 0; t < Nt; ++t)
+        for (int q = 0; q < 3; ++q) {
+            const V3 v = tv[3 * t + q];
+            triV[9 * (size_t)t + 3 * q] = v.x - tg[t].cx;
+            triV[9 * (size_t)t + 3 * q + 1] = v.y - tg[t].cy;
+            triV[9 * (size_t)t + 3 * q + 2] = v.z - tg[t].cz;
+        }
+    HIPCHK(c->dTriV.ensure(sizeof(double) * triV.size()));
+    HIPCHK(hipMemcpyAsync(c->dTriV.p, triV.data(), sizeof(double) * triV.size(), hipMemcpyHostToDevice, s));
     HIPCHK(c->dTri.ensure(sizeof(TriGeo) * Nt));
     HIPCHK(c->dReg.ensure(sizeof(Pt) * reg.size()));
     HIPCHK(c->dNearPts.ensure(sizeof(Pt) * nearPts.size()));
@@ -778,10 +791,12 @@ efie_status ensure_geometry(efie_ctx* c, const efie_opts* o) {
 // (x^(P+1)/(P+1)!) e^x <= 1e-17, x = |k|max * Rmax, where Rmax bounds the
 // (softened) distance of every near and self point pair.  -1 if P > 60.
 int moment_terms(const efie_ctx* c, const efie_opts* o, double kmax) {
+    const bool extract = o->near_mode == EFIE_NEAR_EXTRACT;  // no softening
     const double dN = o->near_sing_eps * c->lMax;  // centroid distance of a near pair
-    const double aN = o->near_alpha * c->aNearMax;
+    const double aN = extract ? 0.0 : o->near_alpha * c->aNearMax;
+    const double aS = extract ? 0.0 : c->aSelfMax;
     const double rN = std::sqrt((dN + 2 * c->rNearMax) * (dN + 2 * c->rNearMax) + aN * aN);
-    const double rS = std::sqrt(4 * c->rSelfMax * c->rSelfMax + c->aSelfMax * c->aSelfMax);
+    const double rS = std::sqrt(4 * c->rSelfMax * c->rSelfMax + aS * aS);
     const double x = kmax * std::max(rN, rS) * 1.0001;
     double term = std::exp(x);  // e^x x^n / n!
     for (int n = 1; n <= 61; ++n) {
@@ -794,7 +809,8 @@ int moment_terms(const efie_ctx* c, const efie_opts* o, double kmax) {
 // Moments of the near and self pairs for series length P+1 (cached; a
 // longer series than cached recomputes them).
 efie_status ensure_moments(efie_ctx* c, const efie_opts* o, int P, efie_stats* st) {
-    if (c->momP >= P && c->nearAlphaMom == o->near_alpha) return ok();
+    const bool extract = o->near_mode == EFIE_NEAR_EXTRACT;
+    if (c->momP >= P && c->momExtract == extract && (extract || c->nearAlphaMom == o->near_alpha)) return ok();
     hipStream_t s = c->stream;
     const size_t per = (size_t)(P + 1) * 8;
     HIPCHK(c->dQn.ensure(sizeof(double) * per * (size_t)std::max(1LL, c->nNear)));
@@ -802,7 +818,8 @@ efie_status ensure_moments(efie_ctx* c, const efie_opts* o, int P, efie_stats* s
     HIPCHK(hipEventRecord(c->ev[6], s));
     HIPCHK(launch_moments(c->dNearPts.as<Pt>(), c->NpNear, c->dTri.as<TriGeo>(), c->dAeq.as<double>(), o->near_alpha,
                           c->dList.as<int2>(), (int)c->nNear, c->dSelfPts.as<Pt>(), c->Nq2s, c->NsSelf,
-                          c->dASelf.as<double>(), c->Nt, P, c->dQn.as<double>(), c->dQs.as<double>(), s));
+                          c->dASelf.as<double>(), c->Nt, P, c->dQn.as<double>(), c->dQs.as<double>(),
+                          extract ? c->dTriV.as<double>() : nullptr, s));
     HIPCHK(hipEventRecord(c->ev[7], s));
     HIPCHK(hipStreamSynchronize(s));
     float ms = 0;
@@ -810,6 +827,7 @@ efie_status ensure_moments(efie_ctx* c, const efie_opts* o, int P, efie_stats* s
     if (st) st->ms_moments += ms;
     c->momP = P;
     c->nearAlphaMom = o->near_alpha;
+    c->momExtract = extract;
     return ok();
 }
 
@@ -967,9 +985,14 @@ efie_status prepare(efie_ctx* c, const efie_opts* o, double selfAlphaCentroid) {
 // series when asked for and short enough, otherwise direct quadrature.
 efie_status choose_near(efie_ctx* c, const efie_opts* o, double kmax, efie_stats* st, bool* useMoments) {
     *useMoments = false;
-    if (o->model != EFIE_MODEL_QUADRATURE || o->near_mode != EFIE_NEAR_MOMENTS) return ok();
+    if (o->model != EFIE_MODEL_QUADRATURE || o->near_mode == EFIE_NEAR_DIRECT) return ok();
     const int P = moment_terms(c, o, kmax);
-    if (P < 0) return ok();
+    if (P < 0) {
+        // the extraction exists only in series form
+        if (o->near_mode == EFIE_NEAR_EXTRACT)
+            return fail(EFIE_E_ARGS, "efie:opts", "near_mode extract: |k| times the near-pair size is too large");
+        return ok();
+    }
     efie_status e = ensure_moments(c, o, P, st);
     if (e) return e;
     *useMoments = true;

@@ -404,7 +404,7 @@ __global__ __launch_bounds__(256) void k_rec_self(const Pt* __restrict__ selfPts
 
 constexpr int kMomChunk = 4;
 
-template <bool SELF>
+template <bool SELF, bool EXTRACT = false>
 __device__ void moment_pair(const Pt* __restrict__ P1, const Pt* __restrict__ P2, int Np, int Nq2,
                             double ox, double oy, double oz, double aNear2,
                             const double* __restrict__ aSelf, int P, double* __restrict__ out) {
@@ -438,7 +438,9 @@ __device__ void moment_pair(const Pt* __restrict__ P1, const Pt* __restrict__ P2
                 double dx = xi - q2.x, dy = yi - q2.y, dz = zi - q2.z;
                 double R2 = dx * dx + dy * dy + dz * dz;
                 double R;
-                if (!SELF) {
+                if (EXTRACT) {
+                    R = sqrt(R2);  // unsoftened; only q >= 1 (R^0, R^1, ...) is summed
+                } else if (!SELF) {
                     R = sqrt(R2 + aNear2);
                 } else if (j / Nq2 == psub) {
                     R = sqrt(R2 + aS2);
@@ -446,17 +448,33 @@ __device__ void moment_pair(const Pt* __restrict__ P1, const Pt* __restrict__ P2
                     R = sqrt(R2);
                     R = R > 1e-15 ? R : 1e-15;
                 }
-                // R^(q0-1)
-                double rp = 1.0 / R;
-                for (int e = 0; e < q0; ++e) rp *= R;
+                if (EXTRACT) {
+                    // R^(q-1) for q >= 1; the q = 0 slot (1/R) is the closed-form
+                    // static part written by k_static_extract
+                    double rp = 1.0;
+                    for (int e = 1; e < q0; ++e) rp *= R;
 #pragma unroll
-                for (int c = 0; c < kMomChunk; ++c) {
-                    const double t = q2.w * rp;
-                    g[c] += t;
-                    hx[c] += t * q2.x;
-                    hy[c] += t * q2.y;
-                    hz[c] += t * q2.z;
-                    rp *= R;
+                    for (int c = 0; c < kMomChunk; ++c) {
+                        const double t = (q0 + c == 0) ? 0.0 : q2.w * rp;
+                        g[c] += t;
+                        hx[c] += t * q2.x;
+                        hy[c] += t * q2.y;
+                        hz[c] += t * q2.z;
+                        if (q0 + c >= 1) rp *= R;
+                    }
+                } else {
+                    // R^(q0-1)
+                    double rp = 1.0 / R;
+                    for (int e = 0; e < q0; ++e) rp *= R;
+#pragma unroll
+                    for (int c = 0; c < kMomChunk; ++c) {
+                        const double t = q2.w * rp;
+                        g[c] += t;
+                        hx[c] += t * q2.x;
+                        hy[c] += t * q2.y;
+                        hz[c] += t * q2.z;
+                        rp *= R;
+                    }
                 }
             }
 #pragma unroll
@@ -482,6 +500,7 @@ __device__ void moment_pair(const Pt* __restrict__ P1, const Pt* __restrict__ P2
     }
 }
 
+template <bool EXTRACT>
 __global__ __launch_bounds__(256) void k_moment_near(const Pt* __restrict__ subPts, int Np,
                                                      const TriGeo* __restrict__ tg,
                                                      const double* __restrict__ aeq,
@@ -494,11 +513,12 @@ __global__ __launch_bounds__(256) void k_moment_near(const Pt* __restrict__ subP
     const TriGeo g1 = tg[tp.x], g2 = tg[tp.y];
     double a1 = aeq[tp.x], a2 = aeq[tp.y];
     double as = near_alpha * (a1 < a2 ? a1 : a2);
-    moment_pair<false>(subPts + (size_t)tp.x * Np, subPts + (size_t)tp.y * Np, Np, Np,
-                       g1.cx - g2.cx, g1.cy - g2.cy, g1.cz - g2.cz, as * as, nullptr, P,
-                       Q + (size_t)pair * (P + 1) * 8);
+    moment_pair<false, EXTRACT>(subPts + (size_t)tp.x * Np, subPts + (size_t)tp.y * Np, Np, Np,
+                                g1.cx - g2.cx, g1.cy - g2.cy, g1.cz - g2.cz, as * as, nullptr, P,
+                                Q + (size_t)pair * (P + 1) * 8);
 }
 
+template <bool EXTRACT>
 __global__ __launch_bounds__(256) void k_moment_self(const Pt* __restrict__ selfPts, int Nq2,
                                                      int Ns, const double* __restrict__ aSelf,
                                                      int Nt, int P, double* __restrict__ Q) {
@@ -506,8 +526,121 @@ __global__ __launch_bounds__(256) void k_moment_self(const Pt* __restrict__ self
     if (t >= Nt) return;
     const int Np = Ns * Nq2;
     const Pt* Pp = selfPts + (size_t)t * Np;
-    moment_pair<true>(Pp, Pp, Np, Nq2, 0.0, 0.0, 0.0, 0.0, aSelf + (size_t)t * Ns, P,
-                      Q + (size_t)t * (P + 1) * 8);
+    moment_pair<true, EXTRACT>(Pp, Pp, Np, Nq2, 0.0, 0.0, 0.0, 0.0, aSelf + (size_t)t * Ns, P,
+                               Q + (size_t)t * (P + 1) * 8);
+}
+
+// ------------------------------------------- singularity extraction -----
+// Mode "extract": G = 1/(4 pi R) + (exp(-jkR) - 1)/(4 pi R).  The smooth
+// part is the q >= 1 tail of the moment series above (unsoftened); the
+// static part replaces the q = 0 moments: for every outer quadrature point
+// r of the lower triangle, the inner integrals over the whole other
+// triangle T2 are taken in closed form,
+//   int_T2 1/R dS'            = I0(r)
+//   int_T2 (r' - c2)/R dS'    = I1(r) + (rho - c2) I0(r)
+// (rho = projection of r onto T2's plane), and summed with the outer
+// weights into the eight moments.
+//
+// Closed form (Wilton, Rao, Glisson, Schaubert, Al-Bundak and Butler, IEEE
+// TAP 32(3), 1984; Graglia, IEEE TAP 41(10), 1993), edge by edge (edge i
+// from v_i to v_i+1, counter-clockwise about the unit normal n): l-, l+
+// in-line coordinates of the edge ends, P0 signed distance of rho from the
+// edge line, h height of r above the plane, R0^2 = P0^2 + h^2, R-, R+ the
+// distances to the edge ends, u the outward in-plane edge normal:
+//   f = ln((R+ + l+)/(R- + l-)),  b = atan(P0 l+/(R0^2 + |h| R+)) - atan(P0 l-/(R0^2 + |h| R-))
+//   I0 = sum P0 f - |h| b,   I1 = 1/2 sum u (R0^2 f + l+ R+ - l- R-).
+// f is evaluated without cancellation: ln((R- - l-)/(R+ - l+)) when both l
+// are negative, ln((R+ + l+)(R- - l-)/R0^2) when rho projects inside the
+// edge; an edge whose line passes through r contributes no f term.
+__device__ void tri_potentials(double rx, double ry, double rz, const double* __restrict__ v, double& I0,
+                               double& I1x, double& I1y, double& I1z, double& hx, double& hy, double& hz) {
+    const double e1x = v[3] - v[0], e1y = v[4] - v[1], e1z = v[5] - v[2];
+    const double e2x = v[6] - v[0], e2y = v[7] - v[1], e2z = v[8] - v[2];
+    double nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
+    const double nl = sqrt(nx * nx + ny * ny + nz * nz);
+    nx /= nl;
+    ny /= nl;
+    nz /= nl;
+    const double h = (rx - v[0]) * nx + (ry - v[1]) * ny + (rz - v[2]) * nz;
+    const double ah = fabs(h);
+    const double px = rx - h * nx, py = ry - h * ny, pz = rz - h * nz;  // rho
+    const double scale = sqrt(e1x * e1x + e1y * e1y + e1z * e1z) + sqrt(e2x * e2x + e2y * e2y + e2z * e2z);
+    I0 = 0.0;
+    I1x = I1y = I1z = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const double* a = v + 3 * i;
+        const double* b = v + 3 * ((i + 1) % 3);
+        const double sx = b[0] - a[0], sy = b[1] - a[1], sz = b[2] - a[2];
+        const double L = sqrt(sx * sx + sy * sy + sz * sz);
+        const double lx = sx / L, ly = sy / L, lz = sz / L;
+        const double ux = ly * nz - lz * ny, uy = lz * nx - lx * nz, uz = lx * ny - ly * nx;
+        const double amx = a[0] - px, amy = a[1] - py, amz = a[2] - pz;
+        const double lm = amx * lx + amy * ly + amz * lz;
+        const double lp = (b[0] - px) * lx + (b[1] - py) * ly + (b[2] - pz) * lz;
+        const double P0 = amx * ux + amy * uy + amz * uz;
+        const double R02 = P0 * P0 + h * h;
+        const double Rm = sqrt(R02 + lm * lm), Rp = sqrt(R02 + lp * lp);
+        double f;
+        if (R02 <= 1e-28 * scale * scale) f = 0.0;
+        else if (lm >= 0.0) f = log((Rp + lp) / (Rm + lm));
+        else if (lp <= 0.0) f = log((Rm - lm) / (Rp - lp));
+        else f = log((Rp + lp) * (Rm - lm) / R02);
+        double beta = 0.0;
+        if (ah > 0.0) beta = atan(P0 * lp / (R02 + ah * Rp)) - atan(P0 * lm / (R02 + ah * Rm));
+        I0 += P0 * f - ah * beta;
+        const double c = 0.5 * (R02 * f + lp * Rp - lm * Rm);
+        I1x += c * ux;
+        I1y += c * uy;
+        I1z += c * uz;
+    }
+    hx = px;
+    hy = py;
+    hz = pz;
+}
+
+// Static moments of one pair (one wave): outer points P1 relative to
+// centroid c1, inner triangle vertices v2 relative to c2, o = c1 - c2.
+__device__ void static_pair(const Pt* __restrict__ P1, int Np, const double* __restrict__ v2, double ox,
+                            double oy, double oz, double* __restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = lane; i < Np; i += 64) {
+        const Pt p = P1[i];
+        double I0, I1x, I1y, I1z, rhx, rhy, rhz;
+        tri_potentials(p.x + ox, p.y + oy, p.z + oz, v2, I0, I1x, I1y, I1z, rhx, rhy, rhz);
+        // int (r' - c2)/R = I1 + (rho - c2) I0, rho relative to c2 already
+        const double vx = I1x + rhx * I0, vy = I1y + rhy * I0, vz = I1z + rhz * I0;
+        acc[0] += p.w * I0;
+        acc[1] += p.w * I0 * p.x;
+        acc[2] += p.w * I0 * p.y;
+        acc[3] += p.w * I0 * p.z;
+        acc[4] += p.w * vx;
+        acc[5] += p.w * vy;
+        acc[6] += p.w * vz;
+        acc[7] += p.w * (p.x * vx + p.y * vy + p.z * vz);
+    }
+#pragma unroll
+    for (int x = 0; x < 8; ++x) {
+        const double v = wave_sum(acc[x]);
+        if (lane == 0) out[x] = v;
+    }
+}
+
+// Near pairs (SELF false: list entries, outer = the lower triangle) or
+// self pairs (SELF true: triangle t with itself); writes Q[pair][0][*].
+template <bool SELF>
+__global__ __launch_bounds__(256) void k_static_extract(const Pt* __restrict__ pts, int Np,
+                                                        const TriGeo* __restrict__ tg,
+                                                        const double* __restrict__ triV,
+                                                        const int2* __restrict__ list, int nPairs, int P,
+                                                        double* __restrict__ Q) {
+    const int pair = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (pair >= nPairs) return;
+    const int2 tp = SELF ? make_int2(pair, pair) : list[pair];
+    const TriGeo g1 = tg[tp.x], g2 = tg[tp.y];
+    static_pair(pts + (size_t)tp.x * Np, Np, triV + 9 * (size_t)tp.y, g1.cx - g2.cx, g1.cy - g2.cy,
+                g1.cz - g2.cz, Q + (size_t)pair * (P + 1) * 8);
 }
 
 // Records of near pairs (threads 0..nNear-1) and self pairs (the next Nt)
@@ -887,11 +1020,26 @@ hipError_t launch_rec_self(const Pt* selfPts, int Nq2, int Ns, const double* aSe
 hipError_t launch_moments(const Pt* subPts, int Np, const TriGeo* tg, const double* aeq,
                           double near_alpha, const int2* list, int nPairs, const Pt* selfPts,
                           int Nq2, int Ns, const double* aSelf, int Nt, int P, double* Qn,
-                          double* Qs, hipStream_t s) {
-    if (nPairs > 0)
-        hipLaunchKernelGGL(k_moment_near, dim3((nPairs + 3) / 4), dim3(256), 0, s, subPts, Np, tg, aeq,
-                           near_alpha, list, nPairs, P, Qn);
-    hipLaunchKernelGGL(k_moment_self, dim3((Nt + 3) / 4), dim3(256), 0, s, selfPts, Nq2, Ns, aSelf, Nt, P, Qs);
+                          double* Qs, const double* triV, hipStream_t s) {
+    if (!triV) {
+        if (nPairs > 0)
+            hipLaunchKernelGGL(k_moment_near<false>, dim3((nPairs + 3) / 4), dim3(256), 0, s, subPts, Np, tg, aeq,
+                               near_alpha, list, nPairs, P, Qn);
+        hipLaunchKernelGGL(k_moment_self<false>, dim3((Nt + 3) / 4), dim3(256), 0, s, selfPts, Nq2, Ns, aSelf, Nt,
+                           P, Qs);
+        return hipGetLastError();
+    }
+    // singularity extraction: smooth tail, then the static q = 0 slot
+    if (nPairs > 0) {
+        hipLaunchKernelGGL(k_moment_near<true>, dim3((nPairs + 3) / 4), dim3(256), 0, s, subPts, Np, tg, aeq, 0.0,
+                           list, nPairs, P, Qn);
+        hipLaunchKernelGGL(k_static_extract<false>, dim3((nPairs + 3) / 4), dim3(256), 0, s, subPts, Np, tg, triV,
+                           list, nPairs, P, Qn);
+    }
+    hipLaunchKernelGGL(k_moment_self<true>, dim3((Nt + 3) / 4), dim3(256), 0, s, selfPts, Nq2, Ns, aSelf, Nt, P,
+                       Qs);
+    hipLaunchKernelGGL(k_static_extract<true>, dim3((Nt + 3) / 4), dim3(256), 0, s, selfPts, Ns * Nq2, tg, triV,
+                       nullptr, Nt, P, Qs);
     return hipGetLastError();
 }
 

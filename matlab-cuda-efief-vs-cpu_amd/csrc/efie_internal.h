@@ -77,10 +77,13 @@ hipError_t launch_rec_self(const Pt* selfPts, int Nq2, int Ns, const double* aSe
 // Frequency-independent moments Q[pair][q][8], q = 0..P, of the near pairs
 // (Qn) and the self pairs (Qs), and the per-frequency records built from
 // them (near pairs and self pairs).
+// With triV (vertices relative to the centroid, [Nt][9]) non-null: the
+// singularity-extraction moments (unsoftened q >= 1 tail plus the
+// closed-form static q = 0 slot).
 hipError_t launch_moments(const Pt* subPts, int Np, const TriGeo* tg, const double* aeq,
                           double near_alpha, const int2* list, int nPairs, const Pt* selfPts,
                           int Nq2, int Ns, const double* aSelf, int Nt, int P, double* Qn,
-                          double* Qs, hipStream_t s);
+                          double* Qs, const double* triV, hipStream_t s);
 hipError_t launch_poly_records(const double* Qn, const int2* list, int nNear, const double* Qs,
                                int Nt, int P, const Freq* fr, Rec* recs, hipStream_t s);
 

@@ -31,7 +31,7 @@ _ip = C.POINTER(C.c_int)
 EFIE_OK = 0
 MODEL_QUADRATURE, MODEL_CENTROID = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
-NEAR_DIRECT, NEAR_MOMENTS = 0, 1
+NEAR_DIRECT, NEAR_MOMENTS, NEAR_EXTRACT = 0, 1, 2
 
 # physics constants of the reference drivers (main_patch_efie_rwg.m:218-220)
 C0 = 299792458.0

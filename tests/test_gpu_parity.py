@@ -205,3 +205,35 @@ def test_mex_entry_points_on_device(efie_mod, orc):
                     np.array([pe, ge, 1.0, spread, 0.2]), opts)
     assert _entry_rel(Z[g["rows"] - 1], g["zrows"]) <= ZTOL
     assert np.abs(Vr[:, 0] - g["vrhs"]).max() <= 1e-15
+
+
+# ------------------------------------------- singularity extraction --
+
+@pytest.mark.parametrize("ok", [0, 1, 2])
+def test_extract_full_z_tiny(efie_mod, orc, ctx, ok):
+    """near_mode EXTRACT (closed-form static part + smooth moment series)
+    against the oracle's extraction (orc_pair_extract) on the whole Z."""
+    V, F, r, pe, ge, spread, P = _setup(efie_mod, orc, "tiny")
+    ctx.set_problem(V, F, r)
+    o = dict(W.OPTS_C1)
+    o.update(OPTSETS[ok])
+    w = 2 * np.pi * 2.5e9
+    eps = W.EPS0 * (2.2 - 0.01j)
+    k = orc.wavenumber(w, W.MU0, eps)
+    Zg, _ = ctx.assemble_zv(k, w, W.MU0, eps, [pe, ge, 1.0, spread, 0.2],
+                            efie_mod.opts(**o, near_mode=efie_mod.NEAR_EXTRACT))
+    Zo = P.z_full(k, w, W.MU0, eps, orc.opts(**o, extract=1))
+    assert _entry_rel(Zg, Zo) <= ZTOL
+    assert np.array_equal(Zg, Zg.T)
+
+
+def test_extract_sweep_tiny(efie_mod, orc, ctx):
+    V, F, r, pe, ge, spread, P = _setup(efie_mod, orc, "tiny")
+    ctx.set_problem(V, F, r)
+    fr = W.freqs("tiny")
+    zo, so = P.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0, orc.opts(**W.OPTS_C1, extract=1))
+    for prec in (0, 1):
+        zin, s11 = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0,
+                             efie_mod.opts(**W.OPTS_C1, precision=prec, near_mode=efie_mod.NEAR_EXTRACT))
+        assert np.max(np.abs(zin - zo) / np.abs(zo)) <= ZIN_TOL
+        assert np.max(np.abs(s11 - so)) <= ZIN_TOL
