@@ -9,7 +9,9 @@ resident in HBM.  Two paths are timed:
         (as the reference does) and an FP64 LU;
   fast  the headline path: near and self pairs from frequency-independent
         moments, complex FP32 LU on the matrix cores and FP64 iterative
-        refinement.
+        refinement;
+  surrogate  as fast, with the near pairs from the learned surrogate (trained
+        on the GPU first, outside the timed sweeps).
 `value` is the fast path's frequency points per second; the JSON line also
 carries the fp64 path's rate, the largest |dZin|/|Zin| between the two, a
 roofline entry for the dominant kernel (from a profiled extra sweep) and the
@@ -171,6 +173,7 @@ def main():
     ap.add_argument("--config", default="c2_6k")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--nf", type=int, default=0, help="use only the first NF frequencies (profiling runs)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run the plain copy kernel on this many MiB (counter calibration)")
@@ -202,6 +205,27 @@ def main():
             res["fp64"] = {"value": len(fr) * a.steps / dt_64, "ms_per_step": 1e3 * dt_64 / a.steps,
                            "stats": st_64}
             res["max_rel_err_zin_fast_vs_fp64"] = float(np.max(np.abs(zin_f - zin_64) / np.abs(zin_64)))
+        if not a.no_surrogate:
+            # learned near pairs: train on the GPU (one-off, outside the timed
+            # sweeps), then the mixed + batched + surrogate sweep
+            t0 = time.perf_counter()
+            rep = ctx.sur_train(efie.opts(**W.OPTS_C1))
+            t_train = time.perf_counter() - t0
+            log(f"surrogate trained in {t_train:.2f} s")
+            osur = efie.opts(**W.OPTS_C1, precision=efie.PREC_MIXED, near_mode=efie.NEAR_SURROGATE, ir_tol=1e-10)
+            dt_s, (zin_s, _, st_s) = timed(ctx, osur, fr, pe, ge, spread, a.steps, a.warmup, "surrogate")
+            _, _, st_sp = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0,
+                                    efie.opts(**W.OPTS_C1, precision=efie.PREC_MIXED,
+                                              near_mode=efie.NEAR_SURROGATE, ir_tol=1e-10, profile=1), stats=True)
+            sur = {"value": len(fr) * a.steps / dt_s, "ms_per_step": 1e3 * dt_s / a.steps,
+                   "train_s": t_train, "train": rep, "n_sur_pairs": st_s["n_sur_pairs"],
+                   "n_sur_fallback": st_s["n_sur_fallback"],
+                   "near_stage_ms": {k: st_sp["kernels"][k]["ms"] for k in ("surrogate", "near", "poly")
+                                     if k in st_sp["kernels"]}}
+            if not a.no_fp64:
+                sur["max_rel_err_zin_vs_fp64"] = float(np.max(np.abs(zin_s - zin_64) / np.abs(zin_64)))
+                sur["vs_fp64"] = sur["value"] / res["fp64"]["value"]
+            res["surrogate"] = sur
         # profiled pass of the headline path for the roofline
         oprof = efie.opts(**W.OPTS_C1, precision=efie.PREC_MIXED, near_mode=efie.NEAR_MOMENTS, ir_tol=1e-10,
                       profile=1)
@@ -223,6 +247,7 @@ def main():
                                f"({fr[0]/1e9:.1f}-{fr[-1]/1e9:.1f} GHz), 7-pt regular rule, 6-pt x16 near/self"},
         "roofline": roofline(st_p, n, len(fr), True),
         "fast": {"stats": st_f},
+        "fast_vs_fp64": (value / res["fp64"]["value"]) if "fp64" in res else None,
         **res,
         "profile": st_p["kernels"],
     }

@@ -73,8 +73,12 @@ typedef struct {
  * extraction: the static 1/(4 pi R) part of the inner integral in closed
  * form over the whole source triangle, the smooth remainder
  * (exp(-jkR) - 1)/(4 pi R) by the subdivided near/self rule (as a moment
- * series).  It has no counterpart in the reference. */
-enum { EFIE_NEAR_DIRECT = 0, EFIE_NEAR_MOMENTS = 1, EFIE_NEAR_EXTRACT = 2 };
+ * series).  It has no counterpart in the reference.  SURROGATE (declared
+ * with its training entry points in efie_surrogate.h) replaces the
+ * subdivision quadrature of near pairs by the regular product rule plus a
+ * learned correction, with the quadrature as fallback outside the trained
+ * envelope; self pairs as MOMENTS. */
+enum { EFIE_NEAR_DIRECT = 0, EFIE_NEAR_MOMENTS = 1, EFIE_NEAR_EXTRACT = 2, EFIE_NEAR_SURROGATE = 3 };
 
 /* Kernel classes timed when efie_opts.profile is set. */
 enum {
@@ -89,7 +93,8 @@ enum {
     EFIE_K_GEMM = 8,     /* LU trailing update (matrix cores)                 */
     EFIE_K_TRSV = 9,     /* triangular solves                                 */
     EFIE_K_OTHER = 10,   /* conversions, residuals, epilogue                  */
-    EFIE_NKCLASS = 11
+    EFIE_K_SURROGATE = 11, /* near-pair network corrections                   */
+    EFIE_NKCLASS = 12
 };
 
 /* Per-call timings (HIP events on the context stream) and counters. */
@@ -107,6 +112,10 @@ typedef struct {
     /* profile == 1: summed device time and launch count per kernel class  */
     double ms_kernel[EFIE_NKCLASS];
     int n_kernel[EFIE_NKCLASS];
+    /* near_mode SURROGATE: near pairs given by the network, and those
+     * integrated by the quadrature because they left its envelope        */
+    long long n_sur_pairs;
+    long long n_sur_fallback;
 } efie_stats;
 
 /* "accurate" preset (main_patch_efie_rwg.m:281-289), FP64, quadrature. */

@@ -3,6 +3,7 @@
 #include "../../include/efie.h"
 #include "../../include/efie_diag.h"
 #include "efie_internal.h"
+#include "efie_sur.h"
 
 #include <hip/hip_runtime.h>
 
@@ -148,6 +149,35 @@ struct GeoKey {
 
 }  // namespace
 
+namespace efie {
+
+efie_status sur_fail(efie_status st, const char* id, const char* msg) { return fail(st, id, "%s", msg); }
+
+void sur_rule_tables(int quad_order, int near_order, int near_subdiv, std::vector<double>& sub,
+                     std::vector<double>& reg) {
+    // rules mapped onto the unit vectors: the mapped points are barycentric
+    double u[16], v[16], w[16];
+    const V3 A{1, 0, 0}, B{0, 1, 0}, C{0, 0, 1};
+    reg.clear();
+    sub.clear();
+    const int nq = quad_rule(quad_order, u, v, w);
+    for (int i = 0; i < nq; ++i) {
+        const V3 b = map_ref(A, B, C, u[i], v[i]);
+        reg.insert(reg.end(), {b.x, b.y, b.z, w[i]});
+    }
+    const int n2 = quad_rule(near_order, u, v, w);
+    std::vector<V3> kids;
+    subdivide(A, B, C, near_subdiv, kids);
+    const double frac = 1.0 / (double)(1 << (2 * near_subdiv));
+    for (size_t p = 0; p < kids.size(); p += 3)
+        for (int i = 0; i < n2; ++i) {
+            const V3 b = map_ref(kids[p], kids[p + 1], kids[p + 2], u[i], v[i]);
+            sub.insert(sub.end(), {b.x, b.y, b.z, w[i] * frac});
+        }
+}
+
+}  // namespace efie
+
 namespace {
 struct RawE {
     int a, b, raw;
@@ -207,7 +237,18 @@ struct efie_ctx {
     int momP = -1;
     double nearAlphaMom = -1;
     bool momExtract = false;
+    bool momSelfOnly = false;  // only the self-pair moments were computed
     DBuf dTriV;  // triangle vertices relative to the centroid [Nt][9]
+
+    // near-pair surrogate: the installed model and, per mesh, the pairs it
+    // takes (dSurPairs, nSurIn), the pairs left to the quadrature (dSurFb,
+    // nSurFb) and the near map without the former (dSurMat)
+    std::vector<float> surBlob;
+    bool surExact = false, surValid = false;
+    double surAlpha = -1, surLmax = 0, surDmax = 0;
+    int nSurIn = 0, nSurFb = 0, surNp = 0, surNq = 0;
+    DBuf dSurModel, dSurPairs, dSurFb, dSurMat, dSurTri18, dSurT, dSurSub, dSurReg;
+    DBuf dMlpA, dMlpB, dMlpC, dMlpD, dMlpE;  // scratch of the network entry points
 
     DBuf dTri, dReg, dNearPts, dSelfPts, dASelf, dAeq, dNearMat, dRowCnt, dList;
     DBuf dSideTri, dSide, dMid, dLen, dQn, dQs, dRad;
@@ -600,7 +641,8 @@ efie_status check_opts(const efie_opts* o) {
     if (o->model != EFIE_MODEL_QUADRATURE) return fail(EFIE_E_ARGS, "efie:opts", "unknown model");
     if (o->near_subdiv < 0 || o->near_subdiv > 3 || o->self_subdiv < 0 || o->self_subdiv > 3)
         return fail(EFIE_E_ARGS, "efie:opts", "subdivision depth must be 0..3");
-    if (o->near_mode != EFIE_NEAR_DIRECT && o->near_mode != EFIE_NEAR_MOMENTS && o->near_mode != EFIE_NEAR_EXTRACT)
+    if (o->near_mode != EFIE_NEAR_DIRECT && o->near_mode != EFIE_NEAR_MOMENTS && o->near_mode != EFIE_NEAR_EXTRACT &&
+        o->near_mode != EFIE_NEAR_SURROGATE)
         return fail(EFIE_E_ARGS, "efie:opts", "unknown near_mode");
     return ok();
 }
@@ -784,6 +826,7 @@ efie_status ensure_geometry(efie_ctx* c, const efie_opts* o) {
     c->key = k;
     c->geoValid = true;
     c->momP = -1;  // moments belong to the old geometry
+    c->surValid = false;
     return ok();
 }
 
@@ -806,18 +849,20 @@ int moment_terms(const efie_ctx* c, const efie_opts* o, double kmax) {
     return -1;
 }
 
-// Moments of the near and self pairs for series length P+1 (cached; a
-// longer series than cached recomputes them).
-efie_status ensure_moments(efie_ctx* c, const efie_opts* o, int P, efie_stats* st) {
+// Moments of the near and self pairs (selfOnly: self pairs only) for
+// series length P+1 (cached; a longer series than cached recomputes them).
+efie_status ensure_moments(efie_ctx* c, const efie_opts* o, int P, efie_stats* st, bool selfOnly) {
     const bool extract = o->near_mode == EFIE_NEAR_EXTRACT;
-    if (c->momP >= P && c->momExtract == extract && (extract || c->nearAlphaMom == o->near_alpha)) return ok();
+    if (c->momP >= P && c->momExtract == extract && (extract || c->nearAlphaMom == o->near_alpha) &&
+        (selfOnly || !c->momSelfOnly))
+        return ok();
     hipStream_t s = c->stream;
     const size_t per = (size_t)(P + 1) * 8;
     HIPCHK(c->dQn.ensure(sizeof(double) * per * (size_t)std::max(1LL, c->nNear)));
     HIPCHK(c->dQs.ensure(sizeof(double) * per * (size_t)c->Nt));
     HIPCHK(hipEventRecord(c->ev[6], s));
     HIPCHK(launch_moments(c->dNearPts.as<Pt>(), c->NpNear, c->dTri.as<TriGeo>(), c->dAeq.as<double>(), o->near_alpha,
-                          c->dList.as<int2>(), (int)c->nNear, c->dSelfPts.as<Pt>(), c->Nq2s, c->NsSelf,
+                          c->dList.as<int2>(), selfOnly ? 0 : (int)c->nNear, c->dSelfPts.as<Pt>(), c->Nq2s, c->NsSelf,
                           c->dASelf.as<double>(), c->Nt, P, c->dQn.as<double>(), c->dQs.as<double>(),
                           extract ? c->dTriV.as<double>() : nullptr, s));
     HIPCHK(hipEventRecord(c->ev[7], s));
@@ -828,6 +873,7 @@ efie_status ensure_moments(efie_ctx* c, const efie_opts* o, int P, efie_stats* s
     c->momP = P;
     c->nearAlphaMom = o->near_alpha;
     c->momExtract = extract;
+    c->momSelfOnly = selfOnly;
     return ok();
 }
 
@@ -899,12 +945,18 @@ Freq make_freq(std::complex<double> k, double omega, double mu, std::complex<dou
     return f;
 }
 
+// How the near and self pairs of one call are integrated.
+struct NearPlan {
+    bool momNear = false;  // near-pair records from the cached moments
+    bool momSelf = false;  // self-pair records from the cached moments
+    bool sur = false;      // near pairs in the surrogate's envelope: regular rule + network
+};
+
 // Fill one Z (interleaved complex, element (m,n) at Z[2*(m*ld + n)], which
-// for the symmetric Z is also the column-major matrix).  useMoments:
-// near/self records from the cached moments (momP terms).  With Z32
+// for the symmetric Z is also the column-major matrix).  With Z32
 // non-null the quadrature model also writes the complex64 copy
 // (Z32[2*(m*ld + n)]) and sets *wrote32.
-efie_status fill_z(efie_ctx* c, const efie_opts* o, bool useMoments, const Freq* dfr, double* Z, size_t ld,
+efie_status fill_z(efie_ctx* c, const efie_opts* o, const NearPlan& plan, const Freq* dfr, double* Z, size_t ld,
                    float* Z32 = nullptr, bool* wrote32 = nullptr, hipStream_t s = nullptr) {
     if (!s) s = c->stream;
     Prof pf = make_prof(c);
@@ -920,26 +972,42 @@ efie_status fill_z(efie_ctx* c, const efie_opts* o, bool useMoments, const Freq*
     const int Nt = c->Nt;
     HIPCHK(c->dRecs.ensure(sizeof(Rec) * rec_index(0, Nt)));
     Rec* recs = c->dRecs.as<Rec>();
-    if (useMoments) {
+    if (plan.momSelf) {
         pf.begin(EFIE_K_POLY);
-        HIPCHK(launch_poly_records(c->dQn.as<double>(), c->dList.as<int2>(), (int)c->nNear, c->dQs.as<double>(), Nt,
-                                   c->momP, dfr, recs, s));
+        HIPCHK(launch_poly_records(c->dQn.as<double>(), c->dList.as<int2>(), plan.momNear ? (int)c->nNear : 0,
+                                   c->dQs.as<double>(), Nt, c->momP, dfr, recs, s));
         pf.end(EFIE_K_POLY);
     } else {
         pf.begin(EFIE_K_SELF);
         HIPCHK(launch_rec_self(c->dSelfPts.as<Pt>(), c->Nq2s, c->NsSelf, c->dASelf.as<double>(), c->dRad.as<double>(),
                                c->aSelfMax, Nt, dfr, recs, s));
         pf.end(EFIE_K_SELF);
+    }
+    const int nDirect = plan.momNear ? 0 : plan.sur ? c->nSurFb : (int)c->nNear;
+    if (nDirect > 0) {
         pf.begin(EFIE_K_NEAR);
         HIPCHK(launch_rec_near(c->dNearPts.as<Pt>(), c->NpNear, c->dTri.as<TriGeo>(), c->dRad.as<double>(),
-                               c->dAeq.as<double>(),
-                               o->near_alpha, c->dList.as<int2>(), (int)c->nNear, dfr, recs, s));
+                               c->dAeq.as<double>(), o->near_alpha,
+                               plan.sur ? c->dSurFb.as<int2>() : c->dList.as<int2>(), nDirect, dfr, recs, s));
         pf.end(EFIE_K_NEAR);
     }
     pf.begin(EFIE_K_REGULAR);
     HIPCHK(launch_rec_regular(c->dReg.as<Pt>(), c->Nq, c->dTri.as<TriGeo>(), c->dRad.as<double>(), Nt,
-                              c->dNearMat.as<uint8_t>(), dfr, recs, s));
+                              plan.sur ? c->dSurMat.as<uint8_t>() : c->dNearMat.as<uint8_t>(), dfr, recs, s));
     pf.end(EFIE_K_REGULAR);
+    if (plan.sur) {
+        // the regular rule wrote the base records of the surrogate's pairs
+        pf.begin(EFIE_K_SURROGATE);
+        const double* Tb = nullptr;
+        if (c->surExact) {
+            HIPCHK(launch_sur_labels(c->dSurTri18.as<double>(), nullptr, dfr, c->dSurPairs.as<SurPair>(), c->nSurIn,
+                                     c->dSurSub.as<double>(), c->surNp, c->dSurReg.as<double>(), c->surNq,
+                                     o->near_alpha, c->dSurT.as<double>(), nullptr, s));
+            Tb = c->dSurT.as<double>();
+        }
+        HIPCHK(launch_sur_apply(c->dSurPairs.as<SurPair>(), c->nSurIn, c->dSurModel.as<float>(), Tb, dfr, recs, s));
+        pf.end(EFIE_K_SURROGATE, c->surExact ? 2 : 1);
+    }
     pf.begin(EFIE_K_COMBINE);
     HIPCHK(launch_combine(recs, c->dSideTri.as<int>(), c->dSide.as<Side>(), c->Ne, dfr, Z, ld, Z32, ld, s));
     pf.end(EFIE_K_COMBINE);
@@ -981,11 +1049,110 @@ efie_status prepare(efie_ctx* c, const efie_opts* o, double selfAlphaCentroid) {
     return ensure_geometry(c, o);
 }
 
-// Decide the near/self integration for a set of wavenumbers: the moment
-// series when asked for and short enough, otherwise direct quadrature.
-efie_status choose_near(efie_ctx* c, const efie_opts* o, double kmax, efie_stats* st, bool* useMoments) {
-    *useMoments = false;
+// Per-mesh data of the surrogate: the near pairs inside the installed
+// model's envelope (frames, features), the rest, and the near map without
+// the former.  Quadrature options other than the trained ones leave every
+// pair to the quadrature.
+efie_status ensure_surrogate(efie_ctx* c, const efie_opts* o) {
+    if (c->surValid && c->surAlpha == o->near_alpha) return ok();
+    hipStream_t s = c->stream;
+    const float* env = c->surBlob.data() + kSurModel;
+    const float* meta = env + 2 * kMlpIn;
+    const bool optsOk = (float)o->near_alpha == meta[kMetaNearAlpha] && o->near_subdiv == (int)meta[kMetaNearSubdiv] &&
+                        o->near_order == (int)meta[kMetaNearOrder] && o->quad_order == (int)meta[kMetaQuadOrder];
+    const long long nN = c->nNear;
+    std::vector<int2> list((size_t)nN);
+    if (nN > 0) {
+        HIPCHK(hipMemcpyAsync(list.data(), c->dList.p, sizeof(int2) * (size_t)nN, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    std::vector<SurPair> in;
+    std::vector<int2> fb;
+    std::vector<double> t18;
+    double lmax = 0, dmax = 0;
+    for (long long i = 0; i < nN; ++i) {
+        const int t1 = list[i].x, t2 = list[i].y;
+        V3 v[6];
+        tri_vertices(c, t1, v[0], v[1], v[2]);
+        tri_vertices(c, t2, v[3], v[4], v[5]);
+        double w[18], tt[18];
+        for (int q = 0; q < 6; ++q) {
+            w[3 * q] = v[q].x;
+            w[3 * q + 1] = v[q].y;
+            w[3 * q + 2] = v[q].z;
+        }
+        SurPair p;
+        const double dl = sur_frame(w, w + 9, &p, tt);
+        bool inside = optsOk && dl >= meta[kMetaDlMin] && dl <= meta[kMetaDlMax];
+        for (int f = 0; f < 12 && inside; ++f) inside = p.feat[f] >= env[f] && p.feat[f] <= env[kMlpIn + f];
+        if (!inside) {
+            fb.push_back(list[i]);
+            continue;
+        }
+        p.t1 = t1;
+        p.t2 = t2;
+        in.push_back(p);
+        t18.insert(t18.end(), tt, tt + 18);
+        lmax = std::max(lmax, p.L);
+        dmax = std::max(dmax, p.d);
+    }
+    std::vector<double> sub, reg;
+    sur_rule_tables(o->quad_order, o->near_order, o->near_subdiv, sub, reg);
+    const size_t nIn = in.size(), nFb = fb.size();
+    HIPCHK(c->dSurPairs.ensure(sizeof(SurPair) * std::max<size_t>(1, nIn)));
+    HIPCHK(c->dSurTri18.ensure(sizeof(double) * 18 * std::max<size_t>(1, nIn)));
+    HIPCHK(c->dSurT.ensure(sizeof(double) * 16 * std::max<size_t>(1, nIn)));
+    HIPCHK(c->dSurFb.ensure(sizeof(int2) * std::max<size_t>(1, nFb)));
+    HIPCHK(c->dSurMat.ensure((size_t)c->Nt * c->Nt));
+    HIPCHK(c->dSurModel.ensure(sizeof(float) * kSurModel));
+    HIPCHK(c->dSurSub.ensure(sizeof(double) * sub.size()));
+    HIPCHK(c->dSurReg.ensure(sizeof(double) * reg.size()));
+    if (nIn) {
+        HIPCHK(hipMemcpyAsync(c->dSurPairs.p, in.data(), sizeof(SurPair) * nIn, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(c->dSurTri18.p, t18.data(), sizeof(double) * t18.size(), hipMemcpyHostToDevice, s));
+    }
+    if (nFb) HIPCHK(hipMemcpyAsync(c->dSurFb.p, fb.data(), sizeof(int2) * nFb, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->dSurModel.p, c->surBlob.data(), sizeof(float) * kSurModel, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->dSurSub.p, sub.data(), sizeof(double) * sub.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->dSurReg.p, reg.data(), sizeof(double) * reg.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(launch_sur_map(c->dNearMat.as<uint8_t>(), c->dSurPairs.as<SurPair>(), (int)nIn, c->Nt,
+                          c->dSurMat.as<uint8_t>(), s));
+    HIPCHK(hipStreamSynchronize(s));
+    c->nSurIn = (int)nIn;
+    c->nSurFb = (int)nFb;
+    c->surNp = (int)(sub.size() / 4);
+    c->surNq = (int)(reg.size() / 4);
+    c->surLmax = lmax;
+    c->surDmax = dmax;
+    c->surAlpha = o->near_alpha;
+    c->surValid = true;
+    return ok();
+}
+
+// Decide the near/self integration for a set of wavenumbers (largest
+// |k| kmax; lossy: some k is complex): the moment series when asked for
+// and short enough, otherwise direct quadrature; near_mode SURROGATE takes
+// the near pairs inside the model's envelope when every k of the call is
+// inside it too.
+efie_status choose_near(efie_ctx* c, const efie_opts* o, double kmax, bool lossy, efie_stats* st,
+                        NearPlan* plan) {
+    *plan = NearPlan();
     if (o->model != EFIE_MODEL_QUADRATURE || o->near_mode == EFIE_NEAR_DIRECT) return ok();
+    const bool sur = o->near_mode == EFIE_NEAR_SURROGATE;
+    if (sur) {
+        if (c->surBlob.empty())
+            return fail(EFIE_E_STATE, "efie:surrogate", "near_mode surrogate needs a model (efie_sur_train/efie_sur_set)");
+        efie_status e = ensure_surrogate(c, o);
+        if (e) return e;
+        const float* env = c->surBlob.data() + kSurModel;
+        const float* meta = env + 2 * kMlpIn;
+        plan->sur = c->nSurIn > 0 && !lossy && kmax * c->surLmax <= meta[kMetaKlMax] &&
+                    kmax * c->surDmax <= env[kMlpIn + 12];
+        if (st) {
+            st->n_sur_pairs = plan->sur ? c->nSurIn : 0;
+            st->n_sur_fallback = plan->sur ? c->nSurFb : c->nNear;
+        }
+    }
     const int P = moment_terms(c, o, kmax);
     if (P < 0) {
         // the extraction exists only in series form
@@ -993,9 +1160,10 @@ efie_status choose_near(efie_ctx* c, const efie_opts* o, double kmax, efie_stats
             return fail(EFIE_E_ARGS, "efie:opts", "near_mode extract: |k| times the near-pair size is too large");
         return ok();
     }
-    efie_status e = ensure_moments(c, o, P, st);
+    efie_status e = ensure_moments(c, o, P, st, sur);
     if (e) return e;
-    *useMoments = true;
+    plan->momSelf = true;
+    plan->momNear = !sur;
     if (st) st->moment_terms = c->momP + 1;
     return ok();
 }
@@ -1258,8 +1426,8 @@ efie_status efie_assemble_zv(efie_ctx* c, double k_re, double k_im, double omega
     c->prof = false;
     st = make_rhs(c, pe, ge, exc[2], exc[3]);
     if (st) return st;
-    bool mom = false;
-    st = choose_near(c, &o, std::hypot(k_re, k_im), nullptr, &mom);
+    NearPlan plan;
+    st = choose_near(c, &o, std::hypot(k_re, k_im), k_im != 0.0, nullptr, &plan);
     if (st) return st;
     const int Ne = c->Ne;
     hipStream_t s = c->stream;
@@ -1277,7 +1445,7 @@ efie_status efie_assemble_zv(efie_ctx* c, double k_re, double k_im, double omega
     HIPCHK(c->dFreq.ensure(sizeof(Freq)));
     HIPCHK(hipMemcpyAsync(c->dFreq.p, &fr, sizeof fr, hipMemcpyHostToDevice, s));
     HIPCHK(c->dZ.ensure(sizeof(double) * 2 * (size_t)Ne * Ne));
-    st = fill_z(c, &o, mom, c->dFreq.as<Freq>(), c->dZ.as<double>(), (size_t)Ne);
+    st = fill_z(c, &o, plan, c->dFreq.as<Freq>(), c->dZ.as<double>(), (size_t)Ne);
     if (st) return st;
     HIPCHK(hipMemcpyAsync(Z, c->dZ.p, sizeof(double) * 2 * (size_t)Ne * Ne, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(Vrhs, c->dV.p, sizeof(double) * 2 * (size_t)Ne, hipMemcpyDeviceToHost, s));
@@ -1319,8 +1487,10 @@ efie_status efie_sweep(efie_ctx* c, const double* freqs, int Nf, double mu, doub
         ks[f] = omega * std::sqrt(std::complex<double>(mu, 0.0) * eps);
         kmax = std::max(kmax, std::abs(ks[f]));
     }
-    bool mom = false;
-    st = choose_near(c, &o, kmax, S, &mom);
+    bool lossy = false;
+    for (int f = 0; f < Nf; ++f) lossy = lossy || ks[f].imag() != 0.0;
+    NearPlan plan;
+    st = choose_near(c, &o, kmax, lossy, S, &plan);
     if (st) return st;
 
     const int n = c->Ne;
@@ -1367,7 +1537,7 @@ efie_status efie_sweep(efie_ctx* c, const double* freqs, int Nf, double mu, doub
         bool h32 = Z32b != nullptr;
         for (int i = 0; i < nb; ++i) {
             bool w32 = false;
-            efie_status e = fill_z(c, &o, mom, dfr + i, Zb + 2 * zElems * i, (size_t)lda,
+            efie_status e = fill_z(c, &o, plan, dfr + i, Zb + 2 * zElems * i, (size_t)lda,
                                    Z32b ? Z32b + 2 * zElems * i : nullptr, &w32, sf);
             if (e) return e;
             h32 = h32 && w32;
@@ -1475,7 +1645,9 @@ efie_status efie_sweep(efie_ctx* c, const double* freqs, int Nf, double mu, doub
     if (o.model == EFIE_MODEL_QUADRATURE) {
         const long long Nt = c->Nt, np = (long long)c->NpNear, ns = (long long)c->NsSelf * c->Nq2s;
         const long long reg = Nt * (Nt - 1) / 2 - c->nNear;
-        S->n_green = reg * c->Nq * c->Nq + (mom ? 0 : c->nNear * np * np + Nt * ns * ns);
+        const long long nearDirect = plan.momNear ? 0 : plan.sur ? c->nSurFb : c->nNear;
+        S->n_green = (reg + (plan.sur ? c->nSurIn : 0)) * c->Nq * c->Nq + nearDirect * np * np +
+                     (plan.momSelf ? 0 : Nt * ns * ns);
     }
     c->prof = false;
     return ok();
@@ -1548,6 +1720,98 @@ efie_status efie_zgesv(efie_ctx* c, int n, int batch, const double* A, const dou
     S->ms_factor = t2;
     S->ms_solve = t3;
     S->ms_total = t2 + t3;
+    return ok();
+}
+
+// ------------------------------------------------------------ surrogate --
+
+void efie_sur_cfg_default(efie_sur_cfg* g) {
+    if (!g) return;
+    g->n_pairs = 2000000;
+    g->n_val = 20000;
+    g->epochs = 60;
+    g->batch = 4096;
+    g->lr = 2e-3;
+    g->kl_max = 0.35;
+    g->seed = 1;
+}
+
+int efie_sur_blob_floats(void) { return kSurBlob; }
+int efie_mlp_params(void) { return kMlpParams; }
+
+efie_status efie_sur_train(efie_ctx* c, const efie_opts* opts, const efie_sur_cfg* cfg, efie_sur_report* rep) {
+    if (!c) return fail(EFIE_E_ARGS, "efie:args", "null context");
+    HIPCHK(hipSetDevice(c->device));
+    efie_opts o;
+    if (opts) o = *opts;
+    else efie_opts_default(&o);
+    efie_sur_cfg g;
+    if (cfg) g = *cfg;
+    else efie_sur_cfg_default(&g);
+    std::vector<float> blob;
+    efie_status st = sur_train(c->stream, &o, &g, blob, rep);
+    if (st) return st;
+    c->surBlob.swap(blob);
+    c->surValid = false;
+    return ok();
+}
+
+efie_status efie_sur_get(efie_ctx* c, float* blob) {
+    if (!c || !blob) return fail(EFIE_E_ARGS, "efie:args", "null argument");
+    if (c->surBlob.empty()) return fail(EFIE_E_STATE, "efie:surrogate", "no model installed");
+    std::copy(c->surBlob.begin(), c->surBlob.end(), blob);
+    return ok();
+}
+
+efie_status efie_sur_set(efie_ctx* c, const float* blob) {
+    if (!c || !blob) return fail(EFIE_E_ARGS, "efie:args", "null argument");
+    if (blob[kSurModel + 2 * kMlpIn + kMetaVersion] != kSurVersion)
+        return fail(EFIE_E_ARGS, "efie:surrogate", "not a surrogate model blob of this version");
+    c->surBlob.assign(blob, blob + kSurBlob);
+    c->surValid = false;
+    return ok();
+}
+
+efie_status efie_sur_set_exact(efie_ctx* c, int on) {
+    if (!c) return fail(EFIE_E_ARGS, "efie:args", "null context");
+    c->surExact = on != 0;
+    return ok();
+}
+
+efie_status efie_mlp_forward(efie_ctx* c, const float* params, const float* X, int N, float* Y) {
+    if (!c || !params || !X || !Y || N < 1) return fail(EFIE_E_ARGS, "efie:args", "bad arguments");
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    HIPCHK(c->dMlpA.ensure(sizeof(float) * kMlpParams));
+    HIPCHK(c->dMlpB.ensure(sizeof(float) * kMlpIn * (size_t)N));
+    HIPCHK(c->dMlpC.ensure(sizeof(float) * kMlpOut * (size_t)N));
+    HIPCHK(hipMemcpyAsync(c->dMlpA.p, params, sizeof(float) * kMlpParams, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->dMlpB.p, X, sizeof(float) * kMlpIn * (size_t)N, hipMemcpyHostToDevice, s));
+    HIPCHK(launch_mlp_forward(c->dMlpB.as<float>(), c->dMlpA.as<float>(), c->dMlpC.as<float>(), N, s));
+    HIPCHK(hipMemcpyAsync(Y, c->dMlpC.p, sizeof(float) * kMlpOut * (size_t)N, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return ok();
+}
+
+efie_status efie_mlp_loss_grad(efie_ctx* c, const float* params, const float* X, const float* T, int N, float* loss,
+                               float* grad) {
+    if (!c || !params || !X || !T || !loss || !grad || N < 1) return fail(EFIE_E_ARGS, "efie:args", "bad arguments");
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    HIPCHK(c->dMlpA.ensure(sizeof(float) * (kMlpParams + 1)));
+    HIPCHK(c->dMlpB.ensure(sizeof(float) * kMlpIn * (size_t)N));
+    HIPCHK(c->dMlpC.ensure(sizeof(float) * kMlpOut * (size_t)N));
+    HIPCHK(c->dMlpD.ensure(sizeof(float) * mlp_grad_work_floats(N)));
+    HIPCHK(c->dMlpE.ensure(sizeof(float) * kMlpParams));
+    HIPCHK(hipMemcpyAsync(c->dMlpA.p, params, sizeof(float) * kMlpParams, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->dMlpB.p, X, sizeof(float) * kMlpIn * (size_t)N, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->dMlpC.p, T, sizeof(float) * kMlpOut * (size_t)N, hipMemcpyHostToDevice, s));
+    float* dLoss = c->dMlpA.as<float>() + kMlpParams;
+    HIPCHK(launch_mlp_loss_grad(c->dMlpB.as<float>(), c->dMlpC.as<float>(), c->dMlpA.as<float>(), N,
+                                c->dMlpD.as<float>(), c->dMlpE.as<float>(), dLoss, s));
+    HIPCHK(hipMemcpyAsync(grad, c->dMlpE.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(loss, dLoss, sizeof(float), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
     return ok();
 }
 

@@ -298,11 +298,12 @@ __global__ __launch_bounds__(256) void k_rec_regular(const Pt* __restrict__ pts,
 // share a row and split its j range.  Partial moments are reduced across
 // the wave at the end.
 
+// The wave-reduced moments end up in res[16] of every lane.
 template <bool SELF>
-__device__ void subdiv_pair(const Pt* __restrict__ P1, const Pt* __restrict__ P2, int Np,
-                            int Nq2, double ox, double oy, double oz, double aNear2,
-                            const double* __restrict__ aSelf, double kr, double ki, double dmax,
-                            Rec* __restrict__ out) {
+__device__ __forceinline__ void subdiv_moments(const Pt* __restrict__ P1, const Pt* __restrict__ P2, int Np,
+                                               int Nq2, double ox, double oy, double oz, double aNear2,
+                                               const double* __restrict__ aSelf, double kr, double ki,
+                                               double dmax, double* res) {
     const int lane = threadIdx.x & 63;
     // the softened distance stays within dmax of sqrt(|o|^2 + a^2)
     const PairGreen pg = pair_green(kr, ki, sqrt(ox * ox + oy * oy + oz * oz + aNear2), dmax);
@@ -348,10 +349,18 @@ __device__ void subdiv_pair(const Pt* __restrict__ P1, const Pt* __restrict__ P2
     }
     acc_scale(a, pg.e0r, pg.e0i);
     double* v = &a.s0r;
-    double res[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) res[k] = wave_sum(v[k]);
-    if (lane == 0) {
+}
+
+template <bool SELF>
+__device__ void subdiv_pair(const Pt* __restrict__ P1, const Pt* __restrict__ P2, int Np,
+                            int Nq2, double ox, double oy, double oz, double aNear2,
+                            const double* __restrict__ aSelf, double kr, double ki, double dmax,
+                            Rec* __restrict__ out) {
+    double res[16];
+    subdiv_moments<SELF>(P1, P2, Np, Nq2, ox, oy, oz, aNear2, aSelf, kr, ki, dmax, res);
+    if ((threadIdx.x & 63) == 0) {
         double2* p = reinterpret_cast<double2*>(out->v);
 #pragma unroll
         for (int k = 0; k < 8; ++k) p[k] = make_double2(res[2 * k], res[2 * k + 1]);
@@ -390,6 +399,94 @@ __global__ __launch_bounds__(256) void k_rec_self(const Pt* __restrict__ selfPts
     // self pair: R0 = 0 and |R - R0| <= 2 rad + a
     subdiv_pair<true>(P, P, Np, Nq2, 0.0, 0.0, 0.0, 0.0, aSelf + (size_t)t * Ns, fr->kr, fr->ki,
                       2.0 * rad[t] + aSelfMax, recs + rec_index(t, t));
+}
+
+// ------------------------------------------------ surrogate labels ----
+// One wave per pair of triangles given by 18 normalised vertex coordinates
+// (test triangle t1 first).  The rule points of both rules are formed in
+// LDS from barycentric tables, the subdivided softened near-pair moments
+// (as k_rec_near) and the plain product-rule moments (as k_rec_regular) are
+// integrated by the same wave code, and the label is their difference
+// times exp(+j kt dt) 4 pi, dt being the centroid distance.
+
+__global__ __launch_bounds__(256) void k_sur_labels(const double* __restrict__ tri18, const double* __restrict__ kt,
+                                                    const Freq* __restrict__ fr, const SurPair* __restrict__ pairs,
+                                                    int n, const double* __restrict__ subBary, int Np,
+                                                    const double* __restrict__ regBary, int Nq, double near_alpha,
+                                                    double* __restrict__ T, double* __restrict__ nrm) {
+    extern __shared__ Pt lp[];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int pair = blockIdx.x * 4 + wave;
+    if (pair >= n) return;
+    Pt* s1 = lp + (size_t)wave * 2 * (Np + Nq);
+    Pt* s2 = s1 + Np;
+    Pt* r1 = s2 + Np;
+    Pt* r2 = r1 + Nq;
+    double v[18];
+#pragma unroll
+    for (int i = 0; i < 18; ++i) v[i] = tri18[(size_t)pair * 18 + i];
+    double c[6], a2[2], rad[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const double* w = v + 9 * t;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) c[3 * t + d] = (w[d] + w[3 + d] + w[6 + d]) / 3.0;
+        const double ex = w[3] - w[0], ey = w[4] - w[1], ez = w[5] - w[2];
+        const double fx = w[6] - w[0], fy = w[7] - w[1], fz = w[8] - w[2];
+        const double nx = ey * fz - ez * fy, ny = ez * fx - ex * fz, nz = ex * fy - ey * fx;
+        a2[t] = sqrt(nx * nx + ny * ny + nz * nz);
+        double r = 0.0;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const double dx = w[3 * q] - c[3 * t], dy = w[3 * q + 1] - c[3 * t + 1], dz = w[3 * q + 2] - c[3 * t + 2];
+            r = fmax(r, sqrt(dx * dx + dy * dy + dz * dz));
+        }
+        rad[t] = r;
+    }
+    // rule points relative to each centroid, weight = fraction * twice the area
+    for (int e = lane; e < 2 * (Np + Nq); e += 64) {
+        const bool sub = e < 2 * Np;
+        const int idx = sub ? e % Np : (e - 2 * Np) % Nq;
+        const int t = sub ? e / Np : (e - 2 * Np) / Nq;
+        const double* b = (sub ? subBary : regBary) + 4 * idx;
+        const double* w = v + 9 * t;
+        Pt q;
+        q.x = b[0] * w[0] + b[1] * w[3] + b[2] * w[6] - c[3 * t];
+        q.y = b[0] * w[1] + b[1] * w[4] + b[2] * w[7] - c[3 * t + 1];
+        q.z = b[0] * w[2] + b[1] * w[5] + b[2] * w[8] - c[3 * t + 2];
+        q.w = b[3] * a2[t];
+        lp[(size_t)wave * 2 * (Np + Nq) + e] = q;
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    const double ox = c[0] - c[3], oy = c[1] - c[4], oz = c[2] - c[5];
+    const double ae1 = sqrt(0.5 * a2[0] / kPi), ae2 = sqrt(0.5 * a2[1] / kPi);
+    const double as = near_alpha * (ae1 < ae2 ? ae1 : ae2);
+    const double k = fr ? fr->kr * pairs[pair].L : kt[pair];
+    double resN[16], resB[16];
+    subdiv_moments<false>(s1, s2, Np, Np, ox, oy, oz, as * as, nullptr, k, 0.0, rad[0] + rad[1], resN);
+    subdiv_moments<false>(r1, r2, Nq, Nq, ox, oy, oz, 0.0, nullptr, k, 0.0, rad[0] + rad[1], resB);
+    double sn, cs;
+    sincos_cw(k * sqrt(ox * ox + oy * oy + oz * oz), &sn, &cs);
+    cs *= 4.0 * kPi;
+    sn *= 4.0 * kPi;
+    if (lane < 8) {
+        double dr = 0.0, di = 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if (q == lane) {
+                dr = resN[2 * q] - resB[2 * q];
+                di = resN[2 * q + 1] - resB[2 * q + 1];
+            }
+        T[(size_t)pair * 16 + 2 * lane] = dr * cs - di * sn;
+        T[(size_t)pair * 16 + 2 * lane + 1] = dr * sn + di * cs;
+    }
+    if (nrm && lane == 0) {
+        double q = 0.0;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) q += resN[i] * resN[i];
+        nrm[pair] = 4.0 * kPi * sqrt(q);
+    }
 }
 
 // ------------------------------------ frequency-independent moments ---
@@ -1014,6 +1111,17 @@ hipError_t launch_rec_self(const Pt* selfPts, int Nq2, int Ns, const double* aSe
                            double aSelfMax, int Nt, const Freq* fr, Rec* recs, hipStream_t s) {
     hipLaunchKernelGGL(k_rec_self, dim3((Nt + 3) / 4), dim3(256), 0, s, selfPts, Nq2, Ns, aSelf, rad, aSelfMax,
                        Nt, fr, recs);
+    return hipGetLastError();
+}
+
+hipError_t launch_sur_labels(const double* tri18, const double* kt, const Freq* fr, const SurPair* pairs, int n,
+                             const double* subBary, int Np, const double* regBary, int Nq, double near_alpha,
+                             double* T, double* nrm, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const size_t lds = sizeof(Pt) * 4 * 2 * (size_t)(Np + Nq);
+    if (lds > 64 * 1024) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_sur_labels, dim3((n + 3) / 4), dim3(256), lds, s, tri18, kt, fr, pairs, n, subBary, Np,
+                       regBary, Nq, near_alpha, T, nrm);
     return hipGetLastError();
 }
 

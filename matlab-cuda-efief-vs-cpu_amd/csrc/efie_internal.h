@@ -195,6 +195,59 @@ struct Prof {
     }
 };
 
+// ------------------------------------------------ near-pair surrogate ---
+// Network 13 -> 64 -> 64 -> 64 -> 16 (GELU), FP32, parameters in one flat
+// array W0[64][13] b0 W1[64][64] b1 W2[64][64] b2 W3[16][64] b3.
+constexpr int kMlpIn = 13, kMlpHidden = 64, kMlpOut = 16;
+constexpr int kMlpParams = kMlpHidden * kMlpIn + kMlpHidden + 2 * (kMlpHidden * kMlpHidden + kMlpHidden) +
+                           kMlpOut * kMlpHidden + kMlpOut;
+// Standardisation stored after the parameters in the device copy of a
+// trained model: xmean[13] xinv[13] (1/std) ymean[16] ystd[16].
+constexpr int kMlpNorm = 2 * kMlpIn + 2 * kMlpOut;
+
+// A near pair (t1 < t2) in the test triangle's frame: origin at the
+// centroid of t1, rows of Rm = (e_x along its first edge, e_y, normal),
+// lengths in units of L = sqrt(4 A1 / sqrt(3)); d is the centroid distance.
+// feat: the 12 normalised coordinates (x0, y0, x1 of t1's first two
+// vertices; all of t2's three vertices).
+struct SurPair {
+    int t1, t2;
+    float feat[12];
+    double Rm[9];
+    double L, d;
+};
+
+// Labels of the surrogate: for each pair (18 normalised vertex coordinates,
+// t1's three then t2's three, and the normalised wavenumber kt = k L) the
+// difference between the subdivided near-pair moments (softened, as
+// rwg_pair_integrals_near_singular.m) and the plain product rule of the
+// regular pairs, times exp(+j kt dt) 4 pi: T[p][16] (re, im of S0, S1,
+// S1', S2); nrm[p] (if non-null) = 4 pi |near-pair moments|.  With fr
+// non-null, kt = fr->kr * pairs[p].L instead of kt[p].  subBary/regBary:
+// rule points of the subdivided / regular rule as barycentric weights of
+// the parent's vertices and weight fractions of twice its area ([n][4]).
+hipError_t launch_sur_labels(const double* tri18, const double* kt, const Freq* fr, const SurPair* pairs, int n,
+                             const double* subBary, int Np, const double* regBary, int Nq, double near_alpha,
+                             double* T, double* nrm, hipStream_t s);
+// MLP kernels (plain arrays: X [N][13], Y and targets [N][16]).
+hipError_t launch_mlp_forward(const float* X, const float* params, float* Y, int N, hipStream_t s);
+// Gradient of the loss sum((y - t)^2) / (16 N) over the N samples; work
+// needs mlp_grad_work_floats(N) floats; lossOut gets the loss (one float).
+size_t mlp_grad_work_floats(int N);
+hipError_t launch_mlp_loss_grad(const float* X, const float* T, const float* params, int N, float* work,
+                                float* grad, float* lossOut, hipStream_t s);
+hipError_t launch_adam(float* p, const float* g, float* m, float* v, int n, float lr, int step, hipStream_t s);
+// Add the network's correction to the base records of the n pairs at
+// wavenumber fr->kr (model: params then the kMlpNorm standardisation).
+// With Tbuf non-null the correction is taken from Tbuf (labels, [n][16])
+// instead of the network.
+hipError_t launch_sur_apply(const SurPair* pairs, int n, const float* model, const double* Tbuf, const Freq* fr,
+                            Rec* recs, hipStream_t s);
+// Near map of the surrogate mode: copy of nearMat with the pairs handled by
+// the surrogate cleared (they get the regular rule as base).
+hipError_t launch_sur_map(const uint8_t* nearMat, const SurPair* pairs, int n, int Nt, uint8_t* out,
+                          hipStream_t s);
+
 // Zin epilogue of batch element b: x[e] = X[b*strideX + e*xinc] (complex),
 // Iport = sum(wP.*len.*x) - sum(wG.*len.*x) (or x[p]-x[g] in the centroid
 // model), Zin = V0/Iport, S11 = (Zin-Z0)/(Zin+Z0); ZinS11[4b..4b+3].

@@ -31,7 +31,7 @@ _ip = C.POINTER(C.c_int)
 EFIE_OK = 0
 MODEL_QUADRATURE, MODEL_CENTROID = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
-NEAR_DIRECT, NEAR_MOMENTS, NEAR_EXTRACT = 0, 1, 2
+NEAR_DIRECT, NEAR_MOMENTS, NEAR_EXTRACT, NEAR_SURROGATE = 0, 1, 2, 3
 
 # physics constants of the reference drivers (main_patch_efie_rwg.m:218-220)
 C0 = 299792458.0
@@ -58,7 +58,7 @@ class Opts(C.Structure):
 
 
 KERNEL_CLASSES = ["regular", "near", "self", "poly", "combine", "tslu", "laswp", "panel", "gemm",
-                  "trsv", "other"]
+                  "trsv", "other", "surrogate"]
 
 
 class Stats(C.Structure):
@@ -68,6 +68,7 @@ class Stats(C.Structure):
         ("ir_iters_max", C.c_int), ("ir_resid_max", C.c_double), ("moment_terms", C.c_int),
         ("ms_moments", C.c_double), ("ms_kernel", C.c_double * len(KERNEL_CLASSES)),
         ("n_kernel", C.c_int * len(KERNEL_CLASSES)),
+        ("n_sur_pairs", C.c_longlong), ("n_sur_fallback", C.c_longlong),
     ]
 
     def as_dict(self):
@@ -77,11 +78,34 @@ class Stats(C.Structure):
         return d
 
 
+class SurCfg(C.Structure):
+    """efie_sur_cfg (include/efie_surrogate.h)."""
+    _fields_ = [
+        ("n_pairs", C.c_longlong), ("n_val", C.c_int), ("epochs", C.c_int), ("batch", C.c_int),
+        ("lr", C.c_double), ("kl_max", C.c_double), ("seed", C.c_ulonglong),
+    ]
+
+
+class SurReport(C.Structure):
+    """efie_sur_report (include/efie_surrogate.h)."""
+    _fields_ = [
+        ("n_train", C.c_longlong), ("n_val", C.c_longlong), ("steps", C.c_int), ("ms_sample", C.c_double),
+        ("ms_label", C.c_double), ("ms_train", C.c_double), ("train_loss", C.c_double), ("val_loss", C.c_double),
+        ("val_rel_rms", C.c_double), ("val_rel_max", C.c_double), ("base_rel_rms", C.c_double),
+        ("base_rel_max", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 EXPORTS = [
     "efie_opts_default", "efie_opts_preset", "efie_last_error", "efie_last_error_id",
     "efie_mesh_counts", "efie_make_mesh", "efie_rwg_count", "efie_build_rwg", "efie_pick_port",
     "efie_create", "efie_destroy", "efie_set_mesh", "efie_set_rwg", "efie_assemble_zv",
     "efie_sweep", "efie_zgesv", "efie_diag_copy",
+    "efie_sur_cfg_default", "efie_sur_blob_floats", "efie_sur_train", "efie_sur_get", "efie_sur_set",
+    "efie_sur_set_exact", "efie_mlp_params", "efie_mlp_forward", "efie_mlp_loss_grad",
 ]
 
 _lib = None
@@ -115,6 +139,17 @@ def lib():
                                  C.c_double, C.c_double, C.c_double, C.POINTER(Opts), _dp, _dp, C.POINTER(Stats)]
         L.efie_zgesv.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, C.POINTER(Stats)]
         L.efie_diag_copy.argtypes = [C.c_void_p, C.c_size_t, C.c_int, _dp]
+        _fp = C.POINTER(C.c_float)
+        L.efie_sur_cfg_default.argtypes = [C.POINTER(SurCfg)]
+        L.efie_sur_cfg_default.restype = None
+        L.efie_sur_blob_floats.restype = C.c_int
+        L.efie_mlp_params.restype = C.c_int
+        L.efie_sur_train.argtypes = [C.c_void_p, C.POINTER(Opts), C.POINTER(SurCfg), C.POINTER(SurReport)]
+        L.efie_sur_get.argtypes = [C.c_void_p, _fp]
+        L.efie_sur_set.argtypes = [C.c_void_p, _fp]
+        L.efie_sur_set_exact.argtypes = [C.c_void_p, C.c_int]
+        L.efie_mlp_forward.argtypes = [C.c_void_p, _fp, _fp, C.c_int, _fp]
+        L.efie_mlp_loss_grad.argtypes = [C.c_void_p, _fp, _fp, _fp, C.c_int, _fp, _fp]
         _lib = L
     return _lib
 
@@ -292,6 +327,55 @@ class Context:
         x = x[0] if single else x
         return (x, st.as_dict()) if stats else x
 
+
+    # ---- near-pair surrogate (include/efie_surrogate.h) ----
+
+    def sur_train(self, o: Optional[Opts] = None, **cfg):
+        """Train the near-pair surrogate for the options o and install it;
+        returns the training report as a dict."""
+        c = SurCfg()
+        lib().efie_sur_cfg_default(C.byref(c))
+        for k, v in cfg.items():
+            if not hasattr(c, k):
+                raise AttributeError(k)
+            setattr(c, k, v)
+        rep = SurReport()
+        _check(lib().efie_sur_train(self._c, C.byref(o or opts()), C.byref(c), C.byref(rep)))
+        return rep.as_dict()
+
+    def sur_get(self):
+        b = np.zeros(lib().efie_sur_blob_floats(), dtype=np.float32)
+        _check(lib().efie_sur_get(self._c, b.ctypes.data_as(C.POINTER(C.c_float))))
+        return b
+
+    def sur_set(self, blob):
+        b = np.ascontiguousarray(blob, dtype=np.float32)
+        if b.shape != (lib().efie_sur_blob_floats(),):
+            raise ValueError("surrogate blob has the wrong size")
+        _check(lib().efie_sur_set(self._c, b.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def sur_set_exact(self, on=True):
+        _check(lib().efie_sur_set_exact(self._c, 1 if on else 0))
+
+    def mlp_forward(self, params, X):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.zeros((X.shape[0], 16), dtype=np.float32)
+        fp = C.POINTER(C.c_float)
+        _check(lib().efie_mlp_forward(self._c, p.ctypes.data_as(fp), X.ctypes.data_as(fp), X.shape[0],
+                                      Y.ctypes.data_as(fp)))
+        return Y
+
+    def mlp_loss_grad(self, params, X, T):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        g = np.zeros(lib().efie_mlp_params(), dtype=np.float32)
+        loss = C.c_float()
+        fp = C.POINTER(C.c_float)
+        _check(lib().efie_mlp_loss_grad(self._c, p.ctypes.data_as(fp), X.ctypes.data_as(fp), T.ctypes.data_as(fp),
+                                        X.shape[0], C.byref(loss), g.ctypes.data_as(fp)))
+        return loss.value, g
 
     def diag_copy(self, nbytes, reps=10):
         """Average time (ms) of a plain device copy of nbytes (efie_diag.h)."""

@@ -21,7 +21,7 @@ def _declared(header):
 
 def test_library_exports_every_declared_symbol(efie_mod):
     L = ctypes.CDLL(efie_mod.LIB_PATH)
-    names = _declared("efie.h") + _declared("efie_diag.h")
+    names = _declared("efie.h") + _declared("efie_diag.h") + _declared("efie_surrogate.h")
     assert len(names) >= 17
     for n in names:
         assert hasattr(L, n), n
