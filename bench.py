@@ -174,6 +174,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--lu-kernel", type=int, default=0, help="0 auto, 1 FP32 MFMA, 2 bf16x6 (mixed path)")
+    ap.add_argument("--ir-tol", type=float, default=1e-10)
     ap.add_argument("--nf", type=int, default=0, help="use only the first NF frequencies (profiling runs)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run the plain copy kernel on this many MiB (counter calibration)")
@@ -195,7 +197,8 @@ def main():
     log(f"config {a.config}: Ne={n} Nt={F.shape[0]} Nf={len(fr)}")
     o64 = efie.opts(**W.OPTS_C1, precision=efie.PREC_FP64, near_mode=efie.NEAR_DIRECT)
     # refinement to ||V - Zx|| <= 1e-10 ||V||
-    ofast = efie.opts(**W.OPTS_C1, precision=efie.PREC_MIXED, near_mode=efie.NEAR_MOMENTS, ir_tol=1e-10)
+    ofast = efie.opts(**W.OPTS_C1, precision=efie.PREC_MIXED, near_mode=efie.NEAR_MOMENTS, ir_tol=a.ir_tol,
+                      lu_kernel=a.lu_kernel)
     with efie.Context(0) as ctx:
         ctx.set_problem(V, F, r)
         dt_fast, (zin_f, s11_f, st_f) = timed(ctx, ofast, fr, pe, ge, spread, a.steps, a.warmup, "fast")
@@ -212,11 +215,13 @@ def main():
             rep = ctx.sur_train(efie.opts(**W.OPTS_C1))
             t_train = time.perf_counter() - t0
             log(f"surrogate trained in {t_train:.2f} s")
-            osur = efie.opts(**W.OPTS_C1, precision=efie.PREC_MIXED, near_mode=efie.NEAR_SURROGATE, ir_tol=1e-10)
+            osur = efie.opts(**W.OPTS_C1, precision=efie.PREC_MIXED, near_mode=efie.NEAR_SURROGATE, ir_tol=a.ir_tol,
+                             lu_kernel=a.lu_kernel)
             dt_s, (zin_s, _, st_s) = timed(ctx, osur, fr, pe, ge, spread, a.steps, a.warmup, "surrogate")
             _, _, st_sp = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0,
                                     efie.opts(**W.OPTS_C1, precision=efie.PREC_MIXED,
-                                              near_mode=efie.NEAR_SURROGATE, ir_tol=1e-10, profile=1), stats=True)
+                                              near_mode=efie.NEAR_SURROGATE, ir_tol=a.ir_tol,
+                                              lu_kernel=a.lu_kernel, profile=1), stats=True)
             sur = {"value": len(fr) * a.steps / dt_s, "ms_per_step": 1e3 * dt_s / a.steps,
                    "train_s": t_train, "train": rep, "n_sur_pairs": st_s["n_sur_pairs"],
                    "n_sur_fallback": st_s["n_sur_fallback"],
@@ -227,8 +232,8 @@ def main():
                 sur["vs_fp64"] = sur["value"] / res["fp64"]["value"]
             res["surrogate"] = sur
         # profiled pass of the headline path for the roofline
-        oprof = efie.opts(**W.OPTS_C1, precision=efie.PREC_MIXED, near_mode=efie.NEAR_MOMENTS, ir_tol=1e-10,
-                      profile=1)
+        oprof = efie.opts(**W.OPTS_C1, precision=efie.PREC_MIXED, near_mode=efie.NEAR_MOMENTS, ir_tol=a.ir_tol,
+                      lu_kernel=a.lu_kernel, profile=1)
         _, _, st_p = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0, oprof, stats=True)
         log("profiled pass done")
     Nt = F.shape[0]

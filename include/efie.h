@@ -62,7 +62,14 @@ typedef struct {
     int ir_max_iter;    /* mixed: refinement iteration cap                    */
     int near_mode;      /* EFIE_NEAR_*: how near and self pairs are integrated */
     int profile;        /* 1: time every kernel class into efie_stats          */
+    int lu_kernel;      /* EFIE_LU_*: matrix-core form of the FP32 LU update   */
 } efie_opts;
+
+/* lu_kernel (mixed precision only): FP32 runs the LU's trailing update on
+ * the FP32 matrix cores; BF16X6 splits every FP32 operand into three bf16
+ * parts and accumulates six part products in FP32 on the bf16 matrix cores
+ * (half the matrix-core time of FP32; experimental); AUTO picks FP32. */
+enum { EFIE_LU_AUTO = 0, EFIE_LU_FP32 = 1, EFIE_LU_BF16X6 = 2 };
 
 /* near_mode: DIRECT evaluates the subdivided near/self quadrature at every
  * frequency as the reference does; MOMENTS expands the Green's function in

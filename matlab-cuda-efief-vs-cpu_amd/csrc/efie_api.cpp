@@ -295,6 +295,7 @@ void efie_opts_default(efie_opts* o) {
     o->ir_max_iter = 10;
     o->near_mode = EFIE_NEAR_DIRECT;
     o->profile = 0;
+    o->lu_kernel = EFIE_LU_AUTO;
 }
 
 efie_status efie_opts_preset(efie_opts* o, const char* name) {
@@ -637,6 +638,8 @@ void tri_vertices(const efie_ctx* c, int t, V3& A, V3& B, V3& C) {
 efie_status check_opts(const efie_opts* o) {
     if (o->precision != EFIE_PREC_FP64 && o->precision != EFIE_PREC_MIXED)
         return fail(EFIE_E_ARGS, "efie:opts", "unknown precision");
+    if (o->lu_kernel < EFIE_LU_AUTO || o->lu_kernel > EFIE_LU_BF16X6)
+        return fail(EFIE_E_ARGS, "efie:opts", "unknown lu_kernel");
     if (o->model == EFIE_MODEL_CENTROID) return ok();
     if (o->model != EFIE_MODEL_QUADRATURE) return fail(EFIE_E_ARGS, "efie:opts", "unknown model");
     if (o->near_subdiv < 0 || o->near_subdiv > 3 || o->self_subdiv < 0 || o->self_subdiv > 3)
@@ -1280,7 +1283,8 @@ efie_status factor_batch(efie_ctx* c, const efie_opts& o, int n, int lda, int nb
     // look-ahead on the panel stream, except when per-kernel profiling
     // needs every kernel on one stream
     LuWork wk{c->dCand.as<int>(), c->dCnt.as<int>(), c->dIpiv.as<int>(), bb.info,
-              c->dSwp.as<int>(), c->prof ? nullptr : c->panelStream, c->evPanel, c->evAhead};
+              c->dSwp.as<int>(), c->prof ? nullptr : c->panelStream, c->evPanel, c->evAhead,
+              o.lu_kernel == EFIE_LU_BF16X6};
     HIPCHK(launch_fill_int(bb.info, 0, nb, s));
     if (o.precision != EFIE_PREC_MIXED) {
         HIPCHK(lu_factor<double>(bb.Z, 2 * zElems, n, lda, nb, wk, s, &pf));

@@ -126,6 +126,7 @@ struct LuWork {
     // main stream itself); the two events order them
     hipStream_t panel;
     hipEvent_t evPanel, evAhead;
+    bool splitBf16;  // FP32 factorisation: update on the bf16 matrix cores (k_gemm_x6)
 };
 // ints per batch element of LuWork::swaps
 int lu_swap_stride();

@@ -818,6 +818,188 @@ __global__ __launch_bounds__(256) void k_gemm(T* A, size_t strideA, int lda, int
     }
 }
 
+// ------------------------------------------- split-bf16 trailing update --
+// FP32 C -= A B on the bf16 matrix cores.  Every FP32 operand value is split
+// exactly into three bf16 parts by truncation, x = hi + mid + lo: hi keeps
+// the top 8 significand bits of x, mid the top 8 of the remainder x - hi,
+// and lo = x - hi - mid has at most 8 significant bits left, so all three
+// are exact bf16 values.  Of the nine part products the six down to 2^-16
+// relative (hh, hm, mh, hl, mm, lh) are accumulated in FP32 by
+// v_mfma_f32_16x16x32_bf16; the three dropped ones are below 2^-24
+// relative, the size of FP32 rounding, so the update is as accurate as the
+// FP32 MFMA one while the six bf16 products take half the matrix-core time
+// of one FP32 product.  The complex product is formed in 4M form with four
+// accumulators per tile (RR = ar br, II = ai bi, RI = ar bi, IR = ai br).
+//
+// Tile 64 x 64 per 256-thread block, four waves of 32 x 32 (2 x 2 MFMA
+// tiles); K in chunks of 32 complex values.  Each thread loads 8
+// consecutive k of one A row and of one B column per chunk, splits them and
+// writes 16 bytes to each of the six part planes (re/im x hi/mid/lo) of the
+// chunk, stored [row or column][k] with 80-byte rows, double-buffered.
+// Operand layout of the MFMA: A[i][k] in lane i + 16 (k / 8), vector slot
+// k % 8; B[k][j] in lane j + 16 (k / 8); the result as for the f32 16x16
+// instructions.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+constexpr int X6_KC = 32, X6_LDK = 40, X6_PL = 64 * X6_LDK;
+
+__device__ __forceinline__ uint32_t pack_hi16(uint32_t a, uint32_t b) { return (a >> 16) | (b & 0xffff0000u); }
+
+// Split 8 FP32 values into hi / mid / lo bf16 vectors (as 4 dwords each).
+__device__ __forceinline__ void split8(const float* x, uint4& h, uint4& m, uint4& l) {
+    uint32_t hh[8], mm[8], ll[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        const uint32_t hb = __float_as_uint(x[s]) & 0xffff0000u;
+        const float r1 = x[s] - __uint_as_float(hb);
+        const uint32_t mb = __float_as_uint(r1) & 0xffff0000u;
+        hh[s] = hb;
+        mm[s] = mb;
+        ll[s] = __float_as_uint(r1 - __uint_as_float(mb));
+    }
+    h = make_uint4(pack_hi16(hh[0], hh[1]), pack_hi16(hh[2], hh[3]), pack_hi16(hh[4], hh[5]), pack_hi16(hh[6], hh[7]));
+    m = make_uint4(pack_hi16(mm[0], mm[1]), pack_hi16(mm[2], mm[3]), pack_hi16(mm[4], mm[5]), pack_hi16(mm[6], mm[7]));
+    l = make_uint4(pack_hi16(ll[0], ll[1]), pack_hi16(ll[2], ll[3]), pack_hi16(ll[4], ll[5]), pack_hi16(ll[6], ll[7]));
+}
+
+__device__ __forceinline__ mf<float>::v4 mfma_bf(const uint4& a, const uint4& b, mf<float>::v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
+                                                   0, 0);
+}
+
+__global__ __launch_bounds__(256) void k_gemm_x6(float* A, size_t strideA, int lda, int r0, int r1, int c0, int c1,
+                                                 int k0, int K, int gx, int gy, int nblk) {
+    typedef mf<float>::v4 v4;
+    __shared__ uint4 As[2][6][X6_PL / 8];
+    __shared__ uint4 Bs[2][6][X6_PL / 8];
+
+    // XCD-aware tile order (as k_gemm)
+    int id = blockIdx.x;
+    {
+        const int q = nblk >> 3, rr = nblk & 7, x = id & 7, loc = id >> 3;
+        id = x < rr ? x * (q + 1) + loc : rr * (q + 1) + (x - rr) * q + loc;
+    }
+    const int bx = id % gx;
+    id /= gx;
+    const int by = id % gy;
+    const int b = id / gy;
+    float* Ab = A + (size_t)b * strideA;
+    const int rb = r0 + by * 64, cb = c0 + bx * 64;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
+    const int li = lane & 15, lg = lane >> 4;
+    // loader role: A row / B column t & 63, k group t >> 6
+    const int lr = tid & 63, lk = (tid >> 6) * 8;
+
+    v4 acc[2][2][4];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) acc[x][y][c] = v4{0, 0, 0, 0};
+
+    // unconditional clamped loads; out-of-range values are zeroed at the split
+    float4 ra[4];
+    float2 rbv[8];
+    const int arow = rb + lr < r1 ? rb + lr : r1 - 1;
+    const int bcol = cb + lr < c1 ? cb + lr : c1 - 1;
+    auto gload = [&](int kc) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int kk = kc + lk + 2 * u;
+            ra[u] = *reinterpret_cast<const float4*>(Ab + 2 * ((size_t)arow * lda + k0 + (kk < K ? kk : 0)));
+        }
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const int kk = kc + lk + s;
+            rbv[s] = *reinterpret_cast<const float2*>(Ab + 2 * ((size_t)(k0 + (kk < K ? kk : K - 1)) * lda + bcol));
+        }
+    };
+    auto sstore = [&](int buf, int kc) {
+        const bool rowOk = rb + lr < r1, colOk = cb + lr < c1;
+        float ar[8], ai[8], br[8], bi[8];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const bool ok0 = rowOk && kc + lk + 2 * u < K, ok1 = rowOk && kc + lk + 2 * u + 1 < K;
+            ar[2 * u] = ok0 ? ra[u].x : 0.0f;
+            ai[2 * u] = ok0 ? ra[u].y : 0.0f;
+            ar[2 * u + 1] = ok1 ? ra[u].z : 0.0f;
+            ai[2 * u + 1] = ok1 ? ra[u].w : 0.0f;
+        }
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const bool ok = colOk && kc + lk + s < K;
+            br[s] = ok ? rbv[s].x : 0.0f;
+            bi[s] = ok ? rbv[s].y : 0.0f;
+        }
+        const int o = (lr * X6_LDK + lk) / 8;
+        uint4 h, m, l;
+        split8(ar, h, m, l);
+        As[buf][0][o] = h; As[buf][1][o] = m; As[buf][2][o] = l;
+        split8(ai, h, m, l);
+        As[buf][3][o] = h; As[buf][4][o] = m; As[buf][5][o] = l;
+        split8(br, h, m, l);
+        Bs[buf][0][o] = h; Bs[buf][1][o] = m; Bs[buf][2][o] = l;
+        split8(bi, h, m, l);
+        Bs[buf][3][o] = h; Bs[buf][4][o] = m; Bs[buf][5][o] = l;
+    };
+
+    // the six part products: (A part, B part)
+    constexpr int PA[6] = {0, 0, 1, 0, 1, 2}, PB[6] = {0, 1, 0, 2, 1, 0};
+    const int nch = (K + X6_KC - 1) / X6_KC;
+    gload(0);
+    sstore(0, 0);
+    __syncthreads();
+    for (int ch = 0; ch < nch; ++ch) {
+        const int buf = ch & 1;
+        if (ch + 1 < nch) gload((ch + 1) * X6_KC);
+        uint4 af[2][6], bf[2][6];
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int p = 0; p < 6; ++p) af[x][p] = As[buf][p][((wr + 16 * x + li) * X6_LDK + 8 * lg) / 8];
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int p = 0; p < 6; ++p) bf[y][p] = Bs[buf][p][((wc + 16 * y + li) * X6_LDK + 8 * lg) / 8];
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y) {
+                    acc[x][y][0] = mfma_bf(af[x][PA[q]], bf[y][PB[q]], acc[x][y][0]);          // ar br
+                    acc[x][y][1] = mfma_bf(af[x][3 + PA[q]], bf[y][3 + PB[q]], acc[x][y][1]);  // ai bi
+                    acc[x][y][2] = mfma_bf(af[x][PA[q]], bf[y][3 + PB[q]], acc[x][y][2]);      // ar bi
+                    acc[x][y][3] = mfma_bf(af[x][3 + PA[q]], bf[y][PB[q]], acc[x][y][3]);      // ai br
+                }
+        if (ch + 1 < nch) sstore(buf ^ 1, (ch + 1) * X6_KC);
+        __syncthreads();
+    }
+
+    // C -= product, one row of tiles at a time (loads first, clamped)
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+        cpx<float> cv[2][4];
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = rb + wr + 16 * x + 4 * lg + v, col = cb + wc + 16 * y + li;
+                cv[y][v] = ld(Ab + 2 * ((size_t)(row < r1 ? row : r1 - 1) * lda + (col < c1 ? col : c1 - 1)));
+            }
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = rb + wr + 16 * x + 4 * lg + v, col = cb + wc + 16 * y + li;
+                const float pr = acc[x][y][0][v] - acc[x][y][1][v], pi = acc[x][y][2][v] + acc[x][y][3][v];
+                if (row < r1 && col < c1)
+                    st(Ab + 2 * ((size_t)row * lda + col), cpx<float>{cv[y][v].r - pr, cv[y][v].i - pi});
+            }
+    }
+}
+
 // ------------------------------------------------- triangular solves -----
 // Blocked triangular solve on a vector x (element e of batch b at
 // X[b*strideX + 2*e*xs]), upper (U x = y, non-unit diagonal, bottom block
@@ -1074,8 +1256,17 @@ static void gemm_launch(T* A, size_t strideA, int lda, int batch, int r0, int r1
 // Returns the number of kernels launched (0 for an empty update).
 template <typename T>
 static int gemm(T* A, size_t strideA, int lda, int batch, int r0, int r1, int c0, int c1, int k0, int K,
-                hipStream_t s) {
+                hipStream_t s, bool splitBf16) {
     if (r1 <= r0 || c1 <= c0 || K <= 0) return 0;
+    if constexpr (std::is_same<T, float>::value) {
+        if (splitBf16) {
+            const int gx = (c1 - c0 + 63) / 64, gy = (r1 - r0 + 63) / 64;
+            const int nblk = gx * gy * batch;
+            hipLaunchKernelGGL(k_gemm_x6, dim3(nblk), dim3(256), 0, s, A, strideA, lda, r0, r1, c0, c1, k0, K, gx,
+                               gy, nblk);
+            return 1;
+        }
+    }
     gemm_launch<T, 64, 64>(A, strideA, lda, batch, r0, r1, c0, c1, k0, K, s);
     return 1;
 }
@@ -1117,7 +1308,7 @@ static void panel_phase(T* A, size_t strideA, int n, int lda, int batch, const L
         np += panel_cols<T>(A, strideA, lda, batch, c0, w, c0 + w, jend, sp);
         pf->end(7, np);
         pf->begin(8);
-        pf->end(8, gemm<T>(A, strideA, lda, batch, c0 + w, n, c0 + w, jend, c0, w, sp));
+        pf->end(8, gemm<T>(A, strideA, lda, batch, c0 + w, n, c0 + w, jend, c0, w, sp, wk.splitBf16));
     }
 }
 
@@ -1168,13 +1359,13 @@ hipError_t lu_factor(T* A, size_t strideA, int n, int lda, int batch, LuWork wk,
             pf->begin(7);
             pf->end(7, panel_cols<T>(A, strideA, lda, batch, c0, w, jend, n + 1, s));
             pf->begin(8);
-            pf->end(8, gemm<T>(A, strideA, lda, batch, c0 + w, jend, jend, n + 1, c0, w, s));
+            pf->end(8, gemm<T>(A, strideA, lda, batch, c0 + w, jend, jend, n + 1, c0, w, s, wk.splitBf16));
         }
         if (jend >= n) break;
         // trailing update: the next block's columns, then (while its panels
         // are factored on the panel stream) the remaining columns
         pf->begin(8);
-        pf->end(8, gemm<T>(A, strideA, lda, batch, jend, n, jend, jnext, j0, wb, s));
+        pf->end(8, gemm<T>(A, strideA, lda, batch, jend, n, jend, jnext, j0, wb, s, wk.splitBf16));
         if (sp != s) {
             hipEventRecord(wk.evAhead, s);
             hipStreamWaitEvent(sp, wk.evAhead, 0);
@@ -1182,7 +1373,7 @@ hipError_t lu_factor(T* A, size_t strideA, int n, int lda, int batch, LuWork wk,
         panel_phase<T>(A, strideA, n, lda, batch, wk, jend, jnext, nodeStride, sp, pf);
         if (sp != s) hipEventRecord(wk.evPanel, sp);
         pf->begin(8);
-        pf->end(8, gemm<T>(A, strideA, lda, batch, jend, n, jnext, n + 1, j0, wb, s));
+        pf->end(8, gemm<T>(A, strideA, lda, batch, jend, n, jnext, n + 1, j0, wb, s, wk.splitBf16));
     }
     return hipGetLastError();
 }

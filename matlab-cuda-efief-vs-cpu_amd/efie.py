@@ -31,6 +31,7 @@ _ip = C.POINTER(C.c_int)
 EFIE_OK = 0
 MODEL_QUADRATURE, MODEL_CENTROID = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
+LU_AUTO, LU_FP32, LU_BF16X6 = 0, 1, 2
 NEAR_DIRECT, NEAR_MOMENTS, NEAR_EXTRACT, NEAR_SURROGATE = 0, 1, 2, 3
 
 # physics constants of the reference drivers (main_patch_efie_rwg.m:218-220)
@@ -54,6 +55,7 @@ class Opts(C.Structure):
         ("self_alpha", C.c_double), ("near_subdiv", C.c_int), ("near_alpha", C.c_double),
         ("near_order", C.c_int), ("model", C.c_int), ("precision", C.c_int), ("batch", C.c_int),
         ("ir_tol", C.c_double), ("ir_max_iter", C.c_int), ("near_mode", C.c_int), ("profile", C.c_int),
+        ("lu_kernel", C.c_int),
     ]
 
 

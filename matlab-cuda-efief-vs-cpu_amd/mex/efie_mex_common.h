@@ -75,6 +75,7 @@ inline void read_opts(const mxArray* s, efie_opts* o) {
     o->ir_tol = field_or(s, "ir_tol", o->ir_tol);
     o->ir_max_iter = (int)field_or(s, "ir_max_iter", o->ir_max_iter);
     o->near_mode = (int)field_or(s, "near_mode", o->near_mode);
+    o->lu_kernel = (int)field_or(s, "lu_kernel", o->lu_kernel);
 }
 
 // Arguments 1-12 of both MEX functions: V, F, edge, plusTri, minusTri,

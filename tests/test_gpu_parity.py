@@ -207,6 +207,24 @@ def test_mex_entry_points_on_device(efie_mod, orc):
     assert np.abs(Vr[:, 0] - g["vrhs"]).max() <= 1e-15
 
 
+@pytest.mark.parametrize("lu_kernel", [1, 2])
+def test_c1_sweep_lu_kernels(efie_mod, orc, ctx, lu_kernel):
+    """The mixed path with either matrix-core form of the FP32 LU update
+    (FP32 MFMA, or the exact three-way bf16 split with six part products)
+    refines to the golden Zin, in as many refinement steps."""
+    g = np.load(W_GOLDEN("c1.npz"))
+    V, F, r, pe, ge, spread, _ = _setup(efie_mod, orc, "c1")
+    ctx.set_problem(V, F, r)
+    zin, s11, st = ctx.sweep(g["freqs"], W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0,
+                             efie_mod.opts(**W.OPTS_C1, precision=1, lu_kernel=lu_kernel), stats=True)
+    assert np.max(np.abs(zin - g["zin"]) / np.abs(g["zin"])) <= ZIN_TOL
+    assert np.max(np.abs(s11 - g["s11"])) <= ZIN_TOL
+    assert st["ir_resid_max"] <= 1e-11 and st["ir_iters_max"] <= 8
+    ref = ctx.sweep(g["freqs"], W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0,
+                    efie_mod.opts(**W.OPTS_C1, precision=1, lu_kernel=1), stats=True)[2]
+    assert st["ir_iters_max"] <= ref["ir_iters_max"] + 1
+
+
 # ------------------------------------------- singularity extraction --
 
 @pytest.mark.parametrize("ok", [0, 1, 2])
