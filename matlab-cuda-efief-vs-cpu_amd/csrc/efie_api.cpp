@@ -259,6 +259,7 @@ struct efie_ctx {
     DBuf dRecs, dFreq, dZ, dZ32, dWp, dWg, dSums, dV, dWl, dOut;
     DBuf dCand, dCnt, dIpiv, dInfo, dSwp, dPerm, dX, dR, dR32, dRn2, dBn2, dDone, dIters, dResid, dB;
     DBuf dFlags, dTickets, dErr;
+    DBuf dSplit;  // bf16 planes of the split-bf16 LU update (lu_kernel BF16X6)
     TrsvWork tw{nullptr, nullptr, nullptr, 0};
 
     // profiling (opts.profile): event pairs per kernel class
@@ -1215,6 +1216,7 @@ efie_status ensure_solver(efie_ctx* c, const efie_opts& o, int n, int lda, int B
         HIPCHK(c->dDone.ensure(sizeof(int) * B));
         HIPCHK(c->dIters.ensure(sizeof(int) * B));
         HIPCHK(c->dResid.ensure(sizeof(double) * B));
+        if (o.lu_kernel == EFIE_LU_BF16X6) HIPCHK(c->dSplit.ensure(lu_split_bytes(n) * B));
     }
     const int nodes = lu_tree_nodes(n);
     HIPCHK(c->dCand.ensure(sizeof(int) * (size_t)B * nodes * lu_cand_stride()));
@@ -1284,7 +1286,7 @@ efie_status factor_batch(efie_ctx* c, const efie_opts& o, int n, int lda, int nb
     // needs every kernel on one stream
     LuWork wk{c->dCand.as<int>(), c->dCnt.as<int>(), c->dIpiv.as<int>(), bb.info,
               c->dSwp.as<int>(), c->prof ? nullptr : c->panelStream, c->evPanel, c->evAhead,
-              o.lu_kernel == EFIE_LU_BF16X6};
+              (o.precision == EFIE_PREC_MIXED && o.lu_kernel == EFIE_LU_BF16X6) ? c->dSplit.p : nullptr};
     HIPCHK(launch_fill_int(bb.info, 0, nb, s));
     if (o.precision != EFIE_PREC_MIXED) {
         HIPCHK(lu_factor<double>(bb.Z, 2 * zElems, n, lda, nb, wk, s, &pf));

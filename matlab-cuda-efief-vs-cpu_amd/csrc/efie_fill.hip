@@ -270,7 +270,7 @@ __global__ __launch_bounds__(256) void k_rec_regular(const Pt* __restrict__ pts,
             const Pt pi = p1[i];
             const double xi = pi.x + ox, yi = pi.y + oy, zi = pi.z + oz;
             double gr = 0, gi = 0, hxr = 0, hxi = 0, hyr = 0, hyi = 0, hzr = 0, hzi = 0;
-#pragma unroll 1
+#pragma unroll
             for (int j = 0; j < NQ; ++j) {
                 const Pt q2 = p2[j];
                 const double dx = xi - q2.x, dy = yi - q2.y, dz = zi - q2.z;

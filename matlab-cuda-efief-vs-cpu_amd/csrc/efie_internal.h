@@ -126,8 +126,12 @@ struct LuWork {
     // main stream itself); the two events order them
     hipStream_t panel;
     hipEvent_t evPanel, evAhead;
-    bool splitBf16;  // FP32 factorisation: update on the bf16 matrix cores (k_gemm_x6)
+    // FP32 factorisation: non-null = the trailing updates run on the bf16
+    // matrix cores from a split of L21 and U12 (lu_split_bytes(n) per batch
+    // element)
+    void* splitPlanes;
 };
+size_t lu_split_bytes(int n);
 // ints per batch element of LuWork::swaps
 int lu_swap_stride();
 

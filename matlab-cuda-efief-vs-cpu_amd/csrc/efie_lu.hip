@@ -819,46 +819,76 @@ __global__ __launch_bounds__(256) void k_gemm(T* A, size_t strideA, int lda, int
 }
 
 // ------------------------------------------- split-bf16 trailing update --
-// FP32 C -= A B on the bf16 matrix cores.  Every FP32 operand value is split
-// exactly into three bf16 parts by truncation, x = hi + mid + lo: hi keeps
-// the top 8 significand bits of x, mid the top 8 of the remainder x - hi,
-// and lo = x - hi - mid has at most 8 significant bits left, so all three
-// are exact bf16 values.  Of the nine part products the six down to 2^-16
-// relative (hh, hm, mh, hl, mm, lh) are accumulated in FP32 by
-// v_mfma_f32_16x16x32_bf16; the three dropped ones are below 2^-24
-// relative, the size of FP32 rounding, so the update is as accurate as the
-// FP32 MFMA one while the six bf16 products take half the matrix-core time
-// of one FP32 product.  The complex product is formed in 4M form with four
-// accumulators per tile (RR = ar br, II = ai bi, RI = ar bi, IR = ai br).
-//
-// Tile 64 x 64 per 256-thread block, four waves of 32 x 32 (2 x 2 MFMA
-// tiles); K in chunks of 32 complex values.  Each thread loads 8
-// consecutive k of one A row and of one B column per chunk, splits them and
-// writes 16 bytes to each of the six part planes (re/im x hi/mid/lo) of the
-// chunk, stored [row or column][k] with 80-byte rows, double-buffered.
-// Operand layout of the MFMA: A[i][k] in lane i + 16 (k / 8), vector slot
-// k % 8; B[k][j] in lane j + 16 (k / 8); the result as for the f32 16x16
-// instructions.
+// FP32 C -= A B on the bf16 matrix cores.  Before the trailing update of an
+// outer block, k_split6 splits every FP32 value of L21 and U12 once into
+// three bf16 parts by round-to-nearest, x = hi + mid + lo + e with
+// |mid| <= 2^-8 |x|, |lo| <= 2^-16 |x| and |e| <= 2^-25 |x|, stored as six
+// planes (re/im x hi/mid/lo) of [outer index][k] with k contiguous: rows of
+// L21, columns of U12 (transposed), both zero-padded to whole tiles and
+// 32-wide k chunks.  k_gemm_p6 then accumulates, for each of the four real
+// products of the complex product (RR = ar br, II = ai bi, RI = ar bi,
+// IR = ai br), the six part products down to 2^-16 relative (hh, hm, mh,
+// hl, mm, lh) in FP32 with v_mfma_f32_16x16x32_bf16; the dropped ones are
+// below 2^-23 relative and of random sign, the size of FP32 rounding, so
+// the update is as accurate as the FP32 MFMA one at half its matrix-core
+// time.  MFMA operand layout (measured on gfx950): A[i][k] in lane
+// i + 16 (k / 8), vector slot k % 8; B[k][j] in lane j + 16 (k / 8); the
+// result as for the f32 16x16 instructions.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-constexpr int X6_KC = 32, X6_LDK = 40, X6_PL = 64 * X6_LDK;
+constexpr int P6_KC = 32;               // k per chunk (one MFMA k step)
+constexpr int P6_LDK = 40;              // bf16 per LDS row: 32 + 8 (80-byte rows spread the banks)
+constexpr int P6_PL = 64 * P6_LDK;      // bf16 per LDS plane
 
-__device__ __forceinline__ uint32_t pack_hi16(uint32_t a, uint32_t b) { return (a >> 16) | (b & 0xffff0000u); }
+// round-to-nearest-even FP32 -> bf16 bits (finite inputs)
+__device__ __forceinline__ uint32_t bf16_rn(float x) {
+    const uint32_t u = __float_as_uint(x);
+    return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+}
+__device__ __forceinline__ float bf16_f(uint32_t b) { return __uint_as_float(b << 16); }
 
-// Split 8 FP32 values into hi / mid / lo bf16 vectors (as 4 dwords each).
-__device__ __forceinline__ void split8(const float* x, uint4& h, uint4& m, uint4& l) {
-    uint32_t hh[8], mm[8], ll[8];
+// Plane layout of one batch element: 6 planes of [nOuter][ldp] bf16,
+// nOuter a multiple of 64, ldp (= padded K) a multiple of 32.
+// T = false: plane row o holds A[r0 + o][k0 .. k0 + K) (rows of L21);
+// T = true:  plane row o holds A[k0 .. k0 + K)[c0 + o]  (columns of U12).
+// One thread per 8 consecutive k of one plane row; values outside
+// o < nOut, k < K are zero.
+template <bool T>
+__global__ __launch_bounds__(256) void k_split6(const float* A, size_t strideA, int lda, int base, int nOut,
+                                                int k0, int K, uint16_t* planes, size_t batchStride, int nOuter,
+                                                int ldp) {
+    const int b = blockIdx.y;
+    const int gpr = ldp / 8;  // k groups per plane row
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nOuter * gpr) return;
+    const int o = e / gpr, kk = (e % gpr) * 8;
+    const float* Ab = A + (size_t)b * strideA;
+    uint32_t hr[8], mr[8], lr[8], hi[8], mi[8], li[8];
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
-        const uint32_t hb = __float_as_uint(x[s]) & 0xffff0000u;
-        const float r1 = x[s] - __uint_as_float(hb);
-        const uint32_t mb = __float_as_uint(r1) & 0xffff0000u;
-        hh[s] = hb;
-        mm[s] = mb;
-        ll[s] = __float_as_uint(r1 - __uint_as_float(mb));
+        const int k = kk + s;
+        cpx<float> v{0.0f, 0.0f};
+        if (o < nOut && k < K)
+            v = T ? ld(Ab + 2 * ((size_t)(k0 + k) * lda + base + o)) : ld(Ab + 2 * ((size_t)(base + o) * lda + k0 + k));
+        hr[s] = bf16_rn(v.r);
+        const float r1 = v.r - bf16_f(hr[s]);
+        mr[s] = bf16_rn(r1);
+        lr[s] = bf16_rn(r1 - bf16_f(mr[s]));
+        hi[s] = bf16_rn(v.i);
+        const float i1 = v.i - bf16_f(hi[s]);
+        mi[s] = bf16_rn(i1);
+        li[s] = bf16_rn(i1 - bf16_f(mi[s]));
     }
-    h = make_uint4(pack_hi16(hh[0], hh[1]), pack_hi16(hh[2], hh[3]), pack_hi16(hh[4], hh[5]), pack_hi16(hh[6], hh[7]));
-    m = make_uint4(pack_hi16(mm[0], mm[1]), pack_hi16(mm[2], mm[3]), pack_hi16(mm[4], mm[5]), pack_hi16(mm[6], mm[7]));
-    l = make_uint4(pack_hi16(ll[0], ll[1]), pack_hi16(ll[2], ll[3]), pack_hi16(ll[4], ll[5]), pack_hi16(ll[6], ll[7]));
+    auto pk = [](const uint32_t* q) {
+        return make_uint4(q[0] | (q[1] << 16), q[2] | (q[3] << 16), q[4] | (q[5] << 16), q[6] | (q[7] << 16));
+    };
+    const size_t pl = (size_t)nOuter * ldp;
+    uint16_t* P = planes + (size_t)b * batchStride + (size_t)o * ldp + kk;
+    *reinterpret_cast<uint4*>(P + 0 * pl) = pk(hr);
+    *reinterpret_cast<uint4*>(P + 1 * pl) = pk(mr);
+    *reinterpret_cast<uint4*>(P + 2 * pl) = pk(lr);
+    *reinterpret_cast<uint4*>(P + 3 * pl) = pk(hi);
+    *reinterpret_cast<uint4*>(P + 4 * pl) = pk(mi);
+    *reinterpret_cast<uint4*>(P + 5 * pl) = pk(li);
 }
 
 __device__ __forceinline__ mf<float>::v4 mfma_bf(const uint4& a, const uint4& b, mf<float>::v4 c) {
@@ -866,13 +896,18 @@ __device__ __forceinline__ mf<float>::v4 mfma_bf(const uint4& a, const uint4& b,
                                                    0, 0);
 }
 
-__global__ __launch_bounds__(256) void k_gemm_x6(float* A, size_t strideA, int lda, int r0, int r1, int c0, int c1,
-                                                 int k0, int K, int gx, int gy, int nblk) {
+// C[r0:r1, c0:c1) -= L21 U12 from the planes: PA rows r - r0 (nOuterA
+// rows per plane), PB rows c - cBase (nOuterB), ldp = padded K.  Tile
+// 64 x 64 per 256-thread block, four waves of 32 x 32 (2 x 2 MFMA tiles),
+// a chunk of 32 k staged in LDS (single buffer, the next chunk's 16-byte
+// plane runs prefetched into registers); XCD-aware block order as k_gemm.
+__global__ __launch_bounds__(256, 2) void k_gemm_p6(float* A, size_t strideA, int lda, int r0, int r1, int c0, int c1,
+                                                    const uint16_t* PA, const uint16_t* PB, size_t batchStride,
+                                                    int nOuterA, int nOuterB, int cBase, int ldp, int gx, int gy,
+                                                    int nblk) {
     typedef mf<float>::v4 v4;
-    __shared__ uint4 As[2][6][X6_PL / 8];
-    __shared__ uint4 Bs[2][6][X6_PL / 8];
-
-    // XCD-aware tile order (as k_gemm)
+    __shared__ uint4 As[6][P6_PL / 8];
+    __shared__ uint4 Bs[6][P6_PL / 8];
     int id = blockIdx.x;
     {
         const int q = nblk >> 3, rr = nblk & 7, x = id & 7, loc = id >> 3;
@@ -887,8 +922,11 @@ __global__ __launch_bounds__(256) void k_gemm_x6(float* A, size_t strideA, int l
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
     const int li = lane & 15, lg = lane >> 4;
-    // loader role: A row / B column t & 63, k group t >> 6
-    const int lr = tid & 63, lk = (tid >> 6) * 8;
+    // loader: plane row tid / 4 of the tile, k run 8 (tid % 4)
+    const int lrow = tid >> 2, lk = (tid & 3) * 8;
+    const size_t plA = (size_t)nOuterA * ldp, plB = (size_t)nOuterB * ldp;
+    const uint16_t* pa = PA + (size_t)b * batchStride + (size_t)(rb - r0 + lrow) * ldp + lk;
+    const uint16_t* pb = PB + (size_t)b * batchStride + (size_t)(cb - cBase + lrow) * ldp + lk;
 
     v4 acc[2][2][4];
 #pragma unroll
@@ -898,83 +936,57 @@ __global__ __launch_bounds__(256) void k_gemm_x6(float* A, size_t strideA, int l
 #pragma unroll
             for (int c = 0; c < 4; ++c) acc[x][y][c] = v4{0, 0, 0, 0};
 
-    // unconditional clamped loads; out-of-range values are zeroed at the split
-    float4 ra[4];
-    float2 rbv[8];
-    const int arow = rb + lr < r1 ? rb + lr : r1 - 1;
-    const int bcol = cb + lr < c1 ? cb + lr : c1 - 1;
-    auto gload = [&](int kc) {
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int kk = kc + lk + 2 * u;
-            ra[u] = *reinterpret_cast<const float4*>(Ab + 2 * ((size_t)arow * lda + k0 + (kk < K ? kk : 0)));
-        }
-#pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            const int kk = kc + lk + s;
-            rbv[s] = *reinterpret_cast<const float2*>(Ab + 2 * ((size_t)(k0 + (kk < K ? kk : K - 1)) * lda + bcol));
-        }
-    };
-    auto sstore = [&](int buf, int kc) {
-        const bool rowOk = rb + lr < r1, colOk = cb + lr < c1;
-        float ar[8], ai[8], br[8], bi[8];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const bool ok0 = rowOk && kc + lk + 2 * u < K, ok1 = rowOk && kc + lk + 2 * u + 1 < K;
-            ar[2 * u] = ok0 ? ra[u].x : 0.0f;
-            ai[2 * u] = ok0 ? ra[u].y : 0.0f;
-            ar[2 * u + 1] = ok1 ? ra[u].z : 0.0f;
-            ai[2 * u + 1] = ok1 ? ra[u].w : 0.0f;
-        }
-#pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            const bool ok = colOk && kc + lk + s < K;
-            br[s] = ok ? rbv[s].x : 0.0f;
-            bi[s] = ok ? rbv[s].y : 0.0f;
-        }
-        const int o = (lr * X6_LDK + lk) / 8;
-        uint4 h, m, l;
-        split8(ar, h, m, l);
-        As[buf][0][o] = h; As[buf][1][o] = m; As[buf][2][o] = l;
-        split8(ai, h, m, l);
-        As[buf][3][o] = h; As[buf][4][o] = m; As[buf][5][o] = l;
-        split8(br, h, m, l);
-        Bs[buf][0][o] = h; Bs[buf][1][o] = m; Bs[buf][2][o] = l;
-        split8(bi, h, m, l);
-        Bs[buf][3][o] = h; Bs[buf][4][o] = m; Bs[buf][5][o] = l;
-    };
-
-    // the six part products: (A part, B part)
-    constexpr int PA[6] = {0, 0, 1, 0, 1, 2}, PB[6] = {0, 1, 0, 2, 1, 0};
-    const int nch = (K + X6_KC - 1) / X6_KC;
-    gload(0);
-    sstore(0, 0);
-    __syncthreads();
+    constexpr int PAi[6] = {0, 0, 1, 0, 1, 2}, PBi[6] = {0, 1, 0, 2, 1, 0};
+    const int nch = ldp / P6_KC;
+    const int so = (lrow * P6_LDK + lk) / 8;
+    uint4 ra0 = *reinterpret_cast<const uint4*>(pa), ra1 = *reinterpret_cast<const uint4*>(pa + plA),
+          ra2 = *reinterpret_cast<const uint4*>(pa + 2 * plA), ra3 = *reinterpret_cast<const uint4*>(pa + 3 * plA),
+          ra4 = *reinterpret_cast<const uint4*>(pa + 4 * plA), ra5 = *reinterpret_cast<const uint4*>(pa + 5 * plA);
+    uint4 rb0 = *reinterpret_cast<const uint4*>(pb), rb1 = *reinterpret_cast<const uint4*>(pb + plB),
+          rb2 = *reinterpret_cast<const uint4*>(pb + 2 * plB), rb3 = *reinterpret_cast<const uint4*>(pb + 3 * plB),
+          rb4 = *reinterpret_cast<const uint4*>(pb + 4 * plB), rb5 = *reinterpret_cast<const uint4*>(pb + 5 * plB);
     for (int ch = 0; ch < nch; ++ch) {
-        const int buf = ch & 1;
-        if (ch + 1 < nch) gload((ch + 1) * X6_KC);
-        uint4 af[2][6], bf[2][6];
+        __syncthreads();  // the previous chunk's operands have been read
+        As[0][so] = ra0; As[1][so] = ra1; As[2][so] = ra2; As[3][so] = ra3; As[4][so] = ra4; As[5][so] = ra5;
+        Bs[0][so] = rb0; Bs[1][so] = rb1; Bs[2][so] = rb2; Bs[3][so] = rb3; Bs[4][so] = rb4; Bs[5][so] = rb5;
+        __syncthreads();
+        {
+            // the next chunk (the last one again at the end: loads stay
+            // unconditional and in bounds)
+            const int kc = (ch + 1 < nch ? ch + 1 : ch) * P6_KC;
+            ra0 = *reinterpret_cast<const uint4*>(pa + kc);
+            ra1 = *reinterpret_cast<const uint4*>(pa + plA + kc);
+            ra2 = *reinterpret_cast<const uint4*>(pa + 2 * plA + kc);
+            ra3 = *reinterpret_cast<const uint4*>(pa + 3 * plA + kc);
+            ra4 = *reinterpret_cast<const uint4*>(pa + 4 * plA + kc);
+            ra5 = *reinterpret_cast<const uint4*>(pa + 5 * plA + kc);
+            rb0 = *reinterpret_cast<const uint4*>(pb + kc);
+            rb1 = *reinterpret_cast<const uint4*>(pb + plB + kc);
+            rb2 = *reinterpret_cast<const uint4*>(pb + 2 * plB + kc);
+            rb3 = *reinterpret_cast<const uint4*>(pb + 3 * plB + kc);
+            rb4 = *reinterpret_cast<const uint4*>(pb + 4 * plB + kc);
+            rb5 = *reinterpret_cast<const uint4*>(pb + 5 * plB + kc);
+        }
+        uint4 af[2][6];
 #pragma unroll
         for (int x = 0; x < 2; ++x)
 #pragma unroll
-            for (int p = 0; p < 6; ++p) af[x][p] = As[buf][p][((wr + 16 * x + li) * X6_LDK + 8 * lg) / 8];
+            for (int p = 0; p < 6; ++p) af[x][p] = As[p][((wr + 16 * x + li) * P6_LDK + 8 * lg) / 8];
 #pragma unroll
-        for (int y = 0; y < 2; ++y)
+        for (int y = 0; y < 2; ++y) {
+            uint4 bf[6];
 #pragma unroll
-            for (int p = 0; p < 6; ++p) bf[y][p] = Bs[buf][p][((wc + 16 * y + li) * X6_LDK + 8 * lg) / 8];
+            for (int p = 0; p < 6; ++p) bf[p] = Bs[p][((wc + 16 * y + li) * P6_LDK + 8 * lg) / 8];
 #pragma unroll
-        for (int q = 0; q < 6; ++q)
+            for (int q = 0; q < 6; ++q)
 #pragma unroll
-            for (int x = 0; x < 2; ++x)
-#pragma unroll
-                for (int y = 0; y < 2; ++y) {
-                    acc[x][y][0] = mfma_bf(af[x][PA[q]], bf[y][PB[q]], acc[x][y][0]);          // ar br
-                    acc[x][y][1] = mfma_bf(af[x][3 + PA[q]], bf[y][3 + PB[q]], acc[x][y][1]);  // ai bi
-                    acc[x][y][2] = mfma_bf(af[x][PA[q]], bf[y][3 + PB[q]], acc[x][y][2]);      // ar bi
-                    acc[x][y][3] = mfma_bf(af[x][3 + PA[q]], bf[y][PB[q]], acc[x][y][3]);      // ai br
+                for (int x = 0; x < 2; ++x) {
+                    acc[x][y][0] = mfma_bf(af[x][PAi[q]], bf[PBi[q]], acc[x][y][0]);          // ar br
+                    acc[x][y][1] = mfma_bf(af[x][3 + PAi[q]], bf[3 + PBi[q]], acc[x][y][1]);  // ai bi
+                    acc[x][y][2] = mfma_bf(af[x][PAi[q]], bf[3 + PBi[q]], acc[x][y][2]);      // ar bi
+                    acc[x][y][3] = mfma_bf(af[x][3 + PAi[q]], bf[PBi[q]], acc[x][y][3]);      // ai br
                 }
-        if (ch + 1 < nch) sstore(buf ^ 1, (ch + 1) * X6_KC);
-        __syncthreads();
+        }
     }
 
     // C -= product, one row of tiles at a time (loads first, clamped)
@@ -1256,19 +1268,48 @@ static void gemm_launch(T* A, size_t strideA, int lda, int batch, int r0, int r1
 // Returns the number of kernels launched (0 for an empty update).
 template <typename T>
 static int gemm(T* A, size_t strideA, int lda, int batch, int r0, int r1, int c0, int c1, int k0, int K,
-                hipStream_t s, bool splitBf16) {
+                hipStream_t s) {
     if (r1 <= r0 || c1 <= c0 || K <= 0) return 0;
-    if constexpr (std::is_same<T, float>::value) {
-        if (splitBf16) {
-            const int gx = (c1 - c0 + 63) / 64, gy = (r1 - r0 + 63) / 64;
-            const int nblk = gx * gy * batch;
-            hipLaunchKernelGGL(k_gemm_x6, dim3(nblk), dim3(256), 0, s, A, strideA, lda, r0, r1, c0, c1, k0, K, gx,
-                               gy, nblk);
-            return 1;
-        }
-    }
     gemm_launch<T, 64, 64>(A, strideA, lda, batch, r0, r1, c0, c1, k0, K, s);
     return 1;
+}
+
+// Plane geometry of the split-bf16 update for an n x n factorisation:
+// per batch element the L21 planes (rows padded to 64) then the U12 planes
+// (n + 1 columns padded to 64), 6 planes each of ldp = WB bf16 per row.
+static int p6_outer(int n) { return (n + 1 + 63) / 64 * 64; }
+size_t lu_split_bytes(int n) { return (size_t)2 * 6 * p6_outer(n) * WB * sizeof(uint16_t); }
+
+// Trailing update A[r0:n, c0:c1) -= L21 U12 of the outer block j0..j0+wb
+// from the planes (split by split_block).
+static int gemm_p6(float* A, size_t strideA, int lda, int batch, int r0, int r1, int c0, int c1,
+                   const LuWork& wk, int n, int jend, hipStream_t s) {
+    if (r1 <= r0 || c1 <= c0) return 0;
+    const int no = p6_outer(n);
+    const size_t bs = lu_split_bytes(n) / sizeof(uint16_t);
+    const uint16_t* PA = static_cast<const uint16_t*>(wk.splitPlanes);
+    const uint16_t* PB = PA + (size_t)6 * no * WB;
+    const int gx = (c1 - c0 + 63) / 64, gy = (r1 - r0 + 63) / 64;
+    const int nblk = gx * gy * batch;
+    hipLaunchKernelGGL(k_gemm_p6, dim3(nblk), dim3(256), 0, s, A, strideA, lda, r0, r1, c0, c1, PA, PB, bs, no, no,
+                       jend, WB, gx, gy, nblk);
+    return 1;
+}
+
+// Planes of L21 = A[jend:n, j0:jend) and U12 = A[j0:jend, jend:n+1).
+static int split_block(float* A, size_t strideA, int lda, int batch, int n, int j0, int jend, const LuWork& wk,
+                       hipStream_t s) {
+    const int no = p6_outer(n), wb = jend - j0;
+    const size_t bs = lu_split_bytes(n) / sizeof(uint16_t);
+    uint16_t* PA = static_cast<uint16_t*>(wk.splitPlanes);
+    uint16_t* PB = PA + (size_t)6 * no * WB;
+    const int nA = (n - jend + 63) / 64 * 64, nB = (n + 1 - jend + 63) / 64 * 64;
+    const int tA = nA * (WB / 8), tB = nB * (WB / 8);
+    hipLaunchKernelGGL(k_split6<false>, dim3((tA + 255) / 256, batch), dim3(256), 0, s, A, strideA, lda, jend,
+                       n - jend, j0, wb, PA, bs, no, WB);
+    hipLaunchKernelGGL(k_split6<true>, dim3((tB + 255) / 256, batch), dim3(256), 0, s, A, strideA, lda, jend,
+                       n + 1 - jend, j0, wb, PB, bs, no, WB);
+    return 2;
 }
 
 template <typename T>
@@ -1308,7 +1349,7 @@ static void panel_phase(T* A, size_t strideA, int n, int lda, int batch, const L
         np += panel_cols<T>(A, strideA, lda, batch, c0, w, c0 + w, jend, sp);
         pf->end(7, np);
         pf->begin(8);
-        pf->end(8, gemm<T>(A, strideA, lda, batch, c0 + w, n, c0 + w, jend, c0, w, sp, wk.splitBf16));
+        pf->end(8, gemm<T>(A, strideA, lda, batch, c0 + w, n, c0 + w, jend, c0, w, sp));
     }
 }
 
@@ -1359,13 +1400,21 @@ hipError_t lu_factor(T* A, size_t strideA, int n, int lda, int batch, LuWork wk,
             pf->begin(7);
             pf->end(7, panel_cols<T>(A, strideA, lda, batch, c0, w, jend, n + 1, s));
             pf->begin(8);
-            pf->end(8, gemm<T>(A, strideA, lda, batch, c0 + w, jend, jend, n + 1, c0, w, s, wk.splitBf16));
+            pf->end(8, gemm<T>(A, strideA, lda, batch, c0 + w, jend, jend, n + 1, c0, w, s));
         }
         if (jend >= n) break;
         // trailing update: the next block's columns, then (while its panels
         // are factored on the panel stream) the remaining columns
+        bool p6 = false;
+        if constexpr (std::is_same<T, float>::value) p6 = wk.splitPlanes != nullptr;
         pf->begin(8);
-        pf->end(8, gemm<T>(A, strideA, lda, batch, jend, n, jend, jnext, j0, wb, s, wk.splitBf16));
+        if constexpr (std::is_same<T, float>::value) {
+            if (p6) {
+                const int ns = split_block(A, strideA, lda, batch, n, j0, jend, wk, s);
+                pf->end(8, ns + gemm_p6(A, strideA, lda, batch, jend, n, jend, jnext, wk, n, jend, s));
+            }
+        }
+        if (!p6) pf->end(8, gemm<T>(A, strideA, lda, batch, jend, n, jend, jnext, j0, wb, s));
         if (sp != s) {
             hipEventRecord(wk.evAhead, s);
             hipStreamWaitEvent(sp, wk.evAhead, 0);
@@ -1373,7 +1422,10 @@ hipError_t lu_factor(T* A, size_t strideA, int n, int lda, int batch, LuWork wk,
         panel_phase<T>(A, strideA, n, lda, batch, wk, jend, jnext, nodeStride, sp, pf);
         if (sp != s) hipEventRecord(wk.evPanel, sp);
         pf->begin(8);
-        pf->end(8, gemm<T>(A, strideA, lda, batch, jend, n, jnext, n + 1, j0, wb, s, wk.splitBf16));
+        if constexpr (std::is_same<T, float>::value) {
+            if (p6) pf->end(8, gemm_p6(A, strideA, lda, batch, jend, n, jnext, n + 1, wk, n, jend, s));
+        }
+        if (!p6) pf->end(8, gemm<T>(A, strideA, lda, batch, jend, n, jnext, n + 1, j0, wb, s));
     }
     return hipGetLastError();
 }
