@@ -165,6 +165,55 @@ def cpu_baseline(V, F, Ne, fr, threads=None):
                       f"one frequency point = {fill_s + t_lu:.0f} s"}
 
 
+def bench_c4(a):
+    """The batched uneven sweep: 32 seeded geometries (N 1000-8000), 64
+    frequencies each, mixed path, run as one sweep per geometry on one
+    context (tables, geometry caches and solver buffers are rebuilt or grown
+    per geometry inside the timed region)."""
+    geoms = W.c4_geometries()
+    fr = np.linspace(*W.C4["freqs"])
+    o = efie.opts(**W.OPTS_C1, precision=efie.PREC_MIXED, near_mode=efie.NEAR_MOMENTS, ir_tol=a.ir_tol,
+                  lu_kernel=a.lu_kernel)
+    o64 = efie.opts(**W.OPTS_C1, precision=efie.PREC_FP64, near_mode=efie.NEAR_DIRECT)
+    probs = []
+    for g in geoms:
+        V, F = efie.make_patch_with_ground_mesh(*g["geom"], *g["cells"])
+        r = efie.build_rwg_interior(V, F)
+        pP, pG = W.c4_feed_points(g["geom"])
+        probs.append((g, V, F, r, efie.pick_port_edge_by_point_and_layer(r, pP, 1),
+                      efie.pick_port_edge_by_point_and_layer(r, pG, -1), 1.5 * r.len.mean()))
+    per, errs = [], []
+    with efie.Context(0) as ctx:
+        g, V, F, r, pe, ge, sp = probs[0]
+        ctx.set_problem(V, F, r)
+        ctx.sweep(fr[:4], W.MU0, W.EPS0, pe, ge, 1.0, sp, 50.0, o)   # warm-up
+        t_all = time.perf_counter()
+        for i, (g, V, F, r, pe, ge, sp) in enumerate(probs):
+            t0 = time.perf_counter()
+            ctx.set_problem(V, F, r)
+            zin, _, st = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, sp, 50.0, o, stats=True)
+            dt = time.perf_counter() - t0
+            per.append({"N": int(r.Ne), "s": dt, "freq_per_s": len(fr) / dt, "ir_iters_max": st["ir_iters_max"]})
+            log(f"[c4] geometry {i}: N={r.Ne} {dt:.2f} s")
+        t_all = time.perf_counter() - t_all
+        if not a.no_fp64:
+            # accuracy against the FP64 path on every fourth geometry (untimed)
+            for g, V, F, r, pe, ge, sp in probs[::4]:
+                ctx.set_problem(V, F, r)
+                z64 = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, sp, 50.0, o64)[0]
+                zm = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, sp, 50.0, o)[0]
+                errs.append(float(np.max(np.abs(zm - z64) / np.abs(z64))))
+    npts = len(fr) * len(probs)
+    line = {"metric": "freq points/s (Z fill+LU), 32 uneven geometries, mixed path",
+            "value": npts / t_all, "unit": "freq points/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+            "ms_per_step": 1e3 * t_all, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "c128 fill, c64 LU + c128 refinement", "data": "synthetic",
+            "config": {"workload": f"c4: {len(probs)} geometries (seed {W.C4['seed']}), N {min(p['N'] for p in per)}-"
+                                   f"{max(p['N'] for p in per)}, {len(fr)} frequencies 1-4 GHz each"},
+            "max_rel_err_zin_vs_fp64": max(errs) if errs else None, "per_geometry": per}
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -185,6 +234,12 @@ def main():
             nbytes = a.copy_probe_mb << 20
             ms = ctx.diag_copy(nbytes, 5)
         print(json.dumps({"copy_bytes": nbytes, "ms": ms, "GB/s": 2 * nbytes / (ms * 1e-3) / 1e9}))
+        return
+    if a.config == "c4":
+        if a.gpus != 1:
+            print("bench.py measures one GPU", file=sys.stderr)
+            sys.exit(2)
+        bench_c4(a)
         return
     if a.gpus != 1:
         print(f"bench.py measures one GPU (asked for {a.gpus}): the sweep would shard by frequency "

@@ -37,6 +37,37 @@ OPTS_C1 = dict(quad_order=7, near_sing_eps=2.5, self_subdiv=2, self_alpha=0.20,
                near_subdiv=2, near_alpha=0.15, near_order=4)
 
 
+# Batched uneven sweep: geometries drawn from a seed (patch W, L uniform in
+# 20-50 mm, h in 0.8-3.2 mm, ground 1.5-2.0x the patch, cell counts chosen so
+# that N is uniform in 1000-8000 RWG), 64 frequencies in 1-4 GHz each.
+C4 = dict(seed=20251203, n_geom=32, n_range=(1000, 8000), freqs=(1.0e9, 4.0e9, 64))
+
+
+def c4_geometries(n_geom=None, n_range=None, seed=None):
+    """[{name, geom (Lp, Wp, Lg, Wg, h), cells (np, np, ng, ng)}] of the
+    batched uneven sweep.  Cell counts are square per layer (the mesh
+    generator's stride, see the module docstring); with ng ~ f np the RWG
+    count is N = 3 np^2 - 2 np + 3 ng^2 - 2 ng ~ 3 np^2 (1 + f^2)."""
+    n_geom = C4["n_geom"] if n_geom is None else n_geom
+    lo, hi = C4["n_range"] if n_range is None else n_range
+    rng = np.random.Generator(np.random.PCG64(C4["seed"] if seed is None else seed))
+    out = []
+    for g in range(n_geom):
+        Wp, Lp = rng.uniform(20e-3, 50e-3, 2)
+        h = rng.uniform(0.8e-3, 3.2e-3)
+        f = rng.uniform(1.5, 2.0)
+        target = rng.uniform(lo, hi)
+        npc = max(2, int(round(np.sqrt(target / (3.0 * (1.0 + f * f))))))
+        ngc = max(npc + 1, int(round(f * npc)))
+        out.append(dict(name=f"c4_{g}", geom=(Lp, Wp, f * Lp, f * Wp, h), cells=(npc, npc, ngc, ngc)))
+    return out
+
+
+def c4_feed_points(geom):
+    Lp, Wp, Lg, Wg, h = geom
+    return np.array([-Wp / 2, 0.0, +h / 2]), np.array([-Wp / 2, 0.0, -h / 2])
+
+
 def freqs(name):
     a, b, n = CONFIGS[name]["freqs"]
     return np.linspace(a, b, n)

@@ -225,6 +225,29 @@ def test_c1_sweep_lu_kernels(efie_mod, orc, ctx, lu_kernel):
     assert st["ir_iters_max"] <= ref["ir_iters_max"] + 1
 
 
+def test_c4_family_small(efie_mod, orc, ctx):
+    """Geometries of the batched uneven workload's family (rectangular
+    cells, random layer gap and ground size), scaled down, against the
+    oracle's sweep: FP64 and mixed paths, one context re-used across
+    geometries of different N."""
+    fr = np.linspace(1.0e9, 4.0e9, 3)
+    for g in W.c4_geometries(n_geom=2, n_range=(120, 260), seed=11):
+        V, F = efie_mod.make_patch_with_ground_mesh(*g["geom"], *g["cells"])
+        r = efie_mod.build_rwg_interior(V, F)
+        pP, pG = W.c4_feed_points(g["geom"])
+        pe = efie_mod.pick_port_edge_by_point_and_layer(r, pP, 1)
+        ge = efie_mod.pick_port_edge_by_point_and_layer(r, pG, -1)
+        spread = 1.5 * r.len.mean()
+        P = orc.Problem(V, F, orc.build_rwg(V, F))
+        zo, so = P.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0, orc.opts(**W.OPTS_C1))
+        ctx.set_problem(V, F, r)
+        for prec in (0, 1):
+            zin, s11 = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0,
+                                 efie_mod.opts(**W.OPTS_C1, precision=prec, near_mode=1))
+            assert np.max(np.abs(zin - zo) / np.abs(zo)) <= ZIN_TOL
+            assert np.max(np.abs(s11 - so)) <= ZIN_TOL
+
+
 # ------------------------------------------- singularity extraction --
 
 @pytest.mark.parametrize("ok", [0, 1, 2])
