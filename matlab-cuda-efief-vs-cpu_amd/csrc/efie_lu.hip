@@ -640,8 +640,6 @@ struct gemm_vec<double> {
     typedef double2 v;
 };
 
-constexpr int kGroupRows = 8;
-
 template <typename T, int BM, int BN, bool M3 = false>
 __global__ __launch_bounds__(256) void k_gemm(T* A, size_t strideA, int lda, int r0, int r1, int c0,
                                               int c1, int k0, int K, int gx, int gy, int nblk) {
@@ -665,17 +663,10 @@ __global__ __launch_bounds__(256) void k_gemm(T* A, size_t strideA, int lda, int
         const int q = nblk >> 3, rr = nblk & 7, x = id & 7, loc = id >> 3;
         id = x < rr ? x * (q + 1) + loc : rr * (q + 1) + (x - rr) * q + loc;
     }
-    // within a matrix, tiles go in groups of kGroupRows tile rows, column
-    // by column: the group's A row panels stay in L2 while each B column
-    // panel is read once per group
-    const int per = gx * gy;
-    const int b = id / per;
-    int t = id - b * per;
-    const int g0 = (t / (kGroupRows * gx)) * kGroupRows;
-    const int gm = gy - g0 < kGroupRows ? gy - g0 : kGroupRows;
-    t -= g0 * gx;
-    const int by = g0 + t % gm;
-    const int bx = t / gm;
+    const int bx = id % gx;
+    id /= gx;
+    const int by = id % gy;
+    const int b = id / gy;
     T* Ab = A + (size_t)b * strideA;
     const int rb = r0 + by * BM;
     const int cb = c0 + bx * BN;
