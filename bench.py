@@ -8,8 +8,9 @@ resident in HBM.  Two paths are timed:
   fp64  the plain method: per-frequency quadrature of every triangle pair
         (as the reference does) and an FP64 LU;
   fast  the headline path: near and self pairs from frequency-independent
-        moments, complex FP32 LU on the matrix cores and FP64 iterative
-        refinement;
+        moments, the complex-symmetric Z factored in FP32 on the matrix cores
+        without pivoting (frequencies where that does not refine are redone
+        with the pivoted FP32 LU) and FP64 iterative refinement;
   surrogate  as fast, with the near pairs from the learned surrogate (trained
         on the GPU first, outside the timed sweeps).
 `value` is the fast path's frequency points per second; the JSON line also
@@ -88,6 +89,36 @@ def lu_gemm_flops(n):
     return f
 
 
+def lu_gemm_flops_sym(n, tile=64):
+    """Flops of the matrix-core updates of the unpivoted complex-symmetric
+    factorisation of size n (no rhs column): the updates inside each outer
+    block as in the LU, and the trailing update on the 64 x 64 tiles the
+    kernel computes (tiles in strictly upper WB x WB blocks are skipped)."""
+    f = 0.0
+
+    def g(r0, r1, c0, c1, K):
+        return 8.0 * max(0, r1 - r0) * max(0, c1 - c0) * K if K > 0 else 0.0
+
+    def g_lower(r0, r1, c0, c1, K):
+        t = 0.0
+        for rb in range(r0, r1, tile):
+            for cb in range(c0, c1, tile):
+                if rb // WB >= cb // WB:
+                    t += g(rb, min(r1, rb + tile), cb, min(c1, cb + tile), K)
+        return t
+    for j0 in range(0, n, WB):
+        jend = min(n, j0 + WB)
+        for c0 in range(j0, jend, NB):
+            w = min(NB, jend - c0)
+            f += g(c0 + w, n, c0 + w, jend, w)           # update inside the block
+        if jend >= n:
+            break
+        jnext = n if n - jend < WB else jend + WB
+        f += g_lower(jend, n, jend, jnext, jend - j0)    # the next block's columns
+        f += g_lower(jend, n, jnext, n, jend - j0)       # the rest, lower blocks only
+    return f
+
+
 def traffic(kernel):
     """HBM bytes per launch of `kernel` on this workload, measured with
     rocprofv3 FETCH_SIZE / WRITE_SIZE and scaled by the same counters on a
@@ -100,19 +131,29 @@ def traffic(kernel):
         return None
 
 
-def roofline(st, n, nf, fast):
+def roofline(st, n, nf, fast, sym=False):
     """Dominant kernel class of a profiled sweep and its achieved rate."""
     ks = st["kernels"]
     name = max(ks, key=lambda k: ks[k]["ms"])
     ms, cnt = ks[name]["ms"], ks[name]["n"]
     if name == "gemm":
-        work = lu_gemm_flops(n) * nf
+        # the mixed sweep factors every frequency without pivoting and those
+        # it had to redo (n_lu_retry) once more with pivoting
+        retry = st.get("n_lu_retry", 0)
+        if fast and sym:
+            per = lu_gemm_flops_sym(n)
+            work = per * nf + lu_gemm_flops(n) * retry
+            desc = (f"8*M*N*K flop per computed update tile ({per:.4e} per unpivoted symmetric factorisation, "
+                    f"{lu_gemm_flops(n):.4e} per pivoted LU; {nf} + {retry} redone with pivoting)")
+        else:
+            work = lu_gemm_flops(n) * nf
+            desc = f"8*M*N*K flop per update ({lu_gemm_flops(n):.4e} per LU)"
         prec = "fp32" if fast else "fp64"
         ach = work / (ms * 1e-3) / 1e12
         return {"kernel": f"k_gemm<{'float' if fast else 'double'}>", "bound": "mfma", "achieved": ach,
                 "peak": PEAK[prec], "unit": "TFLOP/s", "frac": ach / PEAK[prec], "traffic": traffic("k_gemm"),
-                "work": f"8*M*N*K flop per update ({lu_gemm_flops(n):.4e} per LU), {work/cnt:.3e} flop/launch "
-                        f"avg over {cnt} launches, {ms/cnt*1e3:.1f} us/launch avg"}
+                "work": f"{desc}, {work/cnt:.3e} flop/launch avg over {cnt} launches, "
+                        f"{ms/cnt*1e3:.1f} us/launch avg"}
     if name in ("regular", "near", "self"):
         evals = st["n_green_" + name]
         work = evals * FLOP_PER_GREEN * nf
@@ -305,7 +346,7 @@ def main():
                                f"{W.CONFIGS[a.config]['cells'][1]} + ground {W.CONFIGS[a.config]['cells'][2]}x"
                                f"{W.CONFIGS[a.config]['cells'][3]} cells, Ne={n}, Nt={Nt}, Nf={len(fr)} "
                                f"({fr[0]/1e9:.1f}-{fr[-1]/1e9:.1f} GHz), 7-pt regular rule, 6-pt x16 near/self"},
-        "roofline": roofline(st_p, n, len(fr), True),
+        "roofline": roofline(st_p, n, len(fr), True, sym=a.lu_kernel in (0, 3)),
         "fast": {"stats": st_f},
         "fast_vs_fp64": (value / res["fp64"]["value"]) if "fp64" in res else None,
         **res,

@@ -65,11 +65,19 @@ typedef struct {
     int lu_kernel;      /* EFIE_LU_*: matrix-core form of the FP32 LU update   */
 } efie_opts;
 
-/* lu_kernel (mixed precision only): FP32 runs the LU's trailing update on
- * the FP32 matrix cores; BF16X6 splits every FP32 operand into three bf16
- * parts and accumulates six part products in FP32 on the bf16 matrix cores
- * (half the matrix-core time of FP32; experimental); AUTO picks FP32. */
-enum { EFIE_LU_AUTO = 0, EFIE_LU_FP32 = 1, EFIE_LU_BF16X6 = 2 };
+/* lu_kernel (mixed precision only): FP32 runs the pivoted LU's trailing
+ * update on the FP32 matrix cores; BF16X6 splits every FP32 operand into
+ * three bf16 parts and accumulates six part products in FP32 on the bf16
+ * matrix cores (experimental); SYM uses that Z is complex symmetric: the
+ * FP32 factorisation runs without pivoting as A = L D L^T, so the trailing
+ * update covers only the lower block triangle and the pivot search and row
+ * interchanges disappear; the FP64 refinement corrects its larger rounding
+ * growth, and a frequency where it does not reach max(ir_tol, 1e-11)
+ * (near-zero pivots close to a resonance) is solved again with the pivoted
+ * FP32 factorisation (efie_stats.n_lu_retry).  Results have the accuracy of
+ * the pivoted path.  AUTO picks SYM in efie_sweep (Z is complex symmetric)
+ * and FP32 in efie_zgesv (general matrices). */
+enum { EFIE_LU_AUTO = 0, EFIE_LU_FP32 = 1, EFIE_LU_BF16X6 = 2, EFIE_LU_SYM = 3 };
 
 /* near_mode: DIRECT evaluates the subdivided near/self quadrature at every
  * frequency as the reference does; MOMENTS expands the Green's function in
@@ -123,6 +131,9 @@ typedef struct {
      * integrated by the quadrature because they left its envelope        */
     long long n_sur_pairs;
     long long n_sur_fallback;
+    /* lu_kernel SYM: frequencies solved again with pivoting because the
+     * unpivoted factorisation met a zero pivot or did not refine to ir_tol */
+    int n_lu_retry;
 } efie_stats;
 
 /* "accurate" preset (main_patch_efie_rwg.m:281-289), FP64, quadrature. */

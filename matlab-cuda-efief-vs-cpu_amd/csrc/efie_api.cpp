@@ -639,7 +639,7 @@ void tri_vertices(const efie_ctx* c, int t, V3& A, V3& B, V3& C) {
 efie_status check_opts(const efie_opts* o) {
     if (o->precision != EFIE_PREC_FP64 && o->precision != EFIE_PREC_MIXED)
         return fail(EFIE_E_ARGS, "efie:opts", "unknown precision");
-    if (o->lu_kernel < EFIE_LU_AUTO || o->lu_kernel > EFIE_LU_BF16X6)
+    if (o->lu_kernel < EFIE_LU_AUTO || o->lu_kernel > EFIE_LU_SYM)
         return fail(EFIE_E_ARGS, "efie:opts", "unknown lu_kernel");
     if (o->model == EFIE_MODEL_CENTROID) return ok();
     if (o->model != EFIE_MODEL_QUADRATURE) return fail(EFIE_E_ARGS, "efie:opts", "unknown model");
@@ -1286,7 +1286,8 @@ efie_status factor_batch(efie_ctx* c, const efie_opts& o, int n, int lda, int nb
     // needs every kernel on one stream
     LuWork wk{c->dCand.as<int>(), c->dCnt.as<int>(), c->dIpiv.as<int>(), bb.info,
               c->dSwp.as<int>(), c->prof ? nullptr : c->panelStream, c->evPanel, c->evAhead,
-              (o.precision == EFIE_PREC_MIXED && o.lu_kernel == EFIE_LU_BF16X6) ? c->dSplit.p : nullptr};
+              (o.precision == EFIE_PREC_MIXED && o.lu_kernel == EFIE_LU_BF16X6) ? c->dSplit.p : nullptr,
+              o.precision == EFIE_PREC_MIXED && o.lu_kernel == EFIE_LU_SYM};
     HIPCHK(launch_fill_int(bb.info, 0, nb, s));
     if (o.precision != EFIE_PREC_MIXED) {
         HIPCHK(lu_factor<double>(bb.Z, 2 * zElems, n, lda, nb, wk, s, &pf));
@@ -1322,11 +1323,20 @@ efie_status refine_batch(efie_ctx* c, const efie_opts& o, int n, int lda, int nb
         return ok();
     }
     float* Z32 = bb.Z32;
-    pf.begin(EFIE_K_TRSV);
-    HIPCHK(lu_backsolve<float>(Z32, 2 * zElems, n, lda, nb, si, &c->tw));
-    pf.end(EFIE_K_TRSV);
+    if (o.lu_kernel == EFIE_LU_SYM) {
+        // no rhs column: the first refinement step from x = 0 is the solve
+        pf.begin(EFIE_K_OTHER);
+        HIPCHK(launch_fill_dbl(c->dX.as<double>(), 0.0, 2 * n * nb, si));
+        pf.end(EFIE_K_OTHER);
+    } else {
+        pf.begin(EFIE_K_TRSV);
+        HIPCHK(lu_backsolve<float>(Z32, 2 * zElems, n, lda, nb, si, &c->tw));
+        pf.end(EFIE_K_TRSV);
+        pf.begin(EFIE_K_OTHER);
+        HIPCHK(launch_col_to_vec(Z32, 2 * zElems, n, lda, c->dX.as<double>(), nb, si));
+        pf.end(EFIE_K_OTHER);
+    }
     pf.begin(EFIE_K_OTHER);
-    HIPCHK(launch_col_to_vec(Z32, 2 * zElems, n, lda, c->dX.as<double>(), nb, si));
     HIPCHK(launch_fill_int(c->dDone.as<int>(), 0, nb, si));
     HIPCHK(launch_fill_int(c->dIters.as<int>(), 0, nb, si));
     HIPCHK(launch_fill_dbl(c->dRn2.as<double>(), 0.0, nb, si));
@@ -1336,7 +1346,8 @@ efie_status refine_batch(efie_ctx* c, const efie_opts& o, int n, int lda, int nb
         HIPCHK(launch_residual(bb.Z, 2 * zElems, n, lda, V, vStride, c->dX.as<double>(), c->dR.as<double>(),
                                c->dRn2.as<double>(), c->dDone.as<int>(), nb, si));
         HIPCHK(launch_ir_check(c->dRn2.as<double>(), c->dBn2.as<double>(), o.ir_tol, nb, c->dDone.as<int>(),
-                               c->dIters.as<int>(), c->dResid.as<double>(), it == 0, it == o.ir_max_iter, si));
+                               c->dIters.as<int>(), c->dResid.as<double>(), it == 0, it == o.ir_max_iter,
+                               o.lu_kernel == EFIE_LU_SYM ? o.ir_max_iter - it : 0, si));
         pf.end(EFIE_K_OTHER);
         if (it == o.ir_max_iter) break;
         pf.begin(EFIE_K_OTHER);
@@ -1379,8 +1390,12 @@ efie_status stage_stats(efie_ctx* c, const efie_opts& o, int nb, int slot, int B
 // After a batch's slot event has completed: zero pivots become
 // efie:singular, refinement counters go into the stats, profile marks are
 // summed.
+// retry (lu_kernel SYM): frequencies whose unpivoted factorisation hit a
+// zero pivot or did not refine to max(ir_tol, 1e-11) (1e-11: above the
+// FP64 residual floor of these systems) are appended (as offsets into freqs)
+// instead of failing; they are solved again with pivoting.
 efie_status finish_batch(efie_ctx* c, const efie_opts& o, int nb, int slot, int B, efie_stats* S,
-                         const double* freqs) {
+                         const double* freqs, std::vector<int>* retry = nullptr) {
     const char* h = static_cast<const char*>(c->hStat) + stat_slot_bytes(B) * slot;
     const int* info = reinterpret_cast<const int*>(h);
     const int* iters = info + B;
@@ -1394,6 +1409,10 @@ efie_status finish_batch(efie_ctx* c, const efie_opts& o, int nb, int slot, int 
         return fail(EFIE_E_HIP, "efie:hip", "triangular solve: a block wait exceeded its limit");
     }
     for (int i = 0; i < nb; ++i) {
+        if (retry && mixed && (info[i] != 0 || !(resid[i] <= std::max(o.ir_tol, 1e-11)))) {
+            retry->push_back(i);
+            continue;
+        }
         if (info[i] != 0) {
             if (freqs)
                 return fail(EFIE_E_SINGULAR, "efie:singular", "Z is singular at f=%g Hz (zero pivot in column %d)",
@@ -1472,6 +1491,9 @@ efie_status efie_sweep(efie_ctx* c, const double* freqs, int Nf, double mu, doub
     if (!c->haveRwg) return fail(EFIE_E_STATE, "efie:state", "mesh and RWG tables must be set first");
     if (patchEdge < 1 || patchEdge > c->Ne) return fail(EFIE_E_PORT, "mex:port", "patchEdge out of range.");
     if (groundEdge < 1 || groundEdge > c->Ne) return fail(EFIE_E_PORT, "mex:port", "groundEdge out of range.");
+    // Z is complex symmetric: the mixed path factors it without pivoting
+    // unless asked otherwise (frequencies that need pivoting are redone)
+    if (o.lu_kernel == EFIE_LU_AUTO && o.precision == EFIE_PREC_MIXED) o.lu_kernel = EFIE_LU_SYM;
     // the centroid sweep MEX hard-codes self_alpha = 0.20 (patch_sweep_cuda.cu:269)
     efie_status st = prepare(c, &o, 0.20);
     if (st) return st;
@@ -1567,6 +1589,8 @@ efie_status efie_sweep(efie_ctx* c, const double* freqs, int Nf, double mu, doub
         HIPCHK(hipStreamWaitEvent(sf, c->ev[0], 0));
         HIPCHK(hipStreamWaitEvent(si, c->ev[0], 0));
     }
+    const bool symRetry = mixed && o.lu_kernel == EFIE_LU_SYM;
+    std::vector<int> retry;
     auto finish = [&](int bj) -> efie_status {
         const int f0 = bj * B, nb = std::min(B, Nf - f0), buf = bj % nbuf;
         HIPCHK(hipEventSynchronize(c->evIR[buf]));
@@ -1577,7 +1601,11 @@ efie_status efie_sweep(efie_ctx* c, const double* freqs, int Nf, double mu, doub
         msFill += t1;
         msFac += t2;
         msSolve += t3;
-        return finish_batch(c, o, nb, buf, B, S, freqs + f0);
+        if (!symRetry) return finish_batch(c, o, nb, buf, B, S, freqs + f0);
+        std::vector<int> rb;
+        efie_status e = finish_batch(c, o, nb, buf, B, S, freqs + f0, &rb);
+        for (int i : rb) retry.push_back(f0 + i);
+        return e;
     };
     st = enqueue_fill(0);
     if (st) return st;
@@ -1647,6 +1675,36 @@ efie_status efie_sweep(efie_ctx* c, const double* freqs, int Nf, double mu, doub
     S->ms_fill = msFill;
     S->ms_factor = msFac;
     S->ms_solve = msSolve;
+    if (!retry.empty()) {
+        // the frequencies the unpivoted factorisation could not solve, again
+        // with the pivoted one (a sweep of their own)
+        const int k = (int)retry.size();
+        std::vector<double> fr2(k), zin2(2 * (size_t)k), s112(2 * (size_t)k);
+        for (int i = 0; i < k; ++i) fr2[i] = freqs[retry[i]];
+        efie_opts o2 = o;
+        o2.lu_kernel = EFIE_LU_FP32;
+        efie_stats S2;
+        st = efie_sweep(c, fr2.data(), k, mu, eps_re, eps_im, patchEdge, groundEdge, V0, spread, Z0, &o2,
+                        zin2.data(), s112.data(), &S2);
+        if (st) return st;
+        for (int i = 0; i < k; ++i) {
+            Zin[2 * retry[i]] = zin2[2 * i];
+            Zin[2 * retry[i] + 1] = zin2[2 * i + 1];
+            S11[2 * retry[i]] = s112[2 * i];
+            S11[2 * retry[i] + 1] = s112[2 * i + 1];
+        }
+        S->ms_total += S2.ms_total;
+        S->ms_fill += S2.ms_fill;
+        S->ms_factor += S2.ms_factor;
+        S->ms_solve += S2.ms_solve;
+        S->ir_iters_max = std::max(S->ir_iters_max, S2.ir_iters_max);
+        S->ir_resid_max = std::max(S->ir_resid_max, S2.ir_resid_max);
+        for (int q = 0; q < EFIE_NKCLASS; ++q) {
+            S->ms_kernel[q] += S2.ms_kernel[q];
+            S->n_kernel[q] += S2.n_kernel[q];
+        }
+        S->n_lu_retry = k;
+    }
     S->n_near_pairs = c->nNear;
     if (o.model == EFIE_MODEL_QUADRATURE) {
         const long long Nt = c->Nt, np = (long long)c->NpNear, ns = (long long)c->NsSelf * c->Nq2s;
@@ -1669,6 +1727,7 @@ efie_status efie_zgesv(efie_ctx* c, int n, int batch, const double* A, const dou
     efie_opts o;
     efie_opts_default(&o);
     o.precision = precision;
+    o.lu_kernel = EFIE_LU_FP32;  // general matrices: pivoted
     efie_stats local;
     efie_stats* S = stats ? stats : &local;
     std::memset(S, 0, sizeof *S);

@@ -130,6 +130,8 @@ struct LuWork {
     // matrix cores from a split of L21 and U12 (lu_split_bytes(n) per batch
     // element)
     void* splitPlanes;
+    // complex-symmetric A: factor without pivoting, U = D L^T (lu_factor_sym)
+    bool noPivot;
 };
 size_t lu_split_bytes(int n);
 // ints per batch element of LuWork::swaps
@@ -176,8 +178,10 @@ hipError_t launch_col_to_vec(const float* A32, size_t strideA, int n, int lda, d
 hipError_t launch_residual(const double* A64, size_t strideA, int n, int lda, const double* V,
                            size_t vStride, const double* X, double* R, double* rn2, const int* done,
                            int batch, hipStream_t s);
+// remaining > 0 (a retry follows): also stop a system whose last reduction
+// factor, kept for the remaining steps, would not reach tol.
 hipError_t launch_ir_check(double* rn2, const double* vn2, double tol, int batch, int* done,
-                           int* iters, double* resid, int firstPass, int lastPass, hipStream_t s);
+                           int* iters, double* resid, int firstPass, int lastPass, int remaining, hipStream_t s);
 hipError_t launch_perm_f32(const double* R, const int* perm, int n, int batch, float* R32,
                            const int* done, hipStream_t s);
 hipError_t launch_add_f32(double* X, const float* D32, int n, int batch, const int* done,

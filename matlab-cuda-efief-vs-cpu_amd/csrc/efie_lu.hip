@@ -454,6 +454,42 @@ __global__ __launch_bounds__(256) void k_laswp_rest(T* A, size_t strideA, int ld
     for (int k = 0; k < nslots; ++k) swap_column(Ab, lda, c, L + k * lu_swap_stride_c);
 }
 
+// ------------------------------------------ symmetric factorisation ------
+// For a complex symmetric A (A = A^T) elimination without pivoting gives
+// U = D L^T with D the diagonal of U, so the block row U12 of an outer block
+// need not be eliminated: k_sym_u12 writes U[k][j] = U[k][k] L[j][k] for
+// rows k in [k0, k1) and columns j in [c0, c1), through a 64 x 32 LDS tile
+// (coalesced on both sides).  It also leaves the final U in place for the
+// triangular solves.
+template <typename T>
+__global__ __launch_bounds__(256) void k_sym_u12(T* A, size_t strideA, int lda, int k0, int k1, int c0, int c1) {
+    __shared__ cpx<T> t[64][33];
+    __shared__ cpx<T> dg[32];
+    const int b = blockIdx.z;
+    T* Ab = A + (size_t)b * strideA;
+    const int jb = c0 + blockIdx.x * 64, kb = k0 + blockIdx.y * 32;
+    const int tid = threadIdx.x;
+    // read L[j][k], j in [jb, jb+64), k in [kb, kb+32): 32 consecutive k per row
+    for (int e = tid; e < 64 * 32; e += 256) {
+        const int jj = e / 32, kk = e % 32, j = jb + jj, k = kb + kk;
+        cpx<T> v{(T)0, (T)0};
+        if (j < c1 && k < k1) v = ld(Ab + 2 * ((size_t)j * lda + k));
+        t[jj][kk] = v;
+    }
+    if (tid < 32) {
+        const int k = kb + tid;
+        cpx<T> v{(T)0, (T)0};
+        if (k < k1) v = ld(Ab + 2 * ((size_t)k * lda + k));
+        dg[tid] = v;
+    }
+    __syncthreads();
+    // write U[k][j] = d_k L[j][k]: 64 consecutive j per row
+    for (int e = tid; e < 32 * 64; e += 256) {
+        const int kk = e / 64, jj = e % 64, k = kb + kk, j = jb + jj;
+        if (k < k1 && j < c1) st(Ab + 2 * ((size_t)k * lda + j), cmul(dg[kk], t[jj][kk]));
+    }
+}
+
 // ----------------------------------------------------- panel finish ------
 // k_panel_diag: A11 (w x w, after the interchanges) = L11 U11 in place,
 // without pivoting (the tournament already ordered the pivot rows).
@@ -642,7 +678,7 @@ struct gemm_vec<double> {
 
 template <typename T, int BM, int BN, bool M3 = false>
 __global__ __launch_bounds__(256) void k_gemm(T* A, size_t strideA, int lda, int r0, int r1, int c0,
-                                              int c1, int k0, int K, int gx, int gy, int nblk) {
+                                              int c1, int k0, int K, int gx, int gy, int nblk, int skipUpper) {
     typedef typename mf<T>::v4 v4;
     typedef typename gemm_vec<T>::v vec;
     constexpr int CB = (int)sizeof(cpx<T>);   // bytes per complex value
@@ -670,6 +706,9 @@ __global__ __launch_bounds__(256) void k_gemm(T* A, size_t strideA, int lda, int
     T* Ab = A + (size_t)b * strideA;
     const int rb = r0 + by * BM;
     const int cb = c0 + bx * BN;
+    // symmetric factorisation: tiles in strictly upper WB x WB blocks are
+    // never read again
+    if (skipUpper && rb / WB < cb / WB) return;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wv = tid >> 6;
     const int wr = (wv >> 1) * (BM / 2), wc = (wv & 1) * (BN / 2);
@@ -1252,11 +1291,11 @@ int lu_swap_stride() { return kSwapSlots * lu_swap_stride_c; }
 
 template <typename T, int BM, int BN>
 static void gemm_launch(T* A, size_t strideA, int lda, int batch, int r0, int r1, int c0, int c1, int k0, int K,
-                        hipStream_t s) {
+                        hipStream_t s, int skipUpper = 0) {
     const int gx = (c1 - c0 + BN - 1) / BN, gy = (r1 - r0 + BM - 1) / BM;
     const int nblk = gx * gy * batch;
     hipLaunchKernelGGL((k_gemm<T, BM, BN, true>), dim3(nblk), dim3(256), 0, s, A, strideA, lda, r0, r1, c0, c1, k0,
-                       K, gx, gy, nblk);
+                       K, gx, gy, nblk, skipUpper);
 }
 
 // 64 x 64 tiles: four blocks per CU keep 16 waves resident, which hides
@@ -1268,9 +1307,9 @@ static void gemm_launch(T* A, size_t strideA, int lda, int batch, int r0, int r1
 // Returns the number of kernels launched (0 for an empty update).
 template <typename T>
 static int gemm(T* A, size_t strideA, int lda, int batch, int r0, int r1, int c0, int c1, int k0, int K,
-                hipStream_t s) {
+                hipStream_t s, int skipUpper = 0) {
     if (r1 <= r0 || c1 <= c0 || K <= 0) return 0;
-    gemm_launch<T, 64, 64>(A, strideA, lda, batch, r0, r1, c0, c1, k0, K, s);
+    gemm_launch<T, 64, 64>(A, strideA, lda, batch, r0, r1, c0, c1, k0, K, s, skipUpper);
     return 1;
 }
 
@@ -1329,15 +1368,17 @@ static void panel_phase(T* A, size_t strideA, int n, int lda, int batch, const L
     for (int c0 = j0; c0 < jend; c0 += NB) {
         const int w = jend - c0 < NB ? jend - c0 : NB;
         const int slot = (c0 - j0) / NB;
-        TreeGeom g = tree_geom(n - c0);
-        pf->begin(5);
-        hipLaunchKernelGGL(k_tslu<T>, dim3(g.levelCount[0], batch), dim3(64), 0, sp, A, strideA, n, lda, c0, w, g,
-                           nodeStride, wk.cand, wk.counters, wk.ipiv, wk.info, wk.swaps, slot);
-        pf->end(5);
-        pf->begin(6);
-        hipLaunchKernelGGL(k_laswp<T>, dim3((jend - j0 + 255) / 256, batch), dim3(256), 0, sp, A, strideA, lda,
-                           wk.swaps, slot, j0, jend);
-        pf->end(6);
+        if (!wk.noPivot) {
+            TreeGeom g = tree_geom(n - c0);
+            pf->begin(5);
+            hipLaunchKernelGGL(k_tslu<T>, dim3(g.levelCount[0], batch), dim3(64), 0, sp, A, strideA, n, lda, c0, w,
+                               g, nodeStride, wk.cand, wk.counters, wk.ipiv, wk.info, wk.swaps, slot);
+            pf->end(5);
+            pf->begin(6);
+            hipLaunchKernelGGL(k_laswp<T>, dim3((jend - j0 + 255) / 256, batch), dim3(256), 0, sp, A, strideA, lda,
+                               wk.swaps, slot, j0, jend);
+            pf->end(6);
+        }
         pf->begin(7);
         int np = 1;
         hipLaunchKernelGGL(k_panel_diag<T>, dim3(batch), dim3(64), 0, sp, A, strideA, lda, c0, w, wk.info);
@@ -1363,11 +1404,60 @@ static void panel_phase(T* A, size_t strideA, int n, int lda, int batch, const L
 // updates the remaining columns.  The two never touch the same columns:
 // the panels stay inside their block, and their interchanges reach the
 // other columns only after the update that reads those rows has finished.
+__global__ void k_ident_ipiv(int* ipiv, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) ipiv[(size_t)blockIdx.y * n + i] = i + 1;
+}
+
+// Complex-symmetric factorisation without pivoting: A = L U with U = D L^T,
+// L stored below the diagonal and U on and above it.  Outer blocks and
+// look-ahead as in lu_factor; the panels of a block are panel_phase without
+// the tournament and interchanges; the block row U12 comes from L21 by
+// k_sym_u12 instead of elimination; the trailing update runs only on the
+// lower WB-block triangle, since the strictly upper blocks are never read
+// again.  The rhs column is not carried (the refinement starts from x = 0);
+// the pivots are the identity.
+template <typename T>
+static hipError_t lu_factor_sym(T* A, size_t strideA, int n, int lda, int batch, const LuWork& wk, hipStream_t s,
+                                const Prof* pf) {
+    hipStream_t sp = wk.panel ? wk.panel : s;
+    hipLaunchKernelGGL(k_ident_ipiv, dim3((n + 255) / 256, batch), dim3(256), 0, s, wk.ipiv, n);
+    if (sp != s) {
+        hipEventRecord(wk.evAhead, s);
+        hipStreamWaitEvent(sp, wk.evAhead, 0);
+    }
+    panel_phase<T>(A, strideA, n, lda, batch, wk, 0, n < WB ? n : WB, 0, sp, pf);
+    if (sp != s) hipEventRecord(wk.evPanel, sp);
+    for (int j0 = 0; j0 < n; j0 += WB) {
+        const int wb = n - j0 < WB ? n - j0 : WB;
+        const int jend = j0 + wb;
+        const int jnext = n - jend < WB ? n : jend + WB;
+        if (sp != s) hipStreamWaitEvent(s, wk.evPanel, 0);
+        if (jend >= n) break;
+        pf->begin(7);
+        hipLaunchKernelGGL(k_sym_u12<T>, dim3((n - jend + 63) / 64, (wb + 31) / 32, batch), dim3(256), 0, s, A,
+                           strideA, lda, j0, jend, jend, n);
+        pf->end(7);
+        pf->begin(8);
+        pf->end(8, gemm<T>(A, strideA, lda, batch, jend, n, jend, jnext, j0, wb, s, 1));
+        if (sp != s) {
+            hipEventRecord(wk.evAhead, s);
+            hipStreamWaitEvent(sp, wk.evAhead, 0);
+        }
+        panel_phase<T>(A, strideA, n, lda, batch, wk, jend, jnext, 0, sp, pf);
+        if (sp != s) hipEventRecord(wk.evPanel, sp);
+        pf->begin(8);
+        pf->end(8, gemm<T>(A, strideA, lda, batch, jend, n, jnext, n, j0, wb, s, 1));
+    }
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t lu_factor(T* A, size_t strideA, int n, int lda, int batch, LuWork wk, hipStream_t s,
                      const Prof* pf) {
     Prof none;
     if (!pf) pf = &none;
+    if (wk.noPivot) return lu_factor_sym<T>(A, strideA, n, lda, batch, wk, s, pf);
     hipStream_t sp = wk.panel ? wk.panel : s;
     // candidate and counter slots per batch element are sized for the first
     // (largest) panel; every counter is back at zero when a launch ends

@@ -63,14 +63,18 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ A64
 // keeps the last residual ratio and counts corrections; clears rn2.
 __global__ void k_ir_check(double* __restrict__ rn2, const double* __restrict__ vn2, double tol, int batch,
                            int* __restrict__ done, int* __restrict__ iters, double* __restrict__ resid,
-                           int firstPass, int lastPass) {
+                           int firstPass, int lastPass, int remaining) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= batch) return;
     if (!done[b]) {
         const double ratio = sqrt(rn2[b] / vn2[b]);
         const bool stalled = !firstPass && ratio > 0.5 * resid[b];
+        // remaining > 0: also stop when this step's reduction, kept up for
+        // the remaining steps, would not reach tol (the caller retries)
+        const bool hopeless =
+            !firstPass && remaining > 0 && ratio * pow(ratio / resid[b], (double)remaining) > tol;
         resid[b] = firstPass ? ratio : fmin(ratio, resid[b]);
-        if (ratio <= tol || stalled) done[b] = 1;
+        if (ratio <= tol || stalled || hopeless) done[b] = 1;
         else if (!lastPass) iters[b] += 1;
     }
     rn2[b] = 0.0;
@@ -145,9 +149,9 @@ hipError_t launch_residual(const double* A64, size_t strideA, int n, int lda, co
 }
 
 hipError_t launch_ir_check(double* rn2, const double* vn2, double tol, int batch, int* done, int* iters,
-                           double* resid, int firstPass, int lastPass, hipStream_t s) {
+                           double* resid, int firstPass, int lastPass, int remaining, hipStream_t s) {
     hipLaunchKernelGGL(k_ir_check, dim3((batch + 63) / 64), dim3(64), 0, s, rn2, vn2, tol, batch, done, iters,
-                       resid, firstPass, lastPass);
+                       resid, firstPass, lastPass, remaining);
     return hipGetLastError();
 }
 

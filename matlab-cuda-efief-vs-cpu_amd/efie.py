@@ -31,7 +31,7 @@ _ip = C.POINTER(C.c_int)
 EFIE_OK = 0
 MODEL_QUADRATURE, MODEL_CENTROID = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
-LU_AUTO, LU_FP32, LU_BF16X6 = 0, 1, 2
+LU_AUTO, LU_FP32, LU_BF16X6, LU_SYM = 0, 1, 2, 3
 NEAR_DIRECT, NEAR_MOMENTS, NEAR_EXTRACT, NEAR_SURROGATE = 0, 1, 2, 3
 
 # physics constants of the reference drivers (main_patch_efie_rwg.m:218-220)
@@ -70,7 +70,7 @@ class Stats(C.Structure):
         ("ir_iters_max", C.c_int), ("ir_resid_max", C.c_double), ("moment_terms", C.c_int),
         ("ms_moments", C.c_double), ("ms_kernel", C.c_double * len(KERNEL_CLASSES)),
         ("n_kernel", C.c_int * len(KERNEL_CLASSES)),
-        ("n_sur_pairs", C.c_longlong), ("n_sur_fallback", C.c_longlong),
+        ("n_sur_pairs", C.c_longlong), ("n_sur_fallback", C.c_longlong), ("n_lu_retry", C.c_int),
     ]
 
     def as_dict(self):

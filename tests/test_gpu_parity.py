@@ -207,11 +207,11 @@ def test_mex_entry_points_on_device(efie_mod, orc):
     assert np.abs(Vr[:, 0] - g["vrhs"]).max() <= 1e-15
 
 
-@pytest.mark.parametrize("lu_kernel", [1, 2])
+@pytest.mark.parametrize("lu_kernel", [1, 2, 3])
 def test_c1_sweep_lu_kernels(efie_mod, orc, ctx, lu_kernel):
-    """The mixed path with either matrix-core form of the FP32 LU update
-    (FP32 MFMA, or the exact three-way bf16 split with six part products)
-    refines to the golden Zin, in as many refinement steps."""
+    """The mixed path with each factorisation form (pivoted LU with the FP32
+    MFMA update, with the split-bf16 update, or the unpivoted complex-symmetric
+    factorisation) refines to the golden Zin in about as many steps."""
     g = np.load(W_GOLDEN("c1.npz"))
     V, F, r, pe, ge, spread, _ = _setup(efie_mod, orc, "c1")
     ctx.set_problem(V, F, r)
@@ -222,7 +222,30 @@ def test_c1_sweep_lu_kernels(efie_mod, orc, ctx, lu_kernel):
     assert st["ir_resid_max"] <= 1e-11 and st["ir_iters_max"] <= 8
     ref = ctx.sweep(g["freqs"], W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0,
                     efie_mod.opts(**W.OPTS_C1, precision=1, lu_kernel=1), stats=True)[2]
-    assert st["ir_iters_max"] <= ref["ir_iters_max"] + 1
+    # SYM starts the refinement from x = 0 (no rhs column through the
+    # factorisation), one step more than the pivoted forms
+    assert st["ir_iters_max"] <= ref["ir_iters_max"] + (2 if lu_kernel == 3 else 1)
+
+
+def test_sym_factorisation_retry_c2(efie_mod, ctx):
+    """lu_kernel SYM at c2_6k on frequencies around the band where the
+    unpivoted factorisation meets near-zero pivots (2.3-2.6 GHz): those are
+    solved again with pivoting, and every Zin equals the pivoted path's to
+    the refinement accuracy."""
+    V, F = efie_mod.make_patch_with_ground_mesh(*W.mesh_args("c2_6k"))
+    r = efie_mod.build_rwg_interior(V, F)
+    pP, pG = W.feed_points("c2_6k")
+    pe = efie_mod.pick_port_edge_by_point_and_layer(r, pP, 1)
+    ge = efie_mod.pick_port_edge_by_point_and_layer(r, pG, -1)
+    spread = 1.5 * r.len.mean()
+    ctx.set_problem(V, F, r)
+    fr = np.array([1.6e9, 2.3e9, 2.42e9, 2.5e9, 3.1e9])
+    o = dict(W.OPTS_C1, precision=1, near_mode=1, ir_tol=1e-10)
+    zp = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0, efie_mod.opts(**o, lu_kernel=1))[0]
+    zs, _, st = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0, efie_mod.opts(**o, lu_kernel=3), stats=True)
+    assert 1 <= st["n_lu_retry"] < len(fr)
+    assert st["ir_resid_max"] <= 1e-10
+    assert np.max(np.abs(zs - zp) / np.abs(zp)) <= 1e-9
 
 
 def test_c4_family_small(efie_mod, orc, ctx):
