@@ -493,9 +493,17 @@ __global__ __launch_bounds__(256) void k_sym_u12(T* A, size_t strideA, int lda, 
 // ----------------------------------------------------- panel finish ------
 // k_panel_diag: A11 (w x w, after the interchanges) = L11 U11 in place,
 // without pivoting (the tournament already ordered the pivot rows).
-template <typename T>
+// SYM (complex-symmetric A11, no interchanges): every step reads the lower
+// triangle only.  Column c of the active matrix (d_c L[k][c] for k > c)
+// stands in for row c, and lane c stores it as its U row, so U11 = D L11^T
+// exactly.  The rounding of the two triangles then cannot drift apart;
+// with a drift, U = D L^T of the outer blocks would not be the U of the
+// elimination, and near resonance small pivots amplify that mismatch into a
+// backward error of order 1e-4 in FP32.
+template <typename T, bool SYM = false>
 __global__ __launch_bounds__(64) void k_panel_diag(T* A, size_t strideA, int lda, int j0, int w, int* info) {
-    // one wave, lane k holds row k of the block; row c is broadcast through LDS
+    // one wave, lane k holds row k of the block; row c (SYM: column c) is
+    // broadcast through LDS
     __shared__ cpx<T> prow[NB];
     const int b = blockIdx.x;
     T* Ab = A + (size_t)b * strideA;
@@ -510,7 +518,9 @@ __global__ __launch_bounds__(64) void k_panel_diag(T* A, size_t strideA, int lda
 #pragma unroll
     for (int c = 0; c < NB; ++c) {
         if (c < w) {
-            if (lane == c) {
+            if constexpr (SYM) {
+                if (lane >= c && lane < w) prow[lane] = a[c];
+            } else if (lane == c) {
 #pragma unroll
                 for (int k = c; k < NB; ++k)
                     if (k < w) prow[k] = a[k];
@@ -526,6 +536,10 @@ __global__ __launch_bounds__(64) void k_panel_diag(T* A, size_t strideA, int lda
 #pragma unroll
                 for (int k = c + 1; k < NB; ++k)
                     if (k < w) a[k] = cfms(a[k], l, prow[k]);
+            } else if (SYM && lane == c) {
+#pragma unroll
+                for (int k = c + 1; k < NB; ++k)
+                    if (k < w) a[k] = prow[k];
             }
             __syncthreads();
         }
@@ -1381,13 +1395,26 @@ static void panel_phase(T* A, size_t strideA, int n, int lda, int batch, const L
         }
         pf->begin(7);
         int np = 1;
-        hipLaunchKernelGGL(k_panel_diag<T>, dim3(batch), dim3(64), 0, sp, A, strideA, lda, c0, w, wk.info);
+        if (wk.noPivot)
+            hipLaunchKernelGGL((k_panel_diag<T, true>), dim3(batch), dim3(64), 0, sp, A, strideA, lda, c0, w,
+                               wk.info);
+        else
+            hipLaunchKernelGGL((k_panel_diag<T, false>), dim3(batch), dim3(64), 0, sp, A, strideA, lda, c0, w,
+                               wk.info);
         if (c0 + w < n) {
             hipLaunchKernelGGL(k_panel_rows<T>, dim3((n - c0 - w + 255) / 256, batch), dim3(256), 0, sp, A, strideA,
                                lda, c0, w, c0 + w, n);
             ++np;
         }
-        np += panel_cols<T>(A, strideA, lda, batch, c0, w, c0 + w, jend, sp);
+        if (!wk.noPivot) {
+            np += panel_cols<T>(A, strideA, lda, batch, c0, w, c0 + w, jend, sp);
+        } else if (c0 + w < jend) {
+            // the block-local U12 from L21 as well (the upper triangle is
+            // never read in the symmetric factorisation)
+            hipLaunchKernelGGL(k_sym_u12<T>, dim3((jend - c0 - w + 63) / 64, (w + 31) / 32, batch), dim3(256), 0,
+                               sp, A, strideA, lda, c0, c0 + w, c0 + w, jend);
+            ++np;
+        }
         pf->end(7, np);
         pf->begin(8);
         pf->end(8, gemm<T>(A, strideA, lda, batch, c0 + w, n, c0 + w, jend, c0, w, sp));
