@@ -91,31 +91,28 @@ def lu_gemm_flops(n):
 
 def lu_gemm_flops_sym(n, tile=64):
     """Flops of the matrix-core updates of the unpivoted complex-symmetric
-    factorisation of size n (no rhs column): the updates inside each outer
-    block as in the LU, and the trailing update on the 64 x 64 tiles the
-    kernel computes (tiles in strictly upper WB x WB blocks are skipped)."""
+    factorisation of size n (no rhs column), on the 64 x 64 tiles the kernel
+    computes: every update skips the tiles strictly above the diagonal
+    (tile rows rb..rb+63 all above tile columns cb.., i.e. rb + 64 <= cb)."""
     f = 0.0
-
-    def g(r0, r1, c0, c1, K):
-        return 8.0 * max(0, r1 - r0) * max(0, c1 - c0) * K if K > 0 else 0.0
 
     def g_lower(r0, r1, c0, c1, K):
         t = 0.0
         for rb in range(r0, r1, tile):
             for cb in range(c0, c1, tile):
-                if rb // WB >= cb // WB:
-                    t += g(rb, min(r1, rb + tile), cb, min(c1, cb + tile), K)
+                if rb + tile > cb:
+                    t += 8.0 * (min(r1, rb + tile) - rb) * (min(c1, cb + tile) - cb) * K
         return t
     for j0 in range(0, n, WB):
         jend = min(n, j0 + WB)
         for c0 in range(j0, jend, NB):
             w = min(NB, jend - c0)
-            f += g(c0 + w, n, c0 + w, jend, w)           # update inside the block
+            f += g_lower(c0 + w, n, c0 + w, jend, w)     # update inside the block
         if jend >= n:
             break
         jnext = n if n - jend < WB else jend + WB
         f += g_lower(jend, n, jend, jnext, jend - j0)    # the next block's columns
-        f += g_lower(jend, n, jnext, n, jend - j0)       # the rest, lower blocks only
+        f += g_lower(jend, n, jnext, n, jend - j0)       # the rest
     return f
 
 

@@ -720,9 +720,9 @@ __global__ __launch_bounds__(256) void k_gemm(T* A, size_t strideA, int lda, int
     T* Ab = A + (size_t)b * strideA;
     const int rb = r0 + by * BM;
     const int cb = c0 + bx * BN;
-    // symmetric factorisation: tiles in strictly upper WB x WB blocks are
-    // never read again
-    if (skipUpper && rb / WB < cb / WB) return;
+    // symmetric factorisation: tiles strictly above the diagonal are never
+    // read again (U comes from L, and the panels read the lower triangle)
+    if (skipUpper && rb + BM <= cb) return;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wv = tid >> 6;
     const int wr = (wv >> 1) * (BM / 2), wc = (wv & 1) * (BN / 2);
@@ -1116,7 +1116,10 @@ __global__ __launch_bounds__(256) void k_trsv_diag(const T* A, size_t strideA, i
 
 // Single-launch blocked triangular solve.  Workgroups take tickets in
 // dispatch order; ticket v solves block v (lower, top first) or block
-// nblk-1-v (upper, bottom first).  A workgroup subtracts the contribution of
+// nblk-1-v (upper, bottom first).  The grid is (batch, nblk), so dispatch
+// walks the matrices first: the resident workgroups hold the leading blocks
+// of every matrix and all chains advance together (with blocks first, the
+// chip filled with the blocks of a few matrices, most of them waiting).  A workgroup subtracts the contribution of
 // each earlier block as soon as that block's flag carries this launch's
 // epoch, then solves its diagonal block and publishes its x with a flag.
 // It only waits on workgroups with smaller tickets, which were dispatched
@@ -1133,7 +1136,7 @@ __global__ __launch_bounds__(256) void k_trsv_fused(const T* A, size_t strideA, 
     __shared__ cpx<T> xc[64];
     __shared__ cpx<T> part[4][64];
     __shared__ int s_vid;
-    const int b = blockIdx.y;
+    const int b = blockIdx.x;
     if (skip && skip[b]) return;
     const int tid = threadIdx.x;
     const int nblk = (n + 63) / 64;
@@ -1417,7 +1420,7 @@ static void panel_phase(T* A, size_t strideA, int n, int lda, int batch, const L
         }
         pf->end(7, np);
         pf->begin(8);
-        pf->end(8, gemm<T>(A, strideA, lda, batch, c0 + w, n, c0 + w, jend, c0, w, sp));
+        pf->end(8, gemm<T>(A, strideA, lda, batch, c0 + w, n, c0 + w, jend, c0, w, sp, wk.noPivot ? 1 : 0));
     }
 }
 
@@ -1440,10 +1443,10 @@ __global__ void k_ident_ipiv(int* ipiv, int n) {
 // L stored below the diagonal and U on and above it.  Outer blocks and
 // look-ahead as in lu_factor; the panels of a block are panel_phase without
 // the tournament and interchanges; the block row U12 comes from L21 by
-// k_sym_u12 instead of elimination; the trailing update runs only on the
-// lower WB-block triangle, since the strictly upper blocks are never read
-// again.  The rhs column is not carried (the refinement starts from x = 0);
-// the pivots are the identity.
+// k_sym_u12 instead of elimination; the updates skip the 64 x 64 tiles
+// strictly above the diagonal, which are never read again.  The rhs column
+// is not carried (the refinement starts from x = 0); the pivots are the
+// identity.
 template <typename T>
 static hipError_t lu_factor_sym(T* A, size_t strideA, int n, int lda, int batch, const LuWork& wk, hipStream_t s,
                                 const Prof* pf) {
@@ -1555,10 +1558,10 @@ hipError_t lu_trsv(const T* A, size_t strideA, int n, int lda, T* X, size_t stri
         ++tw->epoch;
         hipLaunchKernelGGL(k_zero_int, dim3((batch + 255) / 256), dim3(256), 0, s, tw->tickets, batch);
         if (lower)
-            hipLaunchKernelGGL((k_trsv_fused<T, true>), dim3(nblk, batch), dim3(256), 0, s, A, strideA, lda, n, X,
+            hipLaunchKernelGGL((k_trsv_fused<T, true>), dim3(batch, nblk), dim3(256), 0, s, A, strideA, lda, n, X,
                                strideX, xs, tw->flags, tw->tickets, tw->epoch, skip, tw->err);
         else
-            hipLaunchKernelGGL((k_trsv_fused<T, false>), dim3(nblk, batch), dim3(256), 0, s, A, strideA, lda, n, X,
+            hipLaunchKernelGGL((k_trsv_fused<T, false>), dim3(batch, nblk), dim3(256), 0, s, A, strideA, lda, n, X,
                                strideX, xs, tw->flags, tw->tickets, tw->epoch, skip, tw->err);
         return hipGetLastError();
     }
