@@ -261,7 +261,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
-    ap.add_argument("--lu-kernel", type=int, default=0, help="0 auto, 1 FP32 MFMA, 2 bf16x6 (mixed path)")
+    ap.add_argument("--lu-kernel", type=int, default=0, help="0 auto (symmetric for the mixed sweep), 1 FP32 MFMA pivoted, 2 bf16x6 pivoted, 3 symmetric FP32")
     ap.add_argument("--ir-tol", type=float, default=1e-10)
     ap.add_argument("--nf", type=int, default=0, help="use only the first NF frequencies (profiling runs)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,

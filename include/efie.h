@@ -69,14 +69,15 @@ typedef struct {
  * update on the FP32 matrix cores; BF16X6 splits every FP32 operand into
  * three bf16 parts and accumulates six part products in FP32 on the bf16
  * matrix cores (experimental); SYM uses that Z is complex symmetric: the
- * FP32 factorisation runs without pivoting as A = L D L^T, so the trailing
- * update covers only the lower block triangle and the pivot search and row
+ * FP32 factorisation runs without pivoting as A = L D L^T, every step
+ * reading the lower triangle only (U = D L^T), so the updates cover only the
+ * lower triangle (about half the flops) and the pivot search and row
  * interchanges disappear; the FP64 refinement corrects its larger rounding
- * growth, and a frequency where it does not reach max(ir_tol, 1e-11)
- * (near-zero pivots close to a resonance) is solved again with the pivoted
- * FP32 factorisation (efie_stats.n_lu_retry).  Results have the accuracy of
- * the pivoted path.  AUTO picks SYM in efie_sweep (Z is complex symmetric)
- * and FP32 in efie_zgesv (general matrices). */
+ * growth.  A frequency that does not reach max(ir_tol, 1e-11) (a zero or
+ * tiny pivot) is solved again with the pivoted FP32 factorisation
+ * (efie_stats.n_lu_retry).  Results have the accuracy of the pivoted path.
+ * AUTO picks SYM in efie_sweep (Z is complex symmetric) and FP32 in
+ * efie_zgesv (general matrices). */
 enum { EFIE_LU_AUTO = 0, EFIE_LU_FP32 = 1, EFIE_LU_BF16X6 = 2, EFIE_LU_SYM = 3 };
 
 /* near_mode: DIRECT evaluates the subdivided near/self quadrature at every
@@ -132,7 +133,8 @@ typedef struct {
     long long n_sur_pairs;
     long long n_sur_fallback;
     /* lu_kernel SYM: frequencies solved again with pivoting because the
-     * unpivoted factorisation met a zero pivot or did not refine to ir_tol */
+     * unpivoted factorisation met a zero pivot or did not refine to
+     * max(ir_tol, 1e-11)                                                  */
     int n_lu_retry;
 } efie_stats;
 
