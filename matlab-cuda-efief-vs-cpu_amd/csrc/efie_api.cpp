@@ -257,7 +257,7 @@ struct efie_ctx {
     double cAlpha = -1;
     // per-call buffers
     DBuf dRecs, dFreq, dZ, dZ32, dWp, dWg, dSums, dV, dWl, dOut;
-    DBuf dCand, dCnt, dIpiv, dInfo, dSwp, dPerm, dX, dR, dR32, dRn2, dBn2, dDone, dIters, dResid, dB;
+    DBuf dCand, dCnt, dIpiv, dInfo, dSwp, dPerm, dX, dR, dR32, dRn2, dBn2, dDone, dIters, dResid, dB, dP;
     DBuf dFlags, dTickets, dErr;
     DBuf dSplit;  // bf16 planes of the split-bf16 LU update (lu_kernel BF16X6)
     TrsvWork tw{nullptr, nullptr, nullptr, 0};
@@ -1211,6 +1211,7 @@ efie_status ensure_solver(efie_ctx* c, const efie_opts& o, int n, int lda, int B
         HIPCHK(c->dX.ensure(sizeof(double) * 2 * (size_t)B * n));
         HIPCHK(c->dR.ensure(sizeof(double) * 2 * (size_t)B * n));
         HIPCHK(c->dR32.ensure(sizeof(float) * 2 * (size_t)B * n));
+        HIPCHK(c->dP.ensure(residual_sym_bytes(n, B)));
         HIPCHK(c->dRn2.ensure(sizeof(double) * B));
         HIPCHK(c->dBn2.ensure(sizeof(double) * B));
         HIPCHK(c->dDone.ensure(sizeof(int) * B));
@@ -1306,12 +1307,14 @@ efie_status factor_batch(efie_ctx* c, const efie_opts& o, int n, int lda, int nb
 }
 
 // Solution of a factored batch on stream si.  FP64: back substitution on
-// column n.  MIXED: back substitution in FP32, then FP64 iterative
-// refinement against the FP64 matrix with right-hand sides V + b*vStride.
-// Returns where the FP64 solutions are.
+// column n.  MIXED: back substitution in FP32 (symmetric factorisation:
+// x = 0), then FP64 iterative refinement against the FP64 matrix with
+// right-hand sides V + b*vStride; symZ (the matrix is complex symmetric
+// bit-exact) lets the residual read only its upper triangle.  Returns where
+// the FP64 solutions are.
 efie_status refine_batch(efie_ctx* c, const efie_opts& o, int n, int lda, int nb, const double* V, size_t vStride,
-                         const Prof& pf, const BatchBufs& bb, hipStream_t si, const double** X, size_t* xStride,
-                         size_t* xinc) {
+                         bool symZ, const Prof& pf, const BatchBufs& bb, hipStream_t si, const double** X,
+                         size_t* xStride, size_t* xinc) {
     const size_t zElems = (size_t)n * lda;
     if (o.precision != EFIE_PREC_MIXED) {
         pf.begin(EFIE_K_TRSV);
@@ -1343,8 +1346,13 @@ efie_status refine_batch(efie_ctx* c, const efie_opts& o, int n, int lda, int nb
     pf.end(EFIE_K_OTHER);
     for (int it = 0; it <= o.ir_max_iter; ++it) {
         pf.begin(EFIE_K_OTHER);
-        HIPCHK(launch_residual(bb.Z, 2 * zElems, n, lda, V, vStride, c->dX.as<double>(), c->dR.as<double>(),
-                               c->dRn2.as<double>(), c->dDone.as<int>(), nb, si));
+        if (symZ)
+            HIPCHK(launch_residual_sym(bb.Z, 2 * zElems, n, lda, V, vStride, c->dX.as<double>(),
+                                       c->dR.as<double>(), c->dP.as<double>(), c->dRn2.as<double>(),
+                                       c->dDone.as<int>(), nb, it == 0 && o.lu_kernel == EFIE_LU_SYM, si));
+        else
+            HIPCHK(launch_residual(bb.Z, 2 * zElems, n, lda, V, vStride, c->dX.as<double>(), c->dR.as<double>(),
+                                   c->dRn2.as<double>(), c->dDone.as<int>(), nb, si));
         HIPCHK(launch_ir_check(c->dRn2.as<double>(), c->dBn2.as<double>(), o.ir_tol, nb, c->dDone.as<int>(),
                                c->dIters.as<int>(), c->dResid.as<double>(), it == 0, it == o.ir_max_iter,
                                o.lu_kernel == EFIE_LU_SYM ? o.ir_max_iter - it : 0, si));
@@ -1621,7 +1629,8 @@ efie_status efie_sweep(efie_ctx* c, const double* freqs, int Nf, double mu, doub
         HIPCHK(hipEventRecord(c->evIRStart[buf], si));
         const double* X = nullptr;
         size_t xStride = 0, xinc = 0;
-        st = refine_batch(c, o, n, lda, nb, c->dV.as<double>(), 0, pf, bb, si, &X, &xStride, &xinc);
+        st = refine_batch(c, o, n, lda, nb, c->dV.as<double>(), 0, o.model != EFIE_MODEL_CENTROID, pf, bb, si, &X,
+                           &xStride, &xinc);
         if (st) return st;
         pf.begin(EFIE_K_OTHER);
         HIPCHK(launch_epilogue(X, xStride, xinc, c->dWl.as<double>(), n, patchEdge - 1, groundEdge - 1,
@@ -1767,7 +1776,8 @@ efie_status efie_zgesv(efie_ctx* c, int n, int batch, const double* A, const dou
     st = factor_batch(c, o, n, lda, batch, pf, false, bb);
     if (st) return st;
     HIPCHK(hipEventRecord(c->ev[3], s));
-    st = refine_batch(c, o, n, lda, batch, c->dB.as<double>(), 2 * (size_t)n, pf, bb, s, &X, &xStride, &xinc);
+    st = refine_batch(c, o, n, lda, batch, c->dB.as<double>(), 2 * (size_t)n, false, pf, bb, s, &X, &xStride,
+                       &xinc);
     if (st) return st;
     st = stage_stats(c, o, batch, 0, batch, bb, s);
     if (st) return st;

@@ -178,6 +178,13 @@ hipError_t launch_col_to_vec(const float* A32, size_t strideA, int n, int lda, d
 hipError_t launch_residual(const double* A64, size_t strideA, int n, int lda, const double* V,
                            size_t vStride, const double* X, double* R, double* rn2, const int* done,
                            int batch, hipStream_t s);
+// The same for a complex symmetric A64 (A64 = A64^T bit-exact), reading
+// only its upper triangle; P holds residual_sym_bytes(n, batch) of partial
+// sums.  xZero: X = 0 (R = V, A64 and X not read).
+hipError_t launch_residual_sym(const double* A64, size_t strideA, int n, int lda, const double* V, size_t vStride,
+                               const double* X, double* R, double* P, double* rn2, const int* done, int batch,
+                               int xZero, hipStream_t s);
+size_t residual_sym_bytes(int n, int batch);
 // remaining > 0 (a retry follows): also stop a system whose last reduction
 // factor, kept for the remaining steps, would not reach tol.
 hipError_t launch_ir_check(double* rn2, const double* vn2, double tol, int batch, int* done,

@@ -58,6 +58,128 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ A64
     }
 }
 
+// ------------------------------------------------ symmetric residual ---
+// For Z = Z^T (bit-exact: the record fill mirrors its tiles) R = V - Z x
+// reads only the 64 x 64 tiles on and above the diagonal, half the bytes.
+// k_symv_strips: one block per (matrix b, 64-row strip I) walks the tiles
+// (I, J >= I); it adds Z_IJ x_J to the strip's rows (kept in registers,
+// summed over the lanes at the end, written to Y) and, for J > I, writes
+// Z_IJ^T x_I, the contribution to the rows of strip J, as the partial
+// P[b][I][64J..64J+63].  k_symv_finish sums Y and the partials of the
+// strips above in a fixed order (no atomics: the result does not depend on
+// scheduling) and forms R and ||R||^2.  Thread (wave w, lane l) owns column
+// l of every tile and rows w + 4k, k < 16, of the strip, so each load
+// instruction of a wave reads one 1 KB tile row.
+constexpr int kSymRows = 16;
+
+__global__ __launch_bounds__(256) void k_symv_strips(const double* __restrict__ A64, size_t strideA, int n, int lda,
+                                                     const double* __restrict__ X, double* __restrict__ Y,
+                                                     double* __restrict__ P, const int* __restrict__ done) {
+    __shared__ double2 part[2][4][64];
+    const int b = blockIdx.x, I = blockIdx.y;
+    if (done[b]) return;
+    const int nblk = (n + 63) / 64;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const double2* Ab = reinterpret_cast<const double2*>(A64 + (size_t)b * strideA);
+    const double2* xb = reinterpret_cast<const double2*>(X) + (size_t)b * n;
+    const int r0 = 64 * I;
+    double2 xr[kSymRows], acc[kSymRows];
+#pragma unroll
+    for (int k = 0; k < kSymRows; ++k) {
+        const int r = r0 + w + 4 * k;
+        const double2 v = xb[r < n ? r : n - 1];
+        xr[k] = r < n ? v : make_double2(0.0, 0.0);
+        acc[k] = make_double2(0.0, 0.0);
+    }
+    for (int J = I; J < nblk; ++J) {
+        const int col = 64 * J + lane;
+        const int cc = col < n ? col : n - 1;
+        // all loads unconditional from clamped addresses, masked afterwards
+        const double2 xv = xb[cc];
+        double2 z[kSymRows];
+#pragma unroll
+        for (int k = 0; k < kSymRows; ++k) {
+            const int r = r0 + w + 4 * k;
+            z[k] = Ab[(size_t)(r < n ? r : n - 1) * lda + cc];
+        }
+        const double2 xj = col < n ? xv : make_double2(0.0, 0.0);
+        double pr = 0.0, pi = 0.0;
+#pragma unroll
+        for (int k = 0; k < kSymRows; ++k) {
+            acc[k].x += z[k].x * xj.x - z[k].y * xj.y;
+            acc[k].y += z[k].x * xj.y + z[k].y * xj.x;
+            pr += z[k].x * xr[k].x - z[k].y * xr[k].y;
+            pi += z[k].x * xr[k].y + z[k].y * xr[k].x;
+        }
+        if (J > I) {
+            // two buffers: wave 0 reads buffer J&1 before it reaches the
+            // barrier of tile J+1, and J+2 writes it only after that barrier
+            part[J & 1][w][lane] = make_double2(pr, pi);
+            __syncthreads();
+            if (w == 0 && col < n) {
+                double2 s = part[J & 1][0][lane];
+#pragma unroll
+                for (int q = 1; q < 4; ++q) {
+                    s.x += part[J & 1][q][lane].x;
+                    s.y += part[J & 1][q][lane].y;
+                }
+                reinterpret_cast<double2*>(P)[((size_t)b * nblk + I) * n + col] = s;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kSymRows; ++k) {
+        double sr = acc[k].x, si = acc[k].y;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            sr += __shfl_xor(sr, o, 64);
+            si += __shfl_xor(si, o, 64);
+        }
+        const int r = r0 + w + 4 * k;
+        if (lane == 0 && r < n) reinterpret_cast<double2*>(Y)[(size_t)b * n + r] = make_double2(sr, si);
+    }
+}
+
+// R[b][i] = V[b][i] - (Y[b][i] + sum over strips I < i/64 of P[b][I][i]),
+// in place over Y; rn2[b] += ||R||^2 (one atomic per block).  xZero: x = 0,
+// so R = V without reading Y or P.
+__global__ __launch_bounds__(256) void k_symv_finish(const double* __restrict__ V, size_t vStride, int n,
+                                                     const double* __restrict__ P, double* __restrict__ R,
+                                                     double* __restrict__ rn2, const int* __restrict__ done,
+                                                     int xZero) {
+    __shared__ double red[4];
+    const int b = blockIdx.y;
+    if (done[b]) return;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int nblk = (n + 63) / 64;
+    double e = 0.0;
+    if (i < n) {
+        double2* y = reinterpret_cast<double2*>(R) + (size_t)b * n + i;
+        double sx = 0.0, sy = 0.0;
+        if (!xZero) {
+            const double2 y0 = *y;
+            sx = y0.x;
+            sy = y0.y;
+            const double2* p = reinterpret_cast<const double2*>(P) + (size_t)b * nblk * n + i;
+            const int nI = i / 64;
+            for (int I = 0; I < nI; ++I) {
+                const double2 q = p[(size_t)I * n];
+                sx += q.x;
+                sy += q.y;
+            }
+        }
+        const double2 v = reinterpret_cast<const double2*>(V + (size_t)b * vStride)[i];
+        const double rr = v.x - sx, ri = v.y - sy;
+        *y = make_double2(rr, ri);
+        e = rr * rr + ri * ri;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(rn2 + b, (red[0] + red[1]) + (red[2] + red[3]));
+}
+
 // done[b] once ||R|| <= tol ||V|| or the residual stopped shrinking (less
 // than a factor 2 per correction: the FP64 evaluation floor of V - Z x);
 // keeps the last residual ratio and counts corrections; clears rn2.
@@ -147,6 +269,19 @@ hipError_t launch_residual(const double* A64, size_t strideA, int n, int lda, co
                        X, R, rn2, done);
     return hipGetLastError();
 }
+
+hipError_t launch_residual_sym(const double* A64, size_t strideA, int n, int lda, const double* V, size_t vStride,
+                               const double* X, double* R, double* P, double* rn2, const int* done, int batch,
+                               int xZero, hipStream_t s) {
+    if (!xZero)
+        hipLaunchKernelGGL(k_symv_strips, dim3(batch, (n + 63) / 64), dim3(256), 0, s, A64, strideA, n, lda, X, R,
+                           P, done);
+    hipLaunchKernelGGL(k_symv_finish, dim3((n + 255) / 256, batch), dim3(256), 0, s, V, vStride, n, P, R, rn2, done,
+                       xZero);
+    return hipGetLastError();
+}
+
+size_t residual_sym_bytes(int n, int batch) { return sizeof(double) * 2 * (size_t)batch * ((n + 63) / 64) * n; }
 
 hipError_t launch_ir_check(double* rn2, const double* vn2, double tol, int batch, int* done, int* iters,
                            double* resid, int firstPass, int lastPass, int remaining, hipStream_t s) {
