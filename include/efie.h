@@ -207,6 +207,62 @@ efie_status efie_sweep(efie_ctx* ctx, const double* freqs, int Nf, double mu,
                        const efie_opts* opts, double* Zin, double* S11,
                        efie_stats* stats);
 
+/* ---- precomputed quadrature caches (the prepack struct) ------------------ */
+
+/* The quadrature caches of efie_precompute.m:1-97 as packed by
+ * efie_precompute_pack.m:1-59 (the `prepack` argument of
+ * patch_sweep_fastquad_cuda.cu:463-557, ggassemble_ZV_EFIE_RWG_DIFF_GPU_FAST.cu
+ * :542-641 and assemble_ZV_EFIE_RWG_DIFF_GPU_FAST.cu:277-392), in MATLAB's
+ * column-major layouts:
+ *   r_reg [Nq x 3 x Nt], w_reg [Nq x Nt]   regular points, weights x 2A
+ *   nearMat [Nt x Nt] (uint8)              near pairs (the diagonal is not read)
+ *   self_r [Nq2 x 3 x NsS x Nt], self_w [Nq2 x NsS x Nt], self_a [NsS x Nt]
+ *   near_r [Nq2 x 3 x NsN x Nt], near_w [Nq2 x NsN x Nt]
+ *   aeqTri [Nt]                            sqrt(A/pi), near softening length
+ * The kernels support Nq and Nq2 in {1, 3, 6, 7} and NsS, NsN in
+ * {1, 4, 16, 64}; nearMat must be symmetric. */
+typedef struct {
+    int Nq, Nq2, NsS, NsN;
+    const double* r_reg;
+    const double* w_reg;
+    const unsigned char* nearMat;
+    const double* self_r;
+    const double* self_w;
+    const double* self_a;
+    const double* near_r;
+    const double* near_w;
+    const double* aeqTri;
+    double near_alpha;
+} efie_prepack;
+
+/* Install the caches on the context's problem (set by efie_set_mesh +
+ * efie_set_rwg, or efie_set_rwg_tables): the quadrature-model fills that
+ * follow (efie_assemble_zv, efie_assemble_z, efie_sweep) integrate with
+ * these points, weights, softening lengths and near pairs, and take
+ * near_alpha from the pack; the rule options quad_order, near_order,
+ * near_subdiv, self_subdiv, self_alpha, near_sing_eps and near_alpha of
+ * efie_opts are not used.  near_mode DIRECT or MOMENTS.  pk = NULL returns
+ * to the rule-derived tables; new tables (efie_set_mesh, efie_set_rwg,
+ * efie_set_rwg_tables) drop the pack. */
+efie_status efie_set_prepack(efie_ctx* ctx, const efie_prepack* pk);
+
+/* A problem without the mesh arrays, as assemble_ZV_EFIE_RWG_DIFF_GPU_FAST
+ * (8 inputs, .cu:277-339) receives it: the RWG tables of the rwg struct
+ * and Nt triangles.  Only efie_assemble_z with a prepack can fill it: the
+ * triangles' centroids and areas come from the pack's regular points
+ * (sum w r / sum w and 2 sum w: the rules integrate linear functions
+ * exactly and their weights sum to 1/2). */
+efie_status efie_set_rwg_tables(efie_ctx* ctx, int Nt, const int* plusTri, const int* minusTri,
+                                const int* plusSign, const int* minusSign, const double* rp,
+                                const double* rm, const double* Ap, const double* Am,
+                                const double* len, int Ne);
+
+/* Z alone (no excitation) for k, omega, mu, eps: the Z output of
+ * assemble_ZV_EFIE_RWG_DIFF_GPU_FAST.cu:483-560.  Z is [Ne x Ne]
+ * column-major interleaved complex; quadrature model only. */
+efie_status efie_assemble_z(efie_ctx* ctx, double k_re, double k_im, double omega, double mu,
+                            double eps_re, double eps_im, const efie_opts* opts, double* Z);
+
 /* Dense complex solve A x = b for `batch` independent systems (A: n x n
  * column-major interleaved complex, batch after batch; b, x: n each), the
  * cusolverDnZgetrf + cusolverDnZgetrs pair of the reference

@@ -140,10 +140,11 @@ void subdivide(V3 A, V3 B, V3 C, int depth, std::vector<V3>& out) {
 struct GeoKey {
     int quad_order = -1, near_order = -1, near_subdiv = -1, self_subdiv = -1;
     double near_sing_eps = -1, self_alpha = -1;
+    int pack = 0;  // install count of the prepack in use, 0: rule-derived tables
     bool operator==(const GeoKey& o) const {
         return quad_order == o.quad_order && near_order == o.near_order &&
                near_subdiv == o.near_subdiv && self_subdiv == o.self_subdiv &&
-               near_sing_eps == o.near_sing_eps && self_alpha == o.self_alpha;
+               near_sing_eps == o.near_sing_eps && self_alpha == o.self_alpha && pack == o.pack;
     }
 };
 
@@ -225,6 +226,17 @@ struct efie_ctx {
     std::vector<int> edge, plusTri, minusTri, plusSign, minusSign;
     std::vector<double> rp, rm, Ap, Am, len;
     bool haveMesh = false, haveRwg = false, midValid = false;
+    // efie_set_rwg_tables: RWG tables without V, F and edges
+    bool meshless = false;
+
+    // prepacked quadrature caches (efie_set_prepack) in the internal
+    // layouts, positions absolute; packGen counts installs
+    bool havePack = false;
+    int packGen = 0, pkNq = 0, pkNq2 = 0, pkNsS = 0, pkNsN = 0;
+    double pkNearAlpha = 0;
+    std::vector<Pt> pkReg, pkNear, pkSelf;
+    std::vector<double> pkASelf, pkAeq;
+    std::vector<uint8_t> pkNearMat;  // [t1][t2], diagonal 0
 
     // geometry cache (quadrature tables etc.)
     bool geoValid = false;
@@ -232,6 +244,7 @@ struct efie_ctx {
     int Nq = 0, NpNear = 0, Nq2s = 0, NsSelf = 0;
     long long nNear = 0;
     double rNearMax = 0, rSelfMax = 0, lMax = 0, aNearMax = 0, aSelfMax = 0;
+    double dNearMax = 0;  // bound on the centroid distance of a near pair
     // moment cache (near_mode MOMENTS or EXTRACT), valid for series length
     // momP + 1 of the scheme momExtract says
     int momP = -1;
@@ -274,6 +287,20 @@ struct efie_ctx {
     int openCls = -1;
     hipEvent_t openEv = nullptr;
 };
+
+namespace {
+// Back to the rule-derived quadrature tables.
+void drop_pack(efie_ctx* c) {
+    if (c->havePack) c->geoValid = false;
+    c->havePack = false;
+    c->pkReg.clear();
+    c->pkNear.clear();
+    c->pkSelf.clear();
+    c->pkASelf.clear();
+    c->pkAeq.clear();
+    c->pkNearMat.clear();
+}
+}  // namespace
 
 extern "C" {
 
@@ -538,6 +565,8 @@ efie_status efie_set_mesh(efie_ctx* c, const double* V, int Nv, const int* F, in
     c->midValid = false;
     c->geoValid = false;
     c->cAlpha = -1;
+    c->meshless = false;
+    drop_pack(c);
     return ok();
 }
 
@@ -573,6 +602,105 @@ efie_status efie_set_rwg(efie_ctx* c, const int* edge, const int* plusTri, const
     c->midValid = false;
     c->geoValid = false;
     c->cAlpha = -1;
+    drop_pack(c);
+    return ok();
+}
+
+efie_status efie_set_rwg_tables(efie_ctx* c, int Nt, const int* plusTri, const int* minusTri, const int* plusSign,
+                                const int* minusSign, const double* rp, const double* rm, const double* Ap,
+                                const double* Am, const double* len, int Ne) {
+    if (!c) return fail(EFIE_E_ARGS, "efie:args", "null context");
+    if (!plusTri || !minusTri || !plusSign || !minusSign || !rp || !rm || !Ap || !Am || !len)
+        return fail(EFIE_E_ARGS, "efie:args", "null table");
+    if (Nt < 1) return fail(EFIE_E_ARGS, "mex:pre", "r_reg third dim must be Nt >= 1");
+    if (Ne < 1) return fail(EFIE_E_ARGS, "mex:rwg", "Missing tri/sign fields");
+    for (int e = 0; e < Ne; ++e) {
+        if (plusTri[e] < 0 || plusTri[e] > Nt) return fail(EFIE_E_MESH, "mex:rwg", "plusTri out of range");
+        if (minusTri[e] < 0 || minusTri[e] > Nt) return fail(EFIE_E_MESH, "mex:rwg", "minusTri out of range");
+    }
+    c->Nv = 0;
+    c->Nt = Nt;
+    c->V.clear();
+    c->F.clear();
+    c->edge.clear();
+    c->Ne = Ne;
+    c->plusTri.assign(plusTri, plusTri + Ne);
+    c->minusTri.assign(minusTri, minusTri + Ne);
+    c->plusSign.assign(plusSign, plusSign + Ne);
+    c->minusSign.assign(minusSign, minusSign + Ne);
+    c->rp.assign(rp, rp + 3 * (size_t)Ne);
+    c->rm.assign(rm, rm + 3 * (size_t)Ne);
+    c->Ap.assign(Ap, Ap + Ne);
+    c->Am.assign(Am, Am + Ne);
+    c->len.assign(len, len + Ne);
+    c->haveMesh = false;
+    c->meshless = true;
+    c->haveRwg = true;
+    c->midValid = false;
+    c->geoValid = false;
+    c->cAlpha = -1;
+    drop_pack(c);
+    return ok();
+}
+
+efie_status efie_set_prepack(efie_ctx* c, const efie_prepack* pk) {
+    if (!c) return fail(EFIE_E_ARGS, "efie:args", "null context");
+    if (!pk) {
+        drop_pack(c);
+        return ok();
+    }
+    if (!c->haveRwg) return fail(EFIE_E_STATE, "efie:state", "mesh and RWG tables must be set first");
+    if (!pk->r_reg || !pk->w_reg || !pk->nearMat || !pk->self_r || !pk->self_w || !pk->self_a || !pk->near_r ||
+        !pk->near_w || !pk->aeqTri)
+        return fail(EFIE_E_ARGS, "mex:prepack", "Missing field in prepack.");
+    auto rule = [](int n) { return n == 1 || n == 3 || n == 6 || n == 7; };
+    auto subs = [](int n) { return n == 1 || n == 4 || n == 16 || n == 64; };
+    if (!rule(pk->Nq)) return fail(EFIE_E_ARGS, "mex:r_reg", "r_reg: Nq must be 1, 3, 6 or 7 points.");
+    if (!rule(pk->Nq2)) return fail(EFIE_E_ARGS, "mex:self_r", "self_r: Nq2 must be 1, 3, 6 or 7 points.");
+    if (!subs(pk->NsS)) return fail(EFIE_E_ARGS, "mex:self_r", "self_r: NsS must be 1, 4, 16 or 64.");
+    if (!subs(pk->NsN)) return fail(EFIE_E_ARGS, "mex:near_r", "near_r: NsN must be 1, 4, 16 or 64.");
+    const int Nt = c->Nt, Nq = pk->Nq, Nq2 = pk->Nq2, NsS = pk->NsS, NsN = pk->NsN;
+    std::vector<uint8_t> nm((size_t)Nt * Nt);
+    for (int t1 = 0; t1 < Nt; ++t1)
+        for (int t2 = 0; t2 < Nt; ++t2) {
+            const bool a = pk->nearMat[(size_t)t1 + (size_t)Nt * t2] != 0;
+            if (a != (pk->nearMat[(size_t)t2 + (size_t)Nt * t1] != 0))
+                return fail(EFIE_E_ARGS, "mex:nearMat", "nearMat must be symmetric.");
+            nm[(size_t)t1 * Nt + t2] = (a && t1 != t2) ? 1 : 0;
+        }
+    // efie_precompute_pack.m layouts: x(i, d, p, t) at i + Nq2 (d + 3 (p + Ns t))
+    auto cvt = [&](const double* r, const double* w, int Ns, std::vector<Pt>& out) {
+        out.resize((size_t)Nt * Ns * Nq2);
+        for (int t = 0; t < Nt; ++t)
+            for (int p = 0; p < Ns; ++p)
+                for (int i = 0; i < Nq2; ++i) {
+                    const size_t b = (size_t)i + (size_t)Nq2 * 3 * ((size_t)p + (size_t)Ns * t);
+                    out[((size_t)t * Ns + p) * Nq2 + i] = {r[b], r[b + Nq2], r[b + 2 * (size_t)Nq2],
+                                                           w[(size_t)i + (size_t)Nq2 * ((size_t)p + (size_t)Ns * t)]};
+                }
+    };
+    c->pkReg.resize((size_t)Nt * Nq);
+    for (int t = 0; t < Nt; ++t)
+        for (int i = 0; i < Nq; ++i) {
+            const size_t b = (size_t)i + (size_t)Nq * 3 * t;
+            c->pkReg[(size_t)t * Nq + i] = {pk->r_reg[b], pk->r_reg[b + Nq], pk->r_reg[b + 2 * (size_t)Nq],
+                                            pk->w_reg[(size_t)i + (size_t)Nq * t]};
+        }
+    cvt(pk->near_r, pk->near_w, NsN, c->pkNear);
+    cvt(pk->self_r, pk->self_w, NsS, c->pkSelf);
+    c->pkASelf.resize((size_t)Nt * NsS);
+    for (int t = 0; t < Nt; ++t)
+        for (int p = 0; p < NsS; ++p) c->pkASelf[(size_t)t * NsS + p] = pk->self_a[(size_t)p + (size_t)NsS * t];
+    c->pkAeq.assign(pk->aeqTri, pk->aeqTri + Nt);
+    c->pkNearMat.swap(nm);
+    c->pkNq = Nq;
+    c->pkNq2 = Nq2;
+    c->pkNsS = NsS;
+    c->pkNsN = NsN;
+    c->pkNearAlpha = pk->near_alpha;
+    c->havePack = true;
+    ++c->packGen;
+    c->geoValid = false;
     return ok();
 }
 
@@ -674,77 +802,134 @@ efie_status ensure_edge_mid(efie_ctx* c) {
 // Frequency-independent data of the quadrature model, cached per options.
 efie_status ensure_geometry(efie_ctx* c, const efie_opts* o) {
     GeoKey k;
-    k.quad_order = o->quad_order;
-    k.near_order = o->near_order;
-    k.near_subdiv = o->near_subdiv;
-    k.self_subdiv = o->self_subdiv;
-    k.near_sing_eps = o->near_sing_eps;
-    k.self_alpha = o->self_alpha;
+    if (c->havePack) {
+        k.pack = c->packGen;  // the rule options do not matter
+    } else {
+        if (c->meshless)
+            return fail(EFIE_E_STATE, "efie:state", "RWG tables without a mesh need a prepack (efie_set_prepack)");
+        k.quad_order = o->quad_order;
+        k.near_order = o->near_order;
+        k.near_subdiv = o->near_subdiv;
+        k.self_subdiv = o->self_subdiv;
+        k.near_sing_eps = o->near_sing_eps;
+        k.self_alpha = o->self_alpha;
+    }
     if (c->geoValid && c->key == k) return ok();
     const int Nt = c->Nt, Ne = c->Ne;
     hipStream_t s = c->stream;
 
     std::vector<TriGeo> tg(Nt);
-    std::vector<V3> tv(3 * (size_t)Nt);
+    std::vector<V3> tv;
     double lmax = 0, aeqMax = 0;
+    if (!c->meshless) {
+        tv.resize(3 * (size_t)Nt);
+        for (int t = 0; t < Nt; ++t) {
+            V3 A, B, C;
+            tri_vertices(c, t, A, B, C);
+            tv[3 * t] = A; tv[3 * t + 1] = B; tv[3 * t + 2] = C;
+            tg[t].cx = (A.x + B.x + C.x) / 3.0;
+            tg[t].cy = (A.y + B.y + C.y) / 3.0;
+            tg[t].cz = (A.z + B.z + C.z) / 3.0;
+            tg[t].area2 = norm(cross(B - A, C - A));
+        }
+    } else {
+        // centroid and 2A from the pack's regular points: the rules
+        // integrate linear functions exactly and their weights sum to 1/2
+        const int Nq = c->pkNq;
+        for (int t = 0; t < Nt; ++t) {
+            double sw = 0, sx = 0, sy = 0, sz = 0;
+            for (int i = 0; i < Nq; ++i) {
+                const Pt q = c->pkReg[(size_t)t * Nq + i];
+                sw += q.w;
+                sx += q.w * q.x;
+                sy += q.w * q.y;
+                sz += q.w * q.z;
+            }
+            if (!(sw > 0)) return fail(EFIE_E_ARGS, "mex:pre", "w_reg of a triangle must have a positive sum");
+            tg[t] = {sx / sw, sy / sw, sz / sw, 2 * sw};
+        }
+    }
     for (int t = 0; t < Nt; ++t) {
-        V3 A, B, C;
-        tri_vertices(c, t, A, B, C);
-        tv[3 * t] = A; tv[3 * t + 1] = B; tv[3 * t + 2] = C;
-        tg[t].cx = (A.x + B.x + C.x) / 3.0;
-        tg[t].cy = (A.y + B.y + C.y) / 3.0;
-        tg[t].cz = (A.z + B.z + C.z) / 3.0;
-        tg[t].area2 = norm(cross(B - A, C - A));
         lmax = std::max(lmax, std::sqrt(4 * (0.5 * tg[t].area2) / std::sqrt(3.0)));
         aeqMax = std::max(aeqMax, std::sqrt((0.5 * tg[t].area2) / kPi));
     }
-    double qu[16], qv[16], qw[16];
-    // regular rule: points relative to the centroid, weight qw * area2
-    const int Nq = quad_rule(o->quad_order, qu, qv, qw);
-    std::vector<Pt> reg((size_t)Nt * Nq);
-    for (int t = 0; t < Nt; ++t)
-        for (int i = 0; i < Nq; ++i) {
-            V3 r = map_ref(tv[3 * t], tv[3 * t + 1], tv[3 * t + 2], qu[i], qv[i]);
-            reg[(size_t)t * Nq + i] = {r.x - tg[t].cx, r.y - tg[t].cy, r.z - tg[t].cz, qw[i] * tg[t].area2};
-        }
-    // subdivided rules for near and self pairs
-    double su[16], sv[16], sw[16];
-    const int Nq2 = quad_rule(o->near_order, su, sv, sw);
+    int Nq, Nq2, NsN, NsS;
+    std::vector<Pt> reg, nearPts, selfPts;
+    std::vector<double> aSelf, aeq(Nt), rad(Nt, 0.0);
     double rmaxN = 0, rmaxS = 0, aSelfMax = 0;
-    auto subdiv_pts = [&](int depth, std::vector<Pt>& pts, std::vector<double>* aSelf, double& rmax) {
-        const int Ns = 1 << (2 * depth);
-        pts.resize((size_t)Nt * Ns * Nq2);
-        if (aSelf) aSelf->resize((size_t)Nt * Ns);
-        std::vector<V3> sub;
-        for (int t = 0; t < Nt; ++t) {
-            subdivide(tv[3 * t], tv[3 * t + 1], tv[3 * t + 2], depth, sub);
-            for (int p = 0; p < Ns; ++p) {
-                V3 A = sub[3 * p], B = sub[3 * p + 1], C = sub[3 * p + 2];
-                const double a2p = norm(cross(B - A, C - A));
-                if (aSelf) {
-                    const double as = o->self_alpha * std::sqrt((0.5 * a2p) / kPi);
-                    (*aSelf)[(size_t)t * Ns + p] = as;
-                    aSelfMax = std::max(aSelfMax, as);
-                }
-                for (int i = 0; i < Nq2; ++i) {
-                    V3 r = map_ref(A, B, C, su[i], sv[i]);
-                    Pt q = {r.x - tg[t].cx, r.y - tg[t].cy, r.z - tg[t].cz, sw[i] * a2p};
-                    pts[((size_t)t * Ns + p) * Nq2 + i] = q;
-                    rmax = std::max(rmax, std::sqrt(q.x * q.x + q.y * q.y + q.z * q.z));
+    // distance of a point from its triangle's centroid, folded into rad
+    auto rel = [&](const Pt& q, int t, double& rmax) {
+        Pt r = {q.x - tg[t].cx, q.y - tg[t].cy, q.z - tg[t].cz, q.w};
+        const double d = std::sqrt(r.x * r.x + r.y * r.y + r.z * r.z);
+        rmax = std::max(rmax, d);
+        rad[t] = std::max(rad[t], d);
+        return r;
+    };
+    if (c->havePack) {
+        Nq = c->pkNq;
+        Nq2 = c->pkNq2;
+        NsN = c->pkNsN;
+        NsS = c->pkNsS;
+        double rmaxR = 0;
+        reg.resize(c->pkReg.size());
+        for (size_t i = 0; i < reg.size(); ++i) reg[i] = rel(c->pkReg[i], (int)(i / Nq), rmaxR);
+        nearPts.resize(c->pkNear.size());
+        for (size_t i = 0; i < nearPts.size(); ++i) nearPts[i] = rel(c->pkNear[i], (int)(i / ((size_t)NsN * Nq2)), rmaxN);
+        selfPts.resize(c->pkSelf.size());
+        for (size_t i = 0; i < selfPts.size(); ++i) selfPts[i] = rel(c->pkSelf[i], (int)(i / ((size_t)NsS * Nq2)), rmaxS);
+        aSelf = c->pkASelf;
+        for (double a : aSelf) aSelfMax = std::max(aSelfMax, a);
+        aeq = c->pkAeq;
+        aeqMax = 0;
+        for (double a : aeq) aeqMax = std::max(aeqMax, a);
+    } else {
+        double qu[16], qv[16], qw[16];
+        // regular rule: points relative to the centroid, weight qw * area2
+        Nq = quad_rule(o->quad_order, qu, qv, qw);
+        reg.resize((size_t)Nt * Nq);
+        for (int t = 0; t < Nt; ++t)
+            for (int i = 0; i < Nq; ++i) {
+                V3 r = map_ref(tv[3 * t], tv[3 * t + 1], tv[3 * t + 2], qu[i], qv[i]);
+                reg[(size_t)t * Nq + i] = {r.x - tg[t].cx, r.y - tg[t].cy, r.z - tg[t].cz, qw[i] * tg[t].area2};
+            }
+        // subdivided rules for near and self pairs
+        double su[16], sv[16], sw[16];
+        Nq2 = quad_rule(o->near_order, su, sv, sw);
+        auto subdiv_pts = [&](int depth, std::vector<Pt>& pts, std::vector<double>* aS, double& rmax) {
+            const int Ns = 1 << (2 * depth);
+            pts.resize((size_t)Nt * Ns * Nq2);
+            if (aS) aS->resize((size_t)Nt * Ns);
+            std::vector<V3> sub;
+            for (int t = 0; t < Nt; ++t) {
+                subdivide(tv[3 * t], tv[3 * t + 1], tv[3 * t + 2], depth, sub);
+                for (int p = 0; p < Ns; ++p) {
+                    V3 A = sub[3 * p], B = sub[3 * p + 1], C = sub[3 * p + 2];
+                    const double a2p = norm(cross(B - A, C - A));
+                    if (aS) {
+                        const double as = o->self_alpha * std::sqrt((0.5 * a2p) / kPi);
+                        (*aS)[(size_t)t * Ns + p] = as;
+                        aSelfMax = std::max(aSelfMax, as);
+                    }
+                    for (int i = 0; i < Nq2; ++i) {
+                        V3 r = map_ref(A, B, C, su[i], sv[i]);
+                        Pt q = {r.x - tg[t].cx, r.y - tg[t].cy, r.z - tg[t].cz, sw[i] * a2p};
+                        pts[((size_t)t * Ns + p) * Nq2 + i] = q;
+                        rmax = std::max(rmax, std::sqrt(q.x * q.x + q.y * q.y + q.z * q.z));
+                    }
                 }
             }
-        }
-        return Ns;
-    };
-    std::vector<Pt> nearPts, selfPts;
-    std::vector<double> aSelf;
-    const int NsN = subdiv_pts(o->near_subdiv, nearPts, nullptr, rmaxN);
-    const int NsS = subdiv_pts(o->self_subdiv, selfPts, &aSelf, rmaxS);
-    std::vector<double> aeq(Nt), rad(Nt);
-    for (int t = 0; t < Nt; ++t) {
-        aeq[t] = std::sqrt((0.5 * tg[t].area2) / kPi);
+            return Ns;
+        };
+        NsN = subdiv_pts(o->near_subdiv, nearPts, nullptr, rmaxN);
+        NsS = subdiv_pts(o->self_subdiv, selfPts, &aSelf, rmaxS);
+        for (int t = 0; t < Nt; ++t) aeq[t] = std::sqrt((0.5 * tg[t].area2) / kPi);
+    }
+    // radius bound of each triangle about its centroid: its vertices (and,
+    // with a prepack, every given point)
+    for (int t = 0; t < Nt && !c->meshless; ++t) {
         const V3 cc{tg[t].cx, tg[t].cy, tg[t].cz};
-        rad[t] = std::max(norm(tv[3 * t] - cc), std::max(norm(tv[3 * t + 1] - cc), norm(tv[3 * t + 2] - cc)));
+        rad[t] = std::max(rad[t], std::max(norm(tv[3 * t] - cc), std::max(norm(tv[3 * t + 1] - cc),
+                                                                         norm(tv[3 * t + 2] - cc))));
     }
 
     // RWG sides: f = c (r - v); rwg_eval_on_triangle.m tests t == plusTri first
@@ -769,8 +954,10 @@ efie_status ensure_geometry(efie_ctx* c, const efie_opts* o) {
         }
     }
 
-    std::vector<double> triV(9 * (size_t)Nt);
-    for (int t = 0; t < Nt; ++t)
+    // vertices relative to the centroid (singularity extraction; zero
+    // without mesh arrays, where extraction is not offered)
+    std::vector<double> triV(9 * (size_t)Nt, 0.0);
+    for (int t = 0; t < Nt && !c->meshless; ++t)
         for (int q = 0; q < 3; ++q) {
             const V3 v = tv[3 * t + q];
             triV[9 * (size_t)t + 3 * q] = v.x - tg[t].cx;
@@ -800,11 +987,25 @@ efie_status ensure_geometry(efie_ctx* c, const efie_opts* o) {
     HIPCHK(hipMemcpyAsync(c->dSideTri.p, sideTri.data(), sizeof(int) * sideTri.size(), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(c->dSide.p, side.data(), sizeof(Side) * side.size(), hipMemcpyHostToDevice, s));
 
-    // near pairs (triangles_are_near.m) on the device: map, row counts, list
-    HIPCHK(launch_near_map(c->dTri.as<TriGeo>(), Nt, o->near_sing_eps, c->dNearMat.as<uint8_t>(),
-                           c->dRowCnt.as<int>(), s));
-    std::vector<int> cnt(Nt);
-    HIPCHK(hipMemcpyAsync(cnt.data(), c->dRowCnt.p, sizeof(int) * Nt, hipMemcpyDeviceToHost, s));
+    // near pairs: the pack's map, or triangles_are_near.m on the device;
+    // then the row counts of t2 > t1 and the ordered pair list
+    std::vector<int> cnt(Nt, 0);
+    double dNearMax = 0;
+    if (c->havePack) {
+        for (int t1 = 0; t1 < Nt; ++t1)
+            for (int t2 = t1 + 1; t2 < Nt; ++t2)
+                if (c->pkNearMat[(size_t)t1 * Nt + t2]) {
+                    ++cnt[t1];
+                    const double dx = tg[t1].cx - tg[t2].cx, dy = tg[t1].cy - tg[t2].cy, dz = tg[t1].cz - tg[t2].cz;
+                    dNearMax = std::max(dNearMax, std::sqrt(dx * dx + dy * dy + dz * dz));
+                }
+        HIPCHK(hipMemcpyAsync(c->dNearMat.p, c->pkNearMat.data(), (size_t)Nt * Nt, hipMemcpyHostToDevice, s));
+    } else {
+        dNearMax = o->near_sing_eps * lmax;
+        HIPCHK(launch_near_map(c->dTri.as<TriGeo>(), Nt, o->near_sing_eps, c->dNearMat.as<uint8_t>(),
+                               c->dRowCnt.as<int>(), s));
+        HIPCHK(hipMemcpyAsync(cnt.data(), c->dRowCnt.p, sizeof(int) * Nt, hipMemcpyDeviceToHost, s));
+    }
     HIPCHK(hipStreamSynchronize(s));
     std::vector<int> start(Nt);
     long long tot = 0;
@@ -825,6 +1026,7 @@ efie_status ensure_geometry(efie_ctx* c, const efie_opts* o) {
     c->rNearMax = rmaxN;
     c->rSelfMax = rmaxS;
     c->lMax = lmax;
+    c->dNearMax = dNearMax;
     c->aNearMax = aeqMax;
     c->aSelfMax = aSelfMax;
     c->key = k;
@@ -839,7 +1041,7 @@ efie_status ensure_geometry(efie_ctx* c, const efie_opts* o) {
 // (softened) distance of every near and self point pair.  -1 if P > 60.
 int moment_terms(const efie_ctx* c, const efie_opts* o, double kmax) {
     const bool extract = o->near_mode == EFIE_NEAR_EXTRACT;  // no softening
-    const double dN = o->near_sing_eps * c->lMax;  // centroid distance of a near pair
+    const double dN = c->dNearMax;  // centroid distance of a near pair
     const double aN = extract ? 0.0 : o->near_alpha * c->aNearMax;
     const double aS = extract ? 0.0 : c->aSelfMax;
     const double rN = std::sqrt((dN + 2 * c->rNearMax) * (dN + 2 * c->rNearMax) + aN * aN);
@@ -1022,6 +1224,7 @@ efie_status fill_z(efie_ctx* c, const efie_opts* o, const NearPlan& plan, const 
 // Port weights and V on the device; fails with mex:rhs like the reference
 // when a layer gets no weight.
 efie_status make_rhs(efie_ctx* c, int pe, int ge, double V0, double spread) {
+    if (c->meshless) return fail(EFIE_E_STATE, "efie:state", "the excitation needs the mesh arrays (efie_set_mesh)");
     hipStream_t s = c->stream;
     const int Ne = c->Ne;
     HIPCHK(c->dWp.ensure(sizeof(double) * Ne));
@@ -1041,17 +1244,40 @@ efie_status make_rhs(efie_ctx* c, int pe, int ge, double V0, double spread) {
     return ok();
 }
 
-efie_status prepare(efie_ctx* c, const efie_opts* o, double selfAlphaCentroid) {
-    if (!c->haveMesh || !c->haveRwg)
+// With a prepack the near softening is the pack's, and only the near modes
+// that integrate the pack's points apply; without mesh arrays only the
+// quadrature model can run.
+efie_status apply_pack(efie_ctx* c, efie_opts* o) {
+    if (c->meshless && o->model != EFIE_MODEL_QUADRATURE)
+        return fail(EFIE_E_STATE, "efie:state", "the centroid model needs the mesh arrays");
+    if (!c->havePack || o->model != EFIE_MODEL_QUADRATURE) return ok();
+    if (o->near_mode != EFIE_NEAR_DIRECT && o->near_mode != EFIE_NEAR_MOMENTS)
+        return fail(EFIE_E_ARGS, "efie:opts", "with a prepack near_mode must be DIRECT or MOMENTS");
+    o->near_alpha = c->pkNearAlpha;
+    return ok();
+}
+
+efie_status prepare(efie_ctx* c, efie_opts* o, double selfAlphaCentroid) {
+    if (!(c->haveMesh || c->meshless) || !c->haveRwg)
         return fail(EFIE_E_STATE, "efie:state", "mesh and RWG tables must be set first");
     HIPCHK(hipSetDevice(c->device));
     efie_status st = check_opts(o);
     if (st) return st;
-    st = ensure_edge_mid(c);
+    st = apply_pack(c, o);
     if (st) return st;
+    if (!c->meshless) {
+        st = ensure_edge_mid(c);
+        if (st) return st;
+    }
     if (o->model == EFIE_MODEL_CENTROID) return ensure_centroid(c, selfAlphaCentroid);
     return ensure_geometry(c, o);
 }
+
+// Z of one frequency into the host array Z ([Ne x Ne] column-major
+// interleaved complex), prepare() done: alpha = j omega mu, beta =
+// 1/(j omega eps) (gassemble_ZV_EFIE_RWG_DIFF_GPU_FAST.cu:341-351).
+efie_status fill_single(efie_ctx* c, const efie_opts& o, double k_re, double k_im, double omega, double mu,
+                        double eps_re, double eps_im, double* Z);
 
 // Per-mesh data of the surrogate: the near pairs inside the installed
 // model's envelope (frames, features), the rest, and the near map without
@@ -1169,6 +1395,33 @@ efie_status choose_near(efie_ctx* c, const efie_opts* o, double kmax, bool lossy
     plan->momSelf = true;
     plan->momNear = !sur;
     if (st) st->moment_terms = c->momP + 1;
+    return ok();
+}
+
+efie_status fill_single(efie_ctx* c, const efie_opts& o, double k_re, double k_im, double omega, double mu,
+                        double eps_re, double eps_im, double* Z) {
+    NearPlan plan;
+    efie_status st = choose_near(c, &o, std::hypot(k_re, k_im), k_im != 0.0, nullptr, &plan);
+    if (st) return st;
+    const int Ne = c->Ne;
+    hipStream_t s = c->stream;
+    Freq fr;
+    fr.kr = k_re;
+    fr.ki = k_im;
+    fr.ar = 0.0;
+    fr.ai = omega * mu;
+    {
+        double dr = -omega * eps_im, di = omega * eps_re, d2 = dr * dr + di * di;
+        fr.br = dr / d2;
+        fr.bi = -di / d2;
+    }
+    HIPCHK(c->dFreq.ensure(sizeof(Freq)));
+    HIPCHK(hipMemcpyAsync(c->dFreq.p, &fr, sizeof fr, hipMemcpyHostToDevice, s));
+    HIPCHK(c->dZ.ensure(sizeof(double) * 2 * (size_t)Ne * Ne));
+    st = fill_z(c, &o, plan, c->dFreq.as<Freq>(), c->dZ.as<double>(), (size_t)Ne);
+    if (st) return st;
+    HIPCHK(hipMemcpyAsync(Z, c->dZ.p, sizeof(double) * 2 * (size_t)Ne * Ne, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
     return ok();
 }
 
@@ -1459,31 +1712,26 @@ efie_status efie_assemble_zv(efie_ctx* c, double k_re, double k_im, double omega
     c->prof = false;
     st = make_rhs(c, pe, ge, exc[2], exc[3]);
     if (st) return st;
-    NearPlan plan;
-    st = choose_near(c, &o, std::hypot(k_re, k_im), k_im != 0.0, nullptr, &plan);
+    st = fill_single(c, o, k_re, k_im, omega, mu, eps_re, eps_im, Z);
     if (st) return st;
-    const int Ne = c->Ne;
-    hipStream_t s = c->stream;
-    Freq fr;
-    fr.kr = k_re;
-    fr.ki = k_im;
-    // alpha = j omega mu, beta = 1/(j omega eps) (gassemble_ZV_EFIE_RWG_DIFF_GPU_FAST.cu:341-351)
-    fr.ar = 0.0;
-    fr.ai = omega * mu;
-    {
-        double dr = -omega * eps_im, di = omega * eps_re, d2 = dr * dr + di * di;
-        fr.br = dr / d2;
-        fr.bi = -di / d2;
-    }
-    HIPCHK(c->dFreq.ensure(sizeof(Freq)));
-    HIPCHK(hipMemcpyAsync(c->dFreq.p, &fr, sizeof fr, hipMemcpyHostToDevice, s));
-    HIPCHK(c->dZ.ensure(sizeof(double) * 2 * (size_t)Ne * Ne));
-    st = fill_z(c, &o, plan, c->dFreq.as<Freq>(), c->dZ.as<double>(), (size_t)Ne);
-    if (st) return st;
-    HIPCHK(hipMemcpyAsync(Z, c->dZ.p, sizeof(double) * 2 * (size_t)Ne * Ne, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(Vrhs, c->dV.p, sizeof(double) * 2 * (size_t)Ne, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipMemcpyAsync(Vrhs, c->dV.p, sizeof(double) * 2 * (size_t)c->Ne, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return ok();
+}
+
+efie_status efie_assemble_z(efie_ctx* c, double k_re, double k_im, double omega, double mu, double eps_re,
+                            double eps_im, const efie_opts* opts, double* Z) {
+    if (!c || !Z) return fail(EFIE_E_ARGS, "efie:args", "null argument");
+    efie_opts o;
+    if (opts) o = *opts;
+    else efie_opts_default(&o);
+    if (!c->haveRwg) return fail(EFIE_E_STATE, "efie:state", "mesh and RWG tables must be set first");
+    if (o.model != EFIE_MODEL_QUADRATURE)
+        return fail(EFIE_E_ARGS, "efie:opts", "efie_assemble_z computes the quadrature model only");
+    efie_status st = prepare(c, &o, 0.20);
+    if (st) return st;
+    c->prof = false;
+    return fill_single(c, o, k_re, k_im, omega, mu, eps_re, eps_im, Z);
 }
 
 efie_status efie_sweep(efie_ctx* c, const double* freqs, int Nf, double mu, double eps_re,

@@ -105,7 +105,7 @@ EXPORTS = [
     "efie_opts_default", "efie_opts_preset", "efie_last_error", "efie_last_error_id",
     "efie_mesh_counts", "efie_make_mesh", "efie_rwg_count", "efie_build_rwg", "efie_pick_port",
     "efie_create", "efie_destroy", "efie_set_mesh", "efie_set_rwg", "efie_assemble_zv",
-    "efie_sweep", "efie_zgesv", "efie_diag_copy",
+    "efie_sweep", "efie_zgesv", "efie_diag_copy", "efie_set_prepack", "efie_set_rwg_tables", "efie_assemble_z",
     "efie_sur_cfg_default", "efie_sur_blob_floats", "efie_sur_train", "efie_sur_get", "efie_sur_set",
     "efie_sur_set_exact", "efie_mlp_params", "efie_mlp_forward", "efie_mlp_loss_grad",
 ]
@@ -137,6 +137,9 @@ def lib():
         L.efie_set_mesh.argtypes = [C.c_void_p, _dp, C.c_int, _ip, C.c_int]
         L.efie_set_rwg.argtypes = [C.c_void_p] + [_ip] * 5 + [_dp] * 5 + [C.c_int]
         L.efie_assemble_zv.argtypes = [C.c_void_p] + [C.c_double] * 6 + [_dp, C.c_int, C.POINTER(Opts), _dp, _dp]
+        L.efie_set_prepack.argtypes = [C.c_void_p, C.c_void_p]
+        L.efie_set_rwg_tables.argtypes = [C.c_void_p, C.c_int] + [_ip] * 4 + [_dp] * 5 + [C.c_int]
+        L.efie_assemble_z.argtypes = [C.c_void_p] + [C.c_double] * 6 + [C.POINTER(Opts), _dp]
         L.efie_sweep.argtypes = [C.c_void_p, _dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
                                  C.c_double, C.c_double, C.c_double, C.POINTER(Opts), _dp, _dp, C.POINTER(Stats)]
         L.efie_zgesv.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, C.POINTER(Stats)]
@@ -241,6 +244,16 @@ def pick_port_edge_by_point_and_layer(rwg: Rwg, p, layerSign) -> int:
 
 # ---------------------------------------------------------------- device --
 
+class Prepack(C.Structure):
+    """efie_prepack: the prepack struct of efie_precompute_pack.m as plain
+    arrays (MATLAB column-major layouts)."""
+    _fields_ = [("Nq", C.c_int), ("Nq2", C.c_int), ("NsS", C.c_int), ("NsN", C.c_int),
+                ("r_reg", C.c_void_p), ("w_reg", C.c_void_p), ("nearMat", C.c_void_p),
+                ("self_r", C.c_void_p), ("self_w", C.c_void_p), ("self_a", C.c_void_p),
+                ("near_r", C.c_void_p), ("near_w", C.c_void_p), ("aeqTri", C.c_void_p),
+                ("near_alpha", C.c_double)]
+
+
 class Context:
     """A device context holding one mesh and its RWG tables in HBM."""
 
@@ -284,6 +297,41 @@ class Context:
         self.set_mesh(V, F)
         return self.set_rwg(rwg.edge, rwg.plusTri, rwg.minusTri, rwg.plusSign, rwg.minusSign,
                             rwg.rp, rwg.rm, rwg.Ap, rwg.Am, rwg.len)
+
+    def set_prepack(self, pk):
+        """Install a prepack (dict with the fields of efie_precompute_pack.m:
+        r_reg [Nq,3,Nt], w_reg [Nq,Nt], nearMat [Nt,Nt] uint8, self_r
+        [Nq2,3,NsS,Nt], self_w [Nq2,NsS,Nt], self_a [NsS,Nt], near_r, near_w,
+        aeqTri, near_alpha); None returns to the rule-derived tables."""
+        if pk is None:
+            _check(lib().efie_set_prepack(self._c, None))
+            return self
+        arr = {k: np.asfortranarray(pk[k], dtype=np.float64) for k in
+               ("r_reg", "w_reg", "self_r", "self_w", "self_a", "near_r", "near_w", "aeqTri")}
+        arr["nearMat"] = np.asfortranarray(pk["nearMat"], dtype=np.uint8)
+        s = Prepack(Nq=arr["r_reg"].shape[0], Nq2=arr["self_r"].shape[0], NsS=arr["self_r"].shape[2],
+                    NsN=arr["near_r"].shape[2], near_alpha=float(pk["near_alpha"]),
+                    **{k: v.ctypes.data for k, v in arr.items()})
+        _check(lib().efie_set_prepack(self._c, C.byref(s)))
+        return self
+
+    def set_rwg_tables(self, Nt, plusTri, minusTri, plusSign, minusSign, rp, rm, Ap, Am, len_):
+        """RWG tables without the mesh arrays (the rwg struct of the 8-input
+        assemble_ZV_EFIE_RWG_DIFF_GPU_FAST); a prepack must follow."""
+        a = [np.asfortranarray(x, dtype=np.int32) for x in (plusTri, minusTri, plusSign, minusSign)]
+        b = [np.asfortranarray(x, dtype=np.float64) for x in (rp, rm, Ap, Am, len_)]
+        Ne = a[0].shape[0]
+        _check(lib().efie_set_rwg_tables(self._c, int(Nt), *[_i(x) for x in a], *[_d(x) for x in b], Ne))
+        self.Ne = Ne
+        return self
+
+    def assemble_z(self, k, omega, mu, eps, o: Optional[Opts] = None):
+        k, eps = complex(k), complex(eps)
+        Ne = self.Ne
+        Z = np.zeros(2 * Ne * Ne)
+        o = o or opts()
+        _check(lib().efie_assemble_z(self._c, k.real, k.imag, omega, mu, eps.real, eps.imag, C.byref(o), _d(Z)))
+        return Z.view(np.complex128).reshape((Ne, Ne), order="F")
 
     def assemble_zv(self, k, omega, mu, eps, exc, o: Optional[Opts] = None):
         k, eps = complex(k), complex(eps)

@@ -101,6 +101,68 @@ inline void check_tables(const mxArray* const* prhs, const char* edgeId, const c
     (void)Nt;
 }
 
+// The prepack struct of efie_precompute_pack.m:1-59, with the checks of
+// ggassemble_ZV_EFIE_RWG_DIFF_GPU_FAST.cu:608-641 plus the element counts
+// that keep every read in bounds.  selfA names the field of the self
+// softening lengths ("self_a"; the 8-input entry point also accepts
+// "self_aSoft", the name its reference source reads).  The struct's
+// near_alpha field is read when nearAlphaField is set.
+inline void read_prepack(const mxArray* pk, mwSize Nt, bool nearAlphaField, efie_prepack* out) {
+    check(pk && mxIsStruct(pk), "mex:prepack", "prepack must be a struct from efie_precompute_pack.");
+    auto get = [&](const char* name) -> const mxArray* {
+        const mxArray* f = mxGetField(pk, 0, name);
+        check(f != nullptr, "mex:prepack", "Missing field in prepack.");
+        return f;
+    };
+    const mxArray *rreg = get("r_reg"), *wreg = get("w_reg"), *nm = get("nearMat");
+    const mxArray *sr = get("self_r"), *sw = get("self_w");
+    const mxArray* sa = mxGetField(pk, 0, "self_a");
+    if (!sa) sa = mxGetField(pk, 0, "self_aSoft");
+    check(sa != nullptr, "mex:prepack", "Missing field in prepack.");
+    const mxArray *nr = get("near_r"), *nw = get("near_w"), *aeq = get("aeqTri");
+    check(mxIsDouble(rreg) && mxGetNumberOfDimensions(rreg) == 3, "mex:r_reg", "r_reg must be [Nq x 3 x Nt] double.");
+    check(mxIsDouble(wreg) && mxGetNumberOfDimensions(wreg) == 2, "mex:w_reg", "w_reg must be [Nq x Nt] double.");
+    check(mxIsUint8(nm), "mex:nearMat", "nearMat must be uint8 [Nt x Nt].");
+    check(mxIsDouble(sr) && mxGetNumberOfDimensions(sr) == 4, "mex:self_r", "self_r must be [Nq2 x 3 x NsS x Nt].");
+    check(mxIsDouble(sw) && mxGetNumberOfDimensions(sw) == 3, "mex:self_w", "self_w must be [Nq2 x NsS x Nt].");
+    check(mxIsDouble(sa) && mxGetNumberOfDimensions(sa) == 2, "mex:self_a", "self_a must be [NsS x Nt].");
+    check(mxIsDouble(nr) && mxGetNumberOfDimensions(nr) == 4, "mex:near_r", "near_r must be [Nq2 x 3 x NsN x Nt].");
+    check(mxIsDouble(nw) && mxGetNumberOfDimensions(nw) == 3, "mex:near_w", "near_w must be [Nq2 x NsN x Nt].");
+    check(mxIsDouble(aeq) && mxGetNumberOfElements(aeq) == Nt, "mex:aeqTri", "aeqTri must be [Nt x 1].");
+    const mwSize* dr = mxGetDimensions(rreg);
+    check(dr[1] == 3, "mex:r_reg", "r_reg dim2 must be 3.");
+    check(dr[2] == Nt, "mex:r_reg", "r_reg dim3 must be Nt.");
+    const mwSize Nq = dr[0];
+    check(mxGetM(wreg) == Nq && mxGetN(wreg) == Nt, "mex:w_reg", "w_reg must be [Nq x Nt] double.");
+    check(mxGetNumberOfElements(nm) == Nt * Nt, "mex:nearMat", "nearMat must be uint8 [Nt x Nt].");
+    const mwSize* ds = mxGetDimensions(sr);
+    check(ds[1] == 3, "mex:self_r", "self_r dim2 must be 3.");
+    check(ds[3] == Nt, "mex:self_r", "self_r dim4 must be Nt.");
+    const mwSize Nq2 = ds[0], NsS = ds[2];
+    const mwSize* dn = mxGetDimensions(nr);
+    check(dn[0] == Nq2, "mex:near_r", "near_r Nq2 mismatch.");
+    check(dn[1] == 3, "mex:near_r", "near_r dim2 must be 3.");
+    check(dn[3] == Nt, "mex:near_r", "near_r dim4 must be Nt.");
+    const mwSize NsN = dn[2];
+    check(mxGetNumberOfElements(sw) == Nq2 * NsS * Nt, "mex:self_w", "self_w must be [Nq2 x NsS x Nt].");
+    check(mxGetNumberOfElements(sa) == NsS * Nt, "mex:self_a", "self_a must be [NsS x Nt].");
+    check(mxGetNumberOfElements(nw) == Nq2 * NsN * Nt, "mex:near_w", "near_w must be [Nq2 x NsN x Nt].");
+    out->Nq = (int)Nq;
+    out->Nq2 = (int)Nq2;
+    out->NsS = (int)NsS;
+    out->NsN = (int)NsN;
+    out->r_reg = mxGetDoubles(rreg);
+    out->w_reg = mxGetDoubles(wreg);
+    out->nearMat = mxGetUint8s(nm);
+    out->self_r = mxGetDoubles(sr);
+    out->self_w = mxGetDoubles(sw);
+    out->self_a = mxGetDoubles(sa);
+    out->near_r = mxGetDoubles(nr);
+    out->near_w = mxGetDoubles(nw);
+    out->aeqTri = mxGetDoubles(aeq);
+    out->near_alpha = nearAlphaField ? mxGetScalar(get("near_alpha")) : 0.15;
+}
+
 inline void upload_tables(efie_ctx* c, const mxArray* const* prhs) {
     const mxArray *Vmx = prhs[0], *Fmx = prhs[1], *Emx = prhs[2];
     const mwSize Nv = mxGetM(Vmx), Nt = mxGetM(Fmx), Ne = mxGetM(Emx);

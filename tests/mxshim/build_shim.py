@@ -1,6 +1,6 @@
 """Builds the CPU-side test libraries next to libefie_hip.so (g++): the mx
-API shim and the two MEX entry points against it (_build/libmxshim.so,
-_build/mex_patch_sweep_cuda.so, _build/mex_gassemble_ZV_EFIE_RWG_DIFF_GPU_FAST.so)
+API shim and the MEX entry points against it (_build/libmxshim.so,
+_build/mex_<name>.so for each of MEX_NAMES)
 and a host build of the device sine/cosine (_build/libefie_math_host.so).
 """
 import os
@@ -10,7 +10,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 PKG = os.path.join(ROOT, "matlab-cuda-efief-vs-cpu_amd")
 OUT = os.path.join(PKG, "_build")
-MEX_NAMES = ["patch_sweep_cuda", "gassemble_ZV_EFIE_RWG_DIFF_GPU_FAST"]
+MEX_NAMES = ["patch_sweep_cuda", "gassemble_ZV_EFIE_RWG_DIFF_GPU_FAST", "patch_sweep_fastquad_cuda",
+             "ggassemble_ZV_EFIE_RWG_DIFF_GPU_FAST", "assemble_ZV_EFIE_RWG_DIFF_GPU_FAST"]
 
 
 def build():

@@ -16,6 +16,7 @@ extern "C" {
 typedef size_t mwSize;
 typedef size_t mwIndex;
 typedef int32_t mxInt32;
+typedef uint8_t mxUint8;
 typedef enum { mxREAL = 0, mxCOMPLEX = 1 } mxComplexity;
 typedef struct {
     double real, imag;
@@ -30,6 +31,10 @@ int mxIsDouble(const mxArray* a);
 int mxIsInt32(const mxArray* a);
 int mxIsComplex(const mxArray* a);
 int mxIsStruct(const mxArray* a);
+int mxIsUint8(const mxArray* a);
+mwSize mxGetNumberOfDimensions(const mxArray* a);
+const mwSize* mxGetDimensions(const mxArray* a);
+mxUint8* mxGetUint8s(const mxArray* a);
 double mxGetScalar(const mxArray* a);
 double* mxGetDoubles(const mxArray* a);
 mxComplexDouble* mxGetComplexDoubles(const mxArray* a);

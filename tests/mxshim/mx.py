@@ -25,6 +25,8 @@ def shim():
         L.shim_double.argtypes = [C.c_size_t, C.c_size_t, C.c_int, C.c_void_p]
         L.shim_int32.restype = C.c_void_p
         L.shim_int32.argtypes = [C.c_size_t, C.c_size_t, C.c_void_p]
+        L.shim_nd.restype = C.c_void_p
+        L.shim_nd.argtypes = [C.c_int, C.c_size_t, C.POINTER(C.c_size_t), C.c_void_p]
         L.shim_struct.restype = C.c_void_p
         L.shim_struct.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_void_p)]
         L.shim_data.restype = C.c_void_p
@@ -62,6 +64,12 @@ def to_mx(v):
         a = a.reshape(1, 1)
     elif a.ndim == 1:
         a = a.reshape(-1, 1)
+    if a.dtype == np.uint8 or a.ndim > 2:
+        # N-d or uint8: class 1 double, 2 int32, 4 uint8
+        cls = 4 if a.dtype == np.uint8 else (2 if a.dtype == np.int32 else 1)
+        a = np.asfortranarray(a if cls != 1 else a.astype(np.float64))
+        dims = (C.c_size_t * a.ndim)(*a.shape)
+        return S.shim_nd(cls, a.ndim, dims, a.ctypes.data)
     if a.dtype == np.int32:
         a = np.asfortranarray(a)
         return S.shim_int32(a.shape[0], a.shape[1], a.ctypes.data)

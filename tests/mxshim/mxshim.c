@@ -9,8 +9,10 @@
 
 #include "mex.h"
 
-enum { CLS_DOUBLE = 1, CLS_INT32 = 2, CLS_STRUCT = 3 };
+enum { CLS_DOUBLE = 1, CLS_INT32 = 2, CLS_STRUCT = 3, CLS_UINT8 = 4 };
 
+/* dims[] holds every dimension (MATLAB drops trailing singletons beyond
+ * the second); m = dims[0], n = the product of the others */
 struct mxArray_tag {
     int cls;
     size_t m, n;
@@ -19,6 +21,8 @@ struct mxArray_tag {
     int nfields;
     char** names;
     mxArray** values;
+    size_t ndim;
+    size_t dims[8];
 };
 
 static jmp_buf g_jmp;
@@ -32,6 +36,9 @@ static mxArray* alloc(int cls, size_t m, size_t n, int cplx, size_t esz) {
     a->m = m;
     a->n = n;
     a->cplx = cplx;
+    a->ndim = 2;
+    a->dims[0] = m;
+    a->dims[1] = n;
     size_t cnt = m * n;
     a->data = calloc(cnt > 0 ? cnt : 1, esz * (cplx ? 2 : 1));
     return a;
@@ -45,10 +52,15 @@ int mxIsDouble(const mxArray* a) { return a->cls == CLS_DOUBLE; }
 int mxIsInt32(const mxArray* a) { return a->cls == CLS_INT32; }
 int mxIsComplex(const mxArray* a) { return a->cplx; }
 int mxIsStruct(const mxArray* a) { return a->cls == CLS_STRUCT; }
+int mxIsUint8(const mxArray* a) { return a->cls == CLS_UINT8; }
+mwSize mxGetNumberOfDimensions(const mxArray* a) { return a->ndim; }
+const mwSize* mxGetDimensions(const mxArray* a) { return a->dims; }
+mxUint8* mxGetUint8s(const mxArray* a) { return a->cls == CLS_UINT8 ? (mxUint8*)a->data : NULL; }
 double mxGetScalar(const mxArray* a) {
     if (a->m * a->n == 0) return 0.0;
     if (a->cls == CLS_INT32) return (double)((int32_t*)a->data)[0];
     if (a->cls == CLS_DOUBLE) return ((double*)a->data)[0];
+    if (a->cls == CLS_UINT8) return (double)((uint8_t*)a->data)[0];
     return 0.0;
 }
 double* mxGetDoubles(const mxArray* a) { return (a->cls == CLS_DOUBLE && !a->cplx) ? (double*)a->data : NULL; }
@@ -85,6 +97,20 @@ mxArray* shim_double(size_t m, size_t n, int cplx, const double* src) {
 mxArray* shim_int32(size_t m, size_t n, const int32_t* src) {
     mxArray* a = alloc(CLS_INT32, m, n, 0, 4);
     if (src) memcpy(a->data, src, m * n * 4);
+    return a;
+}
+/* an N-d array (ndim <= 8) of class cls (1 double, 2 int32, 4 uint8),
+ * column-major data; trailing singleton dimensions beyond the second are
+ * dropped as MATLAB does */
+mxArray* shim_nd(int cls, size_t ndim, const size_t* dims, const void* src) {
+    size_t nd = ndim, cnt = 1;
+    while (nd > 2 && dims[nd - 1] == 1) --nd;
+    for (size_t k = 0; k < ndim; ++k) cnt *= dims[k];
+    const size_t esz = cls == CLS_DOUBLE ? 8 : (cls == CLS_INT32 ? 4 : 1);
+    mxArray* a = alloc(cls, dims[0], ndim > 0 ? cnt / (dims[0] ? dims[0] : 1) : 1, 0, esz);
+    a->ndim = nd < 2 ? 2 : nd;
+    for (size_t k = 0; k < 8; ++k) a->dims[k] = k < ndim ? dims[k] : 1;
+    if (src) memcpy(a->data, src, cnt * esz);
     return a;
 }
 mxArray* shim_struct(int nf, const char** names, mxArray** values) {

@@ -207,6 +207,130 @@ def test_mex_entry_points_on_device(efie_mod, orc):
     assert np.abs(Vr[:, 0] - g["vrhs"]).max() <= 1e-15
 
 
+def _c1_prepack(V, F, **kw):
+    import prepack_np
+    o = dict(W.OPTS_C1)
+    keys = ("quad_order", "near_order", "near_sing_eps", "self_subdiv", "self_alpha", "near_subdiv", "near_alpha")
+    a = {k: o[k] for k in keys}
+    a.update(kw)
+    return prepack_np.make_prepack(V, F, **a)
+
+
+def test_prepack_entry_points_on_device(efie_mod, orc):
+    """patch_sweep_fastquad_cuda, ggassemble_ZV_EFIE_RWG_DIFF_GPU_FAST and the
+    8-input assemble_ZV_EFIE_RWG_DIFF_GPU_FAST, given the prepack that
+    efie_precompute_pack.m makes for the C1 options (restated in
+    tests/prepack_np.py): Zin/S11, Z rows and Vrhs against the golden
+    vectors, FP64 and mixed."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "mxshim"))
+    import mx
+    g = np.load(W_GOLDEN("c1.npz"))
+    V, F, r, pe, ge, spread, _ = _setup(efie_mod, orc, "c1")
+    pk = _c1_prepack(V, F)
+    tables = [V, F, r.edge, r.plusTri, r.minusTri, r.plusSign, r.minusSign, r.rp, r.rm, r.Ap, r.Am, r.len]
+    for extra in ([], [{"precision": 1.0}]):
+        zin, s11 = mx.call("patch_sweep_fastquad_cuda", 2, *tables, pk, g["freqs"], W.MU0, complex(W.EPS0, 0),
+                           float(pe), float(ge), 1.0, spread, 50.0, *extra)
+        assert np.max(np.abs(zin[:, 0] - g["zin"]) / np.abs(g["zin"])) <= ZIN_TOL
+        assert np.max(np.abs(s11[:, 0] - g["s11"])) <= ZIN_TOL
+    w = 2 * np.pi * g["freqs"][0]
+    k = w * np.sqrt(W.MU0 * W.EPS0)
+    Z, Vr = mx.call("ggassemble_ZV_EFIE_RWG_DIFF_GPU_FAST", 2, *tables, pk, complex(k, 0), w, W.MU0, W.EPS0,
+                    np.array([pe, ge, 1.0, spread]))
+    assert _entry_rel(Z[g["rows"] - 1], g["zrows"]) <= ZTOL
+    assert np.abs(Vr[:, 0] - g["vrhs"]).max() <= 1e-15
+    rwg = dict(plusTri=r.plusTri, minusTri=r.minusTri, plusSign=r.plusSign, minusSign=r.minusSign, len=r.len,
+               Ap=r.Ap, Am=r.Am, rp=r.rp, rm=r.rm, center=r.center, zSign=r.zSign.astype(np.float64))
+    exc = dict(patchEdge=float(pe), groundEdge=float(ge), V0=1.0, spreadRadius=spread)
+    Z8, V8 = mx.call("assemble_ZV_EFIE_RWG_DIFF_GPU_FAST", 2, rwg, pk, complex(k, 0), w, W.MU0, W.EPS0, exc,
+                     {"near_alpha": W.OPTS_C1["near_alpha"]})
+    assert _entry_rel(Z8[g["rows"] - 1], g["zrows"]) <= ZTOL
+    assert np.abs(V8[:, 0] - g["vrhs"]).max() <= 1e-15
+    # the self softening under the field name the reference source reads
+    pk2 = dict(pk)
+    pk2["self_aSoft"] = pk2.pop("self_a")
+    Z8b, _ = mx.call("assemble_ZV_EFIE_RWG_DIFF_GPU_FAST", 2, rwg, pk2, complex(k, 0), w, W.MU0, W.EPS0, exc,
+                     {"near_alpha": W.OPTS_C1["near_alpha"]})
+    assert np.array_equal(Z8b, Z8)
+
+
+def test_prepack_drives_the_fill(efie_mod, orc, ctx):
+    """The pack's data define Z, not the rule options: a pack made with
+    near_alpha 0.3 equals the rules at 0.3 (and not the opts' 0.15); a pack
+    with an empty near matrix equals the rules with near_sing_eps 0 (every
+    pair of distinct triangles regular); MOMENTS with a pack equals DIRECT
+    with it; the sweep with the pack equals the sweep with the rules;
+    efie_set_prepack(NULL) returns to the rules; extraction is refused."""
+    V, F, r, pe, ge, spread, _ = _setup(efie_mod, orc, "c1")
+    ctx.set_problem(V, F, r)
+    o = dict(W.OPTS_C1)
+    w = 2 * np.pi * 2.7e9
+    k = w * np.sqrt(W.MU0 * W.EPS0)
+    exc = [pe, ge, 1.0, spread, 0.2]
+
+    def Z(**kw):
+        return ctx.assemble_zv(k, w, W.MU0, W.EPS0, exc, efie_mod.opts(**kw))[0]
+    base = Z(**o)
+    ctx.set_prepack(_c1_prepack(V, F, near_alpha=0.3))
+    ZA = Z(**o)
+    ctx.set_prepack(None)
+    assert _entry_rel(ZA, Z(**dict(o, near_alpha=0.3))) <= 1e-12
+    assert _entry_rel(ZA, base) > 1e-6
+    pk0 = _c1_prepack(V, F)
+    pk0["nearMat"] = np.zeros_like(pk0["nearMat"])
+    ctx.set_prepack(pk0)
+    Z0 = Z(**o)
+    ctx.set_prepack(None)
+    assert _entry_rel(Z0, Z(**dict(o, near_sing_eps=0.0))) <= 1e-12
+    assert _entry_rel(Z0, base) > 1e-6
+    pk = _c1_prepack(V, F)
+    ctx.set_prepack(pk)
+    Zd = Z(**o)
+    assert _entry_rel(Zd, base) <= 1e-12
+    assert _entry_rel(Z(**dict(o, near_mode=efie_mod.NEAR_MOMENTS)), Zd) <= 1e-12
+    with pytest.raises(efie_mod.EfieError) as e:
+        Z(**dict(o, near_mode=efie_mod.NEAR_EXTRACT))
+    assert e.value.ident == "efie:opts"
+    fr = W.freqs("c1")
+    zp = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0, efie_mod.opts(**o, precision=1, near_mode=1))[0]
+    ctx.set_prepack(None)
+    zr = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0, efie_mod.opts(**o))[0]
+    assert np.max(np.abs(zp - zr) / np.abs(zr)) <= 1e-10
+    assert _entry_rel(Z(**o), base) == 0.0
+
+
+def test_meshless_problem(efie_mod, orc, ctx):
+    """RWG tables without V/F/edges plus a prepack (the 8-input entry
+    point's inputs): Z equals the mesh problem's with the same pack; the
+    excitation and the sweep need the mesh arrays; without a pack the fill
+    is refused."""
+    V, F, r, pe, ge, spread, _ = _setup(efie_mod, orc, "c1")
+    pk = _c1_prepack(V, F)
+    o = efie_mod.opts(**W.OPTS_C1)
+    w = 2 * np.pi * 2.2e9
+    k = w * np.sqrt(W.MU0 * W.EPS0)
+    ctx.set_problem(V, F, r)
+    ctx.set_prepack(pk)
+    Zm = ctx.assemble_z(k, w, W.MU0, W.EPS0, o)
+    ctx.set_prepack(None)
+    with efie_mod.Context(0) as c:
+        c.set_rwg_tables(F.shape[0], r.plusTri, r.minusTri, r.plusSign, r.minusSign, r.rp, r.rm, r.Ap, r.Am, r.len)
+        with pytest.raises(efie_mod.EfieError) as e:
+            c.assemble_z(k, w, W.MU0, W.EPS0, o)
+        assert e.value.ident == "efie:state"
+        c.set_prepack(pk)
+        Zl = c.assemble_z(k, w, W.MU0, W.EPS0, o)
+        assert _entry_rel(Zl, Zm) <= 1e-12
+        with pytest.raises(efie_mod.EfieError) as e:
+            c.assemble_zv(k, w, W.MU0, W.EPS0, [pe, ge, 1.0, spread, 0.2], o)
+        assert e.value.ident == "efie:state"
+        with pytest.raises(efie_mod.EfieError) as e:
+            c.sweep(W.freqs("c1"), W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0, o)
+        assert e.value.ident == "efie:state"
+
+
 @pytest.mark.parametrize("lu_kernel", [1, 2, 3])
 def test_c1_sweep_lu_kernels(efie_mod, orc, ctx, lu_kernel):
     """The mixed path with each factorisation form (pivoted LU with the FP32
