@@ -169,25 +169,33 @@ def roofline(st, n, nf, fast, sym=False):
             "traffic": None, "work": f"{ms:.1f} ms over {cnt} launches"}
 
 
-def cpu_baseline(V, F, Ne, fr, threads=None):
+def baseline_rows(Ne, threads):
+    """The rows of Z the CPU baseline fills (1-based), 48 per thread."""
+    return np.linspace(1, Ne, 48 * threads).astype(np.int32)
+
+
+def cpu_baseline(V, F, Ne, fr, threads, gpu_rows=None):
     """The reference's CPU path on the host cores: the oracle's restatement
     of assemble_ZV_EFIE_RWG_DIFF_CPU filling a sample of rows of Z (rows in
     parallel, as assemble_ZV_EFIE_RWG_DIFF_CPUp.m does with parfor),
     extrapolated to all rows, plus the dense solve of one full-size system
     by LAPACK zgesv (numpy, the routine behind MATLAB's Z\\V), timed at the
-    real size.  Both at the same thread count."""
+    real size.  Both at the same thread count.  gpu_rows: the same rows of
+    the GPU's Z at that frequency (the fill of the mixed sweep), checked
+    against the oracle's rows (the error of Z the mixed configuration asks
+    to report)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    threads = threads or min(os.cpu_count() or 1, 16)
     ro = O.build_rwg(V, F)
     P = O.Problem(V, F, ro)
     o = O.opts(**W.OPTS_C1)
     fmid = fr[len(fr) // 2]
     w = 2 * np.pi * fmid
     k = O.wavenumber(w, W.MU0, W.EPS0)
-    rows = 48 * threads
+    rws = baseline_rows(Ne, threads)
+    rows = len(rws)
     t = time.perf_counter()
-    P.z_rows(np.linspace(1, Ne, rows).astype(np.int32), k, w, W.MU0, W.EPS0, o, threads)
+    Zo = P.z_rows(rws, k, w, W.MU0, W.EPS0, o, threads)
     t_rows = time.perf_counter() - t
     rng = np.random.default_rng(0)
     A = rng.standard_normal((Ne, Ne)) + 1j * rng.standard_normal((Ne, Ne))
@@ -196,7 +204,11 @@ def cpu_baseline(V, F, Ne, fr, threads=None):
     np.linalg.solve(A, b)
     t_lu = time.perf_counter() - t
     fill_s = t_rows * Ne / rows
-    return {"value": 1.0 / (fill_s + t_lu), "unit": "freq points/s", "cores": threads, "kind": "port",
+    zerr = {}
+    if gpu_rows is not None:
+        zerr = {"z_rows_max_rel_err_gpu_vs_oracle": float(np.max(np.abs(gpu_rows - Zo) / np.abs(Zo))),
+                "z_rows_checked": f"{rows} rows x {Ne} at {fmid/1e9:.2f} GHz"}
+    return {"value": 1.0 / (fill_s + t_lu), "unit": "freq points/s", "cores": threads, "kind": "port", **zerr,
             "sample": f"oracle (C restatement of assemble_ZV_EFIE_RWG_DIFF_CPU) fill of {rows} of {Ne} rows of Z "
                       f"at {fmid/1e9:.2f} GHz on {threads} threads ({t_rows:.1f} s, x{Ne/rows:.1f} -> "
                       f"{fill_s:.0f} s per frequency) + LAPACK zgesv (numpy) at n={Ne} ({t_lu:.1f} s); "
@@ -205,13 +217,18 @@ def cpu_baseline(V, F, Ne, fr, threads=None):
 
 def bench_c4(a):
     """The batched uneven sweep: 32 seeded geometries (N 1000-8000), 64
-    frequencies each, mixed path, run as one sweep per geometry on one
-    context (tables, geometry caches and solver buffers are rebuilt or grown
-    per geometry inside the timed region)."""
+    frequencies each, mixed path.  Geometries are handed out largest first
+    to `--c4-workers` host threads (default 1), each with its own context
+    (streams, buffers); with two, the sweeps of two geometries run side by
+    side on the GPU (measured: 224.9 against 224.6 points/s, so the ends of
+    one geometry's pipeline are not idle GPU time).  Tables, geometry caches
+    and solver buffers are rebuilt or grown per geometry inside the timed
+    region."""
+    import threading
     geoms = W.c4_geometries()
     fr = np.linspace(*W.C4["freqs"])
     o = efie.opts(**W.OPTS_C1, precision=efie.PREC_MIXED, near_mode=efie.NEAR_MOMENTS, ir_tol=a.ir_tol,
-                  lu_kernel=a.lu_kernel)
+                  lu_kernel=a.lu_kernel, batch=a.batch)
     o64 = efie.opts(**W.OPTS_C1, precision=efie.PREC_FP64, near_mode=efie.NEAR_DIRECT)
     probs = []
     for g in geoms:
@@ -220,34 +237,64 @@ def bench_c4(a):
         pP, pG = W.c4_feed_points(g["geom"])
         probs.append((g, V, F, r, efie.pick_port_edge_by_point_and_layer(r, pP, 1),
                       efie.pick_port_edge_by_point_and_layer(r, pG, -1), 1.5 * r.len.mean()))
-    per, errs = [], []
-    with efie.Context(0) as ctx:
-        g, V, F, r, pe, ge, sp = probs[0]
-        ctx.set_problem(V, F, r)
-        ctx.sweep(fr[:4], W.MU0, W.EPS0, pe, ge, 1.0, sp, 50.0, o)   # warm-up
-        t_all = time.perf_counter()
-        for i, (g, V, F, r, pe, ge, sp) in enumerate(probs):
-            t0 = time.perf_counter()
+    order = sorted(range(len(probs)), key=lambda i: -probs[i][3].Ne)
+    nw = max(1, a.c4_workers)
+    ctxs = [efie.Context(0) for _ in range(nw)]
+    per, errs = [None] * len(probs), []
+    try:
+        for ctx in ctxs:   # warm-up: kernels loaded, streams created
+            g, V, F, r, pe, ge, sp = probs[order[-1]]
             ctx.set_problem(V, F, r)
-            zin, _, st = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, sp, 50.0, o, stats=True)
-            dt = time.perf_counter() - t0
-            per.append({"N": int(r.Ne), "s": dt, "freq_per_s": len(fr) / dt, "ir_iters_max": st["ir_iters_max"]})
-            log(f"[c4] geometry {i}: N={r.Ne} {dt:.2f} s")
+            ctx.sweep(fr[:4], W.MU0, W.EPS0, pe, ge, 1.0, sp, 50.0, o)
+        lock = threading.Lock()
+        queue = list(order)
+        fails = []
+
+        def worker(ctx):
+            try:
+                while True:
+                    with lock:
+                        if not queue:
+                            return
+                        i = queue.pop(0)
+                    g, V, F, r, pe, ge, sp = probs[i]
+                    t0 = time.perf_counter()
+                    ctx.set_problem(V, F, r)
+                    zin, _, st = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, sp, 50.0, o, stats=True)
+                    dt = time.perf_counter() - t0
+                    per[i] = {"N": int(r.Ne), "s": dt, "freq_per_s": len(fr) / dt, "ir_iters_max": st["ir_iters_max"],
+                              "n_lu_retry": st["n_lu_retry"]}
+                    log(f"[c4] geometry {i}: N={r.Ne} {dt:.2f} s")
+            except Exception as e:   # reported after the join
+                fails.append(e)
+        t_all = time.perf_counter()
+        ths = [threading.Thread(target=worker, args=(ctx,)) for ctx in ctxs]
+        for t in ths:
+            t.start()
+        for t in ths:
+            t.join()
         t_all = time.perf_counter() - t_all
+        if fails:
+            raise fails[0]
         if not a.no_fp64:
             # accuracy against the FP64 path on every fourth geometry (untimed)
+            ctx = ctxs[0]
             for g, V, F, r, pe, ge, sp in probs[::4]:
                 ctx.set_problem(V, F, r)
                 z64 = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, sp, 50.0, o64)[0]
                 zm = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, sp, 50.0, o)[0]
                 errs.append(float(np.max(np.abs(zm - z64) / np.abs(z64))))
+    finally:
+        for ctx in ctxs:
+            ctx.close()
     npts = len(fr) * len(probs)
     line = {"metric": "freq points/s (Z fill+LU), 32 uneven geometries, mixed path",
             "value": npts / t_all, "unit": "freq points/s", "n_gpus": 1, "steps": 1, "warmup": 1,
             "ms_per_step": 1e3 * t_all, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "c128 fill, c64 LU + c128 refinement", "data": "synthetic",
             "config": {"workload": f"c4: {len(probs)} geometries (seed {W.C4['seed']}), N {min(p['N'] for p in per)}-"
-                                   f"{max(p['N'] for p in per)}, {len(fr)} frequencies 1-4 GHz each"},
+                                   f"{max(p['N'] for p in per)}, {len(fr)} frequencies 1-4 GHz each",
+                       "workers": nw},
             "max_rel_err_zin_vs_fp64": max(errs) if errs else None, "per_geometry": per}
     print(json.dumps(line))
 
@@ -263,6 +310,9 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--lu-kernel", type=int, default=0, help="0 auto (symmetric for the mixed sweep), 1 FP32 MFMA pivoted, 2 bf16x6 pivoted, 3 symmetric FP32")
     ap.add_argument("--ir-tol", type=float, default=1e-10)
+    ap.add_argument("--batch", type=int, default=0, help="frequencies per batch (0: the library's choice)")
+    ap.add_argument("--c4-workers", type=int, default=1,
+                    help="host threads (one context each) sweeping c4 geometries side by side")
     ap.add_argument("--nf", type=int, default=0, help="use only the first NF frequencies (profiling runs)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run the plain copy kernel on this many MiB (counter calibration)")
@@ -291,7 +341,7 @@ def main():
     o64 = efie.opts(**W.OPTS_C1, precision=efie.PREC_FP64, near_mode=efie.NEAR_DIRECT)
     # refinement to ||V - Zx|| <= 1e-10 ||V||
     ofast = efie.opts(**W.OPTS_C1, precision=efie.PREC_MIXED, near_mode=efie.NEAR_MOMENTS, ir_tol=a.ir_tol,
-                      lu_kernel=a.lu_kernel)
+                      lu_kernel=a.lu_kernel, batch=a.batch)
     with efie.Context(0) as ctx:
         ctx.set_problem(V, F, r)
         dt_fast, (zin_f, s11_f, st_f) = timed(ctx, ofast, fr, pe, ge, spread, a.steps, a.warmup, "fast")
@@ -309,12 +359,12 @@ def main():
             t_train = time.perf_counter() - t0
             log(f"surrogate trained in {t_train:.2f} s")
             osur = efie.opts(**W.OPTS_C1, precision=efie.PREC_MIXED, near_mode=efie.NEAR_SURROGATE, ir_tol=a.ir_tol,
-                             lu_kernel=a.lu_kernel)
+                             lu_kernel=a.lu_kernel, batch=a.batch)
             dt_s, (zin_s, _, st_s) = timed(ctx, osur, fr, pe, ge, spread, a.steps, a.warmup, "surrogate")
             _, _, st_sp = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0,
                                     efie.opts(**W.OPTS_C1, precision=efie.PREC_MIXED,
                                               near_mode=efie.NEAR_SURROGATE, ir_tol=a.ir_tol,
-                                              lu_kernel=a.lu_kernel, profile=1), stats=True)
+                                              lu_kernel=a.lu_kernel, batch=a.batch, profile=1), stats=True)
             sur = {"value": len(fr) * a.steps / dt_s, "ms_per_step": 1e3 * dt_s / a.steps,
                    "train_s": t_train, "train": rep, "n_sur_pairs": st_s["n_sur_pairs"],
                    "n_sur_fallback": st_s["n_sur_fallback"],
@@ -326,9 +376,19 @@ def main():
             res["surrogate"] = sur
         # profiled pass of the headline path for the roofline
         oprof = efie.opts(**W.OPTS_C1, precision=efie.PREC_MIXED, near_mode=efie.NEAR_MOMENTS, ir_tol=a.ir_tol,
-                      lu_kernel=a.lu_kernel, profile=1)
+                      lu_kernel=a.lu_kernel, batch=a.batch, profile=1)
         _, _, st_p = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0, oprof, stats=True)
         log("profiled pass done")
+        threads = min(os.cpu_count() or 1, 16)
+        gpu_rows = None
+        if not a.no_cpu:
+            # the fill of the mixed sweep at the baseline's frequency, rows the
+            # baseline fills too
+            wm = 2 * np.pi * fr[len(fr) // 2]
+            Zg, _ = ctx.assemble_zv(wm * np.sqrt(W.MU0 * W.EPS0), wm, W.MU0, W.EPS0, [pe, ge, 1.0, spread, 0.2],
+                                    efie.opts(**W.OPTS_C1, near_mode=efie.NEAR_MOMENTS))
+            gpu_rows = Zg[baseline_rows(n, threads) - 1].copy()
+            del Zg
     Nt = F.shape[0]
     reg_pairs = Nt * (Nt - 1) // 2 - st_p["n_near_pairs"]
     nq = {4: 6, 7: 7, 2: 3, 1: 1}.get(W.OPTS_C1["quad_order"], 6)
@@ -350,7 +410,7 @@ def main():
         "profile": st_p["kernels"],
     }
     if not a.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(V, F, n, fr)
+        line["cpu_baseline"] = cpu_baseline(V, F, n, fr, threads, gpu_rows)
         log("cpu baseline done")
     print(json.dumps(line))
 
