@@ -204,11 +204,27 @@ def cpu_baseline(V, F, Ne, fr, threads, gpu_rows=None):
     np.linalg.solve(A, b)
     t_lu = time.perf_counter() - t
     fill_s = t_rows * Ne / rows
+    # one core: 8 rows of the fill, and zgesv at n = 2048 scaled by n^3
+    from threadpoolctl import threadpool_limits
+    r1 = np.linspace(1, Ne, 8).astype(np.int32)
+    t = time.perf_counter()
+    P.z_rows(r1, k, w, W.MU0, W.EPS0, o, 1)
+    t_rows1 = time.perf_counter() - t
+    n1 = min(Ne, 2048)
+    with threadpool_limits(limits=1):
+        t = time.perf_counter()
+        np.linalg.solve(A[:n1, :n1], b[:n1])
+        t_lu1 = (time.perf_counter() - t) * (Ne / n1) ** 3
+    fill1 = t_rows1 * Ne / len(r1)
+    single = {"value": 1.0 / (fill1 + t_lu1), "unit": "freq points/s", "cores": 1,
+              "sample": f"{len(r1)} rows of the fill ({t_rows1:.1f} s, x{Ne/len(r1):.0f} -> {fill1:.0f} s) + "
+                        f"zgesv at n={n1} on one thread x(Ne/n)^3 -> {t_lu1:.0f} s"}
     zerr = {}
     if gpu_rows is not None:
         zerr = {"z_rows_max_rel_err_gpu_vs_oracle": float(np.max(np.abs(gpu_rows - Zo) / np.abs(Zo))),
                 "z_rows_checked": f"{rows} rows x {Ne} at {fmid/1e9:.2f} GHz"}
-    return {"value": 1.0 / (fill_s + t_lu), "unit": "freq points/s", "cores": threads, "kind": "port", **zerr,
+    return {"value": 1.0 / (fill_s + t_lu), "unit": "freq points/s", "cores": threads, "kind": "port",
+            "single_core": single, **zerr,
             "sample": f"oracle (C restatement of assemble_ZV_EFIE_RWG_DIFF_CPU) fill of {rows} of {Ne} rows of Z "
                       f"at {fmid/1e9:.2f} GHz on {threads} threads ({t_rows:.1f} s, x{Ne/rows:.1f} -> "
                       f"{fill_s:.0f} s per frequency) + LAPACK zgesv (numpy) at n={Ne} ({t_lu:.1f} s); "
