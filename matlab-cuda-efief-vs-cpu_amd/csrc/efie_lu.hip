@@ -1119,13 +1119,28 @@ __global__ __launch_bounds__(256) void k_trsv_diag(const T* A, size_t strideA, i
 // nblk-1-v (upper, bottom first).  The grid is (batch, nblk), so dispatch
 // walks the matrices first: the resident workgroups hold the leading blocks
 // of every matrix and all chains advance together (with blocks first, the
-// chip filled with the blocks of a few matrices, most of them waiting).  A workgroup subtracts the contribution of
-// each earlier block as soon as that block's flag carries this launch's
-// epoch, then solves its diagonal block and publishes its x with a flag.
-// It only waits on workgroups with smaller tickets, which were dispatched
-// before it, so the waits always end; a bounded spin reports an error
-// instead of waiting forever.
+// chip filled with the blocks of a few matrices, most of them waiting).
+// A workgroup subtracts the contribution of the earlier blocks as they are
+// published: it waits for the next one it needs, then takes the run of
+// blocks already published after it (flags are set in ticket order), up to
+// kTrsvRun at a time, loads their x into LDS with one round of loads and
+// applies them; then it solves its diagonal block and publishes its x with
+// a flag.  It only waits on workgroups with smaller tickets, which were
+// dispatched before it, so the waits always end; a bounded spin reports an
+// error instead of waiting forever.
 constexpr long long kSpinLimit = 1LL << 28;
+constexpr int kTrsvRun = 8;
+
+// element j of a wave-held vector, read from lane j (j wave-uniform)
+__device__ __forceinline__ float lane_read(float v, int j) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+__device__ __forceinline__ double lane_read(double v, int j) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, j);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), j);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 
 template <typename T, bool LOWER>
 __global__ __launch_bounds__(256) void k_trsv_fused(const T* A, size_t strideA, int lda, int n, T* X,
@@ -1133,9 +1148,9 @@ __global__ __launch_bounds__(256) void k_trsv_fused(const T* A, size_t strideA, 
                                                     int epoch, const int* skip, int* err) {
     __shared__ cpx<T> u[64][65];
     __shared__ cpx<T> xb[64];
-    __shared__ cpx<T> xc[64];
+    __shared__ cpx<T> xc[kTrsvRun][64];
     __shared__ cpx<T> part[4][64];
-    __shared__ int s_vid;
+    __shared__ int s_vid, s_avail;
     const int b = blockIdx.x;
     if (skip && skip[b]) return;
     const int tid = threadIdx.x;
@@ -1166,78 +1181,104 @@ __global__ __launch_bounds__(256) void k_trsv_fused(const T* A, size_t strideA, 
     }
     for (int i = tid; i < bs; i += 256) xb[i] = ld(Xb + 2 * (size_t)(r0 + i) * xs);
     // thread (row i, quarter q) accumulates sum_j M[r0+i][c0+j] x_c[j], j in
-    // quarter q; its 16 matrix values are loaded before the wait for the
-    // block they multiply
+    // quarter q, over the earlier blocks c
     const int i = tid >> 2, q = tid & 3;
     const int ic = i < bs ? i : bs - 1;
     const T* mrow = Ab + 2 * (size_t)(r0 + ic) * lda;
     cpx<T> acc = {0, 0};
-    for (int step = 0; step < vid; ++step) {
-        const int c = LOWER ? step : nblk - 1 - step;
-        const int c0 = c * 64;
+    auto blk = [&](int st) { return LOWER ? st : nblk - 1 - st; };
+    // this thread's 16 values of row ic in block column blk(st)
+    auto load_m = [&](int st, cpx<T>* m) {
+        const int c0 = blk(st) * 64;
         const int cs = n - c0 < 64 ? n - c0 : 64;
-        cpx<T> m[16];
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
             const int j = q * 16 + k;
             m[k] = ld(mrow + 2 * (c0 + (j < cs ? j : cs - 1)));
         }
+    };
+    cpx<T> m[16], mn[16];
+    for (int step = 0; step < vid;) {
+        load_m(step, m);  // in flight during the wait
         if (tid == 0) {
             long long spins = 0;
-            while (__hip_atomic_load(fl + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+            while (__hip_atomic_load(fl + blk(step), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
                 __builtin_amdgcn_s_sleep(1);
                 if (++spins > kSpinLimit) {
                     atomicExch(err, 1);
                     break;
                 }
             }
+            int av = step + 1;
+            while (av < vid && av < step + kTrsvRun &&
+                   __hip_atomic_load(fl + blk(av), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch)
+                ++av;
+            s_avail = av;
         }
         __syncthreads();
         acquire_wg();
-        for (int j = tid; j < cs; j += 256) xc[j] = ald_c(Xb + 2 * (size_t)(c0 + j) * xs);
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            const int j = q * 16 + k;
-            if (j < cs) {
-                acc.r += m[k].r * xc[j].r - m[k].i * xc[j].i;
-                acc.i += m[k].r * xc[j].i + m[k].i * xc[j].r;
-            }
+        const int av = s_avail;
+        // x of the published run, one round of loads
+        for (int e = tid; e < (av - step) * 64; e += 256) {
+            const int c0 = blk(step + (e >> 6)) * 64, j = e & 63;
+            cpx<T> v = {0, 0};
+            if (c0 + j < n) v = ald_c(Xb + 2 * (size_t)(c0 + j) * xs);
+            xc[e >> 6][j] = v;
         }
         __syncthreads();
+        for (int st = step; st < av; ++st) {
+            const int c0 = blk(st) * 64;
+            const int cs = n - c0 < 64 ? n - c0 : 64;
+            if (st + 1 < av) load_m(st + 1, mn);  // next block's values before this one's products
+            const cpx<T>* xr = xc[st - step];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const int j = q * 16 + k;
+                if (j < cs) {
+                    acc.r += m[k].r * xr[j].r - m[k].i * xr[j].i;
+                    acc.i += m[k].r * xr[j].i + m[k].i * xr[j].r;
+                }
+            }
+            if (st + 1 < av) {
+#pragma unroll
+                for (int k = 0; k < 16; ++k) m[k] = mn[k];
+            }
+        }
+        __syncthreads();  // xc is refilled by the next run
+        step = av;
     }
     part[q][i] = acc;
     __syncthreads();
-    if (tid < bs) {
-        cpx<T> v = xb[tid];
-        for (int k = 0; k < 4; ++k) {
-            v.r -= part[k][tid].r;
-            v.i -= part[k][tid].i;
-        }
-        xb[tid] = v;
-    }
-    __syncthreads();
+    // diagonal solve by wave 0 with x in registers (lane k holds x_k);
+    // x_j reaches every lane by readlane, so the steps need no LDS round
+    // trip or barrier
     if (tid < 64) {
+        cpx<T> v = {0, 0};
+        if (tid < bs) {
+            v = xb[tid];
+            for (int k = 0; k < 4; ++k) {
+                v.r -= part[k][tid].r;
+                v.i -= part[k][tid].i;
+            }
+        }
         for (int jj = 0; jj < bs; ++jj) {
             const int j = LOWER ? jj : bs - 1 - jj;
-            if (!LOWER && tid == 0) {
+            cpx<T> xj;
+            xj.r = lane_read(v.r, j);
+            xj.i = lane_read(v.i, j);
+            if (!LOWER) {
                 const cpx<T> d = u[j][j];
-                xb[j] = (d.r == (T)0 && d.i == (T)0) ? xb[j] : cmul(xb[j], cinv(d));
+                xj = (d.r == (T)0 && d.i == (T)0) ? xj : cmul(xj, cinv(d));
+                if (tid == j) v = xj;
             }
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-            const cpx<T> xj = xb[j];
             const bool act = LOWER ? (tid > j && tid < bs) : (tid < j);
             if (act) {
                 const cpx<T> l = u[tid][j];
-                cpx<T> v = xb[tid];
                 v.r -= l.r * xj.r - l.i * xj.i;
                 v.i -= l.r * xj.i + l.i * xj.r;
-                xb[tid] = v;
             }
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         }
+        if (tid < bs) xb[tid] = v;
     }
     __syncthreads();
     for (int k = tid; k < bs; k += 256) ast_c(Xb + 2 * (size_t)(r0 + k) * xs, xb[k]);
