@@ -366,7 +366,11 @@ def main():
             dt_64, (zin_64, s11_64, st_64) = timed(ctx, o64, fr, pe, ge, spread, a.steps, a.warmup, "fp64")
             res["fp64"] = {"value": len(fr) * a.steps / dt_64, "ms_per_step": 1e3 * dt_64 / a.steps,
                            "stats": st_64}
-            res["max_rel_err_zin_fast_vs_fp64"] = float(np.max(np.abs(zin_f - zin_64) / np.abs(zin_64)))
+            e = np.abs(zin_f - zin_64) / np.abs(zin_64)
+            res["max_rel_err_zin_fast_vs_fp64"] = float(np.max(e))
+            # per-frequency errors: their spread and where the largest falls
+            res["zin_err_per_freq"] = {"median": float(np.median(e)), "p90": float(np.quantile(e, 0.9)),
+                                       "max_at_ghz": float(fr[int(np.argmax(e))] / 1e9)}
         if not a.no_surrogate:
             # learned near pairs: train on the GPU (one-off, outside the timed
             # sweeps), then the mixed + batched + surrogate sweep
