@@ -153,8 +153,8 @@ struct TrsvWork {
     int epoch;
 };
 
-// Back substitution U x = y in place on column n of A (tw may be null: one
-// launch pair per 64-row block instead of the single-launch solve).
+// Back substitution U x = y in place on column n of A (single-launch
+// solve; tw holds its flags, tickets and error word).
 template <typename T>
 hipError_t lu_backsolve(T* A, size_t strideA, int n, int lda, int batch, hipStream_t s, TrsvWork* tw);
 // Triangular solve with the factors in A on a vector x (element e of batch

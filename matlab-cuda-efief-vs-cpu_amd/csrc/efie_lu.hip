@@ -1068,52 +1068,8 @@ __global__ __launch_bounds__(256, 2) void k_gemm_p6(float* A, size_t strideA, in
 // ------------------------------------------------- triangular solves -----
 // Blocked triangular solve on a vector x (element e of batch b at
 // X[b*strideX + 2*e*xs]), upper (U x = y, non-unit diagonal, bottom block
-// first) or unit lower (L y = r, top block first): the diagonal block is
-// solved by one wave in LDS, then the rows not yet solved are updated by a
-// GEMV.  Batch elements with skip[b] != 0 are left untouched.
-template <typename T, bool LOWER>
-__global__ __launch_bounds__(256) void k_trsv_diag(const T* A, size_t strideA, int lda, T* X,
-                                                   size_t strideX, size_t xs, int r0, int bs,
-                                                   const int* skip) {
-    __shared__ cpx<T> u[64][65];
-    __shared__ cpx<T> x[64];
-    const int b = blockIdx.x;
-    if (skip && skip[b]) return;
-    const T* Ab = A + (size_t)b * strideA;
-    T* Xb = X + (size_t)b * strideX;
-    const int tid = threadIdx.x;
-    for (int e = tid; e < bs * bs; e += 256) {
-        int i = e / bs, k = e % bs;
-        u[i][k] = ld(Ab + 2 * ((size_t)(r0 + i) * lda + r0 + k));
-    }
-    for (int i = tid; i < bs; i += 256) x[i] = ld(Xb + 2 * (size_t)(r0 + i) * xs);
-    __syncthreads();
-    if (tid < 64) {
-        for (int jj = 0; jj < bs; ++jj) {
-            const int j = LOWER ? jj : bs - 1 - jj;
-            if (!LOWER && tid == 0) {
-                cpx<T> d = u[j][j];
-                x[j] = (d.r == (T)0 && d.i == (T)0) ? x[j] : cmul(x[j], cinv(d));
-            }
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-            const cpx<T> xj = x[j];
-            const bool act = LOWER ? (tid > j && tid < bs) : (tid < j);
-            if (act) {
-                cpx<T> l = u[tid][j];
-                cpx<T> v = x[tid];
-                v.r -= l.r * xj.r - l.i * xj.i;
-                v.i -= l.r * xj.i + l.i * xj.r;
-                x[tid] = v;
-            }
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-        }
-    }
-    __syncthreads();
-    for (int i = tid; i < bs; i += 256) st(Xb + 2 * (size_t)(r0 + i) * xs, x[i]);
-}
-
+// first) or unit lower (L y = r, top block first), in 64-row blocks.
+//
 // Single-launch blocked triangular solve.  Workgroups take tickets in
 // dispatch order; ticket v solves block v (lower, top first) or block
 // nblk-1-v (upper, bottom first).  The grid is (batch, nblk), so dispatch
@@ -1285,39 +1241,6 @@ __global__ __launch_bounds__(256) void k_trsv_fused(const T* A, size_t strideA, 
     release_agent();
     __syncthreads();
     if (tid == 0) __hip_atomic_store(fl + bk, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// rows [rlo, rhi) -= A[rows, r0:r0+bs] * x[r0:r0+bs]; one wave per row
-template <typename T>
-__global__ __launch_bounds__(256) void k_trsv_update(const T* A, size_t strideA, int lda, T* X,
-                                                     size_t strideX, size_t xs, int r0, int bs,
-                                                     int rlo, int rhi, const int* skip) {
-    const int b = blockIdx.y;
-    if (skip && skip[b]) return;
-    const T* Ab = A + (size_t)b * strideA;
-    T* Xb = X + (size_t)b * strideX;
-    const int lane = threadIdx.x & 63;
-    const int row = rlo + blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (row >= rhi) return;
-    cpx<T> s = {0, 0};
-    if (lane < bs) {
-        cpx<T> u = ld(Ab + 2 * ((size_t)row * lda + r0 + lane));
-        cpx<T> xv = ld(Xb + 2 * (size_t)(r0 + lane) * xs);
-        s.r = u.r * xv.r - u.i * xv.i;
-        s.i = u.r * xv.i + u.i * xv.r;
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        s.r += __shfl_xor(s.r, o, 64);
-        s.i += __shfl_xor(s.i, o, 64);
-    }
-    if (lane == 0) {
-        T* y = Xb + 2 * (size_t)row * xs;
-        cpx<T> v = ld(y);
-        v.r -= s.r;
-        v.i -= s.i;
-        st(y, v);
-    }
 }
 
 // perm[b][i] = source row of position i after the sequential swaps of ipiv
@@ -1595,32 +1518,15 @@ template <typename T>
 hipError_t lu_trsv(const T* A, size_t strideA, int n, int lda, T* X, size_t strideX, size_t xs,
                    int batch, bool lower, const int* skip, hipStream_t s, TrsvWork* tw) {
     const int nblk = (n + 63) / 64;
-    if (tw) {
-        ++tw->epoch;
-        hipLaunchKernelGGL(k_zero_int, dim3((batch + 255) / 256), dim3(256), 0, s, tw->tickets, batch);
-        if (lower)
-            hipLaunchKernelGGL((k_trsv_fused<T, true>), dim3(batch, nblk), dim3(256), 0, s, A, strideA, lda, n, X,
-                               strideX, xs, tw->flags, tw->tickets, tw->epoch, skip, tw->err);
-        else
-            hipLaunchKernelGGL((k_trsv_fused<T, false>), dim3(batch, nblk), dim3(256), 0, s, A, strideA, lda, n, X,
-                               strideX, xs, tw->flags, tw->tickets, tw->epoch, skip, tw->err);
-        return hipGetLastError();
-    }
-    for (int q = 0; q < nblk; ++q) {
-        const int bk = lower ? q : nblk - 1 - q;
-        const int r0 = bk * 64;
-        const int bs = n - r0 < 64 ? n - r0 : 64;
-        if (lower)
-            hipLaunchKernelGGL((k_trsv_diag<T, true>), dim3(batch), dim3(256), 0, s, A, strideA, lda, X, strideX,
-                               xs, r0, bs, skip);
-        else
-            hipLaunchKernelGGL((k_trsv_diag<T, false>), dim3(batch), dim3(256), 0, s, A, strideA, lda, X, strideX,
-                               xs, r0, bs, skip);
-        const int rlo = lower ? r0 + bs : 0, rhi = lower ? n : r0;
-        if (rhi > rlo)
-            hipLaunchKernelGGL(k_trsv_update<T>, dim3((rhi - rlo + 3) / 4, batch), dim3(256), 0, s, A, strideA,
-                               lda, X, strideX, xs, r0, bs, rlo, rhi, skip);
-    }
+    if (!tw) return hipErrorInvalidValue;
+    ++tw->epoch;
+    hipLaunchKernelGGL(k_zero_int, dim3((batch + 255) / 256), dim3(256), 0, s, tw->tickets, batch);
+    if (lower)
+        hipLaunchKernelGGL((k_trsv_fused<T, true>), dim3(batch, nblk), dim3(256), 0, s, A, strideA, lda, n, X,
+                           strideX, xs, tw->flags, tw->tickets, tw->epoch, skip, tw->err);
+    else
+        hipLaunchKernelGGL((k_trsv_fused<T, false>), dim3(batch, nblk), dim3(256), 0, s, A, strideA, lda, n, X,
+                           strideX, xs, tw->flags, tw->tickets, tw->epoch, skip, tw->err);
     return hipGetLastError();
 }
 
