@@ -1388,20 +1388,20 @@ static void panel_phase(T* A, size_t strideA, int n, int lda, int batch, const L
     }
 }
 
-// Blocked right-looking LU with look-ahead.  Outer blocks of WB columns are
-// factored as NB-wide sub-panels (panel_phase) whose interchanges touch
-// only the block's columns at first.  For every block: apply its
-// interchanges to all other columns (incl. the rhs column n), U12 = L^-1
-// A12 over the columns right of it, then the trailing update A22 -= L21
-// U12 with K = WB, split in two: the next block's columns first, after
-// which that block's panels run on the panel stream while the main stream
-// updates the remaining columns.  The two never touch the same columns:
-// the panels stay inside their block, and their interchanges reach the
-// other columns only after the update that reads those rows has finished.
+// ipiv = identity (the symmetric factorisation does not pivot)
 __global__ void k_ident_ipiv(int* ipiv, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) ipiv[(size_t)blockIdx.y * n + i] = i + 1;
 }
+
+// The first failure of a sequence of stream and event calls (launch
+// errors surface through hipGetLastError at the end of the sequence).
+struct HipFirst {
+    hipError_t e = hipSuccess;
+    void operator()(hipError_t r) {
+        if (e == hipSuccess && r != hipSuccess) e = r;
+    }
+};
 
 // Complex-symmetric factorisation without pivoting: A = L U with U = D L^T,
 // L stored below the diagonal and U on and above it.  Outer blocks and
@@ -1414,19 +1414,20 @@ __global__ void k_ident_ipiv(int* ipiv, int n) {
 template <typename T>
 static hipError_t lu_factor_sym(T* A, size_t strideA, int n, int lda, int batch, const LuWork& wk, hipStream_t s,
                                 const Prof* pf) {
+    HipFirst chk;
     hipStream_t sp = wk.panel ? wk.panel : s;
     hipLaunchKernelGGL(k_ident_ipiv, dim3((n + 255) / 256, batch), dim3(256), 0, s, wk.ipiv, n);
     if (sp != s) {
-        hipEventRecord(wk.evAhead, s);
-        hipStreamWaitEvent(sp, wk.evAhead, 0);
+        chk(hipEventRecord(wk.evAhead, s));
+        chk(hipStreamWaitEvent(sp, wk.evAhead, 0));
     }
     panel_phase<T>(A, strideA, n, lda, batch, wk, 0, n < WB ? n : WB, 0, sp, pf);
-    if (sp != s) hipEventRecord(wk.evPanel, sp);
+    if (sp != s) chk(hipEventRecord(wk.evPanel, sp));
     for (int j0 = 0; j0 < n; j0 += WB) {
         const int wb = n - j0 < WB ? n - j0 : WB;
         const int jend = j0 + wb;
         const int jnext = n - jend < WB ? n : jend + WB;
-        if (sp != s) hipStreamWaitEvent(s, wk.evPanel, 0);
+        if (sp != s) chk(hipStreamWaitEvent(s, wk.evPanel, 0));
         if (jend >= n) break;
         pf->begin(7);
         hipLaunchKernelGGL(k_sym_u12<T>, dim3((n - jend + 63) / 64, (wb + 31) / 32, batch), dim3(256), 0, s, A,
@@ -1435,41 +1436,52 @@ static hipError_t lu_factor_sym(T* A, size_t strideA, int n, int lda, int batch,
         pf->begin(8);
         pf->end(8, gemm<T>(A, strideA, lda, batch, jend, n, jend, jnext, j0, wb, s, 1));
         if (sp != s) {
-            hipEventRecord(wk.evAhead, s);
-            hipStreamWaitEvent(sp, wk.evAhead, 0);
+            chk(hipEventRecord(wk.evAhead, s));
+            chk(hipStreamWaitEvent(sp, wk.evAhead, 0));
         }
         panel_phase<T>(A, strideA, n, lda, batch, wk, jend, jnext, 0, sp, pf);
-        if (sp != s) hipEventRecord(wk.evPanel, sp);
+        if (sp != s) chk(hipEventRecord(wk.evPanel, sp));
         pf->begin(8);
         pf->end(8, gemm<T>(A, strideA, lda, batch, jend, n, jnext, n, j0, wb, s, 1));
     }
-    return hipGetLastError();
+    return chk.e != hipSuccess ? chk.e : hipGetLastError();
 }
 
+// Blocked right-looking LU with look-ahead.  Outer blocks of WB columns are
+// factored as NB-wide sub-panels (panel_phase) whose interchanges touch
+// only the block's columns at first.  For every block: apply its
+// interchanges to all other columns (incl. the rhs column n), U12 = L^-1
+// A12 over the columns right of it, then the trailing update A22 -= L21
+// U12 with K = WB, split in two: the next block's columns first, after
+// which that block's panels run on the panel stream while the main stream
+// updates the remaining columns.  The two never touch the same columns:
+// the panels stay inside their block, and their interchanges reach the
+// other columns only after the update that reads those rows has finished.
 template <typename T>
 hipError_t lu_factor(T* A, size_t strideA, int n, int lda, int batch, LuWork wk, hipStream_t s,
                      const Prof* pf) {
     Prof none;
     if (!pf) pf = &none;
     if (wk.noPivot) return lu_factor_sym<T>(A, strideA, n, lda, batch, wk, s, pf);
+    HipFirst chk;
     hipStream_t sp = wk.panel ? wk.panel : s;
     // candidate and counter slots per batch element are sized for the first
     // (largest) panel; every counter is back at zero when a launch ends
     const int nodeStride = tree_geom(n).nNodes;
     if (sp != s) {
         // the panel stream starts after everything already queued on s
-        hipEventRecord(wk.evAhead, s);
-        hipStreamWaitEvent(sp, wk.evAhead, 0);
+        chk(hipEventRecord(wk.evAhead, s));
+        chk(hipStreamWaitEvent(sp, wk.evAhead, 0));
     }
     panel_phase<T>(A, strideA, n, lda, batch, wk, 0, n < WB ? n : WB, nodeStride, sp, pf);
     if (sp != s) {
-        hipEventRecord(wk.evPanel, sp);
+        chk(hipEventRecord(wk.evPanel, sp));
     }
     for (int j0 = 0; j0 < n; j0 += WB) {
         const int wb = n - j0 < WB ? n - j0 : WB;
         const int jend = j0 + wb;
         const int jnext = n - jend < WB ? n : jend + WB;
-        if (sp != s) hipStreamWaitEvent(s, wk.evPanel, 0);
+        if (sp != s) chk(hipStreamWaitEvent(s, wk.evPanel, 0));
         // this block's interchanges on the columns outside it
         const int nrest = j0 + (n + 1 - jend);
         if (nrest > 0) {
@@ -1500,18 +1512,18 @@ hipError_t lu_factor(T* A, size_t strideA, int n, int lda, int batch, LuWork wk,
         }
         if (!p6) pf->end(8, gemm<T>(A, strideA, lda, batch, jend, n, jend, jnext, j0, wb, s));
         if (sp != s) {
-            hipEventRecord(wk.evAhead, s);
-            hipStreamWaitEvent(sp, wk.evAhead, 0);
+            chk(hipEventRecord(wk.evAhead, s));
+            chk(hipStreamWaitEvent(sp, wk.evAhead, 0));
         }
         panel_phase<T>(A, strideA, n, lda, batch, wk, jend, jnext, nodeStride, sp, pf);
-        if (sp != s) hipEventRecord(wk.evPanel, sp);
+        if (sp != s) chk(hipEventRecord(wk.evPanel, sp));
         pf->begin(8);
         if constexpr (std::is_same<T, float>::value) {
             if (p6) pf->end(8, gemm_p6(A, strideA, lda, batch, jend, n, jnext, n + 1, wk, n, jend, s));
         }
         if (!p6) pf->end(8, gemm<T>(A, strideA, lda, batch, jend, n, jnext, n + 1, j0, wb, s));
     }
-    return hipGetLastError();
+    return chk.e != hipSuccess ? chk.e : hipGetLastError();
 }
 
 template <typename T>
