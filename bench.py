@@ -169,6 +169,34 @@ def roofline(st, n, nf, fast, sym=False):
             "traffic": None, "work": f"{ms:.1f} ms over {cnt} launches"}
 
 
+def fill_rates(st, n, Nt, nf):
+    """Achieved rates of the two fill kernels of a profiled sweep against
+    MI355X peaks: the regular records (FP64 VALU work: Green evaluations x
+    FLOP_PER_GREEN) and the combine (HBM: the records read once plus the
+    FP64 and FP32 Z written once, per frequency; and the bytes rocprof's
+    calibrated counters measured for it)."""
+    ks = st["kernels"]
+    out = {}
+    if "regular" in ks and ks["regular"]["ms"] > 0:
+        work = st["n_green_regular"] * FLOP_PER_GREEN * nf
+        ach = work / (ks["regular"]["ms"] * 1e-3) / 1e12
+        out["k_rec_regular"] = {"bound": "fp64 valu", "achieved": ach, "peak": PEAK["fp64"], "unit": "TFLOP/s",
+                                "frac": ach / PEAK["fp64"],
+                                "work": f"{st['n_green_regular']:.3e} Green evaluations x {FLOP_PER_GREEN} flop "
+                                        f"per frequency"}
+    if "combine" in ks and ks["combine"]["ms"] > 0:
+        lda = ((n + 1) + 31) // 32 * 32
+        alg = 128.0 * Nt * (Nt + 1) / 2 + 24.0 * n * lda
+        t = ks["combine"]["ms"] * 1e-3 / nf
+        meas = traffic("k_combine")
+        out["k_combine"] = {"bound": "hbm", "achieved": alg / t / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                            "frac": alg / t / 1e9 / PEAK_HBM_GBS,
+                            "measured_bytes_GBps": (meas / t / 1e9) if meas else None,
+                            "work": f"records once ({128 * Nt * (Nt + 1) / 2:.3e} B) + FP64 and FP32 Z once "
+                                    f"({24 * n * lda:.3e} B) per frequency, {t * 1e6:.0f} us per launch"}
+    return out
+
+
 def baseline_rows(Ne, threads):
     """The rows of Z the CPU baseline fills (1-based), 48 per thread."""
     return np.linspace(1, Ne, 48 * threads).astype(np.int32)
@@ -428,6 +456,7 @@ def main():
         "fast_vs_fp64": (value / res["fp64"]["value"]) if "fp64" in res else None,
         **res,
         "profile": st_p["kernels"],
+        "fill": fill_rates(st_p, n, Nt, len(fr)),
     }
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(V, F, n, fr, threads, gpu_rows)
