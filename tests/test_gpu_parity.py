@@ -384,6 +384,37 @@ def test_sym_factorisation_c2(efie_mod, ctx):
     assert np.max(np.abs(zs2 - zp2) / np.abs(zp2)) <= 1e-12
 
 
+def test_c2_6k_scale_rows_and_zin(efie_mod, orc, ctx):
+    """GPU scale (c2_6k, Ne = 6009): 32 sampled rows of Z at 2.5 GHz
+    against the oracle's rows (direct and moment near modes, 1e-10
+    entry-relative), and the mixed path's Zin against the FP64 path's at 5
+    frequencies across the resonance band (1e-9)."""
+    import os
+    V, F = efie_mod.make_patch_with_ground_mesh(*W.mesh_args("c2_6k"))
+    r = efie_mod.build_rwg_interior(V, F)
+    pP, pG = W.feed_points("c2_6k")
+    pe = efie_mod.pick_port_edge_by_point_and_layer(r, pP, 1)
+    ge = efie_mod.pick_port_edge_by_point_and_layer(r, pG, -1)
+    spread = 1.5 * r.len.mean()
+    ctx.set_problem(V, F, r)
+    w = 2 * np.pi * 2.5e9
+    k = w * np.sqrt(W.MU0 * W.EPS0)
+    rows = np.linspace(1, r.Ne, 32).astype(np.int32)
+    P = orc.Problem(V, F, orc.build_rwg(V, F))
+    Zo = P.z_rows(rows, orc.wavenumber(w, W.MU0, W.EPS0), w, W.MU0, W.EPS0, orc.opts(**W.OPTS_C1),
+                  threads=min(os.cpu_count() or 1, 16))
+    for nm in (efie_mod.NEAR_DIRECT, efie_mod.NEAR_MOMENTS):
+        Z, _ = ctx.assemble_zv(k, w, W.MU0, W.EPS0, [pe, ge, 1.0, spread, 0.2],
+                               efie_mod.opts(**W.OPTS_C1, near_mode=nm))
+        assert _entry_rel(Z[rows - 1], Zo) <= ZTOL
+        del Z
+    fr = np.array([1.6e9, 2.3e9, 2.42e9, 2.5e9, 3.4e9])
+    z64 = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0, efie_mod.opts(**W.OPTS_C1))[0]
+    zm = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0,
+                   efie_mod.opts(**W.OPTS_C1, precision=1, near_mode=1, ir_tol=1e-10))[0]
+    assert np.max(np.abs(zm - z64) / np.abs(z64)) <= 1e-9
+
+
 def test_c4_family_small(efie_mod, orc, ctx):
     """Geometries of the batched uneven workload's family (rectangular
     cells, random layer gap and ground size), scaled down, against the
