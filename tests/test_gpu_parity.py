@@ -146,11 +146,16 @@ def test_c1_sweep_vs_golden(efie_mod, orc, ctx, prec, near_mode, batch):
         assert st["ir_resid_max"] <= 1e-11 and st["ir_iters_max"] <= 8
 
 
-def test_c1_centroid_sweep_vs_golden(efie_mod, orc, ctx):
+@pytest.mark.parametrize("prec", [0, 1])
+def test_c1_centroid_sweep_vs_golden(efie_mod, orc, ctx, prec):
+    """Centroid model, FP64 and mixed.  The centroid Z is symmetric only to
+    rounding, so the mixed path factorises its lower triangle (LDL^T) and
+    refines against the full matrix: Zin still reaches the FP64 result."""
     g = np.load(W_GOLDEN("c1.npz"))
     V, F, r, pe, ge, spread, _ = _setup(efie_mod, orc, "c1")
     ctx.set_problem(V, F, r)
-    zin, s11 = ctx.sweep(g["freqs"], W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0, efie_mod.opts("centroid"))
+    o = efie_mod.opts("centroid", precision=prec)
+    zin, s11 = ctx.sweep(g["freqs"], W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0, o)
     assert np.max(np.abs(zin - g["zin_centroid"]) / np.abs(g["zin_centroid"])) <= ZIN_TOL
     assert np.max(np.abs(s11 - g["s11_centroid"])) <= ZIN_TOL
 
