@@ -38,10 +38,6 @@ __device__ __forceinline__ double rsqrt64(double x) {
     return y;
 }
 
-// Green's function of one triangle pair: e^{-jkR} = e^{-jkR0} e^{-jk(R-R0)}
-// with R0 the centroid distance, so each point pair needs only cos/sin of
-// the small phase kr (R - R0) when its bound dmax (largest |R - R0| of the
-// pair) keeps kr dmax <= 0.75; otherwise the full reduction is used.
 // Green's function of one triangle pair, factored about the centroid
 // distance R0: G(R) = e0 * e^{-jk(R - R0)} / R with e0 = e^{-jk R0}/(4 pi).
 // The inner sums accumulate only e^{-jk(R - R0)}/R; the caller multiplies
@@ -70,28 +66,45 @@ __device__ __forceinline__ PairGreen pair_green(double kr, double ki, double R0,
     return g;
 }
 
-// e^{-jk(R - R0)} / R for R^2 = R2 (R2 > 0).
-__device__ __forceinline__ void green_rel(const PairGreen& g, double R2, double& gr, double& gi) {
+// e^{-jk(R - R0)} / R = af (c - j s) for R^2 = R2 (R2 > 0) with the phase
+// series MODE (any mode at or above the pair's own is as accurate) and
+// LOSSY = (ki != 0).
+template <int MODE, bool LOSSY>
+__device__ __forceinline__ void green_cs(const PairGreen& g, double R2, double& c, double& s, double& af) {
     const double invR = rsqrt64(R2);
     const double d = R2 * invR - g.R0;
-    if (g.mode <= 1) {
-        double s, c;
-        if (g.mode == 0) sincos_small_short(g.kr * d, &s, &c);
+    af = invR;
+    if constexpr (MODE <= 1) {
+        if constexpr (MODE == 0) sincos_small_short(g.kr * d, &s, &c);
         else sincos_small(g.kr * d, &s, &c);
-        // e^{ki d} = 1 + ki d + (ki d)^2/2 + ... for |ki d| <= 1e-3
-        double af = invR;
-        if (g.ki != 0.0) {
+        if constexpr (LOSSY) {
+            // e^{ki d} = 1 + ki d + (ki d)^2/2 + ... for |ki d| <= 1e-3
             const double z = g.ki * d;
             af *= 1.0 + z * (1.0 + z * (0.5 + z * (1.0 / 6.0 + z * (1.0 / 24.0))));
         }
-        gr = c * af;
-        gi = -s * af;
     } else {
-        double s, c;
         sincos_cw(g.kr * d, &s, &c);
-        const double af = (g.ki == 0.0) ? invR : exp(g.ki * d) * invR;
-        gr = c * af;
-        gi = -s * af;
+        if constexpr (LOSSY) af = exp(g.ki * d) * invR;
+    }
+}
+
+template <int MODE, bool LOSSY>
+__device__ __forceinline__ void green_t(const PairGreen& g, double R2, double& gr, double& gi) {
+    double c, s, af;
+    green_cs<MODE, LOSSY>(g, R2, c, s, af);
+    gr = c * af;
+    gi = -s * af;
+}
+
+__device__ __forceinline__ void green_rel(const PairGreen& g, double R2, double& gr, double& gi) {
+    if (g.ki != 0.0) {
+        if (g.mode == 0) green_t<0, true>(g, R2, gr, gi);
+        else if (g.mode == 1) green_t<1, true>(g, R2, gr, gi);
+        else green_t<2, true>(g, R2, gr, gi);
+    } else {
+        if (g.mode == 0) green_t<0, false>(g, R2, gr, gi);
+        else if (g.mode == 1) green_t<1, false>(g, R2, gr, gi);
+        else green_t<2, false>(g, R2, gr, gi);
     }
 }
 
@@ -221,9 +234,64 @@ __device__ __forceinline__ double wave_sum(double v) {
 // registers, the source points are wave-uniform loads.
 
 constexpr int kRegTile = 64;
+// blocks per 64 x 64 tile pair (each takes kRegTile / kRegSplit source
+// triangles): a finer grid shortens the tail of the last dispatch round
+constexpr int kRegSplit = 2;
 
+// The NQ x NQ product-rule sums of one regular pair.  MODE and LOSSY are
+// wave-uniform, so the loop body has no branches and the NQ source points
+// of one test point are evaluated side by side.
+template <int NQ, int MODE, bool LOSSY>
+__device__ __forceinline__ void regular_sums(const Pt* __restrict__ p1, const Pt* __restrict__ p2, double ox,
+                                             double oy, double oz, const PairGreen& pg, Acc& a) {
+    acc_zero(a);
+#pragma unroll 1
+    for (int i = 0; i < NQ; ++i) {
+        const Pt pi = p1[i];
+        const double xi = pi.x + ox, yi = pi.y + oy, zi = pi.z + oz;
+        double gr = 0, gi = 0, hxr = 0, hxi = 0, hyr = 0, hyi = 0, hzr = 0, hzi = 0;
+        auto point = [&](int j) {
+            const Pt q2 = p2[j];
+            const double dx = xi - q2.x, dy = yi - q2.y, dz = zi - q2.z;
+            // R < 1e-15 -> 1e-15 (R2 >= 0, so a max is the same clamp)
+            const double R2 = fmax(dx * dx + dy * dy + dz * dz, 1e-30);
+            double c, s, af;
+            green_cs<MODE, LOSSY>(pg, R2, c, s, af);
+            // W G = (W af) (c - j s); the minus sign goes into the sums
+            const double wa = q2.w * af;
+            const double wr = c * wa, ws = s * wa;
+            gr += wr; gi -= ws;
+            hxr += wr * q2.x; hxi -= ws * q2.x;
+            hyr += wr * q2.y; hyi -= ws * q2.y;
+            hzr += wr * q2.z; hzi -= ws * q2.z;
+        };
+        if constexpr (MODE == 2) {
+            // the long range reduction: rolled, so it does not set the
+            // register budget of the short-series loops
+#pragma unroll 1
+            for (int j = 0; j < NQ; ++j) point(j);
+        } else {
+#pragma unroll
+            for (int j = 0; j < NQ; ++j) point(j);
+        }
+        acc_fold(a, pi.w, pi.x, pi.y, pi.z, gr, gi, hxr, hxi, hyr, hyi, hzr, hzi);
+    }
+}
+
+template <int NQ, bool LOSSY>
+__device__ __forceinline__ void regular_sums_mode(int mode, const Pt* __restrict__ p1, const Pt* __restrict__ p2,
+                                                  double ox, double oy, double oz, const PairGreen& pg, Acc& a) {
+    if (mode == 0) regular_sums<NQ, 0, LOSSY>(p1, p2, ox, oy, oz, pg, a);
+    else if (mode == 1) regular_sums<NQ, 1, LOSSY>(p1, p2, ox, oy, oz, pg, a);
+    else regular_sums<NQ, 2, LOSSY>(p1, p2, ox, oy, oz, pg, a);
+}
+
+// One block per (64 x 64 tile pair, part): lane = test triangle t1 of tile
+// ti, each wave walks its share of the part's source triangles t2 of tile
+// tj, one pair per lane at a time.  The wave takes the largest phase-series
+// mode its active lanes need.
 template <int NQ>
-__global__ __launch_bounds__(256) void k_rec_regular(const Pt* __restrict__ pts,
+__global__ __launch_bounds__(256, 3) void k_rec_regular(const Pt* __restrict__ pts,
                                                      const TriGeo* __restrict__ tg,
                                                      const double* __restrict__ rad, int Nt,
                                                      const uint8_t* __restrict__ nearMat,
@@ -240,6 +308,7 @@ __global__ __launch_bounds__(256) void k_rec_regular(const Pt* __restrict__ pts,
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int t1 = ti * kRegTile + lane;
     const double kr = fr->kr, ki = fr->ki;
+    const bool lossy = ki != 0.0;
 
     // the block's 64 x NQ test points are staged in LDS (a register array
     // indexed by the rolled loop below would live in scratch)
@@ -251,41 +320,26 @@ __global__ __launch_bounds__(256) void k_rec_regular(const Pt* __restrict__ pts,
     __syncthreads();
     const int t1c = t1 < Nt ? t1 : Nt - 1;
     const TriGeo g1 = tg[t1c];
+    const double rad1 = rad[t1c];
     const Pt* p1 = sp + lane * NQ;
 
-    constexpr int perWave = kRegTile / 4;
+    constexpr int perWave = kRegTile / (4 * kRegSplit);
+    const int t2base = tj * kRegTile + (int)blockIdx.y * (kRegTile / kRegSplit) + wave * perWave;
     for (int q = 0; q < perWave; ++q) {
-        const int t2 = tj * kRegTile + wave * perWave + q;
+        const int t2 = t2base + q;
         if (t2 >= Nt) break;
-        bool active = (t1 < t2) && (nearMat[(size_t)t2 * Nt + t1] == 0);
-        if (!active) continue;
+        const bool active = (t1 < t2) && (nearMat[(size_t)t2 * Nt + t1] == 0);
         const TriGeo g2 = tg[t2];
         const Pt* p2 = pts + (size_t)t2 * NQ;
         const double ox = g1.cx - g2.cx, oy = g1.cy - g2.cy, oz = g1.cz - g2.cz;
-        const PairGreen pg = pair_green(kr, ki, sqrt(ox * ox + oy * oy + oz * oz), rad[t1] + rad[t2]);
+        const PairGreen pg = pair_green(kr, ki, sqrt(ox * ox + oy * oy + oz * oz), rad1 + rad[t2]);
+        const unsigned long long need2 = __ballot(active && pg.mode == 2);
+        const unsigned long long need1 = __ballot(active && pg.mode == 1);
+        const int mode = need2 ? 2 : (need1 ? 1 : 0);
+        if (!active) continue;
         Acc a;
-        acc_zero(a);
-#pragma unroll 1
-        for (int i = 0; i < NQ; ++i) {
-            const Pt pi = p1[i];
-            const double xi = pi.x + ox, yi = pi.y + oy, zi = pi.z + oz;
-            double gr = 0, gi = 0, hxr = 0, hxi = 0, hyr = 0, hyi = 0, hzr = 0, hzi = 0;
-#pragma unroll
-            for (int j = 0; j < NQ; ++j) {
-                const Pt q2 = p2[j];
-                const double dx = xi - q2.x, dy = yi - q2.y, dz = zi - q2.z;
-                double R2 = dx * dx + dy * dy + dz * dz;
-                R2 = R2 < 1e-30 ? 1e-30 : R2;  // R < 1e-15 -> 1e-15
-                double Gr, Gi;
-                green_rel(pg, R2, Gr, Gi);
-                const double wr = q2.w * Gr, wi = q2.w * Gi;
-                gr += wr; gi += wi;
-                hxr += wr * q2.x; hxi += wi * q2.x;
-                hyr += wr * q2.y; hyi += wi * q2.y;
-                hzr += wr * q2.z; hzi += wi * q2.z;
-            }
-            acc_fold(a, pi.w, pi.x, pi.y, pi.z, gr, gi, hxr, hxi, hyr, hyi, hzr, hzi);
-        }
+        if (lossy) regular_sums_mode<NQ, true>(mode, p1, p2, ox, oy, oz, pg, a);
+        else regular_sums_mode<NQ, false>(mode, p1, p2, ox, oy, oz, pg, a);
         acc_scale(a, pg.e0r, pg.e0i);
         acc_store(recs + rec_index(t1, t2), a);
     }
@@ -1086,7 +1140,7 @@ hipError_t launch_rec_regular(const Pt* regPts, int Nq, const TriGeo* tg, const 
                               const uint8_t* nearMat, const Freq* fr, Rec* recs, hipStream_t s) {
     long long T = (Nt + kRegTile - 1) / kRegTile;
     long long nb = T * (T + 1) / 2;
-    dim3 g((unsigned)nb), blk(256);
+    dim3 g((unsigned)nb, kRegSplit), blk(256);
     switch (Nq) {
         case 1: hipLaunchKernelGGL(k_rec_regular<1>, g, blk, 0, s, regPts, tg, rad, Nt, nearMat, fr, recs); break;
         case 3: hipLaunchKernelGGL(k_rec_regular<3>, g, blk, 0, s, regPts, tg, rad, Nt, nearMat, fr, recs); break;
