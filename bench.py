@@ -93,7 +93,10 @@ def lu_gemm_flops_sym(n, tile=64):
     """Flops of the matrix-core updates of the unpivoted complex-symmetric
     factorisation of size n (no rhs column), on the 64 x 64 tiles the kernel
     computes: every update skips the tiles strictly above the diagonal
-    (tile rows rb..rb+63 all above tile columns cb.., i.e. rb + 64 <= cb)."""
+    (tile rows rb..rb+63 all above tile columns cb.., i.e. rb + 64 <= cb).
+    Inside an outer block the columns are taken by halves split at a
+    multiple of NB: left half, the right half's update with K = the left
+    half's width, right half."""
     f = 0.0
 
     def g_lower(r0, r1, c0, c1, K):
@@ -103,11 +106,15 @@ def lu_gemm_flops_sym(n, tile=64):
                 if rb + tile > cb:
                     t += 8.0 * (min(r1, rb + tile) - rb) * (min(c1, cb + tile) - cb) * K
         return t
+
+    def halves(c0, c1):
+        if c1 - c0 <= NB:
+            return 0.0
+        mid = c0 + (c1 - c0 + 2 * NB - 1) // (2 * NB) * NB
+        return halves(c0, mid) + g_lower(mid, n, mid, c1, mid - c0) + halves(mid, c1)
     for j0 in range(0, n, WB):
         jend = min(n, j0 + WB)
-        for c0 in range(j0, jend, NB):
-            w = min(NB, jend - c0)
-            f += g_lower(c0 + w, n, c0 + w, jend, w)     # update inside the block
+        f += halves(j0, jend)                            # updates inside the block
         if jend >= n:
             break
         jnext = n if n - jend < WB else jend + WB

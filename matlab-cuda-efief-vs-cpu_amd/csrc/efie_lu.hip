@@ -1340,12 +1340,51 @@ static int panel_cols(T* A, size_t strideA, int lda, int batch, int d, int w, in
     return 1;
 }
 
+// Columns [c0, c1) of an outer block of the symmetric factorisation, all
+// updates from columns left of c0 applied: halves split at a multiple of
+// NB, the left half factored, then U = D L^T of its rows for the right
+// half's columns and the right half's update with K = the left half's
+// width, then the right half.  Against NB-wide steps each followed by a
+// K = NB update of the rest of the block, the flops are the same (with 64
+// x 64 update tiles) and the update passes over the block's columns halve.
+template <typename T>
+static void sym_block(T* A, size_t strideA, int n, int lda, int batch, const LuWork& wk, int c0, int c1,
+                      hipStream_t sp, const Prof* pf) {
+    if (c1 - c0 <= NB) {
+        const int w = c1 - c0;
+        pf->begin(7);
+        hipLaunchKernelGGL((k_panel_diag<T, true>), dim3(batch), dim3(64), 0, sp, A, strideA, lda, c0, w, wk.info);
+        int np = 1;
+        if (c1 < n) {
+            hipLaunchKernelGGL(k_panel_rows<T>, dim3((n - c1 + 255) / 256, batch), dim3(256), 0, sp, A, strideA, lda,
+                               c0, w, c1, n);
+            ++np;
+        }
+        pf->end(7, np);
+        return;
+    }
+    const int mid = c0 + (c1 - c0 + 2 * NB - 1) / (2 * NB) * NB;
+    sym_block<T>(A, strideA, n, lda, batch, wk, c0, mid, sp, pf);
+    pf->begin(7);
+    hipLaunchKernelGGL(k_sym_u12<T>, dim3((c1 - mid + 63) / 64, (mid - c0 + 31) / 32, batch), dim3(256), 0, sp, A,
+                       strideA, lda, c0, mid, mid, c1);
+    pf->end(7);
+    pf->begin(8);
+    pf->end(8, gemm<T>(A, strideA, lda, batch, mid, n, mid, c1, c0, mid - c0, sp, 1));
+    sym_block<T>(A, strideA, n, lda, batch, wk, mid, c1, sp, pf);
+}
+
 // Panels of the outer block [j0, jend): for every NB-wide sub-panel the
 // tournament, its interchanges on the block's own columns, the diagonal
 // block, L21, the block-local U12 and the update of the rest of the block.
+// The symmetric factorisation takes the block by halves (sym_block).
 template <typename T>
 static void panel_phase(T* A, size_t strideA, int n, int lda, int batch, const LuWork& wk, int j0, int jend,
                         int nodeStride, hipStream_t sp, const Prof* pf) {
+    if (wk.noPivot) {
+        sym_block<T>(A, strideA, n, lda, batch, wk, j0, jend, sp, pf);
+        return;
+    }
     for (int c0 = j0; c0 < jend; c0 += NB) {
         const int w = jend - c0 < NB ? jend - c0 : NB;
         const int slot = (c0 - j0) / NB;
