@@ -50,6 +50,20 @@ __device__ __forceinline__ cpx<T> ld(const T* p) {
         return {v.x, v.y};
     }
 }
+// two consecutive complex values from a 16-byte-aligned address, as one
+// 16-byte load (float) or two (double)
+template <typename T>
+__device__ __forceinline__ void ld_pair(const T* p, cpx<T>& a, cpx<T>& b) {
+    if constexpr (std::is_same<T, double>::value) {
+        const double2 u = reinterpret_cast<const double2*>(p)[0], v = reinterpret_cast<const double2*>(p)[1];
+        a = {u.x, u.y};
+        b = {v.x, v.y};
+    } else {
+        const float4 u = *reinterpret_cast<const float4*>(p);
+        a = {u.x, u.y};
+        b = {u.z, u.w};
+    }
+}
 template <typename T>
 __device__ __forceinline__ void st(T* p, cpx<T> v) {
     if constexpr (std::is_same<T, double>::value)
@@ -1087,6 +1101,31 @@ __global__ __launch_bounds__(256, 2) void k_gemm_p6(float* A, size_t strideA, in
 constexpr long long kSpinLimit = 1LL << 28;
 constexpr int kTrsvRun = 8;
 
+// x from the lane the DPP control CTRL names within the 16-lane row
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double x) {
+    const long long b = __double_as_longlong(x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, true);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+// sum over the 16 lanes of a row, the same bits in every lane: swap
+// neighbours (quad_perm 1,0,3,2), pairs (quad_perm 2,3,0,1), mirror the
+// half rows, mirror the row; each step adds two equal partial sums in
+// either order
+template <typename T>
+__device__ __forceinline__ T row16_sum(T v) {
+    v += dpp_mov<0xB1>(v);
+    v += dpp_mov<0x4E>(v);
+    v += dpp_mov<0x141>(v);
+    v += dpp_mov<0x140>(v);
+    return v;
+}
+
 // element j of a wave-held vector, read from lane j (j wave-uniform)
 __device__ __forceinline__ float lane_read(float v, int j) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
@@ -1105,7 +1144,7 @@ __global__ __launch_bounds__(256) void k_trsv_fused(const T* A, size_t strideA, 
     __shared__ cpx<T> u[64][65];
     __shared__ cpx<T> xb[64];
     __shared__ cpx<T> xc[kTrsvRun][64];
-    __shared__ cpx<T> part[4][64];
+    __shared__ cpx<T> part[2][64];
     __shared__ int s_vid, s_avail;
     const int b = blockIdx.x;
     if (skip && skip[b]) return;
@@ -1136,21 +1175,37 @@ __global__ __launch_bounds__(256) void k_trsv_fused(const T* A, size_t strideA, 
         }
     }
     for (int i = tid; i < bs; i += 256) xb[i] = ld(Xb + 2 * (size_t)(r0 + i) * xs);
-    // thread (row i, quarter q) accumulates sum_j M[r0+i][c0+j] x_c[j], j in
-    // quarter q, over the earlier blocks c
-    const int i = tid >> 2, q = tid & 3;
-    const int ic = i < bs ? i : bs - 1;
-    const T* mrow = Ab + 2 * (size_t)(r0 + ic) * lda;
-    cpx<T> acc = {0, 0};
+    // thread (wave w, half h, pair p) accumulates, for its 8 rows
+    // 16w + 2k + h, the products of columns 2p and 2p+1 of every earlier
+    // block column with x; one load instruction of a wave then covers two
+    // contiguous 64-column row segments.  The 32 lanes of a half are summed
+    // once, after the last block column.
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6), h = (tid >> 5) & 1, p = tid & 31;
+    cpx<T> acc[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc[k] = {0, 0};
     auto blk = [&](int st) { return LOWER ? st : nblk - 1 - st; };
-    // this thread's 16 values of row ic in block column blk(st)
+    // columns 2p, 2p+1 of this thread's rows in block column blk(st); the
+    // pair start is clamped into the block's columns (the row has lda > n
+    // entries, so the pair's second column stays in the row).  Columns past
+    // the block are zeroed where the values are used, so no select follows
+    // the loads (that would make the prefetch wait for them).
     auto load_m = [&](int st, cpx<T>* m) {
         const int c0 = blk(st) * 64;
         const int cs = n - c0 < 64 ? n - c0 : 64;
+        const int jp = 2 * p < cs ? 2 * p : ((cs - 1) & ~1);
+        if (bs == 64) {
+            // full block: wave-uniform row bases, one lane offset
+            const int off = 2 * (h * lda + jp);
 #pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            const int j = q * 16 + k;
-            m[k] = ld(mrow + 2 * (c0 + (j < cs ? j : cs - 1)));
+            for (int k = 0; k < 8; ++k)
+                ld_pair(Ab + 2 * ((size_t)(r0 + 16 * w + 2 * k) * lda + c0) + off, m[2 * k], m[2 * k + 1]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int rr = 16 * w + 2 * k + h;
+                ld_pair(Ab + 2 * ((size_t)(r0 + (rr < bs ? rr : bs - 1)) * lda + c0 + jp), m[2 * k], m[2 * k + 1]);
+            }
         }
     };
     cpx<T> m[16], mn[16];
@@ -1183,17 +1238,22 @@ __global__ __launch_bounds__(256) void k_trsv_fused(const T* A, size_t strideA, 
         }
         __syncthreads();
         for (int st = step; st < av; ++st) {
-            const int c0 = blk(st) * 64;
-            const int cs = n - c0 < 64 ? n - c0 : 64;
             if (st + 1 < av) load_m(st + 1, mn);  // next block's values before this one's products
-            const cpx<T>* xr = xc[st - step];
+            const cpx<T> x0 = xc[st - step][2 * p], x1 = xc[st - step][2 * p + 1];
+            const int cs = n - blk(st) * 64;
+            if (cs < 64) {
+                // last block column: whatever the padding columns hold
 #pragma unroll
-            for (int k = 0; k < 16; ++k) {
-                const int j = q * 16 + k;
-                if (j < cs) {
-                    acc.r += m[k].r * xr[j].r - m[k].i * xr[j].i;
-                    acc.i += m[k].r * xr[j].i + m[k].i * xr[j].r;
+                for (int k = 0; k < 8; ++k) {
+                    if (2 * p >= cs) m[2 * k] = {0, 0};
+                    if (2 * p + 1 >= cs) m[2 * k + 1] = {0, 0};
                 }
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const cpx<T> a = m[2 * k], c = m[2 * k + 1];
+                acc[k].r += a.r * x0.r - a.i * x0.i + c.r * x1.r - c.i * x1.i;
+                acc[k].i += a.r * x0.i + a.i * x0.r + c.r * x1.i + c.i * x1.r;
             }
             if (st + 1 < av) {
 #pragma unroll
@@ -1203,7 +1263,17 @@ __global__ __launch_bounds__(256) void k_trsv_fused(const T* A, size_t strideA, 
         __syncthreads();  // xc is refilled by the next run
         step = av;
     }
-    part[q][i] = acc;
+    // each 16-lane row of pairs sums its partials in registers; the two
+    // rows of a half meet in part
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        acc[k].r = row16_sum(acc[k].r);
+        acc[k].i = row16_sum(acc[k].i);
+    }
+    if ((p & 15) == 0) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) part[p >> 4][16 * w + 2 * k + h] = acc[k];
+    }
     __syncthreads();
     // diagonal solve by wave 0 with x in registers (lane k holds x_k);
     // x_j reaches every lane by readlane, so the steps need no LDS round
@@ -1212,10 +1282,8 @@ __global__ __launch_bounds__(256) void k_trsv_fused(const T* A, size_t strideA, 
         cpx<T> v = {0, 0};
         if (tid < bs) {
             v = xb[tid];
-            for (int k = 0; k < 4; ++k) {
-                v.r -= part[k][tid].r;
-                v.i -= part[k][tid].i;
-            }
+            v.r -= part[0][tid].r + part[1][tid].r;
+            v.i -= part[0][tid].i + part[1][tid].i;
         }
         for (int jj = 0; jj < bs; ++jj) {
             const int j = LOWER ? jj : bs - 1 - jj;
@@ -1569,7 +1637,9 @@ template <typename T>
 hipError_t lu_trsv(const T* A, size_t strideA, int n, int lda, T* X, size_t strideX, size_t xs,
                    int batch, bool lower, const int* skip, hipStream_t s, TrsvWork* tw) {
     const int nblk = (n + 63) / 64;
-    if (!tw) return hipErrorInvalidValue;
+    // the kernel loads column pairs with 16-byte loads (lda even) and may
+    // touch column n of a row (lda > n)
+    if (!tw || (lda & 1) || lda <= n) return hipErrorInvalidValue;
     ++tw->epoch;
     hipLaunchKernelGGL(k_zero_int, dim3((batch + 255) / 256), dim3(256), 0, s, tw->tickets, batch);
     if (lower)
