@@ -842,77 +842,42 @@ __global__ __launch_bounds__(256) void k_poly_records(const double* __restrict__
 // Z is complex symmetric (Z(m,n) = Z(n,m): the four side pairs of (n,m)
 // are those of (m,n) with the roles swapped), so one block computes a
 // 16 x 16 tile (I, J) with I <= J and writes it twice, the mirror through
-// an LDS transpose so both stores are row-contiguous (on a diagonal tile
-// the entries with m <= n, mirrored to the others).  Thread (r, c) owns
-// entry (16I + r, 16J + c).  All table and record loads are unconditional
-// (indices clamped, missing triangles masked afterwards) so they are in
-// flight together.  With Z32 set, the complex64 copy of the mixed-precision
-// solve is written in the same pass.
-__device__ __forceinline__ void combine_entry(const Rec* __restrict__ recs, const int* __restrict__ sideTri,
-                                              const Side* __restrict__ side, int m, int n, const Freq& f,
-                                              double& zr, double& zi) {
-    int tm[2], tn[2];
-    Side sm[2], sn[2];
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        tm[k] = sideTri[2 * m + k];
-        tn[k] = sideTri[2 * n + k];
-        sm[k] = side[2 * m + k];
-        sn[k] = side[2 * n + k];
-    }
-    double2 R[2][2][8];
-#pragma unroll
-    for (int sa = 0; sa < 2; ++sa)
-#pragma unroll
-        for (int sb = 0; sb < 2; ++sb) {
-            const int t1 = tm[sa] < 0 ? 0 : tm[sa], t2 = tn[sb] < 0 ? 0 : tn[sb];
-            const double2* rp = reinterpret_cast<const double2*>(
-                recs[t1 > t2 ? rec_index(t2, t1) : rec_index(t1, t2)].v);
-#pragma unroll
-            for (int q = 0; q < 8; ++q) R[sa][sb][q] = rp[q];
-        }
-    zr = 0.0;
-    zi = 0.0;
-#pragma unroll
-    for (int sa = 0; sa < 2; ++sa)
-#pragma unroll
-        for (int sb = 0; sb < 2; ++sb) {
-            const bool sw = tm[sa] > tn[sb];
-            const double2 s0 = R[sa][sb][0], s2 = R[sa][sb][7];
-            double2 s1x = R[sa][sb][1], s1y = R[sa][sb][2], s1z = R[sa][sb][3];
-            double2 s1px = R[sa][sb][4], s1py = R[sa][sb][5], s1pz = R[sa][sb][6];
-            if (sw) {
-                double2 t;
-                t = s1x; s1x = s1px; s1px = t;
-                t = s1y; s1y = s1py; s1py = t;
-                t = s1z; s1z = s1pz; s1pz = t;
-            }
-            const Side a = sm[sa], c = sn[sb];
-            const double uu = a.ux * c.ux + a.uy * c.uy + a.uz * c.uz;
-            // S2 - S1.un - S1'.um + S0 (um.un)
-            double ivr = s2.x - (s1x.x * c.ux + s1y.x * c.uy + s1z.x * c.uz) -
-                         (s1px.x * a.ux + s1py.x * a.uy + s1pz.x * a.uz) + s0.x * uu;
-            double ivi = s2.y - (s1x.y * c.ux + s1y.y * c.uy + s1z.y * c.uz) -
-                         (s1px.y * a.ux + s1py.y * a.uy + s1pz.y * a.uz) + s0.y * uu;
-            const double cc = a.c * c.c;
-            ivr *= cc;
-            ivi *= cc;
-            const double isr = 4.0 * cc * s0.x, isi = 4.0 * cc * s0.y;
-            // Zmn + 1j*omega*mu*Ivec + (1/(1j*omega*eps))*Isca; a missing
-            // triangle (index -1) contributes nothing
-            const bool ok = tm[sa] >= 0 && tn[sb] >= 0;
-            const double dr = (f.ar * ivr - f.ai * ivi) + (f.br * isr - f.bi * isi);
-            const double di = (f.ar * ivi + f.ai * ivr) + (f.br * isi + f.bi * isr);
-            zr += ok ? dr : 0.0;
-            zi += ok ? di : 0.0;
-        }
+// LDS so both stores are row-contiguous (on a diagonal tile the entries
+// with m <= n, mirrored to the others).  With Z32 set, the complex64 copy
+// of the mixed-precision solve is written in the same pass.
+//
+// Eight lanes per entry.  Lane q of a group holds moment q of each of the
+// entry's four records, so a group reads whole 128-byte records and a wave
+// load instruction touches 8 cache lines (with a lane per entry: 64).
+// Every moment enters the entry with a real coefficient (the side vectors,
+// their dot product, 1, and 4 for the scalar term), so each lane sums its
+// weighted moment over the four side pairs and the group adds its 8 lanes
+// in registers (DPP).  The block takes its tile in 8 passes of 32 entries,
+// the tile's side tables staged in LDS.  Record loads are unconditional
+// (triangle indices clamped, missing triangles weighted 0).
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov_d(double x) {
+    const long long b = __double_as_longlong(x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, true);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+// sum over the 8 lanes of an aligned group, the same bits in every lane
+// (swap neighbours, swap pairs, mirror the group)
+__device__ __forceinline__ double oct_sum(double v) {
+    v += dpp_mov_d<0xB1>(v);
+    v += dpp_mov_d<0x4E>(v);
+    v += dpp_mov_d<0x141>(v);
+    return v;
 }
 
 __global__ __launch_bounds__(256) void k_combine(const Rec* __restrict__ recs, const int* __restrict__ sideTri,
-                                                 const Side* __restrict__ side, int Ne,
-                                                 const Freq* __restrict__ fr, double* __restrict__ Z, size_t ld,
-                                                 float* __restrict__ Z32, size_t ld32, int T, int nblk) {
+                                                  const Side* __restrict__ side, int Ne,
+                                                  const Freq* __restrict__ fr, double* __restrict__ Z, size_t ld,
+                                                  float* __restrict__ Z32, size_t ld32, int nblk) {
     __shared__ double2 tile[16][17];
+    __shared__ int sT[2][16][2];
+    __shared__ Side sS[2][16][2];
     // XCD-aware order (each XCD takes a contiguous run of tiles), then the
     // linear tile index -> (I <= J)
     long long id = blockIdx.x;
@@ -924,22 +889,74 @@ __global__ __launch_bounds__(256) void k_combine(const Rec* __restrict__ recs, c
     while ((long long)(J + 1) * (J + 2) / 2 <= id) ++J;
     while ((long long)J * (J + 1) / 2 > id) --J;
     const int I = (int)(id - (long long)J * (J + 1) / 2);
-    (void)T;
+    if (threadIdx.x < 64) {
+        // [0]: the tile's 16 rows (edges 16I..), [1]: its 16 columns
+        const int which = threadIdx.x >> 5, e = (threadIdx.x & 31) >> 1, sd = threadIdx.x & 1;
+        const int edge = 16 * (which ? J : I) + e;
+        const int ec = edge < Ne ? edge : Ne - 1;
+        sT[which][e][sd] = sideTri[2 * ec + sd];
+        sS[which][e][sd] = side[2 * ec + sd];
+    }
+    const Freq f = *fr;
+    __syncthreads();
+    const int g = threadIdx.x >> 3, q = threadIdx.x & 7;
+    const int d = (q - 1) % 3;  // component of the side vector for q = 1..6
+#pragma unroll 2
+    for (int ps = 0; ps < 8; ++ps) {
+        const int e = ps * 32 + g, r = e >> 4, c = e & 15;
+        int tm[2], tn[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            tm[k] = sT[0][r][k];
+            tn[k] = sT[1][c][k];
+        }
+        double2 R[2][2];
+#pragma unroll
+        for (int sa = 0; sa < 2; ++sa)
+#pragma unroll
+            for (int sb = 0; sb < 2; ++sb) {
+                const int t1 = tm[sa] < 0 ? 0 : tm[sa], t2 = tn[sb] < 0 ? 0 : tn[sb];
+                R[sa][sb] = reinterpret_cast<const double2*>(
+                    recs[t1 > t2 ? rec_index(t2, t1) : rec_index(t1, t2)].v)[q];
+            }
+        double vr = 0.0, vi = 0.0, sr = 0.0, si = 0.0;
+#pragma unroll
+        for (int sa = 0; sa < 2; ++sa)
+#pragma unroll
+            for (int sb = 0; sb < 2; ++sb) {
+                const Side a = sS[0][r][sa], b = sS[1][c][sb];
+                // S2 - S1.un - S1'.um + S0 (um.un), S1 and S1' exchanged
+                // when the record is stored the other way round
+                const bool sw = tm[sa] > tn[sb];
+                const double ua = d == 0 ? a.ux : (d == 1 ? a.uy : a.uz);
+                const double ub = d == 0 ? b.ux : (d == 1 ? b.uy : b.uz);
+                double coef;
+                if (q == 0) coef = a.ux * b.ux + a.uy * b.uy + a.uz * b.uz;
+                else if (q < 4) coef = -(sw ? ua : ub);
+                else if (q < 7) coef = -(sw ? ub : ua);
+                else coef = 1.0;
+                // a missing triangle (index -1) contributes nothing
+                const double w = (tm[sa] >= 0 && tn[sb] >= 0) ? a.c * b.c : 0.0;
+                const double wc = w * coef;
+                vr += wc * R[sa][sb].x;
+                vi += wc * R[sa][sb].y;
+                sr += 4.0 * w * R[sa][sb].x;
+                si += 4.0 * w * R[sa][sb].y;
+            }
+        vr = oct_sum(vr);
+        vi = oct_sum(vi);
+        if (q == 0)
+            tile[r][c] = make_double2((f.ar * vr - f.ai * vi) + (f.br * sr - f.bi * si),
+                                      (f.ar * vi + f.ai * vr) + (f.br * si + f.bi * sr));
+    }
+    __syncthreads();
     const int r = threadIdx.x >> 4, c = threadIdx.x & 15;
     const int m = 16 * I + r, n = 16 * J + c;
-    const Freq f = *fr;
-    double zr, zi;
-    combine_entry(recs, sideTri, side, m < Ne ? m : Ne - 1, n < Ne ? n : Ne - 1, f, zr, zi);
-    // the upper triangle (m <= n) is computed; on a diagonal tile the lower
-    // half comes from the mirror as on every other tile, so Z is exactly
-    // symmetric whatever discretisation the records carry
     if (m < Ne && n < Ne && (I != J || r <= c)) {
-        *reinterpret_cast<double2*>(Z + 2 * ((size_t)m * ld + n)) = make_double2(zr, zi);
-        if (Z32) *reinterpret_cast<float2*>(Z32 + 2 * ((size_t)m * ld32 + n)) = make_float2((float)zr, (float)zi);
+        const double2 v = tile[r][c];
+        *reinterpret_cast<double2*>(Z + 2 * ((size_t)m * ld + n)) = v;
+        if (Z32) *reinterpret_cast<float2*>(Z32 + 2 * ((size_t)m * ld32 + n)) = make_float2((float)v.x, (float)v.y);
     }
-    tile[r][c] = make_double2(zr, zi);
-    __syncthreads();
-    // mirror: thread (r, c) writes entry (16J + r, 16I + c) = tile[c][r]
     const int mm = 16 * J + r, nn = 16 * I + c;
     if (mm < Ne && nn < Ne && (I != J || r > c)) {
         const double2 v = tile[c][r];
@@ -1218,7 +1235,7 @@ hipError_t launch_combine(const Rec* recs, const int* sideTri, const Side* side,
     const int T = (Ne + 15) / 16;
     const long long nb = (long long)T * (T + 1) / 2;
     hipLaunchKernelGGL(k_combine, dim3((unsigned)nb), dim3(256), 0, s, recs, sideTri, side, Ne, fr, Z, ld, Z32,
-                       ld32, T, (int)nb);
+                       ld32, (int)nb);
     return hipGetLastError();
 }
 
