@@ -160,6 +160,36 @@ def test_c1_centroid_sweep_vs_golden(efie_mod, orc, ctx, prec):
     assert np.max(np.abs(s11 - g["s11_centroid"])) <= ZIN_TOL
 
 
+@pytest.mark.parametrize("cells", [(1, 1, 1, 1), (1, 2, 1, 1), (2, 1, 2, 2), (2, 2, 2, 2), (2, 2, 3, 2)])
+def test_small_meshes(efie_mod, orc, ctx, cells):
+    """Ne = 2, 4, 11, 16 and 21: Z of one or two partial 16 x 16 tiles, fewer
+    source triangles than one regular-fill block takes, a symmetric
+    factorisation narrower than one 32-column panel.  Full Z in both near
+    modes against the oracle (and exactly symmetric), the mixed sweep
+    against the FP64 sweep."""
+    V, F = efie_mod.make_patch_with_ground_mesh(*W.GEOM, *cells)
+    r = efie_mod.build_rwg_interior(V, F)
+    pP, pG = W.feed_points("micro")
+    pe = efie_mod.pick_port_edge_by_point_and_layer(r, pP, 1)
+    ge = efie_mod.pick_port_edge_by_point_and_layer(r, pG, -1)
+    P = orc.Problem(V, F, orc.build_rwg(V, F))
+    spread = 1.5 * r.len.mean()
+    ctx.set_problem(V, F, r)
+    w = 2 * np.pi * 2.5e9
+    k = orc.wavenumber(w, W.MU0, W.EPS0)
+    Zo = P.z_full(k, w, W.MU0, W.EPS0, orc.opts(**W.OPTS_C1))
+    for nm in (0, 1):
+        Zg, _ = ctx.assemble_zv(k, w, W.MU0, W.EPS0, [pe, ge, 1.0, spread, 0.2],
+                                efie_mod.opts(**W.OPTS_C1, near_mode=nm))
+        assert _entry_rel(Zg, Zo) <= ZTOL
+        assert np.array_equal(Zg, Zg.T)
+    fr = np.linspace(1.5e9, 3.5e9, 7)
+    z64 = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0, efie_mod.opts(**W.OPTS_C1, precision=0))[0]
+    zmx = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0,
+                    efie_mod.opts(**W.OPTS_C1, precision=1, near_mode=1))[0]
+    assert np.max(np.abs(zmx - z64) / np.abs(z64)) <= ZIN_TOL
+
+
 def test_lossy_sweep_tiny_vs_oracle(efie_mod, orc, ctx):
     V, F, r, pe, ge, spread, P = _setup(efie_mod, orc, "tiny")
     ctx.set_problem(V, F, r)
