@@ -1347,10 +1347,12 @@ static void gemm_launch(T* A, size_t strideA, int lda, int batch, int r0, int r1
                        K, gx, gy, nblk, skipUpper);
 }
 
-// 64 x 64 tiles: four blocks per CU keep 16 waves resident, which hides
-// the operand and C latency better than larger tiles at lower occupancy
-// (measured at n = 6009, K = 128: 112 TFLOP/s FP32 against 96 for
-// 128 x 128).  The complex product uses three real products (3M:
+// 64 x 64 tiles: three blocks per CU (FP32: 84 VGPRs + 48 accumulation
+// registers) keep 12 waves resident, which hides the operand and C latency
+// better than larger tiles at lower occupancy (measured at n = 6009,
+// K = 128: 112 TFLOP/s FP32 against 96 for 128 x 128).  A fourth block
+// (the registers capped at 128) gained about 1 % on the update and
+// nothing on the sweep.  The complex product uses three real products (3M:
 // ar br, ai bi, (ar+ai)(br+bi)); with K = 256 the FP32 update runs at
 // 145 TFLOP/s counted as 8 M N K.
 // Returns the number of kernels launched (0 for an empty update).
