@@ -940,11 +940,13 @@ __global__ __launch_bounds__(256) void k_combine(const Rec* __restrict__ recs, c
                 const double wc = w * coef;
                 vr += wc * R[sa][sb].x;
                 vi += wc * R[sa][sb].y;
-                sr += 4.0 * w * R[sa][sb].x;
-                si += 4.0 * w * R[sa][sb].y;
+                sr += w * R[sa][sb].x;  // the scalar term's factor 4 is applied
+                si += w * R[sa][sb].y;  // once below (exact: a power of two)
             }
         vr = oct_sum(vr);
         vi = oct_sum(vi);
+        sr *= 4.0;
+        si *= 4.0;
         if (q == 0)
             tile[r][c] = make_double2((f.ar * vr - f.ai * vi) + (f.br * sr - f.bi * si),
                                       (f.ar * vi + f.ai * vr) + (f.br * si + f.bi * sr));
