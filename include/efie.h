@@ -75,7 +75,9 @@ typedef struct {
  * interchanges disappear; the FP64 refinement corrects its larger rounding
  * growth.  A frequency that does not reach max(ir_tol, 1e-11) (a zero or
  * tiny pivot) is solved again with the pivoted FP32 factorisation
- * (efie_stats.n_lu_retry).  Results have the accuracy of the pivoted path.
+ * (efie_stats.n_lu_retry), and one the pivoted FP32 forms do not refine to
+ * that bound with the FP64 LU (efie_stats.n_fp64_retry), so every result of
+ * a mixed sweep is refined to max(ir_tol, 1e-11) or comes from the FP64 LU.
  * AUTO picks SYM in efie_sweep (Z is complex symmetric) and FP32 in
  * efie_zgesv (general matrices). */
 enum { EFIE_LU_AUTO = 0, EFIE_LU_FP32 = 1, EFIE_LU_BF16X6 = 2, EFIE_LU_SYM = 3 };
@@ -136,6 +138,11 @@ typedef struct {
      * unpivoted factorisation met a zero pivot or did not refine to
      * max(ir_tol, 1e-11)                                                  */
     int n_lu_retry;
+    /* mixed precision: systems solved again by the FP64 LU because the
+     * pivoted FP32 factorisation met a zero pivot or its refinement did not
+     * reach max(ir_tol, 1e-11) (stalled or at ir_max_iter); their results
+     * have the FP64 path's accuracy                                        */
+    int n_fp64_retry;
 } efie_stats;
 
 /* "accurate" preset (main_patch_efie_rwg.m:281-289), FP64, quadrature. */
@@ -268,7 +275,10 @@ efie_status efie_assemble_z(efie_ctx* ctx, double k_re, double k_im, double omeg
  * cusolverDnZgetrf + cusolverDnZgetrs pair of the reference
  * (patch_sweep_cuda.cu:473-480).  precision FP64: blocked LU with
  * tournament pivoting on the matrix cores; MIXED: complex FP32 LU plus FP64
- * iterative refinement (ir_tol 1e-14, at most 10 corrections). */
+ * iterative refinement (ir_tol 1e-14, at most 10 corrections); a system
+ * whose FP32 factorisation meets a zero pivot or whose refinement stalls
+ * above 1e-11 is solved again by the FP64 LU (efie_stats.n_fp64_retry), and
+ * only a zero pivot there returns EFIE_E_SINGULAR. */
 efie_status efie_zgesv(efie_ctx* ctx, int n, int batch, const double* A,
                        const double* b, int precision, double* x,
                        efie_stats* stats);

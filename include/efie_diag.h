@@ -17,6 +17,12 @@ extern "C" {
  * with one streaming kernel per copy; *ms = average kernel time. */
 efie_status efie_diag_copy(efie_ctx* ctx, size_t bytes, int reps, double* ms);
 
+/* Process-wide counts of frequency-independent geometry builds (quadrature
+ * points, near lists) and near/self moment integrations, over all
+ * contexts: a caller that sets the same tables again (one MEX call after
+ * another) sees neither count move. */
+efie_status efie_diag_cache_counts(long long* geometry_builds, long long* moment_builds);
+
 #ifdef __cplusplus
 }
 #endif

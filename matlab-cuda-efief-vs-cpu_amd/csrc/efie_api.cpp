@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdarg>
 #include <cmath>
 #include <complex>
@@ -289,17 +290,19 @@ struct efie_ctx {
 };
 
 namespace {
-// Back to the rule-derived quadrature tables.
-void drop_pack(efie_ctx* c) {
-    if (c->havePack) c->geoValid = false;
-    c->havePack = false;
-    c->pkReg.clear();
-    c->pkNear.clear();
-    c->pkSelf.clear();
-    c->pkASelf.clear();
-    c->pkAeq.clear();
-    c->pkNearMat.clear();
+// Back to the rule-derived quadrature tables.  The converted pack stays
+// stored, so that installing the same pack again (efie_set_prepack) finds
+// the geometry built from it still valid; the geometry key (pack install
+// number or rule options) decides which tables a cached geometry belongs to.
+void drop_pack(efie_ctx* c) { c->havePack = false; }
+
+template <typename T>
+bool same_table(const std::vector<T>& have, const T* in, size_t n) {
+    return have.size() == n && (n == 0 || std::memcmp(have.data(), in, sizeof(T) * n) == 0);
 }
+
+// process-wide counts of geometry and moment builds (efie_diag_cache_counts)
+std::atomic<long long> g_geoBuilds{0}, g_momBuilds{0};
 }  // namespace
 
 extern "C" {
@@ -556,6 +559,15 @@ efie_status efie_set_mesh(efie_ctx* c, const double* V, int Nv, const int* F, in
     if (Nv < 3 || Nt < 1) return fail(EFIE_E_ARGS, "mex:V", "V must be [Nv x 3] double.");
     for (long long i = 0; i < 3LL * Nt; ++i)
         if (F[i] < 1 || F[i] > Nv) return fail(EFIE_E_MESH, "mex:F", "F has invalid indices.");
+    // the same mesh again (a MEX call per frequency or per sweep): the
+    // derived geometry, moments and centroid tables stay valid; the RWG
+    // tables must still follow (and are compared in turn)
+    if (c->haveMesh && !c->meshless && same_table(c->V, V, 3 * (size_t)Nv) && same_table(c->F, F, 3 * (size_t)Nt) &&
+        Nv == c->Nv && Nt == c->Nt) {
+        c->haveRwg = false;
+        drop_pack(c);
+        return ok();
+    }
     c->Nv = Nv;
     c->Nt = Nt;
     c->V.assign(V, V + 3 * (size_t)Nv);
@@ -587,6 +599,15 @@ efie_status efie_set_rwg(efie_ctx* c, const int* edge, const int* plusTri, const
         if (edge[e] < 1 || edge[e] > c->Nv || edge[e + Ne] < 1 || edge[e + Ne] > c->Nv)
             return fail(EFIE_E_MESH, "mex:edge", "edge has invalid vertex indices.");
     }
+    if (!c->meshless && Ne == c->Ne && same_table(c->edge, edge, 2 * (size_t)Ne) &&
+        same_table(c->plusTri, plusTri, Ne) && same_table(c->minusTri, minusTri, Ne) &&
+        same_table(c->plusSign, plusSign, Ne) && same_table(c->minusSign, minusSign, Ne) &&
+        same_table(c->rp, rp, 3 * (size_t)Ne) && same_table(c->rm, rm, 3 * (size_t)Ne) && same_table(c->Ap, Ap, Ne) &&
+        same_table(c->Am, Am, Ne) && same_table(c->len, len, Ne)) {
+        c->haveRwg = true;  // unchanged tables keep every cache
+        drop_pack(c);
+        return ok();
+    }
     c->Ne = Ne;
     c->edge.assign(edge, edge + 2 * (size_t)Ne);
     c->plusTri.assign(plusTri, plusTri + Ne);
@@ -617,6 +638,14 @@ efie_status efie_set_rwg_tables(efie_ctx* c, int Nt, const int* plusTri, const i
     for (int e = 0; e < Ne; ++e) {
         if (plusTri[e] < 0 || plusTri[e] > Nt) return fail(EFIE_E_MESH, "mex:rwg", "plusTri out of range");
         if (minusTri[e] < 0 || minusTri[e] > Nt) return fail(EFIE_E_MESH, "mex:rwg", "minusTri out of range");
+    }
+    if (c->meshless && c->haveRwg && Nt == c->Nt && Ne == c->Ne && same_table(c->plusTri, plusTri, Ne) &&
+        same_table(c->minusTri, minusTri, Ne) && same_table(c->plusSign, plusSign, Ne) &&
+        same_table(c->minusSign, minusSign, Ne) && same_table(c->rp, rp, 3 * (size_t)Ne) &&
+        same_table(c->rm, rm, 3 * (size_t)Ne) && same_table(c->Ap, Ap, Ne) && same_table(c->Am, Am, Ne) &&
+        same_table(c->len, len, Ne)) {
+        drop_pack(c);  // unchanged tables keep every cache
+        return ok();
     }
     c->Nv = 0;
     c->Nt = Nt;
@@ -679,18 +708,34 @@ efie_status efie_set_prepack(efie_ctx* c, const efie_prepack* pk) {
                                                            w[(size_t)i + (size_t)Nq2 * ((size_t)p + (size_t)Ns * t)]};
                 }
     };
-    c->pkReg.resize((size_t)Nt * Nq);
+    std::vector<Pt> reg((size_t)Nt * Nq);
     for (int t = 0; t < Nt; ++t)
         for (int i = 0; i < Nq; ++i) {
             const size_t b = (size_t)i + (size_t)Nq * 3 * t;
-            c->pkReg[(size_t)t * Nq + i] = {pk->r_reg[b], pk->r_reg[b + Nq], pk->r_reg[b + 2 * (size_t)Nq],
-                                            pk->w_reg[(size_t)i + (size_t)Nq * t]};
+            reg[(size_t)t * Nq + i] = {pk->r_reg[b], pk->r_reg[b + Nq], pk->r_reg[b + 2 * (size_t)Nq],
+                                       pk->w_reg[(size_t)i + (size_t)Nq * t]};
         }
-    cvt(pk->near_r, pk->near_w, NsN, c->pkNear);
-    cvt(pk->self_r, pk->self_w, NsS, c->pkSelf);
-    c->pkASelf.resize((size_t)Nt * NsS);
+    std::vector<Pt> near, self;
+    cvt(pk->near_r, pk->near_w, NsN, near);
+    cvt(pk->self_r, pk->self_w, NsS, self);
+    std::vector<double> aSelf((size_t)Nt * NsS);
     for (int t = 0; t < Nt; ++t)
-        for (int p = 0; p < NsS; ++p) c->pkASelf[(size_t)t * NsS + p] = pk->self_a[(size_t)p + (size_t)NsS * t];
+        for (int p = 0; p < NsS; ++p) aSelf[(size_t)t * NsS + p] = pk->self_a[(size_t)p + (size_t)NsS * t];
+    // the pack installed last, again: the geometry built from it stays valid
+    const bool same = c->packGen > 0 && c->pkNq == Nq && c->pkNq2 == Nq2 && c->pkNsS == NsS && c->pkNsN == NsN &&
+                      c->pkNearAlpha == pk->near_alpha && same_table(c->pkReg, reg.data(), reg.size()) &&
+                      same_table(c->pkNear, near.data(), near.size()) &&
+                      same_table(c->pkSelf, self.data(), self.size()) &&
+                      same_table(c->pkASelf, aSelf.data(), aSelf.size()) &&
+                      same_table(c->pkAeq, pk->aeqTri, (size_t)Nt) && same_table(c->pkNearMat, nm.data(), nm.size());
+    if (same) {
+        c->havePack = true;
+        return ok();
+    }
+    c->pkReg.swap(reg);
+    c->pkNear.swap(near);
+    c->pkSelf.swap(self);
+    c->pkASelf.swap(aSelf);
     c->pkAeq.assign(pk->aeqTri, pk->aeqTri + Nt);
     c->pkNearMat.swap(nm);
     c->pkNq = Nq;
@@ -769,6 +814,8 @@ efie_status check_opts(const efie_opts* o) {
         return fail(EFIE_E_ARGS, "efie:opts", "unknown precision");
     if (o->lu_kernel < EFIE_LU_AUTO || o->lu_kernel > EFIE_LU_SYM)
         return fail(EFIE_E_ARGS, "efie:opts", "unknown lu_kernel");
+    if (o->ir_max_iter < 0) return fail(EFIE_E_ARGS, "efie:opts", "ir_max_iter must be >= 0");
+    if (!(o->ir_tol > 0.0)) return fail(EFIE_E_ARGS, "efie:opts", "ir_tol must be > 0");
     if (o->model == EFIE_MODEL_CENTROID) return ok();
     if (o->model != EFIE_MODEL_QUADRATURE) return fail(EFIE_E_ARGS, "efie:opts", "unknown model");
     if (o->near_subdiv < 0 || o->near_subdiv > 3 || o->self_subdiv < 0 || o->self_subdiv > 3)
@@ -1031,6 +1078,7 @@ efie_status ensure_geometry(efie_ctx* c, const efie_opts* o) {
     c->aSelfMax = aSelfMax;
     c->key = k;
     c->geoValid = true;
+    g_geoBuilds.fetch_add(1);
     c->momP = -1;  // moments belong to the old geometry
     c->surValid = false;
     return ok();
@@ -1076,6 +1124,7 @@ efie_status ensure_moments(efie_ctx* c, const efie_opts* o, int P, efie_stats* s
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, c->ev[6], c->ev[7]));
     if (st) st->ms_moments += ms;
+    g_momBuilds.fetch_add(1);
     c->momP = P;
     c->nearAlphaMom = o->near_alpha;
     c->momExtract = extract;
@@ -1596,6 +1645,7 @@ efie_status refine_batch(efie_ctx* c, const efie_opts& o, int n, int lda, int nb
     HIPCHK(launch_fill_int(c->dDone.as<int>(), 0, nb, si));
     HIPCHK(launch_fill_int(c->dIters.as<int>(), 0, nb, si));
     HIPCHK(launch_fill_dbl(c->dRn2.as<double>(), 0.0, nb, si));
+    HIPCHK(launch_fill_dbl(c->dResid.as<double>(), 1.0, nb, si));  // never a stale value from an older call
     pf.end(EFIE_K_OTHER);
     for (int it = 0; it <= o.ir_max_iter; ++it) {
         pf.begin(EFIE_K_OTHER);
@@ -1651,10 +1701,12 @@ efie_status stage_stats(efie_ctx* c, const efie_opts& o, int nb, int slot, int B
 // After a batch's slot event has completed: zero pivots become
 // efie:singular, refinement counters go into the stats, profile marks are
 // summed.
-// retry (lu_kernel SYM): frequencies whose unpivoted factorisation hit a
-// zero pivot or did not refine to max(ir_tol, 1e-11) (1e-11: above the
-// FP64 residual floor of these systems) are appended (as offsets into freqs)
-// instead of failing; they are solved again with pivoting.
+// retry (mixed precision): systems whose FP32 factorisation hit a zero
+// pivot or whose refinement did not reach max(ir_tol, 1e-11) (1e-11: above
+// the FP64 residual floor of these systems) are appended (as offsets into
+// the batch) instead of failing; the caller solves them again, the
+// unpivoted symmetric form with the pivoted FP32 LU, the pivoted forms with
+// the FP64 LU.
 efie_status finish_batch(efie_ctx* c, const efie_opts& o, int nb, int slot, int B, efie_stats* S,
                          const double* freqs, std::vector<int>* retry = nullptr) {
     const char* h = static_cast<const char*>(c->hStat) + stat_slot_bytes(B) * slot;
@@ -1845,7 +1897,9 @@ efie_status efie_sweep(efie_ctx* c, const double* freqs, int Nf, double mu, doub
         HIPCHK(hipStreamWaitEvent(sf, c->ev[0], 0));
         HIPCHK(hipStreamWaitEvent(si, c->ev[0], 0));
     }
-    const bool symRetry = mixed && o.lu_kernel == EFIE_LU_SYM;
+    // mixed precision: systems the FP32 factorisation plus refinement did
+    // not solve are redone after the sweep
+    const bool symRetry = mixed;
     std::vector<int> retry;
     auto finish = [&](int bj) -> efie_status {
         const int f0 = bj * B, nb = std::min(B, Nf - f0), buf = bj % nbuf;
@@ -1933,13 +1987,17 @@ efie_status efie_sweep(efie_ctx* c, const double* freqs, int Nf, double mu, doub
     S->ms_factor = msFac;
     S->ms_solve = msSolve;
     if (!retry.empty()) {
-        // the frequencies the unpivoted factorisation could not solve, again
-        // with the pivoted one (a sweep of their own)
+        // the frequencies the mixed solve could not finish, again in a sweep
+        // of their own: after the unpivoted symmetric factorisation with the
+        // pivoted FP32 LU (whose own failures go on to FP64), after a
+        // pivoted FP32 LU with the FP64 LU
         const int k = (int)retry.size();
         std::vector<double> fr2(k), zin2(2 * (size_t)k), s112(2 * (size_t)k);
         for (int i = 0; i < k; ++i) fr2[i] = freqs[retry[i]];
         efie_opts o2 = o;
-        o2.lu_kernel = EFIE_LU_FP32;
+        const bool toPivoted = o.lu_kernel == EFIE_LU_SYM;
+        if (toPivoted) o2.lu_kernel = EFIE_LU_FP32;
+        else o2.precision = EFIE_PREC_FP64;
         efie_stats S2;
         st = efie_sweep(c, fr2.data(), k, mu, eps_re, eps_im, patchEdge, groundEdge, V0, spread, Z0, &o2,
                         zin2.data(), s112.data(), &S2);
@@ -1960,7 +2018,12 @@ efie_status efie_sweep(efie_ctx* c, const double* freqs, int Nf, double mu, doub
             S->ms_kernel[q] += S2.ms_kernel[q];
             S->n_kernel[q] += S2.n_kernel[q];
         }
-        S->n_lu_retry = k;
+        if (toPivoted) {
+            S->n_lu_retry = k;
+            S->n_fp64_retry = S2.n_fp64_retry;
+        } else {
+            S->n_fp64_retry = k;
+        }
     }
     S->n_near_pairs = c->nNear;
     if (o.model == EFIE_MODEL_QUADRATURE) {
@@ -2031,7 +2094,8 @@ efie_status efie_zgesv(efie_ctx* c, int n, int batch, const double* A, const dou
     if (st) return st;
     HIPCHK(hipEventRecord(c->ev[4], s));
     HIPCHK(hipStreamSynchronize(s));
-    st = finish_batch(c, o, batch, 0, batch, S, nullptr);
+    std::vector<int> redo;
+    st = finish_batch(c, o, batch, 0, batch, S, nullptr, precision == EFIE_PREC_MIXED ? &redo : nullptr);
     if (st) return st;
     for (int q = 0; q < batch; ++q)
         HIPCHK(hipMemcpy2DAsync(x + 2 * (size_t)n * q, sizeof(double) * 2, X + xStride * q, sizeof(double) * xinc,
@@ -2043,6 +2107,18 @@ efie_status efie_zgesv(efie_ctx* c, int n, int batch, const double* A, const dou
     S->ms_factor = t2;
     S->ms_solve = t3;
     S->ms_total = t2 + t3;
+    // systems whose FP32 factorisation met a zero pivot or whose
+    // refinement stalled above max(ir_tol, 1e-11): solved again by the FP64 LU
+    for (int q : redo) {
+        efie_stats S2;
+        st = efie_zgesv(c, n, 1, A + 2 * (size_t)n * n * q, b + 2 * (size_t)n * q, EFIE_PREC_FP64,
+                        x + 2 * (size_t)n * q, &S2);
+        if (st) return st;
+        S->ms_factor += S2.ms_factor;
+        S->ms_solve += S2.ms_solve;
+        S->ms_total += S2.ms_total;
+    }
+    S->n_fp64_retry = (int)redo.size();
     return ok();
 }
 
@@ -2135,6 +2211,13 @@ efie_status efie_mlp_loss_grad(efie_ctx* c, const float* params, const float* X,
     HIPCHK(hipMemcpyAsync(grad, c->dMlpE.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(loss, dLoss, sizeof(float), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    return ok();
+}
+
+efie_status efie_diag_cache_counts(long long* geometry_builds, long long* moment_builds) {
+    if (!geometry_builds || !moment_builds) return fail(EFIE_E_ARGS, "efie:args", "null argument");
+    *geometry_builds = g_geoBuilds.load();
+    *moment_builds = g_momBuilds.load();
     return ok();
 }
 

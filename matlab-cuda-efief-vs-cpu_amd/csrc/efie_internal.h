@@ -165,6 +165,7 @@ hipError_t lu_trsv(const T* A, size_t strideA, int n, int lda, T* X, size_t stri
                    int batch, bool lower, const int* skip, hipStream_t s, TrsvWork* tw);
 // Row permutation equivalent to the sequential interchanges of ipiv.
 hipError_t lu_make_perm(const int* ipiv, int n, int batch, int* perm, hipStream_t s);
+size_t lu_perm_lds_limit();
 
 // ------------------------------------------------ mixed precision -------
 hipError_t launch_to_f32(const double* A64, float* A32, size_t complexElems, hipStream_t s);

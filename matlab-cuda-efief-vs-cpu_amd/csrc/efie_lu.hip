@@ -126,9 +126,9 @@ static TreeGeom tree_geom(int M) {
 
 // Cross-workgroup exchange of small data: agent-scope relaxed atomics read
 // and write the coherent copy directly, and the writer's release fence
-// orders them before the flag or counter that publishes them.  No
-// agent-scope acquire fence is used on the reading side: on a multi-die
-// device that fence invalidates the whole L2 of the reading die.
+// orders them before the flag or counter that publishes them; on the
+// reading side the thread that observes the flag takes an agent-scope
+// acquire fence (once per wait, not per datum).
 __device__ __forceinline__ int ald(const int* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -150,7 +150,12 @@ __device__ __forceinline__ void ast_c(T* p, cpx<T> v) {
 // orders this thread's earlier memory operations before later ones
 // (release to the agent; no cache invalidation)
 __device__ __forceinline__ void release_agent() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
-// compiler/wave ordering after an observed flag (the data is read atomically)
+// taken by the one thread that observed a flag or counter published by
+// another workgroup (possibly on another XCD): the agent-scope acquire
+// orders every later read after that observation; the workgroup barrier
+// that follows, with acquire_wg in the other threads, extends the order to
+// the whole block
+__device__ __forceinline__ void acquire_agent() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 __device__ __forceinline__ void acquire_wg() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup"); }
 
 // Largest 64-bit key of the wave, returned wave-uniformly: DPP steps
@@ -395,6 +400,7 @@ __global__ __launch_bounds__(64) void k_tslu(T* A, size_t strideA, int n, int ld
                 if (old == 0) go = 0;
                 else ast(c, 0);  // last arrival resets the counter for the next panel
             }
+            if (go) acquire_agent();
             s_go = go;
         }
         __syncthreads();
@@ -1224,6 +1230,7 @@ __global__ __launch_bounds__(256) void k_trsv_fused(const T* A, size_t strideA, 
             while (av < vid && av < step + kTrsvRun &&
                    __hip_atomic_load(fl + blk(av), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch)
                 ++av;
+            acquire_agent();
             s_avail = av;
         }
         __syncthreads();
@@ -1312,23 +1319,29 @@ __global__ __launch_bounds__(256) void k_trsv_fused(const T* A, size_t strideA, 
 }
 
 // perm[b][i] = source row of position i after the sequential swaps of ipiv
-// (applying ipiv to a vector v gives v_new[i] = v[perm[i]]).
+// (applying ipiv to a vector v gives v_new[i] = v[perm[i]]).  The swaps run
+// in LDS (n ints, up to the device's per-block limit); INLDS false swaps in
+// place in perm itself (one thread, global memory: slower, any n).
+template <bool INLDS>
 __global__ __launch_bounds__(256) void k_make_perm(const int* ipiv, int n, int* perm) {
     extern __shared__ int sh[];
     const int b = blockIdx.x;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) sh[i] = i;
+    int* w = INLDS ? sh : perm + (size_t)b * n;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) w[i] = i;
     __syncthreads();
     if (threadIdx.x == 0) {
         const int* ip = ipiv + (size_t)b * n;
         for (int i = 0; i < n; ++i) {
             int q = ip[i] - 1;
-            int t = sh[i];
-            sh[i] = sh[q];
-            sh[q] = t;
+            int t = w[i];
+            w[i] = w[q];
+            w[q] = t;
         }
     }
-    __syncthreads();
-    for (int i = threadIdx.x; i < n; i += blockDim.x) perm[(size_t)b * n + i] = sh[i];
+    if (INLDS) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += blockDim.x) perm[(size_t)b * n + i] = sh[i];
+    }
 }
 
 // ---------------------------------------------------------- drivers ------
@@ -1660,8 +1673,29 @@ hipError_t lu_backsolve(T* A, size_t strideA, int n, int lda, int batch, hipStre
 
 
 
+// Largest dynamic LDS one block of k_make_perm may take on the current
+// device (the opt-in limit, 160 KiB on gfx950; 64 KiB if it cannot be read).
+size_t lu_perm_lds_limit() {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&v, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess || v <= 0)
+        return 65536;
+    return (size_t)v;
+}
+
 hipError_t lu_make_perm(const int* ipiv, int n, int batch, int* perm, hipStream_t s) {
-    hipLaunchKernelGGL(k_make_perm, dim3(batch), dim3(256), sizeof(int) * (size_t)n, s, ipiv, n, perm);
+    const size_t lds = sizeof(int) * (size_t)n;
+    if (lds <= lu_perm_lds_limit()) {
+        if (lds > 65536) {
+            // above the default dynamic-LDS limit the kernel opts in
+            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_make_perm<true>),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return e;
+        }
+        hipLaunchKernelGGL(k_make_perm<true>, dim3(batch), dim3(256), lds, s, ipiv, n, perm);
+    } else {
+        hipLaunchKernelGGL(k_make_perm<false>, dim3(batch), dim3(256), 0, s, ipiv, n, perm);
+    }
     return hipGetLastError();
 }
 

@@ -71,6 +71,7 @@ class Stats(C.Structure):
         ("ms_moments", C.c_double), ("ms_kernel", C.c_double * len(KERNEL_CLASSES)),
         ("n_kernel", C.c_int * len(KERNEL_CLASSES)),
         ("n_sur_pairs", C.c_longlong), ("n_sur_fallback", C.c_longlong), ("n_lu_retry", C.c_int),
+        ("n_fp64_retry", C.c_int),
     ]
 
     def as_dict(self):
@@ -105,7 +106,7 @@ EXPORTS = [
     "efie_opts_default", "efie_opts_preset", "efie_last_error", "efie_last_error_id",
     "efie_mesh_counts", "efie_make_mesh", "efie_rwg_count", "efie_build_rwg", "efie_pick_port",
     "efie_create", "efie_destroy", "efie_set_mesh", "efie_set_rwg", "efie_assemble_zv",
-    "efie_sweep", "efie_zgesv", "efie_diag_copy", "efie_set_prepack", "efie_set_rwg_tables", "efie_assemble_z",
+    "efie_sweep", "efie_zgesv", "efie_diag_copy", "efie_diag_cache_counts", "efie_set_prepack", "efie_set_rwg_tables", "efie_assemble_z",
     "efie_sur_cfg_default", "efie_sur_blob_floats", "efie_sur_train", "efie_sur_get", "efie_sur_set",
     "efie_sur_set_exact", "efie_mlp_params", "efie_mlp_forward", "efie_mlp_loss_grad",
 ]
@@ -144,6 +145,7 @@ def lib():
                                  C.c_double, C.c_double, C.c_double, C.POINTER(Opts), _dp, _dp, C.POINTER(Stats)]
         L.efie_zgesv.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, C.POINTER(Stats)]
         L.efie_diag_copy.argtypes = [C.c_void_p, C.c_size_t, C.c_int, _dp]
+        L.efie_diag_cache_counts.argtypes = [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
         _fp = C.POINTER(C.c_float)
         L.efie_sur_cfg_default.argtypes = [C.POINTER(SurCfg)]
         L.efie_sur_cfg_default.restype = None
@@ -426,6 +428,14 @@ class Context:
         _check(lib().efie_mlp_loss_grad(self._c, p.ctypes.data_as(fp), X.ctypes.data_as(fp), T.ctypes.data_as(fp),
                                         X.shape[0], C.byref(loss), g.ctypes.data_as(fp)))
         return loss.value, g
+
+    @staticmethod
+    def cache_counts():
+        """(geometry builds, moment integrations) so far in this process,
+        over all contexts (efie_diag.h)."""
+        g, m = C.c_longlong(), C.c_longlong()
+        _check(lib().efie_diag_cache_counts(C.byref(g), C.byref(m)))
+        return g.value, m.value
 
     def diag_copy(self, nbytes, reps=10):
         """Average time (ms) of a plain device copy of nbytes (efie_diag.h)."""
