@@ -82,11 +82,18 @@ typedef struct {
  * efie_zgesv (general matrices). */
 enum { EFIE_LU_AUTO = 0, EFIE_LU_FP32 = 1, EFIE_LU_BF16X6 = 2, EFIE_LU_SYM = 3 };
 
-/* near_mode: DIRECT evaluates the subdivided near/self quadrature at every
- * frequency as the reference does; MOMENTS expands the Green's function in
- * powers of k R, integrates the frequency-independent moments once per mesh
- * and sums the series per frequency (tail below 1e-17 relative; falls back
- * to DIRECT when k R is too large for that).  EXTRACT replaces the
+/* near_mode: DIRECT evaluates every triangle pair's quadrature (regular,
+ * subdivided near, subdivided self) at every frequency as the reference
+ * does; MOMENTS expands the Green's function of every pair about its
+ * centroid distance R0, exp(-jkR)/R = exp(-jkR0) sum_q (-jk)^q (R-R0)^q /
+ * (q! R), integrates the frequency-independent coefficients of the same
+ * quadrature rules once per mesh and forms Z of a whole batch of
+ * frequencies from them in one pass (series tail below 1e-17 relative);
+ * when |k| times the pair size needs more than 18 terms, or the series
+ * would not fit in a third of the free device memory, the regular pairs
+ * are integrated per frequency and only the near and self pairs use a
+ * moment series (falling back to DIRECT when that is too long too).
+ * EXTRACT replaces the
  * reference's softened scheme on near and self pairs by singularity
  * extraction: the static 1/(4 pi R) part of the inner integral in closed
  * form over the whole source triangle, the smooth remainder
@@ -112,7 +119,8 @@ enum {
     EFIE_K_TRSV = 9,     /* triangular solves                                 */
     EFIE_K_OTHER = 10,   /* conversions, residuals, epilogue                  */
     EFIE_K_SURROGATE = 11, /* near-pair network corrections                   */
-    EFIE_NKCLASS = 12
+    EFIE_K_SERIES = 12,  /* Z of a batch from the all-pairs series (MOMENTS)  */
+    EFIE_NKCLASS = 13
 };
 
 /* Per-call timings (HIP events on the context stream) and counters. */

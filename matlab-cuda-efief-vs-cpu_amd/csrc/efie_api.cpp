@@ -253,6 +253,13 @@ struct efie_ctx {
     bool momExtract = false;
     bool momSelfOnly = false;  // only the self-pair moments were computed
     DBuf dTriV;  // triangle vertices relative to the centroid [Nt][9]
+    // frequency-independent series of every triangle pair (near_mode
+    // MOMENTS), valid for P = serP and near_alpha = serAlpha
+    DBuf dQall;
+    int serP = -1;
+    double serAlpha = -1;
+    double radMax = 0;  // largest point radius of a triangle about its centroid
+    double dcMax = 0;   // largest centroid distance of an RWG's two triangles
 
     // near-pair surrogate: the installed model and, per mesh, the pairs it
     // takes (dSurPairs, nSurIn), the pairs left to the quadrature (dSurFb,
@@ -1076,10 +1083,23 @@ efie_status ensure_geometry(efie_ctx* c, const efie_opts* o) {
     c->dNearMax = dNearMax;
     c->aNearMax = aeqMax;
     c->aSelfMax = aSelfMax;
+    double radMax = 0, dcMax = 0;
+    for (int t = 0; t < Nt; ++t) radMax = std::max(radMax, rad[t]);
+    for (int e = 0; e < Ne; ++e) {
+        const int tp = c->plusTri[e], tm = c->minusTri[e];
+        if (tp > 0 && tm > 0) {
+            const double dx = tg[tp - 1].cx - tg[tm - 1].cx, dy = tg[tp - 1].cy - tg[tm - 1].cy,
+                         dz = tg[tp - 1].cz - tg[tm - 1].cz;
+            dcMax = std::max(dcMax, std::sqrt(dx * dx + dy * dy + dz * dz));
+        }
+    }
+    c->radMax = radMax;
+    c->dcMax = dcMax;
     c->key = k;
     c->geoValid = true;
     g_geoBuilds.fetch_add(1);
-    c->momP = -1;  // moments belong to the old geometry
+    c->serP = -1;  // series and moments belong to the old geometry
+    c->momP = -1;
     c->surValid = false;
     return ok();
 }
@@ -1129,6 +1149,62 @@ efie_status ensure_moments(efie_ctx* c, const efie_opts* o, int P, efie_stats* s
     c->nearAlphaMom = o->near_alpha;
     c->momExtract = extract;
     c->momSelfOnly = selfOnly;
+    return ok();
+}
+
+// Series length of the all-pairs fill (near_mode MOMENTS): the smallest P
+// with (x^(P+1)/(P+1)!) e^x <= 1e-17, x = |k|max * b, where b bounds
+// |R - R0e| for every point pair of every entry: the distance of a point
+// pair from its own pair's centroid distance (two point radii plus the
+// near or self softening length) plus the spread of the four pairs'
+// centroid distances about their mean (at most the centroid distances of
+// the two RWG's own triangle pairs).  -1 if P would exceed 17.
+int series_terms(const efie_ctx* c, const efie_opts* o, double kmax) {
+    const double soft = std::max(o->near_alpha * c->aNearMax, c->aSelfMax);
+    const double x = kmax * (2 * c->radMax + soft + 2 * c->dcMax) * 1.0001;
+    double term = std::exp(x);
+    for (int n = 1; n <= 18; ++n) {
+        term *= x / n;
+        if (term <= 1e-17) return n - 1;
+    }
+    return -1;
+}
+
+// Frequency-independent series of all triangle pairs for at least P + 1
+// terms (rounded up to a size the fill kernel is built for).  *ready is
+// false when the series is too long or would take more than a third of
+// the free device memory; the caller then fills per frequency.
+efie_status ensure_series(efie_ctx* c, const efie_opts* o, double kmax, efie_stats* st, bool* ready) {
+    *ready = false;
+    const int P0 = series_terms(c, o, kmax);
+    if (P0 < 0) return ok();
+    if (c->serP >= P0 && c->serAlpha == o->near_alpha && c->dQall.p) {
+        *ready = true;
+        return ok();
+    }
+    const int NT = series_nt(P0);
+    if (NT == 0) return ok();
+    const int P = NT - 3;
+    const size_t bytes = sizeof(double) * 8 * (size_t)(P + 1) * rec_index(0, c->Nt);
+    size_t freeB = 0, totB = 0;
+    HIPCHK(hipMemGetInfo(&freeB, &totB));
+    if (bytes > c->dQall.bytes && bytes - c->dQall.bytes > freeB / 3) return ok();
+    hipStream_t s = c->stream;
+    HIPCHK(c->dQall.ensure(bytes));
+    HIPCHK(hipEventRecord(c->ev[6], s));
+    HIPCHK(launch_series(c->dReg.as<Pt>(), c->Nq, c->dTri.as<TriGeo>(), c->Nt, c->dNearMat.as<uint8_t>(),
+                         c->dNearPts.as<Pt>(), c->NpNear, c->Nq2s, c->dAeq.as<double>(), o->near_alpha,
+                         c->dList.as<int2>(), (int)c->nNear, c->dSelfPts.as<Pt>(), c->NsSelf, c->dASelf.as<double>(),
+                         P, c->dQall.as<double>(), s));
+    HIPCHK(hipEventRecord(c->ev[7], s));
+    HIPCHK(hipStreamSynchronize(s));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev[6], c->ev[7]));
+    if (st) st->ms_moments += ms;
+    g_momBuilds.fetch_add(1);
+    c->serP = P;
+    c->serAlpha = o->near_alpha;
+    *ready = true;
     return ok();
 }
 
@@ -1205,6 +1281,9 @@ struct NearPlan {
     bool momNear = false;  // near-pair records from the cached moments
     bool momSelf = false;  // self-pair records from the cached moments
     bool sur = false;      // near pairs in the surrogate's envelope: regular rule + network
+    bool series = false;   // every pair from the frequency-independent series (one kernel per batch)
+    int serNT = 0;         // its coefficients per entry
+    bool lossy = false;    // some k of the call is complex
 };
 
 // Fill one Z (interleaved complex, element (m,n) at Z[2*(m*ld + n)], which
@@ -1222,6 +1301,14 @@ efie_status fill_z(efie_ctx* c, const efie_opts* o, const NearPlan& plan, const 
                                c->dCW.as<double>(), c->dCT.as<int>(), c->dCAsoft.as<double>(), c->Ne, dfr,
                                Z, ld, 1, s));
         pf.end(EFIE_K_OTHER);
+        return ok();
+    }
+    if (plan.series) {
+        pf.begin(EFIE_K_SERIES);
+        HIPCHK(launch_series_fill(c->dQall.as<double>(), plan.serNT, plan.lossy, c->dTri.as<TriGeo>(),
+                                  c->dSideTri.as<int>(), c->dSide.as<Side>(), c->Ne, dfr, 1, Z, ld, 0, Z32, ld, 0, s));
+        pf.end(EFIE_K_SERIES);
+        if (wrote32) *wrote32 = Z32 != nullptr;
         return ok();
     }
     const int Nt = c->Nt;
@@ -1416,7 +1503,19 @@ efie_status ensure_surrogate(efie_ctx* c, const efie_opts* o) {
 efie_status choose_near(efie_ctx* c, const efie_opts* o, double kmax, bool lossy, efie_stats* st,
                         NearPlan* plan) {
     *plan = NearPlan();
+    plan->lossy = lossy;
     if (o->model != EFIE_MODEL_QUADRATURE || o->near_mode == EFIE_NEAR_DIRECT) return ok();
+    if (o->near_mode == EFIE_NEAR_MOMENTS) {
+        bool ready = false;
+        efie_status e = ensure_series(c, o, kmax, st, &ready);
+        if (e) return e;
+        if (ready) {
+            plan->series = true;
+            plan->serNT = c->serP + 3;
+            if (st) st->moment_terms = c->serP + 1;
+            return ok();
+        }
+    }
     const bool sur = o->near_mode == EFIE_NEAR_SURROGATE;
     if (sur) {
         if (c->surBlob.empty())
@@ -1871,12 +1970,22 @@ efie_status efie_sweep(efie_ctx* c, const double* freqs, int Nf, double mu, doub
         double* Zb = zbuf(buf);
         float* Z32b = z32buf(buf);
         bool h32 = Z32b != nullptr;
-        for (int i = 0; i < nb; ++i) {
-            bool w32 = false;
-            efie_status e = fill_z(c, &o, plan, dfr + i, Zb + 2 * zElems * i, (size_t)lda,
-                                   Z32b ? Z32b + 2 * zElems * i : nullptr, &w32, sf);
-            if (e) return e;
-            h32 = h32 && w32;
+        if (plan.series) {
+            // every frequency of the batch in one pass over the series
+            Prof pff = make_prof(c);
+            pff.begin(EFIE_K_SERIES);
+            HIPCHK(launch_series_fill(c->dQall.as<double>(), plan.serNT, plan.lossy, c->dTri.as<TriGeo>(),
+                                      c->dSideTri.as<int>(), c->dSide.as<Side>(), n, dfr, nb, Zb, (size_t)lda,
+                                      2 * zElems, Z32b, (size_t)lda, 2 * zElems, sf));
+            pff.end(EFIE_K_SERIES);
+        } else {
+            for (int i = 0; i < nb; ++i) {
+                bool w32 = false;
+                efie_status e = fill_z(c, &o, plan, dfr + i, Zb + 2 * zElems * i, (size_t)lda,
+                                       Z32b ? Z32b + 2 * zElems * i : nullptr, &w32, sf);
+                if (e) return e;
+                h32 = h32 && w32;
+            }
         }
         // rhs into column n (of the complex64 copy too when the fill wrote it)
         HIPCHK(launch_set_rhs(c->dV.as<double>(), Zb, h32 ? Z32b : nullptr, n, lda, zElems, nb, sf));
@@ -2030,8 +2139,9 @@ efie_status efie_sweep(efie_ctx* c, const double* freqs, int Nf, double mu, doub
         const long long Nt = c->Nt, np = (long long)c->NpNear, ns = (long long)c->NsSelf * c->Nq2s;
         const long long reg = Nt * (Nt - 1) / 2 - c->nNear;
         const long long nearDirect = plan.momNear ? 0 : plan.sur ? c->nSurFb : c->nNear;
-        S->n_green = (reg + (plan.sur ? c->nSurIn : 0)) * c->Nq * c->Nq + nearDirect * np * np +
-                     (plan.momSelf ? 0 : Nt * ns * ns);
+        S->n_green = plan.series ? 0
+                                 : (reg + (plan.sur ? c->nSurIn : 0)) * c->Nq * c->Nq + nearDirect * np * np +
+                                       (plan.momSelf ? 0 : Nt * ns * ns);
     }
     c->prof = false;
     return ok();

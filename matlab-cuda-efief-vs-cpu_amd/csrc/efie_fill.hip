@@ -967,6 +967,414 @@ __global__ __launch_bounds__(256) void k_combine(const Rec* __restrict__ recs, c
     }
 }
 
+// --------------------------------------- frequency-independent series ----
+// The fill of near_mode MOMENTS, for every triangle pair.  With R0 the
+// centroid distance of the pair (0 for a triangle with itself) and
+// z = -jk, the Green's function factors as
+//   e^{-jkR}/R = e^{-jk R0} sum_q z^q (R - R0)^q / (q! R),
+// so the eight moments of the pair's record are
+//   e^{-jk R0}/(4 pi) sum_q z^q Q[q][x],  Q[q][x] = sum W X (R - R0)^q/(q! R)
+// with real, frequency-independent Q (X = 1, rho, rho', rho.rho'; R the
+// same (softened) distance the reference rule uses for the pair).  Q is
+// integrated once per mesh for q = 0..P, P chosen so the series tail is
+// below 1e-17 relative for the sweep's largest |k|.
+//
+// With alpha = j omega mu = -eta z and beta = 1/(j omega eps) = -eta/z
+// (eta = sqrt(mu/eps)), one pair's share of Z(m,n) is
+//   alpha Ivec + beta Isca = -eta e^{-jkR0}/(4 pi) sum_{s=-1}^{P+1} z^s D[s],
+//   D[s] = V[s-1] + S[s+1],
+// where V[q] and S[q] are the vector and scalar combinations of Q[q] with
+// the two RWG sides (the coefficients of the record combine below).  The
+// four pairs of an entry are brought to one phase centre R0e (their mean
+// R0): e^{-jk R0} = e^{-jk R0e} e^{z (R0 - R0e)}, and multiplying a series
+// by e^{z d} is the exact Cauchy product with d^i/i!.  Per frequency an
+// entry then costs one phase and one polynomial in z.
+
+constexpr int kSerChunk = 4;
+
+// (R - R0)^q / q! / R for q = q0.. is t, t*d/(q0+1), ...: 1/n for n = 1..32
+__constant__ double kInvN[33] = {0.0,        1.0,        1.0 / 2,    1.0 / 3,    1.0 / 4,    1.0 / 5,    1.0 / 6,
+                                 1.0 / 7,    1.0 / 8,    1.0 / 9,    1.0 / 10,   1.0 / 11,   1.0 / 12,   1.0 / 13,
+                                 1.0 / 14,   1.0 / 15,   1.0 / 16,   1.0 / 17,   1.0 / 18,   1.0 / 19,   1.0 / 20,
+                                 1.0 / 21,   1.0 / 22,   1.0 / 23,   1.0 / 24,   1.0 / 25,   1.0 / 26,   1.0 / 27,
+                                 1.0 / 28,   1.0 / 29,   1.0 / 30,   1.0 / 31,   1.0 / 32};
+
+// d^q0 / q0! times w/R
+__device__ __forceinline__ double series_head(double wInvR, double d, int q0) {
+    double t = wInvR;
+    for (int e = 1; e <= q0; ++e) t *= d * kInvN[e];
+    return t;
+}
+
+// Regular pairs: tile of 64 test triangles (one per lane) by 64 source
+// triangles, as the per-frequency regular kernel; each pair's NQ x NQ
+// product rule is summed in chunks of kSerChunk orders.
+template <int NQ>
+__global__ __launch_bounds__(256) void k_series_regular(const Pt* __restrict__ pts, const TriGeo* __restrict__ tg,
+                                                        int Nt, const uint8_t* __restrict__ nearMat, int P,
+                                                        double* __restrict__ Q) {
+    long long b = blockIdx.x;
+    int tj = (int)((sqrt(8.0 * (double)b + 1.0) - 1.0) * 0.5);
+    while ((long long)(tj + 1) * (tj + 2) / 2 <= b) ++tj;
+    while ((long long)tj * (tj + 1) / 2 > b) --tj;
+    int ti = (int)(b - (long long)tj * (tj + 1) / 2);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int t1 = ti * kRegTile + lane;
+    __shared__ Pt sp[kRegTile * NQ];
+    for (int e = threadIdx.x; e < kRegTile * NQ; e += 256) {
+        const int t = ti * kRegTile + e / NQ;
+        sp[e] = pts[(size_t)(t < Nt ? t : Nt - 1) * NQ + e % NQ];
+    }
+    __syncthreads();
+    const int t1c = t1 < Nt ? t1 : Nt - 1;
+    const TriGeo g1 = tg[t1c];
+    const Pt* p1 = sp + lane * NQ;
+    constexpr int perWave = kRegTile / (4 * kRegSplit);
+    const int t2base = tj * kRegTile + (int)blockIdx.y * (kRegTile / kRegSplit) + wave * perWave;
+    const size_t stride = (size_t)(P + 1) * 8;
+    for (int qq = 0; qq < perWave; ++qq) {
+        const int t2 = t2base + qq;
+        if (t2 >= Nt) break;
+        if (!(t1 < t2) || nearMat[(size_t)t2 * Nt + t1] != 0) continue;
+        const TriGeo g2 = tg[t2];
+        const Pt* p2 = pts + (size_t)t2 * NQ;
+        const double ox = g1.cx - g2.cx, oy = g1.cy - g2.cy, oz = g1.cz - g2.cz;
+        const double R0 = sqrt(ox * ox + oy * oy + oz * oz);
+        double* out = Q + rec_index(t1, t2) * stride;
+        for (int q0 = 0; q0 <= P; q0 += kSerChunk) {
+            double acc[kSerChunk][8];
+#pragma unroll
+            for (int c = 0; c < kSerChunk; ++c)
+#pragma unroll
+                for (int x = 0; x < 8; ++x) acc[c][x] = 0.0;
+#pragma unroll 1
+            for (int i = 0; i < NQ; ++i) {
+                const Pt pi = p1[i];
+                const double xi = pi.x + ox, yi = pi.y + oy, zi = pi.z + oz;
+                double g[kSerChunk], hx[kSerChunk], hy[kSerChunk], hz[kSerChunk];
+#pragma unroll
+                for (int c = 0; c < kSerChunk; ++c) g[c] = hx[c] = hy[c] = hz[c] = 0.0;
+#pragma unroll
+                for (int j = 0; j < NQ; ++j) {
+                    const Pt q2 = p2[j];
+                    const double dx = xi - q2.x, dy = yi - q2.y, dz = zi - q2.z;
+                    // R < 1e-15 -> 1e-15, as the regular rule
+                    const double R2 = fmax(dx * dx + dy * dy + dz * dz, 1e-30);
+                    const double invR = rsqrt64(R2);
+                    const double d = R2 * invR - R0;
+                    double t = series_head(q2.w * invR, d, q0);
+#pragma unroll
+                    for (int c = 0; c < kSerChunk; ++c) {
+                        g[c] += t;
+                        hx[c] += t * q2.x;
+                        hy[c] += t * q2.y;
+                        hz[c] += t * q2.z;
+                        t *= d * kInvN[q0 + c + 1];
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < kSerChunk; ++c) {
+                    const double w = pi.w;
+                    acc[c][0] += w * g[c];
+                    acc[c][1] += w * g[c] * pi.x;
+                    acc[c][2] += w * g[c] * pi.y;
+                    acc[c][3] += w * g[c] * pi.z;
+                    acc[c][4] += w * hx[c];
+                    acc[c][5] += w * hy[c];
+                    acc[c][6] += w * hz[c];
+                    acc[c][7] += w * (pi.x * hx[c] + pi.y * hy[c] + pi.z * hz[c]);
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < kSerChunk; ++c)
+                if (q0 + c <= P) {
+                    double2* o2 = reinterpret_cast<double2*>(out + (size_t)(q0 + c) * 8);
+#pragma unroll
+                    for (int x = 0; x < 4; ++x) o2[x] = make_double2(acc[c][2 * x], acc[c][2 * x + 1]);
+                }
+        }
+    }
+}
+
+// Subdivided pairs (one wave per pair): near pairs with the softened
+// distance sqrt(R^2 + a^2) on every point pair (near_singular.m:41-42),
+// self pairs (R0 = 0) softened on coincident sub-triangles and clamped at
+// 1e-15 elsewhere (singular.m:46-50).  Lanes split the Np x Np point pairs
+// as the per-frequency subdivided kernel does.
+template <bool SELF>
+__device__ void series_subdiv(const Pt* __restrict__ P1, const Pt* __restrict__ P2, int Np, int Nq2, double ox,
+                              double oy, double oz, double R0, double aNear2, const double* __restrict__ aSelf, int P,
+                              double* __restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    for (int q0 = 0; q0 <= P; q0 += kSerChunk) {
+        double acc[kSerChunk][8];
+#pragma unroll
+        for (int c = 0; c < kSerChunk; ++c)
+#pragma unroll
+            for (int x = 0; x < 8; ++x) acc[c][x] = 0.0;
+        for (int base = 0; base < Np; base += 64) {
+            const int r = min(64, Np - base);
+            const int parts = 64 / r;
+            const int part = lane / r;
+            const int i = base + lane % r;
+            if (part >= parts) continue;
+            const int j0 = (Np * part) / parts, j1 = (Np * (part + 1)) / parts;
+            const Pt pi = P1[i];
+            const double xi = pi.x + ox, yi = pi.y + oy, zi = pi.z + oz;
+            const int psub = i / Nq2;
+            double aS2 = 0.0;
+            if (SELF) {
+                const double as = aSelf[psub];
+                aS2 = as * as;
+            }
+            double g[kSerChunk], hx[kSerChunk], hy[kSerChunk], hz[kSerChunk];
+#pragma unroll
+            for (int c = 0; c < kSerChunk; ++c) g[c] = hx[c] = hy[c] = hz[c] = 0.0;
+            for (int j = j0; j < j1; ++j) {
+                const Pt q2 = P2[j];
+                const double dx = xi - q2.x, dy = yi - q2.y, dz = zi - q2.z;
+                double R2 = dx * dx + dy * dy + dz * dz;
+                double R;
+                if (!SELF) {
+                    R = sqrt(R2 + aNear2);
+                } else if (j / Nq2 == psub) {
+                    R = sqrt(R2 + aS2);
+                } else {
+                    R = sqrt(R2);
+                    R = R > 1e-15 ? R : 1e-15;
+                }
+                const double d = R - R0;
+                double t = series_head(q2.w / R, d, q0);
+#pragma unroll
+                for (int c = 0; c < kSerChunk; ++c) {
+                    g[c] += t;
+                    hx[c] += t * q2.x;
+                    hy[c] += t * q2.y;
+                    hz[c] += t * q2.z;
+                    t *= d * kInvN[q0 + c + 1];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < kSerChunk; ++c) {
+                const double w = pi.w;
+                acc[c][0] += w * g[c];
+                acc[c][1] += w * g[c] * pi.x;
+                acc[c][2] += w * g[c] * pi.y;
+                acc[c][3] += w * g[c] * pi.z;
+                acc[c][4] += w * hx[c];
+                acc[c][5] += w * hy[c];
+                acc[c][6] += w * hz[c];
+                acc[c][7] += w * (pi.x * hx[c] + pi.y * hy[c] + pi.z * hz[c]);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < kSerChunk; ++c)
+#pragma unroll
+            for (int x = 0; x < 8; ++x) {
+                const double v = wave_sum(acc[c][x]);
+                if (lane == 0 && q0 + c <= P) out[(size_t)(q0 + c) * 8 + x] = v;
+            }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_series_near(const Pt* __restrict__ subPts, int Np, int Nq2,
+                                                     const TriGeo* __restrict__ tg, const double* __restrict__ aeq,
+                                                     double near_alpha, const int2* __restrict__ list, int nPairs,
+                                                     int P, double* __restrict__ Q) {
+    const int pair = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (pair >= nPairs) return;
+    const int2 tp = list[pair];
+    const TriGeo g1 = tg[tp.x], g2 = tg[tp.y];
+    // a = near_alpha * min(sqrt(A1/pi), sqrt(A2/pi))  (near_singular.m:16-19)
+    const double a1 = aeq[tp.x], a2 = aeq[tp.y];
+    const double as = near_alpha * (a1 < a2 ? a1 : a2);
+    const double ox = g1.cx - g2.cx, oy = g1.cy - g2.cy, oz = g1.cz - g2.cz;
+    series_subdiv<false>(subPts + (size_t)tp.x * Np, subPts + (size_t)tp.y * Np, Np, Nq2, ox, oy, oz,
+                         sqrt(ox * ox + oy * oy + oz * oz), as * as, nullptr, P,
+                         Q + rec_index(tp.x, tp.y) * (size_t)(P + 1) * 8);
+}
+
+__global__ __launch_bounds__(256) void k_series_self(const Pt* __restrict__ selfPts, int Nq2, int Ns,
+                                                     const double* __restrict__ aSelf, int Nt, int P,
+                                                     double* __restrict__ Q) {
+    const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= Nt) return;
+    const int Np = Ns * Nq2;
+    const Pt* Pp = selfPts + (size_t)t * Np;
+    series_subdiv<true>(Pp, Pp, Np, Nq2, 0.0, 0.0, 0.0, 0.0, 0.0, aSelf + (size_t)t * Ns, P,
+                        Q + rec_index(t, t) * (size_t)(P + 1) * 8);
+}
+
+// Z of a batch of nf frequencies from the series: one block per 16 x 16
+// upper-triangular tile of Z, one entry per thread.  Each thread forms its
+// entry's D[0..NT-1] (NT = P + 3 orders z^-1 .. z^(P+1)) about the common
+// centre R0e once, then for every frequency evaluates
+//   Z(m,n) = -eta/(4 pi) e^{-jk R0e} z^{-1} sum_j z^j D[j]
+// and writes the tile and its mirror (through LDS) to matrix f of the
+// batch (FP64, and FP32 when Z32 is set).
+template <int NT, bool LOSSY>
+__global__ __launch_bounds__(256) void k_series_fill(const double* __restrict__ Q, const TriGeo* __restrict__ tg,
+                                                     const int* __restrict__ sideTri, const Side* __restrict__ side,
+                                                     int Ne, const Freq* __restrict__ fr, int nf,
+                                                     double* __restrict__ Z, size_t ld, size_t zStride,
+                                                     float* __restrict__ Z32, size_t ld32, size_t z32Stride,
+                                                     int nblk) {
+    constexpr int P = NT - 3;
+    constexpr size_t stride = (size_t)(P + 1) * 8;
+    __shared__ double2 tile[16][17];
+    long long id = blockIdx.x;
+    {
+        const int q = nblk >> 3, rr = nblk & 7, x = (int)(id & 7), loc = (int)(id >> 3);
+        id = x < rr ? (long long)x * (q + 1) + loc : (long long)rr * (q + 1) + (long long)(x - rr) * q + loc;
+    }
+    int J = (int)((sqrt(8.0 * (double)id + 1.0) - 1.0) * 0.5);
+    while ((long long)(J + 1) * (J + 2) / 2 <= id) ++J;
+    while ((long long)J * (J + 1) / 2 > id) --J;
+    const int I = (int)(id - (long long)J * (J + 1) / 2);
+    const int r = threadIdx.x >> 4, cI = threadIdx.x & 15;
+    const int m = 16 * I + r, n = 16 * J + cI;
+    const int mc = m < Ne ? m : Ne - 1, nc = n < Ne ? n : Ne - 1;
+
+    int tm[2], tn[2];
+    Side sa[2], sb[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        tm[k] = sideTri[2 * mc + k];
+        tn[k] = sideTri[2 * nc + k];
+        sa[k] = side[2 * mc + k];
+        sb[k] = side[2 * nc + k];
+    }
+    // phase centres of the four pairs and their mean
+    double R0p[2][2], R0e = 0.0;
+    int np = 0;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            R0p[a][b] = 0.0;
+            if (tm[a] >= 0 && tn[b] >= 0) {
+                const TriGeo g1 = tg[tm[a]], g2 = tg[tn[b]];
+                const double dx = g1.cx - g2.cx, dy = g1.cy - g2.cy, dz = g1.cz - g2.cz;
+                R0p[a][b] = sqrt(dx * dx + dy * dy + dz * dz);
+                R0e += R0p[a][b];
+                ++np;
+            }
+        }
+    R0e = np > 0 ? R0e / np : 0.0;
+
+    double D[NT];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) D[j] = 0.0;
+#pragma unroll 1
+    for (int ab = 0; ab < 4; ++ab) {
+        const int a = ab >> 1, b = ab & 1;
+        if (tm[a] < 0 || tn[b] < 0) continue;
+        const bool sw = tm[a] > tn[b];
+        const double* q = Q + (sw ? rec_index(tn[b], tm[a]) : rec_index(tm[a], tn[b])) * stride;
+        const Side A = sa[a], B = sb[b];
+        const double cab = A.c * B.c;
+        // S2 - S1.un - S1'.um + S0 (um.un); S1 and S1' exchanged when the
+        // pair is stored the other way round
+        const double k0 = cab * (A.ux * B.ux + A.uy * B.uy + A.uz * B.uz);
+        const double k1x = -cab * (sw ? A.ux : B.ux), k1y = -cab * (sw ? A.uy : B.uy), k1z = -cab * (sw ? A.uz : B.uz);
+        const double k4x = -cab * (sw ? B.ux : A.ux), k4y = -cab * (sw ? B.uy : A.uy), k4z = -cab * (sw ? B.uz : A.uz);
+        const double ks = 4.0 * cab;
+        // D[j] (order z^(j-1)) = V[j-2] + S[j]
+        double Dp[NT];
+#pragma unroll
+        for (int j = 0; j < NT; ++j) Dp[j] = 0.0;
+#pragma unroll
+        for (int qq = 0; qq <= P; ++qq) {
+            const double2* v = reinterpret_cast<const double2*>(q + (size_t)qq * 8);
+            const double2 v0 = v[0], v1 = v[1], v2 = v[2], v3 = v[3];
+            const double V = k0 * v0.x + k1x * v0.y + k1y * v1.x + k1z * v1.y + k4x * v2.x + k4y * v2.y +
+                             k4z * v3.x + cab * v3.y;
+            Dp[qq] += ks * v0.x;
+            Dp[qq + 2] += V;
+        }
+        // shift to the common centre: times e^{z d}, d = R0 - R0e
+        const double d = R0p[a][b] - R0e;
+        double e[NT];
+        e[0] = 1.0;
+#pragma unroll
+        for (int j = 1; j < NT; ++j) e[j] = e[j - 1] * d * kInvN[j];
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+            double s = 0.0;
+#pragma unroll
+            for (int i = 0; i <= j; ++i) s += Dp[i] * e[j - i];
+            D[j] += s;
+        }
+    }
+
+    const bool upper = m < Ne && n < Ne && (I != J || r <= cI);
+    const int mm = 16 * J + r, nn = 16 * I + cI;
+    const bool lower = mm < Ne && nn < Ne && (I != J || r > cI);
+    for (int f = 0; f < nf; ++f) {
+        const Freq F = fr[f];
+        // z = -jk = ki - j kr; -eta/(4 pi) = -alpha/(j k 4 pi) = alpha/(4 pi z)
+        double vr, vi;
+        if (!LOSSY) {
+            // z = -j kr: sum_j z^j D[j] = A - j kr B with A, B even/odd
+            // polynomials in u = -kr^2; times 1/z = j/kr: B + j A/kr
+            const double u = -F.kr * F.kr;
+            double A = D[NT - 1 - ((NT - 1) & 1)], B = D[NT - 1 - (NT & 1)];
+#pragma unroll
+            for (int j = NT - 3 - ((NT - 1) & 1); j >= 0; j -= 2) A = A * u + D[j];
+#pragma unroll
+            for (int j = NT - 3 - (NT & 1); j >= 1; j -= 2) B = B * u + D[j];
+            vr = B;
+            vi = A / F.kr;
+        } else {
+            const double zr = F.ki, zi = -F.kr;
+            double pr = D[NT - 1], pi = 0.0;
+#pragma unroll
+            for (int j = NT - 2; j >= 0; --j) {
+                const double tr = pr * zr - pi * zi + D[j];
+                pi = pr * zi + pi * zr;
+                pr = tr;
+            }
+            const double z2 = zr * zr + zi * zi;
+            vr = (pr * zr + pi * zi) / z2;
+            vi = (pi * zr - pr * zi) / z2;
+        }
+        // e^{-jk R0e} = e^{ki R0e} (cos - j sin)(kr R0e)
+        double s, c;
+        sincos_cw(F.kr * R0e, &s, &c);
+        if (LOSSY) {
+            const double amp = exp(F.ki * R0e);
+            s *= amp;
+            c *= amp;
+        }
+        const double wr = vr * c + vi * s, wi = vi * c - vr * s;
+        // alpha/(4 pi z)
+        double fr_, fi_;
+        {
+            const double zr = F.ki, zi = -F.kr, z2 = zr * zr + zi * zi;
+            const double ar = F.ar * (0.25 / kPi), ai = F.ai * (0.25 / kPi);
+            fr_ = (ar * zr + ai * zi) / z2;
+            fi_ = (ai * zr - ar * zi) / z2;
+        }
+        const double2 val = make_double2(fr_ * wr - fi_ * wi, fr_ * wi + fi_ * wr);
+        double* Zf = Z + zStride * f;
+        float* Z32f = Z32 ? Z32 + z32Stride * f : nullptr;
+        tile[r][cI] = val;
+        if (upper) {
+            *reinterpret_cast<double2*>(Zf + 2 * ((size_t)m * ld + n)) = val;
+            if (Z32f) *reinterpret_cast<float2*>(Z32f + 2 * ((size_t)m * ld32 + n)) = make_float2((float)val.x, (float)val.y);
+        }
+        __syncthreads();
+        if (lower) {
+            const double2 v = tile[cI][r];
+            *reinterpret_cast<double2*>(Zf + 2 * ((size_t)mm * ld + nn)) = v;
+            if (Z32f)
+                *reinterpret_cast<float2*>(Z32f + 2 * ((size_t)mm * ld32 + nn)) = make_float2((float)v.x, (float)v.y);
+        }
+        __syncthreads();
+    }
+}
+
 // ----------------------------------------------- reference centroid fill --
 // kernel_Z_centroid (patch_sweep_cuda.cu:94-153): one point per triangle.
 
@@ -1229,6 +1637,62 @@ hipError_t launch_poly_records(const double* Qn, const int2* list, int nNear, co
     const int n = nNear + Nt;
     hipLaunchKernelGGL(k_poly_records, dim3((n + 255) / 256), dim3(256), 0, s, Qn, list, nNear, Qs, Nt, P, fr,
                        recs);
+    return hipGetLastError();
+}
+
+hipError_t launch_series(const Pt* regPts, int Nq, const TriGeo* tg, int Nt, const uint8_t* nearMat,
+                         const Pt* subPts, int Np, int Nq2, const double* aeq, double near_alpha, const int2* list,
+                         int nPairs, const Pt* selfPts, int Ns, const double* aSelf, int P, double* Q,
+                         hipStream_t s) {
+    if (P < 0 || P + 1 > 32) return hipErrorInvalidValue;
+    long long T = (Nt + kRegTile - 1) / kRegTile;
+    long long nb = T * (T + 1) / 2;
+    dim3 g((unsigned)nb, kRegSplit), blk(256);
+    switch (Nq) {
+        case 1: hipLaunchKernelGGL(k_series_regular<1>, g, blk, 0, s, regPts, tg, Nt, nearMat, P, Q); break;
+        case 3: hipLaunchKernelGGL(k_series_regular<3>, g, blk, 0, s, regPts, tg, Nt, nearMat, P, Q); break;
+        case 6: hipLaunchKernelGGL(k_series_regular<6>, g, blk, 0, s, regPts, tg, Nt, nearMat, P, Q); break;
+        case 7: hipLaunchKernelGGL(k_series_regular<7>, g, blk, 0, s, regPts, tg, Nt, nearMat, P, Q); break;
+        default: return hipErrorInvalidValue;
+    }
+    if (nPairs > 0)
+        hipLaunchKernelGGL(k_series_near, dim3((nPairs + 3) / 4), dim3(256), 0, s, subPts, Np, Nq2, tg, aeq,
+                           near_alpha, list, nPairs, P, Q);
+    hipLaunchKernelGGL(k_series_self, dim3((Nt + 3) / 4), dim3(256), 0, s, selfPts, Nq2, Ns, aSelf, Nt, P, Q);
+    return hipGetLastError();
+}
+
+int series_nt(int P) {
+    for (int nt : {10, 12, 14, 16, 18, 20})
+        if (P + 3 <= nt) return nt;
+    return 0;
+}
+
+hipError_t launch_series_fill(const double* Q, int NT, bool lossy, const TriGeo* tg, const int* sideTri,
+                              const Side* side, int Ne, const Freq* fr, int nf, double* Z, size_t ld,
+                              size_t zStride, float* Z32, size_t ld32, size_t z32Stride, hipStream_t s) {
+    const int T = (Ne + 15) / 16;
+    const long long nb = (long long)T * (T + 1) / 2;
+    const dim3 g((unsigned)nb), blk(256);
+#define EFIE_SER_CASE(N)                                                                                         \
+    case N:                                                                                                       \
+        if (lossy)                                                                                                \
+            hipLaunchKernelGGL((k_series_fill<N, true>), g, blk, 0, s, Q, tg, sideTri, side, Ne, fr, nf, Z, ld,   \
+                               zStride, Z32, ld32, z32Stride, (int)nb);                                           \
+        else                                                                                                      \
+            hipLaunchKernelGGL((k_series_fill<N, false>), g, blk, 0, s, Q, tg, sideTri, side, Ne, fr, nf, Z, ld,  \
+                               zStride, Z32, ld32, z32Stride, (int)nb);                                           \
+        break;
+    switch (NT) {
+        EFIE_SER_CASE(10)
+        EFIE_SER_CASE(12)
+        EFIE_SER_CASE(14)
+        EFIE_SER_CASE(16)
+        EFIE_SER_CASE(18)
+        EFIE_SER_CASE(20)
+        default: return hipErrorInvalidValue;
+    }
+#undef EFIE_SER_CASE
     return hipGetLastError();
 }
 

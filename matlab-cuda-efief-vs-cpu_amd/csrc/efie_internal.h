@@ -87,6 +87,26 @@ hipError_t launch_moments(const Pt* subPts, int Np, const TriGeo* tg, const doub
 hipError_t launch_poly_records(const double* Qn, const int2* list, int nNear, const double* Qs,
                                int Nt, int P, const Freq* fr, Rec* recs, hipStream_t s);
 
+// Frequency-independent series of every triangle pair {t1 <= t2} about its
+// centroid distance R0 (0 for t1 = t2): Q[rec_index(t1,t2)][q][8] =
+// sum W X (R - R0)^q / (q! R), q = 0..P (P + 1 <= 32), with the regular rule
+// for pairs outside nearMat, the subdivided near rule for the nPairs pairs of
+// list and the self rule for t1 = t2.
+hipError_t launch_series(const Pt* regPts, int Nq, const TriGeo* tg, int Nt, const uint8_t* nearMat,
+                         const Pt* subPts, int Np, int Nq2, const double* aeq, double near_alpha, const int2* list,
+                         int nPairs, const Pt* selfPts, int Ns, const double* aSelf, int P, double* Q,
+                         hipStream_t s);
+// Number of series coefficients per entry (P + 3, rounded up to a size the
+// fill kernel is built for: 10..20, even) or 0 if P is too large.
+int series_nt(int P);
+// Z of nf frequencies from the series (P = NT - 3): matrix f at
+// Z + f*zStride, element (m,n) at 2*(m*ld + n), and its complex64 copy at
+// Z32 + f*z32Stride when Z32 is set.  lossy: some k has a nonzero
+// imaginary part.
+hipError_t launch_series_fill(const double* Q, int NT, bool lossy, const TriGeo* tg, const int* sideTri,
+                              const Side* side, int Ne, const Freq* fr, int nf, double* Z, size_t ld,
+                              size_t zStride, float* Z32, size_t ld32, size_t z32Stride, hipStream_t s);
+
 // Z(m,n) = sum over the four side pairs of alpha*Ivec + beta*Isca.
 // Output element (m,n) goes to Z[m*ld + n] (interleaved complex; Z is
 // symmetric, so this is also the column-major matrix with leading

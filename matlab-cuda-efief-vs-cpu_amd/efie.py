@@ -60,7 +60,7 @@ class Opts(C.Structure):
 
 
 KERNEL_CLASSES = ["regular", "near", "self", "poly", "combine", "tslu", "laswp", "panel", "gemm",
-                  "trsv", "other", "surrogate"]
+                  "trsv", "other", "surrogate", "series"]
 
 
 class Stats(C.Structure):

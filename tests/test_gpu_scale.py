@@ -143,9 +143,11 @@ def test_c4_full_size(efie_mod, ctx, c4):
     """All 32 geometries x 64 frequencies through the mixed path (FP32
     LDL^T + FP64 refinement to 1e-10): every frequency either refines to
     max(ir_tol, 1e-11) or is solved again (counted); every fourth geometry's
-    Zin against the FP64 path -- within 1e-9 at the workload's refinement
-    tolerance 1e-10 (a residual of 1e-10 relative shows as up to a few
-    1e-10 in Zin near resonance) and within 1e-10 at tolerance 1e-11."""
+    Zin against the FP64 path -- within 1e-9 of the plain FP64 path (direct
+    quadrature) at the workload's refinement tolerance 1e-10 (a residual of
+    1e-10 relative shows as up to a few 1e-10 in Zin near resonance), and
+    within 1e-10 of the FP64 path with the same fill (series) at tolerance
+    1e-11, which isolates the FP32 factorisation's share."""
     fr = np.linspace(*W.C4["freqs"])
     o = dict(W.OPTS_C1, precision=efie_mod.PREC_MIXED, near_mode=efie_mod.NEAR_MOMENTS)
     ns = [p[3].Ne for p in c4]
@@ -166,9 +168,11 @@ def test_c4_full_size(efie_mod, ctx, c4):
         g, V, F, r, pe, ge, spread = c4[i]
         ctx.set_problem(V, F, r)
         z64 = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0, efie_mod.opts(**W.OPTS_C1))[0]
+        z64m = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0,
+                         efie_mod.opts(**W.OPTS_C1, near_mode=efie_mod.NEAR_MOMENTS))[0]
         z11 = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0, efie_mod.opts(**o, ir_tol=1e-11))[0]
         e10 = float(np.max(np.abs(zm[i] - z64) / np.abs(z64)))
-        e11 = float(np.max(np.abs(z11 - z64) / np.abs(z64)))
+        e11 = float(np.max(np.abs(z11 - z64m) / np.abs(z64m)))
         print(f"  geometry {i}: N={r.Ne} |dZin|/|Zin| {e10:.2e} (ir_tol 1e-10), {e11:.2e} (1e-11)", flush=True)
         worst, worst11 = max(worst, e10), max(worst11, e11)
     assert worst <= 1e-9
