@@ -7,10 +7,14 @@ solve, Zin/S11 at every frequency) with the mesh and RWG tables already
 resident in HBM.  Two paths are timed:
   fp64  the plain method: per-frequency quadrature of every triangle pair
         (as the reference does) and an FP64 LU;
-  fast  the headline path: near and self pairs from frequency-independent
-        moments, the complex-symmetric Z factored in FP32 on the matrix cores
-        without pivoting (frequencies where that does not refine are redone
-        with the pivoted FP32 LU) and FP64 iterative refinement;
+  fast  the headline path: every triangle pair from its frequency-independent
+        series (integrated once per mesh, by the first sweep; Z of a whole
+        batch of frequencies formed in one pass), the complex-symmetric Z
+        factored in FP32 on the matrix cores without pivoting (frequencies
+        where that does not refine are redone with the pivoted FP32 LU, then
+        FP64) and FP64 iterative refinement;
+  fp64 series fill  the fast path's fill with the FP64 LU: the share of the
+        speed-up that the precision alone gives;
   surrogate  as fast, with the near pairs from the learned surrogate (trained
         on the GPU first, outside the timed sweeps).
 `value` is the fast path's frequency points per second; the JSON line also
@@ -53,9 +57,15 @@ def setup(config):
     return V, F, r, pe, ge, 1.5 * r.len.mean(), W.freqs(config)
 
 
-def timed(ctx, o, fr, pe, ge, spread, steps, warmup, name):
+def timed(ctx, o, fr, pe, ge, spread, steps, warmup, name, first=None):
+    """Time `steps` sweeps after `warmup` untimed ones; `first` (a dict)
+    receives the stats of the first warmup sweep, which carries the one-off
+    per-mesh work (the frequency-independent series) that later sweeps of
+    the same mesh reuse."""
     for i in range(warmup):
-        ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0, o)
+        st0 = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0, o, stats=True)[2]
+        if i == 0 and first is not None:
+            first.update(st0)
         log(f"[{name}] warmup {i + 1}/{warmup} done")
     t0 = time.perf_counter()
     out = None
@@ -154,15 +164,19 @@ def roofline(st, n, nf, fast, sym=False):
             desc = f"8*M*N*K flop per update ({lu_gemm_flops(n):.4e} per LU)"
         prec = "fp32" if fast else "fp64"
         ach = work / (ms * 1e-3) / 1e12
+        # the kernel forms each complex product from three real products
+        # (3M), so the matrix cores execute 6 M N K of the 8 M N K counted
         return {"kernel": f"k_gemm<{'float' if fast else 'double'}>", "bound": "mfma", "achieved": ach,
-                "peak": PEAK[prec], "unit": "TFLOP/s", "frac": ach / PEAK[prec], "traffic": traffic("k_gemm"),
+                "peak": PEAK[prec], "unit": "TFLOP/s", "frac": ach / PEAK[prec],
+                "mfma_busy_frac": 0.75 * ach / PEAK[prec],
+                "traffic": traffic("k_gemm" if fast else "k_gemm_f64"),
                 "work": f"{desc}, {work/cnt:.3e} flop/launch avg over {cnt} launches, "
                         f"{ms/cnt*1e3:.1f} us/launch avg"}
     if name in ("regular", "near", "self"):
         evals = st["n_green_" + name]
         work = evals * FLOP_PER_GREEN * nf
         ach = work / (ms * 1e-3) / 1e12
-        return {"kernel": "k_rec_" + name, "bound": "mfma", "achieved": ach, "peak": PEAK["fp64"],
+        return {"kernel": "k_rec_" + name, "bound": "fp64 valu", "achieved": ach, "peak": PEAK["fp64"],
                 "unit": "TFLOP/s", "frac": ach / PEAK["fp64"], "traffic": None,
                 "work": f"{evals:.3e} Green evaluations x {FLOP_PER_GREEN} flop per launch, {cnt} launches, "
                         f"{ms/cnt*1e3:.1f} us/launch avg (FP64 vector peak = FP64 matrix peak)"}
@@ -172,8 +186,31 @@ def roofline(st, n, nf, fast, sym=False):
         return {"kernel": "k_combine", "bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": ach / PEAK_HBM_GBS, "traffic": None,
                 "work": f"16*N^2 B of Z written per launch, {cnt} launches, {ms/cnt*1e3:.1f} us/launch avg"}
+    if name == "series":
+        return dict(kernel="k_series_fill", **series_rate(st, n, nf))
     return {"kernel": name, "bound": "mfma", "achieved": None, "peak": None, "unit": None, "frac": None,
             "traffic": None, "work": f"{ms:.1f} ms over {cnt} launches"}
+
+
+def series_rate(st, n, nf, Nt=None, z32=True):
+    """HBM rate of the all-pairs series fill (k_series_fill): per launch the
+    batch's Z matrices are written once (16 B FP64 + 8 B FP32 per entry)
+    and the series coefficients of every triangle pair read once
+    ((P+1) x 8 doubles per pair)."""
+    ks = st["kernels"]["series"]
+    P = st["moment_terms"] - 1
+    Nt = Nt or st.get("Nt")
+    per_f = (24.0 if z32 else 16.0) * n * n
+    q = 64.0 * (P + 1) * Nt * (Nt + 1) / 2
+    launches = ks["n"]
+    alg = (per_f * nf + q * launches) / launches
+    t = ks["ms"] * 1e-3 / launches
+    meas = traffic("k_series_fill")
+    return {"bound": "hbm", "achieved": alg / t / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": alg / t / 1e9 / PEAK_HBM_GBS, "traffic": meas,
+            "work": f"{nf / launches:.1f} frequencies per launch x {per_f:.3e} B of Z written + {q:.3e} B of series "
+                    f"read ({alg:.3e} B per launch), {t * 1e3:.2f} ms per launch, {t * 1e3 * launches / nf:.3f} ms "
+                    f"per frequency"}
 
 
 def fill_rates(st, n, Nt, nf):
@@ -184,6 +221,8 @@ def fill_rates(st, n, Nt, nf):
     calibrated counters measured for it)."""
     ks = st["kernels"]
     out = {}
+    if "series" in ks and ks["series"]["ms"] > 0:
+        out["k_series_fill"] = series_rate(st, n, nf, Nt)
     if "regular" in ks and ks["regular"]["ms"] > 0:
         work = st["n_green_regular"] * FLOP_PER_GREEN * nf
         ach = work / (ks["regular"]["ms"] * 1e-3) / 1e12
@@ -365,6 +404,9 @@ def main():
     ap.add_argument("--c4-workers", type=int, default=1,
                     help="host threads (one context each) sweeping c4 geometries side by side")
     ap.add_argument("--nf", type=int, default=0, help="use only the first NF frequencies (profiling runs)")
+    ap.add_argument("--profile-leg", choices=["fast", "fp64", "fp64-series"], default=None,
+                    help="only one warmup and one sweep of this leg, kernels serialised on one stream as in the "
+                         "profiled sweep (for kernel traces: rocprofv3)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run the plain copy kernel on this many MiB (counter calibration)")
     a = ap.parse_args()
@@ -388,6 +430,20 @@ def main():
     if a.nf:
         fr = fr[: a.nf]
     n = r.Ne
+    if a.profile_leg:
+        kw = {"fast": dict(precision=efie.PREC_MIXED, near_mode=efie.NEAR_MOMENTS, ir_tol=a.ir_tol,
+                           lu_kernel=a.lu_kernel, batch=a.batch),
+              "fp64": dict(precision=efie.PREC_FP64, near_mode=efie.NEAR_DIRECT),
+              "fp64-series": dict(precision=efie.PREC_FP64, near_mode=efie.NEAR_MOMENTS)}[a.profile_leg]
+        with efie.Context(0) as ctx:
+            ctx.set_problem(V, F, r)
+            # profile=1: every kernel on one stream, as the profiled sweep of
+            # the full run whose per-class times the roofline entries use
+            dt, (_, _, st) = timed(ctx, efie.opts(**W.OPTS_C1, **kw, profile=1), fr, pe, ge, spread, 1, 1,
+                                   a.profile_leg)
+        print(json.dumps({"leg": a.profile_leg, "config": a.config, "Ne": n, "Nf": len(fr), "s": dt,
+                          "freq_per_s": len(fr) / dt, "stats": st}))
+        return
     log(f"config {a.config}: Ne={n} Nt={F.shape[0]} Nf={len(fr)}")
     o64 = efie.opts(**W.OPTS_C1, precision=efie.PREC_FP64, near_mode=efie.NEAR_DIRECT)
     # refinement to ||V - Zx|| <= 1e-10 ||V||
@@ -395,12 +451,25 @@ def main():
                       lu_kernel=a.lu_kernel, batch=a.batch)
     with efie.Context(0) as ctx:
         ctx.set_problem(V, F, r)
-        dt_fast, (zin_f, s11_f, st_f) = timed(ctx, ofast, fr, pe, ge, spread, a.steps, a.warmup, "fast")
+        first = {}
+        dt_fast, (zin_f, s11_f, st_f) = timed(ctx, ofast, fr, pe, ge, spread, a.steps, a.warmup, "fast", first)
         res = {}
         if not a.no_fp64:
             dt_64, (zin_64, s11_64, st_64) = timed(ctx, o64, fr, pe, ge, spread, a.steps, a.warmup, "fp64")
+            _, _, st_64p = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0,
+                                     efie.opts(**W.OPTS_C1, precision=efie.PREC_FP64, near_mode=efie.NEAR_DIRECT,
+                                               profile=1), stats=True)
             res["fp64"] = {"value": len(fr) * a.steps / dt_64, "ms_per_step": 1e3 * dt_64 / a.steps,
-                           "stats": st_64}
+                           "stats": st_64, "profile": st_64p["kernels"],
+                           "roofline": roofline(st_64p, n, len(fr), False)}
+            # the same fill algorithm as the fast path (series), FP64 LU: the
+            # speed-up the precision alone gives
+            o64m = efie.opts(**W.OPTS_C1, precision=efie.PREC_FP64, near_mode=efie.NEAR_MOMENTS)
+            dt_64m, (zin_64m, _, _) = timed(ctx, o64m, fr, pe, ge, spread, a.steps, a.warmup, "fp64 series")
+            res["fp64_series_fill"] = {
+                "value": len(fr) * a.steps / dt_64m, "ms_per_step": 1e3 * dt_64m / a.steps,
+                "max_rel_err_zin_vs_fp64": float(np.max(np.abs(zin_64m - zin_64) / np.abs(zin_64)))}
+            log("fp64 legs done")
             e = np.abs(zin_f - zin_64) / np.abs(zin_64)
             res["max_rel_err_zin_fast_vs_fp64"] = float(np.max(e))
             # per-frequency errors: their spread and where the largest falls
@@ -445,6 +514,7 @@ def main():
             gpu_rows = Zg[baseline_rows(n, threads) - 1].copy()
             del Zg
     Nt = F.shape[0]
+    st_p["Nt"] = Nt
     reg_pairs = Nt * (Nt - 1) // 2 - st_p["n_near_pairs"]
     nq = {4: 6, 7: 7, 2: 3, 1: 1}.get(W.OPTS_C1["quad_order"], 6)
     st_p["n_green_regular"] = reg_pairs * nq * nq
@@ -459,8 +529,14 @@ def main():
                                f"{W.CONFIGS[a.config]['cells'][3]} cells, Ne={n}, Nt={Nt}, Nf={len(fr)} "
                                f"({fr[0]/1e9:.1f}-{fr[-1]/1e9:.1f} GHz), 7-pt regular rule, 6-pt x16 near/self"},
         "roofline": roofline(st_p, n, len(fr), True, sym=a.lu_kernel in (0, 3)),
-        "fast": {"stats": st_f},
+        "fast": {"stats": st_f,
+                 "one_off_per_mesh_ms": first.get("ms_moments", 0.0),
+                 "one_off_note": "the frequency-independent series of every triangle pair, integrated by the first "
+                                 "sweep of a mesh and reused by later sweeps of it (the timed steps); "
+                                 "value_incl_one_off charges it to every step"},
+        "value_incl_one_off": len(fr) / (dt_fast / a.steps + 1e-3 * first.get("ms_moments", 0.0)),
         "fast_vs_fp64": (value / res["fp64"]["value"]) if "fp64" in res else None,
+        "fast_vs_fp64_same_fill": (value / res["fp64_series_fill"]["value"]) if "fp64_series_fill" in res else None,
         **res,
         "profile": st_p["kernels"],
         "fill": fill_rates(st_p, n, Nt, len(fr)),

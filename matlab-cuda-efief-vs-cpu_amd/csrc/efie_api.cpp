@@ -255,7 +255,7 @@ struct efie_ctx {
     DBuf dTriV;  // triangle vertices relative to the centroid [Nt][9]
     // frequency-independent series of every triangle pair (near_mode
     // MOMENTS), valid for P = serP and near_alpha = serAlpha
-    DBuf dQall;
+    DBuf dQall, dSerC;  // pair series (transient) and per-entry coefficients
     int serP = -1;
     double serAlpha = -1;
     double radMax = 0;  // largest point radius of a triangle about its centroid
@@ -1178,26 +1178,33 @@ efie_status ensure_series(efie_ctx* c, const efie_opts* o, double kmax, efie_sta
     *ready = false;
     const int P0 = series_terms(c, o, kmax);
     if (P0 < 0) return ok();
-    if (c->serP >= P0 && c->serAlpha == o->near_alpha && c->dQall.p) {
+    if (c->serP >= P0 && c->serAlpha == o->near_alpha && c->dSerC.p) {
         *ready = true;
         return ok();
     }
     const int NT = series_nt(P0);
     if (NT == 0) return ok();
     const int P = NT - 3;
-    const size_t bytes = sizeof(double) * 8 * (size_t)(P + 1) * rec_index(0, c->Nt);
+    const size_t qBytes = sizeof(double) * 8 * (size_t)(P + 1) * rec_index(0, c->Nt);
+    const size_t cBytes = series_coeff_bytes(c->Ne, NT);
     size_t freeB = 0, totB = 0;
     HIPCHK(hipMemGetInfo(&freeB, &totB));
-    if (bytes > c->dQall.bytes && bytes - c->dQall.bytes > freeB / 3) return ok();
+    const size_t have = c->dSerC.bytes;
+    if (qBytes + cBytes > have && qBytes + cBytes - have > freeB / 3) return ok();
     hipStream_t s = c->stream;
-    HIPCHK(c->dQall.ensure(bytes));
+    c->serP = -1;
+    HIPCHK(c->dSerC.ensure(cBytes));
+    HIPCHK(c->dQall.ensure(qBytes));
     HIPCHK(hipEventRecord(c->ev[6], s));
     HIPCHK(launch_series(c->dReg.as<Pt>(), c->Nq, c->dTri.as<TriGeo>(), c->Nt, c->dNearMat.as<uint8_t>(),
                          c->dNearPts.as<Pt>(), c->NpNear, c->Nq2s, c->dAeq.as<double>(), o->near_alpha,
                          c->dList.as<int2>(), (int)c->nNear, c->dSelfPts.as<Pt>(), c->NsSelf, c->dASelf.as<double>(),
                          P, c->dQall.as<double>(), s));
+    HIPCHK(launch_series_coeffs(c->dQall.as<double>(), NT, c->dTri.as<TriGeo>(), c->dSideTri.as<int>(),
+                                c->dSide.as<Side>(), c->Ne, c->dSerC.as<double>(), s));
     HIPCHK(hipEventRecord(c->ev[7], s));
     HIPCHK(hipStreamSynchronize(s));
+    c->dQall.release();  // only the per-entry coefficients are kept
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, c->ev[6], c->ev[7]));
     if (st) st->ms_moments += ms;
@@ -1262,6 +1269,15 @@ efie_status ensure_centroid(efie_ctx* c, double selfAlpha) {
     return ok();
 }
 
+// h = -alpha / (4 pi k^2): with z = -jk, alpha = -eta z and the series
+// fill's -eta/(4 pi) e^{-jkR0} z^{-1} p(z) is h e^{-jkR0} p(z).
+void set_series_factor(Freq& f) {
+    const std::complex<double> k(f.kr, f.ki), a(f.ar, f.ai);
+    const std::complex<double> h = -a / (4.0 * kPi * k * k);
+    f.hr = h.real();
+    f.hi = h.imag();
+}
+
 Freq make_freq(std::complex<double> k, double omega, double mu, std::complex<double> eps) {
     Freq f;
     f.kr = k.real();
@@ -1273,6 +1289,7 @@ Freq make_freq(std::complex<double> k, double omega, double mu, std::complex<dou
     double d = jwe.real() * jwe.real() + jwe.imag() * jwe.imag();
     f.br = jwe.real() / d;
     f.bi = -jwe.imag() / d;
+    set_series_factor(f);
     return f;
 }
 
@@ -1305,8 +1322,8 @@ efie_status fill_z(efie_ctx* c, const efie_opts* o, const NearPlan& plan, const 
     }
     if (plan.series) {
         pf.begin(EFIE_K_SERIES);
-        HIPCHK(launch_series_fill(c->dQall.as<double>(), plan.serNT, plan.lossy, c->dTri.as<TriGeo>(),
-                                  c->dSideTri.as<int>(), c->dSide.as<Side>(), c->Ne, dfr, 1, Z, ld, 0, Z32, ld, 0, s));
+        HIPCHK(launch_series_fill(c->dSerC.as<double>(), plan.serNT, plan.lossy, c->Ne, dfr, 1, Z, ld, 0, Z32, ld,
+                                  0, s));
         pf.end(EFIE_K_SERIES);
         if (wrote32) *wrote32 = Z32 != nullptr;
         return ok();
@@ -1563,6 +1580,7 @@ efie_status fill_single(efie_ctx* c, const efie_opts& o, double k_re, double k_i
         fr.br = dr / d2;
         fr.bi = -di / d2;
     }
+    set_series_factor(fr);
     HIPCHK(c->dFreq.ensure(sizeof(Freq)));
     HIPCHK(hipMemcpyAsync(c->dFreq.p, &fr, sizeof fr, hipMemcpyHostToDevice, s));
     HIPCHK(c->dZ.ensure(sizeof(double) * 2 * (size_t)Ne * Ne));
@@ -1974,8 +1992,7 @@ efie_status efie_sweep(efie_ctx* c, const double* freqs, int Nf, double mu, doub
             // every frequency of the batch in one pass over the series
             Prof pff = make_prof(c);
             pff.begin(EFIE_K_SERIES);
-            HIPCHK(launch_series_fill(c->dQall.as<double>(), plan.serNT, plan.lossy, c->dTri.as<TriGeo>(),
-                                      c->dSideTri.as<int>(), c->dSide.as<Side>(), n, dfr, nb, Zb, (size_t)lda,
+            HIPCHK(launch_series_fill(c->dSerC.as<double>(), plan.serNT, plan.lossy, n, dfr, nb, Zb, (size_t)lda,
                                       2 * zElems, Z32b, (size_t)lda, 2 * zElems, sf));
             pff.end(EFIE_K_SERIES);
         } else {

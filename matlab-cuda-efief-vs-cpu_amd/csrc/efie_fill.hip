@@ -1206,28 +1206,21 @@ __global__ __launch_bounds__(256) void k_series_self(const Pt* __restrict__ self
                         Q + rec_index(t, t) * (size_t)(P + 1) * 8);
 }
 
-// Z of a batch of nf frequencies from the series: one block per 16 x 16
-// upper-triangular tile of Z, one entry per thread.  Each thread forms its
-// entry's D[0..NT-1] (NT = P + 3 orders z^-1 .. z^(P+1)) about the common
-// centre R0e once, then for every frequency evaluates
-//   Z(m,n) = -eta/(4 pi) e^{-jk R0e} z^{-1} sum_j z^j D[j]
-// and writes the tile and its mirror (through LDS) to matrix f of the
-// batch (FP64, and FP32 when Z32 is set).
-template <int NT, bool LOSSY>
-__global__ __launch_bounds__(256) void k_series_fill(const double* __restrict__ Q, const TriGeo* __restrict__ tg,
-                                                     const int* __restrict__ sideTri, const Side* __restrict__ side,
-                                                     int Ne, const Freq* __restrict__ fr, int nf,
-                                                     double* __restrict__ Z, size_t ld, size_t zStride,
-                                                     float* __restrict__ Z32, size_t ld32, size_t z32Stride,
-                                                     int nblk) {
+// Per-entry coefficients of the series (once per mesh and series length):
+// one block per 16 x 16 upper-triangular tile of Z, one entry per thread.
+// Each thread combines its entry's four pair series with the RWG side
+// coefficients into D[0..NT-1] (NT = P + 3 orders z^-1 .. z^(P+1)) about
+// the common centre R0e and stores them, with R0e, as
+// C[(tile * (NT + 1) + j) * 256 + thread] (coefficient-major per tile, so
+// the fill reads them with coalesced loads).
+template <int NT>
+__global__ __launch_bounds__(256) void k_series_coeffs(const double* __restrict__ Q, const TriGeo* __restrict__ tg,
+                                                       const int* __restrict__ sideTri,
+                                                       const Side* __restrict__ side, int Ne,
+                                                       double* __restrict__ C) {
     constexpr int P = NT - 3;
     constexpr size_t stride = (size_t)(P + 1) * 8;
-    __shared__ double2 tile[16][17];
-    long long id = blockIdx.x;
-    {
-        const int q = nblk >> 3, rr = nblk & 7, x = (int)(id & 7), loc = (int)(id >> 3);
-        id = x < rr ? (long long)x * (q + 1) + loc : (long long)rr * (q + 1) + (long long)(x - rr) * q + loc;
-    }
+    const long long id = blockIdx.x;
     int J = (int)((sqrt(8.0 * (double)id + 1.0) - 1.0) * 0.5);
     while ((long long)(J + 1) * (J + 2) / 2 <= id) ++J;
     while ((long long)J * (J + 1) / 2 > id) --J;
@@ -1308,36 +1301,70 @@ __global__ __launch_bounds__(256) void k_series_fill(const double* __restrict__ 
         }
     }
 
+    double* out = C + (size_t)id * (NT + 1) * 256 + threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < NT; ++j) out[(size_t)j * 256] = D[j];
+    out[(size_t)NT * 256] = R0e;
+}
+
+// Z of a batch of nf frequencies from the per-entry coefficients C (above):
+// one block per 16 x 16 upper-triangular tile, one entry per thread; for
+// every frequency
+//   Z(m,n) = -eta/(4 pi) e^{-jk R0e} z^{-1} sum_j z^j D[j]
+//          = h e^{-jk R0e} sum_j z^j D[j],  h = -alpha/(4 pi k^2) (Freq.hr/hi),
+// written with its mirror (through LDS) to matrix f of the batch (FP64, and
+// FP32 when Z32 is set).
+template <int NT, bool LOSSY>
+__global__ __launch_bounds__(256) void k_series_fill(const double* __restrict__ C, int Ne,
+                                                     const Freq* __restrict__ fr, int nf, double* __restrict__ Z,
+                                                     size_t ld, size_t zStride, float* __restrict__ Z32,
+                                                     size_t ld32, size_t z32Stride, int nblk) {
+    __shared__ double2 tile[16][17];
+    long long id = blockIdx.x;
+    {
+        const int q = nblk >> 3, rr = nblk & 7, x = (int)(id & 7), loc = (int)(id >> 3);
+        id = x < rr ? (long long)x * (q + 1) + loc : (long long)rr * (q + 1) + (long long)(x - rr) * q + loc;
+    }
+    int J = (int)((sqrt(8.0 * (double)id + 1.0) - 1.0) * 0.5);
+    while ((long long)(J + 1) * (J + 2) / 2 <= id) ++J;
+    while ((long long)J * (J + 1) / 2 > id) --J;
+    const int I = (int)(id - (long long)J * (J + 1) / 2);
+    const int r = threadIdx.x >> 4, cI = threadIdx.x & 15;
+    const int m = 16 * I + r, n = 16 * J + cI;
+    const double* in = C + (size_t)id * (NT + 1) * 256 + threadIdx.x;
+    double D[NT];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) D[j] = in[(size_t)j * 256];
+    const double R0e = in[(size_t)NT * 256];
+
     const bool upper = m < Ne && n < Ne && (I != J || r <= cI);
     const int mm = 16 * J + r, nn = 16 * I + cI;
     const bool lower = mm < Ne && nn < Ne && (I != J || r > cI);
     for (int f = 0; f < nf; ++f) {
         const Freq F = fr[f];
-        // z = -jk = ki - j kr; -eta/(4 pi) = -alpha/(j k 4 pi) = alpha/(4 pi z)
-        double vr, vi;
+        // p = sum_j z^j D[j], z = -jk = ki - j kr
+        double pr, pi;
         if (!LOSSY) {
-            // z = -j kr: sum_j z^j D[j] = A - j kr B with A, B even/odd
-            // polynomials in u = -kr^2; times 1/z = j/kr: B + j A/kr
+            // z = -j kr: p = A - j kr B with A, B the even and odd parts,
+            // polynomials in u = -kr^2
             const double u = -F.kr * F.kr;
             double A = D[NT - 1 - ((NT - 1) & 1)], B = D[NT - 1 - (NT & 1)];
 #pragma unroll
             for (int j = NT - 3 - ((NT - 1) & 1); j >= 0; j -= 2) A = A * u + D[j];
 #pragma unroll
             for (int j = NT - 3 - (NT & 1); j >= 1; j -= 2) B = B * u + D[j];
-            vr = B;
-            vi = A / F.kr;
+            pr = A;
+            pi = -F.kr * B;
         } else {
             const double zr = F.ki, zi = -F.kr;
-            double pr = D[NT - 1], pi = 0.0;
+            pr = D[NT - 1];
+            pi = 0.0;
 #pragma unroll
             for (int j = NT - 2; j >= 0; --j) {
                 const double tr = pr * zr - pi * zi + D[j];
                 pi = pr * zi + pi * zr;
                 pr = tr;
             }
-            const double z2 = zr * zr + zi * zi;
-            vr = (pr * zr + pi * zi) / z2;
-            vi = (pi * zr - pr * zi) / z2;
         }
         // e^{-jk R0e} = e^{ki R0e} (cos - j sin)(kr R0e)
         double s, c;
@@ -1347,16 +1374,9 @@ __global__ __launch_bounds__(256) void k_series_fill(const double* __restrict__ 
             s *= amp;
             c *= amp;
         }
-        const double wr = vr * c + vi * s, wi = vi * c - vr * s;
-        // alpha/(4 pi z)
-        double fr_, fi_;
-        {
-            const double zr = F.ki, zi = -F.kr, z2 = zr * zr + zi * zi;
-            const double ar = F.ar * (0.25 / kPi), ai = F.ai * (0.25 / kPi);
-            fr_ = (ar * zr + ai * zi) / z2;
-            fi_ = (ai * zr - ar * zi) / z2;
-        }
-        const double2 val = make_double2(fr_ * wr - fi_ * wi, fr_ * wi + fi_ * wr);
+        const double wr = pr * c + pi * s, wi = pi * c - pr * s;
+        // -eta/(4 pi) z^{-1} = -alpha/(4 pi k^2) = h
+        const double2 val = make_double2(F.hr * wr - F.hi * wi, F.hr * wi + F.hi * wr);
         double* Zf = Z + zStride * f;
         float* Z32f = Z32 ? Z32 + z32Stride * f : nullptr;
         tile[r][cI] = val;
@@ -1668,20 +1688,41 @@ int series_nt(int P) {
     return 0;
 }
 
-hipError_t launch_series_fill(const double* Q, int NT, bool lossy, const TriGeo* tg, const int* sideTri,
-                              const Side* side, int Ne, const Freq* fr, int nf, double* Z, size_t ld,
-                              size_t zStride, float* Z32, size_t ld32, size_t z32Stride, hipStream_t s) {
+size_t series_coeff_bytes(int Ne, int NT) {
+    const long long T = (Ne + 15) / 16;
+    return sizeof(double) * (size_t)(T * (T + 1) / 2) * (size_t)(NT + 1) * 256;
+}
+
+hipError_t launch_series_coeffs(const double* Q, int NT, const TriGeo* tg, const int* sideTri, const Side* side,
+                                int Ne, double* C, hipStream_t s) {
+    const int T = (Ne + 15) / 16;
+    const long long nb = (long long)T * (T + 1) / 2;
+    const dim3 g((unsigned)nb), blk(256);
+    switch (NT) {
+        case 10: hipLaunchKernelGGL(k_series_coeffs<10>, g, blk, 0, s, Q, tg, sideTri, side, Ne, C); break;
+        case 12: hipLaunchKernelGGL(k_series_coeffs<12>, g, blk, 0, s, Q, tg, sideTri, side, Ne, C); break;
+        case 14: hipLaunchKernelGGL(k_series_coeffs<14>, g, blk, 0, s, Q, tg, sideTri, side, Ne, C); break;
+        case 16: hipLaunchKernelGGL(k_series_coeffs<16>, g, blk, 0, s, Q, tg, sideTri, side, Ne, C); break;
+        case 18: hipLaunchKernelGGL(k_series_coeffs<18>, g, blk, 0, s, Q, tg, sideTri, side, Ne, C); break;
+        case 20: hipLaunchKernelGGL(k_series_coeffs<20>, g, blk, 0, s, Q, tg, sideTri, side, Ne, C); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_series_fill(const double* C, int NT, bool lossy, int Ne, const Freq* fr, int nf, double* Z,
+                              size_t ld, size_t zStride, float* Z32, size_t ld32, size_t z32Stride, hipStream_t s) {
     const int T = (Ne + 15) / 16;
     const long long nb = (long long)T * (T + 1) / 2;
     const dim3 g((unsigned)nb), blk(256);
 #define EFIE_SER_CASE(N)                                                                                         \
     case N:                                                                                                       \
         if (lossy)                                                                                                \
-            hipLaunchKernelGGL((k_series_fill<N, true>), g, blk, 0, s, Q, tg, sideTri, side, Ne, fr, nf, Z, ld,   \
-                               zStride, Z32, ld32, z32Stride, (int)nb);                                           \
+            hipLaunchKernelGGL((k_series_fill<N, true>), g, blk, 0, s, C, Ne, fr, nf, Z, ld, zStride, Z32, ld32,  \
+                               z32Stride, (int)nb);                                                               \
         else                                                                                                      \
-            hipLaunchKernelGGL((k_series_fill<N, false>), g, blk, 0, s, Q, tg, sideTri, side, Ne, fr, nf, Z, ld,  \
-                               zStride, Z32, ld32, z32Stride, (int)nb);                                           \
+            hipLaunchKernelGGL((k_series_fill<N, false>), g, blk, 0, s, C, Ne, fr, nf, Z, ld, zStride, Z32, ld32, \
+                               z32Stride, (int)nb);                                                               \
         break;
     switch (NT) {
         EFIE_SER_CASE(10)

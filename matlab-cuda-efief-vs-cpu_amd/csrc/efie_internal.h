@@ -48,6 +48,7 @@ struct Freq {
     double kr, ki;       // k = kr + j ki
     double ar, ai;       // alpha = j omega mu
     double br, bi;       // beta = 1/(j omega eps)
+    double hr, hi;       // -alpha / (4 pi k^2): the series fill's factor
 };
 
 // --------------------------------------------------------- launchers ----
@@ -99,13 +100,18 @@ hipError_t launch_series(const Pt* regPts, int Nq, const TriGeo* tg, int Nt, con
 // Number of series coefficients per entry (P + 3, rounded up to a size the
 // fill kernel is built for: 10..20, even) or 0 if P is too large.
 int series_nt(int P);
-// Z of nf frequencies from the series (P = NT - 3): matrix f at
-// Z + f*zStride, element (m,n) at 2*(m*ld + n), and its complex64 copy at
+// Per-entry coefficients of the upper-triangular 16 x 16 tiles of Z from
+// the pair series Q (P = NT - 3): NT coefficients and the entry's phase
+// centre, series_coeff_bytes(Ne, NT) bytes.
+size_t series_coeff_bytes(int Ne, int NT);
+hipError_t launch_series_coeffs(const double* Q, int NT, const TriGeo* tg, const int* sideTri, const Side* side,
+                                int Ne, double* C, hipStream_t s);
+// Z of nf frequencies from the coefficients: matrix f at Z + f*zStride,
+// element (m,n) at 2*(m*ld + n), and its complex64 copy at
 // Z32 + f*z32Stride when Z32 is set.  lossy: some k has a nonzero
 // imaginary part.
-hipError_t launch_series_fill(const double* Q, int NT, bool lossy, const TriGeo* tg, const int* sideTri,
-                              const Side* side, int Ne, const Freq* fr, int nf, double* Z, size_t ld,
-                              size_t zStride, float* Z32, size_t ld32, size_t z32Stride, hipStream_t s);
+hipError_t launch_series_fill(const double* C, int NT, bool lossy, int Ne, const Freq* fr, int nf, double* Z,
+                              size_t ld, size_t zStride, float* Z32, size_t ld32, size_t z32Stride, hipStream_t s);
 
 // Z(m,n) = sum over the four side pairs of alpha*Ivec + beta*Isca.
 // Output element (m,n) goes to Z[m*ld + n] (interleaved complex; Z is
