@@ -291,6 +291,24 @@ efie_status efie_zgesv(efie_ctx* ctx, int n, int batch, const double* A,
                        const double* b, int precision, double* x,
                        efie_stats* stats);
 
+/* No counterpart in the reference.  efie_zgesv for callers that cannot
+ * allocate device memory: code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc.  Same arguments, results, error
+ * behaviour and FP64 fallback as efie_zgesv; it calls no hipMalloc,
+ * hipFree or other allocating routine: every device buffer of the solve is
+ * carved from one static device array of EFIE_NOALLOC_SCRATCH_BYTES per
+ * GPU, which each GPU the library runs on holds until the process ends.
+ * The limit: batch <= 64 and the solve's buffers (about 16 n (n + 32)
+ * batch bytes for FP64, 24 n (n + 32) batch for MIXED, plus O(n batch))
+ * within that array, so n up to about 1900 for one FP64 system; larger
+ * calls return EFIE_E_ARGS.  All calls on one GPU share the array and it
+ * keeps the last call's contents (the solve clears what must start at
+ * zero): calls that overlap on one GPU, from any context, race. */
+#define EFIE_NOALLOC_SCRATCH_BYTES (64u << 20)
+efie_status efie_zgesv_noalloc(efie_ctx* ctx, int n, int batch, const double* A,
+                               const double* b, int precision, double* x,
+                               efie_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

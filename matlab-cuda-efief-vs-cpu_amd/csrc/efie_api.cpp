@@ -282,6 +282,8 @@ struct efie_ctx {
     DBuf dFlags, dTickets, dErr;
     DBuf dSplit;  // bf16 planes of the split-bf16 LU update (lu_kernel BF16X6)
     TrsvWork tw{nullptr, nullptr, nullptr, 0};
+    // this GPU's copy of the allocation-free solve's scratch array
+    void* scratch = nullptr;
 
     // profiling (opts.profile): event pairs per kernel class
     bool prof = false;
@@ -534,6 +536,11 @@ efie_status efie_create(efie_ctx** out, int device) {
             efie_destroy(c);
             return fail(EFIE_E_HIP, "efie:hip", "hipEventCreate: %s", hipGetErrorString(e));
         }
+    }
+    e = noalloc_scratch(device, &c->scratch);
+    if (e != hipSuccess) {
+        efie_destroy(c);
+        return fail(EFIE_E_HIP, "efie:hip", "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
     }
     *out = c;
     return ok();
@@ -1617,7 +1624,7 @@ int batch_size(const efie_opts& o, size_t zElems, int Nf) {
 
 // Host bytes of one batch's results: info and iterations (int), final
 // relative residuals (double), the wait-limit flag (int).
-size_t stat_slot_bytes(int B) { return (size_t)B * (2 * sizeof(int) + sizeof(double)) + 64; }
+constexpr size_t stat_slot_bytes(int B) { return (size_t)B * (2 * sizeof(int) + sizeof(double)) + 64; }
 
 // Device buffers of the batched solver for n unknowns and B matrices.
 efie_status ensure_solver(efie_ctx* c, const efie_opts& o, int n, int lda, int B, int nbuf = 1) {
@@ -1675,13 +1682,50 @@ efie_status ensure_solver(efie_ctx* c, const efie_opts& o, int n, int lda, int B
     return ok();
 }
 
+// Device work buffers of the batched solver besides the matrices: pivot
+// tournament (candidates, arrival counters, pivots, swap lists), the
+// split-bf16 planes, refinement vectors and per-matrix state, the
+// triangular solve's flags.
+struct SolverWork {
+    int *cand, *cnt, *ipiv, *swp, *err;
+    void* split;
+    double *X, *R, *P, *Rn2, *Bn2, *resid;
+    float* R32;
+    int *done, *iters;
+    TrsvWork* tw;
+};
+
+SolverWork solver_work(efie_ctx* c) {
+    SolverWork w;
+    w.cand = c->dCand.as<int>();
+    w.cnt = c->dCnt.as<int>();
+    w.ipiv = c->dIpiv.as<int>();
+    w.swp = c->dSwp.as<int>();
+    w.err = c->dErr.as<int>();
+    w.split = c->dSplit.p;
+    w.X = c->dX.as<double>();
+    w.R = c->dR.as<double>();
+    w.P = c->dP.as<double>();
+    w.Rn2 = c->dRn2.as<double>();
+    w.Bn2 = c->dBn2.as<double>();
+    w.resid = c->dResid.as<double>();
+    w.R32 = c->dR32.as<float>();
+    w.done = c->dDone.as<int>();
+    w.iters = c->dIters.as<int>();
+    w.tw = &c->tw;
+    return w;
+}
+
 // Buffers of one batch: matrices (row-major, rhs in column n), the FP32
-// copy (mixed precision), first-zero-pivot flags and row permutations.
+// copy (mixed precision), first-zero-pivot flags, row permutations, the
+// solver's work buffers and the host slot its results are copied to.
 struct BatchBufs {
     double* Z;
     float* Z32;
     int* info;
     int* perm;
+    SolverWork w;
+    char* host;
 };
 
 BatchBufs batch_bufs(efie_ctx* c, const efie_opts& o, int n, int lda, int B, int buf) {
@@ -1691,6 +1735,8 @@ BatchBufs batch_bufs(efie_ctx* c, const efie_opts& o, int n, int lda, int B, int
     bb.Z32 = o.precision == EFIE_PREC_MIXED ? c->dZ32.as<float>() + 2 * zElems * (size_t)B * buf : nullptr;
     bb.info = c->dInfo.as<int>() + (size_t)B * buf;
     bb.perm = o.precision == EFIE_PREC_MIXED ? c->dPerm.as<int>() + (size_t)B * n * buf : nullptr;
+    bb.w = solver_work(c);
+    bb.host = static_cast<char*>(c->hStat) + stat_slot_bytes(B) * buf;
     return bb;
 }
 
@@ -1704,9 +1750,9 @@ efie_status factor_batch(efie_ctx* c, const efie_opts& o, int n, int lda, int nb
     const size_t zElems = (size_t)n * lda;
     // look-ahead on the panel stream, except when per-kernel profiling
     // needs every kernel on one stream
-    LuWork wk{c->dCand.as<int>(), c->dCnt.as<int>(), c->dIpiv.as<int>(), bb.info,
-              c->dSwp.as<int>(), c->prof ? nullptr : c->panelStream, c->evPanel, c->evAhead,
-              (o.precision == EFIE_PREC_MIXED && o.lu_kernel == EFIE_LU_BF16X6) ? c->dSplit.p : nullptr,
+    LuWork wk{bb.w.cand, bb.w.cnt, bb.w.ipiv, bb.info,
+              bb.w.swp, c->prof ? nullptr : c->panelStream, c->evPanel, c->evAhead,
+              (o.precision == EFIE_PREC_MIXED && o.lu_kernel == EFIE_LU_BF16X6) ? bb.w.split : nullptr,
               o.precision == EFIE_PREC_MIXED && o.lu_kernel == EFIE_LU_SYM};
     HIPCHK(launch_fill_int(bb.info, 0, nb, s));
     if (o.precision != EFIE_PREC_MIXED) {
@@ -1720,7 +1766,7 @@ efie_status factor_batch(efie_ctx* c, const efie_opts& o, int n, int lda, int nb
     }
     HIPCHK(lu_factor<float>(bb.Z32, 2 * zElems, n, lda, nb, wk, s, &pf));
     pf.begin(EFIE_K_OTHER);
-    HIPCHK(lu_make_perm(c->dIpiv.as<int>(), n, nb, bb.perm, s));
+    HIPCHK(lu_make_perm(bb.w.ipiv, n, nb, bb.perm, s));
     pf.end(EFIE_K_OTHER);
     return ok();
 }
@@ -1735,9 +1781,10 @@ efie_status refine_batch(efie_ctx* c, const efie_opts& o, int n, int lda, int nb
                          bool symZ, const Prof& pf, const BatchBufs& bb, hipStream_t si, const double** X,
                          size_t* xStride, size_t* xinc) {
     const size_t zElems = (size_t)n * lda;
+    const SolverWork& w = bb.w;
     if (o.precision != EFIE_PREC_MIXED) {
         pf.begin(EFIE_K_TRSV);
-        HIPCHK(lu_backsolve<double>(bb.Z, 2 * zElems, n, lda, nb, si, &c->tw));
+        HIPCHK(lu_backsolve<double>(bb.Z, 2 * zElems, n, lda, nb, si, w.tw));
         pf.end(EFIE_K_TRSV);
         *X = bb.Z + 2 * n;
         *xStride = 2 * zElems;
@@ -1748,50 +1795,50 @@ efie_status refine_batch(efie_ctx* c, const efie_opts& o, int n, int lda, int nb
     if (o.lu_kernel == EFIE_LU_SYM) {
         // no rhs column: the first refinement step from x = 0 is the solve
         pf.begin(EFIE_K_OTHER);
-        HIPCHK(launch_fill_dbl(c->dX.as<double>(), 0.0, 2 * n * nb, si));
+        HIPCHK(launch_fill_dbl(w.X, 0.0, 2 * n * nb, si));
         pf.end(EFIE_K_OTHER);
     } else {
         pf.begin(EFIE_K_TRSV);
-        HIPCHK(lu_backsolve<float>(Z32, 2 * zElems, n, lda, nb, si, &c->tw));
+        HIPCHK(lu_backsolve<float>(Z32, 2 * zElems, n, lda, nb, si, w.tw));
         pf.end(EFIE_K_TRSV);
         pf.begin(EFIE_K_OTHER);
-        HIPCHK(launch_col_to_vec(Z32, 2 * zElems, n, lda, c->dX.as<double>(), nb, si));
+        HIPCHK(launch_col_to_vec(Z32, 2 * zElems, n, lda, w.X, nb, si));
         pf.end(EFIE_K_OTHER);
     }
     pf.begin(EFIE_K_OTHER);
-    HIPCHK(launch_fill_int(c->dDone.as<int>(), 0, nb, si));
-    HIPCHK(launch_fill_int(c->dIters.as<int>(), 0, nb, si));
-    HIPCHK(launch_fill_dbl(c->dRn2.as<double>(), 0.0, nb, si));
-    HIPCHK(launch_fill_dbl(c->dResid.as<double>(), 1.0, nb, si));  // never a stale value from an older call
+    HIPCHK(launch_fill_int(w.done, 0, nb, si));
+    HIPCHK(launch_fill_int(w.iters, 0, nb, si));
+    HIPCHK(launch_fill_dbl(w.Rn2, 0.0, nb, si));
+    HIPCHK(launch_fill_dbl(w.resid, 1.0, nb, si));  // never a stale value from an older call
     pf.end(EFIE_K_OTHER);
     for (int it = 0; it <= o.ir_max_iter; ++it) {
         pf.begin(EFIE_K_OTHER);
         if (symZ)
-            HIPCHK(launch_residual_sym(bb.Z, 2 * zElems, n, lda, V, vStride, c->dX.as<double>(),
-                                       c->dR.as<double>(), c->dP.as<double>(), c->dRn2.as<double>(),
-                                       c->dDone.as<int>(), nb, it == 0 && o.lu_kernel == EFIE_LU_SYM, si));
+            HIPCHK(launch_residual_sym(bb.Z, 2 * zElems, n, lda, V, vStride, w.X,
+                                       w.R, w.P, w.Rn2,
+                                       w.done, nb, it == 0 && o.lu_kernel == EFIE_LU_SYM, si));
         else
-            HIPCHK(launch_residual(bb.Z, 2 * zElems, n, lda, V, vStride, c->dX.as<double>(), c->dR.as<double>(),
-                                   c->dRn2.as<double>(), c->dDone.as<int>(), nb, si));
-        HIPCHK(launch_ir_check(c->dRn2.as<double>(), c->dBn2.as<double>(), o.ir_tol, nb, c->dDone.as<int>(),
-                               c->dIters.as<int>(), c->dResid.as<double>(), it == 0, it == o.ir_max_iter,
+            HIPCHK(launch_residual(bb.Z, 2 * zElems, n, lda, V, vStride, w.X, w.R,
+                                   w.Rn2, w.done, nb, si));
+        HIPCHK(launch_ir_check(w.Rn2, w.Bn2, o.ir_tol, nb, w.done,
+                               w.iters, w.resid, it == 0, it == o.ir_max_iter,
                                o.lu_kernel == EFIE_LU_SYM ? o.ir_max_iter - it : 0, si));
         pf.end(EFIE_K_OTHER);
         if (it == o.ir_max_iter) break;
         pf.begin(EFIE_K_OTHER);
-        HIPCHK(launch_perm_f32(c->dR.as<double>(), bb.perm, n, nb, c->dR32.as<float>(), c->dDone.as<int>(), si));
+        HIPCHK(launch_perm_f32(w.R, bb.perm, n, nb, w.R32, w.done, si));
         pf.end(EFIE_K_OTHER);
         pf.begin(EFIE_K_TRSV);
-        HIPCHK(lu_trsv<float>(Z32, 2 * zElems, n, lda, c->dR32.as<float>(), 2 * (size_t)n, 1, nb, true,
-                              c->dDone.as<int>(), si, &c->tw));
-        HIPCHK(lu_trsv<float>(Z32, 2 * zElems, n, lda, c->dR32.as<float>(), 2 * (size_t)n, 1, nb, false,
-                              c->dDone.as<int>(), si, &c->tw));
+        HIPCHK(lu_trsv<float>(Z32, 2 * zElems, n, lda, w.R32, 2 * (size_t)n, 1, nb, true,
+                              w.done, si, w.tw));
+        HIPCHK(lu_trsv<float>(Z32, 2 * zElems, n, lda, w.R32, 2 * (size_t)n, 1, nb, false,
+                              w.done, si, w.tw));
         pf.end(EFIE_K_TRSV);
         pf.begin(EFIE_K_OTHER);
-        HIPCHK(launch_add_f32(c->dX.as<double>(), c->dR32.as<float>(), n, nb, c->dDone.as<int>(), si));
+        HIPCHK(launch_add_f32(w.X, w.R32, n, nb, w.done, si));
         pf.end(EFIE_K_OTHER);
     }
-    *X = c->dX.as<double>();
+    *X = w.X;
     *xStride = 2 * (size_t)n;
     *xinc = 2;
     return ok();
@@ -1799,18 +1846,17 @@ efie_status refine_batch(efie_ctx* c, const efie_opts& o, int n, int lda, int nb
 
 // Queue the copy of a batch's results into host slot `slot` (pinned, so
 // the copy is asynchronous) on stream si.
-efie_status stage_stats(efie_ctx* c, const efie_opts& o, int nb, int slot, int B, const BatchBufs& bb,
-                        hipStream_t si) {
-    char* h = static_cast<char*>(c->hStat) + stat_slot_bytes(B) * slot;
+efie_status stage_stats(const efie_opts& o, int nb, int B, const BatchBufs& bb, hipStream_t si) {
+    char* h = bb.host;
     int* info = reinterpret_cast<int*>(h);
     int* iters = info + B;
     double* resid = reinterpret_cast<double*>(h + 2 * sizeof(int) * (size_t)B);
     int* werr = reinterpret_cast<int*>(h + (2 * sizeof(int) + sizeof(double)) * (size_t)B);
     HIPCHK(hipMemcpyAsync(info, bb.info, sizeof(int) * nb, hipMemcpyDeviceToHost, si));
-    HIPCHK(hipMemcpyAsync(werr, c->dErr.p, sizeof(int), hipMemcpyDeviceToHost, si));
+    HIPCHK(hipMemcpyAsync(werr, bb.w.err, sizeof(int), hipMemcpyDeviceToHost, si));
     if (o.precision == EFIE_PREC_MIXED) {
-        HIPCHK(hipMemcpyAsync(iters, c->dIters.p, sizeof(int) * nb, hipMemcpyDeviceToHost, si));
-        HIPCHK(hipMemcpyAsync(resid, c->dResid.p, sizeof(double) * nb, hipMemcpyDeviceToHost, si));
+        HIPCHK(hipMemcpyAsync(iters, bb.w.iters, sizeof(int) * nb, hipMemcpyDeviceToHost, si));
+        HIPCHK(hipMemcpyAsync(resid, bb.w.resid, sizeof(double) * nb, hipMemcpyDeviceToHost, si));
     }
     return ok();
 }
@@ -1824,9 +1870,10 @@ efie_status stage_stats(efie_ctx* c, const efie_opts& o, int nb, int slot, int B
 // the batch) instead of failing; the caller solves them again, the
 // unpivoted symmetric form with the pivoted FP32 LU, the pivoted forms with
 // the FP64 LU.
-efie_status finish_batch(efie_ctx* c, const efie_opts& o, int nb, int slot, int B, efie_stats* S,
-                         const double* freqs, std::vector<int>* retry = nullptr) {
-    const char* h = static_cast<const char*>(c->hStat) + stat_slot_bytes(B) * slot;
+efie_status finish_batch(efie_ctx* c, const efie_opts& o, int nb, const BatchBufs& bb, int B, efie_stats* S,
+                         const double* freqs, int* retry = nullptr, int* nretry = nullptr) {
+    if (nretry) *nretry = 0;
+    const char* h = bb.host;
     const int* info = reinterpret_cast<const int*>(h);
     const int* iters = info + B;
     const double* resid = reinterpret_cast<const double*>(h + 2 * sizeof(int) * (size_t)B);
@@ -1834,13 +1881,13 @@ efie_status finish_batch(efie_ctx* c, const efie_opts& o, int nb, int slot, int 
     const bool mixed = o.precision == EFIE_PREC_MIXED;
     prof_collect(c, S);
     if (werr) {
-        HIPCHK(launch_fill_int(c->dErr.as<int>(), 0, 1, c->stream));
+        HIPCHK(launch_fill_int(bb.w.err, 0, 1, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         return fail(EFIE_E_HIP, "efie:hip", "triangular solve: a block wait exceeded its limit");
     }
     for (int i = 0; i < nb; ++i) {
         if (retry && mixed && (info[i] != 0 || !(resid[i] <= std::max(o.ir_tol, 1e-11)))) {
-            retry->push_back(i);
+            retry[(*nretry)++] = i;
             continue;
         }
         if (info[i] != 0) {
@@ -2037,10 +2084,12 @@ efie_status efie_sweep(efie_ctx* c, const double* freqs, int Nf, double mu, doub
         msFill += t1;
         msFac += t2;
         msSolve += t3;
-        if (!symRetry) return finish_batch(c, o, nb, buf, B, S, freqs + f0);
-        std::vector<int> rb;
-        efie_status e = finish_batch(c, o, nb, buf, B, S, freqs + f0, &rb);
-        for (int i : rb) retry.push_back(f0 + i);
+        const BatchBufs bbj = batch_bufs(c, o, n, lda, B, buf);
+        if (!symRetry) return finish_batch(c, o, nb, bbj, B, S, freqs + f0);
+        std::vector<int> rb(nb);
+        int nrb = 0;
+        efie_status e = finish_batch(c, o, nb, bbj, B, S, freqs + f0, rb.data(), &nrb);
+        for (int i = 0; i < nrb; ++i) retry.push_back(f0 + rb[i]);
         return e;
     };
     st = enqueue_fill(0);
@@ -2065,7 +2114,7 @@ efie_status efie_sweep(efie_ctx* c, const double* freqs, int Nf, double mu, doub
                                o.model == EFIE_MODEL_CENTROID, V0, Z0, nb, c->dOut.as<double>() + 4 * (size_t)f0,
                                si));
         pf.end(EFIE_K_OTHER);
-        st = stage_stats(c, o, nb, buf, B, bb, si);
+        st = stage_stats(o, nb, B, bb, si);
         if (st) return st;
         HIPCHK(hipEventRecord(c->evIR[buf], si));
         if (overlap) {
@@ -2217,12 +2266,15 @@ efie_status efie_zgesv(efie_ctx* c, int n, int batch, const double* A, const dou
     st = refine_batch(c, o, n, lda, batch, c->dB.as<double>(), 2 * (size_t)n, false, pf, bb, s, &X, &xStride,
                        &xinc);
     if (st) return st;
-    st = stage_stats(c, o, batch, 0, batch, bb, s);
+    st = stage_stats(o, batch, batch, bb, s);
     if (st) return st;
     HIPCHK(hipEventRecord(c->ev[4], s));
     HIPCHK(hipStreamSynchronize(s));
-    std::vector<int> redo;
-    st = finish_batch(c, o, batch, 0, batch, S, nullptr, precision == EFIE_PREC_MIXED ? &redo : nullptr);
+    std::vector<int> redo(batch);
+    int nredo = 0;
+    st = finish_batch(c, o, batch, bb, batch, S, nullptr, precision == EFIE_PREC_MIXED ? redo.data() : nullptr,
+                      &nredo);
+    redo.resize(nredo);
     if (st) return st;
     for (int q = 0; q < batch; ++q)
         HIPCHK(hipMemcpy2DAsync(x + 2 * (size_t)n * q, sizeof(double) * 2, X + xStride * q, sizeof(double) * xinc,
@@ -2246,6 +2298,132 @@ efie_status efie_zgesv(efie_ctx* c, int n, int batch, const double* A, const dou
         S->ms_total += S2.ms_total;
     }
     S->n_fp64_retry = (int)redo.size();
+    return ok();
+}
+
+efie_status efie_zgesv_noalloc(efie_ctx* c, int n, int batch, const double* A, const double* b, int precision,
+                               double* x, efie_stats* stats) {
+    if (!c || !A || !b || !x) return fail(EFIE_E_ARGS, "efie:args", "null argument");
+    if (n < 1 || batch < 1) return fail(EFIE_E_ARGS, "efie:args", "n and batch must be >= 1");
+    if (batch > kNoAllocMaxBatch)
+        return fail(EFIE_E_ARGS, "efie:args", "batch %d exceeds the allocation-free solve's %d", batch,
+                    kNoAllocMaxBatch);
+    if (precision != EFIE_PREC_FP64 && precision != EFIE_PREC_MIXED)
+        return fail(EFIE_E_ARGS, "efie:opts", "unknown precision");
+    if (!c->scratch) return fail(EFIE_E_STATE, "efie:state", "no scratch array on this context's device");
+    HIPCHK(hipSetDevice(c->device));
+    efie_opts o;
+    efie_opts_default(&o);
+    o.precision = precision;
+    o.lu_kernel = EFIE_LU_FP32;  // general matrices: pivoted
+    const bool mixed = precision == EFIE_PREC_MIXED;
+    efie_stats local;
+    efie_stats* S = stats ? stats : &local;
+    std::memset(S, 0, sizeof *S);
+    const int lda = ((n + 1) + 31) & ~31;
+    const size_t zElems = (size_t)n * lda;
+    const size_t nb = (size_t)batch, nn = (size_t)n;
+    // every buffer of the solve, carved from the scratch array (256-byte aligned)
+    char* base = static_cast<char*>(c->scratch);
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        off = (off + 255) & ~(size_t)255;
+        char* p = base + off;
+        off += bytes;
+        return static_cast<void*>(p);
+    };
+    const int nodes = lu_tree_nodes(n);
+    double* Z = static_cast<double*>(take(16 * zElems * nb));
+    float* Z32 = mixed ? static_cast<float*>(take(8 * zElems * nb)) : nullptr;
+    double* dB = static_cast<double*>(take(16 * nn * nb));
+    SolverWork w{};
+    w.cand = static_cast<int*>(take(sizeof(int) * nb * nodes * lu_cand_stride()));
+    w.cnt = static_cast<int*>(take(sizeof(int) * nb * nodes));
+    w.ipiv = static_cast<int*>(take(sizeof(int) * nb * nn));
+    int* info = static_cast<int*>(take(sizeof(int) * nb));
+    w.swp = static_cast<int*>(take(sizeof(int) * nb * lu_swap_stride()));
+    int* perm = mixed ? static_cast<int*>(take(sizeof(int) * nb * nn)) : nullptr;
+    w.X = static_cast<double*>(take(16 * nb * nn));
+    w.R = static_cast<double*>(take(16 * nb * nn));
+    w.R32 = static_cast<float*>(take(8 * nb * nn));
+    w.Rn2 = static_cast<double*>(take(sizeof(double) * nb));
+    w.Bn2 = static_cast<double*>(take(sizeof(double) * nb));
+    w.resid = static_cast<double*>(take(sizeof(double) * nb));
+    w.done = static_cast<int*>(take(sizeof(int) * nb));
+    w.iters = static_cast<int*>(take(sizeof(int) * nb));
+    const size_t nflags = nb * (size_t)((n + 63) / 64);
+    int* flags = static_cast<int*>(take(sizeof(int) * nflags));
+    int* tickets = static_cast<int*>(take(sizeof(int) * nb));
+    w.err = static_cast<int*>(take(sizeof(int)));
+    if (off > kNoAllocScratchBytes)
+        return fail(EFIE_E_ARGS, "efie:args",
+                    "n = %d, batch = %d needs %zu bytes; the allocation-free solve has %zu (use efie_zgesv)", n,
+                    batch, off, kNoAllocScratchBytes);
+    // the array keeps the previous call's contents: clear what must start
+    // at zero (matrix padding, tournament counters, solve flags, error word)
+    hipStream_t s = c->stream;
+    HIPCHK(launch_fill_dbl(Z, 0.0, (int)(2 * zElems * nb), s));
+    HIPCHK(launch_fill_int(w.cnt, 0, (int)(nb * nodes), s));
+    HIPCHK(launch_fill_int(flags, 0, (int)nflags, s));
+    HIPCHK(launch_fill_int(w.err, 0, 1, s));
+    TrsvWork tw{flags, tickets, w.err, 0};
+    w.tw = &tw;
+    // column-major A (A^T row-major) into the rows of Z, transposed in
+    // place; the right-hand side into column n
+    double bn2[kNoAllocMaxBatch];
+    for (int q = 0; q < batch; ++q) {
+        double* Zq = Z + 2 * zElems * q;
+        HIPCHK(hipMemcpy2DAsync(Zq, 16 * (size_t)lda, A + 2 * nn * nn * q, 16 * nn, 16 * nn, nn,
+                                hipMemcpyHostToDevice, s));
+        HIPCHK(launch_transpose_square(Zq, n, (size_t)lda, s));
+        HIPCHK(hipMemcpy2DAsync(Zq + 2 * nn, 16 * (size_t)lda, b + 2 * nn * q, 16, 16, nn, hipMemcpyHostToDevice, s));
+        double v = 0;
+        for (size_t i = 0; i < 2 * nn; ++i) v += b[2 * nn * q + i] * b[2 * nn * q + i];
+        bn2[q] = v;
+    }
+    HIPCHK(hipMemcpyAsync(dB, b, 16 * nn * nb, hipMemcpyHostToDevice, s));
+    if (mixed) HIPCHK(hipMemcpyAsync(w.Bn2, bn2, sizeof(double) * nb, hipMemcpyHostToDevice, s));
+    alignas(16) char host[stat_slot_bytes(kNoAllocMaxBatch)];
+    BatchBufs bb{Z, Z32, info, perm, w, host};
+    Prof pf = make_prof(c);
+    HIPCHK(hipEventRecord(c->ev[2], s));
+    const double* X = nullptr;
+    size_t xStride = 0, xinc = 0;
+    efie_status st = factor_batch(c, o, n, lda, batch, pf, false, bb);
+    if (st) return st;
+    HIPCHK(hipEventRecord(c->ev[3], s));
+    st = refine_batch(c, o, n, lda, batch, dB, 2 * nn, false, pf, bb, s, &X, &xStride, &xinc);
+    if (st) return st;
+    st = stage_stats(o, batch, batch, bb, s);
+    if (st) return st;
+    HIPCHK(hipEventRecord(c->ev[4], s));
+    HIPCHK(hipStreamSynchronize(s));
+    int redo[kNoAllocMaxBatch];
+    int nredo = 0;
+    st = finish_batch(c, o, batch, bb, batch, S, nullptr, mixed ? redo : nullptr, &nredo);
+    if (st) return st;
+    for (int q = 0; q < batch; ++q)
+        HIPCHK(hipMemcpy2DAsync(x + 2 * nn * q, sizeof(double) * 2, X + xStride * q, sizeof(double) * xinc,
+                                sizeof(double) * 2, nn, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    float t2 = 0, t3 = 0;
+    HIPCHK(hipEventElapsedTime(&t2, c->ev[2], c->ev[3]));
+    HIPCHK(hipEventElapsedTime(&t3, c->ev[3], c->ev[4]));
+    S->ms_factor = t2;
+    S->ms_solve = t3;
+    S->ms_total = t2 + t3;
+    // systems the FP32 factorisation could not solve: the FP64 LU, in the
+    // same scratch array (the other results are already out)
+    for (int i = 0; i < nredo; ++i) {
+        const int q = redo[i];
+        efie_stats S2;
+        st = efie_zgesv_noalloc(c, n, 1, A + 2 * nn * nn * q, b + 2 * nn * q, EFIE_PREC_FP64, x + 2 * nn * q, &S2);
+        if (st) return st;
+        S->ms_factor += S2.ms_factor;
+        S->ms_solve += S2.ms_solve;
+        S->ms_total += S2.ms_total;
+    }
+    S->n_fp64_retry = nredo;
     return ok();
 }
 

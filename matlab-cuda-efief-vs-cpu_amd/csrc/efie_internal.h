@@ -193,6 +193,19 @@ hipError_t lu_trsv(const T* A, size_t strideA, int n, int lda, T* X, size_t stri
 hipError_t lu_make_perm(const int* ipiv, int n, int batch, int* perm, hipStream_t s);
 size_t lu_perm_lds_limit();
 
+// ------------------------------------------- allocation-free solve ------
+// One static device array per GPU (efie_noalloc.hip) from which
+// efie_zgesv_noalloc carves every buffer of its solve.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;  // EFIE_NOALLOC_SCRATCH_BYTES of efie.h
+constexpr int kNoAllocMaxDevices = 64;
+constexpr int kNoAllocMaxBatch = 64;
+// Device address of `device`'s copy of the array (looked up once per device
+// with hipGetSymbolAddress, then cached).
+hipError_t noalloc_scratch(int device, void** ptr);
+// In-place transpose of the leading n x n block of a row-major complex
+// (interleaved double) matrix with leading dimension ld.
+hipError_t launch_transpose_square(double* A, int n, size_t ld, hipStream_t s);
+
 // ------------------------------------------------ mixed precision -------
 hipError_t launch_to_f32(const double* A64, float* A32, size_t complexElems, hipStream_t s);
 // Column n of the nb matrices (complex elements zElems apart, row-major,

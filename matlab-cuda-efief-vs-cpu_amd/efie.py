@@ -106,7 +106,7 @@ EXPORTS = [
     "efie_opts_default", "efie_opts_preset", "efie_last_error", "efie_last_error_id",
     "efie_mesh_counts", "efie_make_mesh", "efie_rwg_count", "efie_build_rwg", "efie_pick_port",
     "efie_create", "efie_destroy", "efie_set_mesh", "efie_set_rwg", "efie_assemble_zv",
-    "efie_sweep", "efie_zgesv", "efie_diag_copy", "efie_diag_cache_counts", "efie_set_prepack", "efie_set_rwg_tables", "efie_assemble_z",
+    "efie_sweep", "efie_zgesv", "efie_zgesv_noalloc", "efie_diag_copy", "efie_diag_cache_counts", "efie_set_prepack", "efie_set_rwg_tables", "efie_assemble_z",
     "efie_sur_cfg_default", "efie_sur_blob_floats", "efie_sur_train", "efie_sur_get", "efie_sur_set",
     "efie_sur_set_exact", "efie_mlp_params", "efie_mlp_forward", "efie_mlp_loss_grad",
 ]
@@ -144,6 +144,7 @@ def lib():
         L.efie_sweep.argtypes = [C.c_void_p, _dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
                                  C.c_double, C.c_double, C.c_double, C.POINTER(Opts), _dp, _dp, C.POINTER(Stats)]
         L.efie_zgesv.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, C.POINTER(Stats)]
+        L.efie_zgesv_noalloc.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, C.POINTER(Stats)]
         L.efie_diag_copy.argtypes = [C.c_void_p, C.c_size_t, C.c_int, _dp]
         L.efie_diag_cache_counts.argtypes = [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
         _fp = C.POINTER(C.c_float)
@@ -361,8 +362,10 @@ class Context:
         return (*out, st.as_dict()) if stats else out
 
 
-    def zgesv(self, A, b, precision=PREC_FP64, stats=False):
-        """x = A \\ b for a batch: A [batch, n, n] (or [n, n]), b [batch, n]."""
+    def zgesv(self, A, b, precision=PREC_FP64, stats=False, noalloc=False):
+        """x = A \\ b for a batch: A [batch, n, n] (or [n, n]), b [batch, n].
+        noalloc: through efie_zgesv_noalloc (no device allocation; the
+        library's static scratch array)."""
         A = np.asarray(A, dtype=np.complex128)
         single = A.ndim == 2
         if single:
@@ -373,7 +376,8 @@ class Context:
         bf = np.ascontiguousarray(b, dtype=np.complex128)
         x = np.zeros((nb, n), dtype=np.complex128)
         st = Stats()
-        _check(lib().efie_zgesv(self._c, n, nb, Af.view(np.float64).ctypes.data_as(_dp),
+        fn = lib().efie_zgesv_noalloc if noalloc else lib().efie_zgesv
+        _check(fn(self._c, n, nb, Af.view(np.float64).ctypes.data_as(_dp),
                                 bf.view(np.float64).ctypes.data_as(_dp), precision,
                                 x.view(np.float64).ctypes.data_as(_dp), C.byref(st)))
         x = x[0] if single else x
