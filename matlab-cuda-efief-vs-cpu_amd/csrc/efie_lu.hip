@@ -710,8 +710,12 @@ struct gemm_vec<double> {
     typedef double2 v;
 };
 
+// FP64: 158 VGPRs + 96 accumulation registers would leave two waves per
+// SIMD; asking for three (a few spilled registers) makes the c2_6k FP64
+// factorisation's updates 20 % faster (532 -> 426 ms per 51 matrices).
 template <typename T, int BM, int BN, bool M3 = false>
-__global__ __launch_bounds__(256) void k_gemm(T* A, size_t strideA, int lda, int r0, int r1, int c0,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 3 : 1))) void k_gemm(
+    T* A, size_t strideA, int lda, int r0, int r1, int c0,
                                               int c1, int k0, int K, int gx, int gy, int nblk, int skipUpper) {
     typedef typename mf<T>::v4 v4;
     typedef typename gemm_vec<T>::v vec;
