@@ -89,7 +89,7 @@ enum { EFIE_LU_AUTO = 0, EFIE_LU_FP32 = 1, EFIE_LU_BF16X6 = 2, EFIE_LU_SYM = 3 }
  * (q! R), integrates the frequency-independent coefficients of the same
  * quadrature rules once per mesh and forms Z of a whole batch of
  * frequencies from them in one pass (series tail below 1e-17 relative);
- * when |k| times the pair size needs more than 18 terms, or the series
+ * when |k| times the pair size needs more than 22 terms, or the series
  * would not fit in a third of the free device memory, the regular pairs
  * are integrated per frequency and only the near and self pairs use a
  * moment series (falling back to DIRECT when that is too long too).

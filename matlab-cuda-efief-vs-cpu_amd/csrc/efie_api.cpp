@@ -1163,14 +1163,15 @@ efie_status ensure_moments(efie_ctx* c, const efie_opts* o, int P, efie_stats* s
 // with (x^(P+1)/(P+1)!) e^x <= 1e-17, x = |k|max * b, where b bounds
 // |R - R0e| for every point pair of every entry: the distance of a point
 // pair from its own pair's centroid distance (two point radii plus the
-// near or self softening length) plus the spread of the four pairs'
-// centroid distances about their mean (at most the centroid distances of
-// the two RWG's own triangle pairs).  -1 if P would exceed 17.
+// near or self softening length) plus the distance of that centroid
+// distance from the entry's centre R0e, the middle of the four pairs'
+// (half their range, which the two RWGs' own centroid distances bound:
+// at most dcMax).  -1 if P would exceed 21.
 int series_terms(const efie_ctx* c, const efie_opts* o, double kmax) {
     const double soft = std::max(o->near_alpha * c->aNearMax, c->aSelfMax);
-    const double x = kmax * (2 * c->radMax + soft + 2 * c->dcMax) * 1.0001;
+    const double x = kmax * (2 * c->radMax + soft + c->dcMax) * 1.0001;
     double term = std::exp(x);
-    for (int n = 1; n <= 18; ++n) {
+    for (int n = 1; n <= 22; ++n) {
         term *= x / n;
         if (term <= 1e-17) return n - 1;
     }

@@ -1135,17 +1135,16 @@ __device__ void series_subdiv(const Pt* __restrict__ P1, const Pt* __restrict__ 
                 const Pt q2 = P2[j];
                 const double dx = xi - q2.x, dy = yi - q2.y, dz = zi - q2.z;
                 double R2 = dx * dx + dy * dy + dz * dz;
-                double R;
                 if (!SELF) {
-                    R = sqrt(R2 + aNear2);
+                    R2 = R2 + aNear2;
                 } else if (j / Nq2 == psub) {
-                    R = sqrt(R2 + aS2);
+                    R2 = R2 + aS2;
                 } else {
-                    R = sqrt(R2);
-                    R = R > 1e-15 ? R : 1e-15;
+                    R2 = R2 > 1e-30 ? R2 : 1e-30;  // max(R, 1e-15)
                 }
-                const double d = R - R0;
-                double t = series_head(q2.w / R, d, q0);
+                const double invR = rsqrt64(R2);
+                const double d = R2 * invR - R0;
+                double t = series_head(q2.w * invR, d, q0);
 #pragma unroll
                 for (int c = 0; c < kSerChunk; ++c) {
                     g[c] += t;
@@ -1210,7 +1209,8 @@ __global__ __launch_bounds__(256) void k_series_self(const Pt* __restrict__ self
 // one block per 16 x 16 upper-triangular tile of Z, one entry per thread.
 // Each thread combines its entry's four pair series with the RWG side
 // coefficients into D[0..NT-1] (NT = P + 3 orders z^-1 .. z^(P+1)) about
-// the common centre R0e and stores them, with R0e, as
+// the common centre R0e (the middle of the four centroid distances) and
+// stores them, with R0e, as
 // C[(tile * (NT + 1) + j) * 256 + thread] (coefficient-major per tile, so
 // the fill reads them with coalesced loads).
 template <int NT>
@@ -1238,9 +1238,10 @@ __global__ __launch_bounds__(256) void k_series_coeffs(const double* __restrict_
         sa[k] = side[2 * mc + k];
         sb[k] = side[2 * nc + k];
     }
-    // phase centres of the four pairs and their mean
-    double R0p[2][2], R0e = 0.0;
-    int np = 0;
+    // phase centres of the four pairs and the middle of their range (each
+    // then lies within half the range, at most the centroid distance of one
+    // RWG's two triangles, of the common centre)
+    double R0p[2][2], lo = 1e300, hi = -1e300;
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -1250,11 +1251,11 @@ __global__ __launch_bounds__(256) void k_series_coeffs(const double* __restrict_
                 const TriGeo g1 = tg[tm[a]], g2 = tg[tn[b]];
                 const double dx = g1.cx - g2.cx, dy = g1.cy - g2.cy, dz = g1.cz - g2.cz;
                 R0p[a][b] = sqrt(dx * dx + dy * dy + dz * dz);
-                R0e += R0p[a][b];
-                ++np;
+                lo = fmin(lo, R0p[a][b]);
+                hi = fmax(hi, R0p[a][b]);
             }
         }
-    R0e = np > 0 ? R0e / np : 0.0;
+    const double R0e = hi >= lo ? 0.5 * (lo + hi) : 0.0;
 
     double D[NT];
 #pragma unroll
@@ -1683,7 +1684,7 @@ hipError_t launch_series(const Pt* regPts, int Nq, const TriGeo* tg, int Nt, con
 }
 
 int series_nt(int P) {
-    for (int nt : {10, 12, 14, 16, 18, 20})
+    for (int nt : {10, 12, 14, 16, 18, 20, 22, 24})
         if (P + 3 <= nt) return nt;
     return 0;
 }
@@ -1705,6 +1706,8 @@ hipError_t launch_series_coeffs(const double* Q, int NT, const TriGeo* tg, const
         case 16: hipLaunchKernelGGL(k_series_coeffs<16>, g, blk, 0, s, Q, tg, sideTri, side, Ne, C); break;
         case 18: hipLaunchKernelGGL(k_series_coeffs<18>, g, blk, 0, s, Q, tg, sideTri, side, Ne, C); break;
         case 20: hipLaunchKernelGGL(k_series_coeffs<20>, g, blk, 0, s, Q, tg, sideTri, side, Ne, C); break;
+        case 22: hipLaunchKernelGGL(k_series_coeffs<22>, g, blk, 0, s, Q, tg, sideTri, side, Ne, C); break;
+        case 24: hipLaunchKernelGGL(k_series_coeffs<24>, g, blk, 0, s, Q, tg, sideTri, side, Ne, C); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1731,6 +1734,8 @@ hipError_t launch_series_fill(const double* C, int NT, bool lossy, int Ne, const
         EFIE_SER_CASE(16)
         EFIE_SER_CASE(18)
         EFIE_SER_CASE(20)
+        EFIE_SER_CASE(22)
+        EFIE_SER_CASE(24)
         default: return hipErrorInvalidValue;
     }
 #undef EFIE_SER_CASE
