@@ -23,6 +23,12 @@ efie_status efie_diag_copy(efie_ctx* ctx, size_t bytes, int reps, double* ms);
  * another) sees neither count move. */
 efie_status efie_diag_cache_counts(long long* geometry_builds, long long* moment_builds);
 
+/* The unordered near triangle pairs (0-based t1 < t2, ordered by t1, then
+ * t2) the quadrature fill of *opts uses on the context's mesh: *n is their
+ * number; the first min(*n, cap) go to pairs[2k], pairs[2k+1] (pairs may
+ * be NULL with cap 0). */
+efie_status efie_diag_near_pairs(efie_ctx* ctx, const efie_opts* opts, long long* n, int* pairs, long long cap);
+
 #ifdef __cplusplus
 }
 #endif

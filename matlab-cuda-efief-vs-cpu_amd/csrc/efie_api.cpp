@@ -272,6 +272,8 @@ struct efie_ctx {
     DBuf dMlpA, dMlpB, dMlpC, dMlpD, dMlpE;  // scratch of the network entry points
 
     DBuf dTri, dReg, dNearPts, dSelfPts, dASelf, dAeq, dNearMat, dRowCnt, dList;
+    DBuf dCellOf, dCellStart, dCellTri;  // centroid bins of the near search
+    double nearEps = 0;  // near test of the regular kernels (0: the map dNearMat)
     DBuf dSideTri, dSide, dMid, dLen, dQn, dQs, dRad;
     // centroid model tables
     DBuf dCPos, dCF, dCDv, dCW, dCT, dCAsoft;
@@ -860,6 +862,63 @@ efie_status ensure_edge_mid(efie_ctx* c) {
     return ok();
 }
 
+// Centroids binned into cubic cells of side h (just above `reach`, the
+// largest centroid distance of a near pair), cell by cell: the near search
+// then visits a triangle's own cell and the 26 around it.  Extents with
+// more cells than 8 per triangle get larger cells (any h >= reach is
+// correct).
+struct CellBins {
+    double x0 = 0, y0 = 0, z0 = 0, h = 1;
+    int nx = 1, ny = 1, nz = 1;
+    std::vector<int> cellOf, start, tri;
+};
+
+CellBins bin_centroids(const std::vector<TriGeo>& tg, double reach) {
+    CellBins b;
+    const int Nt = (int)tg.size();
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (const TriGeo& g : tg) {
+        const double c[3] = {g.cx, g.cy, g.cz};
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = std::min(lo[d], c[d]);
+            hi[d] = std::max(hi[d], c[d]);
+        }
+    }
+    double h = std::max(reach * (1.0 + 1e-9), 1e-300);
+    auto dims = [&](double hh, int* n) {
+        double cells = 1;
+        for (int d = 0; d < 3; ++d) {
+            n[d] = (int)std::min(1e6, std::floor((hi[d] - lo[d]) / hh) + 1.0);
+            cells *= n[d];
+        }
+        return cells;
+    };
+    int n[3];
+    while (dims(h, n) > 8.0 * Nt + 64) h *= 2;
+    b.x0 = lo[0];
+    b.y0 = lo[1];
+    b.z0 = lo[2];
+    b.h = h;
+    b.nx = n[0];
+    b.ny = n[1];
+    b.nz = n[2];
+    const size_t ncell = (size_t)n[0] * n[1] * n[2];
+    b.cellOf.resize(Nt);
+    b.start.assign(ncell + 1, 0);
+    for (int t = 0; t < Nt; ++t) {
+        const double c[3] = {tg[t].cx, tg[t].cy, tg[t].cz};
+        int ix[3];
+        for (int d = 0; d < 3; ++d) ix[d] = std::min(n[d] - 1, std::max(0, (int)((c[d] - lo[d]) / h)));
+        b.cellOf[t] = (ix[2] * n[1] + ix[1]) * n[0] + ix[0];
+        ++b.start[b.cellOf[t] + 1];
+    }
+    for (size_t i = 0; i < ncell; ++i) b.start[i + 1] += b.start[i];
+    b.tri.resize(Nt);
+    std::vector<int> fill(b.start.begin(), b.start.end() - 1);
+    for (int t = 0; t < Nt; ++t) b.tri[fill[b.cellOf[t]]++] = t;
+    return b;
+}
+
 // Frequency-independent data of the quadrature model, cached per options.
 efie_status ensure_geometry(efie_ctx* c, const efie_opts* o) {
     GeoKey k;
@@ -1036,7 +1095,6 @@ efie_status ensure_geometry(efie_ctx* c, const efie_opts* o) {
     HIPCHK(c->dRad.ensure(sizeof(double) * Nt));
     HIPCHK(c->dSideTri.ensure(sizeof(int) * sideTri.size()));
     HIPCHK(c->dSide.ensure(sizeof(Side) * side.size()));
-    HIPCHK(c->dNearMat.ensure((size_t)Nt * Nt));
     HIPCHK(c->dRowCnt.ensure(sizeof(int) * Nt));
     HIPCHK(hipMemcpyAsync(c->dTri.p, tg.data(), sizeof(TriGeo) * Nt, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(c->dReg.p, reg.data(), sizeof(Pt) * reg.size(), hipMemcpyHostToDevice, s));
@@ -1048,10 +1106,13 @@ efie_status ensure_geometry(efie_ctx* c, const efie_opts* o) {
     HIPCHK(hipMemcpyAsync(c->dSideTri.p, sideTri.data(), sizeof(int) * sideTri.size(), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(c->dSide.p, side.data(), sizeof(Side) * side.size(), hipMemcpyHostToDevice, s));
 
-    // near pairs: the pack's map, or triangles_are_near.m on the device;
-    // then the row counts of t2 > t1 and the ordered pair list
+    // near pairs: the pack's map (kept on the device: the pack defines
+    // them), or triangles_are_near.m on the device over binned centroids
+    // (no Nt x Nt map); then the row counts of t2 > t1 and the pair list
+    // ordered by t1, then t2
     std::vector<int> cnt(Nt, 0);
     double dNearMax = 0;
+    CellBins bins;
     if (c->havePack) {
         for (int t1 = 0; t1 < Nt; ++t1)
             for (int t2 = t1 + 1; t2 < Nt; ++t2)
@@ -1060,11 +1121,22 @@ efie_status ensure_geometry(efie_ctx* c, const efie_opts* o) {
                     const double dx = tg[t1].cx - tg[t2].cx, dy = tg[t1].cy - tg[t2].cy, dz = tg[t1].cz - tg[t2].cz;
                     dNearMax = std::max(dNearMax, std::sqrt(dx * dx + dy * dy + dz * dz));
                 }
+        HIPCHK(c->dNearMat.ensure((size_t)Nt * Nt));
         HIPCHK(hipMemcpyAsync(c->dNearMat.p, c->pkNearMat.data(), (size_t)Nt * Nt, hipMemcpyHostToDevice, s));
     } else {
+        c->dNearMat.release();
         dNearMax = o->near_sing_eps * lmax;
-        HIPCHK(launch_near_map(c->dTri.as<TriGeo>(), Nt, o->near_sing_eps, c->dNearMat.as<uint8_t>(),
-                               c->dRowCnt.as<int>(), s));
+        bins = bin_centroids(tg, dNearMax);
+        HIPCHK(c->dCellOf.ensure(sizeof(int) * Nt));
+        HIPCHK(c->dCellStart.ensure(sizeof(int) * bins.start.size()));
+        HIPCHK(c->dCellTri.ensure(sizeof(int) * Nt));
+        HIPCHK(hipMemcpyAsync(c->dCellOf.p, bins.cellOf.data(), sizeof(int) * Nt, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(c->dCellStart.p, bins.start.data(), sizeof(int) * bins.start.size(),
+                              hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(c->dCellTri.p, bins.tri.data(), sizeof(int) * Nt, hipMemcpyHostToDevice, s));
+        HIPCHK(launch_near_bins(c->dTri.as<TriGeo>(), Nt, o->near_sing_eps, bins.x0, bins.y0, bins.z0, bins.h,
+                                bins.nx, bins.ny, bins.nz, c->dCellOf.as<int>(), c->dCellStart.as<int>(),
+                                c->dCellTri.as<int>(), c->dRowCnt.as<int>(), nullptr, nullptr, s));
         HIPCHK(hipMemcpyAsync(cnt.data(), c->dRowCnt.p, sizeof(int) * Nt, hipMemcpyDeviceToHost, s));
     }
     HIPCHK(hipStreamSynchronize(s));
@@ -1076,8 +1148,14 @@ efie_status ensure_geometry(efie_ctx* c, const efie_opts* o) {
     }
     HIPCHK(c->dList.ensure(sizeof(int2) * (size_t)std::max(1LL, tot)));
     HIPCHK(hipMemcpyAsync(c->dRowCnt.p, start.data(), sizeof(int) * Nt, hipMemcpyHostToDevice, s));
-    HIPCHK(launch_near_list(c->dNearMat.as<uint8_t>(), c->dRowCnt.as<int>(), Nt, c->dList.as<int2>(), s));
+    if (c->havePack)
+        HIPCHK(launch_near_list(c->dNearMat.as<uint8_t>(), c->dRowCnt.as<int>(), Nt, c->dList.as<int2>(), s));
+    else
+        HIPCHK(launch_near_bins(c->dTri.as<TriGeo>(), Nt, o->near_sing_eps, bins.x0, bins.y0, bins.z0, bins.h,
+                                bins.nx, bins.ny, bins.nz, c->dCellOf.as<int>(), c->dCellStart.as<int>(),
+                                c->dCellTri.as<int>(), nullptr, c->dRowCnt.as<int>(), c->dList.as<int2>(), s));
     HIPCHK(hipStreamSynchronize(s));
+    c->nearEps = c->havePack ? 0.0 : o->near_sing_eps;
 
     c->Nq = Nq;
     c->NpNear = NsN * Nq2;
@@ -1204,7 +1282,8 @@ efie_status ensure_series(efie_ctx* c, const efie_opts* o, double kmax, efie_sta
     HIPCHK(c->dSerC.ensure(cBytes));
     HIPCHK(c->dQall.ensure(qBytes));
     HIPCHK(hipEventRecord(c->ev[6], s));
-    HIPCHK(launch_series(c->dReg.as<Pt>(), c->Nq, c->dTri.as<TriGeo>(), c->Nt, c->dNearMat.as<uint8_t>(),
+    HIPCHK(launch_series(c->dReg.as<Pt>(), c->Nq, c->dTri.as<TriGeo>(), c->Nt,
+                         c->havePack ? c->dNearMat.as<uint8_t>() : nullptr, c->nearEps,
                          c->dNearPts.as<Pt>(), c->NpNear, c->Nq2s, c->dAeq.as<double>(), o->near_alpha,
                          c->dList.as<int2>(), (int)c->nNear, c->dSelfPts.as<Pt>(), c->NsSelf, c->dASelf.as<double>(),
                          P, c->dQall.as<double>(), s));
@@ -1360,7 +1439,9 @@ efie_status fill_z(efie_ctx* c, const efie_opts* o, const NearPlan& plan, const 
     }
     pf.begin(EFIE_K_REGULAR);
     HIPCHK(launch_rec_regular(c->dReg.as<Pt>(), c->Nq, c->dTri.as<TriGeo>(), c->dRad.as<double>(), Nt,
-                              plan.sur ? c->dSurMat.as<uint8_t>() : c->dNearMat.as<uint8_t>(), dfr, recs, s));
+                              plan.sur ? c->dSurMat.as<uint8_t>()
+                                       : (c->havePack ? c->dNearMat.as<uint8_t>() : nullptr),
+                              c->nearEps, dfr, recs, s));
     pf.end(EFIE_K_REGULAR);
     if (plan.sur) {
         // the regular rule wrote the base records of the surrogate's pairs
@@ -1506,8 +1587,7 @@ efie_status ensure_surrogate(efie_ctx* c, const efie_opts* o) {
     HIPCHK(hipMemcpyAsync(c->dSurModel.p, c->surBlob.data(), sizeof(float) * kSurModel, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(c->dSurSub.p, sub.data(), sizeof(double) * sub.size(), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(c->dSurReg.p, reg.data(), sizeof(double) * reg.size(), hipMemcpyHostToDevice, s));
-    HIPCHK(launch_sur_map(c->dNearMat.as<uint8_t>(), c->dSurPairs.as<SurPair>(), (int)nIn, c->Nt,
-                          c->dSurMat.as<uint8_t>(), s));
+    HIPCHK(launch_sur_map(c->dSurFb.as<int2>(), (int)nFb, c->Nt, c->dSurMat.as<uint8_t>(), s));
     HIPCHK(hipStreamSynchronize(s));
     c->nSurIn = (int)nIn;
     c->nSurFb = (int)nFb;
@@ -2524,6 +2604,23 @@ efie_status efie_diag_cache_counts(long long* geometry_builds, long long* moment
     if (!geometry_builds || !moment_builds) return fail(EFIE_E_ARGS, "efie:args", "null argument");
     *geometry_builds = g_geoBuilds.load();
     *moment_builds = g_momBuilds.load();
+    return ok();
+}
+
+efie_status efie_diag_near_pairs(efie_ctx* c, const efie_opts* opts, long long* n, int* pairs, long long cap) {
+    if (!c || !n || (cap > 0 && !pairs)) return fail(EFIE_E_ARGS, "efie:args", "null argument");
+    efie_opts o;
+    if (opts) o = *opts;
+    else efie_opts_default(&o);
+    o.model = EFIE_MODEL_QUADRATURE;
+    efie_status st = prepare(c, &o, 0.20);
+    if (st) return st;
+    *n = c->nNear;
+    const long long k = std::min(cap, c->nNear);
+    if (k > 0) {
+        HIPCHK(hipMemcpyAsync(pairs, c->dList.p, sizeof(int2) * (size_t)k, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
     return ok();
 }
 

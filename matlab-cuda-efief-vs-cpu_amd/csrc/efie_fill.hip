@@ -123,25 +123,65 @@ __device__ bool near_test(const TriGeo a, const TriGeo b, double eps) {
 
 // ------------------------------------------------------------ near map --
 
-__global__ __launch_bounds__(256) void k_near_map(const TriGeo* __restrict__ tg, int Nt,
-                                                  double eps, uint8_t* __restrict__ nearMat,
-                                                  int* __restrict__ rowCount) {
-    __shared__ int cnt[256];
-    int t1 = blockIdx.x;
-    TriGeo a = tg[t1];
-    int c = 0;
-    for (int t2 = threadIdx.x; t2 < Nt; t2 += blockDim.x) {
-        bool nr = (t1 != t2) && near_test(a, tg[t2], eps);
-        nearMat[(size_t)t1 * Nt + t2] = nr ? 1 : 0;
-        c += (nr && t2 > t1) ? 1 : 0;
+// Near pairs by cell binning: triangle centroids are binned into cubic
+// cells of side h >= near_sing_eps * max(l), which bounds the distance of
+// every near pair, so a triangle's near partners lie in its own cell or
+// the 26 around it.  `cellTri` lists the triangles cell by cell (offsets
+// `cellStart`, ncell + 1 entries), `cellOf` is each triangle's cell.
+struct CellGrid {
+    double x0, y0, z0, h;
+    int nx, ny, nz;
+};
+
+__device__ __forceinline__ void cell_coords(const CellGrid& g, int c, int& ix, int& iy, int& iz) {
+    ix = c % g.nx;
+    iy = (c / g.nx) % g.ny;
+    iz = c / (g.nx * g.ny);
+}
+
+// COUNT: rowCount[t1] = number of near t2 > t1.  Otherwise the near t2 > t1
+// of row t1 go to list[rowStart[t1]..] sorted by t2 (the order of the dense
+// scan over t2 that the reference's near matrix implies).
+template <bool COUNT>
+__global__ __launch_bounds__(256) void k_near_bins(const TriGeo* __restrict__ tg, int Nt, double eps, CellGrid g,
+                                                   const int* __restrict__ cellOf, const int* __restrict__ cellStart,
+                                                   const int* __restrict__ cellTri, int* __restrict__ rowCount,
+                                                   const int* __restrict__ rowStart, int2* __restrict__ list) {
+    const int t1 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t1 >= Nt) return;
+    const TriGeo a = tg[t1];
+    int ix, iy, iz;
+    cell_coords(g, cellOf[t1], ix, iy, iz);
+    int cnt = 0;
+    int2* row = COUNT ? nullptr : list + rowStart[t1];
+    for (int dz = -1; dz <= 1; ++dz) {
+        const int z = iz + dz;
+        if (z < 0 || z >= g.nz) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int y = iy + dy;
+            if (y < 0 || y >= g.ny) continue;
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int x = ix + dx;
+                if (x < 0 || x >= g.nx) continue;
+                const int c = (z * g.ny + y) * g.nx + x;
+                for (int k = cellStart[c]; k < cellStart[c + 1]; ++k) {
+                    const int t2 = cellTri[k];
+                    if (t2 <= t1 || !near_test(a, tg[t2], eps)) continue;
+                    if (!COUNT) {
+                        // insertion into the row, kept sorted by t2
+                        int p = cnt;
+                        while (p > 0 && row[p - 1].y > t2) {
+                            row[p] = row[p - 1];
+                            --p;
+                        }
+                        row[p] = make_int2(t1, t2);
+                    }
+                    ++cnt;
+                }
+            }
+        }
     }
-    cnt[threadIdx.x] = c;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) cnt[threadIdx.x] += cnt[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) rowCount[t1] = cnt[0];
+    if (COUNT) rowCount[t1] = cnt;
 }
 
 // Ordered compaction of the near pairs (t1 < t2) of one row.
@@ -294,7 +334,7 @@ template <int NQ>
 __global__ __launch_bounds__(256, 3) void k_rec_regular(const Pt* __restrict__ pts,
                                                      const TriGeo* __restrict__ tg,
                                                      const double* __restrict__ rad, int Nt,
-                                                     const uint8_t* __restrict__ nearMat,
+                                                     const uint8_t* __restrict__ nearMat, double eps,
                                                      const Freq* __restrict__ fr,
                                                      Rec* __restrict__ recs) {
     // linear block id -> tile pair (ti <= tj)
@@ -328,8 +368,10 @@ __global__ __launch_bounds__(256, 3) void k_rec_regular(const Pt* __restrict__ p
     for (int q = 0; q < perWave; ++q) {
         const int t2 = t2base + q;
         if (t2 >= Nt) break;
-        const bool active = (t1 < t2) && (nearMat[(size_t)t2 * Nt + t1] == 0);
         const TriGeo g2 = tg[t2];
+        // near pairs: the given map (a prepack, the surrogate's fallback
+        // pairs) or the reference's near test itself
+        const bool active = (t1 < t2) && !(nearMat ? nearMat[(size_t)t2 * Nt + t1] != 0 : near_test(g1, g2, eps));
         const Pt* p2 = pts + (size_t)t2 * NQ;
         const double ox = g1.cx - g2.cx, oy = g1.cy - g2.cy, oz = g1.cz - g2.cz;
         const PairGreen pg = pair_green(kr, ki, sqrt(ox * ox + oy * oy + oz * oz), rad1 + rad[t2]);
@@ -1011,8 +1053,8 @@ __device__ __forceinline__ double series_head(double wInvR, double d, int q0) {
 // product rule is summed in chunks of kSerChunk orders.
 template <int NQ>
 __global__ __launch_bounds__(256) void k_series_regular(const Pt* __restrict__ pts, const TriGeo* __restrict__ tg,
-                                                        int Nt, const uint8_t* __restrict__ nearMat, int P,
-                                                        double* __restrict__ Q) {
+                                                        int Nt, const uint8_t* __restrict__ nearMat, double eps,
+                                                        int P, double* __restrict__ Q) {
     long long b = blockIdx.x;
     int tj = (int)((sqrt(8.0 * (double)b + 1.0) - 1.0) * 0.5);
     while ((long long)(tj + 1) * (tj + 2) / 2 <= b) ++tj;
@@ -1036,8 +1078,8 @@ __global__ __launch_bounds__(256) void k_series_regular(const Pt* __restrict__ p
     for (int qq = 0; qq < perWave; ++qq) {
         const int t2 = t2base + qq;
         if (t2 >= Nt) break;
-        if (!(t1 < t2) || nearMat[(size_t)t2 * Nt + t1] != 0) continue;
         const TriGeo g2 = tg[t2];
+        if (!(t1 < t2) || (nearMat ? nearMat[(size_t)t2 * Nt + t1] != 0 : near_test(g1, g2, eps))) continue;
         const Pt* p2 = pts + (size_t)t2 * NQ;
         const double ox = g1.cx - g2.cx, oy = g1.cy - g2.cy, oz = g1.cz - g2.cz;
         const double R0 = sqrt(ox * ox + oy * oy + oz * oz);
@@ -1572,9 +1614,17 @@ __global__ __launch_bounds__(1024) void k_epilogue(const double* __restrict__ X,
 
 // ------------------------------------------------------------ launchers --
 
-hipError_t launch_near_map(const TriGeo* tg, int Nt, double eps, uint8_t* nearMat, int* rowCount,
-                           hipStream_t s) {
-    hipLaunchKernelGGL(k_near_map, dim3(Nt), dim3(256), 0, s, tg, Nt, eps, nearMat, rowCount);
+hipError_t launch_near_bins(const TriGeo* tg, int Nt, double eps, double x0, double y0, double z0, double h, int nx,
+                            int ny, int nz, const int* cellOf, const int* cellStart, const int* cellTri,
+                            int* rowCount, const int* rowStart, int2* list, hipStream_t s) {
+    const CellGrid g{x0, y0, z0, h, nx, ny, nz};
+    const dim3 grid((Nt + 255) / 256), blk(256);
+    if (!list)
+        hipLaunchKernelGGL(k_near_bins<true>, grid, blk, 0, s, tg, Nt, eps, g, cellOf, cellStart, cellTri, rowCount,
+                           nullptr, nullptr);
+    else
+        hipLaunchKernelGGL(k_near_bins<false>, grid, blk, 0, s, tg, Nt, eps, g, cellOf, cellStart, cellTri, nullptr,
+                           rowStart, list);
     return hipGetLastError();
 }
 
@@ -1585,15 +1635,15 @@ hipError_t launch_near_list(const uint8_t* nearMat, const int* rowStart, int Nt,
 }
 
 hipError_t launch_rec_regular(const Pt* regPts, int Nq, const TriGeo* tg, const double* rad, int Nt,
-                              const uint8_t* nearMat, const Freq* fr, Rec* recs, hipStream_t s) {
+                              const uint8_t* nearMat, double eps, const Freq* fr, Rec* recs, hipStream_t s) {
     long long T = (Nt + kRegTile - 1) / kRegTile;
     long long nb = T * (T + 1) / 2;
     dim3 g((unsigned)nb, kRegSplit), blk(256);
     switch (Nq) {
-        case 1: hipLaunchKernelGGL(k_rec_regular<1>, g, blk, 0, s, regPts, tg, rad, Nt, nearMat, fr, recs); break;
-        case 3: hipLaunchKernelGGL(k_rec_regular<3>, g, blk, 0, s, regPts, tg, rad, Nt, nearMat, fr, recs); break;
-        case 6: hipLaunchKernelGGL(k_rec_regular<6>, g, blk, 0, s, regPts, tg, rad, Nt, nearMat, fr, recs); break;
-        case 7: hipLaunchKernelGGL(k_rec_regular<7>, g, blk, 0, s, regPts, tg, rad, Nt, nearMat, fr, recs); break;
+        case 1: hipLaunchKernelGGL(k_rec_regular<1>, g, blk, 0, s, regPts, tg, rad, Nt, nearMat, eps, fr, recs); break;
+        case 3: hipLaunchKernelGGL(k_rec_regular<3>, g, blk, 0, s, regPts, tg, rad, Nt, nearMat, eps, fr, recs); break;
+        case 6: hipLaunchKernelGGL(k_rec_regular<6>, g, blk, 0, s, regPts, tg, rad, Nt, nearMat, eps, fr, recs); break;
+        case 7: hipLaunchKernelGGL(k_rec_regular<7>, g, blk, 0, s, regPts, tg, rad, Nt, nearMat, eps, fr, recs); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1661,7 +1711,7 @@ hipError_t launch_poly_records(const double* Qn, const int2* list, int nNear, co
     return hipGetLastError();
 }
 
-hipError_t launch_series(const Pt* regPts, int Nq, const TriGeo* tg, int Nt, const uint8_t* nearMat,
+hipError_t launch_series(const Pt* regPts, int Nq, const TriGeo* tg, int Nt, const uint8_t* nearMat, double eps,
                          const Pt* subPts, int Np, int Nq2, const double* aeq, double near_alpha, const int2* list,
                          int nPairs, const Pt* selfPts, int Ns, const double* aSelf, int P, double* Q,
                          hipStream_t s) {
@@ -1670,10 +1720,10 @@ hipError_t launch_series(const Pt* regPts, int Nq, const TriGeo* tg, int Nt, con
     long long nb = T * (T + 1) / 2;
     dim3 g((unsigned)nb, kRegSplit), blk(256);
     switch (Nq) {
-        case 1: hipLaunchKernelGGL(k_series_regular<1>, g, blk, 0, s, regPts, tg, Nt, nearMat, P, Q); break;
-        case 3: hipLaunchKernelGGL(k_series_regular<3>, g, blk, 0, s, regPts, tg, Nt, nearMat, P, Q); break;
-        case 6: hipLaunchKernelGGL(k_series_regular<6>, g, blk, 0, s, regPts, tg, Nt, nearMat, P, Q); break;
-        case 7: hipLaunchKernelGGL(k_series_regular<7>, g, blk, 0, s, regPts, tg, Nt, nearMat, P, Q); break;
+        case 1: hipLaunchKernelGGL(k_series_regular<1>, g, blk, 0, s, regPts, tg, Nt, nearMat, eps, P, Q); break;
+        case 3: hipLaunchKernelGGL(k_series_regular<3>, g, blk, 0, s, regPts, tg, Nt, nearMat, eps, P, Q); break;
+        case 6: hipLaunchKernelGGL(k_series_regular<6>, g, blk, 0, s, regPts, tg, Nt, nearMat, eps, P, Q); break;
+        case 7: hipLaunchKernelGGL(k_series_regular<7>, g, blk, 0, s, regPts, tg, Nt, nearMat, eps, P, Q); break;
         default: return hipErrorInvalidValue;
     }
     if (nPairs > 0)

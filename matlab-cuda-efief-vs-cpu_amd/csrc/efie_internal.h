@@ -54,16 +54,23 @@ struct Freq {
 // --------------------------------------------------------- launchers ----
 // All launchers enqueue on `s` and return hipGetLastError().
 
-hipError_t launch_near_map(const TriGeo* tg, int Nt, double eps, uint8_t* nearMat,
-                           int* rowCount, hipStream_t s);
+// Near pairs (triangles_are_near.m) by cell binning: cubic cells of side h
+// (>= eps * max(l)) from (x0, y0, z0), nx * ny * nz of them; cellOf[t] the
+// cell of triangle t, cellTri the triangles cell by cell from
+// cellStart[cell] (ncell + 1 offsets).  list == nullptr: rowCount[t1] =
+// near t2 > t1; otherwise those pairs, sorted by t2, to list[rowStart[t1]..].
+hipError_t launch_near_bins(const TriGeo* tg, int Nt, double eps, double x0, double y0, double z0, double h, int nx,
+                            int ny, int nz, const int* cellOf, const int* cellStart, const int* cellTri,
+                            int* rowCount, const int* rowStart, int2* list, hipStream_t s);
 hipError_t launch_near_list(const uint8_t* nearMat, const int* rowStart, int Nt,
                             int2* list, hipStream_t s);
 
-// Regular (non-near, t1 < t2) records; nearMat excludes near pairs.
+// Regular (non-near, t1 < t2) records; near pairs are those of nearMat
+// when it is given, else those the near test with eps finds.
 // rad[t]: largest distance from the centroid of t to its vertices (bounds
 // every quadrature point of t).
 hipError_t launch_rec_regular(const Pt* regPts, int Nq, const TriGeo* tg, const double* rad, int Nt,
-                              const uint8_t* nearMat, const Freq* fr, Rec* recs,
+                              const uint8_t* nearMat, double eps, const Freq* fr, Rec* recs,
                               hipStream_t s);
 // Subdivided near-pair records (rwg_pair_integrals_near_singular.m).
 hipError_t launch_rec_near(const Pt* subPts, int Np, const TriGeo* tg, const double* rad,
@@ -91,9 +98,10 @@ hipError_t launch_poly_records(const double* Qn, const int2* list, int nNear, co
 // Frequency-independent series of every triangle pair {t1 <= t2} about its
 // centroid distance R0 (0 for t1 = t2): Q[rec_index(t1,t2)][q][8] =
 // sum W X (R - R0)^q / (q! R), q = 0..P (P + 1 <= 32), with the regular rule
-// for pairs outside nearMat, the subdivided near rule for the nPairs pairs of
-// list and the self rule for t1 = t2.
-hipError_t launch_series(const Pt* regPts, int Nq, const TriGeo* tg, int Nt, const uint8_t* nearMat,
+// for the pairs that are not near (nearMat, or the near test with eps), the
+// subdivided near rule for the nPairs pairs of list and the self rule for
+// t1 = t2.
+hipError_t launch_series(const Pt* regPts, int Nq, const TriGeo* tg, int Nt, const uint8_t* nearMat, double eps,
                          const Pt* subPts, int Np, int Nq2, const double* aeq, double near_alpha, const int2* list,
                          int nPairs, const Pt* selfPts, int Ns, const double* aSelf, int P, double* Q,
                          hipStream_t s);
@@ -299,10 +307,10 @@ hipError_t launch_adam(float* p, const float* g, float* m, float* v, int n, floa
 // instead of the network.
 hipError_t launch_sur_apply(const SurPair* pairs, int n, const float* model, const double* Tbuf, const Freq* fr,
                             Rec* recs, hipStream_t s);
-// Near map of the surrogate mode: copy of nearMat with the pairs handled by
-// the surrogate cleared (they get the regular rule as base).
-hipError_t launch_sur_map(const uint8_t* nearMat, const SurPair* pairs, int n, int Nt, uint8_t* out,
-                          hipStream_t s);
+// Near map of the surrogate mode: the n pairs of fb (near pairs left to the
+// quadrature) set, all else clear, so the pairs handled by the surrogate get
+// the regular rule as base.
+hipError_t launch_sur_map(const int2* fb, int n, int Nt, uint8_t* out, hipStream_t s);
 
 // Zin epilogue of batch element b: x[e] = X[b*strideX + e*xinc] (complex),
 // Iport = sum(wP.*len.*x) - sum(wG.*len.*x) (or x[p]-x[g] in the centroid

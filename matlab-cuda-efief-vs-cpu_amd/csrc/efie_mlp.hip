@@ -372,15 +372,17 @@ __global__ __launch_bounds__(256) void k_sur_apply(const SurPair* __restrict__ p
     r[15] += L5 * C[15];
 }
 
-__global__ void k_sur_map(const uint8_t* __restrict__ nearMat, const SurPair* __restrict__ pairs, int n, int Nt,
-                          uint8_t* __restrict__ out, int phase) {
+// The near map of the surrogate's sweep: only the pairs left to the
+// quadrature (the fallback list) are near for the regular kernel, which
+// therefore writes the base records of the surrogate's pairs.
+__global__ void k_sur_map(const int2* __restrict__ fb, int n, int Nt, uint8_t* __restrict__ out, int phase) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (phase == 0) {
-        if (i < (size_t)Nt * Nt) out[i] = nearMat[i];
+        if (i < (size_t)Nt * Nt) out[i] = 0;
     } else if ((int)i < n) {
-        const int a = pairs[i].t1, b = pairs[i].t2;
-        out[(size_t)a * Nt + b] = 0;
-        out[(size_t)b * Nt + a] = 0;
+        const int a = fb[i].x, b = fb[i].y;
+        out[(size_t)a * Nt + b] = 1;
+        out[(size_t)b * Nt + a] = 1;
     }
 }
 
@@ -450,12 +452,10 @@ hipError_t launch_sur_apply(const SurPair* pairs, int n, const float* model, con
     return hipGetLastError();
 }
 
-hipError_t launch_sur_map(const uint8_t* nearMat, const SurPair* pairs, int n, int Nt, uint8_t* out,
-                          hipStream_t s) {
+hipError_t launch_sur_map(const int2* fb, int n, int Nt, uint8_t* out, hipStream_t s) {
     const size_t nn = (size_t)Nt * Nt;
-    hipLaunchKernelGGL(k_sur_map, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, nearMat, pairs, n, Nt, out, 0);
-    if (n > 0)
-        hipLaunchKernelGGL(k_sur_map, dim3((n + 255) / 256), dim3(256), 0, s, nearMat, pairs, n, Nt, out, 1);
+    hipLaunchKernelGGL(k_sur_map, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, fb, n, Nt, out, 0);
+    if (n > 0) hipLaunchKernelGGL(k_sur_map, dim3((n + 255) / 256), dim3(256), 0, s, fb, n, Nt, out, 1);
     return hipGetLastError();
 }
 

@@ -106,7 +106,7 @@ EXPORTS = [
     "efie_opts_default", "efie_opts_preset", "efie_last_error", "efie_last_error_id",
     "efie_mesh_counts", "efie_make_mesh", "efie_rwg_count", "efie_build_rwg", "efie_pick_port",
     "efie_create", "efie_destroy", "efie_set_mesh", "efie_set_rwg", "efie_assemble_zv",
-    "efie_sweep", "efie_zgesv", "efie_zgesv_noalloc", "efie_diag_copy", "efie_diag_cache_counts", "efie_set_prepack", "efie_set_rwg_tables", "efie_assemble_z",
+    "efie_sweep", "efie_zgesv", "efie_zgesv_noalloc", "efie_diag_copy", "efie_diag_cache_counts", "efie_diag_near_pairs", "efie_set_prepack", "efie_set_rwg_tables", "efie_assemble_z",
     "efie_sur_cfg_default", "efie_sur_blob_floats", "efie_sur_train", "efie_sur_get", "efie_sur_set",
     "efie_sur_set_exact", "efie_mlp_params", "efie_mlp_forward", "efie_mlp_loss_grad",
 ]
@@ -147,6 +147,7 @@ def lib():
         L.efie_zgesv_noalloc.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, C.POINTER(Stats)]
         L.efie_diag_copy.argtypes = [C.c_void_p, C.c_size_t, C.c_int, _dp]
         L.efie_diag_cache_counts.argtypes = [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+        L.efie_diag_near_pairs.argtypes = [C.c_void_p, C.POINTER(Opts), C.POINTER(C.c_longlong), _ip, C.c_longlong]
         _fp = C.POINTER(C.c_float)
         L.efie_sur_cfg_default.argtypes = [C.POINTER(SurCfg)]
         L.efie_sur_cfg_default.restype = None
@@ -440,6 +441,16 @@ class Context:
         g, m = C.c_longlong(), C.c_longlong()
         _check(lib().efie_diag_cache_counts(C.byref(g), C.byref(m)))
         return g.value, m.value
+
+    def near_pairs(self, o: Optional[Opts] = None):
+        """[n, 2] int32 array of the near triangle pairs (0-based t1 < t2,
+        ordered by t1 then t2) the fill of o uses (efie_diag.h)."""
+        o = o or opts()
+        n = C.c_longlong()
+        _check(lib().efie_diag_near_pairs(self._c, C.byref(o), C.byref(n), None, 0))
+        out = np.zeros((max(1, n.value), 2), dtype=np.int32)
+        _check(lib().efie_diag_near_pairs(self._c, C.byref(o), C.byref(n), out.ctypes.data_as(_ip), n.value))
+        return out[: n.value]
 
     def diag_copy(self, nbytes, reps=10):
         """Average time (ms) of a plain device copy of nbytes (efie_diag.h)."""

@@ -173,3 +173,17 @@ def test_rhs_normalisation(orc):
     v = P.rhs(pe, ge, 1.0, 1e-12)
     assert v[pe - 1] == 0.5 and v[ge - 1] == -0.5
     assert np.count_nonzero(v) == 2
+
+
+def test_near_pairs_numpy_matches_oracle(orc):
+    """The row-wise numpy near test (tests/near_np.py, used to check the
+    device's near lists at GPU-test sizes) equals the oracle's per-pair
+    triangles_are_near on every pair of the c1 mesh, at both presets."""
+    import near_np
+    V, F = orc.make_mesh(*W.mesh_args("c1"))
+    P = orc.Problem(V, F, orc.build_rwg(V, F))
+    Nt = F.shape[0]
+    for eps in (2.5, 1.2):
+        got = near_np.near_pairs(V, F, eps)
+        want = [(t1, t2) for t1 in range(Nt) for t2 in range(t1 + 1, Nt) if P.near(t1 + 1, t2 + 1, eps)]
+        assert [tuple(p) for p in got] == want
