@@ -193,23 +193,23 @@ def roofline(st, n, nf, fast, sym=False):
 
 
 def series_rate(st, n, nf, Nt=None, z32=True):
-    """HBM rate of the all-pairs series fill (k_series_fill): per launch the
-    batch's Z matrices are written once (16 B FP64 + 8 B FP32 per entry)
-    and the series coefficients of every triangle pair read once
-    ((P+1) x 8 doubles per pair)."""
+    """HBM rate of the series fill (k_series_fill): per launch the batch's Z
+    matrices are written once (16 B FP64 + 8 B FP32 per entry) and the
+    per-entry series coefficients read once (P + 4 doubles per entry of the
+    upper-triangular 16 x 16 tiles)."""
     ks = st["kernels"]["series"]
     P = st["moment_terms"] - 1
-    Nt = Nt or st.get("Nt")
     per_f = (24.0 if z32 else 16.0) * n * n
-    q = 64.0 * (P + 1) * Nt * (Nt + 1) / 2
+    T = (n + 15) // 16
+    q = 8.0 * (P + 4) * 256 * (T * (T + 1) // 2)
     launches = ks["n"]
     alg = (per_f * nf + q * launches) / launches
     t = ks["ms"] * 1e-3 / launches
     meas = traffic("k_series_fill")
     return {"bound": "hbm", "achieved": alg / t / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": alg / t / 1e9 / PEAK_HBM_GBS, "traffic": meas,
-            "work": f"{nf / launches:.1f} frequencies per launch x {per_f:.3e} B of Z written + {q:.3e} B of series "
-                    f"read ({alg:.3e} B per launch), {t * 1e3:.2f} ms per launch, {t * 1e3 * launches / nf:.3f} ms "
+            "work": f"{nf / launches:.1f} frequencies per launch x {per_f:.3e} B of Z written + {q:.3e} B of "
+                    f"coefficients read ({alg:.3e} B per launch), {t * 1e3:.2f} ms per launch, {t * 1e3 * launches / nf:.3f} ms "
                     f"per frequency"}
 
 
