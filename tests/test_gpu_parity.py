@@ -132,6 +132,40 @@ def W_GOLDEN(name):
     return os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", name)
 
 
+# Whole Z at the c1 mesh (Ne = 584) against the oracle's whole Z
+# (assemble_ZV_EFIE_RWG_DIFF_CPU.m:9-36): one lossy and one lossless
+# frequency, every near-pair mode (EXTRACT against the oracle's extraction),
+# and the Gaussian excitation at two spreads.
+C1_FULL_W = 2 * np.pi * 2.5e9
+C1_FULL_EPS = {"lossy": W.EPS0 * (2.2 - 0.01j), "lossless": W.EPS0 * 2.2}
+
+
+@pytest.fixture(scope="module")
+def c1_full(efie_mod, orc):
+    V, F, r, pe, ge, spread, P = _setup(efie_mod, orc, "c1")
+    out = {}
+    for name, eps in C1_FULL_EPS.items():
+        k = orc.wavenumber(C1_FULL_W, W.MU0, eps)
+        for ex in ((0, 1) if name == "lossy" else (0,)):
+            out[name, ex] = P.z_full(k, C1_FULL_W, W.MU0, eps, orc.opts(**W.OPTS_C1, extract=ex))
+    return (V, F, r, pe, ge, spread, P), out
+
+
+@pytest.mark.parametrize("case", [("lossy", 0), ("lossy", 1), ("lossy", 2), ("lossless", 0), ("lossless", 1)])
+def test_c1_full_z_vs_oracle(efie_mod, orc, ctx, c1_full, case):
+    (V, F, r, pe, ge, spread, P), Zo = c1_full
+    name, near_mode = case
+    eps = C1_FULL_EPS[name]
+    k = orc.wavenumber(C1_FULL_W, W.MU0, eps)
+    ctx.set_problem(V, F, r)
+    for sp in (spread, 2.5 * spread):
+        Zg, Vg = ctx.assemble_zv(k, C1_FULL_W, W.MU0, eps, [pe, ge, 1.0, sp, 0.2],
+                                 efie_mod.opts(**W.OPTS_C1, near_mode=near_mode))
+        assert np.abs(Vg - P.rhs(pe, ge, 1.0, sp)).max() <= 1e-15
+    assert _entry_rel(Zg, Zo[name, 1 if near_mode == efie_mod.NEAR_EXTRACT else 0]) <= ZTOL
+    assert np.array_equal(Zg, Zg.T)
+
+
 @pytest.mark.parametrize("prec,near_mode,batch", [(0, 0, 0), (0, 1, 2), (1, 1, 0), (1, 0, 5)])
 def test_c1_sweep_vs_golden(efie_mod, orc, ctx, prec, near_mode, batch):
     g = np.load(W_GOLDEN("c1.npz"))

@@ -175,6 +175,29 @@ def test_rhs_normalisation(orc):
     assert np.count_nonzero(v) == 2
 
 
+def test_rhs_gaussian_exponent(orc):
+    """The excitation falls as exp(-(d/s)^2) of the distance d between edge
+    centres on the port's layer (assemble_ZV_EFIE_RWG_DIFF_CPU.m:63).  The
+    port edge has weight 1 before normalisation, so -log(v_e / v_port) is
+    (d_e / s)^2: at spreads s and 2s it falls exactly fourfold, where
+    exp(-d/s) would give a factor 2 and exp(-d^2/s) a factor 2 too."""
+    V, F = orc.make_mesh(*W.mesh_args("c1"))
+    r = orc.build_rwg(V, F)
+    P = orc.Problem(V, F, r)
+    pP, _ = W.feed_points("c1")
+    pe = orc.pick_port(r, pP, 1)
+    ge = orc.pick_port(r, W.feed_points("c1")[1], -1)
+    s = 1.5 * r.len.mean()
+    on = np.flatnonzero(r.zSign == 1)
+    d = np.linalg.norm(r.center[on] - r.center[pe - 1], axis=1)
+    keep = (d > 0) & (d < 2.0 * s)  # weights well above underflow at both spreads
+    l1 = -np.log(P.rhs(pe, ge, 1.0, s).real[on][keep] / P.rhs(pe, ge, 1.0, s).real[pe - 1])
+    l2 = -np.log(P.rhs(pe, ge, 1.0, 2 * s).real[on][keep] / P.rhs(pe, ge, 1.0, 2 * s).real[pe - 1])
+    assert keep.sum() >= 10
+    assert np.allclose(l1, (d[keep] / s) ** 2, rtol=1e-12, atol=0)
+    assert np.allclose(l1 / l2, 4.0, rtol=1e-12, atol=0)
+
+
 def test_near_pairs_numpy_matches_oracle(orc):
     """The row-wise numpy near test (tests/near_np.py, used to check the
     device's near lists at GPU-test sizes) equals the oracle's per-pair
