@@ -353,7 +353,7 @@ def bench_c4(a):
                     zin, _, st = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, sp, 50.0, o, stats=True)
                     dt = time.perf_counter() - t0
                     per[i] = {"N": int(r.Ne), "s": dt, "freq_per_s": len(fr) / dt, "ir_iters_max": st["ir_iters_max"],
-                              "n_lu_retry": st["n_lu_retry"]}
+                              "n_lu_retry": st["n_lu_retry"], "ms_series_build": st["ms_moments"]}
                     log(f"[c4] geometry {i}: N={r.Ne} {dt:.2f} s")
             except Exception as e:   # reported after the join
                 fails.append(e)

@@ -55,8 +55,9 @@ typedef struct {
     int near_order;
     int model;          /* EFIE_MODEL_*                                       */
     int precision;      /* EFIE_PREC_*                                        */
-    int batch;          /* frequencies factored together (0 = automatic: up   *
-                         * to 52, two batches within 60% of free memory)      */
+    int batch;          /* frequencies factored together (0 = automatic: a   *
+                         * batch of 64-256 matrices, more for smaller n, the  *
+                         * buffers within 60% of free memory)                 */
     double ir_tol;      /* mixed: stop refinement at ||V-Zx||/||V|| <= ir_tol  *
                          * or when a correction no longer halves the residual */
     int ir_max_iter;    /* mixed: refinement iteration cap                    */

@@ -1680,27 +1680,34 @@ efie_status fill_single(efie_ctx* c, const efie_opts& o, double k_re, double k_i
 }
 
 
-// Frequencies factored together: latency-bound steps of the LU (pivot
-// selection, panel solves, triangular solves) cost per launch, so a batch
-// holds up to 52 matrices (larger batches leave too little of the sweep
-// to pipeline), limited so two batches fit in 60% of the free device
-// memory, and split evenly over the sweep (201 frequencies at n = 6009 ->
-// 4 batches of 51).
+// Frequencies factored together.  The latency-bound steps of the LU (pivot
+// selection, panel solves, triangular-solve chains) cost per launch, not
+// per matrix, so a batch should hold enough matrix elements to bury them:
+// about kBatchElems (64 matrices at n = 6009, 256 below n ~ 3000), never
+// fewer than 64.  A sweep that fits one batch needs one buffer of
+// matrices; a longer one is pipelined over two (one filled while the other
+// is factored), and its batches are split evenly (201 frequencies at
+// n = 6009 -> 4 batches of 51).  Either way the buffers stay within 60% of
+// the free device memory.
+constexpr double kBatchElems = 2.4e9;
+
 int batch_size(const efie_opts& o, size_t zElems, int Nf) {
-    int B = o.batch > 0 ? o.batch : 52;
+    const size_t per = zElems * (o.precision == EFIE_PREC_MIXED ? 24 : 16);
+    int B = o.batch;
+    if (B <= 0) B = (int)std::min(256.0, std::max(64.0, kBatchElems / (double)std::max<size_t>(zElems, 1)));
+    B = std::max(1, std::min(B, std::max(Nf, 1)));
     size_t freeB = 0, totB = 0;
-    if (hipMemGetInfo(&freeB, &totB) == hipSuccess) {
-        // two batches of matrices are resident while one is filled and the
-        // other factored
-        const size_t per = 2 * zElems * (o.precision == EFIE_PREC_MIXED ? 24 : 16);
-        const size_t cap = (size_t)(0.6 * (double)freeB) / std::max<size_t>(per, 1);
-        if (cap < (size_t)B) B = (int)std::max<size_t>(1, cap);
-    }
+    if (hipMemGetInfo(&freeB, &totB) != hipSuccess) return B;
+    auto cap = [&](int nbuf) {
+        return (size_t)(0.6 * (double)freeB) / std::max<size_t>(per * (size_t)nbuf, 1);
+    };
+    if (B >= Nf && cap(1) >= (size_t)B) return B;  // the whole sweep, one buffer
+    if (cap(2) < (size_t)B) B = (int)std::max<size_t>(1, cap(2));
     if (Nf > 0 && o.batch <= 0) {
         const int nbatch = (Nf + B - 1) / B;
         B = (Nf + nbatch - 1) / nbatch;
     }
-    return std::max(1, std::min(B, std::max(Nf, 1)));
+    return std::max(1, B);
 }
 
 // Host bytes of one batch's results: info and iterations (int), final
