@@ -139,21 +139,97 @@ __device__ __forceinline__ void cell_coords(const CellGrid& g, int c, int& ix, i
     iz = c / (g.nx * g.ny);
 }
 
-// COUNT: rowCount[t1] = number of near t2 > t1.  Otherwise the near t2 > t1
-// of row t1 go to list[rowStart[t1]..] sorted by t2 (the order of the dense
-// scan over t2 that the reference's near matrix implies).
-template <bool COUNT>
-__global__ __launch_bounds__(256) void k_near_bins(const TriGeo* __restrict__ tg, int Nt, double eps, CellGrid g,
-                                                   const int* __restrict__ cellOf, const int* __restrict__ cellStart,
-                                                   const int* __restrict__ cellTri, int* __restrict__ rowCount,
-                                                   const int* __restrict__ rowStart, int2* __restrict__ list) {
+// rowCount[t1] = number of near t2 > t1 (one thread per row).
+__global__ __launch_bounds__(256) void k_near_count(const TriGeo* __restrict__ tg, int Nt, double eps, CellGrid g,
+                                                    const int* __restrict__ cellOf, const int* __restrict__ cellStart,
+                                                    const int* __restrict__ cellTri, int* __restrict__ rowCount) {
     const int t1 = blockIdx.x * blockDim.x + threadIdx.x;
     if (t1 >= Nt) return;
     const TriGeo a = tg[t1];
     int ix, iy, iz;
     cell_coords(g, cellOf[t1], ix, iy, iz);
     int cnt = 0;
-    int2* row = COUNT ? nullptr : list + rowStart[t1];
+    for (int dz = -1; dz <= 1; ++dz) {
+        const int z = iz + dz;
+        if (z < 0 || z >= g.nz) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int y = iy + dy;
+            if (y < 0 || y >= g.ny) continue;
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int x = ix + dx;
+                if (x < 0 || x >= g.nx) continue;
+                const int c = (z * g.ny + y) * g.nx + x;
+                for (int k = cellStart[c]; k < cellStart[c + 1]; ++k) {
+                    const int t2 = cellTri[k];
+                    cnt += (t2 > t1 && near_test(a, tg[t2], eps)) ? 1 : 0;
+                }
+            }
+        }
+    }
+    rowCount[t1] = cnt;
+}
+
+// The near t2 > t1 of row t1 to list[rowStart[t1]..], sorted by t2 (the
+// order of the dense scan over t2 that the reference's near matrix
+// implies).  One wave per row: the lanes test 64 candidates of a cell at a
+// time and append the near ones to the row's LDS buffer (ballot prefix
+// counts); the row is then written in order, each entry at its rank.  A
+// row longer than the buffer is built by lane 0 alone, by insertion into
+// the list.
+constexpr int kNearRowCap = 512;
+
+__global__ __launch_bounds__(256) void k_near_rows(const TriGeo* __restrict__ tg, int Nt, double eps, CellGrid g,
+                                                   const int* __restrict__ cellOf, const int* __restrict__ cellStart,
+                                                   const int* __restrict__ cellTri, const int* __restrict__ rowStart,
+                                                   int2* __restrict__ list) {
+    __shared__ int buf[4][kNearRowCap];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int t1 = blockIdx.x * 4 + wv;
+    if (t1 >= Nt) return;
+    int* rb = buf[wv];
+    const TriGeo a = tg[t1];
+    int ix, iy, iz;
+    cell_coords(g, cellOf[t1], ix, iy, iz);
+    const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    int cnt = 0;
+    for (int dz = -1; dz <= 1; ++dz) {
+        const int z = iz + dz;
+        if (z < 0 || z >= g.nz) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int y = iy + dy;
+            if (y < 0 || y >= g.ny) continue;
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int x = ix + dx;
+                if (x < 0 || x >= g.nx) continue;
+                const int c = (z * g.ny + y) * g.nx + x;
+                const int k1 = cellStart[c + 1];
+                for (int k0 = cellStart[c]; k0 < k1; k0 += 64) {
+                    const int k = k0 + lane;
+                    const int t2 = k < k1 ? cellTri[k] : -1;
+                    const bool nr = t2 > t1 && near_test(a, tg[t2], eps);
+                    const unsigned long long m = __ballot(nr);
+                    const int at = cnt + __popcll(m & below);
+                    if (nr && at < kNearRowCap) rb[at] = t2;
+                    cnt += __popcll(m);
+                }
+            }
+        }
+    }
+    int2* row = list + rowStart[t1];
+    if (cnt <= kNearRowCap) {
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int e = lane; e < cnt; e += 64) {
+            const int v = rb[e];
+            int rank = 0;
+            for (int j = 0; j < cnt; ++j) rank += rb[j] < v ? 1 : 0;
+            row[rank] = make_int2(t1, v);
+        }
+        return;
+    }
+    if (lane != 0) return;
+    int n = 0;
     for (int dz = -1; dz <= 1; ++dz) {
         const int z = iz + dz;
         if (z < 0 || z >= g.nz) continue;
@@ -167,21 +243,17 @@ __global__ __launch_bounds__(256) void k_near_bins(const TriGeo* __restrict__ tg
                 for (int k = cellStart[c]; k < cellStart[c + 1]; ++k) {
                     const int t2 = cellTri[k];
                     if (t2 <= t1 || !near_test(a, tg[t2], eps)) continue;
-                    if (!COUNT) {
-                        // insertion into the row, kept sorted by t2
-                        int p = cnt;
-                        while (p > 0 && row[p - 1].y > t2) {
-                            row[p] = row[p - 1];
-                            --p;
-                        }
-                        row[p] = make_int2(t1, t2);
+                    int p = n;
+                    while (p > 0 && row[p - 1].y > t2) {
+                        row[p] = row[p - 1];
+                        --p;
                     }
-                    ++cnt;
+                    row[p] = make_int2(t1, t2);
+                    ++n;
                 }
             }
         }
     }
-    if (COUNT) rowCount[t1] = cnt;
 }
 
 // Ordered compaction of the near pairs (t1 < t2) of one row.
@@ -1032,7 +1104,6 @@ __global__ __launch_bounds__(256) void k_combine(const Rec* __restrict__ recs, c
 // by e^{z d} is the exact Cauchy product with d^i/i!.  Per frequency an
 // entry then costs one phase and one polynomial in z.
 
-constexpr int kSerChunk = 4;
 
 // (R - R0)^q / q! / R for q = q0.. is t, t*d/(q0+1), ...: 1/n for n = 1..32
 __constant__ double kInvN[33] = {0.0,        1.0,        1.0 / 2,    1.0 / 3,    1.0 / 4,    1.0 / 5,    1.0 / 6,
@@ -1041,16 +1112,42 @@ __constant__ double kInvN[33] = {0.0,        1.0,        1.0 / 2,    1.0 / 3,   
                                  1.0 / 21,   1.0 / 22,   1.0 / 23,   1.0 / 24,   1.0 / 25,   1.0 / 26,   1.0 / 27,
                                  1.0 / 28,   1.0 / 29,   1.0 / 30,   1.0 / 31,   1.0 / 32};
 
-// d^q0 / q0! times w/R
+// Orders summed per pass over the point pairs: regular pairs (49 point
+// pairs, many pairs per block) and subdivided pairs (one wave per pair).
+constexpr int kSerReg = 4;
+constexpr int kSerSub = 8;
+
+constexpr double inv_factorial(int q) {
+    double r = 1.0;
+    for (int i = 2; i <= q; ++i) r /= i;
+    return r;
+}
+// 1/q! for q = 0..32
+__constant__ double kInvFact[33] = {
+    inv_factorial(0),  inv_factorial(1),  inv_factorial(2),  inv_factorial(3),  inv_factorial(4),
+    inv_factorial(5),  inv_factorial(6),  inv_factorial(7),  inv_factorial(8),  inv_factorial(9),
+    inv_factorial(10), inv_factorial(11), inv_factorial(12), inv_factorial(13), inv_factorial(14),
+    inv_factorial(15), inv_factorial(16), inv_factorial(17), inv_factorial(18), inv_factorial(19),
+    inv_factorial(20), inv_factorial(21), inv_factorial(22), inv_factorial(23), inv_factorial(24),
+    inv_factorial(25), inv_factorial(26), inv_factorial(27), inv_factorial(28), inv_factorial(29),
+    inv_factorial(30), inv_factorial(31), inv_factorial(32)};
+
+// d^q0 / q0! times w/R, q0 a multiple of the pass width C (a power of 2):
+// (d^C)^(q0/C) / q0!
+template <int C>
 __device__ __forceinline__ double series_head(double wInvR, double d, int q0) {
-    double t = wInvR;
-    for (int e = 1; e <= q0; ++e) t *= d * kInvN[e];
+    static_assert((C & (C - 1)) == 0, "pass width must be a power of 2");
+    double t = wInvR * kInvFact[q0];
+    double dc = d;
+#pragma unroll
+    for (int k = 1; k < C; k <<= 1) dc *= dc;
+    for (int e = 0; e < q0; e += C) t *= dc;
     return t;
 }
 
 // Regular pairs: tile of 64 test triangles (one per lane) by 64 source
 // triangles, as the per-frequency regular kernel; each pair's NQ x NQ
-// product rule is summed in chunks of kSerChunk orders.
+// product rule is summed in chunks of kSerReg orders.
 template <int NQ>
 __global__ __launch_bounds__(256) void k_series_regular(const Pt* __restrict__ pts, const TriGeo* __restrict__ tg,
                                                         int Nt, const uint8_t* __restrict__ nearMat, double eps,
@@ -1084,19 +1181,19 @@ __global__ __launch_bounds__(256) void k_series_regular(const Pt* __restrict__ p
         const double ox = g1.cx - g2.cx, oy = g1.cy - g2.cy, oz = g1.cz - g2.cz;
         const double R0 = sqrt(ox * ox + oy * oy + oz * oz);
         double* out = Q + rec_index(t1, t2) * stride;
-        for (int q0 = 0; q0 <= P; q0 += kSerChunk) {
-            double acc[kSerChunk][8];
+        for (int q0 = 0; q0 <= P; q0 += kSerReg) {
+            double acc[kSerReg][8];
 #pragma unroll
-            for (int c = 0; c < kSerChunk; ++c)
+            for (int c = 0; c < kSerReg; ++c)
 #pragma unroll
                 for (int x = 0; x < 8; ++x) acc[c][x] = 0.0;
 #pragma unroll 1
             for (int i = 0; i < NQ; ++i) {
                 const Pt pi = p1[i];
                 const double xi = pi.x + ox, yi = pi.y + oy, zi = pi.z + oz;
-                double g[kSerChunk], hx[kSerChunk], hy[kSerChunk], hz[kSerChunk];
+                double g[kSerReg], hx[kSerReg], hy[kSerReg], hz[kSerReg];
 #pragma unroll
-                for (int c = 0; c < kSerChunk; ++c) g[c] = hx[c] = hy[c] = hz[c] = 0.0;
+                for (int c = 0; c < kSerReg; ++c) g[c] = hx[c] = hy[c] = hz[c] = 0.0;
 #pragma unroll
                 for (int j = 0; j < NQ; ++j) {
                     const Pt q2 = p2[j];
@@ -1105,9 +1202,9 @@ __global__ __launch_bounds__(256) void k_series_regular(const Pt* __restrict__ p
                     const double R2 = fmax(dx * dx + dy * dy + dz * dz, 1e-30);
                     const double invR = rsqrt64(R2);
                     const double d = R2 * invR - R0;
-                    double t = series_head(q2.w * invR, d, q0);
+                    double t = series_head<kSerReg>(q2.w * invR, d, q0);
 #pragma unroll
-                    for (int c = 0; c < kSerChunk; ++c) {
+                    for (int c = 0; c < kSerReg; ++c) {
                         g[c] += t;
                         hx[c] += t * q2.x;
                         hy[c] += t * q2.y;
@@ -1116,7 +1213,7 @@ __global__ __launch_bounds__(256) void k_series_regular(const Pt* __restrict__ p
                     }
                 }
 #pragma unroll
-                for (int c = 0; c < kSerChunk; ++c) {
+                for (int c = 0; c < kSerReg; ++c) {
                     const double w = pi.w;
                     acc[c][0] += w * g[c];
                     acc[c][1] += w * g[c] * pi.x;
@@ -1129,7 +1226,7 @@ __global__ __launch_bounds__(256) void k_series_regular(const Pt* __restrict__ p
                 }
             }
 #pragma unroll
-            for (int c = 0; c < kSerChunk; ++c)
+            for (int c = 0; c < kSerReg; ++c)
                 if (q0 + c <= P) {
                     double2* o2 = reinterpret_cast<double2*>(out + (size_t)(q0 + c) * 8);
 #pragma unroll
@@ -1149,10 +1246,10 @@ __device__ void series_subdiv(const Pt* __restrict__ P1, const Pt* __restrict__ 
                               double oy, double oz, double R0, double aNear2, const double* __restrict__ aSelf, int P,
                               double* __restrict__ out) {
     const int lane = threadIdx.x & 63;
-    for (int q0 = 0; q0 <= P; q0 += kSerChunk) {
-        double acc[kSerChunk][8];
+    for (int q0 = 0; q0 <= P; q0 += kSerSub) {
+        double acc[kSerSub][8];
 #pragma unroll
-        for (int c = 0; c < kSerChunk; ++c)
+        for (int c = 0; c < kSerSub; ++c)
 #pragma unroll
             for (int x = 0; x < 8; ++x) acc[c][x] = 0.0;
         for (int base = 0; base < Np; base += 64) {
@@ -1170,9 +1267,9 @@ __device__ void series_subdiv(const Pt* __restrict__ P1, const Pt* __restrict__ 
                 const double as = aSelf[psub];
                 aS2 = as * as;
             }
-            double g[kSerChunk], hx[kSerChunk], hy[kSerChunk], hz[kSerChunk];
+            double g[kSerSub], hx[kSerSub], hy[kSerSub], hz[kSerSub];
 #pragma unroll
-            for (int c = 0; c < kSerChunk; ++c) g[c] = hx[c] = hy[c] = hz[c] = 0.0;
+            for (int c = 0; c < kSerSub; ++c) g[c] = hx[c] = hy[c] = hz[c] = 0.0;
             for (int j = j0; j < j1; ++j) {
                 const Pt q2 = P2[j];
                 const double dx = xi - q2.x, dy = yi - q2.y, dz = zi - q2.z;
@@ -1186,9 +1283,9 @@ __device__ void series_subdiv(const Pt* __restrict__ P1, const Pt* __restrict__ 
                 }
                 const double invR = rsqrt64(R2);
                 const double d = R2 * invR - R0;
-                double t = series_head(q2.w * invR, d, q0);
+                double t = series_head<kSerSub>(q2.w * invR, d, q0);
 #pragma unroll
-                for (int c = 0; c < kSerChunk; ++c) {
+                for (int c = 0; c < kSerSub; ++c) {
                     g[c] += t;
                     hx[c] += t * q2.x;
                     hy[c] += t * q2.y;
@@ -1197,7 +1294,7 @@ __device__ void series_subdiv(const Pt* __restrict__ P1, const Pt* __restrict__ 
                 }
             }
 #pragma unroll
-            for (int c = 0; c < kSerChunk; ++c) {
+            for (int c = 0; c < kSerSub; ++c) {
                 const double w = pi.w;
                 acc[c][0] += w * g[c];
                 acc[c][1] += w * g[c] * pi.x;
@@ -1210,7 +1307,7 @@ __device__ void series_subdiv(const Pt* __restrict__ P1, const Pt* __restrict__ 
             }
         }
 #pragma unroll
-        for (int c = 0; c < kSerChunk; ++c)
+        for (int c = 0; c < kSerSub; ++c)
 #pragma unroll
             for (int x = 0; x < 8; ++x) {
                 const double v = wave_sum(acc[c][x]);
@@ -1302,7 +1399,8 @@ __global__ __launch_bounds__(256) void k_series_coeffs(const double* __restrict_
     double D[NT];
 #pragma unroll
     for (int j = 0; j < NT; ++j) D[j] = 0.0;
-#pragma unroll 1
+    // unrolled, so the side and pair arrays stay in registers
+#pragma unroll
     for (int ab = 0; ab < 4; ++ab) {
         const int a = ab >> 1, b = ab & 1;
         if (tm[a] < 0 || tn[b] < 0) continue;
@@ -1618,13 +1716,12 @@ hipError_t launch_near_bins(const TriGeo* tg, int Nt, double eps, double x0, dou
                             int ny, int nz, const int* cellOf, const int* cellStart, const int* cellTri,
                             int* rowCount, const int* rowStart, int2* list, hipStream_t s) {
     const CellGrid g{x0, y0, z0, h, nx, ny, nz};
-    const dim3 grid((Nt + 255) / 256), blk(256);
     if (!list)
-        hipLaunchKernelGGL(k_near_bins<true>, grid, blk, 0, s, tg, Nt, eps, g, cellOf, cellStart, cellTri, rowCount,
-                           nullptr, nullptr);
+        hipLaunchKernelGGL(k_near_count, dim3((Nt + 255) / 256), dim3(256), 0, s, tg, Nt, eps, g, cellOf, cellStart,
+                           cellTri, rowCount);
     else
-        hipLaunchKernelGGL(k_near_bins<false>, grid, blk, 0, s, tg, Nt, eps, g, cellOf, cellStart, cellTri, nullptr,
-                           rowStart, list);
+        hipLaunchKernelGGL(k_near_rows, dim3((Nt + 3) / 4), dim3(256), 0, s, tg, Nt, eps, g, cellOf, cellStart,
+                           cellTri, rowStart, list);
     return hipGetLastError();
 }
 
