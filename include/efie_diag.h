@@ -29,6 +29,16 @@ efie_status efie_diag_cache_counts(long long* geometry_builds, long long* moment
  * be NULL with cap 0). */
 efie_status efie_diag_near_pairs(efie_ctx* ctx, const efie_opts* opts, long long* n, int* pairs, long long cap);
 
+/* The quadrature points the fill of *opts uses on the context's mesh
+ * (built on the device), copied out: set 0 the regular rule [Nt][Nq],
+ * 1 the near-pair children [Nt][4^near_subdiv][Nq2], 2 the self children
+ * [Nt][4^self_subdiv][Nq2], each point as (x, y, z, w) relative to its
+ * triangle's centroid; set 3 the self softening radii [Nt][4^self_subdiv];
+ * set 4 the radius of each triangle [Nt].  *n is the number of doubles;
+ * the first min(*n, cap) go to out (out may be NULL with cap 0). */
+efie_status efie_diag_quad_points(efie_ctx* ctx, const efie_opts* opts, int set, long long* n, double* out,
+                                  long long cap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -253,6 +253,7 @@ struct efie_ctx {
     bool momExtract = false;
     bool momSelfOnly = false;  // only the self-pair moments were computed
     DBuf dTriV;  // triangle vertices relative to the centroid [Nt][9]
+    DBuf dTriAbs;  // triangle vertices [Nt][9], then 4 radius maxima of the point generator
     // frequency-independent series of every triangle pair (near_mode
     // MOMENTS), valid for P = serP and near_alpha = serAlpha
     DBuf dQall, dSerC;  // pair series (transient) and per-entry coefficients
@@ -974,6 +975,7 @@ efie_status ensure_geometry(efie_ctx* c, const efie_opts* o) {
         aeqMax = std::max(aeqMax, std::sqrt((0.5 * tg[t].area2) / kPi));
     }
     int Nq, Nq2, NsN, NsS;
+    QuadRules rules{};
     std::vector<Pt> reg, nearPts, selfPts;
     std::vector<double> aSelf, aeq(Nt), rad(Nt, 0.0);
     double rmaxN = 0, rmaxS = 0, aSelfMax = 0;
@@ -1003,50 +1005,26 @@ efie_status ensure_geometry(efie_ctx* c, const efie_opts* o) {
         aeqMax = 0;
         for (double a : aeq) aeqMax = std::max(aeqMax, a);
     } else {
-        double qu[16], qv[16], qw[16];
-        // regular rule: points relative to the centroid, weight qw * area2
-        Nq = quad_rule(o->quad_order, qu, qv, qw);
-        reg.resize((size_t)Nt * Nq);
-        for (int t = 0; t < Nt; ++t)
-            for (int i = 0; i < Nq; ++i) {
-                V3 r = map_ref(tv[3 * t], tv[3 * t + 1], tv[3 * t + 2], qu[i], qv[i]);
-                reg[(size_t)t * Nq + i] = {r.x - tg[t].cx, r.y - tg[t].cy, r.z - tg[t].cz, qw[i] * tg[t].area2};
-            }
-        // subdivided rules for near and self pairs
+        // rule-derived points: built on the device from the vertices
+        // (launch_quad_points, below)
         double su[16], sv[16], sw[16];
+        Nq = quad_rule(o->quad_order, rules.ru, rules.rv, rules.rw);
         Nq2 = quad_rule(o->near_order, su, sv, sw);
-        auto subdiv_pts = [&](int depth, std::vector<Pt>& pts, std::vector<double>* aS, double& rmax) {
-            const int Ns = 1 << (2 * depth);
-            pts.resize((size_t)Nt * Ns * Nq2);
-            if (aS) aS->resize((size_t)Nt * Ns);
-            std::vector<V3> sub;
-            for (int t = 0; t < Nt; ++t) {
-                subdivide(tv[3 * t], tv[3 * t + 1], tv[3 * t + 2], depth, sub);
-                for (int p = 0; p < Ns; ++p) {
-                    V3 A = sub[3 * p], B = sub[3 * p + 1], C = sub[3 * p + 2];
-                    const double a2p = norm(cross(B - A, C - A));
-                    if (aS) {
-                        const double as = o->self_alpha * std::sqrt((0.5 * a2p) / kPi);
-                        (*aS)[(size_t)t * Ns + p] = as;
-                        aSelfMax = std::max(aSelfMax, as);
-                    }
-                    for (int i = 0; i < Nq2; ++i) {
-                        V3 r = map_ref(A, B, C, su[i], sv[i]);
-                        Pt q = {r.x - tg[t].cx, r.y - tg[t].cy, r.z - tg[t].cz, sw[i] * a2p};
-                        pts[((size_t)t * Ns + p) * Nq2 + i] = q;
-                        rmax = std::max(rmax, std::sqrt(q.x * q.x + q.y * q.y + q.z * q.z));
-                    }
-                }
-            }
-            return Ns;
-        };
-        NsN = subdiv_pts(o->near_subdiv, nearPts, nullptr, rmaxN);
-        NsS = subdiv_pts(o->self_subdiv, selfPts, &aSelf, rmaxS);
+        for (int i = 0; i < Nq2; ++i) {
+            rules.su[i] = su[i];
+            rules.sv[i] = sv[i];
+            rules.sw[i] = sw[i];
+        }
+        rules.nReg = Nq;
+        rules.nSub = Nq2;
+        NsN = 1 << (2 * o->near_subdiv);
+        NsS = 1 << (2 * o->self_subdiv);
         for (int t = 0; t < Nt; ++t) aeq[t] = std::sqrt((0.5 * tg[t].area2) / kPi);
     }
-    // radius bound of each triangle about its centroid: its vertices (and,
-    // with a prepack, every given point)
-    for (int t = 0; t < Nt && !c->meshless; ++t) {
+    // radius bound of each triangle about its centroid: every given point
+    // and, with mesh arrays, its vertices (the device generator folds in
+    // both for rule-derived points)
+    for (int t = 0; t < Nt && c->havePack && !c->meshless; ++t) {
         const V3 cc{tg[t].cx, tg[t].cy, tg[t].cz};
         rad[t] = std::max(rad[t], std::max(norm(tv[3 * t] - cc), std::max(norm(tv[3 * t + 1] - cc),
                                                                          norm(tv[3 * t + 2] - cc))));
@@ -1087,22 +1065,51 @@ efie_status ensure_geometry(efie_ctx* c, const efie_opts* o) {
     HIPCHK(c->dTriV.ensure(sizeof(double) * triV.size()));
     HIPCHK(hipMemcpyAsync(c->dTriV.p, triV.data(), sizeof(double) * triV.size(), hipMemcpyHostToDevice, s));
     HIPCHK(c->dTri.ensure(sizeof(TriGeo) * Nt));
-    HIPCHK(c->dReg.ensure(sizeof(Pt) * reg.size()));
-    HIPCHK(c->dNearPts.ensure(sizeof(Pt) * nearPts.size()));
-    HIPCHK(c->dSelfPts.ensure(sizeof(Pt) * selfPts.size()));
-    HIPCHK(c->dASelf.ensure(sizeof(double) * aSelf.size()));
+    const size_t nReg = (size_t)Nt * Nq, nNearPts = (size_t)Nt * NsN * Nq2, nSelfPts = (size_t)Nt * NsS * Nq2;
+    HIPCHK(c->dReg.ensure(sizeof(Pt) * nReg));
+    HIPCHK(c->dNearPts.ensure(sizeof(Pt) * nNearPts));
+    HIPCHK(c->dSelfPts.ensure(sizeof(Pt) * nSelfPts));
+    HIPCHK(c->dASelf.ensure(sizeof(double) * (size_t)Nt * NsS));
     HIPCHK(c->dAeq.ensure(sizeof(double) * Nt));
     HIPCHK(c->dRad.ensure(sizeof(double) * Nt));
     HIPCHK(c->dSideTri.ensure(sizeof(int) * sideTri.size()));
     HIPCHK(c->dSide.ensure(sizeof(Side) * side.size()));
     HIPCHK(c->dRowCnt.ensure(sizeof(int) * Nt));
     HIPCHK(hipMemcpyAsync(c->dTri.p, tg.data(), sizeof(TriGeo) * Nt, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(c->dReg.p, reg.data(), sizeof(Pt) * reg.size(), hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(c->dNearPts.p, nearPts.data(), sizeof(Pt) * nearPts.size(), hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(c->dSelfPts.p, selfPts.data(), sizeof(Pt) * selfPts.size(), hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(c->dASelf.p, aSelf.data(), sizeof(double) * aSelf.size(), hipMemcpyHostToDevice, s));
+    double radMax = 0;
+    if (c->havePack) {
+        HIPCHK(hipMemcpyAsync(c->dReg.p, reg.data(), sizeof(Pt) * nReg, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(c->dNearPts.p, nearPts.data(), sizeof(Pt) * nNearPts, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(c->dSelfPts.p, selfPts.data(), sizeof(Pt) * nSelfPts, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(c->dASelf.p, aSelf.data(), sizeof(double) * aSelf.size(), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(c->dRad.p, rad.data(), sizeof(double) * Nt, hipMemcpyHostToDevice, s));
+        for (int t = 0; t < Nt; ++t) radMax = std::max(radMax, rad[t]);
+    } else {
+        // every point on the device from the vertices (9 doubles per
+        // triangle), with the radius maxima
+        std::vector<double> tvAbs(9 * (size_t)Nt);
+        for (int t = 0; t < Nt; ++t)
+            for (int q = 0; q < 3; ++q) {
+                tvAbs[9 * (size_t)t + 3 * q] = tv[3 * t + q].x;
+                tvAbs[9 * (size_t)t + 3 * q + 1] = tv[3 * t + q].y;
+                tvAbs[9 * (size_t)t + 3 * q + 2] = tv[3 * t + q].z;
+            }
+        HIPCHK(c->dTriAbs.ensure(sizeof(double) * (tvAbs.size() + 4)));
+        double* dMx = c->dTriAbs.as<double>() + tvAbs.size();
+        HIPCHK(hipMemcpyAsync(c->dTriAbs.p, tvAbs.data(), sizeof(double) * tvAbs.size(), hipMemcpyHostToDevice, s));
+        HIPCHK(launch_quad_points(c->dTriAbs.as<double>(), c->dTri.as<TriGeo>(), Nt, rules, o->near_subdiv,
+                                  o->self_subdiv, o->self_alpha, c->dReg.as<Pt>(), c->dNearPts.as<Pt>(),
+                                  c->dSelfPts.as<Pt>(), c->dASelf.as<double>(), c->dRad.as<double>(), dMx, s));
+        double mx[4];
+        HIPCHK(hipMemcpyAsync(mx, dMx, sizeof(mx), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        rmaxN = mx[0];
+        rmaxS = mx[1];
+        aSelfMax = mx[2];
+        radMax = mx[3];
+    }
+
     HIPCHK(hipMemcpyAsync(c->dAeq.p, aeq.data(), sizeof(double) * Nt, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(c->dRad.p, rad.data(), sizeof(double) * Nt, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(c->dSideTri.p, sideTri.data(), sizeof(int) * sideTri.size(), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(c->dSide.p, side.data(), sizeof(Side) * side.size(), hipMemcpyHostToDevice, s));
 
@@ -1168,8 +1175,7 @@ efie_status ensure_geometry(efie_ctx* c, const efie_opts* o) {
     c->dNearMax = dNearMax;
     c->aNearMax = aeqMax;
     c->aSelfMax = aSelfMax;
-    double radMax = 0, dcMax = 0;
-    for (int t = 0; t < Nt; ++t) radMax = std::max(radMax, rad[t]);
+    double dcMax = 0;
     for (int e = 0; e < Ne; ++e) {
         const int tp = c->plusTri[e], tm = c->minusTri[e];
         if (tp > 0 && tm > 0) {
@@ -2626,6 +2632,29 @@ efie_status efie_diag_near_pairs(efie_ctx* c, const efie_opts* opts, long long* 
     const long long k = std::min(cap, c->nNear);
     if (k > 0) {
         HIPCHK(hipMemcpyAsync(pairs, c->dList.p, sizeof(int2) * (size_t)k, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    return ok();
+}
+
+efie_status efie_diag_quad_points(efie_ctx* c, const efie_opts* opts, int set, long long* n, double* out,
+                                  long long cap) {
+    if (!c || !n || (cap > 0 && !out) || set < 0 || set > 4)
+        return fail(EFIE_E_ARGS, "efie:args", "bad arguments");
+    efie_opts o;
+    if (opts) o = *opts;
+    else efie_opts_default(&o);
+    o.model = EFIE_MODEL_QUADRATURE;
+    efie_status st = prepare(c, &o, 0.20);
+    if (st) return st;
+    const long long Nt = c->Nt;
+    const DBuf* src[5] = {&c->dReg, &c->dNearPts, &c->dSelfPts, &c->dASelf, &c->dRad};
+    const long long len[5] = {4 * Nt * c->Nq, 4 * Nt * c->NpNear, 4 * Nt * c->NsSelf * c->Nq2s,
+                              Nt * c->NsSelf, Nt};
+    *n = len[set];
+    const long long k = std::min(cap, len[set]);
+    if (k > 0) {
+        HIPCHK(hipMemcpyAsync(out, src[set]->p, sizeof(double) * (size_t)k, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
     }
     return ok();

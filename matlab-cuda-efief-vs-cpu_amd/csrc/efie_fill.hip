@@ -121,6 +121,123 @@ __device__ bool near_test(const TriGeo a, const TriGeo b, double eps) {
     return d < eps * h;
 }
 
+// ---------------------------------------------------- quadrature points --
+
+// Quadrature points of every triangle, relative to its centroid, built on
+// the device from the vertices (efie_precompute.m's tri_quad / subdivided
+// rules): the regular rule on the triangle, and the subdivided rule on each
+// of the 4^depth midpoint children for near and self pairs.  The operations
+// are the host generator's (map_ref_to_vertices.m, subdivide_triangles.m)
+// in the same order with contraction off, so the points are bit-identical
+// to it.  Child p of the subdivision is found digit by digit from the top
+// level (base 4, most significant first): the children of (a, b, c) are
+// (a, ab, ca), (ab, b, bc), (ca, bc, c), (ab, bc, ca).
+struct QV {
+    double x, y, z;
+};
+
+__device__ __forceinline__ QV qv_mid(QV a, QV b) {
+#pragma clang fp contract(off)
+    return {0.5 * (a.x + b.x), 0.5 * (a.y + b.y), 0.5 * (a.z + b.z)};
+}
+
+__device__ __forceinline__ QV qv_map(QV A, QV B, QV C, double u, double v) {
+#pragma clang fp contract(off)
+    return {A.x + u * (B.x - A.x) + v * (C.x - A.x), A.y + u * (B.y - A.y) + v * (C.y - A.y),
+            A.z + u * (B.z - A.z) + v * (C.z - A.z)};
+}
+
+__device__ __forceinline__ double qv_area2(QV A, QV B, QV C) {
+#pragma clang fp contract(off)
+    const double bx = B.x - A.x, by = B.y - A.y, bz = B.z - A.z;
+    const double cx = C.x - A.x, cy = C.y - A.y, cz = C.z - A.z;
+    const double x = by * cz - bz * cy, y = bz * cx - bx * cz, z = bx * cy - by * cx;
+    return sqrt(x * x + y * y + z * z);
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    return v;
+}
+
+// one thread per (triangle, unit): unit 0 the regular rule and the
+// vertices, units 1..NsN the near children, then the NsS self children.
+// rad[t] (max distance of any point or vertex from the centroid) and
+// mx = {near radius, self radius, max self softening, max rad} are maxima
+// of non-negative doubles, kept as their bit patterns.
+__global__ __launch_bounds__(256) void k_quad_points(const double* __restrict__ tv, const TriGeo* __restrict__ tg,
+                                                     int Nt, QuadRules R, int depthN, int depthS, double selfAlpha,
+                                                     Pt* __restrict__ regPts, Pt* __restrict__ nearPts,
+                                                     Pt* __restrict__ selfPts, double* __restrict__ aSelf,
+                                                     unsigned long long* __restrict__ rad,
+                                                     unsigned long long* __restrict__ mx) {
+#pragma clang fp contract(off)
+    const int NsN = 1 << (2 * depthN), NsS = 1 << (2 * depthS);
+    const int units = 1 + NsN + NsS;
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = id < (long long)Nt * units;
+    const int t = live ? (int)(id / units) : 0, u = live ? (int)(id % units) : 0;
+    double r = 0.0, rN = 0.0, rS = 0.0, aS = 0.0;
+    if (live) {
+        const QV A{tv[9 * (size_t)t], tv[9 * (size_t)t + 1], tv[9 * (size_t)t + 2]};
+        const QV B{tv[9 * (size_t)t + 3], tv[9 * (size_t)t + 4], tv[9 * (size_t)t + 5]};
+        const QV C{tv[9 * (size_t)t + 6], tv[9 * (size_t)t + 7], tv[9 * (size_t)t + 8]};
+        const TriGeo g = tg[t];
+        auto dist = [&](QV q) {
+            const double x = q.x - g.cx, y = q.y - g.cy, z = q.z - g.cz;
+            return sqrt(x * x + y * y + z * z);
+        };
+        if (u == 0) {
+            r = fmax(dist(A), fmax(dist(B), dist(C)));
+            for (int i = 0; i < R.nReg; ++i) {
+                const QV q = qv_map(A, B, C, R.ru[i], R.rv[i]);
+                const Pt p = {q.x - g.cx, q.y - g.cy, q.z - g.cz, R.rw[i] * g.area2};
+                regPts[(size_t)t * R.nReg + i] = p;
+                r = fmax(r, sqrt(p.x * p.x + p.y * p.y + p.z * p.z));
+            }
+        } else {
+            const bool self = u > NsN;
+            const int depth = self ? depthS : depthN, Ns = self ? NsS : NsN;
+            const int child = self ? u - 1 - NsN : u - 1;
+            QV a = A, b = B, c = C;
+            for (int d = depth - 1; d >= 0; --d) {
+                const int k = (child >> (2 * d)) & 3;
+                const QV ab = qv_mid(a, b), bc = qv_mid(b, c), ca = qv_mid(c, a);
+                if (k == 0) { b = ab; c = ca; }
+                else if (k == 1) { a = ab; c = bc; }
+                else if (k == 2) { a = ca; b = bc; }
+                else { a = ab; b = bc; c = ca; }
+            }
+            const double a2p = qv_area2(a, b, c);
+            if (self) {
+                aS = selfAlpha * sqrt((0.5 * a2p) / kPi);
+                aSelf[(size_t)t * Ns + child] = aS;
+            }
+            Pt* out = (self ? selfPts : nearPts) + ((size_t)t * Ns + child) * R.nSub;
+            for (int i = 0; i < R.nSub; ++i) {
+                const QV q = qv_map(a, b, c, R.su[i], R.sv[i]);
+                const Pt p = {q.x - g.cx, q.y - g.cy, q.z - g.cz, R.sw[i] * a2p};
+                out[i] = p;
+                const double d = sqrt(p.x * p.x + p.y * p.y + p.z * p.z);
+                r = fmax(r, d);
+                if (self) rS = fmax(rS, d);
+                else rN = fmax(rN, d);
+            }
+        }
+        atomicMax(rad + t, (unsigned long long)__double_as_longlong(r));
+    }
+    rN = wave_max(rN);
+    rS = wave_max(rS);
+    aS = wave_max(aS);
+    r = wave_max(r);
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(mx + 0, (unsigned long long)__double_as_longlong(rN));
+        atomicMax(mx + 1, (unsigned long long)__double_as_longlong(rS));
+        atomicMax(mx + 2, (unsigned long long)__double_as_longlong(aS));
+        atomicMax(mx + 3, (unsigned long long)__double_as_longlong(r));
+    }
+}
+
 // ------------------------------------------------------------ near map --
 
 // Near pairs by cell binning: triangle centroids are binned into cubic
@@ -1711,6 +1828,22 @@ __global__ __launch_bounds__(1024) void k_epilogue(const double* __restrict__ X,
 }
 
 // ------------------------------------------------------------ launchers --
+
+hipError_t launch_quad_points(const double* tv, const TriGeo* tg, int Nt, const QuadRules& R, int depthN,
+                              int depthS, double selfAlpha, Pt* regPts, Pt* nearPts, Pt* selfPts, double* aSelf,
+                              double* rad, double* mx, hipStream_t s) {
+    if (Nt < 1 || depthN < 0 || depthS < 0 || depthN > 4 || depthS > 4 || R.nReg < 1 || R.nReg > 16 ||
+        R.nSub < 1 || R.nSub > 16)
+        return hipErrorInvalidValue;
+    const long long n = (long long)Nt * (1 + (1 << (2 * depthN)) + (1 << (2 * depthS)));
+    hipError_t e = launch_fill_dbl(rad, 0.0, Nt, s);
+    if (e == hipSuccess) e = launch_fill_dbl(mx, 0.0, 4, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_quad_points, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, tv, tg, Nt, R, depthN,
+                       depthS, selfAlpha, regPts, nearPts, selfPts, aSelf,
+                       reinterpret_cast<unsigned long long*>(rad), reinterpret_cast<unsigned long long*>(mx));
+    return hipGetLastError();
+}
 
 hipError_t launch_near_bins(const TriGeo* tg, int Nt, double eps, double x0, double y0, double z0, double h, int nx,
                             int ny, int nz, const int* cellOf, const int* cellStart, const int* cellTri,

@@ -59,6 +59,21 @@ struct Freq {
 // cell of triangle t, cellTri the triangles cell by cell from
 // cellStart[cell] (ncell + 1 offsets).  list == nullptr: rowCount[t1] =
 // near t2 > t1; otherwise those pairs, sorted by t2, to list[rowStart[t1]..].
+// Quadrature rules of the rule-derived geometry (reference coordinates
+// u, v and weights w): the regular rule and the rule on subdivided
+// triangles.
+struct QuadRules {
+    double ru[16], rv[16], rw[16], su[16], sv[16], sw[16];
+    int nReg, nSub;
+};
+// Points of every triangle relative to its centroid from the absolute
+// vertices tv (9 per triangle): regular [Nt][nReg], near children
+// [Nt][4^depthN][nSub], self children [Nt][4^depthS][nSub], the self
+// softening radii [Nt][4^depthS], rad[t] and mx[4] = {near radius, self
+// radius, max softening radius, max rad}.
+hipError_t launch_quad_points(const double* tv, const TriGeo* tg, int Nt, const QuadRules& R, int depthN,
+                              int depthS, double selfAlpha, Pt* regPts, Pt* nearPts, Pt* selfPts, double* aSelf,
+                              double* rad, double* mx, hipStream_t s);
 hipError_t launch_near_bins(const TriGeo* tg, int Nt, double eps, double x0, double y0, double z0, double h, int nx,
                             int ny, int nz, const int* cellOf, const int* cellStart, const int* cellTri,
                             int* rowCount, const int* rowStart, int2* list, hipStream_t s);

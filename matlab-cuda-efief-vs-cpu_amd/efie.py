@@ -106,7 +106,7 @@ EXPORTS = [
     "efie_opts_default", "efie_opts_preset", "efie_last_error", "efie_last_error_id",
     "efie_mesh_counts", "efie_make_mesh", "efie_rwg_count", "efie_build_rwg", "efie_pick_port",
     "efie_create", "efie_destroy", "efie_set_mesh", "efie_set_rwg", "efie_assemble_zv",
-    "efie_sweep", "efie_zgesv", "efie_zgesv_noalloc", "efie_diag_copy", "efie_diag_cache_counts", "efie_diag_near_pairs", "efie_set_prepack", "efie_set_rwg_tables", "efie_assemble_z",
+    "efie_sweep", "efie_zgesv", "efie_zgesv_noalloc", "efie_diag_copy", "efie_diag_cache_counts", "efie_diag_near_pairs", "efie_diag_quad_points", "efie_set_prepack", "efie_set_rwg_tables", "efie_assemble_z",
     "efie_sur_cfg_default", "efie_sur_blob_floats", "efie_sur_train", "efie_sur_get", "efie_sur_set",
     "efie_sur_set_exact", "efie_mlp_params", "efie_mlp_forward", "efie_mlp_loss_grad",
 ]
@@ -148,6 +148,8 @@ def lib():
         L.efie_diag_copy.argtypes = [C.c_void_p, C.c_size_t, C.c_int, _dp]
         L.efie_diag_cache_counts.argtypes = [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
         L.efie_diag_near_pairs.argtypes = [C.c_void_p, C.POINTER(Opts), C.POINTER(C.c_longlong), _ip, C.c_longlong]
+        L.efie_diag_quad_points.argtypes = [C.c_void_p, C.POINTER(Opts), C.c_int, C.POINTER(C.c_longlong), _dp,
+                                            C.c_longlong]
         _fp = C.POINTER(C.c_float)
         L.efie_sur_cfg_default.argtypes = [C.POINTER(SurCfg)]
         L.efie_sur_cfg_default.restype = None
@@ -287,6 +289,7 @@ class Context:
         V = np.asfortranarray(V, dtype=np.float64)
         F = np.asfortranarray(F, dtype=np.int32)
         _check(lib().efie_set_mesh(self._c, _d(V), V.shape[0], _i(F), F.shape[0]))
+        self.Nt = F.shape[0]
         return self
 
     def set_rwg(self, edge, plusTri, minusTri, plusSign, minusSign, rp, rm, Ap, Am, len_):
@@ -327,6 +330,7 @@ class Context:
         Ne = a[0].shape[0]
         _check(lib().efie_set_rwg_tables(self._c, int(Nt), *[_i(x) for x in a], *[_d(x) for x in b], Ne))
         self.Ne = Ne
+        self.Nt = int(Nt)
         return self
 
     def assemble_z(self, k, omega, mu, eps, o: Optional[Opts] = None):
@@ -451,6 +455,20 @@ class Context:
         out = np.zeros((max(1, n.value), 2), dtype=np.int32)
         _check(lib().efie_diag_near_pairs(self._c, C.byref(o), C.byref(n), out.ctypes.data_as(_ip), n.value))
         return out[: n.value]
+
+    def quad_points(self, which, o: Optional[Opts] = None):
+        """The fill's quadrature points on the device (efie_diag.h):
+        which = "regular" [Nt, Nq, 4], "near" / "self" [Nt, Ns, Nq2, 4] as
+        (x, y, z, w) relative to the centroid (flattened per triangle),
+        "self_alpha" [Nt, Ns], "radius" [Nt]."""
+        sets = {"regular": 0, "near": 1, "self": 2, "self_alpha": 3, "radius": 4}
+        o = o or opts()
+        n = C.c_longlong()
+        _check(lib().efie_diag_quad_points(self._c, C.byref(o), sets[which], C.byref(n), None, 0))
+        out = np.zeros(max(1, n.value))
+        _check(lib().efie_diag_quad_points(self._c, C.byref(o), sets[which], C.byref(n), _d(out), n.value))
+        out = out[: n.value]
+        return out.reshape(self.Nt, -1) if which != "radius" else out
 
     def diag_copy(self, nbytes, reps=10):
         """Average time (ms) of a plain device copy of nbytes (efie_diag.h)."""
