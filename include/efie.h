@@ -188,6 +188,24 @@ typedef struct efie_ctx efie_ctx;
 efie_status efie_create(efie_ctx** ctx, int device);
 void efie_destroy(efie_ctx* ctx);
 
+/* ---- the same preprocessing on the device, from a context's mesh -------- */
+
+/* build_rwg_interior.m:1-113 on the device from the mesh given to
+ * efie_set_mesh: *Ne = the number of interior edges. */
+efie_status efie_rwg_count_device(efie_ctx* ctx, int* Ne);
+/* build_rwg_interior.m:1-113 on the device: the tables of efie_build_rwg
+ * (bit-identical to it), copied to the host arrays and installed as the
+ * context's RWG tables, as efie_set_rwg would. */
+efie_status efie_build_rwg_device(efie_ctx* ctx, int Ne, int* edge, int* plusTri,
+                                  int* minusTri, int* plusSign, int* minusSign,
+                                  double* rp, double* rm, double* Ap, double* Am,
+                                  double* len, double* center, int* zSign);
+/* pick_port_edge_by_point_and_layer.m:1-11 on the device, over the edge
+ * centres and layer signs of the last efie_build_rwg_device; *edge is
+ * 1-based (the lowest index among equal distances, as efie_pick_port). */
+efie_status efie_pick_port_device(efie_ctx* ctx, const double p[3], int layerSign,
+                                  int* edge);
+
 /* Mesh and RWG tables: arguments 1-12 of both reference MEX functions
  * (patch_sweep_cuda.cu:4-16, gassemble_ZV_EFIE_RWG_DIFF_GPU_FAST.cu:127-139).
  * Copied to HBM once; frequency-independent geometry (quadrature points,

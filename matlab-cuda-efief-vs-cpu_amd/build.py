@@ -23,7 +23,7 @@ LIB = os.path.join(OUT, "libefie_hip.so")
 ARCH = os.environ.get("EFIE_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["efie_api.cpp", "efie_surrogate.cpp", "efie_fill.hip", "efie_lu.hip", "efie_mixed.hip", "efie_mlp.hip",
-           "efie_noalloc.hip"]
+           "efie_noalloc.hip", "efie_rwg.hip"]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 COMMON = [

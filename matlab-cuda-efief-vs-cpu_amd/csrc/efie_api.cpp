@@ -254,6 +254,12 @@ struct efie_ctx {
     bool momSelfOnly = false;  // only the self-pair moments were computed
     DBuf dTriV;  // triangle vertices relative to the centroid [Nt][9]
     DBuf dTriAbs;  // triangle vertices [Nt][9], then 4 radius maxima of the point generator
+    // RWG tables built on the device: mesh copies, count-pass work, tables,
+    // port-pick partials; rwgDevNe >= 0 once counted for the current mesh,
+    // portValid once the tables (centres, layer signs) are built
+    DBuf dMeshV, dMeshF, dRwgWork, dRwgTab, dPortPart;
+    int rwgDevNe = -1;
+    bool portValid = false;
     // frequency-independent series of every triangle pair (near_mode
     // MOMENTS), valid for P = serP and near_alpha = serAlpha
     DBuf dQall, dSerC;  // pair series (transient) and per-entry coefficients
@@ -589,6 +595,8 @@ efie_status efie_set_mesh(efie_ctx* c, const double* V, int Nv, const int* F, in
     c->Nt = Nt;
     c->V.assign(V, V + 3 * (size_t)Nv);
     c->F.assign(F, F + 3 * (size_t)Nt);
+    c->rwgDevNe = -1;
+    c->portValid = false;
     c->haveMesh = true;
     c->haveRwg = false;
     c->midValid = false;
@@ -641,6 +649,139 @@ efie_status efie_set_rwg(efie_ctx* c, const int* edge, const int* plusTri, const
     c->geoValid = false;
     c->cAlpha = -1;
     drop_pack(c);
+    return ok();
+}
+
+// ------------------------------------------- RWG tables on the device --
+// build_rwg_interior.m:1-113 and pick_port_edge_by_point_and_layer.m:1-11
+// from the context's mesh (efie_rwg.hip).
+
+namespace {
+
+// Work layout of the count pass in dRwgWork (ints, then the int2 slots).
+struct RwgWork {
+    int *cnt, *fill, *nInt, *start, *eStart;
+    int2* slot;
+};
+
+RwgWork rwg_work(efie_ctx* c) {
+    const size_t Nv = c->Nv;
+    int* w = c->dRwgWork.as<int>();
+    RwgWork r;
+    r.cnt = w;
+    r.fill = w + Nv;
+    r.nInt = w + 2 * Nv;
+    r.start = w + 3 * Nv;
+    r.eStart = w + 4 * Nv + 1;
+    const size_t ints = 5 * Nv + 2;
+    r.slot = reinterpret_cast<int2*>(w + ((ints + 1) & ~(size_t)1));
+    return r;
+}
+
+efie_status rwg_count_device(efie_ctx* c) {
+    if (c->rwgDevNe >= 0) return ok();
+    const size_t Nv = c->Nv, Nt = c->Nt;
+    hipStream_t s = c->stream;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(c->dMeshV.ensure(sizeof(double) * 3 * Nv));
+    HIPCHK(c->dMeshF.ensure(sizeof(int) * 3 * Nt));
+    const size_t ints = ((5 * Nv + 2) + 1) & ~(size_t)1;
+    HIPCHK(c->dRwgWork.ensure(sizeof(int) * ints + sizeof(int2) * 3 * Nt));
+    HIPCHK(hipMemcpyAsync(c->dMeshV.p, c->V.data(), sizeof(double) * 3 * Nv, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->dMeshF.p, c->F.data(), sizeof(int) * 3 * Nt, hipMemcpyHostToDevice, s));
+    const RwgWork w = rwg_work(c);
+    HIPCHK(launch_rwg_count(c->dMeshF.as<int>(), c->Nt, c->Nv, w.cnt, w.slot, w.fill, w.nInt, w.start, w.eStart, s));
+    int ne = 0;
+    HIPCHK(hipMemcpyAsync(&ne, w.eStart + Nv, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    c->rwgDevNe = ne;
+    return ok();
+}
+
+}  // namespace
+
+efie_status efie_rwg_count_device(efie_ctx* c, int* Ne) {
+    if (!c || !Ne) return fail(EFIE_E_ARGS, "efie:args", "null argument");
+    if (!c->haveMesh || c->meshless) return fail(EFIE_E_STATE, "efie:state", "efie_set_mesh must come first");
+    efie_status st = rwg_count_device(c);
+    if (st) return st;
+    *Ne = c->rwgDevNe;
+    return ok();
+}
+
+efie_status efie_build_rwg_device(efie_ctx* c, int Ne, int* edge, int* plusTri, int* minusTri, int* plusSign,
+                                  int* minusSign, double* rp, double* rm, double* Ap, double* Am, double* len,
+                                  double* center, int* zSign) {
+    if (!c) return fail(EFIE_E_ARGS, "efie:args", "null context");
+    if (!edge || !plusTri || !minusTri || !plusSign || !minusSign || !rp || !rm || !Ap || !Am || !len || !center ||
+        !zSign)
+        return fail(EFIE_E_ARGS, "efie:args", "null output");
+    if (!c->haveMesh || c->meshless) return fail(EFIE_E_STATE, "efie:state", "efie_set_mesh must come first");
+    efie_status st = rwg_count_device(c);
+    if (st) return st;
+    if (Ne != c->rwgDevNe)
+        return fail(EFIE_E_ARGS, "efie:rwg", "Ne=%d but the mesh has %d interior edges", Ne, c->rwgDevNe);
+    if (Ne < 1) return fail(EFIE_E_MESH, "efie:rwg", "the mesh has no interior edge");
+    const size_t n = Ne;
+    hipStream_t s = c->stream;
+    // tables: ints (edge, tri, sgn: 2 Ne each; zSign, err), then doubles
+    // (rpm 6 Ne, Apm 2 Ne, len Ne, center 3 Ne)
+    const size_t nInts = ((7 * n + 1) + 1) & ~(size_t)1;
+    HIPCHK(c->dRwgTab.ensure(sizeof(int) * nInts + sizeof(double) * 12 * n));
+    int* ti = c->dRwgTab.as<int>();
+    int *dEdge = ti, *dTri = ti + 2 * n, *dSgn = ti + 4 * n, *dZ = ti + 6 * n, *dErr = ti + 7 * n;
+    double* td = reinterpret_cast<double*>(ti + nInts);
+    double *dRpm = td, *dApm = td + 6 * n, *dLen = td + 8 * n, *dCen = td + 9 * n;
+    double zmax = -INFINITY, zmin = INFINITY;
+    for (int v = 0; v < c->Nv; ++v) {
+        zmax = std::max(zmax, c->V[v + 2 * (size_t)c->Nv]);
+        zmin = std::min(zmin, c->V[v + 2 * (size_t)c->Nv]);
+    }
+    const RwgWork w = rwg_work(c);
+    HIPCHK(launch_rwg_tables(c->dMeshV.as<double>(), c->Nv, c->dMeshF.as<int>(), c->Nt, w.start, w.slot, w.eStart, Ne,
+                             0.5 * (zmax + zmin), dEdge, dTri, dSgn, dRpm, dApm, dLen, dCen, dZ, dErr, s));
+    int err = 0;
+    HIPCHK(hipMemcpyAsync(&err, dErr, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(edge, dEdge, sizeof(int) * 2 * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(plusTri, dTri, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(minusTri, dTri + n, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(plusSign, dSgn, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(minusSign, dSgn + n, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(rp, dRpm, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(rm, dRpm + 3 * n, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(Ap, dApm, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(Am, dApm + n, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(len, dLen, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(center, dCen, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(zSign, dZ, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (err != 0x7fffffff) return fail(EFIE_E_MESH, "efie:rwg", "opposite vertex not found for edge %d", err);
+    // the context's RWG tables from here on (unchanged tables keep every cache)
+    st = efie_set_rwg(c, edge, plusTri, minusTri, plusSign, minusSign, rp, rm, Ap, Am, len, Ne);
+    if (st) return st;
+    c->portValid = true;
+    return ok();
+}
+
+efie_status efie_pick_port_device(efie_ctx* c, const double p[3], int layerSign, int* edgeOut) {
+    if (!c || !p || !edgeOut) return fail(EFIE_E_ARGS, "efie:port", "bad arguments");
+    if (!c->portValid || c->rwgDevNe < 1)
+        return fail(EFIE_E_STATE, "efie:state", "efie_build_rwg_device must come first");
+    const int Ne = c->rwgDevNe;
+    const size_t n = Ne, nInts = ((7 * n + 1) + 1) & ~(size_t)1;
+    int* ti = c->dRwgTab.as<int>();
+    const double* dCen = reinterpret_cast<double*>(ti + nInts) + 9 * n;
+    const int nb = port_pick_blocks(Ne);
+    HIPCHK(c->dPortPart.ensure(sizeof(double) * nb + sizeof(int) * (nb + 1)));
+    double* pd = c->dPortPart.as<double>();
+    int* pi = reinterpret_cast<int*>(pd + nb);
+    hipStream_t s = c->stream;
+    HIPCHK(launch_port_pick(dCen, ti + 6 * n, Ne, p, layerSign, pd, pi, pi + nb, s));
+    int e = 0;
+    HIPCHK(hipMemcpyAsync(&e, pi + nb, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (e < 1) return fail(EFIE_E_PORT, "efie:port", "no edge on layer %d", layerSign);
+    *edgeOut = e;
     return ok();
 }
 

@@ -59,6 +59,24 @@ struct Freq {
 // cell of triangle t, cellTri the triangles cell by cell from
 // cellStart[cell] (ncell + 1 offsets).  list == nullptr: rowCount[t1] =
 // near t2 > t1; otherwise those pairs, sorted by t2, to list[rowStart[t1]..].
+// RWG tables on the device (efie_rwg.hip).  Count pass: cnt/fill/nInt
+// [Nv], start/eStart [Nv + 1] (exclusive scans; eStart[Nv] = Ne), slot
+// [3 Nt].  Tables pass: column-major outputs with leading dimension Ne
+// (edge, tri = plus|minus, sgn = plus|minus, rpm = rp|rm, Apm = Ap|Am,
+// len, center, zSign); *err = lowest 1-based edge without a unique
+// opposite vertex, INT_MAX when there is none.
+hipError_t launch_rwg_count(const int* F, int Nt, int Nv, int* cnt, int2* slot, int* fill, int* nInt, int* start,
+                            int* eStart, hipStream_t s);
+hipError_t launch_rwg_tables(const double* V, int Nv, const int* F, int Nt, const int* start, const int2* slot,
+                             const int* eStart, int Ne, double zmid, int* edge, int* tri, int* sgn, double* rpm,
+                             double* Apm, double* len, double* center, int* zSign, int* err, hipStream_t s);
+// Port pick over the edges of one layer: *out = 1-based nearest edge (lowest
+// index among equal distances), 0 when the layer has none; pd/pi hold
+// port_pick_blocks(Ne) partials.
+int port_pick_blocks(int Ne);
+hipError_t launch_port_pick(const double* center, const int* zSign, int Ne, const double p[3], int layer, double* pd,
+                            int* pi, int* out, hipStream_t s);
+
 // Quadrature rules of the rule-derived geometry (reference coordinates
 // u, v and weights w): the regular rule and the rule on subdivided
 // triangles.

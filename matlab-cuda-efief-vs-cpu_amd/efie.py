@@ -106,7 +106,8 @@ EXPORTS = [
     "efie_opts_default", "efie_opts_preset", "efie_last_error", "efie_last_error_id",
     "efie_mesh_counts", "efie_make_mesh", "efie_rwg_count", "efie_build_rwg", "efie_pick_port",
     "efie_create", "efie_destroy", "efie_set_mesh", "efie_set_rwg", "efie_assemble_zv",
-    "efie_sweep", "efie_zgesv", "efie_zgesv_noalloc", "efie_diag_copy", "efie_diag_cache_counts", "efie_diag_near_pairs", "efie_diag_quad_points", "efie_set_prepack", "efie_set_rwg_tables", "efie_assemble_z",
+    "efie_sweep", "efie_zgesv", "efie_zgesv_noalloc", "efie_diag_copy", "efie_diag_cache_counts", "efie_diag_near_pairs", "efie_diag_quad_points", "efie_rwg_count_device", "efie_build_rwg_device",
+    "efie_pick_port_device", "efie_set_prepack", "efie_set_rwg_tables", "efie_assemble_z",
     "efie_sur_cfg_default", "efie_sur_blob_floats", "efie_sur_train", "efie_sur_get", "efie_sur_set",
     "efie_sur_set_exact", "efie_mlp_params", "efie_mlp_forward", "efie_mlp_loss_grad",
 ]
@@ -132,6 +133,9 @@ def lib():
         L.efie_rwg_count.argtypes = [_ip, C.c_int, C.c_int, _ip]
         L.efie_build_rwg.argtypes = [_dp, C.c_int, _ip, C.c_int, C.c_int] + [_ip] * 5 + [_dp] * 6 + [_ip]
         L.efie_pick_port.argtypes = [C.c_int, _dp, _ip, _dp, C.c_int, _ip]
+        L.efie_rwg_count_device.argtypes = [C.c_void_p, _ip]
+        L.efie_build_rwg_device.argtypes = [C.c_void_p, C.c_int] + [_ip] * 5 + [_dp] * 6 + [_ip]
+        L.efie_pick_port_device.argtypes = [C.c_void_p, _dp, C.c_int, _ip]
         L.efie_create.argtypes = [C.POINTER(C.c_void_p), C.c_int]
         L.efie_destroy.argtypes = [C.c_void_p]
         L.efie_destroy.restype = None
@@ -222,6 +226,14 @@ class Rwg:
         return int(self.plusTri.shape[0])
 
 
+def _empty_rwg(Ne) -> Rwg:
+    return Rwg(edge=np.zeros((Ne, 2), np.int32, order="F"), plusTri=np.zeros(Ne, np.int32),
+               minusTri=np.zeros(Ne, np.int32), plusSign=np.zeros(Ne, np.int32),
+               minusSign=np.zeros(Ne, np.int32), rp=np.zeros((Ne, 3), order="F"),
+               rm=np.zeros((Ne, 3), order="F"), Ap=np.zeros(Ne), Am=np.zeros(Ne), len=np.zeros(Ne),
+               center=np.zeros((Ne, 3), order="F"), zSign=np.zeros(Ne, np.int32))
+
+
 def build_rwg_interior(V, F) -> Rwg:
     V = np.asfortranarray(V, dtype=np.float64)
     F = np.asfortranarray(F, dtype=np.int32)
@@ -299,6 +311,26 @@ class Context:
         _check(lib().efie_set_rwg(self._c, *[_i(x) for x in a], *[_d(x) for x in b], Ne))
         self.Ne = Ne
         return self
+
+    def build_rwg(self) -> Rwg:
+        """build_rwg_interior.m on the device from the mesh of set_mesh; the
+        tables become the context's RWG tables (efie_build_rwg_device)."""
+        ne = C.c_int()
+        _check(lib().efie_rwg_count_device(self._c, C.byref(ne)))
+        r = _empty_rwg(ne.value)
+        _check(lib().efie_build_rwg_device(self._c, ne.value, _i(r.edge), _i(r.plusTri), _i(r.minusTri),
+                                           _i(r.plusSign), _i(r.minusSign), _d(r.rp), _d(r.rm), _d(r.Ap),
+                                           _d(r.Am), _d(r.len), _d(r.center), _i(r.zSign)))
+        self.Ne = ne.value
+        return r
+
+    def pick_port(self, p, layerSign) -> int:
+        """pick_port_edge_by_point_and_layer.m on the device over the tables
+        of the last build_rwg (efie_pick_port_device)."""
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        e = C.c_int()
+        _check(lib().efie_pick_port_device(self._c, _d(p), int(layerSign), C.byref(e)))
+        return e.value
 
     def set_problem(self, V, F, rwg: Rwg):
         self.set_mesh(V, F)

@@ -72,3 +72,77 @@ def test_device_quad_points_bit_exact(efie_mod, mirror, ctx, rs):
         want = exp[which].reshape(Nt, -1) if which != "radius" else exp[which]
         assert got.shape == want.shape, which
         assert np.array_equal(got, want), (which, np.max(np.abs(got - want)))
+
+
+RWG_FIELDS = ("edge", "plusTri", "minusTri", "plusSign", "minusSign", "rp", "rm", "Ap", "Am", "len", "center",
+              "zSign")
+
+
+def _mesh(efie_mod, name):
+    if name.startswith("c4_"):
+        g = W.c4_geometries()[int(name[3:])]
+        return efie_mod.make_patch_with_ground_mesh(*g["geom"], *g["cells"])
+    if name.startswith("cells"):
+        return efie_mod.make_patch_with_ground_mesh(*W.GEOM, *[int(x) for x in name[5:].split("_")])
+    return efie_mod.make_patch_with_ground_mesh(*W.mesh_args(name))
+
+
+@pytest.mark.parametrize("name", ["micro", "tiny", "c1", "c2_6k", "c4_19", "c4_11", "cells6_4_5_5", "cells3_5_4_2",
+                                  "cells1_1_1_1"])
+def test_device_rwg_tables_equal_host(efie_mod, ctx, name):
+    """build_rwg_interior.m on the device (efie_build_rwg_device) against
+    the host builder, every table bit for bit, on the workload meshes and on
+    meshes with unequal cell counts (whose faces overlap); then the port
+    pick against efie_pick_port on both layers."""
+    V, F = _mesh(efie_mod, name)
+    r = efie_mod.build_rwg_interior(V, F)
+    ctx.set_mesh(V, F)
+    rd = ctx.build_rwg()
+    for f in RWG_FIELDS:
+        assert np.array_equal(getattr(rd, f), getattr(r, f)), f
+    V = np.asarray(V)
+    rng = np.random.default_rng(len(F))
+    pts = [V.min(axis=0), V.max(axis=0), V.mean(axis=0)] + list(rng.uniform(V.min(axis=0), V.max(axis=0), (8, 3)))
+    pts += [r.center[0], r.center[r.Ne // 2]]  # distance zero to an edge centre
+    for p in pts:
+        for layer in (1, -1):
+            assert ctx.pick_port(p, layer) == efie_mod.pick_port_edge_by_point_and_layer(r, p, layer)
+    with pytest.raises(efie_mod.EfieError) as e:
+        ctx.pick_port(pts[0], 0)
+    assert e.value.ident == "efie:port"
+
+
+def test_device_rwg_installs_tables(efie_mod, ctx):
+    """The device-built tables drive the sweep exactly as the host-built ones
+    given through set_problem."""
+    V, F = _mesh(efie_mod, "c1")
+    ctx.set_mesh(V, F)
+    rd = ctx.build_rwg()
+    pP, pG = W.feed_points("c1")
+    pe, ge = ctx.pick_port(pP, 1), ctx.pick_port(pG, -1)
+    fr = W.freqs("c1")[::4]
+    o = efie_mod.opts(**W.OPTS_C1, precision=0, near_mode=efie_mod.NEAR_MOMENTS)
+    sp = 1.5 * rd.len.mean()
+    z1 = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, sp, 50.0, o)[0]
+    r = efie_mod.build_rwg_interior(V, F)
+    assert (pe, ge) == (efie_mod.pick_port_edge_by_point_and_layer(r, pP, 1),
+                        efie_mod.pick_port_edge_by_point_and_layer(r, pG, -1))
+    ctx.set_problem(V, F, r)
+    z2 = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, sp, 50.0, o)[0]
+    assert np.array_equal(z1, z2)
+
+
+def test_device_rwg_errors_as_host(efie_mod, ctx):
+    """A triangle (1, 2, 1) makes edge (1, 2) interior with no opposite
+    vertex: both builders refuse the mesh with the same error."""
+    V = np.array([[0.0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0]])
+    F = np.array([[1, 2, 1], [2, 4, 3]], dtype=np.int32)
+    with pytest.raises(efie_mod.EfieError) as eh:
+        efie_mod.build_rwg_interior(V, F)
+    ctx.set_mesh(V, F)
+    with pytest.raises(efie_mod.EfieError) as ed:
+        ctx.build_rwg()
+    assert ed.value.ident == eh.value.ident and str(ed.value) == str(eh.value)
+    with pytest.raises(efie_mod.EfieError) as ep:
+        ctx.pick_port([0.0, 0.0, 0.0], 1)
+    assert ep.value.ident == "efie:state"
