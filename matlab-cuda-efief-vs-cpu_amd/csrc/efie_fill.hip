@@ -1469,6 +1469,17 @@ __global__ __launch_bounds__(256) void k_series_self(const Pt* __restrict__ self
 // stores them, with R0e, as
 // C[(tile * (NT + 1) + j) * 256 + thread] (coefficient-major per tile, so
 // the fill reads them with coalesced loads).
+//
+// The tile's entries draw on few triangle pairs: the distinct triangles of
+// its 16 row and 16 column RWGs (at most 32 each, usually ~20) form every
+// pair any entry needs.  Those pairs are staged through LDS one order at a
+// time (8 lanes per pair record, coalesced; the next order in flight in
+// registers), so each record is fetched once per tile instead of once per
+// entry that uses it, and every entry shifts each order's contribution to
+// the common centre as it arrives (D[q + k] += c d^k / k!).  A tile with
+// more than kCoefSlots pairs reads the records directly.
+constexpr int kCoefSlots = 512;
+
 template <int NT>
 __global__ __launch_bounds__(256) void k_series_coeffs(const double* __restrict__ Q, const TriGeo* __restrict__ tg,
                                                        const int* __restrict__ sideTri,
@@ -1476,6 +1487,9 @@ __global__ __launch_bounds__(256) void k_series_coeffs(const double* __restrict_
                                                        double* __restrict__ C) {
     constexpr int P = NT - 3;
     constexpr size_t stride = (size_t)(P + 1) * 8;
+    __shared__ double qs[kCoefSlots][8];
+    __shared__ size_t slotRec[kCoefSlots];
+    __shared__ int rl[32], cl[32], cnt[2];
     const long long id = blockIdx.x;
     int J = (int)((sqrt(8.0 * (double)id + 1.0) - 1.0) * 0.5);
     while ((long long)(J + 1) * (J + 2) / 2 <= id) ++J;
@@ -1513,49 +1527,144 @@ __global__ __launch_bounds__(256) void k_series_coeffs(const double* __restrict_
         }
     const double R0e = hi >= lo ? 0.5 * (lo + hi) : 0.0;
 
+    // distinct triangles of the tile's row and column RWGs (wave 0: lanes
+    // 0-31 the rows' sides, 32-63 the columns'; first occurrences kept)
+    if (threadIdx.x < 64) {
+        const int l = threadIdx.x, k = l & 31;
+        const int e = (l < 32 ? 16 * I : 16 * J) + (k >> 1);
+        const int t = e < Ne ? sideTri[2 * e + (k & 1)] : -1;
+        bool first = t >= 0;
+        for (int q = 0; q < 32; ++q) {
+            const int u = __shfl(t, (l & 32) + q);
+            if (q < k && u == t) first = false;
+        }
+        const unsigned long long mk = __ballot(first);
+        const unsigned long long half = l < 32 ? (mk & 0xffffffffull) : (mk >> 32);
+        const int rank = __popcll(half & ((1ull << k) - 1ull));
+        if (first) (l < 32 ? rl : cl)[rank] = t;
+        if (k == 0) cnt[l >> 5] = __popcll(half);
+    }
+    __syncthreads();
+    const int nR = cnt[0], nC = cnt[1], nP = nR * nC;
+    const bool live = m < Ne && n < Ne;
+
     double D[NT];
 #pragma unroll
     for (int j = 0; j < NT; ++j) D[j] = 0.0;
-    // unrolled, so the side and pair arrays stay in registers
+    if (nP <= kCoefSlots) {
+        for (int s = threadIdx.x; s < nP; s += 256) {
+            const int t1 = rl[s / nC], t2 = cl[s % nC];
+            slotRec[s] = (t1 <= t2 ? rec_index(t1, t2) : rec_index(t2, t1)) * stride;
+        }
+        // this entry's four pairs: slot, side coefficients, shift
+        int slot[4];
+        double k0[4], k1x[4], k1y[4], k1z[4], k4x[4], k4y[4], k4z[4], cab[4], dl[4];
 #pragma unroll
-    for (int ab = 0; ab < 4; ++ab) {
-        const int a = ab >> 1, b = ab & 1;
-        if (tm[a] < 0 || tn[b] < 0) continue;
-        const bool sw = tm[a] > tn[b];
-        const double* q = Q + (sw ? rec_index(tn[b], tm[a]) : rec_index(tm[a], tn[b])) * stride;
-        const Side A = sa[a], B = sb[b];
-        const double cab = A.c * B.c;
-        // S2 - S1.un - S1'.um + S0 (um.un); S1 and S1' exchanged when the
-        // pair is stored the other way round
-        const double k0 = cab * (A.ux * B.ux + A.uy * B.uy + A.uz * B.uz);
-        const double k1x = -cab * (sw ? A.ux : B.ux), k1y = -cab * (sw ? A.uy : B.uy), k1z = -cab * (sw ? A.uz : B.uz);
-        const double k4x = -cab * (sw ? B.ux : A.ux), k4y = -cab * (sw ? B.uy : A.uy), k4z = -cab * (sw ? B.uz : A.uz);
-        const double ks = 4.0 * cab;
-        // D[j] (order z^(j-1)) = V[j-2] + S[j]
-        double Dp[NT];
+        for (int ab = 0; ab < 4; ++ab) {
+            const int a = ab >> 1, b = ab & 1;
+            slot[ab] = -1;
+            k0[ab] = k1x[ab] = k1y[ab] = k1z[ab] = k4x[ab] = k4y[ab] = k4z[ab] = cab[ab] = dl[ab] = 0.0;
+            if (!live || tm[a] < 0 || tn[b] < 0) continue;
+            int ri = 0, ci = 0;
+            for (int i = 0; i < nR; ++i) ri = rl[i] == tm[a] ? i : ri;
+            for (int i = 0; i < nC; ++i) ci = cl[i] == tn[b] ? i : ci;
+            slot[ab] = ri * nC + ci;
+            const bool sw = tm[a] > tn[b];
+            const Side A = sa[a], B = sb[b];
+            const double c2 = A.c * B.c;
+            // S2 - S1.un - S1'.um + S0 (um.un); S1 and S1' exchanged when
+            // the pair is stored the other way round
+            cab[ab] = c2;
+            k0[ab] = c2 * (A.ux * B.ux + A.uy * B.uy + A.uz * B.uz);
+            k1x[ab] = -c2 * (sw ? A.ux : B.ux);
+            k1y[ab] = -c2 * (sw ? A.uy : B.uy);
+            k1z[ab] = -c2 * (sw ? A.uz : B.uz);
+            k4x[ab] = -c2 * (sw ? B.ux : A.ux);
+            k4y[ab] = -c2 * (sw ? B.uy : A.uy);
+            k4z[ab] = -c2 * (sw ? B.uz : A.uz);
+            dl[ab] = R0p[a][b] - R0e;
+        }
+        __syncthreads();  // slotRec
+        constexpr int NU = kCoefSlots * 8 / 256;
+        double pre[NU];
+        auto load = [&](int qq) {
 #pragma unroll
-        for (int j = 0; j < NT; ++j) Dp[j] = 0.0;
+            for (int u = 0; u < NU; ++u) {
+                const int e = threadIdx.x + 256 * u, sl = e >> 3;
+                pre[u] = sl < nP ? Q[slotRec[sl] + (size_t)qq * 8 + (e & 7)] : 0.0;
+            }
+        };
+        load(0);
 #pragma unroll
         for (int qq = 0; qq <= P; ++qq) {
-            const double2* v = reinterpret_cast<const double2*>(q + (size_t)qq * 8);
-            const double2 v0 = v[0], v1 = v[1], v2 = v[2], v3 = v[3];
-            const double V = k0 * v0.x + k1x * v0.y + k1y * v1.x + k1z * v1.y + k4x * v2.x + k4y * v2.y +
-                             k4z * v3.x + cab * v3.y;
-            Dp[qq] += ks * v0.x;
-            Dp[qq + 2] += V;
+            __syncthreads();  // the previous order is consumed
+#pragma unroll
+            for (int u = 0; u < NU; ++u) {
+                const int e = threadIdx.x + 256 * u, sl = e >> 3;
+                if (sl < nP) qs[sl][e & 7] = pre[u];
+            }
+            __syncthreads();
+            if (qq < P) load(qq + 1);
+#pragma unroll
+            for (int ab = 0; ab < 4; ++ab) {
+                if (slot[ab] < 0) continue;
+                const double2* v = reinterpret_cast<const double2*>(qs[slot[ab]]);
+                const double2 v0 = v[0], v1 = v[1], v2 = v[2], v3 = v[3];
+                // D[j] (order z^(j-1)) = V[j-2] + S[j], shifted by e^{z d}
+                const double a0 = 4.0 * cab[ab] * v0.x;
+                const double V = k0[ab] * v0.x + k1x[ab] * v0.y + k1y[ab] * v1.x + k1z[ab] * v1.y +
+                                 k4x[ab] * v2.x + k4y[ab] * v2.y + k4z[ab] * v3.x + cab[ab] * v3.y;
+                double t = 1.0;
+#pragma unroll
+                for (int k = 0; qq + k < NT; ++k) {
+                    D[qq + k] += a0 * t;
+                    if (qq + 2 + k < NT) D[qq + 2 + k] += V * t;
+                    t *= dl[ab] * kInvN[k + 1];
+                }
+            }
         }
-        // shift to the common centre: times e^{z d}, d = R0 - R0e
-        const double d = R0p[a][b] - R0e;
-        double e[NT];
-        e[0] = 1.0;
-#pragma unroll
-        for (int j = 1; j < NT; ++j) e[j] = e[j - 1] * d * kInvN[j];
-#pragma unroll
-        for (int j = 0; j < NT; ++j) {
-            double s = 0.0;
-#pragma unroll
-            for (int i = 0; i <= j; ++i) s += Dp[i] * e[j - i];
-            D[j] += s;
+    } else {
+        // unrolled, so the side and pair arrays stay in registers
+    #pragma unroll
+        for (int ab = 0; ab < 4; ++ab) {
+            const int a = ab >> 1, b = ab & 1;
+            if (tm[a] < 0 || tn[b] < 0) continue;
+            const bool sw = tm[a] > tn[b];
+            const double* q = Q + (sw ? rec_index(tn[b], tm[a]) : rec_index(tm[a], tn[b])) * stride;
+            const Side A = sa[a], B = sb[b];
+            const double cab = A.c * B.c;
+            // S2 - S1.un - S1'.um + S0 (um.un); S1 and S1' exchanged when the
+            // pair is stored the other way round
+            const double k0 = cab * (A.ux * B.ux + A.uy * B.uy + A.uz * B.uz);
+            const double k1x = -cab * (sw ? A.ux : B.ux), k1y = -cab * (sw ? A.uy : B.uy), k1z = -cab * (sw ? A.uz : B.uz);
+            const double k4x = -cab * (sw ? B.ux : A.ux), k4y = -cab * (sw ? B.uy : A.uy), k4z = -cab * (sw ? B.uz : A.uz);
+            const double ks = 4.0 * cab;
+            // D[j] (order z^(j-1)) = V[j-2] + S[j]
+            double Dp[NT];
+    #pragma unroll
+            for (int j = 0; j < NT; ++j) Dp[j] = 0.0;
+    #pragma unroll
+            for (int qq = 0; qq <= P; ++qq) {
+                const double2* v = reinterpret_cast<const double2*>(q + (size_t)qq * 8);
+                const double2 v0 = v[0], v1 = v[1], v2 = v[2], v3 = v[3];
+                const double V = k0 * v0.x + k1x * v0.y + k1y * v1.x + k1z * v1.y + k4x * v2.x + k4y * v2.y +
+                                 k4z * v3.x + cab * v3.y;
+                Dp[qq] += ks * v0.x;
+                Dp[qq + 2] += V;
+            }
+            // shift to the common centre: times e^{z d}, d = R0 - R0e
+            const double d = R0p[a][b] - R0e;
+            double e[NT];
+            e[0] = 1.0;
+    #pragma unroll
+            for (int j = 1; j < NT; ++j) e[j] = e[j - 1] * d * kInvN[j];
+    #pragma unroll
+            for (int j = 0; j < NT; ++j) {
+                double s = 0.0;
+    #pragma unroll
+                for (int i = 0; i <= j; ++i) s += Dp[i] * e[j - i];
+                D[j] += s;
+            }
         }
     }
 
