@@ -218,7 +218,12 @@ struct TrsvWork {
     int* tickets;
     int* err;
     int epoch;
+    // blocks per matrix (each takes row blocks in order until none is
+    // left); 0: one per row block.  A few blocks leave the CUs to work that
+    // runs beside the solve (the next batch's factorisation).
+    int rows = 0;
 };
+constexpr int kTrsvRowsShared = 4;
 
 // Back substitution U x = y in place on column n of A (single-launch
 // solve; tw holds its flags, tickets and error word).

@@ -1172,166 +1172,173 @@ __global__ __launch_bounds__(256) void k_trsv_fused(const T* A, size_t strideA, 
     if (skip && skip[b]) return;
     const int tid = threadIdx.x;
     const int nblk = (n + 63) / 64;
-    if (tid == 0) s_vid = atomicAdd(tickets + b, 1);
-    __syncthreads();
-    const int vid = s_vid;
-    const int bk = LOWER ? vid : nblk - 1 - vid;
-    const int r0 = bk * 64;
-    const int bs = n - r0 < 64 ? n - r0 : 64;
-    const T* Ab = A + (size_t)b * strideA;
-    T* Xb = X + (size_t)b * strideX;
-    int* fl = flags + (size_t)b * nblk;
-    // diagonal block: all 16 loads per thread in flight at once (clamped
-    // into the block, kept only where inside)
-    {
-        cpx<T> v[16];
-#pragma unroll
-        for (int u2 = 0; u2 < 16; ++u2) {
-            const int e = tid + 256 * u2, ii = e >> 6, k = e & 63;
-            v[u2] = ld(Ab + 2 * ((size_t)(r0 + (ii < bs ? ii : bs - 1)) * lda + r0 + (k < bs ? k : bs - 1)));
-        }
-#pragma unroll
-        for (int u2 = 0; u2 < 16; ++u2) {
-            const int e = tid + 256 * u2, ii = e >> 6, k = e & 63;
-            if (ii < bs && k < bs) u[ii][k] = v[u2];
-        }
-    }
-    for (int i = tid; i < bs; i += 256) xb[i] = ld(Xb + 2 * (size_t)(r0 + i) * xs);
-    // thread (wave w, half h, pair p) accumulates, for its 8 rows
-    // 16w + 2k + h, the products of columns 2p and 2p+1 of every earlier
-    // block column with x; one load instruction of a wave then covers two
-    // contiguous 64-column row segments.  The 32 lanes of a half are summed
-    // once, after the last block column.
-    const int w = __builtin_amdgcn_readfirstlane(tid >> 6), h = (tid >> 5) & 1, p = tid & 31;
-    cpx<T> acc[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) acc[k] = {0, 0};
-    auto blk = [&](int st) { return LOWER ? st : nblk - 1 - st; };
-    // columns 2p, 2p+1 of this thread's rows in block column blk(st); the
-    // pair start is clamped into the block's columns (the row has lda > n
-    // entries, so the pair's second column stays in the row).  Columns past
-    // the block are zeroed where the values are used, so no select follows
-    // the loads (that would make the prefetch wait for them).
-    auto load_m = [&](int st, cpx<T>* m) {
-        const int c0 = blk(st) * 64;
-        const int cs = n - c0 < 64 ? n - c0 : 64;
-        const int jp = 2 * p < cs ? 2 * p : ((cs - 1) & ~1);
-        if (bs == 64) {
-            // full block: wave-uniform row bases, one lane offset
-            const int off = 2 * (h * lda + jp);
-#pragma unroll
-            for (int k = 0; k < 8; ++k)
-                ld_pair(Ab + 2 * ((size_t)(r0 + 16 * w + 2 * k) * lda + c0) + off, m[2 * k], m[2 * k + 1]);
-        } else {
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const int rr = 16 * w + 2 * k + h;
-                ld_pair(Ab + 2 * ((size_t)(r0 + (rr < bs ? rr : bs - 1)) * lda + c0 + jp), m[2 * k], m[2 * k + 1]);
+    // row blocks are taken in order (tickets), one after another, until
+    // none is left: a row block waits only on earlier ones, which running
+    // blocks hold, so the solve drains with any number of blocks
+    for (;;) {
+        if (tid == 0) s_vid = atomicAdd(tickets + b, 1);
+        __syncthreads();
+        const int vid = s_vid;
+        if (vid >= nblk) break;
+        const int bk = LOWER ? vid : nblk - 1 - vid;
+        const int r0 = bk * 64;
+        const int bs = n - r0 < 64 ? n - r0 : 64;
+        const T* Ab = A + (size_t)b * strideA;
+        T* Xb = X + (size_t)b * strideX;
+        int* fl = flags + (size_t)b * nblk;
+        // diagonal block: all 16 loads per thread in flight at once (clamped
+        // into the block, kept only where inside)
+        {
+            cpx<T> v[16];
+    #pragma unroll
+            for (int u2 = 0; u2 < 16; ++u2) {
+                const int e = tid + 256 * u2, ii = e >> 6, k = e & 63;
+                v[u2] = ld(Ab + 2 * ((size_t)(r0 + (ii < bs ? ii : bs - 1)) * lda + r0 + (k < bs ? k : bs - 1)));
+            }
+    #pragma unroll
+            for (int u2 = 0; u2 < 16; ++u2) {
+                const int e = tid + 256 * u2, ii = e >> 6, k = e & 63;
+                if (ii < bs && k < bs) u[ii][k] = v[u2];
             }
         }
-    };
-    cpx<T> m[16], mn[16];
-    for (int step = 0; step < vid;) {
-        load_m(step, m);  // in flight during the wait
-        if (tid == 0) {
-            long long spins = 0;
-            while (__hip_atomic_load(fl + blk(step), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-                __builtin_amdgcn_s_sleep(1);
-                if (++spins > kSpinLimit) {
-                    atomicExch(err, 1);
-                    break;
-                }
-            }
-            int av = step + 1;
-            while (av < vid && av < step + kTrsvRun &&
-                   __hip_atomic_load(fl + blk(av), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch)
-                ++av;
-            acquire_agent();
-            s_avail = av;
-        }
-        __syncthreads();
-        acquire_wg();
-        const int av = s_avail;
-        // x of the published run, one round of loads
-        for (int e = tid; e < (av - step) * 64; e += 256) {
-            const int c0 = blk(step + (e >> 6)) * 64, j = e & 63;
-            cpx<T> v = {0, 0};
-            if (c0 + j < n) v = ald_c(Xb + 2 * (size_t)(c0 + j) * xs);
-            xc[e >> 6][j] = v;
-        }
-        __syncthreads();
-        for (int st = step; st < av; ++st) {
-            if (st + 1 < av) load_m(st + 1, mn);  // next block's values before this one's products
-            const cpx<T> x0 = xc[st - step][2 * p], x1 = xc[st - step][2 * p + 1];
-            const int cs = n - blk(st) * 64;
-            if (cs < 64) {
-                // last block column: whatever the padding columns hold
-#pragma unroll
+        for (int i = tid; i < bs; i += 256) xb[i] = ld(Xb + 2 * (size_t)(r0 + i) * xs);
+        // thread (wave w, half h, pair p) accumulates, for its 8 rows
+        // 16w + 2k + h, the products of columns 2p and 2p+1 of every earlier
+        // block column with x; one load instruction of a wave then covers two
+        // contiguous 64-column row segments.  The 32 lanes of a half are summed
+        // once, after the last block column.
+        const int w = __builtin_amdgcn_readfirstlane(tid >> 6), h = (tid >> 5) & 1, p = tid & 31;
+        cpx<T> acc[8];
+    #pragma unroll
+        for (int k = 0; k < 8; ++k) acc[k] = {0, 0};
+        auto blk = [&](int st) { return LOWER ? st : nblk - 1 - st; };
+        // columns 2p, 2p+1 of this thread's rows in block column blk(st); the
+        // pair start is clamped into the block's columns (the row has lda > n
+        // entries, so the pair's second column stays in the row).  Columns past
+        // the block are zeroed where the values are used, so no select follows
+        // the loads (that would make the prefetch wait for them).
+        auto load_m = [&](int st, cpx<T>* m) {
+            const int c0 = blk(st) * 64;
+            const int cs = n - c0 < 64 ? n - c0 : 64;
+            const int jp = 2 * p < cs ? 2 * p : ((cs - 1) & ~1);
+            if (bs == 64) {
+                // full block: wave-uniform row bases, one lane offset
+                const int off = 2 * (h * lda + jp);
+    #pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    ld_pair(Ab + 2 * ((size_t)(r0 + 16 * w + 2 * k) * lda + c0) + off, m[2 * k], m[2 * k + 1]);
+            } else {
+    #pragma unroll
                 for (int k = 0; k < 8; ++k) {
-                    if (2 * p >= cs) m[2 * k] = {0, 0};
-                    if (2 * p + 1 >= cs) m[2 * k + 1] = {0, 0};
+                    const int rr = 16 * w + 2 * k + h;
+                    ld_pair(Ab + 2 * ((size_t)(r0 + (rr < bs ? rr : bs - 1)) * lda + c0 + jp), m[2 * k], m[2 * k + 1]);
                 }
             }
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const cpx<T> a = m[2 * k], c = m[2 * k + 1];
-                acc[k].r += a.r * x0.r - a.i * x0.i + c.r * x1.r - c.i * x1.i;
-                acc[k].i += a.r * x0.i + a.i * x0.r + c.r * x1.i + c.i * x1.r;
+        };
+        cpx<T> m[16], mn[16];
+        for (int step = 0; step < vid;) {
+            load_m(step, m);  // in flight during the wait
+            if (tid == 0) {
+                long long spins = 0;
+                while (__hip_atomic_load(fl + blk(step), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+                    __builtin_amdgcn_s_sleep(1);
+                    if (++spins > kSpinLimit) {
+                        atomicExch(err, 1);
+                        break;
+                    }
+                }
+                int av = step + 1;
+                while (av < vid && av < step + kTrsvRun &&
+                       __hip_atomic_load(fl + blk(av), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch)
+                    ++av;
+                acquire_agent();
+                s_avail = av;
             }
-            if (st + 1 < av) {
-#pragma unroll
-                for (int k = 0; k < 16; ++k) m[k] = mn[k];
+            __syncthreads();
+            acquire_wg();
+            const int av = s_avail;
+            // x of the published run, one round of loads
+            for (int e = tid; e < (av - step) * 64; e += 256) {
+                const int c0 = blk(step + (e >> 6)) * 64, j = e & 63;
+                cpx<T> v = {0, 0};
+                if (c0 + j < n) v = ald_c(Xb + 2 * (size_t)(c0 + j) * xs);
+                xc[e >> 6][j] = v;
             }
+            __syncthreads();
+            for (int st = step; st < av; ++st) {
+                if (st + 1 < av) load_m(st + 1, mn);  // next block's values before this one's products
+                const cpx<T> x0 = xc[st - step][2 * p], x1 = xc[st - step][2 * p + 1];
+                const int cs = n - blk(st) * 64;
+                if (cs < 64) {
+                    // last block column: whatever the padding columns hold
+    #pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        if (2 * p >= cs) m[2 * k] = {0, 0};
+                        if (2 * p + 1 >= cs) m[2 * k + 1] = {0, 0};
+                    }
+                }
+    #pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const cpx<T> a = m[2 * k], c = m[2 * k + 1];
+                    acc[k].r += a.r * x0.r - a.i * x0.i + c.r * x1.r - c.i * x1.i;
+                    acc[k].i += a.r * x0.i + a.i * x0.r + c.r * x1.i + c.i * x1.r;
+                }
+                if (st + 1 < av) {
+    #pragma unroll
+                    for (int k = 0; k < 16; ++k) m[k] = mn[k];
+                }
+            }
+            __syncthreads();  // xc is refilled by the next run
+            step = av;
         }
-        __syncthreads();  // xc is refilled by the next run
-        step = av;
-    }
-    // each 16-lane row of pairs sums its partials in registers; the two
-    // rows of a half meet in part
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        acc[k].r = row16_sum(acc[k].r);
-        acc[k].i = row16_sum(acc[k].i);
-    }
-    if ((p & 15) == 0) {
-#pragma unroll
-        for (int k = 0; k < 8; ++k) part[p >> 4][16 * w + 2 * k + h] = acc[k];
-    }
-    __syncthreads();
-    // diagonal solve by wave 0 with x in registers (lane k holds x_k);
-    // x_j reaches every lane by readlane, so the steps need no LDS round
-    // trip or barrier
-    if (tid < 64) {
-        cpx<T> v = {0, 0};
-        if (tid < bs) {
-            v = xb[tid];
-            v.r -= part[0][tid].r + part[1][tid].r;
-            v.i -= part[0][tid].i + part[1][tid].i;
+        // each 16-lane row of pairs sums its partials in registers; the two
+        // rows of a half meet in part
+    #pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            acc[k].r = row16_sum(acc[k].r);
+            acc[k].i = row16_sum(acc[k].i);
         }
-        for (int jj = 0; jj < bs; ++jj) {
-            const int j = LOWER ? jj : bs - 1 - jj;
-            cpx<T> xj;
-            xj.r = lane_read(v.r, j);
-            xj.i = lane_read(v.i, j);
-            if (!LOWER) {
-                const cpx<T> d = u[j][j];
-                xj = (d.r == (T)0 && d.i == (T)0) ? xj : cmul(xj, cinv(d));
-                if (tid == j) v = xj;
-            }
-            const bool act = LOWER ? (tid > j && tid < bs) : (tid < j);
-            if (act) {
-                const cpx<T> l = u[tid][j];
-                v.r -= l.r * xj.r - l.i * xj.i;
-                v.i -= l.r * xj.i + l.i * xj.r;
-            }
+        if ((p & 15) == 0) {
+    #pragma unroll
+            for (int k = 0; k < 8; ++k) part[p >> 4][16 * w + 2 * k + h] = acc[k];
         }
-        if (tid < bs) xb[tid] = v;
+        __syncthreads();
+        // diagonal solve by wave 0 with x in registers (lane k holds x_k);
+        // x_j reaches every lane by readlane, so the steps need no LDS round
+        // trip or barrier
+        if (tid < 64) {
+            cpx<T> v = {0, 0};
+            if (tid < bs) {
+                v = xb[tid];
+                v.r -= part[0][tid].r + part[1][tid].r;
+                v.i -= part[0][tid].i + part[1][tid].i;
+            }
+            for (int jj = 0; jj < bs; ++jj) {
+                const int j = LOWER ? jj : bs - 1 - jj;
+                cpx<T> xj;
+                xj.r = lane_read(v.r, j);
+                xj.i = lane_read(v.i, j);
+                if (!LOWER) {
+                    const cpx<T> d = u[j][j];
+                    xj = (d.r == (T)0 && d.i == (T)0) ? xj : cmul(xj, cinv(d));
+                    if (tid == j) v = xj;
+                }
+                const bool act = LOWER ? (tid > j && tid < bs) : (tid < j);
+                if (act) {
+                    const cpx<T> l = u[tid][j];
+                    v.r -= l.r * xj.r - l.i * xj.i;
+                    v.i -= l.r * xj.i + l.i * xj.r;
+                }
+            }
+            if (tid < bs) xb[tid] = v;
+        }
+        __syncthreads();
+        for (int k = tid; k < bs; k += 256) ast_c(Xb + 2 * (size_t)(r0 + k) * xs, xb[k]);
+        release_agent();
+        __syncthreads();
+        if (tid == 0) __hip_atomic_store(fl + bk, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();  // LDS and s_vid are reused by the next row block
     }
-    __syncthreads();
-    for (int k = tid; k < bs; k += 256) ast_c(Xb + 2 * (size_t)(r0 + k) * xs, xb[k]);
-    release_agent();
-    __syncthreads();
-    if (tid == 0) __hip_atomic_store(fl + bk, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // perm[b][i] = source row of position i after the sequential swaps of ipiv
@@ -1673,11 +1680,12 @@ hipError_t lu_trsv(const T* A, size_t strideA, int n, int lda, T* X, size_t stri
     if (!tw || (lda & 1) || lda <= n) return hipErrorInvalidValue;
     ++tw->epoch;
     hipLaunchKernelGGL(k_zero_int, dim3((batch + 255) / 256), dim3(256), 0, s, tw->tickets, batch);
+    const int nb = tw->rows > 0 && tw->rows < nblk ? tw->rows : nblk;
     if (lower)
-        hipLaunchKernelGGL((k_trsv_fused<T, true>), dim3(batch, nblk), dim3(256), 0, s, A, strideA, lda, n, X,
+        hipLaunchKernelGGL((k_trsv_fused<T, true>), dim3(batch, nb), dim3(256), 0, s, A, strideA, lda, n, X,
                            strideX, xs, tw->flags, tw->tickets, tw->epoch, skip, tw->err);
     else
-        hipLaunchKernelGGL((k_trsv_fused<T, false>), dim3(batch, nblk), dim3(256), 0, s, A, strideA, lda, n, X,
+        hipLaunchKernelGGL((k_trsv_fused<T, false>), dim3(batch, nb), dim3(256), 0, s, A, strideA, lda, n, X,
                            strideX, xs, tw->flags, tw->tickets, tw->epoch, skip, tw->err);
     return hipGetLastError();
 }
