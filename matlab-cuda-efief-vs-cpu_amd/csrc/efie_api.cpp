@@ -2274,8 +2274,11 @@ efie_status efie_sweep(efie_ctx* c, const double* freqs, int Nf, double mu, doub
             // every frequency of the batch in one pass over the series
             Prof pff = make_prof(c);
             pff.begin(EFIE_K_SERIES);
+            // every fill after the first runs beside a factorisation: a
+            // capped grid leaves that the CUs
             HIPCHK(launch_series_fill(c->dSerC.as<double>(), plan.serNT, plan.lossy, n, dfr, nb, Zb, (size_t)lda,
-                                      2 * zElems, Z32b, (size_t)lda, 2 * zElems, sf));
+                                      2 * zElems, Z32b, (size_t)lda, 2 * zElems, sf,
+                                      overlap && bi > 0 ? kFillBlocksShared : 0));
             pff.end(EFIE_K_SERIES);
         } else {
             for (int i = 0; i < nb; ++i) {

@@ -1687,78 +1687,82 @@ __global__ __launch_bounds__(256) void k_series_fill(const double* __restrict__ 
                                                      size_t ld, size_t zStride, float* __restrict__ Z32,
                                                      size_t ld32, size_t z32Stride, int nblk) {
     __shared__ double2 tile[16][17];
-    long long id = blockIdx.x;
-    {
-        const int q = nblk >> 3, rr = nblk & 7, x = (int)(id & 7), loc = (int)(id >> 3);
-        id = x < rr ? (long long)x * (q + 1) + loc : (long long)rr * (q + 1) + (long long)(x - rr) * q + loc;
-    }
-    int J = (int)((sqrt(8.0 * (double)id + 1.0) - 1.0) * 0.5);
-    while ((long long)(J + 1) * (J + 2) / 2 <= id) ++J;
-    while ((long long)J * (J + 1) / 2 > id) --J;
-    const int I = (int)(id - (long long)J * (J + 1) / 2);
-    const int r = threadIdx.x >> 4, cI = threadIdx.x & 15;
-    const int m = 16 * I + r, n = 16 * J + cI;
-    const double* in = C + (size_t)id * (NT + 1) * 256 + threadIdx.x;
-    double D[NT];
-#pragma unroll
-    for (int j = 0; j < NT; ++j) D[j] = in[(size_t)j * 256];
-    const double R0e = in[(size_t)NT * 256];
+    // tiles in a grid-stride loop: the grid may be smaller than the tile
+    // count (a fill that runs beside a factorisation leaves it the CUs)
+    for (long long t = blockIdx.x; t < nblk; t += gridDim.x) {
+        long long id = t;
+        {
+            const int q = nblk >> 3, rr = nblk & 7, x = (int)(id & 7), loc = (int)(id >> 3);
+            id = x < rr ? (long long)x * (q + 1) + loc : (long long)rr * (q + 1) + (long long)(x - rr) * q + loc;
+        }
+        int J = (int)((sqrt(8.0 * (double)id + 1.0) - 1.0) * 0.5);
+        while ((long long)(J + 1) * (J + 2) / 2 <= id) ++J;
+        while ((long long)J * (J + 1) / 2 > id) --J;
+        const int I = (int)(id - (long long)J * (J + 1) / 2);
+        const int r = threadIdx.x >> 4, cI = threadIdx.x & 15;
+        const int m = 16 * I + r, n = 16 * J + cI;
+        const double* in = C + (size_t)id * (NT + 1) * 256 + threadIdx.x;
+        double D[NT];
+    #pragma unroll
+        for (int j = 0; j < NT; ++j) D[j] = in[(size_t)j * 256];
+        const double R0e = in[(size_t)NT * 256];
 
-    const bool upper = m < Ne && n < Ne && (I != J || r <= cI);
-    const int mm = 16 * J + r, nn = 16 * I + cI;
-    const bool lower = mm < Ne && nn < Ne && (I != J || r > cI);
-    for (int f = 0; f < nf; ++f) {
-        const Freq F = fr[f];
-        // p = sum_j z^j D[j], z = -jk = ki - j kr
-        double pr, pi;
-        if (!LOSSY) {
-            // z = -j kr: p = A - j kr B with A, B the even and odd parts,
-            // polynomials in u = -kr^2
-            const double u = -F.kr * F.kr;
-            double A = D[NT - 1 - ((NT - 1) & 1)], B = D[NT - 1 - (NT & 1)];
-#pragma unroll
-            for (int j = NT - 3 - ((NT - 1) & 1); j >= 0; j -= 2) A = A * u + D[j];
-#pragma unroll
-            for (int j = NT - 3 - (NT & 1); j >= 1; j -= 2) B = B * u + D[j];
-            pr = A;
-            pi = -F.kr * B;
-        } else {
-            const double zr = F.ki, zi = -F.kr;
-            pr = D[NT - 1];
-            pi = 0.0;
-#pragma unroll
-            for (int j = NT - 2; j >= 0; --j) {
-                const double tr = pr * zr - pi * zi + D[j];
-                pi = pr * zi + pi * zr;
-                pr = tr;
+        const bool upper = m < Ne && n < Ne && (I != J || r <= cI);
+        const int mm = 16 * J + r, nn = 16 * I + cI;
+        const bool lower = mm < Ne && nn < Ne && (I != J || r > cI);
+        for (int f = 0; f < nf; ++f) {
+            const Freq F = fr[f];
+            // p = sum_j z^j D[j], z = -jk = ki - j kr
+            double pr, pi;
+            if (!LOSSY) {
+                // z = -j kr: p = A - j kr B with A, B the even and odd parts,
+                // polynomials in u = -kr^2
+                const double u = -F.kr * F.kr;
+                double A = D[NT - 1 - ((NT - 1) & 1)], B = D[NT - 1 - (NT & 1)];
+    #pragma unroll
+                for (int j = NT - 3 - ((NT - 1) & 1); j >= 0; j -= 2) A = A * u + D[j];
+    #pragma unroll
+                for (int j = NT - 3 - (NT & 1); j >= 1; j -= 2) B = B * u + D[j];
+                pr = A;
+                pi = -F.kr * B;
+            } else {
+                const double zr = F.ki, zi = -F.kr;
+                pr = D[NT - 1];
+                pi = 0.0;
+    #pragma unroll
+                for (int j = NT - 2; j >= 0; --j) {
+                    const double tr = pr * zr - pi * zi + D[j];
+                    pi = pr * zi + pi * zr;
+                    pr = tr;
+                }
             }
+            // e^{-jk R0e} = e^{ki R0e} (cos - j sin)(kr R0e)
+            double s, c;
+            sincos_cw(F.kr * R0e, &s, &c);
+            if (LOSSY) {
+                const double amp = exp(F.ki * R0e);
+                s *= amp;
+                c *= amp;
+            }
+            const double wr = pr * c + pi * s, wi = pi * c - pr * s;
+            // -eta/(4 pi) z^{-1} = -alpha/(4 pi k^2) = h
+            const double2 val = make_double2(F.hr * wr - F.hi * wi, F.hr * wi + F.hi * wr);
+            double* Zf = Z + zStride * f;
+            float* Z32f = Z32 ? Z32 + z32Stride * f : nullptr;
+            tile[r][cI] = val;
+            if (upper) {
+                *reinterpret_cast<double2*>(Zf + 2 * ((size_t)m * ld + n)) = val;
+                if (Z32f) *reinterpret_cast<float2*>(Z32f + 2 * ((size_t)m * ld32 + n)) = make_float2((float)val.x, (float)val.y);
+            }
+            __syncthreads();
+            if (lower) {
+                const double2 v = tile[cI][r];
+                *reinterpret_cast<double2*>(Zf + 2 * ((size_t)mm * ld + nn)) = v;
+                if (Z32f)
+                    *reinterpret_cast<float2*>(Z32f + 2 * ((size_t)mm * ld32 + nn)) = make_float2((float)v.x, (float)v.y);
+            }
+            __syncthreads();
         }
-        // e^{-jk R0e} = e^{ki R0e} (cos - j sin)(kr R0e)
-        double s, c;
-        sincos_cw(F.kr * R0e, &s, &c);
-        if (LOSSY) {
-            const double amp = exp(F.ki * R0e);
-            s *= amp;
-            c *= amp;
-        }
-        const double wr = pr * c + pi * s, wi = pi * c - pr * s;
-        // -eta/(4 pi) z^{-1} = -alpha/(4 pi k^2) = h
-        const double2 val = make_double2(F.hr * wr - F.hi * wi, F.hr * wi + F.hi * wr);
-        double* Zf = Z + zStride * f;
-        float* Z32f = Z32 ? Z32 + z32Stride * f : nullptr;
-        tile[r][cI] = val;
-        if (upper) {
-            *reinterpret_cast<double2*>(Zf + 2 * ((size_t)m * ld + n)) = val;
-            if (Z32f) *reinterpret_cast<float2*>(Z32f + 2 * ((size_t)m * ld32 + n)) = make_float2((float)val.x, (float)val.y);
-        }
-        __syncthreads();
-        if (lower) {
-            const double2 v = tile[cI][r];
-            *reinterpret_cast<double2*>(Zf + 2 * ((size_t)mm * ld + nn)) = v;
-            if (Z32f)
-                *reinterpret_cast<float2*>(Z32f + 2 * ((size_t)mm * ld32 + nn)) = make_float2((float)v.x, (float)v.y);
-        }
-        __syncthreads();
     }
 }
 
@@ -2103,10 +2107,11 @@ hipError_t launch_series_coeffs(const double* Q, int NT, const TriGeo* tg, const
 }
 
 hipError_t launch_series_fill(const double* C, int NT, bool lossy, int Ne, const Freq* fr, int nf, double* Z,
-                              size_t ld, size_t zStride, float* Z32, size_t ld32, size_t z32Stride, hipStream_t s) {
+                              size_t ld, size_t zStride, float* Z32, size_t ld32, size_t z32Stride, hipStream_t s,
+                              int maxBlocks) {
     const int T = (Ne + 15) / 16;
     const long long nb = (long long)T * (T + 1) / 2;
-    const dim3 g((unsigned)nb), blk(256);
+    const dim3 g((unsigned)(maxBlocks > 0 && maxBlocks < nb ? maxBlocks : nb)), blk(256);
 #define EFIE_SER_CASE(N)                                                                                         \
     case N:                                                                                                       \
         if (lossy)                                                                                                \

@@ -150,9 +150,13 @@ hipError_t launch_series_coeffs(const double* Q, int NT, const TriGeo* tg, const
 // Z of nf frequencies from the coefficients: matrix f at Z + f*zStride,
 // element (m,n) at 2*(m*ld + n), and its complex64 copy at
 // Z32 + f*z32Stride when Z32 is set.  lossy: some k has a nonzero
-// imaginary part.
+// imaginary part.  maxBlocks > 0 caps the grid (tiles in a grid-stride
+// loop; a multiple of 8 keeps each XCD on its run of tiles).
 hipError_t launch_series_fill(const double* C, int NT, bool lossy, int Ne, const Freq* fr, int nf, double* Z,
-                              size_t ld, size_t zStride, float* Z32, size_t ld32, size_t z32Stride, hipStream_t s);
+                              size_t ld, size_t zStride, float* Z32, size_t ld32, size_t z32Stride, hipStream_t s,
+                              int maxBlocks = 0);
+// blocks of a series fill that runs beside a factorisation
+constexpr int kFillBlocksShared = 256;
 
 // Z(m,n) = sum over the four side pairs of alpha*Ivec + beta*Isca.
 // Output element (m,n) goes to Z[m*ld + n] (interleaved complex; Z is
