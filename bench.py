@@ -305,32 +305,98 @@ def cpu_baseline(V, F, Ne, fr, threads, gpu_rows=None):
                       f"one frequency point = {fill_s + t_lu:.0f} s"}
 
 
+def _c4_problems():
+    out = []
+    for g in W.c4_geometries():
+        V, F = efie.make_patch_with_ground_mesh(*g["geom"], *g["cells"])
+        r = efie.build_rwg_interior(V, F)
+        pP, pG = W.c4_feed_points(g["geom"])
+        out.append((g, V, F, r, efie.pick_port_edge_by_point_and_layer(r, pP, 1),
+                    efie.pick_port_edge_by_point_and_layer(r, pG, -1), 1.5 * r.len.mean()))
+    return out
+
+
 def bench_c4(a):
     """The batched uneven sweep: 32 seeded geometries (N 1000-8000), 64
-    frequencies each, mixed path.  Geometries are handed out largest first
-    to `--c4-workers` host threads (default 1), each with its own context
-    (streams, buffers); with two, the sweeps of two geometries run side by
-    side on the GPU (measured: 224.9 against 224.6 points/s, so the ends of
-    one geometry's pipeline are not idle GPU time).  Tables, geometry caches
-    and solver buffers are rebuilt or grown per geometry inside the timed
-    region."""
-    import threading
-    geoms = W.c4_geometries()
+    frequencies each, mixed path.
+
+    --c4-mode many (default): one efie_sweep_many call on an engine context
+    over 32 problem contexts: the 2048 frequency points form one pipeline
+    of batches (geometries of similar order share a batch, padded to its
+    order), each geometry's one-off work (geometry, near lists, series) runs
+    while earlier batches are factored, and its series is freed after its
+    last batch.  Timed: the table uploads (set_problem) of every geometry
+    and the call.
+    --c4-mode per-geometry: one efie_sweep per geometry, largest first, on
+    `--c4-workers` host threads with a context each."""
+    probs = _c4_problems()
     fr = np.linspace(*W.C4["freqs"])
     o = efie.opts(**W.OPTS_C1, precision=efie.PREC_MIXED, near_mode=efie.NEAR_MOMENTS, ir_tol=a.ir_tol,
                   lu_kernel=a.lu_kernel, batch=a.batch)
     o64 = efie.opts(**W.OPTS_C1, precision=efie.PREC_FP64, near_mode=efie.NEAR_DIRECT)
-    probs = []
-    for g in geoms:
-        V, F = efie.make_patch_with_ground_mesh(*g["geom"], *g["cells"])
-        r = efie.build_rwg_interior(V, F)
-        pP, pG = W.c4_feed_points(g["geom"])
-        probs.append((g, V, F, r, efie.pick_port_edge_by_point_and_layer(r, pP, 1),
-                      efie.pick_port_edge_by_point_and_layer(r, pG, -1), 1.5 * r.len.mean()))
+    errs = []
+    extra = {}
+    if a.c4_mode == "many":
+        eng = efie.Context(0)
+        ctxs = [efie.Context(0) for _ in probs]
+        try:
+            # warm-up: kernels loaded, engine streams in use
+            g, V, F, r, pe, ge, sp = min(probs, key=lambda p: p[3].Ne)
+            ctxs[0].set_problem(V, F, r)
+            efie.sweep_many(eng, [dict(ctx=ctxs[0], freqs=fr[:4], patchEdge=pe, groundEdge=ge, V0=1.0, spread=sp,
+                                       Z0=50.0, release=1)], W.MU0, W.EPS0, o)
+            t0 = time.perf_counter()
+            items = []
+            for c, (g, V, F, r, pe, ge, sp) in zip(ctxs, probs):
+                c.set_problem(V, F, r)
+                items.append(dict(ctx=c, freqs=fr, patchEdge=pe, groundEdge=ge, V0=1.0, spread=sp, Z0=50.0,
+                                  release=1))
+            res, st = efie.sweep_many(eng, items, W.MU0, W.EPS0, o, stats=True)
+            t_all = time.perf_counter() - t0
+            for z, _ in res:
+                if not np.all(np.isfinite(z)):
+                    raise RuntimeError("non-finite Zin in the c4 sweep")
+            extra["stats"] = {k: st[k] for k in ("ms_total", "ms_fill", "ms_factor", "ms_solve", "ms_moments",
+                                                 "ir_iters_max", "ir_resid_max", "n_lu_retry", "n_fp64_retry")}
+            per = [{"N": int(p[3].Ne)} for p in probs]
+            eng.close()  # its batch buffers, before the comparison sweeps take theirs
+            if not a.no_fp64:
+                # accuracy against the FP64 path on every fourth geometry (untimed)
+                for i in range(0, len(probs), 4):
+                    g, V, F, r, pe, ge, sp = probs[i]
+                    z64 = ctxs[i].sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, sp, 50.0, o64)[0]
+                    e = float(np.max(np.abs(res[i][0] - z64) / np.abs(z64)))
+                    per[i]["max_rel_err_zin_vs_fp64"] = e
+                    errs.append(e)
+        finally:
+            for c in ctxs:
+                c.close()
+            eng.close()
+        nw = 1
+    else:
+        per, t_all, nw = _bench_c4_per_geometry(a, probs, fr, o, o64, errs)
+    npts = len(fr) * len(probs)
+    line = {"metric": "freq points/s (Z fill+LU), 32 uneven geometries, mixed path",
+            "value": npts / t_all, "unit": "freq points/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+            "ms_per_step": 1e3 * t_all, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "c128 fill, c64 LU + c128 refinement", "data": "synthetic",
+            "config": {"workload": f"c4: {len(probs)} geometries (seed {W.C4['seed']}), N {min(p['N'] for p in per)}-"
+                                   f"{max(p['N'] for p in per)}, {len(fr)} frequencies 1-4 GHz each",
+                       "mode": a.c4_mode, "workers": nw},
+            "max_rel_err_zin_vs_fp64": max(errs) if errs else None, **extra, "per_geometry": per}
+    print(json.dumps(line))
+
+
+def _bench_c4_per_geometry(a, probs, fr, o, o64, errs):
+    """One efie_sweep per geometry, largest first, handed out to
+    `--c4-workers` host threads (one context each).  Tables, geometry
+    caches and solver buffers are rebuilt or grown per geometry inside the
+    timed region."""
+    import threading
     order = sorted(range(len(probs)), key=lambda i: -probs[i][3].Ne)
     nw = max(1, a.c4_workers)
     ctxs = [efie.Context(0) for _ in range(nw)]
-    per, errs = [None] * len(probs), []
+    per = [None] * len(probs)
     try:
         for ctx in ctxs:   # warm-up: kernels loaded, streams created
             g, V, F, r, pe, ge, sp = probs[order[-1]]
@@ -367,7 +433,6 @@ def bench_c4(a):
         if fails:
             raise fails[0]
         if not a.no_fp64:
-            # accuracy against the FP64 path on every fourth geometry (untimed)
             ctx = ctxs[0]
             for g, V, F, r, pe, ge, sp in probs[::4]:
                 ctx.set_problem(V, F, r)
@@ -377,16 +442,7 @@ def bench_c4(a):
     finally:
         for ctx in ctxs:
             ctx.close()
-    npts = len(fr) * len(probs)
-    line = {"metric": "freq points/s (Z fill+LU), 32 uneven geometries, mixed path",
-            "value": npts / t_all, "unit": "freq points/s", "n_gpus": 1, "steps": 1, "warmup": 1,
-            "ms_per_step": 1e3 * t_all, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-            "dtype": "c128 fill, c64 LU + c128 refinement", "data": "synthetic",
-            "config": {"workload": f"c4: {len(probs)} geometries (seed {W.C4['seed']}), N {min(p['N'] for p in per)}-"
-                                   f"{max(p['N'] for p in per)}, {len(fr)} frequencies 1-4 GHz each",
-                       "workers": nw},
-            "max_rel_err_zin_vs_fp64": max(errs) if errs else None, "per_geometry": per}
-    print(json.dumps(line))
+    return per, t_all, nw
 
 
 def main():
@@ -401,6 +457,8 @@ def main():
     ap.add_argument("--lu-kernel", type=int, default=0, help="0 auto (symmetric for the mixed sweep), 1 FP32 MFMA pivoted, 2 bf16x6 pivoted, 3 symmetric FP32")
     ap.add_argument("--ir-tol", type=float, default=1e-10)
     ap.add_argument("--batch", type=int, default=0, help="frequencies per batch (0: the library's choice)")
+    ap.add_argument("--c4-mode", choices=["many", "per-geometry"], default="many",
+                    help="c4: one efie_sweep_many call (default) or one efie_sweep per geometry")
     ap.add_argument("--c4-workers", type=int, default=1,
                     help="host threads (one context each) sweeping c4 geometries side by side")
     ap.add_argument("--nf", type=int, default=0, help="use only the first NF frequencies (profiling runs)")

@@ -241,6 +241,45 @@ efie_status efie_sweep(efie_ctx* ctx, const double* freqs, int Nf, double mu,
                        const efie_opts* opts, double* Zin, double* S11,
                        efie_stats* stats);
 
+/* ---- many sweeps in one call (geometries of uneven size) ----------------- */
+
+/* One sweep of efie_sweep_many: the context holding its problem (mesh and
+ * RWG tables set by efie_set_mesh + efie_set_rwg, on the engine's device),
+ * arguments 13-20 of patch_sweep_cuda (patch_sweep_cuda.cu:17-27) but for
+ * the shared mu, eps, and the two outputs. */
+typedef struct {
+    efie_ctx* problem;
+    const double* freqs;
+    int Nf;
+    int patchEdge, groundEdge;
+    double V0, spread, Z0;
+    double* Zin;   /* [Nf] interleaved complex                               */
+    double* S11;   /* [Nf] interleaved complex                               */
+    int release;   /* 1: the problem's frequency-independent series
+                    * coefficients (near_mode MOMENTS) are freed once its last
+                    * batch is filled (bounded memory over many problems)     */
+} efie_sweep_item;
+
+/* No counterpart in the reference, whose drivers sweep one geometry per
+ * patch_sweep_cuda call and solve each frequency alone (fill, sync,
+ * Zgetrf, sync: patch_sweep_fastquad_cuda.cu:713-772).  The sweeps of
+ * `count` problems, each with the results and error behaviour of
+ * efie_sweep(items[i].problem, ...) under the shared mu, eps and opts, run
+ * as one pipeline on the engine context's streams and buffers:
+ *   - the frequencies of all problems are cut into batches; problems of
+ *     similar order share a batch, their matrices padded to the largest
+ *     order with the identity (the padded system's solution is the
+ *     unpadded one exactly; the factorisation skips the padding);
+ *   - the next batch is filled while one is factored and the one before is
+ *     refined, also across problems;
+ *   - a problem's one-off work (geometry, near lists, series) runs while
+ *     earlier batches are factored.
+ * The engine need not hold a problem itself; one problem context may
+ * appear in several items.  stats (may be NULL) sums the whole call. */
+efie_status efie_sweep_many(efie_ctx* engine, const efie_sweep_item* items, int count,
+                            double mu, double eps_re, double eps_im,
+                            const efie_opts* opts, efie_stats* stats);
+
 /* ---- precomputed quadrature caches (the prepack struct) ------------------ */
 
 /* The quadrature caches of efie_precompute.m:1-97 as packed by

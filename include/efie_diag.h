@@ -39,6 +39,15 @@ efie_status efie_diag_near_pairs(efie_ctx* ctx, const efie_opts* opts, long long
 efie_status efie_diag_quad_points(efie_ctx* ctx, const efie_opts* opts, int set, long long* n, double* out,
                                   long long cap);
 
+/* The batches efie_sweep_many would form, without device work: count
+ * items of order n[i] with nf[i] frequencies, for precision (EFIE_PREC_*)
+ * and opts.batch `batch` (0 = automatic), with free_bytes of free device
+ * memory.  *nseg is the number of segments; the first min(*nseg, cap) go
+ * to out[6k..6k+5] = (batch, item, first frequency, frequencies, first
+ * matrix in the batch, order of the batch), batch after batch. */
+efie_status efie_diag_pack_plan(const int* n, const int* nf, int count, int precision, int batch,
+                                double free_bytes, int* out, int cap, int* nseg);
+
 #ifdef __cplusplus
 }
 #endif

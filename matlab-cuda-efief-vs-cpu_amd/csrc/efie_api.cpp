@@ -14,6 +14,7 @@
 #include <complex>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -289,6 +290,14 @@ struct efie_ctx {
     DBuf dRecs, dFreq, dZ, dZ32, dWp, dWg, dSums, dV, dWl, dOut;
     DBuf dCand, dCnt, dIpiv, dInfo, dSwp, dPerm, dX, dR, dR32, dRn2, dBn2, dDone, dIters, dResid, dB, dP;
     DBuf dFlags, dTickets, dErr;
+    // efie_sweep_many: per buffer the batch's right-hand sides padded to its
+    // order and the order of each matrix's problem; per item its right-hand
+    // side and port weights
+    DBuf dVp, dNElem, dItemV, dItemWl;
+    // and the scratch of its series builds: the pair series (one build at a
+    // time) and the coefficient slots of released problems
+    DBuf dSerQ;
+    std::vector<std::unique_ptr<DBuf>> dSerPool;
     DBuf dSplit;  // bf16 planes of the split-bf16 LU update (lu_kernel BF16X6)
     TrsvWork tw{nullptr, nullptr, nullptr, 0};
     // this GPU's copy of the allocation-free solve's scratch array
@@ -1861,33 +1870,66 @@ int batch_size(const efie_opts& o, size_t zElems, int Nf) {
 // relative residuals (double), the wait-limit flag (int).
 constexpr size_t stat_slot_bytes(int B) { return (size_t)B * (2 * sizeof(int) + sizeof(double)) + 64; }
 
-// Device buffers of the batched solver for n unknowns and B matrices.
-efie_status ensure_solver(efie_ctx* c, const efie_opts& o, int n, int lda, int B, int nbuf = 1) {
-    const size_t zElems = (size_t)n * lda;
+// Sizes of the batched solver's buffers: per buffer of matrices the complex
+// elements (zBuf) and the batch-element-by-row vectors (vec, the largest
+// B n); the residual's partial sums (pBytes), the pivot tree nodes of all
+// matrices, the bf16 planes, the triangular-solve flags and the most
+// matrices in a batch (B).  A batch of B matrices of order n (leading
+// dimension lda) needs solver_dims(n, lda, B); a sequence of batches of
+// different orders the largest of each.
+struct SolverDims {
+    size_t zBuf = 0, vec = 0, pBytes = 0, nodes = 0, split = 0, flags = 0;
+    int B = 0;
+    void grow(const SolverDims& d) {
+        zBuf = std::max(zBuf, d.zBuf);
+        vec = std::max(vec, d.vec);
+        pBytes = std::max(pBytes, d.pBytes);
+        nodes = std::max(nodes, d.nodes);
+        split = std::max(split, d.split);
+        flags = std::max(flags, d.flags);
+        B = std::max(B, d.B);
+    }
+};
+
+SolverDims solver_dims(int n, int lda, int B) {
+    SolverDims d;
+    d.zBuf = (size_t)n * lda * B;
+    d.vec = (size_t)n * B;
+    d.pBytes = residual_sym_bytes(n, B);
+    d.nodes = (size_t)lu_tree_nodes(n) * B;
+    d.split = lu_split_bytes(n) * B;
+    d.flags = (size_t)B * ((n + 63) / 64);
+    d.B = B;
+    return d;
+}
+
+// Device buffers of the batched solver; nbuf buffers of matrices (and of
+// their permutations, first-zero-pivot flags and ||V||^2).
+efie_status ensure_solver_dims(efie_ctx* c, const efie_opts& o, const SolverDims& d, int nbuf) {
     hipStream_t s = c->stream;
-    HIPCHK(c->dZ.ensure(sizeof(double) * 2 * zElems * B * nbuf));
+    const int B = d.B;
+    HIPCHK(c->dZ.ensure(sizeof(double) * 2 * d.zBuf * nbuf));
     if (o.precision == EFIE_PREC_MIXED) {
-        HIPCHK(c->dZ32.ensure(sizeof(float) * 2 * zElems * B * nbuf));
-        HIPCHK(c->dPerm.ensure(sizeof(int) * (size_t)B * n * nbuf));
-        HIPCHK(c->dX.ensure(sizeof(double) * 2 * (size_t)B * n));
-        HIPCHK(c->dR.ensure(sizeof(double) * 2 * (size_t)B * n));
-        HIPCHK(c->dR32.ensure(sizeof(float) * 2 * (size_t)B * n));
-        HIPCHK(c->dP.ensure(residual_sym_bytes(n, B)));
+        HIPCHK(c->dZ32.ensure(sizeof(float) * 2 * d.zBuf * nbuf));
+        HIPCHK(c->dPerm.ensure(sizeof(int) * d.vec * nbuf));
+        HIPCHK(c->dX.ensure(sizeof(double) * 2 * d.vec));
+        HIPCHK(c->dR.ensure(sizeof(double) * 2 * d.vec));
+        HIPCHK(c->dR32.ensure(sizeof(float) * 2 * d.vec));
+        HIPCHK(c->dP.ensure(d.pBytes));
         HIPCHK(c->dRn2.ensure(sizeof(double) * B));
-        HIPCHK(c->dBn2.ensure(sizeof(double) * B));
+        HIPCHK(c->dBn2.ensure(sizeof(double) * B * nbuf));
         HIPCHK(c->dDone.ensure(sizeof(int) * B));
         HIPCHK(c->dIters.ensure(sizeof(int) * B));
         HIPCHK(c->dResid.ensure(sizeof(double) * B));
-        if (o.lu_kernel == EFIE_LU_BF16X6) HIPCHK(c->dSplit.ensure(lu_split_bytes(n) * B));
+        if (o.lu_kernel == EFIE_LU_BF16X6) HIPCHK(c->dSplit.ensure(d.split));
     }
-    const int nodes = lu_tree_nodes(n);
-    HIPCHK(c->dCand.ensure(sizeof(int) * (size_t)B * nodes * lu_cand_stride()));
-    const size_t cntBytes = sizeof(int) * (size_t)B * nodes;
+    HIPCHK(c->dCand.ensure(sizeof(int) * d.nodes * lu_cand_stride()));
+    const size_t cntBytes = sizeof(int) * d.nodes;
     if (c->dCnt.bytes < cntBytes || !c->dCnt.p) {
         HIPCHK(c->dCnt.ensure(cntBytes));
         HIPCHK(launch_fill_int(c->dCnt.as<int>(), 0, (int)(c->dCnt.bytes / sizeof(int)), s));
     }
-    HIPCHK(c->dIpiv.ensure(sizeof(int) * (size_t)B * n));
+    HIPCHK(c->dIpiv.ensure(sizeof(int) * d.vec));
     HIPCHK(c->dInfo.ensure(sizeof(int) * B * nbuf));
     const size_t statBytes = 2 * stat_slot_bytes(B);
     if (c->hStatBytes < statBytes) {
@@ -1900,7 +1942,7 @@ efie_status ensure_solver(efie_ctx* c, const efie_opts& o, int n, int lda, int B
     HIPCHK(c->dSwp.ensure(sizeof(int) * (size_t)B * lu_swap_stride()));
     // single-launch triangular solve: flags hold the epoch of the last
     // launch that finished each block, so they are zeroed only when new
-    const size_t flagBytes = sizeof(int) * (size_t)B * ((n + 63) / 64);
+    const size_t flagBytes = sizeof(int) * d.flags;
     if (c->dFlags.bytes < flagBytes || !c->dFlags.p) {
         HIPCHK(c->dFlags.ensure(flagBytes));
         HIPCHK(launch_fill_int(c->dFlags.as<int>(), 0, (int)(c->dFlags.bytes / sizeof(int)), s));
@@ -1915,6 +1957,11 @@ efie_status ensure_solver(efie_ctx* c, const efie_opts& o, int n, int lda, int B
     c->tw.tickets = c->dTickets.as<int>();
     c->tw.err = c->dErr.as<int>();
     return ok();
+}
+
+// Device buffers of the batched solver for n unknowns and B matrices.
+efie_status ensure_solver(efie_ctx* c, const efie_opts& o, int n, int lda, int B, int nbuf = 1) {
+    return ensure_solver_dims(c, o, solver_dims(n, lda, B), nbuf);
 }
 
 // Device work buffers of the batched solver besides the matrices: pivot
@@ -1963,24 +2010,30 @@ struct BatchBufs {
     char* host;
 };
 
-BatchBufs batch_bufs(efie_ctx* c, const efie_opts& o, int n, int lda, int B, int buf) {
-    const size_t zElems = (size_t)n * lda;
+// Buffer `buf` of a solver sized by d.
+BatchBufs batch_bufs_dims(efie_ctx* c, const efie_opts& o, const SolverDims& d, int buf) {
     BatchBufs bb;
-    bb.Z = c->dZ.as<double>() + 2 * zElems * (size_t)B * buf;
-    bb.Z32 = o.precision == EFIE_PREC_MIXED ? c->dZ32.as<float>() + 2 * zElems * (size_t)B * buf : nullptr;
-    bb.info = c->dInfo.as<int>() + (size_t)B * buf;
-    bb.perm = o.precision == EFIE_PREC_MIXED ? c->dPerm.as<int>() + (size_t)B * n * buf : nullptr;
+    bb.Z = c->dZ.as<double>() + 2 * d.zBuf * buf;
+    bb.Z32 = o.precision == EFIE_PREC_MIXED ? c->dZ32.as<float>() + 2 * d.zBuf * buf : nullptr;
+    bb.info = c->dInfo.as<int>() + (size_t)d.B * buf;
+    bb.perm = o.precision == EFIE_PREC_MIXED ? c->dPerm.as<int>() + d.vec * buf : nullptr;
     bb.w = solver_work(c);
-    bb.host = static_cast<char*>(c->hStat) + stat_slot_bytes(B) * buf;
+    bb.host = static_cast<char*>(c->hStat) + stat_slot_bytes(d.B) * buf;
     return bb;
+}
+
+BatchBufs batch_bufs(efie_ctx* c, const efie_opts& o, int n, int lda, int B, int buf) {
+    return batch_bufs_dims(c, o, solver_dims(n, lda, B), buf);
 }
 
 // LU of the nb matrices of a batch on the context stream (look-ahead
 // panels on the panel stream).  FP64: in place; MIXED: of the FP32 copy
 // (converted here unless the fill wrote it), and the row permutation of
 // its pivots for the refinement.
+// nElem (packed batches, symmetric factorisation): the order of each
+// matrix's problem, the identity padding beyond it skipped.
 efie_status factor_batch(efie_ctx* c, const efie_opts& o, int n, int lda, int nb, const Prof& pf, bool z32Ready,
-                         const BatchBufs& bb) {
+                         const BatchBufs& bb, const int* nElem = nullptr) {
     hipStream_t s = c->stream;
     const size_t zElems = (size_t)n * lda;
     // look-ahead on the panel stream, except when per-kernel profiling
@@ -1989,6 +2042,7 @@ efie_status factor_batch(efie_ctx* c, const efie_opts& o, int n, int lda, int nb
               bb.w.swp, c->prof ? nullptr : c->panelStream, c->evPanel, c->evAhead,
               (o.precision == EFIE_PREC_MIXED && o.lu_kernel == EFIE_LU_BF16X6) ? bb.w.split : nullptr,
               o.precision == EFIE_PREC_MIXED && o.lu_kernel == EFIE_LU_SYM};
+    if (wk.noPivot) wk.nElem = nElem;
     HIPCHK(launch_fill_int(bb.info, 0, nb, s));
     if (o.precision != EFIE_PREC_MIXED) {
         HIPCHK(lu_factor<double>(bb.Z, 2 * zElems, n, lda, nb, wk, s, &pf));
@@ -2138,6 +2192,77 @@ efie_status finish_batch(efie_ctx* c, const efie_opts& o, int nb, const BatchBuf
         }
     }
     return ok();
+}
+
+
+// ------------------------------------------------------- efie_sweep_many --
+// Frequencies f0..f0+nf-1 of item `item` are matrices e0..e0+nf-1 of a
+// packed batch.
+struct PackSeg {
+    int item, f0, nf, e0;
+};
+
+// A batch of efie_sweep_many: nb matrices of order np (leading dimension
+// lda), each holding one problem's matrix in its leading block.
+struct PackBatch {
+    int np = 0, lda = 0, nb = 0;
+    std::vector<PackSeg> seg;
+};
+
+// Problems of order at least kPackRatio times a batch's order may share it
+// (the padding's factorisation steps are skipped; its triangular solves and
+// residual still read order np).
+constexpr double kPackRatio = 0.75;
+// Matrices per packed batch at most.
+constexpr int kPackMaxBatch = 512;
+
+// Cut the frequencies of the items, taken in `order` (largest order
+// first), into batches.  A batch of order np holds about kBatchElems matrix
+// elements (at least 64 matrices, at most kPackMaxBatch; opts.batch when
+// set), two batches within 60 % of the free device memory.  An item with
+// more frequencies left than one batch holds is split evenly; otherwise the
+// batch takes the rest of it and then the next items while their order is
+// at least kPackRatio np.
+std::vector<PackBatch> plan_packs(const std::vector<int>& order, const std::vector<int>& nOf,
+                                  const std::vector<int>& nfOf, const efie_opts& o, size_t freeB) {
+    const double per = o.precision == EFIE_PREC_MIXED ? 24.0 : 16.0;
+    std::vector<PackBatch> out;
+    size_t oi = 0;
+    int done = 0;  // frequencies of order[oi] already placed
+    while (oi < order.size()) {
+        const int g = order[oi];
+        PackBatch b;
+        b.np = nOf[g];
+        b.lda = ((b.np + 1) + 31) & ~31;  // rhs in column np
+        const double zE = (double)b.np * b.lda;
+        double cap = o.batch > 0 ? o.batch : std::min<double>(kPackMaxBatch, std::max(64.0, kBatchElems / zE));
+        cap = std::min(cap, std::max(1.0, std::floor(0.6 * (double)freeB / (2.0 * per * zE))));
+        const int icap = std::max(1, (int)cap);
+        const int rem = nfOf[g] - done;
+        if (rem > icap) {
+            const int parts = (rem + icap - 1) / icap;
+            const int take = o.batch > 0 ? icap : (rem + parts - 1) / parts;
+            b.seg.push_back({g, done, take, 0});
+            b.nb = take;
+            done += take;
+            out.push_back(std::move(b));
+            continue;
+        }
+        while (oi < order.size() && b.nb < icap) {
+            const int h = order[oi];
+            if (b.nb > 0 && (double)nOf[h] < kPackRatio * b.np) break;
+            const int take = std::min(nfOf[h] - done, icap - b.nb);
+            b.seg.push_back({h, done, take, b.nb});
+            b.nb += take;
+            done += take;
+            if (done == nfOf[h]) {
+                ++oi;
+                done = 0;
+            }
+        }
+        out.push_back(std::move(b));
+    }
+    return out;
 }
 
 }  // namespace
@@ -2453,6 +2578,524 @@ efie_status efie_sweep(efie_ctx* c, const double* freqs, int Nf, double mu, doub
                                        (plan.momSelf ? 0 : Nt * ns * ns);
     }
     c->prof = false;
+    return ok();
+}
+
+efie_status efie_sweep_many(efie_ctx* c, const efie_sweep_item* items, int count, double mu, double eps_re,
+                            double eps_im, const efie_opts* opts, efie_stats* stats) {
+    if (!c || count < 0 || (!items && count > 0)) return fail(EFIE_E_ARGS, "efie:args", "null argument");
+    efie_opts o;
+    if (opts) o = *opts;
+    else efie_opts_default(&o);
+    efie_status st = check_opts(&o);
+    if (st) return st;
+    // Z is complex symmetric: the mixed path factors it without pivoting
+    // unless asked otherwise (as efie_sweep)
+    if (o.lu_kernel == EFIE_LU_AUTO && o.precision == EFIE_PREC_MIXED) o.lu_kernel = EFIE_LU_SYM;
+    for (int g = 0; g < count; ++g) {
+        const efie_sweep_item& it = items[g];
+        if (!it.problem) return fail(EFIE_E_ARGS, "efie:args", "item %d: null problem context", g);
+        if (it.problem->device != c->device)
+            return fail(EFIE_E_ARGS, "efie:args", "item %d: problem on device %d, engine on device %d", g,
+                        it.problem->device, c->device);
+        if (it.Nf < 0) return fail(EFIE_E_ARGS, "mex:freqs", "freqs must be double vector.");
+        if (it.Nf > 0 && (!it.freqs || !it.Zin || !it.S11))
+            return fail(EFIE_E_ARGS, "efie:args", "item %d: null argument", g);
+        if (!it.problem->haveRwg) return fail(EFIE_E_STATE, "efie:state", "mesh and RWG tables must be set first");
+        if (it.patchEdge < 1 || it.patchEdge > it.problem->Ne)
+            return fail(EFIE_E_PORT, "mex:port", "patchEdge out of range.");
+        if (it.groundEdge < 1 || it.groundEdge > it.problem->Ne)
+            return fail(EFIE_E_PORT, "mex:port", "groundEdge out of range.");
+    }
+    HIPCHK(hipSetDevice(c->device));
+    efie_stats local;
+    efie_stats* S = stats ? stats : &local;
+    std::memset(S, 0, sizeof *S);
+    const bool mixed = o.precision == EFIE_PREC_MIXED;
+    // the unpivoted symmetric factorisation carries no rhs column
+    const bool rhsCol = !(mixed && o.lu_kernel == EFIE_LU_SYM);
+
+    // wavenumbers; per problem context the largest |k| of its items (items
+    // sharing a context share its series) and whether all its items
+    // release the series
+    const std::complex<double> eps(eps_re, eps_im);
+    std::vector<std::vector<std::complex<double>>> ks(count);
+    std::vector<int> nOf(count), nfOf(count), ctxOf(count);
+    std::vector<efie_ctx*> ctxs;
+    std::vector<double> kmaxCtx;
+    std::vector<char> relCtx;
+    bool lossy = false;
+    for (int g = 0; g < count; ++g) {
+        const efie_sweep_item& it = items[g];
+        nOf[g] = it.problem->Ne;
+        nfOf[g] = it.Nf;
+        const auto pos = std::find(ctxs.begin(), ctxs.end(), it.problem);
+        ctxOf[g] = (int)(pos - ctxs.begin());
+        if (pos == ctxs.end()) {
+            ctxs.push_back(it.problem);
+            kmaxCtx.push_back(0.0);
+            relCtx.push_back(1);
+        }
+        if (!it.release) relCtx[ctxOf[g]] = 0;
+        ks[g].resize(it.Nf);
+        for (int f = 0; f < it.Nf; ++f) {
+            const double omega = 2.0 * kPi * it.freqs[f];
+            ks[g][f] = omega * std::sqrt(std::complex<double>(mu, 0.0) * eps);
+            kmaxCtx[ctxOf[g]] = std::max(kmaxCtx[ctxOf[g]], std::abs(ks[g][f]));
+            lossy = lossy || ks[g][f].imag() != 0.0;
+        }
+    }
+    const int nCtx = (int)ctxs.size();
+    std::vector<int> order;
+    for (int g = 0; g < count; ++g)
+        if (nfOf[g] > 0) order.push_back(g);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return nOf[a] > nOf[b]; });
+
+    // One-off work of every item before the pipeline (all host round trips
+    // happen here): geometry, excitation (its errors come before any
+    // sweep work), the near plan.  Series that are not cached are only
+    // sized here; they are integrated on the fill stream right before the
+    // item's first batch, into the context's own buffer (kept) or a pool
+    // slot (released items), so the pipeline holds no host wait and no
+    // device free.
+    std::vector<size_t> vOff(count), outOff(count);
+    size_t vTot = 0, outTot = 0;
+    for (int g = 0; g < count; ++g) {
+        vOff[g] = vTot;
+        vTot += 2 * (size_t)nOf[g];
+        outOff[g] = outTot;
+        outTot += (size_t)nfOf[g];
+    }
+    HIPCHK(c->dItemV.ensure(sizeof(double) * vTot));
+    HIPCHK(c->dItemWl.ensure(sizeof(double) * vTot));
+    std::vector<NearPlan> plans(count);
+    std::vector<efie_opts> optsOf(count, o);
+    std::vector<double> vn2(count, 0.0);
+    // per context: series coefficients to read, whether to build them (and
+    // with how many terms), built into the context's own buffer or a slot
+    std::vector<const double*> coefC(nCtx, nullptr);
+    std::vector<int> serNT(nCtx, 0), buildP(nCtx, -1), slotOf(nCtx, -1);
+    std::vector<char> planned(nCtx, 0), freeAfter(nCtx, 0);
+    std::vector<NearPlan> planCtx(nCtx);
+    std::vector<double> alphaCtx(nCtx, 0.0);
+    std::vector<size_t> cBytesOf(nCtx, 0);
+    size_t qMax = 0, cMaxSlot = 0;
+    for (int g = 0; g < count; ++g) {
+        const efie_sweep_item& it = items[g];
+        efie_ctx* p = it.problem;
+        const int ci = ctxOf[g];
+        st = prepare(p, &optsOf[g], 0.20);
+        if (st) return st;
+        st = make_rhs(p, it.patchEdge, it.groundEdge, it.V0, it.spread);
+        if (st) return st;
+        if (nfOf[g] == 0) continue;
+        const size_t nv = 2 * (size_t)nOf[g];
+        std::vector<double> hv(nv);
+        HIPCHK(hipMemcpyAsync(c->dItemV.as<double>() + vOff[g], p->dV.p, sizeof(double) * nv,
+                              hipMemcpyDeviceToDevice, p->stream));
+        HIPCHK(hipMemcpyAsync(c->dItemWl.as<double>() + vOff[g], p->dWl.p, sizeof(double) * nv,
+                              hipMemcpyDeviceToDevice, p->stream));
+        HIPCHK(hipMemcpyAsync(hv.data(), p->dV.p, sizeof(double) * nv, hipMemcpyDeviceToHost, p->stream));
+        HIPCHK(hipStreamSynchronize(p->stream));
+        for (double v : hv) vn2[g] += v * v;
+        const efie_opts& og = optsOf[g];
+        if (planned[ci]) {
+            // a second item of the context: the same plan (same options and |k|)
+            plans[g] = planCtx[ci];
+            continue;
+        }
+        planned[ci] = 1;
+        alphaCtx[ci] = og.near_alpha;
+        S->n_near_pairs += p->nNear;
+        if (og.model == EFIE_MODEL_QUADRATURE && og.near_mode == EFIE_NEAR_MOMENTS) {
+            const int P0 = series_terms(p, &og, kmaxCtx[ci]);
+            const int NT = P0 >= 0 ? series_nt(P0) : 0;
+            if (NT > 0) {
+                NearPlan pl;
+                pl.lossy = lossy;
+                pl.series = true;
+                if (p->serP >= P0 && p->serAlpha == og.near_alpha && p->dSerC.p) {
+                    pl.serNT = p->serP + 3;
+                    coefC[ci] = p->dSerC.as<double>();
+                    freeAfter[ci] = relCtx[ci];
+                } else {
+                    pl.serNT = NT;
+                    buildP[ci] = NT - 3;
+                    cBytesOf[ci] = series_coeff_bytes(p->Ne, NT);
+                    qMax = std::max(qMax, sizeof(double) * 8 * (size_t)(NT - 2) * rec_index(0, p->Nt));
+                    if (relCtx[ci]) {
+                        cMaxSlot = std::max(cMaxSlot, cBytesOf[ci]);
+                    } else {
+                        p->serP = -1;  // valid once built
+                        HIPCHK(p->dSerC.ensure(cBytesOf[ci]));
+                        coefC[ci] = p->dSerC.as<double>();
+                    }
+                }
+                S->moment_terms = std::max(S->moment_terms, pl.serNT - 2);
+                plans[g] = planCtx[ci] = pl;
+                continue;
+            }
+        }
+        const int mt = S->moment_terms;
+        st = choose_near(p, &optsOf[g], kmaxCtx[ci], lossy, S, &plans[g]);
+        if (st) return st;
+        planCtx[ci] = plans[g];
+        S->moment_terms = std::max(mt, S->moment_terms);
+    }
+    if (order.empty()) return ok();
+
+    // the pair-series scratch of the builds (one at a time, on the fill stream)
+    HIPCHK(c->dSerQ.ensure(std::max<size_t>(qMax, 16)));
+    size_t freeB = 0, totB = 0;
+    HIPCHK(hipMemGetInfo(&freeB, &totB));
+    // room for a few pool slots before the batches take their share
+    int nRelBuild = 0;
+    for (int ci = 0; ci < nCtx; ++ci) nRelBuild += buildP[ci] >= 0 && relCtx[ci];
+    const size_t poolEst = cMaxSlot * (size_t)std::min(nRelBuild, 8);
+    const std::vector<PackBatch> batches =
+        plan_packs(order, nOf, nfOf, o, freeB > poolEst ? freeB - poolEst : freeB / 2);
+    const int nBatch = (int)batches.size();
+    std::vector<int> firstCtx(nCtx, -1), lastCtx(nCtx, -1);
+    for (int bi = 0; bi < nBatch; ++bi)
+        for (const PackSeg& sg : batches[bi].seg) {
+            const int ci = ctxOf[sg.item];
+            if (firstCtx[ci] < 0) firstCtx[ci] = bi;
+            lastCtx[ci] = bi;
+        }
+    // pool slots: a released context's slot is taken again by a context
+    // first filled after its last batch (the build is queued on the fill
+    // stream behind that batch's fill, which is the last reader)
+    {
+        std::vector<int> slotFreeFrom;  // batch from which the slot is free
+        for (int bi = 0; bi < nBatch; ++bi)
+            for (const PackSeg& sg : batches[bi].seg) {
+                const int ci = ctxOf[sg.item];
+                if (firstCtx[ci] != bi || buildP[ci] < 0 || !relCtx[ci] || slotOf[ci] >= 0) continue;
+                int sl = -1;
+                for (size_t k = 0; k < slotFreeFrom.size() && sl < 0; ++k)
+                    if (slotFreeFrom[k] <= bi) sl = (int)k;
+                if (sl < 0) {
+                    sl = (int)slotFreeFrom.size();
+                    slotFreeFrom.push_back(0);
+                }
+                slotFreeFrom[sl] = lastCtx[ci] + 1;
+                slotOf[ci] = sl;
+            }
+        while (c->dSerPool.size() < slotFreeFrom.size()) c->dSerPool.emplace_back(new DBuf());
+        for (size_t k = 0; k < slotFreeFrom.size(); ++k) HIPCHK(c->dSerPool[k]->ensure(cMaxSlot));
+        for (int ci = 0; ci < nCtx; ++ci)
+            if (slotOf[ci] >= 0) coefC[ci] = c->dSerPool[slotOf[ci]]->as<double>();
+    }
+
+    c->prof = o.profile != 0;
+    c->marks.clear();
+    c->poolUsed = 0;
+    // batches are pipelined over two buffers unless per-kernel profiling
+    // asks for one stream
+    const bool overlap = nBatch > 1 && !c->prof;
+    const int nbuf = overlap ? 2 : 1;
+    hipStream_t s = c->stream;
+    hipStream_t sf = overlap ? c->fillStream : s;
+    hipStream_t si = overlap ? c->irStream : s;
+    SolverDims dims;
+    for (const PackBatch& b : batches) dims.grow(solver_dims(b.np, b.lda, b.nb));
+    st = ensure_solver_dims(c, o, dims, nbuf);
+    if (st) return st;
+    HIPCHK(c->dFreq.ensure(sizeof(Freq) * dims.B * nbuf));
+    HIPCHK(c->dNElem.ensure(sizeof(int) * dims.B * nbuf));
+    HIPCHK(c->dVp.ensure(sizeof(double) * 2 * dims.vec * nbuf));
+    HIPCHK(c->dOut.ensure(sizeof(double) * 4 * outTot));
+    Prof pf = make_prof(c);
+    // the series builds' timing (one event pair each, destroyed on return)
+    struct EvPairs {
+        std::vector<std::pair<hipEvent_t, hipEvent_t>> v;
+        ~EvPairs() {
+            for (auto& e : v) {
+                if (e.first) (void)hipEventDestroy(e.first);
+                if (e.second) (void)hipEventDestroy(e.second);
+            }
+        }
+    } buildEv;
+    std::vector<char> built(nCtx, 0);
+
+    std::vector<Freq> frH[2];
+    std::vector<int> neH[2];
+    std::vector<double> fqH[2];
+    auto zbuf = [&](int buf) { return c->dZ.as<double>() + 2 * dims.zBuf * (size_t)buf; };
+    auto z32buf = [&](int buf) { return mixed ? c->dZ32.as<float>() + 2 * dims.zBuf * (size_t)buf : nullptr; };
+    auto vpbuf = [&](int buf) { return c->dVp.as<double>() + 2 * dims.vec * (size_t)buf; };
+    auto nebuf = [&](int buf) { return c->dNElem.as<int>() + (size_t)dims.B * buf; };
+    auto bufs = [&](int buf) {
+        BatchBufs bb = batch_bufs_dims(c, o, dims, buf);
+        if (mixed) bb.w.Bn2 = c->dBn2.as<double>() + (size_t)dims.B * buf;
+        return bb;
+    };
+    // fill batch bi into buffer bi % nbuf on stream sf: the series of the
+    // contexts it starts, every segment's matrices (FP64 and the complex64
+    // copy), the identity padding, the padded right-hand sides and their
+    // norms; evFill[buf] marks its end
+    auto enqueue_fill = [&](int bi) -> efie_status {
+        const PackBatch& b = batches[bi];
+        const int buf = bi % nbuf;
+        HIPCHK(hipEventRecord(c->evFillStart[buf], sf));
+        for (const PackSeg& sg : b.seg) {
+            const int ci = ctxOf[sg.item];
+            if (buildP[ci] < 0 || built[ci]) continue;
+            built[ci] = 1;
+            efie_ctx* p = ctxs[ci];
+            buildEv.v.push_back({nullptr, nullptr});
+            HIPCHK(hipEventCreate(&buildEv.v.back().first));
+            HIPCHK(hipEventCreate(&buildEv.v.back().second));
+            HIPCHK(hipEventRecord(buildEv.v.back().first, sf));
+            const int P = buildP[ci];
+            HIPCHK(launch_series(p->dReg.as<Pt>(), p->Nq, p->dTri.as<TriGeo>(), p->Nt,
+                                 p->havePack ? p->dNearMat.as<uint8_t>() : nullptr, p->nearEps,
+                                 p->dNearPts.as<Pt>(), p->NpNear, p->Nq2s, p->dAeq.as<double>(),
+                                 optsOf[sg.item].near_alpha, p->dList.as<int2>(), (int)p->nNear,
+                                 p->dSelfPts.as<Pt>(), p->NsSelf, p->dASelf.as<double>(), P, c->dSerQ.as<double>(),
+                                 sf));
+            HIPCHK(launch_series_coeffs(c->dSerQ.as<double>(), P + 3, p->dTri.as<TriGeo>(), p->dSideTri.as<int>(),
+                                        p->dSide.as<Side>(), p->Ne, const_cast<double*>(coefC[ci]), sf));
+            HIPCHK(hipEventRecord(buildEv.v.back().second, sf));
+        }
+        frH[buf].assign(b.nb, Freq());
+        neH[buf].assign(b.nb, 0);
+        fqH[buf].assign(b.nb, 0.0);
+        for (const PackSeg& sg : b.seg)
+            for (int i = 0; i < sg.nf; ++i) {
+                const int f = sg.f0 + i;
+                const double fq = items[sg.item].freqs[f];
+                frH[buf][sg.e0 + i] = make_freq(ks[sg.item][f], 2.0 * kPi * fq, mu, eps);
+                neH[buf][sg.e0 + i] = nOf[sg.item];
+                fqH[buf][sg.e0 + i] = fq;
+            }
+        Freq* dfr = c->dFreq.as<Freq>() + (size_t)dims.B * buf;
+        HIPCHK(hipMemcpyAsync(dfr, frH[buf].data(), sizeof(Freq) * b.nb, hipMemcpyHostToDevice, sf));
+        HIPCHK(hipMemcpyAsync(nebuf(buf), neH[buf].data(), sizeof(int) * b.nb, hipMemcpyHostToDevice, sf));
+        const size_t zE = (size_t)b.np * b.lda;
+        double* Zb = zbuf(buf);
+        float* Z32b = z32buf(buf);
+        double* Vp = vpbuf(buf);
+        for (const PackSeg& sg : b.seg) {
+            const int g = sg.item, n = nOf[g];
+            efie_ctx* p = items[g].problem;
+            const NearPlan& plan = plans[g];
+            double* Zs = Zb + 2 * zE * sg.e0;
+            float* Z32s = Z32b ? Z32b + 2 * zE * sg.e0 : nullptr;
+            bool w32 = Z32s != nullptr;
+            if (plan.series) {
+                pf.begin(EFIE_K_SERIES);
+                // beside a factorisation a capped grid leaves it the CUs
+                HIPCHK(launch_series_fill(coefC[ctxOf[g]], plan.serNT, plan.lossy, n, dfr + sg.e0, sg.nf, Zs,
+                                          (size_t)b.lda, 2 * zE, Z32s, (size_t)b.lda, 2 * zE, sf,
+                                          overlap && bi > 0 ? kFillBlocksShared : 0));
+                pf.end(EFIE_K_SERIES);
+            } else {
+                for (int i = 0; i < sg.nf; ++i) {
+                    bool w = false;
+                    efie_status e = fill_z(p, &optsOf[g], plan, dfr + sg.e0 + i, Zs + 2 * zE * i, (size_t)b.lda,
+                                           Z32s ? Z32s + 2 * zE * i : nullptr, &w, sf);
+                    if (e) return e;
+                    w32 = w32 && w;
+                }
+            }
+            pf.begin(EFIE_K_OTHER);
+            if (Z32s && !w32) HIPCHK(launch_to_f32(Zs, Z32s, zE * sg.nf, sf));
+            HIPCHK(launch_pad_identity(Zs, Z32s, zE, n, b.np, b.lda, sg.nf, sf));
+            HIPCHK(launch_pack_rhs(c->dItemV.as<double>() + vOff[g], n, b.np, sg.nf, Vp + 2 * (size_t)b.np * sg.e0,
+                                   sf));
+            if (mixed) HIPCHK(launch_fill_dbl(c->dBn2.as<double>() + (size_t)dims.B * buf + sg.e0, vn2[g], sg.nf, sf));
+            pf.end(EFIE_K_OTHER);
+        }
+        if (rhsCol) HIPCHK(launch_set_rhs(Vp, Zb, Z32b, b.np, b.lda, zE, b.nb, sf, 2 * (size_t)b.np));
+        HIPCHK(hipEventRecord(c->evFill[buf], sf));
+        return ok();
+    };
+
+    double msFill = 0, msFac = 0, msSolve = 0;
+    HIPCHK(hipEventRecord(c->ev[0], s));
+    if (overlap) {
+        HIPCHK(hipStreamWaitEvent(sf, c->ev[0], 0));
+        HIPCHK(hipStreamWaitEvent(si, c->ev[0], 0));
+    }
+    // mixed precision: systems the FP32 factorisation plus refinement did
+    // not solve, as (item, frequency), redone after the pipeline
+    std::vector<std::pair<int, int>> retry;
+    auto finish = [&](int bj) -> efie_status {
+        const PackBatch& b = batches[bj];
+        const int buf = bj % nbuf;
+        HIPCHK(hipEventSynchronize(c->evIR[buf]));
+        float t1, t2, t3;
+        HIPCHK(hipEventElapsedTime(&t1, c->evFillStart[buf], c->evFill[buf]));
+        HIPCHK(hipEventElapsedTime(&t2, c->evLUStart[buf], c->evLU[buf]));
+        HIPCHK(hipEventElapsedTime(&t3, c->evIRStart[buf], c->evIR[buf]));
+        msFill += t1;
+        msFac += t2;
+        msSolve += t3;
+        std::vector<int> rb(b.nb);
+        int nrb = 0;
+        efie_status e = finish_batch(c, o, b.nb, bufs(buf), dims.B, S, fqH[buf].data(), mixed ? rb.data() : nullptr,
+                                     mixed ? &nrb : nullptr);
+        for (int i = 0; i < nrb; ++i)
+            for (const PackSeg& sg : b.seg)
+                if (rb[i] >= sg.e0 && rb[i] < sg.e0 + sg.nf) retry.push_back({sg.item, sg.f0 + rb[i] - sg.e0});
+        return e;
+    };
+    st = enqueue_fill(0);
+    if (st) return st;
+    for (int bi = 0; bi < nBatch; ++bi) {
+        const PackBatch& b = batches[bi];
+        const int buf = bi % nbuf;
+        const BatchBufs bb = bufs(buf);
+        HIPCHK(hipStreamWaitEvent(s, c->evFill[buf], 0));
+        HIPCHK(hipEventRecord(c->evLUStart[buf], s));
+        st = factor_batch(c, o, b.np, b.lda, b.nb, pf, true, bb, nebuf(buf));
+        if (st) return st;
+        HIPCHK(hipEventRecord(c->evLU[buf], s));
+        HIPCHK(hipStreamWaitEvent(si, c->evLU[buf], 0));
+        HIPCHK(hipEventRecord(c->evIRStart[buf], si));
+        const double* X = nullptr;
+        size_t xStride = 0, xinc = 0;
+        // beside the next batch's factorisation the triangular solves run
+        // with a few blocks per matrix
+        c->tw.rows = overlap && bi + 1 < nBatch ? kTrsvRowsShared : 0;
+        st = refine_batch(c, o, b.np, b.lda, b.nb, vpbuf(buf), 2 * (size_t)b.np, o.model != EFIE_MODEL_CENTROID, pf,
+                          bb, si, &X, &xStride, &xinc);
+        c->tw.rows = 0;
+        if (st) return st;
+        pf.begin(EFIE_K_OTHER);
+        for (const PackSeg& sg : b.seg) {
+            const efie_sweep_item& it = items[sg.item];
+            HIPCHK(launch_epilogue(X + xStride * sg.e0, xStride, xinc, c->dItemWl.as<double>() + vOff[sg.item],
+                                   nOf[sg.item], it.patchEdge - 1, it.groundEdge - 1,
+                                   o.model == EFIE_MODEL_CENTROID, it.V0, it.Z0, sg.nf,
+                                   c->dOut.as<double>() + 4 * (outOff[sg.item] + sg.f0), si));
+        }
+        pf.end(EFIE_K_OTHER, (int)b.seg.size());
+        st = stage_stats(o, b.nb, dims.B, bb, si);
+        if (st) return st;
+        HIPCHK(hipEventRecord(c->evIR[buf], si));
+        if (overlap) {
+            if (bi >= 1) {
+                st = finish(bi - 1);
+                if (st) return st;
+            }
+            if (bi + 1 < nBatch) {
+                if (bi >= 1) HIPCHK(hipStreamWaitEvent(sf, c->evIR[(bi + 1) % nbuf], 0));
+                st = enqueue_fill(bi + 1);
+                if (st) return st;
+            }
+        } else {
+            st = finish(bi);
+            if (st) return st;
+            if (bi + 1 < nBatch) {
+                st = enqueue_fill(bi + 1);
+                if (st) return st;
+            }
+        }
+    }
+    if (overlap) {
+        st = finish(nBatch - 1);
+        if (st) return st;
+        HIPCHK(hipEventRecord(c->ev[1], si));
+        HIPCHK(hipStreamWaitEvent(s, c->ev[1], 0));
+    }
+    HIPCHK(hipEventRecord(c->ev[5], s));
+    std::vector<double> out(4 * outTot);
+    HIPCHK(hipMemcpyAsync(out.data(), c->dOut.p, sizeof(double) * out.size(), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipStreamSynchronize(sf));
+    for (int g = 0; g < count; ++g)
+        for (int f = 0; f < nfOf[g]; ++f) {
+            const double* r = out.data() + 4 * (outOff[g] + f);
+            items[g].Zin[2 * f] = r[0];
+            items[g].Zin[2 * f + 1] = r[1];
+            items[g].S11[2 * f] = r[2];
+            items[g].S11[2 * f + 1] = r[3];
+        }
+    float tot = 0;
+    HIPCHK(hipEventElapsedTime(&tot, c->ev[0], c->ev[5]));
+    S->ms_total = tot;
+    S->ms_fill = msFill;
+    S->ms_factor = msFac;
+    S->ms_solve = msSolve;
+    for (auto& e : buildEv.v) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e.first, e.second));
+        S->ms_moments += ms;
+        g_momBuilds.fetch_add(1);
+    }
+    // series built into a context's own buffer stay cached there; released
+    // ones (in the pool, or cached before) are gone
+    for (int ci = 0; ci < nCtx; ++ci) {
+        efie_ctx* p = ctxs[ci];
+        if (buildP[ci] >= 0 && !relCtx[ci]) {
+            p->serP = buildP[ci];
+            p->serAlpha = alphaCtx[ci];
+        }
+        if (freeAfter[ci]) {
+            p->dSerC.release();
+            p->serP = -1;
+        }
+    }
+    c->dSerQ.release();
+    for (auto& d : c->dSerPool) d->release();
+    c->prof = false;
+    if (!retry.empty()) {
+        // the frequencies the mixed solve could not finish, again in a call
+        // of their own: after the unpivoted symmetric factorisation with the
+        // pivoted FP32 LU (whose own failures go on to FP64), after a
+        // pivoted FP32 LU with the FP64 LU
+        std::vector<std::vector<int>> fi(count);
+        for (const auto& r : retry) fi[r.first].push_back(r.second);
+        std::vector<std::vector<double>> fr2(count), zin2(count), s112(count);
+        std::vector<efie_sweep_item> sub;
+        for (int g = 0; g < count; ++g) {
+            if (fi[g].empty()) continue;
+            const size_t k = fi[g].size();
+            for (int f : fi[g]) fr2[g].push_back(items[g].freqs[f]);
+            zin2[g].resize(2 * k);
+            s112[g].resize(2 * k);
+            efie_sweep_item it = items[g];
+            it.freqs = fr2[g].data();
+            it.Nf = (int)k;
+            it.Zin = zin2[g].data();
+            it.S11 = s112[g].data();
+            sub.push_back(it);
+        }
+        efie_opts o2 = o;
+        const bool toPivoted = o.lu_kernel == EFIE_LU_SYM;
+        if (toPivoted) o2.lu_kernel = EFIE_LU_FP32;
+        else o2.precision = EFIE_PREC_FP64;
+        efie_stats S2;
+        st = efie_sweep_many(c, sub.data(), (int)sub.size(), mu, eps_re, eps_im, &o2, &S2);
+        if (st) return st;
+        for (int g = 0; g < count; ++g)
+            for (size_t i = 0; i < fi[g].size(); ++i) {
+                const int f = fi[g][i];
+                items[g].Zin[2 * f] = zin2[g][2 * i];
+                items[g].Zin[2 * f + 1] = zin2[g][2 * i + 1];
+                items[g].S11[2 * f] = s112[g][2 * i];
+                items[g].S11[2 * f + 1] = s112[g][2 * i + 1];
+            }
+        S->ms_total += S2.ms_total;
+        S->ms_fill += S2.ms_fill;
+        S->ms_factor += S2.ms_factor;
+        S->ms_solve += S2.ms_solve;
+        S->ms_moments += S2.ms_moments;
+        S->ir_iters_max = std::max(S->ir_iters_max, S2.ir_iters_max);
+        S->ir_resid_max = std::max(S->ir_resid_max, S2.ir_resid_max);
+        for (int q = 0; q < EFIE_NKCLASS; ++q) {
+            S->ms_kernel[q] += S2.ms_kernel[q];
+            S->n_kernel[q] += S2.n_kernel[q];
+        }
+        if (toPivoted) {
+            S->n_lu_retry = (int)retry.size();
+            S->n_fp64_retry = S2.n_fp64_retry;
+        } else {
+            S->n_fp64_retry = (int)retry.size();
+        }
+    }
     return ok();
 }
 
@@ -2806,6 +3449,39 @@ efie_status efie_diag_quad_points(efie_ctx* c, const efie_opts* opts, int set, l
         HIPCHK(hipMemcpyAsync(out, src[set]->p, sizeof(double) * (size_t)k, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
     }
+    return ok();
+}
+
+efie_status efie_diag_pack_plan(const int* n, const int* nf, int count, int precision, int batch,
+                                double free_bytes, int* out, int cap, int* nseg) {
+    if (count < 0 || (count > 0 && (!n || !nf)) || !nseg || cap < 0 || (cap > 0 && !out))
+        return fail(EFIE_E_ARGS, "efie:args", "bad arguments");
+    std::vector<int> nOf(n, n + count), nfOf(nf, nf + count), order;
+    for (int g = 0; g < count; ++g) {
+        if (nOf[g] < 1 || nfOf[g] < 0) return fail(EFIE_E_ARGS, "efie:args", "item %d: bad order or count", g);
+        if (nfOf[g] > 0) order.push_back(g);
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return nOf[a] > nOf[b]; });
+    efie_opts o;
+    efie_opts_default(&o);
+    o.precision = precision;
+    o.batch = batch;
+    const std::vector<PackBatch> bs = plan_packs(order, nOf, nfOf, o, (size_t)std::max(0.0, free_bytes));
+    int k = 0;
+    for (size_t b = 0; b < bs.size(); ++b)
+        for (const PackSeg& sg : bs[b].seg) {
+            if (k < cap) {
+                int* r = out + 6 * (size_t)k;
+                r[0] = (int)b;
+                r[1] = sg.item;
+                r[2] = sg.f0;
+                r[3] = sg.nf;
+                r[4] = sg.e0;
+                r[5] = bs[b].np;
+            }
+            ++k;
+        }
+    *nseg = k;
     return ok();
 }
 

@@ -203,6 +203,10 @@ struct LuWork {
     void* splitPlanes;
     // complex-symmetric A: factor without pivoting, U = D L^T (lu_factor_sym)
     bool noPivot;
+    // lu_factor_sym of a packed batch: matrix b holds a problem of order
+    // nElem[b] in its leading block and the identity beyond (null: order n);
+    // steps wholly inside the padding are skipped
+    const int* nElem = nullptr;
 };
 size_t lu_split_bytes(int n);
 // ints per batch element of LuWork::swaps
@@ -259,9 +263,16 @@ hipError_t launch_transpose_square(double* A, int n, size_t ld, hipStream_t s);
 // ------------------------------------------------ mixed precision -------
 hipError_t launch_to_f32(const double* A64, float* A32, size_t complexElems, hipStream_t s);
 // Column n of the nb matrices (complex elements zElems apart, row-major,
-// leading dimension lda) := V; also in Z32 when non-null.
+// leading dimension lda) := V (matrix b's at V + b*vStride doubles); also in
+// Z32 when non-null.
 hipError_t launch_set_rhs(const double* V, double* Z, float* Z32, int n, int lda, size_t zElems, int nb,
-                          hipStream_t s);
+                          hipStream_t s, size_t vStride = 0);
+// Packed batches (matrices of problems of order n <= np at order np):
+// rows and columns n..np-1 of nf matrices (zElems complex elements apart)
+// := the identity's, in Z and, when non-null, Z32; and nf copies of the
+// right-hand side V[n] padded with zeros to np (out[f] at out + 2 np f).
+hipError_t launch_pad_identity(double* Z, float* Z32, size_t zElems, int n, int np, int lda, int nf, hipStream_t s);
+hipError_t launch_pack_rhs(const double* V, int n, int np, int nf, double* out, hipStream_t s);
 hipError_t launch_col_to_vec(const float* A32, size_t strideA, int n, int lda, double* X, int batch,
                              hipStream_t s);
 // R = V - A64 X per batch element (V[b] at V + b*vStride), rn2[b] += ||R||^2

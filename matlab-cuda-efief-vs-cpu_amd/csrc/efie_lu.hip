@@ -482,12 +482,15 @@ __global__ __launch_bounds__(256) void k_laswp_rest(T* A, size_t strideA, int ld
 // (coalesced on both sides).  It also leaves the final U in place for the
 // triangular solves.
 template <typename T>
-__global__ __launch_bounds__(256) void k_sym_u12(T* A, size_t strideA, int lda, int k0, int k1, int c0, int c1) {
+__global__ __launch_bounds__(256) void k_sym_u12(T* A, size_t strideA, int lda, int k0, int k1, int c0, int c1,
+                                                 const int* __restrict__ nElem) {
     __shared__ cpx<T> t[64][33];
     __shared__ cpx<T> dg[32];
     const int b = blockIdx.z;
     T* Ab = A + (size_t)b * strideA;
     const int jb = c0 + blockIdx.x * 64, kb = k0 + blockIdx.y * 32;
+    // padded matrix: L rows and D entries beyond its order give zero U
+    if (nElem && (jb >= nElem[b] || kb >= nElem[b])) return;
     const int tid = threadIdx.x;
     // read L[j][k], j in [jb, jb+64), k in [kb, kb+32): 32 consecutive k per row
     for (int e = tid; e < 64 * 32; e += 256) {
@@ -521,11 +524,15 @@ __global__ __launch_bounds__(256) void k_sym_u12(T* A, size_t strideA, int lda, 
 // elimination, and near resonance small pivots amplify that mismatch into a
 // backward error of order 1e-4 in FP32.
 template <typename T, bool SYM = false>
-__global__ __launch_bounds__(64) void k_panel_diag(T* A, size_t strideA, int lda, int j0, int w, int* info) {
+__global__ __launch_bounds__(64) void k_panel_diag(T* A, size_t strideA, int lda, int j0, int w, int* info,
+                                                   const int* __restrict__ nElem) {
     // one wave, lane k holds row k of the block; row c (SYM: column c) is
     // broadcast through LDS
     __shared__ cpx<T> prow[NB];
     const int b = blockIdx.x;
+    // padded matrix: a diagonal block in the identity padding is its own
+    // factorisation
+    if (nElem && j0 >= nElem[b]) return;
     T* Ab = A + (size_t)b * strideA;
     const int lane = threadIdx.x;
     cpx<T> a[NB];
@@ -576,9 +583,11 @@ __global__ __launch_bounds__(64) void k_panel_diag(T* A, size_t strideA, int lda
 // k_panel_cols solves columns [clo, chi) of U12 = L11^-1 A12 (rows d..d+w).
 template <typename T>
 __global__ __launch_bounds__(256) void k_panel_rows(T* A, size_t strideA, int lda, int d, int w, int rlo,
-                                                    int rhi) {
+                                                    int rhi, const int* __restrict__ nElem) {
     __shared__ cpx<T> a11[NB * NB];
     const int b = blockIdx.y;
+    // padded matrix: rows in the padding are zero left of the diagonal
+    if (nElem && rlo + (int)blockIdx.x * 256 >= nElem[b]) return;
     T* Ab = A + (size_t)b * strideA;
     const int tid = threadIdx.x;
     for (int e = tid; e < w * w; e += 256) {
@@ -719,7 +728,8 @@ struct gemm_vec<double> {
 template <typename T, int BM, int BN, bool M3 = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 3 : 1))) void k_gemm(
     T* A, size_t strideA, int lda, int r0, int r1, int c0,
-                                              int c1, int k0, int K, int gx, int gy, int nblk, int skipUpper) {
+                                              int c1, int k0, int K, int gx, int gy, int nblk, int skipUpper,
+                                              const int* __restrict__ nElem) {
     typedef typename mf<T>::v4 v4;
     typedef typename gemm_vec<T>::v vec;
     constexpr int CB = (int)sizeof(cpx<T>);   // bytes per complex value
@@ -759,6 +769,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
     // symmetric factorisation: tiles strictly above the diagonal are never
     // read again (U comes from L, and the panels read the lower triangle)
     if (skipUpper && rb + BM <= cb) return;
+    // a padded batch matrix (order nElem[b] in the leading block, identity
+    // beyond): a tile with its rows or its columns in the padding multiplies
+    // zeros and keeps its values
+    if (nElem) {
+        const int ne = nElem[b];
+        if (rb >= ne || cb >= ne) return;
+    }
     const int tid = threadIdx.x;
     const int lane = tid & 63, wv = tid >> 6;
     const int wr = (wv >> 1) * (BM / 2), wc = (wv & 1) * (BN / 2);
@@ -1376,11 +1393,11 @@ int lu_swap_stride() { return kSwapSlots * lu_swap_stride_c; }
 
 template <typename T, int BM, int BN>
 static void gemm_launch(T* A, size_t strideA, int lda, int batch, int r0, int r1, int c0, int c1, int k0, int K,
-                        hipStream_t s, int skipUpper = 0) {
+                        hipStream_t s, int skipUpper = 0, const int* nElem = nullptr) {
     const int gx = (c1 - c0 + BN - 1) / BN, gy = (r1 - r0 + BM - 1) / BM;
     const int nblk = gx * gy * batch;
     hipLaunchKernelGGL((k_gemm<T, BM, BN, true>), dim3(nblk), dim3(256), 0, s, A, strideA, lda, r0, r1, c0, c1, k0,
-                       K, gx, gy, nblk, skipUpper);
+                       K, gx, gy, nblk, skipUpper, nElem);
 }
 
 // 64 x 64 tiles: three blocks per CU (FP32: 84 VGPRs + 48 accumulation
@@ -1394,9 +1411,9 @@ static void gemm_launch(T* A, size_t strideA, int lda, int batch, int r0, int r1
 // Returns the number of kernels launched (0 for an empty update).
 template <typename T>
 static int gemm(T* A, size_t strideA, int lda, int batch, int r0, int r1, int c0, int c1, int k0, int K,
-                hipStream_t s, int skipUpper = 0) {
+                hipStream_t s, int skipUpper = 0, const int* nElem = nullptr) {
     if (r1 <= r0 || c1 <= c0 || K <= 0) return 0;
-    gemm_launch<T, 64, 64>(A, strideA, lda, batch, r0, r1, c0, c1, k0, K, s, skipUpper);
+    gemm_launch<T, 64, 64>(A, strideA, lda, batch, r0, r1, c0, c1, k0, K, s, skipUpper, nElem);
     return 1;
 }
 
@@ -1459,11 +1476,12 @@ static void sym_block(T* A, size_t strideA, int n, int lda, int batch, const LuW
     if (c1 - c0 <= NB) {
         const int w = c1 - c0;
         pf->begin(7);
-        hipLaunchKernelGGL((k_panel_diag<T, true>), dim3(batch), dim3(64), 0, sp, A, strideA, lda, c0, w, wk.info);
+        hipLaunchKernelGGL((k_panel_diag<T, true>), dim3(batch), dim3(64), 0, sp, A, strideA, lda, c0, w, wk.info,
+                           wk.nElem);
         int np = 1;
         if (c1 < n) {
             hipLaunchKernelGGL(k_panel_rows<T>, dim3((n - c1 + 255) / 256, batch), dim3(256), 0, sp, A, strideA, lda,
-                               c0, w, c1, n);
+                               c0, w, c1, n, wk.nElem);
             ++np;
         }
         pf->end(7, np);
@@ -1473,10 +1491,10 @@ static void sym_block(T* A, size_t strideA, int n, int lda, int batch, const LuW
     sym_block<T>(A, strideA, n, lda, batch, wk, c0, mid, sp, pf);
     pf->begin(7);
     hipLaunchKernelGGL(k_sym_u12<T>, dim3((c1 - mid + 63) / 64, (mid - c0 + 31) / 32, batch), dim3(256), 0, sp, A,
-                       strideA, lda, c0, mid, mid, c1);
+                       strideA, lda, c0, mid, mid, c1, wk.nElem);
     pf->end(7);
     pf->begin(8);
-    pf->end(8, gemm<T>(A, strideA, lda, batch, mid, n, mid, c1, c0, mid - c0, sp, 1));
+    pf->end(8, gemm<T>(A, strideA, lda, batch, mid, n, mid, c1, c0, mid - c0, sp, 1, wk.nElem));
     sym_block<T>(A, strideA, n, lda, batch, wk, mid, c1, sp, pf);
 }
 
@@ -1509,13 +1527,13 @@ static void panel_phase(T* A, size_t strideA, int n, int lda, int batch, const L
         int np = 1;
         if (wk.noPivot)
             hipLaunchKernelGGL((k_panel_diag<T, true>), dim3(batch), dim3(64), 0, sp, A, strideA, lda, c0, w,
-                               wk.info);
+                               wk.info, wk.nElem);
         else
             hipLaunchKernelGGL((k_panel_diag<T, false>), dim3(batch), dim3(64), 0, sp, A, strideA, lda, c0, w,
-                               wk.info);
+                               wk.info, nullptr);
         if (c0 + w < n) {
             hipLaunchKernelGGL(k_panel_rows<T>, dim3((n - c0 - w + 255) / 256, batch), dim3(256), 0, sp, A, strideA,
-                               lda, c0, w, c0 + w, n);
+                               lda, c0, w, c0 + w, n, wk.noPivot ? wk.nElem : nullptr);
             ++np;
         }
         if (!wk.noPivot) {
@@ -1524,12 +1542,13 @@ static void panel_phase(T* A, size_t strideA, int n, int lda, int batch, const L
             // the block-local U12 from L21 as well (the upper triangle is
             // never read in the symmetric factorisation)
             hipLaunchKernelGGL(k_sym_u12<T>, dim3((jend - c0 - w + 63) / 64, (w + 31) / 32, batch), dim3(256), 0,
-                               sp, A, strideA, lda, c0, c0 + w, c0 + w, jend);
+                               sp, A, strideA, lda, c0, c0 + w, c0 + w, jend, wk.nElem);
             ++np;
         }
         pf->end(7, np);
         pf->begin(8);
-        pf->end(8, gemm<T>(A, strideA, lda, batch, c0 + w, n, c0 + w, jend, c0, w, sp, wk.noPivot ? 1 : 0));
+        pf->end(8, gemm<T>(A, strideA, lda, batch, c0 + w, n, c0 + w, jend, c0, w, sp, wk.noPivot ? 1 : 0,
+                            wk.noPivot ? wk.nElem : nullptr));
     }
 }
 
@@ -1576,10 +1595,10 @@ static hipError_t lu_factor_sym(T* A, size_t strideA, int n, int lda, int batch,
         if (jend >= n) break;
         pf->begin(7);
         hipLaunchKernelGGL(k_sym_u12<T>, dim3((n - jend + 63) / 64, (wb + 31) / 32, batch), dim3(256), 0, s, A,
-                           strideA, lda, j0, jend, jend, n);
+                           strideA, lda, j0, jend, jend, n, wk.nElem);
         pf->end(7);
         pf->begin(8);
-        pf->end(8, gemm<T>(A, strideA, lda, batch, jend, n, jend, jnext, j0, wb, s, 1));
+        pf->end(8, gemm<T>(A, strideA, lda, batch, jend, n, jend, jnext, j0, wb, s, 1, wk.nElem));
         if (sp != s) {
             chk(hipEventRecord(wk.evAhead, s));
             chk(hipStreamWaitEvent(sp, wk.evAhead, 0));
@@ -1587,7 +1606,7 @@ static hipError_t lu_factor_sym(T* A, size_t strideA, int n, int lda, int batch,
         panel_phase<T>(A, strideA, n, lda, batch, wk, jend, jnext, 0, sp, pf);
         if (sp != s) chk(hipEventRecord(wk.evPanel, sp));
         pf->begin(8);
-        pf->end(8, gemm<T>(A, strideA, lda, batch, jend, n, jnext, n, j0, wb, s, 1));
+        pf->end(8, gemm<T>(A, strideA, lda, batch, jend, n, jnext, n, j0, wb, s, 1, wk.nElem));
     }
     return chk.e != hipSuccess ? chk.e : hipGetLastError();
 }

@@ -313,20 +313,64 @@ hipError_t launch_fill_dbl(double* p, double v, int n, hipStream_t s) {
 
 
 // Column n of every batch matrix (FP64, and the complex64 copy when Z32 is
-// non-null) set to the right-hand side V.
-__global__ __launch_bounds__(256) void k_set_rhs(const double* __restrict__ V, double* __restrict__ Z,
+// non-null) set to the right-hand side V (matrix b's at V + b*vStride).
+__global__ __launch_bounds__(256) void k_set_rhs(const double* __restrict__ V, size_t vStride, double* __restrict__ Z,
                                                  float* __restrict__ Z32, int n, int lda, size_t zElems) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const size_t e = (size_t)blockIdx.y * zElems + (size_t)i * lda + n;
-    const double2 v = reinterpret_cast<const double2*>(V)[i];
+    const double2 v = reinterpret_cast<const double2*>(V + blockIdx.y * vStride)[i];
     reinterpret_cast<double2*>(Z)[e] = v;
     if (Z32) reinterpret_cast<float2*>(Z32)[e] = make_float2((float)v.x, (float)v.y);
 }
 
 hipError_t launch_set_rhs(const double* V, double* Z, float* Z32, int n, int lda, size_t zElems, int nb,
-                          hipStream_t s) {
-    hipLaunchKernelGGL(k_set_rhs, dim3((n + 255) / 256, nb), dim3(256), 0, s, V, Z, Z32, n, lda, zElems);
+                          hipStream_t s, size_t vStride) {
+    hipLaunchKernelGGL(k_set_rhs, dim3((n + 255) / 256, nb), dim3(256), 0, s, V, vStride, Z, Z32, n, lda, zElems);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------ packed batches ---
+// A batch may hold the matrices of several problems at one order np: a
+// problem of order n < np fills the leading n x n block and the rest is the
+// identity.  k_pad_identity writes that rest: row r < n gets zeros in
+// columns n..np-1, row r >= n zeros in columns 0..np-1 but a one at r (FP64
+// matrix, and its complex64 copy when Z32 is set).  Every factorisation and
+// solve treats the padding exactly: its entries meet the problem's only in
+// products with an exact zero, so the leading block factors as the n x n
+// matrix would and the padding's part of every solution is exactly zero.
+// Grid (np, nf): block (r, f) writes row r of matrix f.
+__global__ __launch_bounds__(256) void k_pad_identity(double* __restrict__ Z, float* __restrict__ Z32,
+                                                      size_t zElems, int n, int np, int lda) {
+    const int r = blockIdx.x;
+    const size_t row = (size_t)blockIdx.y * zElems + (size_t)r * lda;
+    for (int j = (r < n ? n : 0) + threadIdx.x; j < np; j += 256) {
+        const double d = j == r ? 1.0 : 0.0;
+        reinterpret_cast<double2*>(Z)[row + j] = make_double2(d, 0.0);
+        if (Z32) reinterpret_cast<float2*>(Z32)[row + j] = make_float2((float)d, 0.0f);
+    }
+}
+
+hipError_t launch_pad_identity(double* Z, float* Z32, size_t zElems, int n, int np, int lda, int nf, hipStream_t s) {
+    if (n >= np || nf <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pad_identity, dim3(np, nf), dim3(256), 0, s, Z, Z32, zElems, n, np, lda);
+    return hipGetLastError();
+}
+
+// out[f][i] = V[i] for i < n and 0 for n <= i < np, f < nf (interleaved
+// complex, out[f] at out + 2 np f): a problem's right-hand side padded to
+// the batch order.
+__global__ __launch_bounds__(256) void k_pack_rhs(const double* __restrict__ V, int n, int np,
+                                                  double* __restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= np) return;
+    const double2 v = i < n ? reinterpret_cast<const double2*>(V)[i] : make_double2(0.0, 0.0);
+    reinterpret_cast<double2*>(out)[(size_t)blockIdx.y * np + i] = v;
+}
+
+hipError_t launch_pack_rhs(const double* V, int n, int np, int nf, double* out, hipStream_t s) {
+    if (nf <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pack_rhs, dim3((np + 255) / 256, nf), dim3(256), 0, s, V, n, np, out);
     return hipGetLastError();
 }
 

@@ -81,6 +81,14 @@ class Stats(C.Structure):
         return d
 
 
+class SweepItem(C.Structure):
+    """efie_sweep_item of efie.h."""
+    _fields_ = [("problem", C.c_void_p), ("freqs", C.POINTER(C.c_double)), ("Nf", C.c_int),
+                ("patchEdge", C.c_int), ("groundEdge", C.c_int), ("V0", C.c_double), ("spread", C.c_double),
+                ("Z0", C.c_double), ("Zin", C.POINTER(C.c_double)), ("S11", C.POINTER(C.c_double)),
+                ("release", C.c_int)]
+
+
 class SurCfg(C.Structure):
     """efie_sur_cfg (include/efie_surrogate.h)."""
     _fields_ = [
@@ -106,7 +114,7 @@ EXPORTS = [
     "efie_opts_default", "efie_opts_preset", "efie_last_error", "efie_last_error_id",
     "efie_mesh_counts", "efie_make_mesh", "efie_rwg_count", "efie_build_rwg", "efie_pick_port",
     "efie_create", "efie_destroy", "efie_set_mesh", "efie_set_rwg", "efie_assemble_zv",
-    "efie_sweep", "efie_zgesv", "efie_zgesv_noalloc", "efie_diag_copy", "efie_diag_cache_counts", "efie_diag_near_pairs", "efie_diag_quad_points", "efie_rwg_count_device", "efie_build_rwg_device",
+    "efie_sweep", "efie_sweep_many", "efie_zgesv", "efie_zgesv_noalloc", "efie_diag_copy", "efie_diag_pack_plan", "efie_diag_cache_counts", "efie_diag_near_pairs", "efie_diag_quad_points", "efie_rwg_count_device", "efie_build_rwg_device",
     "efie_pick_port_device", "efie_set_prepack", "efie_set_rwg_tables", "efie_assemble_z",
     "efie_sur_cfg_default", "efie_sur_blob_floats", "efie_sur_train", "efie_sur_get", "efie_sur_set",
     "efie_sur_set_exact", "efie_mlp_params", "efie_mlp_forward", "efie_mlp_loss_grad",
@@ -147,9 +155,12 @@ def lib():
         L.efie_assemble_z.argtypes = [C.c_void_p] + [C.c_double] * 6 + [C.POINTER(Opts), _dp]
         L.efie_sweep.argtypes = [C.c_void_p, _dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
                                  C.c_double, C.c_double, C.c_double, C.POINTER(Opts), _dp, _dp, C.POINTER(Stats)]
+        L.efie_sweep_many.argtypes = [C.c_void_p, C.POINTER(SweepItem), C.c_int, C.c_double, C.c_double,
+                                      C.c_double, C.POINTER(Opts), C.POINTER(Stats)]
         L.efie_zgesv.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, C.POINTER(Stats)]
         L.efie_zgesv_noalloc.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, C.POINTER(Stats)]
         L.efie_diag_copy.argtypes = [C.c_void_p, C.c_size_t, C.c_int, _dp]
+        L.efie_diag_pack_plan.argtypes = [_ip, _ip, C.c_int, C.c_int, C.c_int, C.c_double, _ip, C.c_int, _ip]
         L.efie_diag_cache_counts.argtypes = [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
         L.efie_diag_near_pairs.argtypes = [C.c_void_p, C.POINTER(Opts), C.POINTER(C.c_longlong), _ip, C.c_longlong]
         L.efie_diag_quad_points.argtypes = [C.c_void_p, C.POINTER(Opts), C.c_int, C.POINTER(C.c_longlong), _dp,
@@ -510,6 +521,47 @@ class Context:
 
 
 # ------------------------------------------------- reference MEX mirrors --
+
+def sweep_many(engine: "Context", items, mu, eps, o: Optional[Opts] = None, stats: bool = False):
+    """Several sweeps in one pipelined call (efie_sweep_many): items are
+    dicts with keys ctx (the Context holding the problem), freqs, patchEdge,
+    groundEdge, V0, spread, Z0 and optionally release.  Returns a list of
+    (Zin, S11) per item, and the call's stats when asked."""
+    eps = complex(eps)
+    n = len(items)
+    arr = (SweepItem * max(n, 1))()
+    keep = []
+    outs = []
+    for i, it in enumerate(items):
+        fr = np.ascontiguousarray(np.ravel(it["freqs"]), dtype=np.float64)
+        Nf = fr.shape[0]
+        zin = np.zeros(2 * Nf)
+        s11 = np.zeros(2 * Nf)
+        keep += [fr, zin, s11]
+        outs.append((zin, s11))
+        arr[i] = SweepItem(it["ctx"]._c, _d(fr), Nf, int(it["patchEdge"]), int(it["groundEdge"]), float(it["V0"]),
+                           float(it["spread"]), float(it["Z0"]), _d(zin), _d(s11), int(it.get("release", 0)))
+    st = Stats()
+    o = o or opts()
+    _check(lib().efie_sweep_many(engine._c, arr, n, mu, eps.real, eps.imag, C.byref(o), C.byref(st)))
+    res = [(z.view(np.complex128).copy(), s.view(np.complex128).copy()) for z, s in outs]
+    return (res, st.as_dict()) if stats else res
+
+
+def pack_plan(n, nf, precision=PREC_MIXED, batch=0, free_bytes=250e9):
+    """Batches efie_sweep_many forms for items of orders n with nf
+    frequencies (efie_diag_pack_plan; no device work): an int array of rows
+    (batch, item, first frequency, frequencies, first matrix, batch order)."""
+    n = np.ascontiguousarray(n, dtype=np.int32)
+    nf = np.ascontiguousarray(nf, dtype=np.int32)
+    k = C.c_int()
+    _check(lib().efie_diag_pack_plan(_i(n), _i(nf), n.shape[0], precision, batch, free_bytes, None, 0,
+                                     C.byref(k)))
+    out = np.zeros(6 * max(k.value, 1), dtype=np.int32)
+    _check(lib().efie_diag_pack_plan(_i(n), _i(nf), n.shape[0], precision, batch, free_bytes, _i(out), k.value,
+                                     C.byref(k)))
+    return out[:6 * k.value].reshape(-1, 6)
+
 
 def patch_sweep_cuda(V, F, edge, plusTri, minusTri, plusSign, minusSign, rp, rm, Ap, Am, len_,
                      freqs, mu, eps, patchEdge, groundEdge, V0, spreadRadius, Z0,

@@ -141,40 +141,50 @@ def c4(efie_mod):
 
 def test_c4_full_size(efie_mod, ctx, c4):
     """All 32 geometries x 64 frequencies through the mixed path (FP32
-    LDL^T + FP64 refinement to 1e-10): every frequency either refines to
-    max(ir_tol, 1e-11) or is solved again (counted); every fourth geometry's
-    Zin against the FP64 path -- within 1e-9 of the plain FP64 path (direct
-    quadrature) at the workload's refinement tolerance 1e-10 (a residual of
-    1e-10 relative shows as up to a few 1e-10 in Zin near resonance), and
-    within 1e-10 of the FP64 path with the same fill (series) at tolerance
-    1e-11, which isolates the FP32 factorisation's share."""
+    LDL^T + FP64 refinement to 1e-10) in one efie_sweep_many call (the
+    geometries packed into shared batches by order, each series freed after
+    its last batch): every frequency either refines to max(ir_tol, 1e-11)
+    or is solved again (counted); every fourth geometry's Zin against the
+    FP64 path -- within 1e-9 of the plain FP64 path (direct quadrature) at
+    the workload's refinement tolerance 1e-10 (a residual of 1e-10 relative
+    shows as up to a few 1e-10 in Zin near resonance), and within 1e-10 of
+    the FP64 path with the same fill (series) at tolerance 1e-11, which
+    isolates the FP32 factorisation's share."""
     fr = np.linspace(*W.C4["freqs"])
     o = dict(W.OPTS_C1, precision=efie_mod.PREC_MIXED, near_mode=efie_mod.NEAR_MOMENTS)
     ns = [p[3].Ne for p in c4]
     assert len(c4) == 32 and min(ns) >= 1000 and max(ns) <= 8000
-    retried = 0
-    zm = {}
-    for i, (g, V, F, r, pe, ge, spread) in enumerate(c4):
-        ctx.set_problem(V, F, r)
-        z, _, st = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0, efie_mod.opts(**o, ir_tol=1e-10),
-                             stats=True)
-        assert np.all(np.isfinite(z))
+    ctxs = [efie_mod.Context(0) for _ in c4]
+    eng = efie_mod.Context(0)
+    try:
+        items = []
+        for c, (g, V, F, r, pe, ge, spread) in zip(ctxs, c4):
+            c.set_problem(V, F, r)
+            items.append(dict(ctx=c, freqs=fr, patchEdge=pe, groundEdge=ge, V0=1.0, spread=spread, Z0=50.0,
+                              release=1))
+        res, st = efie_mod.sweep_many(eng, items, W.MU0, W.EPS0, efie_mod.opts(**o, ir_tol=1e-10), stats=True)
+        eng.close()  # its batch buffers, before the comparison sweeps take theirs
+        assert all(np.all(np.isfinite(z)) for z, _ in res)
         assert st["ir_resid_max"] <= 1e-10
-        retried += st["n_lu_retry"]
-        zm[i] = z
-    print(f"c4: N {min(ns)}-{max(ns)}, {len(c4) * len(fr)} points, {retried} re-solved with pivoting", flush=True)
-    worst, worst11 = 0.0, 0.0
-    for i in range(0, len(c4), 4):
-        g, V, F, r, pe, ge, spread = c4[i]
-        ctx.set_problem(V, F, r)
-        z64 = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0, efie_mod.opts(**W.OPTS_C1))[0]
-        z64m = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0,
-                         efie_mod.opts(**W.OPTS_C1, near_mode=efie_mod.NEAR_MOMENTS))[0]
-        z11 = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0, efie_mod.opts(**o, ir_tol=1e-11))[0]
-        e10 = float(np.max(np.abs(zm[i] - z64) / np.abs(z64)))
-        e11 = float(np.max(np.abs(z11 - z64m) / np.abs(z64m)))
-        print(f"  geometry {i}: N={r.Ne} |dZin|/|Zin| {e10:.2e} (ir_tol 1e-10), {e11:.2e} (1e-11)", flush=True)
-        worst, worst11 = max(worst, e10), max(worst11, e11)
+        print(f"c4: N {min(ns)}-{max(ns)}, {len(c4) * len(fr)} points in {st['ms_total']:.0f} ms, "
+              f"{st['n_lu_retry']} re-solved with pivoting", flush=True)
+        worst, worst11 = 0.0, 0.0
+        for i in range(0, len(c4), 4):
+            g, V, F, r, pe, ge, spread = c4[i]
+            cx = ctxs[i]
+            z64 = cx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0, efie_mod.opts(**W.OPTS_C1))[0]
+            z64m = cx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0,
+                            efie_mod.opts(**W.OPTS_C1, near_mode=efie_mod.NEAR_MOMENTS))[0]
+            z11 = cx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0, efie_mod.opts(**o, ir_tol=1e-11))[0]
+            e10 = float(np.max(np.abs(res[i][0] - z64) / np.abs(z64)))
+            e11 = float(np.max(np.abs(z11 - z64m) / np.abs(z64m)))
+            print(f"  geometry {i}: N={r.Ne} |dZin|/|Zin| {e10:.2e} (ir_tol 1e-10), {e11:.2e} (1e-11)", flush=True)
+            worst, worst11 = max(worst, e10), max(worst11, e11)
+            cx.close()
+    finally:
+        eng.close()
+        for c in ctxs:
+            c.close()
     assert worst <= 1e-9
     assert worst11 <= ZIN_TOL
 
