@@ -305,12 +305,15 @@ def cpu_baseline(V, F, Ne, fr, threads, gpu_rows=None):
                       f"one frequency point = {fill_s + t_lu:.0f} s"}
 
 
-def _c4_problems():
+def _c4_problems(n_max=0):
+    """The c4 geometries (with n_max > 0 only those of fewer than n_max RWG)."""
     out = []
     for g in W.c4_geometries():
         V, F = efie.make_patch_with_ground_mesh(*g["geom"], *g["cells"])
         r = efie.build_rwg_interior(V, F)
         pP, pG = W.c4_feed_points(g["geom"])
+        if n_max > 0 and r.Ne >= n_max:
+            continue
         out.append((g, V, F, r, efie.pick_port_edge_by_point_and_layer(r, pP, 1),
                     efie.pick_port_edge_by_point_and_layer(r, pG, -1), 1.5 * r.len.mean()))
     return out
@@ -323,13 +326,14 @@ def bench_c4(a):
     --c4-mode many (default): one efie_sweep_many call on an engine context
     over 32 problem contexts: the 2048 frequency points form one pipeline
     of batches (geometries of similar order share a batch, padded to its
-    order), each geometry's one-off work (geometry, near lists, series) runs
-    while earlier batches are factored, and its series is freed after its
-    last batch.  Timed: the table uploads (set_problem) of every geometry
-    and the call.
+    order), each geometry's series is integrated on the fill stream between
+    batches and freed after its last batch.  One untimed pass warms the
+    engine (its buffers stay allocated); the timed pass uses new problem
+    contexts: the table uploads (set_problem), every geometry's one-off work
+    (geometry, near lists, excitation, series) and the call.
     --c4-mode per-geometry: one efie_sweep per geometry, largest first, on
     `--c4-workers` host threads with a context each."""
-    probs = _c4_problems()
+    probs = _c4_problems(a.c4_max_n)
     fr = np.linspace(*W.C4["freqs"])
     o = efie.opts(**W.OPTS_C1, precision=efie.PREC_MIXED, near_mode=efie.NEAR_MOMENTS, ir_tol=a.ir_tol,
                   lu_kernel=a.lu_kernel, batch=a.batch)
@@ -338,13 +342,25 @@ def bench_c4(a):
     extra = {}
     if a.c4_mode == "many":
         eng = efie.Context(0)
-        ctxs = [efie.Context(0) for _ in probs]
+        ctxs = []
         try:
-            # warm-up: kernels loaded, engine streams in use
-            g, V, F, r, pe, ge, sp = min(probs, key=lambda p: p[3].Ne)
-            ctxs[0].set_problem(V, F, r)
-            efie.sweep_many(eng, [dict(ctx=ctxs[0], freqs=fr[:4], patchEdge=pe, groundEdge=ge, V0=1.0, spread=sp,
-                                       Z0=50.0, release=1)], W.MU0, W.EPS0, o)
+            # warm-up: the whole workload once on other problem contexts
+            # (kernels loaded, the engine's batch buffers allocated: freshly
+            # recycled device memory can take seconds to hand out)
+            warm = [efie.Context(0) for _ in probs]
+            try:
+                witems = []
+                for c, (g, V, F, r, pe, ge, sp) in zip(warm, probs):
+                    c.set_problem(V, F, r)
+                    witems.append(dict(ctx=c, freqs=fr, patchEdge=pe, groundEdge=ge, V0=1.0, spread=sp, Z0=50.0,
+                                       release=1))
+                efie.sweep_many(eng, witems, W.MU0, W.EPS0, o)
+            finally:
+                for c in warm:
+                    c.close()
+            # timed: new problem contexts (geometry, near lists and series
+            # built again), the engine's buffers reused
+            ctxs = [efie.Context(0) for _ in probs]
             t0 = time.perf_counter()
             items = []
             for c, (g, V, F, r, pe, ge, sp) in zip(ctxs, probs):
@@ -376,7 +392,9 @@ def bench_c4(a):
     else:
         per, t_all, nw = _bench_c4_per_geometry(a, probs, fr, o, o64, errs)
     npts = len(fr) * len(probs)
-    line = {"metric": "freq points/s (Z fill+LU), 32 uneven geometries, mixed path",
+    n3 = float(sum(float(p[3].Ne) ** 3 for p in probs)) * len(fr)
+    extra["n3_per_s"] = n3 / t_all
+    line = {"metric": f"freq points/s (Z fill+LU), {len(probs)} uneven geometries, mixed path",
             "value": npts / t_all, "unit": "freq points/s", "n_gpus": 1, "steps": 1, "warmup": 1,
             "ms_per_step": 1e3 * t_all, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "c128 fill, c64 LU + c128 refinement", "data": "synthetic",
@@ -459,6 +477,8 @@ def main():
     ap.add_argument("--batch", type=int, default=0, help="frequencies per batch (0: the library's choice)")
     ap.add_argument("--c4-mode", choices=["many", "per-geometry"], default="many",
                     help="c4: one efie_sweep_many call (default) or one efie_sweep per geometry")
+    ap.add_argument("--c4-max-n", type=int, default=0,
+                    help="c4: only the geometries of fewer than this many RWG (0: all 32)")
     ap.add_argument("--c4-workers", type=int, default=1,
                     help="host threads (one context each) sweeping c4 geometries side by side")
     ap.add_argument("--nf", type=int, default=0, help="use only the first NF frequencies (profiling runs)")

@@ -2744,14 +2744,22 @@ efie_status efie_sweep_many(efie_ctx* c, const efie_sweep_item* items, int count
     }
     if (order.empty()) return ok();
 
-    // the pair-series scratch of the builds (one at a time, on the fill stream)
+    // the pair-series scratch of the builds (one at a time, on the fill
+    // stream); like the engine's other buffers it is kept for later calls
     HIPCHK(c->dSerQ.ensure(std::max<size_t>(qMax, 16)));
     size_t freeB = 0, totB = 0;
     HIPCHK(hipMemGetInfo(&freeB, &totB));
+    // the engine's matrix buffers are reused (grown if need be), so they
+    // count as free for the batch plan
+    freeB += c->dZ.bytes + c->dZ32.bytes;
     // room for a few pool slots before the batches take their share
     int nRelBuild = 0;
     for (int ci = 0; ci < nCtx; ++ci) nRelBuild += buildP[ci] >= 0 && relCtx[ci];
-    const size_t poolEst = cMaxSlot * (size_t)std::min(nRelBuild, 8);
+    size_t poolEst = 0;  // beyond the slots the engine holds from earlier calls
+    for (int k = 0; k < std::min(nRelBuild, 8); ++k) {
+        const size_t have = k < (int)c->dSerPool.size() ? c->dSerPool[k]->bytes : 0;
+        poolEst += cMaxSlot > have ? cMaxSlot - have : 0;
+    }
     const std::vector<PackBatch> batches =
         plan_packs(order, nOf, nfOf, o, freeB > poolEst ? freeB - poolEst : freeB / 2);
     const int nBatch = (int)batches.size();
@@ -3038,8 +3046,6 @@ efie_status efie_sweep_many(efie_ctx* c, const efie_sweep_item* items, int count
             p->serP = -1;
         }
     }
-    c->dSerQ.release();
-    for (auto& d : c->dSerPool) d->release();
     c->prof = false;
     if (!retry.empty()) {
         // the frequencies the mixed solve could not finish, again in a call
