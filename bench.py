@@ -463,6 +463,66 @@ def _bench_c4_per_geometry(a, probs, fr, o, o64, errs):
     return per, t_all, nw
 
 
+TRACE_CLASSES = ["k_gemm", "k_panel", "k_sym_u12", "k_symv", "k_trsv", "k_series_fill", "k_series_near",
+                 "k_series_coeffs", "k_series_regular", "k_series_self", "k_tslu", "k_pad_identity"]
+
+
+def trace_states(path, window_gap_ms=0.0, range_ms=None):
+    """What the GPU runs over the time of a rocprofv3 kernel trace (CSV):
+    the union of kernel intervals split by the kernel classes active at
+    each moment.  Any moment with a trailing-update GEMM running counts as
+    "k_gemm (+ others)"; the rest is listed by the set of classes running
+    (what is exposed next to no matrix-core work) or "idle".  With
+    window_gap_ms > 0 only the kernels after the last gap of at least that
+    many ms with no kernel running (the last timed call) are counted; with
+    range_ms = (a, b) only the kernels starting between a and b ms after
+    the first one.  Also the summed duration and count of each class."""
+    import csv
+    import collections
+    ev = []
+    with open(path) as fh:
+        for r in csv.DictReader(fh):
+            name = r["Kernel_Name"].split("(")[0].replace("void ", "").replace("efie::", "")
+            ev.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), name))
+    ev.sort()
+    if range_ms:
+        t0 = ev[0][0]
+        ev = [e for e in ev if range_ms[0] * 1e6 <= e[0] - t0 < range_ms[1] * 1e6]
+    if window_gap_ms > 0:
+        end, start = ev[0][1], 0
+        for i, (a0, b0, _) in enumerate(ev[1:], 1):
+            if a0 - end >= window_gap_ms * 1e6:
+                start = i
+            end = max(end, b0)
+        ev = ev[start:]
+
+    def cls(n):
+        return next((k for k in TRACE_CLASSES if n.startswith(k)), "other")
+    chg = sorted([(a0, 1, cls(n)) for a0, _, n in ev] + [(b0, -1, cls(n)) for _, b0, n in ev])
+    act = collections.Counter()
+    states = collections.Counter()
+    last = chg[0][0]
+    for t, d, c in chg:
+        if t > last:
+            if act["k_gemm"] > 0:
+                k = "k_gemm (+ others)"
+            elif sum(act.values()) == 0:
+                k = "idle"
+            else:
+                k = " + ".join(sorted(x for x, v in act.items() if v > 0))
+            states[k] += t - last
+        act[c] += d
+        last = t
+    summed = collections.defaultdict(lambda: [0, 0])
+    for a0, b0, n in ev:
+        summed[cls(n)][0] += b0 - a0
+        summed[cls(n)][1] += 1
+    return {"span_ms": (max(e[1] for e in ev) - ev[0][0]) / 1e6, "kernels": len(ev),
+            "states_ms": {k: v / 1e6 for k, v in states.most_common()},
+            "summed_ms": {k: v[0] / 1e6 for k, v in sorted(summed.items(), key=lambda x: -x[1][0])},
+            "launches": {k: v[1] for k, v in summed.items()}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -487,7 +547,16 @@ def main():
                          "profiled sweep (for kernel traces: rocprofv3)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run the plain copy kernel on this many MiB (counter calibration)")
+    ap.add_argument("--trace-states", default=None, metavar="CSV",
+                    help="only analyse a rocprofv3 kernel trace (CSV): GPU time by the kernel classes running")
+    ap.add_argument("--trace-window-gap-ms", type=float, default=0.0,
+                    help="--trace-states: only the kernels after the last idle gap of this many ms")
+    ap.add_argument("--trace-range-ms", type=float, nargs=2, default=None, metavar=("A", "B"),
+                    help="--trace-states: only the kernels starting A to B ms after the first one")
     a = ap.parse_args()
+    if a.trace_states:
+        print(json.dumps(trace_states(a.trace_states, a.trace_window_gap_ms, a.trace_range_ms), indent=1))
+        return
     if a.copy_probe_mb:
         with efie.Context(0) as ctx:
             nbytes = a.copy_probe_mb << 20
