@@ -187,19 +187,32 @@ def roofline(st, n, nf, fast, sym=False):
                 "frac": ach / PEAK_HBM_GBS, "traffic": None,
                 "work": f"16*N^2 B of Z written per launch, {cnt} launches, {ms/cnt*1e3:.1f} us/launch avg"}
     if name == "series":
-        return dict(kernel="k_series_fill", **series_rate(st, n, nf))
+        return dict(kernel="k_series_fill", **series_rate(st, n, nf, tri=fast and sym))
     return {"kernel": name, "bound": "mfma", "achieved": None, "peak": None, "unit": None, "frac": None,
             "traffic": None, "work": f"{ms:.1f} ms over {cnt} launches"}
 
 
-def series_rate(st, n, nf, Nt=None, z32=True):
+def tri_fill_bytes(n):
+    """Bytes per matrix the series fill writes for the symmetric mixed solve:
+    the FP64 upper triangle with the diagonal and the lower halves of the
+    diagonal 64 x 64 blocks (16 B each), the FP32 lower triangle with the
+    diagonal and the upper halves of the diagonal 16 x 16 tiles (8 B)."""
+    def halves(b):
+        full, rem = divmod(n, b)
+        return full * b * (b - 1) // 2 + rem * (rem - 1) // 2
+    tri = n * (n + 1) // 2
+    return 16.0 * (tri + halves(64)) + 8.0 * (tri + halves(16))
+
+
+def series_rate(st, n, nf, Nt=None, z32=True, tri=False):
     """HBM rate of the series fill (k_series_fill): per launch the batch's Z
-    matrices are written once (16 B FP64 + 8 B FP32 per entry) and the
-    per-entry series coefficients read once (P + 4 doubles per entry of the
-    upper-triangular 16 x 16 tiles)."""
+    matrices are written once (16 B FP64 + 8 B FP32 per entry; for the
+    symmetric mixed solve only the triangles it reads, tri_fill_bytes) and
+    the per-entry series coefficients read once (P + 4 doubles per entry of
+    the upper-triangular 16 x 16 tiles)."""
     ks = st["kernels"]["series"]
     P = st["moment_terms"] - 1
-    per_f = (24.0 if z32 else 16.0) * n * n
+    per_f = tri_fill_bytes(n) if tri else (24.0 if z32 else 16.0) * n * n
     T = (n + 15) // 16
     q = 8.0 * (P + 4) * 256 * (T * (T + 1) // 2)
     launches = ks["n"]
@@ -213,7 +226,7 @@ def series_rate(st, n, nf, Nt=None, z32=True):
                     f"per frequency"}
 
 
-def fill_rates(st, n, Nt, nf):
+def fill_rates(st, n, Nt, nf, tri=False):
     """Achieved rates of the two fill kernels of a profiled sweep against
     MI355X peaks: the regular records (FP64 VALU work: Green evaluations x
     FLOP_PER_GREEN) and the combine (HBM: the records read once plus the
@@ -222,7 +235,7 @@ def fill_rates(st, n, Nt, nf):
     ks = st["kernels"]
     out = {}
     if "series" in ks and ks["series"]["ms"] > 0:
-        out["k_series_fill"] = series_rate(st, n, nf, Nt)
+        out["k_series_fill"] = series_rate(st, n, nf, Nt, tri=tri)
     if "regular" in ks and ks["regular"]["ms"] > 0:
         work = st["n_green_regular"] * FLOP_PER_GREEN * nf
         ach = work / (ks["regular"]["ms"] * 1e-3) / 1e12
@@ -523,6 +536,55 @@ def trace_states(path, window_gap_ms=0.0, range_ms=None):
             "launches": {k: v[1] for k, v in summed.items()}}
 
 
+TRAFFIC_KEYS = {"k_gemm<float": "k_gemm", "k_gemm<double": "k_gemm_f64", "k_series_fill": "k_series_fill",
+                "k_symv_finish": "k_symv_finish", "k_symv_strips": "k_symv_strips",
+                "k_trsv_fused<float, false>": "k_trsv_fused_upper", "k_trsv_fused<float, true>": "k_trsv_fused_lower"}
+
+
+def pmc_summary(root, legs):
+    """HBM bytes per launch from rocprofv3 --pmc runs (one counter per
+    run): root/pmc_copy_{FETCH_SIZE,WRITE_SIZE} hold the plain copy kernel
+    (bench.py --copy-probe-mb) that calibrates the counters, and
+    root/pmc_<leg>_{FETCH_SIZE,WRITE_SIZE} the profiled sweeps.  Each
+    counter is scaled by the bytes the copy kernel moves over what it
+    reads for it; returns the rows of profiles/traffic.json."""
+    import csv
+    import glob
+
+    def load(d):
+        out = {}
+        for f in glob.glob(os.path.join(root, d, "*counter_collection.csv")):
+            with open(f) as fh:
+                for r in csv.DictReader(fh):
+                    k = r["Kernel_Name"].replace("void ", "").replace("efie::", "")
+                    out.setdefault(k, []).append(float(r["Counter_Value"]))
+        return out
+    cf, cw = load("pmc_copy_FETCH_SIZE"), load("pmc_copy_WRITE_SIZE")
+    copy = next(k for k in cf if k.startswith("k_copy16"))
+    nbytes = None
+    for f in glob.glob(os.path.join(root, "pmc_copy_FETCH_SIZE*.log")):
+        for line in open(f):
+            if line.startswith("{") and "copy_bytes" in line:
+                nbytes = json.loads(line)["copy_bytes"]
+    nbytes = nbytes or (1 << 30)
+    fs = nbytes / 1024.0 / (sum(cf[copy]) / len(cf[copy]))
+    ws = nbytes / 1024.0 / (sum(cw[copy]) / len(cw[copy]))
+    rows = {"_calibration": {"copy_kernel": "efie::k_copy16", "copy_bytes_read": nbytes, "copy_bytes_written": nbytes,
+                             "fetch_scale": fs, "write_scale": ws}}
+    for leg in legs:
+        lf, lw = load(f"pmc_{leg}_FETCH_SIZE"), load(f"pmc_{leg}_WRITE_SIZE")
+        for k in lf:
+            key = next((v for p_, v in TRAFFIC_KEYS.items() if k.startswith(p_)), None)
+            if key is None or k not in lw:
+                continue
+            f_avg = sum(lf[k]) / len(lf[k])
+            w_avg = sum(lw[k]) / len(lw[k])
+            rows[key] = {"kernel": "efie::" + k.split("(")[0], "leg": leg, "launches": len(lf[k]),
+                         "fetch_kib_avg": f_avg, "write_kib_avg": w_avg,
+                         "bytes_per_launch": 1024.0 * (f_avg * fs + w_avg * ws)}
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -553,7 +615,20 @@ def main():
                     help="--trace-states: only the kernels after the last idle gap of this many ms")
     ap.add_argument("--trace-range-ms", type=float, nargs=2, default=None, metavar=("A", "B"),
                     help="--trace-states: only the kernels starting A to B ms after the first one")
+    ap.add_argument("--pmc-summary", default=None, metavar="DIR",
+                    help="only turn the rocprofv3 --pmc runs under DIR (pmc_copy_*, pmc_<leg>_*) into the rows of "
+                         "profiles/traffic.json, merged into that file")
+    ap.add_argument("--pmc-legs", nargs="+", default=["fast"], help="--pmc-summary: the profiled legs")
     a = ap.parse_args()
+    if a.pmc_summary:
+        path = os.path.join(ROOT, "profiles", "traffic.json")
+        old = json.load(open(path)) if os.path.exists(path) else {}
+        new = pmc_summary(a.pmc_summary, a.pmc_legs)
+        old.update(new)
+        with open(path, "w") as fh:
+            json.dump(old, fh, indent=1)
+        print(json.dumps(new, indent=1))
+        return
     if a.trace_states:
         print(json.dumps(trace_states(a.trace_states, a.trace_window_gap_ms, a.trace_range_ms), indent=1))
         return
@@ -686,7 +761,7 @@ def main():
         "fast_vs_fp64_same_fill": (value / res["fp64_series_fill"]["value"]) if "fp64_series_fill" in res else None,
         **res,
         "profile": st_p["kernels"],
-        "fill": fill_rates(st_p, n, Nt, len(fr)),
+        "fill": fill_rates(st_p, n, Nt, len(fr), tri=a.lu_kernel in (0, 3)),
     }
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(V, F, n, fr, threads, gpu_rows)

@@ -2382,6 +2382,10 @@ efie_status efie_sweep(efie_ctx* c, const double* freqs, int Nf, double mu, doub
     std::vector<Freq> fr[2] = {std::vector<Freq>(B), std::vector<Freq>(B)};
     bool have32[2] = {false, false};
     const bool mixed = o.precision == EFIE_PREC_MIXED;
+    // the symmetric mixed solve reads the FP64 matrix's upper triangle (and
+    // diagonal blocks) and the FP32 copy's lower one: the series fill writes
+    // only those
+    const bool triFill = mixed && o.lu_kernel == EFIE_LU_SYM && o.model != EFIE_MODEL_CENTROID;
     auto zbuf = [&](int buf) { return c->dZ.as<double>() + 2 * zElems * (size_t)B * buf; };
     auto z32buf = [&](int buf) { return mixed ? c->dZ32.as<float>() + 2 * zElems * (size_t)B * buf : nullptr; };
     // fill batch bi into buffer bi % nbuf on stream sf (Z, its complex64
@@ -2403,7 +2407,7 @@ efie_status efie_sweep(efie_ctx* c, const double* freqs, int Nf, double mu, doub
             // capped grid leaves that the CUs
             HIPCHK(launch_series_fill(c->dSerC.as<double>(), plan.serNT, plan.lossy, n, dfr, nb, Zb, (size_t)lda,
                                       2 * zElems, Z32b, (size_t)lda, 2 * zElems, sf,
-                                      overlap && bi > 0 ? kFillBlocksShared : 0));
+                                      overlap && bi > 0 ? kFillBlocksShared : 0, triFill));
             pff.end(EFIE_K_SERIES);
         } else {
             for (int i = 0; i < nb; ++i) {
@@ -2612,8 +2616,10 @@ efie_status efie_sweep_many(efie_ctx* c, const efie_sweep_item* items, int count
     efie_stats* S = stats ? stats : &local;
     std::memset(S, 0, sizeof *S);
     const bool mixed = o.precision == EFIE_PREC_MIXED;
-    // the unpivoted symmetric factorisation carries no rhs column
+    // the unpivoted symmetric factorisation carries no rhs column, and of
+    // the series fill it needs only the triangles it reads
     const bool rhsCol = !(mixed && o.lu_kernel == EFIE_LU_SYM);
+    const bool triFill = !rhsCol && o.model != EFIE_MODEL_CENTROID;
 
     // wavenumbers; per problem context the largest |k| of its items (items
     // sharing a context share its series) and whether all its items
@@ -2896,7 +2902,7 @@ efie_status efie_sweep_many(efie_ctx* c, const efie_sweep_item* items, int count
                 // beside a factorisation a capped grid leaves it the CUs
                 HIPCHK(launch_series_fill(coefC[ctxOf[g]], plan.serNT, plan.lossy, n, dfr + sg.e0, sg.nf, Zs,
                                           (size_t)b.lda, 2 * zE, Z32s, (size_t)b.lda, 2 * zE, sf,
-                                          overlap && bi > 0 ? kFillBlocksShared : 0));
+                                          overlap && bi > 0 ? kFillBlocksShared : 0, triFill));
                 pf.end(EFIE_K_SERIES);
             } else {
                 for (int i = 0; i < sg.nf; ++i) {

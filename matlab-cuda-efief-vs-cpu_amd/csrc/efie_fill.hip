@@ -1685,7 +1685,7 @@ template <int NT, bool LOSSY>
 __global__ __launch_bounds__(256) void k_series_fill(const double* __restrict__ C, int Ne,
                                                      const Freq* __restrict__ fr, int nf, double* __restrict__ Z,
                                                      size_t ld, size_t zStride, float* __restrict__ Z32,
-                                                     size_t ld32, size_t z32Stride, int nblk) {
+                                                     size_t ld32, size_t z32Stride, int nblk, int tri) {
     __shared__ double2 tile[16][17];
     // tiles in a grid-stride loop: the grid may be smaller than the tile
     // count (a fill that runs beside a factorisation leaves it the CUs)
@@ -1710,6 +1710,10 @@ __global__ __launch_bounds__(256) void k_series_fill(const double* __restrict__ 
         const bool upper = m < Ne && n < Ne && (I != J || r <= cI);
         const int mm = 16 * J + r, nn = 16 * I + cI;
         const bool lower = mm < Ne && nn < Ne && (I != J || r > cI);
+        // tri: only what the symmetric solve reads -- FP64 above the
+        // diagonal and the whole diagonal 64 x 64 blocks (the residual), FP32
+        // below it and on the diagonal (the factorisation)
+        const bool up32 = !tri || I == J, lo64 = !tri || (I >> 2) == (J >> 2);
         for (int f = 0; f < nf; ++f) {
             const Freq F = fr[f];
             // p = sum_j z^j D[j], z = -jk = ki - j kr
@@ -1752,12 +1756,13 @@ __global__ __launch_bounds__(256) void k_series_fill(const double* __restrict__ 
             tile[r][cI] = val;
             if (upper) {
                 *reinterpret_cast<double2*>(Zf + 2 * ((size_t)m * ld + n)) = val;
-                if (Z32f) *reinterpret_cast<float2*>(Z32f + 2 * ((size_t)m * ld32 + n)) = make_float2((float)val.x, (float)val.y);
+                if (Z32f && up32)
+                    *reinterpret_cast<float2*>(Z32f + 2 * ((size_t)m * ld32 + n)) = make_float2((float)val.x, (float)val.y);
             }
             __syncthreads();
             if (lower) {
                 const double2 v = tile[cI][r];
-                *reinterpret_cast<double2*>(Zf + 2 * ((size_t)mm * ld + nn)) = v;
+                if (lo64) *reinterpret_cast<double2*>(Zf + 2 * ((size_t)mm * ld + nn)) = v;
                 if (Z32f)
                     *reinterpret_cast<float2*>(Z32f + 2 * ((size_t)mm * ld32 + nn)) = make_float2((float)v.x, (float)v.y);
             }
@@ -2108,7 +2113,8 @@ hipError_t launch_series_coeffs(const double* Q, int NT, const TriGeo* tg, const
 
 hipError_t launch_series_fill(const double* C, int NT, bool lossy, int Ne, const Freq* fr, int nf, double* Z,
                               size_t ld, size_t zStride, float* Z32, size_t ld32, size_t z32Stride, hipStream_t s,
-                              int maxBlocks) {
+                              int maxBlocks, bool tri) {
+    if (tri && !Z32) return hipErrorInvalidValue;
     const int T = (Ne + 15) / 16;
     const long long nb = (long long)T * (T + 1) / 2;
     const dim3 g((unsigned)(maxBlocks > 0 && maxBlocks < nb ? maxBlocks : nb)), blk(256);
@@ -2116,10 +2122,10 @@ hipError_t launch_series_fill(const double* C, int NT, bool lossy, int Ne, const
     case N:                                                                                                       \
         if (lossy)                                                                                                \
             hipLaunchKernelGGL((k_series_fill<N, true>), g, blk, 0, s, C, Ne, fr, nf, Z, ld, zStride, Z32, ld32,  \
-                               z32Stride, (int)nb);                                                               \
+                               z32Stride, (int)nb, (int)tri);                                                     \
         else                                                                                                      \
             hipLaunchKernelGGL((k_series_fill<N, false>), g, blk, 0, s, C, Ne, fr, nf, Z, ld, zStride, Z32, ld32, \
-                               z32Stride, (int)nb);                                                               \
+                               z32Stride, (int)nb, (int)tri);                                                     \
         break;
     switch (NT) {
         EFIE_SER_CASE(10)

@@ -151,10 +151,14 @@ hipError_t launch_series_coeffs(const double* Q, int NT, const TriGeo* tg, const
 // element (m,n) at 2*(m*ld + n), and its complex64 copy at
 // Z32 + f*z32Stride when Z32 is set.  lossy: some k has a nonzero
 // imaginary part.  maxBlocks > 0 caps the grid (tiles in a grid-stride
-// loop; a multiple of 8 keeps each XCD on its run of tiles).
+// loop; a multiple of 8 keeps each XCD on its run of tiles).  tri (Z32
+// required): only what the complex-symmetric mixed solve reads, the FP64
+// matrix on and above the diagonal plus its diagonal 64 x 64 blocks (the
+// residual) and the FP32 copy on and below the diagonal (the
+// factorisation); the other entries keep what they held.
 hipError_t launch_series_fill(const double* C, int NT, bool lossy, int Ne, const Freq* fr, int nf, double* Z,
                               size_t ld, size_t zStride, float* Z32, size_t ld32, size_t z32Stride, hipStream_t s,
-                              int maxBlocks = 0);
+                              int maxBlocks = 0, bool tri = false);
 // blocks of a series fill that runs beside a factorisation
 constexpr int kFillBlocksShared = 256;
 
