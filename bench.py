@@ -348,7 +348,7 @@ def bench_c4(a):
     `--c4-workers` host threads with a context each."""
     probs = _c4_problems(a.c4_max_n)
     fr = np.linspace(*W.C4["freqs"])
-    o = efie.opts(**W.OPTS_C1, precision=efie.PREC_MIXED, near_mode=efie.NEAR_MOMENTS, ir_tol=a.ir_tol,
+    o = efie.opts(**W.OPTS_C1, precision=efie.PREC_MIXED, near_mode=efie.NEAR_MOMENTS, ir_tol=a.ir_tol, ir_max_iter=a.ir_max_iter,
                   lu_kernel=a.lu_kernel, batch=a.batch)
     o64 = efie.opts(**W.OPTS_C1, precision=efie.PREC_FP64, near_mode=efie.NEAR_DIRECT)
     errs = []
@@ -596,6 +596,7 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--lu-kernel", type=int, default=0, help="0 auto (symmetric for the mixed sweep), 1 FP32 MFMA pivoted, 2 bf16x6 pivoted, 3 symmetric FP32")
     ap.add_argument("--ir-tol", type=float, default=1e-10)
+    ap.add_argument("--ir-max-iter", type=int, default=10, help="refinement corrections at most (the library default)")
     ap.add_argument("--batch", type=int, default=0, help="frequencies per batch (0: the library's choice)")
     ap.add_argument("--c4-mode", choices=["many", "per-geometry"], default="many",
                     help="c4: one efie_sweep_many call (default) or one efie_sweep per geometry")
@@ -653,7 +654,7 @@ def main():
         fr = fr[: a.nf]
     n = r.Ne
     if a.profile_leg:
-        kw = {"fast": dict(precision=efie.PREC_MIXED, near_mode=efie.NEAR_MOMENTS, ir_tol=a.ir_tol,
+        kw = {"fast": dict(precision=efie.PREC_MIXED, near_mode=efie.NEAR_MOMENTS, ir_tol=a.ir_tol, ir_max_iter=a.ir_max_iter,
                            lu_kernel=a.lu_kernel, batch=a.batch),
               "fp64": dict(precision=efie.PREC_FP64, near_mode=efie.NEAR_DIRECT),
               "fp64-series": dict(precision=efie.PREC_FP64, near_mode=efie.NEAR_MOMENTS)}[a.profile_leg]
@@ -669,7 +670,7 @@ def main():
     log(f"config {a.config}: Ne={n} Nt={F.shape[0]} Nf={len(fr)}")
     o64 = efie.opts(**W.OPTS_C1, precision=efie.PREC_FP64, near_mode=efie.NEAR_DIRECT)
     # refinement to ||V - Zx|| <= 1e-10 ||V||
-    ofast = efie.opts(**W.OPTS_C1, precision=efie.PREC_MIXED, near_mode=efie.NEAR_MOMENTS, ir_tol=a.ir_tol,
+    ofast = efie.opts(**W.OPTS_C1, precision=efie.PREC_MIXED, near_mode=efie.NEAR_MOMENTS, ir_tol=a.ir_tol, ir_max_iter=a.ir_max_iter,
                       lu_kernel=a.lu_kernel, batch=a.batch)
     with efie.Context(0) as ctx:
         ctx.set_problem(V, F, r)
@@ -704,12 +705,12 @@ def main():
             rep = ctx.sur_train(efie.opts(**W.OPTS_C1))
             t_train = time.perf_counter() - t0
             log(f"surrogate trained in {t_train:.2f} s")
-            osur = efie.opts(**W.OPTS_C1, precision=efie.PREC_MIXED, near_mode=efie.NEAR_SURROGATE, ir_tol=a.ir_tol,
+            osur = efie.opts(**W.OPTS_C1, precision=efie.PREC_MIXED, near_mode=efie.NEAR_SURROGATE, ir_tol=a.ir_tol, ir_max_iter=a.ir_max_iter,
                              lu_kernel=a.lu_kernel, batch=a.batch)
             dt_s, (zin_s, _, st_s) = timed(ctx, osur, fr, pe, ge, spread, a.steps, a.warmup, "surrogate")
             _, _, st_sp = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0,
                                     efie.opts(**W.OPTS_C1, precision=efie.PREC_MIXED,
-                                              near_mode=efie.NEAR_SURROGATE, ir_tol=a.ir_tol,
+                                              near_mode=efie.NEAR_SURROGATE, ir_tol=a.ir_tol, ir_max_iter=a.ir_max_iter,
                                               lu_kernel=a.lu_kernel, batch=a.batch, profile=1), stats=True)
             sur = {"value": len(fr) * a.steps / dt_s, "ms_per_step": 1e3 * dt_s / a.steps,
                    "train_s": t_train, "train": rep, "n_sur_pairs": st_s["n_sur_pairs"],
@@ -721,7 +722,7 @@ def main():
                 sur["vs_fp64"] = sur["value"] / res["fp64"]["value"]
             res["surrogate"] = sur
         # profiled pass of the headline path for the roofline
-        oprof = efie.opts(**W.OPTS_C1, precision=efie.PREC_MIXED, near_mode=efie.NEAR_MOMENTS, ir_tol=a.ir_tol,
+        oprof = efie.opts(**W.OPTS_C1, precision=efie.PREC_MIXED, near_mode=efie.NEAR_MOMENTS, ir_tol=a.ir_tol, ir_max_iter=a.ir_max_iter,
                       lu_kernel=a.lu_kernel, batch=a.batch, profile=1)
         _, _, st_p = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0, oprof, stats=True)
         log("profiled pass done")

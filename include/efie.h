@@ -341,7 +341,7 @@ efie_status efie_assemble_z(efie_ctx* ctx, double k_re, double k_im, double omeg
  * cusolverDnZgetrf + cusolverDnZgetrs pair of the reference
  * (patch_sweep_cuda.cu:473-480).  precision FP64: blocked LU with
  * tournament pivoting on the matrix cores; MIXED: complex FP32 LU plus FP64
- * iterative refinement (ir_tol 1e-14, at most 10 corrections); a system
+ * iterative refinement (ir_tol 1e-14, at most 16 corrections); a system
  * whose FP32 factorisation meets a zero pivot or whose refinement stalls
  * above 1e-11 is solved again by the FP64 LU (efie_stats.n_fp64_retry), and
  * only a zero pivot there returns EFIE_E_SINGULAR. */

@@ -350,7 +350,7 @@ void efie_opts_default(efie_opts* o) {
     o->precision = EFIE_PREC_FP64;
     o->batch = 0;
     o->ir_tol = 1e-14;
-    o->ir_max_iter = 10;
+    o->ir_max_iter = 16;
     o->near_mode = EFIE_NEAR_DIRECT;
     o->profile = 0;
     o->lu_kernel = EFIE_LU_AUTO;
