@@ -596,7 +596,7 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--lu-kernel", type=int, default=0, help="0 auto (symmetric for the mixed sweep), 1 FP32 MFMA pivoted, 2 bf16x6 pivoted, 3 symmetric FP32")
     ap.add_argument("--ir-tol", type=float, default=1e-10)
-    ap.add_argument("--ir-max-iter", type=int, default=10, help="refinement corrections at most (the library default)")
+    ap.add_argument("--ir-max-iter", type=int, default=16, help="refinement corrections at most (the library default)")
     ap.add_argument("--batch", type=int, default=0, help="frequencies per batch (0: the library's choice)")
     ap.add_argument("--c4-mode", choices=["many", "per-geometry"], default="many",
                     help="c4: one efie_sweep_many call (default) or one efie_sweep per geometry")
