@@ -152,6 +152,9 @@ typedef struct {
      * reach max(ir_tol, 1e-11) (stalled or at ir_max_iter); their results
      * have the FP64 path's accuracy                                        */
     int n_fp64_retry;
+    /* mixed precision: refinement corrections summed over every system
+     * (ir_iters_total / frequencies is the average per system)             */
+    long long ir_iters_total;
 } efie_stats;
 
 /* "accurate" preset (main_patch_efie_rwg.m:281-289), FP64, quadrature. */

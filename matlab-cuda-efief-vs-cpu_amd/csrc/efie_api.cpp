@@ -2189,6 +2189,7 @@ efie_status finish_batch(efie_ctx* c, const efie_opts& o, int nb, const BatchBuf
         if (mixed) {
             S->ir_iters_max = std::max(S->ir_iters_max, iters[i]);
             S->ir_resid_max = std::max(S->ir_resid_max, resid[i]);
+            S->ir_iters_total += iters[i];
         }
     }
     return ok();
@@ -2560,6 +2561,7 @@ efie_status efie_sweep(efie_ctx* c, const double* freqs, int Nf, double mu, doub
         S->ms_factor += S2.ms_factor;
         S->ms_solve += S2.ms_solve;
         S->ir_iters_max = std::max(S->ir_iters_max, S2.ir_iters_max);
+        S->ir_iters_total += S2.ir_iters_total;
         S->ir_resid_max = std::max(S->ir_resid_max, S2.ir_resid_max);
         for (int q = 0; q < EFIE_NKCLASS; ++q) {
             S->ms_kernel[q] += S2.ms_kernel[q];
@@ -3096,6 +3098,7 @@ efie_status efie_sweep_many(efie_ctx* c, const efie_sweep_item* items, int count
         S->ms_solve += S2.ms_solve;
         S->ms_moments += S2.ms_moments;
         S->ir_iters_max = std::max(S->ir_iters_max, S2.ir_iters_max);
+        S->ir_iters_total += S2.ir_iters_total;
         S->ir_resid_max = std::max(S->ir_resid_max, S2.ir_resid_max);
         for (int q = 0; q < EFIE_NKCLASS; ++q) {
             S->ms_kernel[q] += S2.ms_kernel[q];

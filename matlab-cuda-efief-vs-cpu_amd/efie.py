@@ -72,6 +72,7 @@ class Stats(C.Structure):
         ("n_kernel", C.c_int * len(KERNEL_CLASSES)),
         ("n_sur_pairs", C.c_longlong), ("n_sur_fallback", C.c_longlong), ("n_lu_retry", C.c_int),
         ("n_fp64_retry", C.c_int),
+        ("ir_iters_total", C.c_longlong),
     ]
 
     def as_dict(self):
