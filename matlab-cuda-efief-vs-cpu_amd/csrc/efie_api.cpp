@@ -270,13 +270,13 @@ struct efie_ctx {
     double dcMax = 0;   // largest centroid distance of an RWG's two triangles
 
     // near-pair surrogate: the installed model and, per mesh, the pairs it
-    // takes (dSurPairs, nSurIn), the pairs left to the quadrature (dSurFb,
-    // nSurFb) and the near map without the former (dSurMat)
+    // takes (dSurPairs, nSurIn, with their normalised vertices dSurTri18)
+    // and the pairs left to the quadrature (dSurFb, nSurFb)
     std::vector<float> surBlob;
     bool surExact = false, surValid = false;
     double surAlpha = -1, surLmax = 0, surDmax = 0;
     int nSurIn = 0, nSurFb = 0, surNp = 0, surNq = 0;
-    DBuf dSurModel, dSurPairs, dSurFb, dSurMat, dSurTri18, dSurT, dSurSub, dSurReg;
+    DBuf dSurModel, dSurPairs, dSurFb, dSurTri18, dSurT, dSurSub, dSurReg;
     DBuf dMlpA, dMlpB, dMlpC, dMlpD, dMlpE;  // scratch of the network entry points
 
     DBuf dTri, dReg, dNearPts, dSelfPts, dASelf, dAeq, dNearMat, dRowCnt, dList;
@@ -1595,12 +1595,11 @@ efie_status fill_z(efie_ctx* c, const efie_opts* o, const NearPlan& plan, const 
     }
     pf.begin(EFIE_K_REGULAR);
     HIPCHK(launch_rec_regular(c->dReg.as<Pt>(), c->Nq, c->dTri.as<TriGeo>(), c->dRad.as<double>(), Nt,
-                              plan.sur ? c->dSurMat.as<uint8_t>()
-                                       : (c->havePack ? c->dNearMat.as<uint8_t>() : nullptr),
-                              c->nearEps, dfr, recs, s));
+                              c->havePack ? c->dNearMat.as<uint8_t>() : nullptr, c->nearEps, dfr, recs, s));
     pf.end(EFIE_K_REGULAR);
     if (plan.sur) {
-        // the regular rule wrote the base records of the surrogate's pairs
+        // the records of the near pairs inside the envelope: product rule
+        // and network correction in one kernel
         pf.begin(EFIE_K_SURROGATE);
         const double* Tb = nullptr;
         if (c->surExact) {
@@ -1609,7 +1608,8 @@ efie_status fill_z(efie_ctx* c, const efie_opts* o, const NearPlan& plan, const 
                                      o->near_alpha, c->dSurT.as<double>(), nullptr, s));
             Tb = c->dSurT.as<double>();
         }
-        HIPCHK(launch_sur_apply(c->dSurPairs.as<SurPair>(), c->nSurIn, c->dSurModel.as<float>(), Tb, dfr, recs, s));
+        HIPCHK(launch_sur_records(c->dSurPairs.as<SurPair>(), c->nSurIn, c->dSurModel.as<float>(), Tb,
+                                  c->dSurTri18.as<double>(), c->dSurReg.as<double>(), c->surNq, dfr, recs, s));
         pf.end(EFIE_K_SURROGATE, c->surExact ? 2 : 1);
     }
     pf.begin(EFIE_K_COMBINE);
@@ -1731,7 +1731,6 @@ efie_status ensure_surrogate(efie_ctx* c, const efie_opts* o) {
     HIPCHK(c->dSurTri18.ensure(sizeof(double) * 18 * std::max<size_t>(1, nIn)));
     HIPCHK(c->dSurT.ensure(sizeof(double) * 16 * std::max<size_t>(1, nIn)));
     HIPCHK(c->dSurFb.ensure(sizeof(int2) * std::max<size_t>(1, nFb)));
-    HIPCHK(c->dSurMat.ensure((size_t)c->Nt * c->Nt));
     HIPCHK(c->dSurModel.ensure(sizeof(float) * kSurModel));
     HIPCHK(c->dSurSub.ensure(sizeof(double) * sub.size()));
     HIPCHK(c->dSurReg.ensure(sizeof(double) * reg.size()));
@@ -1743,7 +1742,6 @@ efie_status ensure_surrogate(efie_ctx* c, const efie_opts* o) {
     HIPCHK(hipMemcpyAsync(c->dSurModel.p, c->surBlob.data(), sizeof(float) * kSurModel, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(c->dSurSub.p, sub.data(), sizeof(double) * sub.size(), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(c->dSurReg.p, reg.data(), sizeof(double) * reg.size(), hipMemcpyHostToDevice, s));
-    HIPCHK(launch_sur_map(c->dSurFb.as<int2>(), (int)nFb, c->Nt, c->dSurMat.as<uint8_t>(), s));
     HIPCHK(hipStreamSynchronize(s));
     c->nSurIn = (int)nIn;
     c->nSurFb = (int)nFb;

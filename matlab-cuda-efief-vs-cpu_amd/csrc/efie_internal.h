@@ -358,16 +358,15 @@ size_t mlp_grad_work_floats(int N);
 hipError_t launch_mlp_loss_grad(const float* X, const float* T, const float* params, int N, float* work,
                                 float* grad, float* lossOut, hipStream_t s);
 hipError_t launch_adam(float* p, const float* g, float* m, float* v, int n, float lr, int step, hipStream_t s);
-// Add the network's correction to the base records of the n pairs at
-// wavenumber fr->kr (model: params then the kMlpNorm standardisation).
-// With Tbuf non-null the correction is taken from Tbuf (labels, [n][16])
-// instead of the network.
-hipError_t launch_sur_apply(const SurPair* pairs, int n, const float* model, const double* Tbuf, const Freq* fr,
-                            Rec* recs, hipStream_t s);
-// Near map of the surrogate mode: the n pairs of fb (near pairs left to the
-// quadrature) set, all else clear, so the pairs handled by the surrogate get
-// the regular rule as base.
-hipError_t launch_sur_map(const int2* fb, int n, int Nt, uint8_t* out, hipStream_t s);
+// Records of the n near pairs in the network's envelope at wavenumber
+// fr->kr, each written once: the plain product rule of the regular pairs
+// (the rule's points regBary [Nq][4] on the pair's normalised vertices
+// tri18 [n][18]) plus the network's correction (model: params then the
+// kMlpNorm standardisation), or with Tbuf non-null the correction taken
+// from Tbuf (labels, [n][16]).
+hipError_t launch_sur_records(const SurPair* pairs, int n, const float* model, const double* Tbuf,
+                              const double* tri18, const double* regBary, int Nq, const Freq* fr, Rec* recs,
+                              hipStream_t s);
 
 // Zin epilogue of batch element b: x[e] = X[b*strideX + e*xinc] (complex),
 // Iport = sum(wP.*len.*x) - sum(wG.*len.*x) (or x[p]-x[g] in the centroid

@@ -10,8 +10,8 @@
 // registers), and the weight gradient as dZ^T H over the block's samples.
 // Blocks write partial gradients that a second kernel sums in block order,
 // so a gradient is bit-reproducible.  Adam updates the parameters.
-// Inference: k_sur_apply runs the network for 64 near pairs per block and
-// adds the de-normalised correction to each pair's regular-rule record.
+// Inference: k_sur_records writes the records of 64 near pairs per block,
+// the plain product rule plus the network's de-normalised correction.
 // Nothing here has a counterpart in the reference.
 #include "efie_internal.h"
 #include "efie_math.h"
@@ -293,19 +293,112 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
 }
 
 // ------------------------------------------------------------ inference --
-// Correction of near pairs (t1 < t2), 64 per block: the network (MFMA
+// Records of the near pairs inside the network's envelope, 64 pairs per
+// block, each written once: the plain product rule of the regular pairs
+// (sur_base) plus the network's correction of it.  The network (MFMA
 // forward as above, inputs standardised from the pair's features and k d)
 // or a label buffer gives T (re, im of S0, S1, S1', S2, normalised, times
-// exp(+j k d) 4 pi); one thread per pair then adds to the record
-//   S0 += L^3 C0,  S1 += L^4 Rm^T C1,  S1' += L^4 Rm^T C1',  S2 += L^5 C2
-// with C = T exp(-j k d) / (4 pi).
+// exp(+j k d) 4 pi); with B the product rule in the same frame and
+// C = T exp(-j k d) / (4 pi) + B, the record is
+//   S0 = L^3 C0,  S1 = L^4 Rm^T C1,  S1' = L^4 Rm^T C1',  S2 = L^5 C2.
+
+// The product rule on one pair in the surrogate's frame: v holds t1's then
+// t2's vertices (lengths in units of L), reg the rule's points as
+// barycentric weights and weight fractions of twice the area ([Nq][4]), kt
+// the wavenumber times L.  Moments as a record's (S1 about t1's centroid,
+// S1' about t2's) with G = exp(-j kt R) / (4 pi R).  Thread u of a group of
+// four consecutive lanes takes the test points u, u + 4, ...; the group's
+// sums meet by shuffles, every thread of the group ends with all 16.
+__device__ void sur_base(const double* __restrict__ v, const double* __restrict__ reg, int Nq, double kt, int u,
+                         double* res) {
+    double c[6], a2[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const double* w = v + 9 * t;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) c[3 * t + d] = (w[d] + w[3 + d] + w[6 + d]) / 3.0;
+        const double ex = w[3] - w[0], ey = w[4] - w[1], ez = w[5] - w[2];
+        const double fx = w[6] - w[0], fy = w[7] - w[1], fz = w[8] - w[2];
+        const double nx = ey * fz - ez * fy, ny = ez * fx - ex * fz, nz = ex * fy - ey * fx;
+        a2[t] = sqrt(nx * nx + ny * ny + nz * nz);
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) res[k] = 0.0;
+    const double inv4pi = 1.0 / (4.0 * 3.141592653589793238462643383279502884);
+    for (int i = u; i < Nq; i += 4) {
+        const double* b = reg + 4 * i;
+        const double px = b[0] * v[0] + b[1] * v[3] + b[2] * v[6], py = b[0] * v[1] + b[1] * v[4] + b[2] * v[7],
+                     pz = b[0] * v[2] + b[1] * v[5] + b[2] * v[8];
+        const double wi = b[3] * a2[0];
+        double gr = 0, gi = 0, hxr = 0, hxi = 0, hyr = 0, hyi = 0, hzr = 0, hzi = 0;
+        for (int j = 0; j < Nq; ++j) {
+            const double* q = reg + 4 * j;
+            const double qx = q[0] * v[9] + q[1] * v[12] + q[2] * v[15], qy = q[0] * v[10] + q[1] * v[13] + q[2] * v[16],
+                         qz = q[0] * v[11] + q[1] * v[14] + q[2] * v[17];
+            const double dx = px - qx, dy = py - qy, dz = pz - qz;
+            const double R = sqrt(dx * dx + dy * dy + dz * dz);
+            double sn, cs;
+            sincos_cw(kt * R, &sn, &cs);
+            const double f = q[3] * a2[1] * inv4pi / R;
+            const double wr = cs * f, wim = -sn * f;
+            const double rx = qx - c[3], ry = qy - c[4], rz = qz - c[5];
+            gr += wr;
+            gi += wim;
+            hxr += wr * rx;
+            hxi += wim * rx;
+            hyr += wr * ry;
+            hyi += wim * ry;
+            hzr += wr * rz;
+            hzi += wim * rz;
+        }
+        const double rx = px - c[0], ry = py - c[1], rz = pz - c[2];
+        res[0] += wi * gr;
+        res[1] += wi * gi;
+        res[2] += wi * rx * gr;
+        res[3] += wi * rx * gi;
+        res[4] += wi * ry * gr;
+        res[5] += wi * ry * gi;
+        res[6] += wi * rz * gr;
+        res[7] += wi * rz * gi;
+        res[8] += wi * hxr;
+        res[9] += wi * hxi;
+        res[10] += wi * hyr;
+        res[11] += wi * hyi;
+        res[12] += wi * hzr;
+        res[13] += wi * hzi;
+        res[14] += wi * (rx * hxr + ry * hyr + rz * hzr);
+        res[15] += wi * (rx * hxi + ry * hyi + rz * hzi);
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        res[k] += __shfl_xor(res[k], 1);
+        res[k] += __shfl_xor(res[k], 2);
+    }
+}
+
 template <bool FROM_BUF>
-__global__ __launch_bounds__(256) void k_sur_apply(const SurPair* __restrict__ pairs, int n,
-                                                   const float* __restrict__ model, const double* __restrict__ Tbuf,
-                                                   const Freq* __restrict__ fr, Rec* __restrict__ recs) {
+__global__ __launch_bounds__(256) void k_sur_records(const SurPair* __restrict__ pairs, int n,
+                                                     const float* __restrict__ model, const double* __restrict__ Tbuf,
+                                                     const double* __restrict__ tri18, const double* __restrict__ reg,
+                                                     int Nq, const Freq* __restrict__ fr, Rec* __restrict__ recs) {
     extern __shared__ float sm[];
+    __shared__ double sBase[kTB][17];
     const int base = blockIdx.x * kTB;
     const double kr = fr->kr;
+    // the product rule: four threads per pair
+    {
+        const int pl = threadIdx.x >> 2, u = threadIdx.x & 3, p = base + pl;
+        double v[18];
+        const int pc = p < n ? p : n - 1;
+#pragma unroll
+        for (int i = 0; i < 18; ++i) v[i] = tri18[(size_t)pc * 18 + i];
+        double res[16];
+        sur_base(v, reg, Nq, kr * pairs[pc].L, u, res);
+        if (u == 0) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) sBase[pl][k] = res[k];
+        }
+    }
     const float* xm = model + kNP;   // standardisation after the parameters
     const float* xi = xm + kIn;
     const float* ym = xi + kIn;
@@ -330,8 +423,8 @@ __global__ __launch_bounds__(256) void k_sur_apply(const SurPair* __restrict__ p
         // outputs over the wave's own x rows
 #pragma unroll
         for (int v = 0; v < 4; ++v) sX[(s0 + 4 * lg + v) * PX + li] = y[v];
-        __syncthreads();
     }
+    __syncthreads();
     const int t = threadIdx.x, p = base + t;
     if (t >= kTB || p >= n) return;
     const SurPair& P = pairs[p];
@@ -347,13 +440,13 @@ __global__ __launch_bounds__(256) void k_sur_apply(const SurPair* __restrict__ p
     double C[kOut];
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-        C[2 * q] = T[2 * q] * er - T[2 * q + 1] * ei;
-        C[2 * q + 1] = T[2 * q] * ei + T[2 * q + 1] * er;
+        C[2 * q] = T[2 * q] * er - T[2 * q + 1] * ei + sBase[t][2 * q];
+        C[2 * q + 1] = T[2 * q] * ei + T[2 * q + 1] * er + sBase[t][2 * q + 1];
     }
     const double L3 = L * L * L, L4 = L3 * L, L5 = L4 * L;
     double* r = recs[rec_index(P.t1, P.t2)].v;
-    r[0] += L3 * C[0];
-    r[1] += L3 * C[1];
+    r[0] = L3 * C[0];
+    r[1] = L3 * C[1];
 #pragma unroll
     for (int v = 0; v < 2; ++v) {        // S1 (slots 1-3), S1' (slots 4-6)
 #pragma unroll
@@ -364,26 +457,12 @@ __global__ __launch_bounds__(256) void k_sur_apply(const SurPair* __restrict__ p
                 gr += P.Rm[3 * b + a] * C[2 * (1 + 3 * v + b)];
                 gi += P.Rm[3 * b + a] * C[2 * (1 + 3 * v + b) + 1];
             }
-            r[2 * (1 + 3 * v + a)] += L4 * gr;
-            r[2 * (1 + 3 * v + a) + 1] += L4 * gi;
+            r[2 * (1 + 3 * v + a)] = L4 * gr;
+            r[2 * (1 + 3 * v + a) + 1] = L4 * gi;
         }
     }
-    r[14] += L5 * C[14];
-    r[15] += L5 * C[15];
-}
-
-// The near map of the surrogate's sweep: only the pairs left to the
-// quadrature (the fallback list) are near for the regular kernel, which
-// therefore writes the base records of the surrogate's pairs.
-__global__ void k_sur_map(const int2* __restrict__ fb, int n, int Nt, uint8_t* __restrict__ out, int phase) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (phase == 0) {
-        if (i < (size_t)Nt * Nt) out[i] = 0;
-    } else if ((int)i < n) {
-        const int a = fb[i].x, b = fb[i].y;
-        out[(size_t)a * Nt + b] = 1;
-        out[(size_t)b * Nt + a] = 1;
-    }
+    r[14] = L5 * C[14];
+    r[15] = L5 * C[15];
 }
 
 // ------------------------------------------------------------ launchers --
@@ -399,7 +478,7 @@ static hipError_t allow_lds() {
             r = hipFuncSetAttribute((const void*)k_mlp_forward, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)(sizeof(float) * kFwdLds));
         if (r == hipSuccess)
-            r = hipFuncSetAttribute((const void*)k_sur_apply<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+            r = hipFuncSetAttribute((const void*)k_sur_records<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)(sizeof(float) * kFwdLds));
         return r;
     }();
@@ -438,24 +517,20 @@ hipError_t launch_adam(float* p, const float* g, float* m, float* v, int n, floa
     return hipGetLastError();
 }
 
-hipError_t launch_sur_apply(const SurPair* pairs, int n, const float* model, const double* Tbuf, const Freq* fr,
-                            Rec* recs, hipStream_t s) {
+hipError_t launch_sur_records(const SurPair* pairs, int n, const float* model, const double* Tbuf,
+                              const double* tri18, const double* regBary, int Nq, const Freq* fr, Rec* recs,
+                              hipStream_t s) {
     if (n <= 0) return hipSuccess;
+    if (Nq < 1 || Nq > 16) return hipErrorInvalidValue;
     if (Tbuf) {
-        hipLaunchKernelGGL(k_sur_apply<true>, dim3(blocks_of(n)), dim3(256), 0, s, pairs, n, model, Tbuf, fr, recs);
+        hipLaunchKernelGGL(k_sur_records<true>, dim3(blocks_of(n)), dim3(256), 0, s, pairs, n, model, Tbuf, tri18,
+                           regBary, Nq, fr, recs);
     } else {
         hipError_t e = allow_lds();
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_sur_apply<false>, dim3(blocks_of(n)), dim3(256), sizeof(float) * kFwdLds, s, pairs, n,
-                           model, Tbuf, fr, recs);
+        hipLaunchKernelGGL(k_sur_records<false>, dim3(blocks_of(n)), dim3(256), sizeof(float) * kFwdLds, s, pairs, n,
+                           model, Tbuf, tri18, regBary, Nq, fr, recs);
     }
-    return hipGetLastError();
-}
-
-hipError_t launch_sur_map(const int2* fb, int n, int Nt, uint8_t* out, hipStream_t s) {
-    const size_t nn = (size_t)Nt * Nt;
-    hipLaunchKernelGGL(k_sur_map, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, fb, n, Nt, out, 0);
-    if (n > 0) hipLaunchKernelGGL(k_sur_map, dim3((n + 255) / 256), dim3(256), 0, s, fb, n, Nt, out, 1);
     return hipGetLastError();
 }
 
