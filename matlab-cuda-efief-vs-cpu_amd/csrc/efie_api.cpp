@@ -2031,7 +2031,7 @@ BatchBufs batch_bufs(efie_ctx* c, const efie_opts& o, int n, int lda, int B, int
 // nElem (packed batches, symmetric factorisation): the order of each
 // matrix's problem, the identity padding beyond it skipped.
 efie_status factor_batch(efie_ctx* c, const efie_opts& o, int n, int lda, int nb, const Prof& pf, bool z32Ready,
-                         const BatchBufs& bb, const int* nElem = nullptr) {
+                         const BatchBufs& bb, const int* nElem = nullptr, hipStream_t head = nullptr) {
     hipStream_t s = c->stream;
     const size_t zElems = (size_t)n * lda;
     // look-ahead on the panel stream, except when per-kernel profiling
@@ -2040,7 +2040,10 @@ efie_status factor_batch(efie_ctx* c, const efie_opts& o, int n, int lda, int nb
               bb.w.swp, c->prof ? nullptr : c->panelStream, c->evPanel, c->evAhead,
               (o.precision == EFIE_PREC_MIXED && o.lu_kernel == EFIE_LU_BF16X6) ? bb.w.split : nullptr,
               o.precision == EFIE_PREC_MIXED && o.lu_kernel == EFIE_LU_SYM};
-    if (wk.noPivot) wk.nElem = nElem;
+    if (wk.noPivot) {
+        wk.nElem = nElem;
+        wk.head = head;
+    }
     HIPCHK(launch_fill_int(bb.info, 0, nb, s));
     if (o.precision != EFIE_PREC_MIXED) {
         HIPCHK(lu_factor<double>(bb.Z, 2 * zElems, n, lda, nb, wk, s, &pf));
@@ -2465,7 +2468,9 @@ efie_status efie_sweep(efie_ctx* c, const double* freqs, int Nf, double mu, doub
         const BatchBufs bb = batch_bufs(c, o, n, lda, B, buf);
         HIPCHK(hipStreamWaitEvent(s, c->evFill[buf], 0));
         HIPCHK(hipEventRecord(c->evLUStart[buf], s));
-        st = factor_batch(c, o, n, lda, nb, pf, have32[buf], bb);
+        // the first panels of a batch on the fill stream, behind its fill:
+        // beside the previous batch's last factorisation steps
+        st = factor_batch(c, o, n, lda, nb, pf, have32[buf], bb, nullptr, overlap ? sf : nullptr);
         if (st) return st;
         HIPCHK(hipEventRecord(c->evLU[buf], s));
         HIPCHK(hipStreamWaitEvent(si, c->evLU[buf], 0));
@@ -2963,7 +2968,7 @@ efie_status efie_sweep_many(efie_ctx* c, const efie_sweep_item* items, int count
         const BatchBufs bb = bufs(buf);
         HIPCHK(hipStreamWaitEvent(s, c->evFill[buf], 0));
         HIPCHK(hipEventRecord(c->evLUStart[buf], s));
-        st = factor_batch(c, o, b.np, b.lda, b.nb, pf, true, bb, nebuf(buf));
+        st = factor_batch(c, o, b.np, b.lda, b.nb, pf, true, bb, nebuf(buf), overlap ? sf : nullptr);
         if (st) return st;
         HIPCHK(hipEventRecord(c->evLU[buf], s));
         HIPCHK(hipStreamWaitEvent(si, c->evLU[buf], 0));

@@ -211,6 +211,11 @@ struct LuWork {
     // nElem[b] in its leading block and the identity beyond (null: order n);
     // steps wholly inside the padding are skipped
     const int* nElem = nullptr;
+    // lu_factor_sym with look-ahead: the first block's panels run on this
+    // stream, which the caller has ordered after the matrices are complete
+    // (the fill stream), so they can overlap the previous factorisation's
+    // last steps on the main and panel streams; null: on the panel stream
+    hipStream_t head = nullptr;
 };
 size_t lu_split_bytes(int n);
 // ints per batch element of LuWork::swaps

@@ -1599,18 +1599,21 @@ static hipError_t lu_factor_sym(T* A, size_t strideA, int n, int lda, int batch,
                                 const Prof* pf) {
     HipFirst chk;
     hipStream_t sp = wk.panel ? wk.panel : s;
+    // the first block's panels: on the head stream when given (already
+    // ordered after the matrices' fill), else on the panel stream after s
+    hipStream_t sh = wk.panel && wk.head ? wk.head : sp;
     hipLaunchKernelGGL(k_ident_ipiv, dim3((n + 255) / 256, batch), dim3(256), 0, s, wk.ipiv, n);
-    if (sp != s) {
+    if (sh == sp && sp != s) {
         chk(hipEventRecord(wk.evAhead, s));
         chk(hipStreamWaitEvent(sp, wk.evAhead, 0));
     }
-    panel_phase<T>(A, strideA, n, lda, batch, wk, 0, n < WB ? n : WB, 0, sp, pf);
-    if (sp != s) chk(hipEventRecord(wk.evPanel, sp));
+    panel_phase<T>(A, strideA, n, lda, batch, wk, 0, n < WB ? n : WB, 0, sh, pf);
+    if (sh != s) chk(hipEventRecord(wk.evPanel, sh));
     for (int j0 = 0; j0 < n; j0 += WB) {
         const int wb = n - j0 < WB ? n - j0 : WB;
         const int jend = j0 + wb;
         const int jnext = n - jend < WB ? n : jend + WB;
-        if (sp != s) chk(hipStreamWaitEvent(s, wk.evPanel, 0));
+        if ((j0 == 0 ? sh : sp) != s) chk(hipStreamWaitEvent(s, wk.evPanel, 0));
         if (jend >= n) break;
         pf->begin(7);
         hipLaunchKernelGGL(k_sym_u12<T>, dim3((n - jend + 63) / 64, (wb + 31) / 32, batch), dim3(256), 0, s, A,
