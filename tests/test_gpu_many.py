@@ -114,6 +114,21 @@ def test_many_equals_single_sweeps(efie_mod, engine, probs, name):
         assert st["ir_resid_max"] <= max(kw["ir_tol"], 1e-11)
 
 
+def test_many_lossy_medium(efie_mod, engine, probs):
+    """A lossy medium (complex eps: every k complex) through the packed call,
+    mixed symmetric path with the series fill, against the one-problem
+    sweeps."""
+    eps = W.EPS0 * (2.2 - 0.02j)
+    o = efie_mod.opts(**W.OPTS_C1, precision=efie_mod.PREC_MIXED, near_mode=efie_mod.NEAR_MOMENTS, ir_tol=1e-12)
+    res = efie_mod.sweep_many(engine, _items(probs), W.MU0, eps, o)
+    worst = 0.0
+    for p, (zin, _) in zip(probs, res):
+        z1 = p["ctx"].sweep(p["freqs"], W.MU0, eps, p["pe"], p["ge"], 1.0, p["spread"], 50.0, o)[0]
+        worst = max(worst, _rel(zin, z1))
+    print(f"lossy: max rel diff vs single sweeps {worst:.2e}", flush=True)
+    assert worst <= 1e-9
+
+
 def test_many_vs_oracle(efie_mod, orc, engine, probs):
     """The smallest problem and the one padded in a shared batch against the
     oracle's sweep, through the mixed headline path (series fill, FP32
