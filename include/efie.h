@@ -278,7 +278,9 @@ typedef struct {
  *   - a problem's one-off work (geometry, near lists, series) runs while
  *     earlier batches are factored.
  * The engine need not hold a problem itself; one problem context may
- * appear in several items.  stats (may be NULL) sums the whole call. */
+ * appear in several items.  The call uses the problem contexts (their
+ * streams and caches): no other thread may use them meanwhile.  stats
+ * (may be NULL) sums the whole call. */
 efie_status efie_sweep_many(efie_ctx* engine, const efie_sweep_item* items, int count,
                             double mu, double eps_re, double eps_im,
                             const efie_opts* opts, efie_stats* stats);
