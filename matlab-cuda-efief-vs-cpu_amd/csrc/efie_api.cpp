@@ -2056,7 +2056,10 @@ efie_status factor_batch(efie_ctx* c, const efie_opts& o, int n, int lda, int nb
     }
     HIPCHK(lu_factor<float>(bb.Z32, 2 * zElems, n, lda, nb, wk, s, &pf));
     pf.begin(EFIE_K_OTHER);
-    HIPCHK(lu_make_perm(bb.w.ipiv, n, nb, bb.perm, s));
+    if (wk.noPivot)
+        HIPCHK(lu_ident_perm(n, nb, bb.perm, s));
+    else
+        HIPCHK(lu_make_perm(bb.w.ipiv, n, nb, bb.perm, s));
     pf.end(EFIE_K_OTHER);
     return ok();
 }

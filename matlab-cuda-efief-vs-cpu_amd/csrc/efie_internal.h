@@ -254,6 +254,8 @@ hipError_t lu_trsv(const T* A, size_t strideA, int n, int lda, T* X, size_t stri
                    int batch, bool lower, const int* skip, hipStream_t s, TrsvWork* tw);
 // Row permutation equivalent to the sequential interchanges of ipiv.
 hipError_t lu_make_perm(const int* ipiv, int n, int batch, int* perm, hipStream_t s);
+// The identity permutation (a factorisation without interchanges).
+hipError_t lu_ident_perm(int n, int batch, int* perm, hipStream_t s);
 size_t lu_perm_lds_limit();
 
 // ------------------------------------------- allocation-free solve ------

@@ -523,17 +523,11 @@ __global__ __launch_bounds__(256) void k_sym_u12(T* A, size_t strideA, int lda, 
 // with a drift, U = D L^T of the outer blocks would not be the U of the
 // elimination, and near resonance small pivots amplify that mismatch into a
 // backward error of order 1e-4 in FP32.
-template <typename T, bool SYM = false>
-__global__ __launch_bounds__(64) void k_panel_diag(T* A, size_t strideA, int lda, int j0, int w, int* info,
-                                                   const int* __restrict__ nElem) {
-    // one wave, lane k holds row k of the block; row c (SYM: column c) is
-    // broadcast through LDS
-    __shared__ cpx<T> prow[NB];
-    const int b = blockIdx.x;
-    // padded matrix: a diagonal block in the identity padding is its own
-    // factorisation
-    if (nElem && j0 >= nElem[b]) return;
-    T* Ab = A + (size_t)b * strideA;
+// The body of k_panel_diag for a block of width w; inlined once with w =
+// NB (the full sub-panels: every bound a constant, so the column's LDS
+// reads of a step issue together) and once for a narrower last one.
+template <typename T, bool SYM>
+__device__ __forceinline__ void panel_diag_body(T* Ab, int lda, int j0, int w, int* infob, cpx<T>* prow) {
     const int lane = threadIdx.x;
     cpx<T> a[NB];
 #pragma unroll
@@ -556,7 +550,7 @@ __global__ __launch_bounds__(64) void k_panel_diag(T* A, size_t strideA, int lda
             const cpx<T> d = prow[c];
             const bool zp = (d.r == (T)0 && d.i == (T)0);
             if (zp) {
-                if (lane == 0) atomicCAS(info + b, 0, j0 + c + 1);
+                if (lane == 0) atomicCAS(infob, 0, j0 + c + 1);
             } else if (lane > c && lane < w) {
                 const cpx<T> l = cmul(a[c], cinv(d));
                 a[c] = l;
@@ -578,13 +572,55 @@ __global__ __launch_bounds__(64) void k_panel_diag(T* A, size_t strideA, int lda
     }
 }
 
+template <typename T, bool SYM = false>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1))) void k_panel_diag(T* A, size_t strideA, int lda, int j0, int w, int* info,
+                                                   const int* __restrict__ nElem) {
+    // one wave, lane k holds row k of the block; row c (SYM: column c) is
+    // broadcast through LDS
+    __shared__ cpx<T> prow[NB];
+    const int b = blockIdx.x;
+    // padded matrix: a diagonal block in the identity padding is its own
+    // factorisation
+    if (nElem && j0 >= nElem[b]) return;
+    T* Ab = A + (size_t)b * strideA;
+    if (w == NB)
+        panel_diag_body<T, SYM>(Ab, lda, j0, NB, info + b, prow);
+    else
+        panel_diag_body<T, SYM>(Ab, lda, j0, w, info + b, prow);
+}
+
 // With L11 U11 of the w x w diagonal block at (d, d) in place:
 // k_panel_rows solves rows [rlo, rhi) of L21 = A21 U11^-1 (columns d..d+w),
 // k_panel_cols solves columns [clo, chi) of U12 = L11^-1 A12 (rows d..d+w).
+// Row solve x U11 = a of k_panel_rows for a block of width w (x in place
+// of a).  Right-looking: once x_k is final it is taken out of every later
+// column, so the updates of a step are independent of each other; each
+// x_c still receives its terms in the order k = 0, 1, ..., c-1 and is then
+// multiplied by the reciprocal of U11[c][c] (dinv, zero pivots left
+// undivided), as a left-looking substitution would.
 template <typename T>
-__global__ __launch_bounds__(256) void k_panel_rows(T* A, size_t strideA, int lda, int d, int w, int rlo,
+__device__ __forceinline__ void panel_rows_body(cpx<T>* x, const cpx<T>* a11, const cpx<T>* dinv, int w) {
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        if (k < w) {
+            x[k] = cmul(x[k], dinv[k]);
+#pragma unroll
+            for (int c = k + 1; c < NB; ++c) {
+                if (c < w) {
+                    const cpx<T> u = a11[k * NB + c];
+                    x[c].r -= x[k].r * u.r - x[k].i * u.i;
+                    x[c].i -= x[k].r * u.i + x[k].i * u.r;
+                }
+            }
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1))) void k_panel_rows(T* A, size_t strideA, int lda, int d, int w, int rlo,
                                                     int rhi, const int* __restrict__ nElem) {
     __shared__ cpx<T> a11[NB * NB];
+    __shared__ cpx<T> dinv[NB];
     const int b = blockIdx.y;
     // padded matrix: rows in the padding are zero left of the diagonal
     if (nElem && rlo + (int)blockIdx.x * 256 >= nElem[b]) return;
@@ -594,6 +630,11 @@ __global__ __launch_bounds__(256) void k_panel_rows(T* A, size_t strideA, int ld
         int i = e / w, k = e % w;
         a11[i * NB + k] = ld(Ab + 2 * ((size_t)(d + i) * lda + d + k));
     }
+    if (tid < w) {
+        // 1 / U11[c][c]; a zero pivot multiplies by one
+        const cpx<T> dd = ld(Ab + 2 * ((size_t)(d + tid) * lda + d + tid));
+        dinv[tid] = (dd.r == (T)0 && dd.i == (T)0) ? cpx<T>{(T)1, (T)0} : cinv(dd);
+    }
     __syncthreads();
     const int row = rlo + blockIdx.x * 256 + tid;
     if (row >= rhi) return;
@@ -602,20 +643,10 @@ __global__ __launch_bounds__(256) void k_panel_rows(T* A, size_t strideA, int ld
 #pragma unroll
     for (int c = 0; c < NB; ++c)
         if (c < w) x[c] = ld(p + 2 * c);
-#pragma unroll
-    for (int c = 0; c < NB; ++c) {
-        if (c < w) {
-            cpx<T> s = x[c];
-#pragma unroll
-            for (int k = 0; k < c; ++k) {
-                cpx<T> u = a11[k * NB + c];
-                s.r -= x[k].r * u.r - x[k].i * u.i;
-                s.i -= x[k].r * u.i + x[k].i * u.r;
-            }
-            cpx<T> dd = a11[c * NB + c];
-            x[c] = (dd.r == (T)0 && dd.i == (T)0) ? s : cmul(s, cinv(dd));
-        }
-    }
+    if (w == NB)
+        panel_rows_body<T>(x, a11, dinv, NB);
+    else
+        panel_rows_body<T>(x, a11, dinv, w);
 #pragma unroll
     for (int c = 0; c < NB; ++c)
         if (c < w) st(p + 2 * c, x[c]);
@@ -1191,7 +1222,7 @@ __device__ __forceinline__ double lane_read(double v, int j) {
 }
 
 template <typename T, bool LOWER>
-__global__ __launch_bounds__(256) void k_trsv_fused(const T* A, size_t strideA, int lda, int n, T* X,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1))) void k_trsv_fused(const T* A, size_t strideA, int lda, int n, T* X,
                                                     size_t strideX, size_t xs, int* flags, int* tickets,
                                                     int epoch, const int* skip, int* err) {
     __shared__ cpx<T> u[64][65];
@@ -1209,7 +1240,7 @@ __global__ __launch_bounds__(256) void k_trsv_fused(const T* A, size_t strideA, 
     for (;;) {
         if (tid == 0) s_vid = atomicAdd(tickets + b, 1);
         __syncthreads();
-        const int vid = s_vid;
+        const int vid = __builtin_amdgcn_readfirstlane(s_vid);
         if (vid >= nblk) break;
         const int bk = LOWER ? vid : nblk - 1 - vid;
         const int r0 = bk * 64;
@@ -1220,15 +1251,17 @@ __global__ __launch_bounds__(256) void k_trsv_fused(const T* A, size_t strideA, 
         // diagonal block: all 16 loads per thread in flight at once (clamped
         // into the block, kept only where inside)
         {
+            // thread (row group g, column k) takes rows g + 4 u2
             cpx<T> v[16];
+            const int g = __builtin_amdgcn_readfirstlane(tid >> 6), k = tid & 63, kc = k < bs ? k : bs - 1;
     #pragma unroll
             for (int u2 = 0; u2 < 16; ++u2) {
-                const int e = tid + 256 * u2, ii = e >> 6, k = e & 63;
-                v[u2] = ld(Ab + 2 * ((size_t)(r0 + (ii < bs ? ii : bs - 1)) * lda + r0 + (k < bs ? k : bs - 1)));
+                const int ii = g + 4 * u2;
+                v[u2] = ld(Ab + 2 * ((size_t)(r0 + (ii < bs ? ii : bs - 1)) * lda + r0 + kc));
             }
     #pragma unroll
             for (int u2 = 0; u2 < 16; ++u2) {
-                const int e = tid + 256 * u2, ii = e >> 6, k = e & 63;
+                const int ii = g + 4 * u2;
                 if (ii < bs && k < bs) u[ii][k] = v[u2];
             }
         }
@@ -1253,17 +1286,23 @@ __global__ __launch_bounds__(256) void k_trsv_fused(const T* A, size_t strideA, 
             const int cs = n - c0 < 64 ? n - c0 : 64;
             const int jp = 2 * p < cs ? 2 * p : ((cs - 1) & ~1);
             if (bs == 64) {
-                // full block: wave-uniform row bases, one lane offset
-                const int off = 2 * (h * lda + jp);
+                // full block: a wave-uniform base (scalar registers) and one
+                // 32-bit lane offset per row pair
+                const char* rb = reinterpret_cast<const char*>(Ab + 2 * ((size_t)(r0 + 16 * w) * lda + c0));
+                const unsigned o = (unsigned)(2 * (h * lda + jp)) * (unsigned)sizeof(T);
+                const size_t rs = (size_t)(4 * lda) * sizeof(T);
     #pragma unroll
                 for (int k = 0; k < 8; ++k)
-                    ld_pair(Ab + 2 * ((size_t)(r0 + 16 * w + 2 * k) * lda + c0) + off, m[2 * k], m[2 * k + 1]);
+                    ld_pair(reinterpret_cast<const T*>((rb + k * rs) + o), m[2 * k], m[2 * k + 1]);
             } else {
+                // last row block: rows past it read its last row; the lane's
+                // first row and how far its rows may step are all it keeps
+                const int r1 = 16 * w + h < bs ? 16 * w + h : bs - 1;
+                const int lim = bs - 1 - r1;
+                const T* rl = Ab + 2 * ((size_t)(r0 + r1) * lda + c0 + jp);
     #pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    const int rr = 16 * w + 2 * k + h;
-                    ld_pair(Ab + 2 * ((size_t)(r0 + (rr < bs ? rr : bs - 1)) * lda + c0 + jp), m[2 * k], m[2 * k + 1]);
-                }
+                for (int k = 0; k < 8; ++k)
+                    ld_pair(rl + 2 * (size_t)(2 * k < lim ? 2 * k : lim) * lda, m[2 * k], m[2 * k + 1]);
             }
         };
         cpx<T> m[16], mn[16];
@@ -1289,7 +1328,7 @@ __global__ __launch_bounds__(256) void k_trsv_fused(const T* A, size_t strideA, 
             }
             __syncthreads();
             acquire_wg();
-            const int av = s_avail;
+            const int av = __builtin_amdgcn_readfirstlane(s_avail);
             // x of the published run, one round of loads
             for (int e = tid; e < (av - step) * 64; e += 256) {
                 const int c0 = blk(step + (e >> 6)) * 64, j = e & 63;
@@ -1338,7 +1377,10 @@ __global__ __launch_bounds__(256) void k_trsv_fused(const T* A, size_t strideA, 
         __syncthreads();
         // diagonal solve by wave 0 with x in registers (lane k holds x_k);
         // x_j reaches every lane by readlane, so the steps need no LDS round
-        // trip or barrier
+        // trip or barrier.  The sequential chain of a step is the readlane
+        // and the update: each lane's diagonal reciprocal (upper) is formed
+        // before the steps, and a lane's matrix entries are read from LDS
+        // eight steps at a time, the next eight in flight during these.
         if (tid < 64) {
             cpx<T> v = {0, 0};
             if (tid < bs) {
@@ -1346,22 +1388,48 @@ __global__ __launch_bounds__(256) void k_trsv_fused(const T* A, size_t strideA, 
                 v.r -= part[0][tid].r + part[1][tid].r;
                 v.i -= part[0][tid].i + part[1][tid].i;
             }
-            for (int jj = 0; jj < bs; ++jj) {
-                const int j = LOWER ? jj : bs - 1 - jj;
-                cpx<T> xj;
-                xj.r = lane_read(v.r, j);
-                xj.i = lane_read(v.i, j);
-                if (!LOWER) {
-                    const cpx<T> d = u[j][j];
-                    xj = (d.r == (T)0 && d.i == (T)0) ? xj : cmul(xj, cinv(d));
-                    if (tid == j) v = xj;
+            cpx<T> dinv = {(T)1, (T)0};
+            bool dz = true;
+            if (!LOWER && tid < bs) {
+                const cpx<T> d = u[tid][tid];
+                dz = d.r == (T)0 && d.i == (T)0;
+                if (!dz) dinv = cinv(d);
+            }
+            // column of step jj (clamped where jj is past the block: unused)
+            auto col = [&](int jj) { return jj < bs ? (LOWER ? jj : bs - 1 - jj) : 0; };
+            cpx<T> lc[8], ln[8];
+    #pragma unroll
+            for (int q = 0; q < 8; ++q) lc[q] = u[tid][col(q)];
+            for (int c0 = 0; c0 < bs; c0 += 8) {
+                if (c0 + 8 < bs) {
+    #pragma unroll
+                    for (int q = 0; q < 8; ++q) ln[q] = u[tid][col(c0 + 8 + q)];
                 }
-                const bool act = LOWER ? (tid > j && tid < bs) : (tid < j);
-                if (act) {
-                    const cpx<T> l = u[tid][j];
-                    v.r -= l.r * xj.r - l.i * xj.i;
-                    v.i -= l.r * xj.i + l.i * xj.r;
+    #pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    if (c0 + q < bs) {
+                        const int j = col(c0 + q);
+                        cpx<T> xj;
+                        if (LOWER) {
+                            xj.r = lane_read(v.r, j);
+                            xj.i = lane_read(v.i, j);
+                        } else {
+                            // x_j = v_j / d_j, formed in lane j
+                            const cpx<T> wv = dz ? v : cmul(v, dinv);
+                            xj.r = lane_read(wv.r, j);
+                            xj.i = lane_read(wv.i, j);
+                            if (tid == j) v = xj;
+                        }
+                        const bool act = LOWER ? (tid > j && tid < bs) : (tid < j);
+                        if (act) {
+                            const cpx<T> l = lc[q];
+                            v.r -= l.r * xj.r - l.i * xj.i;
+                            v.i -= l.r * xj.i + l.i * xj.r;
+                        }
+                    }
                 }
+    #pragma unroll
+                for (int q = 0; q < 8; ++q) lc[q] = ln[q];
             }
             if (tid < bs) xb[tid] = v;
         }
@@ -1577,6 +1645,13 @@ __global__ void k_ident_ipiv(int* ipiv, int n) {
     if (i < n) ipiv[(size_t)blockIdx.y * n + i] = i + 1;
 }
 
+// perm = identity: the row permutation of a factorisation without
+// interchanges (what k_make_perm makes of an identity ipiv, in parallel)
+__global__ void k_ident_perm(int* perm, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) perm[(size_t)blockIdx.y * n + i] = i;
+}
+
 // The first failure of a sequence of stream and event calls (launch
 // errors surface through hipGetLastError at the end of the sequence).
 struct HipFirst {
@@ -1746,6 +1821,11 @@ size_t lu_perm_lds_limit() {
         hipDeviceGetAttribute(&v, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess || v <= 0)
         return 65536;
     return (size_t)v;
+}
+
+hipError_t lu_ident_perm(int n, int batch, int* perm, hipStream_t s) {
+    hipLaunchKernelGGL(k_ident_perm, dim3((n + 255) / 256, batch), dim3(256), 0, s, perm, n);
+    return hipGetLastError();
 }
 
 hipError_t lu_make_perm(const int* ipiv, int n, int batch, int* perm, hipStream_t s) {

@@ -61,57 +61,73 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ A64
 // ------------------------------------------------ symmetric residual ---
 // For Z = Z^T (bit-exact: the record fill mirrors its tiles) R = V - Z x
 // reads only the 64 x 64 tiles on and above the diagonal, half the bytes.
-// k_symv_strips: one block per (matrix b, 64-row strip I) walks the tiles
-// (I, J >= I); it adds Z_IJ x_J to the strip's rows (kept in registers,
-// summed over the lanes at the end, written to Y) and, for J > I, writes
-// Z_IJ^T x_I, the contribution to the rows of strip J, as the partial
-// P[b][I][64J..64J+63].  k_symv_finish sums Y and the partials of the
-// strips above in a fixed order (no atomics: the result does not depend on
-// scheduling) and forms R and ||R||^2.  Thread (wave w, lane l) owns column
-// l of every tile and rows w + 4k, k < 16, of the strip, so each load
-// instruction of a wave reads one 1 KB tile row.
-constexpr int kSymRows = 16;
+// k_symv_strips: one block per (matrix b, 32-row strip S) walks the 64 x 64
+// tiles J >= D = S/2 (the diagonal block D is stored whole): it adds
+// Z_SJ x_J to the strip's rows (kept in registers, summed over the lanes at
+// the end, written to Y) and, for J > D, writes Z_SJ^T x_S, the
+// contribution to the rows of block J, as the partial P[b][S][64J..64J+63].
+// k_symv_finish sums Y and the partials of the strips above in a fixed
+// order (no atomics: the result does not depend on scheduling) and forms R
+// and ||R||^2.  Thread (wave w, lane l) owns column l of every tile and rows
+// w + 4k, k < 8, of the strip, so each load instruction of a wave reads one
+// 1 KB tile row.
+//
+// The kernel runs beside the trailing-update GEMM of the next batch, whose
+// waves take 128 registers each (four per SIMD): a wave here must stay
+// within the fourth wave's 128 registers, or the GEMM keeps only two waves
+// on every SIMD this kernel shares.  Hence 32-row strips (eight
+// accumulators per thread) and the strip's x in LDS.
+constexpr int kSymRows = 8;
+constexpr int kSymStrip = 4 * kSymRows;
 
-__global__ __launch_bounds__(256) void k_symv_strips(const double* __restrict__ A64, size_t strideA, int n, int lda,
-                                                     const double* __restrict__ X, double* __restrict__ Y,
-                                                     double* __restrict__ P, const int* __restrict__ done) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_symv_strips(
+    const double* __restrict__ A64, size_t strideA, int n, int lda, const double* __restrict__ X,
+    double* __restrict__ Y, double* __restrict__ P, const int* __restrict__ done) {
     __shared__ double2 part[2][4][64];
-    const int b = blockIdx.x, I = blockIdx.y;
+    __shared__ double2 xs[kSymStrip];
+    const int b = blockIdx.x, S = blockIdx.y;
     if (done[b]) return;
-    const int nblk = (n + 63) / 64;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nblk = (n + 63) / 64, ns = (n + kSymStrip - 1) / kSymStrip;
+    // w is wave-uniform: the strip's row addresses stay in scalar registers
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const double2* Ab = reinterpret_cast<const double2*>(A64 + (size_t)b * strideA);
     const double2* xb = reinterpret_cast<const double2*>(X) + (size_t)b * n;
-    const int r0 = 64 * I;
-    double2 xr[kSymRows], acc[kSymRows];
-#pragma unroll
-    for (int k = 0; k < kSymRows; ++k) {
-        const int r = r0 + w + 4 * k;
+    const int r0 = kSymStrip * S, D = r0 / 64;
+    if (threadIdx.x < kSymStrip) {
+        const int r = r0 + threadIdx.x;
         const double2 v = xb[r < n ? r : n - 1];
-        xr[k] = r < n ? v : make_double2(0.0, 0.0);
-        acc[k] = make_double2(0.0, 0.0);
+        xs[threadIdx.x] = r < n ? v : make_double2(0.0, 0.0);
     }
-    for (int J = I; J < nblk; ++J) {
+    __syncthreads();
+    double2 acc[kSymRows];
+#pragma unroll
+    for (int k = 0; k < kSymRows; ++k) acc[k] = make_double2(0.0, 0.0);
+    // tile J: acc += Z_SJ x_J; with the transposed product (J > D) also
+    // Z_SJ^T x_S, summed over the waves into P
+    auto tile = [&](int J, bool tr) {
         const int col = 64 * J + lane;
         const int cc = col < n ? col : n - 1;
         // all loads unconditional from clamped addresses, masked afterwards
         const double2 xv = xb[cc];
+        const double2 xj = col < n ? xv : make_double2(0.0, 0.0);
         double2 z[kSymRows];
 #pragma unroll
         for (int k = 0; k < kSymRows; ++k) {
             const int r = r0 + w + 4 * k;
             z[k] = Ab[(size_t)(r < n ? r : n - 1) * lda + cc];
         }
-        const double2 xj = col < n ? xv : make_double2(0.0, 0.0);
         double pr = 0.0, pi = 0.0;
 #pragma unroll
         for (int k = 0; k < kSymRows; ++k) {
             acc[k].x += z[k].x * xj.x - z[k].y * xj.y;
             acc[k].y += z[k].x * xj.y + z[k].y * xj.x;
-            pr += z[k].x * xr[k].x - z[k].y * xr[k].y;
-            pi += z[k].x * xr[k].y + z[k].y * xr[k].x;
+            if (tr) {
+                const double2 xr = xs[w + 4 * k];
+                pr += z[k].x * xr.x - z[k].y * xr.y;
+                pi += z[k].x * xr.y + z[k].y * xr.x;
+            }
         }
-        if (J > I) {
+        if (tr) {
             // two buffers: wave 0 reads buffer J&1 before it reaches the
             // barrier of tile J+1, and J+2 writes it only after that barrier
             part[J & 1][w][lane] = make_double2(pr, pi);
@@ -123,10 +139,12 @@ __global__ __launch_bounds__(256) void k_symv_strips(const double* __restrict__ 
                     s.x += part[J & 1][q][lane].x;
                     s.y += part[J & 1][q][lane].y;
                 }
-                reinterpret_cast<double2*>(P)[((size_t)b * nblk + I) * n + col] = s;
+                reinterpret_cast<double2*>(P)[((size_t)b * ns + S) * n + col] = s;
             }
         }
-    }
+    };
+    tile(D, false);
+    for (int J = D + 1; J < nblk; ++J) tile(J, true);
 #pragma unroll
     for (int k = 0; k < kSymRows; ++k) {
         double sr = acc[k].x, si = acc[k].y;
@@ -140,9 +158,9 @@ __global__ __launch_bounds__(256) void k_symv_strips(const double* __restrict__ 
     }
 }
 
-// R[b][i] = V[b][i] - (Y[b][i] + sum over strips I < i/64 of P[b][I][i]),
-// in place over Y; rn2[b] += ||R||^2 (one atomic per block).  xZero: x = 0,
-// so R = V without reading Y or P.
+// R[b][i] = V[b][i] - (Y[b][i] + sum over the strips S of the blocks above
+// row i's block of P[b][S][i]), in place over Y; rn2[b] += ||R||^2 (one
+// atomic per block).  xZero: x = 0, so R = V without reading Y or P.
 __global__ __launch_bounds__(256) void k_symv_finish(const double* __restrict__ V, size_t vStride, int n,
                                                      const double* __restrict__ P, double* __restrict__ R,
                                                      double* __restrict__ rn2, const int* __restrict__ done,
@@ -151,7 +169,7 @@ __global__ __launch_bounds__(256) void k_symv_finish(const double* __restrict__ 
     const int b = blockIdx.y;
     if (done[b]) return;
     const int i = blockIdx.x * 256 + threadIdx.x;
-    const int nblk = (n + 63) / 64;
+    const int ns = (n + kSymStrip - 1) / kSymStrip;
     double e = 0.0;
     if (i < n) {
         double2* y = reinterpret_cast<double2*>(R) + (size_t)b * n + i;
@@ -160,10 +178,10 @@ __global__ __launch_bounds__(256) void k_symv_finish(const double* __restrict__ 
             const double2 y0 = *y;
             sx = y0.x;
             sy = y0.y;
-            const double2* p = reinterpret_cast<const double2*>(P) + (size_t)b * nblk * n + i;
-            const int nI = i / 64;
-            for (int I = 0; I < nI; ++I) {
-                const double2 q = p[(size_t)I * n];
+            const double2* p = reinterpret_cast<const double2*>(P) + (size_t)b * ns * n + i;
+            const int nS = (i / 64) * (64 / kSymStrip);
+            for (int S = 0; S < nS; ++S) {
+                const double2 q = p[(size_t)S * n];
                 sx += q.x;
                 sy += q.y;
             }
@@ -274,14 +292,16 @@ hipError_t launch_residual_sym(const double* A64, size_t strideA, int n, int lda
                                const double* X, double* R, double* P, double* rn2, const int* done, int batch,
                                int xZero, hipStream_t s) {
     if (!xZero)
-        hipLaunchKernelGGL(k_symv_strips, dim3(batch, (n + 63) / 64), dim3(256), 0, s, A64, strideA, n, lda, X, R,
-                           P, done);
+        hipLaunchKernelGGL(k_symv_strips, dim3(batch, (n + kSymStrip - 1) / kSymStrip), dim3(256), 0, s, A64,
+                           strideA, n, lda, X, R, P, done);
     hipLaunchKernelGGL(k_symv_finish, dim3((n + 255) / 256, batch), dim3(256), 0, s, V, vStride, n, P, R, rn2, done,
                        xZero);
     return hipGetLastError();
 }
 
-size_t residual_sym_bytes(int n, int batch) { return sizeof(double) * 2 * (size_t)batch * ((n + 63) / 64) * n; }
+size_t residual_sym_bytes(int n, int batch) {
+    return sizeof(double) * 2 * (size_t)batch * ((n + kSymStrip - 1) / kSymStrip) * n;
+}
 
 hipError_t launch_ir_check(double* rn2, const double* vn2, double tol, int batch, int* done, int* iters,
                            double* resid, int firstPass, int lastPass, int remaining, hipStream_t s) {
