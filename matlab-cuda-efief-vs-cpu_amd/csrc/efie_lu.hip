@@ -1282,6 +1282,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
         // the block are zeroed where the values are used, so no select follows
         // the loads (that would make the prefetch wait for them).
         auto load_m = [&](int st, cpx<T>* m) {
+            // an opaque zero: the lane offsets below are formed here, at
+            // each call, and not hoisted out of the loops into registers
+            // held for the whole kernel
+            int z0 = 0;
+            asm volatile("" : "+s"(z0));
             const int c0 = blk(st) * 64;
             const int cs = n - c0 < 64 ? n - c0 : 64;
             const int jp = 2 * p < cs ? 2 * p : ((cs - 1) & ~1);
@@ -1289,7 +1294,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
                 // full block: a wave-uniform base (scalar registers) and one
                 // 32-bit lane offset per row pair
                 const char* rb = reinterpret_cast<const char*>(Ab + 2 * ((size_t)(r0 + 16 * w) * lda + c0));
-                const unsigned o = (unsigned)(2 * (h * lda + jp)) * (unsigned)sizeof(T);
+                const unsigned o = (unsigned)(2 * (h * lda + jp + z0)) * (unsigned)sizeof(T);
                 const size_t rs = (size_t)(4 * lda) * sizeof(T);
     #pragma unroll
                 for (int k = 0; k < 8; ++k)
@@ -1297,7 +1302,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
             } else {
                 // last row block: rows past it read its last row; the lane's
                 // first row and how far its rows may step are all it keeps
-                const int r1 = 16 * w + h < bs ? 16 * w + h : bs - 1;
+                const int r1 = 16 * w + h + z0 < bs ? 16 * w + h + z0 : bs - 1;
                 const int lim = bs - 1 - r1;
                 const T* rl = Ab + 2 * ((size_t)(r0 + r1) * lda + c0 + jp);
     #pragma unroll
@@ -1395,8 +1400,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
                 dz = d.r == (T)0 && d.i == (T)0;
                 if (!dz) dinv = cinv(d);
             }
-            // column of step jj (clamped where jj is past the block: unused)
-            auto col = [&](int jj) { return jj < bs ? (LOWER ? jj : bs - 1 - jj) : 0; };
+            // column of step jj (clamped where jj is past the block: unused);
+            // the opaque zero keeps these LDS reads from being hoisted above
+            // the update loop, where their registers would be held
+            int z1 = 0;
+            asm volatile("" : "+s"(z1));
+            auto col = [&](int jj) { return (jj < bs ? (LOWER ? jj : bs - 1 - jj) : 0) + z1; };
             cpx<T> lc[8], ln[8];
     #pragma unroll
             for (int q = 0; q < 8; ++q) lc[q] = u[tid][col(q)];
