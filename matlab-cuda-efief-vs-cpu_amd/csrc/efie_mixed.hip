@@ -63,58 +63,70 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ A64
 // reads only the 64 x 64 tiles on and above the diagonal, half the bytes.
 // k_symv_strips: one block per (matrix b, 32-row strip S) walks the 64 x 64
 // tiles J >= D = S/2 (the diagonal block D is stored whole): it adds
-// Z_SJ x_J to the strip's rows (kept in registers, summed over the lanes at
-// the end, written to Y) and, for J > D, writes Z_SJ^T x_S, the
-// contribution to the rows of block J, as the partial P[b][S][64J..64J+63].
-// k_symv_finish sums Y and the partials of the strips above in a fixed
-// order (no atomics: the result does not depend on scheduling) and forms R
-// and ||R||^2.  Thread (wave w, lane l) owns column l of every tile and rows
-// w + 4k, k < 8, of the strip, so each load instruction of a wave reads one
-// 1 KB tile row.
+// Z_SJ x_J to the strip's rows (kept in registers, summed over the lanes
+// and waves at the end, written to Y) and, for J > D, writes Z_SJ^T x_S,
+// the contribution to the rows of block J, as the partial
+// P[b][S][64J..64J+63].  k_symv_finish sums Y and the partials of the
+// strips above in a fixed order (no atomics: the result does not depend on
+// scheduling) and forms R and ||R||^2.  Wave w owns columns 16w..16w+15 of
+// every tile and all 32 rows: lane (g, c) = (lane / 16, lane % 16) takes
+// column 16w + c and rows g + 4k, k < 8, so one load instruction reads four
+// 256-byte row segments, and a column's sum over the strip's rows stays
+// within the wave (no barrier per tile).
 //
 // The kernel runs beside the trailing-update GEMM of the next batch, whose
 // waves take 128 registers each (four per SIMD): a wave here must stay
 // within the fourth wave's 128 registers, or the GEMM keeps only two waves
-// on every SIMD this kernel shares.  Hence 32-row strips (eight
-// accumulators per thread) and the strip's x in LDS.
+// on every SIMD this kernel shares; hence 32-row strips (eight rows per
+// thread).
 constexpr int kSymRows = 8;
 constexpr int kSymStrip = 4 * kSymRows;
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_symv_strips(
     const double* __restrict__ A64, size_t strideA, int n, int lda, const double* __restrict__ X,
     double* __restrict__ Y, double* __restrict__ P, const int* __restrict__ done) {
-    __shared__ double2 part[2][4][64];
-    __shared__ double2 xs[kSymStrip];
+    __shared__ double2 red[4][kSymStrip];
     const int b = blockIdx.x, S = blockIdx.y;
     if (done[b]) return;
     const int nblk = (n + 63) / 64, ns = (n + kSymStrip - 1) / kSymStrip;
-    // w is wave-uniform: the strip's row addresses stay in scalar registers
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int g = lane >> 4, c = lane & 15;
     const double2* Ab = reinterpret_cast<const double2*>(A64 + (size_t)b * strideA);
     const double2* xb = reinterpret_cast<const double2*>(X) + (size_t)b * n;
     const int r0 = kSymStrip * S, D = r0 / 64;
-    if (threadIdx.x < kSymStrip) {
-        const int r = r0 + threadIdx.x;
-        const double2 v = xb[r < n ? r : n - 1];
-        xs[threadIdx.x] = r < n ? v : make_double2(0.0, 0.0);
-    }
-    __syncthreads();
-    double2 acc[kSymRows];
+    double2 xr[kSymRows], acc[kSymRows];
 #pragma unroll
-    for (int k = 0; k < kSymRows; ++k) acc[k] = make_double2(0.0, 0.0);
+    for (int k = 0; k < kSymRows; ++k) {
+        const int r = r0 + g + 4 * k;
+        const double2 v = xb[r < n ? r : n - 1];
+        xr[k] = r < n ? v : make_double2(0.0, 0.0);
+        acc[k] = make_double2(0.0, 0.0);
+    }
     // tile J: acc += Z_SJ x_J; with the transposed product (J > D) also
-    // Z_SJ^T x_S, summed over the waves into P
+    // Z_SJ^T x_S into P
     auto tile = [&](int J, bool tr) {
-        const int col = 64 * J + lane;
+        const int col = 64 * J + 16 * w + c;
         const int cc = col < n ? col : n - 1;
         // all loads unconditional from clamped addresses, masked afterwards
         const double2 xv = xb[cc];
         const double2 xj = col < n ? xv : make_double2(0.0, 0.0);
         double2 z[kSymRows];
+        if (r0 + kSymStrip <= n) {
+            // whole strip inside: one lane address, rows a uniform stride apart
+            const double2* zp = Ab + (size_t)(r0 + g) * lda + cc;
 #pragma unroll
-        for (int k = 0; k < kSymRows; ++k) {
-            const int r = r0 + w + 4 * k;
-            z[k] = Ab[(size_t)(r < n ? r : n - 1) * lda + cc];
+            for (int k = 0; k < kSymRows; ++k) z[k] = zp[(size_t)(4 * k) * lda];
+        } else {
+            // last strip: rows past the matrix read its last row (an opaque
+            // zero keeps the clamped offsets out of registers held across
+            // the tiles)
+            int z0 = 0;
+            asm volatile("" : "+s"(z0));
+#pragma unroll
+            for (int k = 0; k < kSymRows; ++k) {
+                const int r = r0 + g + 4 * k + z0;
+                z[k] = Ab[(size_t)(r < n ? r : n - 1) * lda + cc];
+            }
         }
         double pr = 0.0, pi = 0.0;
 #pragma unroll
@@ -122,39 +134,40 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             acc[k].x += z[k].x * xj.x - z[k].y * xj.y;
             acc[k].y += z[k].x * xj.y + z[k].y * xj.x;
             if (tr) {
-                const double2 xr = xs[w + 4 * k];
-                pr += z[k].x * xr.x - z[k].y * xr.y;
-                pi += z[k].x * xr.y + z[k].y * xr.x;
+                pr += z[k].x * xr[k].x - z[k].y * xr[k].y;
+                pi += z[k].x * xr[k].y + z[k].y * xr[k].x;
             }
         }
         if (tr) {
-            // two buffers: wave 0 reads buffer J&1 before it reaches the
-            // barrier of tile J+1, and J+2 writes it only after that barrier
-            part[J & 1][w][lane] = make_double2(pr, pi);
-            __syncthreads();
-            if (w == 0 && col < n) {
-                double2 s = part[J & 1][0][lane];
-#pragma unroll
-                for (int q = 1; q < 4; ++q) {
-                    s.x += part[J & 1][q][lane].x;
-                    s.y += part[J & 1][q][lane].y;
-                }
-                reinterpret_cast<double2*>(P)[((size_t)b * ns + S) * n + col] = s;
-            }
+            // over the four row groups of the column: (g0 + g1) + (g2 + g3)
+            pr += __shfl_xor(pr, 16, 64);
+            pi += __shfl_xor(pi, 16, 64);
+            pr += __shfl_xor(pr, 32, 64);
+            pi += __shfl_xor(pi, 32, 64);
+            if (g == 0 && col < n) reinterpret_cast<double2*>(P)[((size_t)b * ns + S) * n + col] = make_double2(pr, pi);
         }
     };
     tile(D, false);
     for (int J = D + 1; J < nblk; ++J) tile(J, true);
+    // row sums: over the 16 columns of the wave, then over the waves
 #pragma unroll
     for (int k = 0; k < kSymRows; ++k) {
         double sr = acc[k].x, si = acc[k].y;
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
+        for (int o = 1; o < 16; o <<= 1) {
             sr += __shfl_xor(sr, o, 64);
             si += __shfl_xor(si, o, 64);
         }
-        const int r = r0 + w + 4 * k;
-        if (lane == 0 && r < n) reinterpret_cast<double2*>(Y)[(size_t)b * n + r] = make_double2(sr, si);
+        if (c == 0) red[w][g + 4 * k] = make_double2(sr, si);
+    }
+    __syncthreads();
+    if (threadIdx.x < kSymStrip) {
+        const int r = r0 + threadIdx.x;
+        const double2 a0 = red[0][threadIdx.x], a1 = red[1][threadIdx.x], a2 = red[2][threadIdx.x],
+                      a3 = red[3][threadIdx.x];
+        if (r < n)
+            reinterpret_cast<double2*>(Y)[(size_t)b * n + r] =
+                make_double2((a0.x + a1.x) + (a2.x + a3.x), (a0.y + a1.y) + (a2.y + a3.y));
     }
 }
 
