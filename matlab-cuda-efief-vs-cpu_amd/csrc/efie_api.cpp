@@ -211,6 +211,8 @@ struct efie_ctx {
     // third stream: LU look-ahead (the next block's panels)
     hipStream_t panelStream = nullptr;
     hipEvent_t evPanel = nullptr, evAhead = nullptr;
+    // a pipelined factorisation's tail mark, per buffer (LuWork::tailMark)
+    hipEvent_t evTail[2] = {nullptr, nullptr};
     // fourth stream: refinement / back substitution of a batch while the
     // next batch is factored
     hipStream_t irStream = nullptr;
@@ -555,6 +557,13 @@ efie_status efie_create(efie_ctx** out, int device) {
             return fail(EFIE_E_HIP, "efie:hip", "hipEventCreate: %s", hipGetErrorString(e));
         }
     }
+    for (hipEvent_t* ev : {&c->evTail[0], &c->evTail[1]}) {
+        e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            efie_destroy(c);
+            return fail(EFIE_E_HIP, "efie:hip", "hipEventCreate: %s", hipGetErrorString(e));
+        }
+    }
     e = noalloc_scratch(device, &c->scratch);
     if (e != hipSuccess) {
         efie_destroy(c);
@@ -575,7 +584,7 @@ void efie_destroy(efie_ctx* c) {
         if (ev) (void)hipEventDestroy(ev);
     for (hipEvent_t ev : {c->evFill[0], c->evFill[1], c->evFillStart[0], c->evFillStart[1], c->evPanel, c->evAhead,
                           c->evLU[0], c->evLU[1], c->evLUStart[0], c->evLUStart[1], c->evIR[0], c->evIR[1],
-                          c->evIRStart[0], c->evIRStart[1]})
+                          c->evIRStart[0], c->evIRStart[1], c->evTail[0], c->evTail[1]})
         if (ev) (void)hipEventDestroy(ev);
     for (auto& ev : c->pool) (void)hipEventDestroy(ev);
     if (c->hStat) (void)hipHostFree(c->hStat);
@@ -2031,7 +2040,8 @@ BatchBufs batch_bufs(efie_ctx* c, const efie_opts& o, int n, int lda, int B, int
 // nElem (packed batches, symmetric factorisation): the order of each
 // matrix's problem, the identity padding beyond it skipped.
 efie_status factor_batch(efie_ctx* c, const efie_opts& o, int n, int lda, int nb, const Prof& pf, bool z32Ready,
-                         const BatchBufs& bb, const int* nElem = nullptr, hipStream_t head = nullptr) {
+                         const BatchBufs& bb, const int* nElem = nullptr, hipStream_t head = nullptr,
+                         hipEvent_t headWait = nullptr, hipEvent_t tailMark = nullptr) {
     hipStream_t s = c->stream;
     const size_t zElems = (size_t)n * lda;
     // look-ahead on the panel stream, except when per-kernel profiling
@@ -2043,6 +2053,8 @@ efie_status factor_batch(efie_ctx* c, const efie_opts& o, int n, int lda, int nb
     if (wk.noPivot) {
         wk.nElem = nElem;
         wk.head = head;
+        wk.headWait = head ? headWait : nullptr;
+        wk.tailMark = head ? tailMark : nullptr;
     }
     HIPCHK(launch_fill_int(bb.info, 0, nb, s));
     if (o.precision != EFIE_PREC_MIXED) {
@@ -2473,7 +2485,8 @@ efie_status efie_sweep(efie_ctx* c, const double* freqs, int Nf, double mu, doub
         HIPCHK(hipEventRecord(c->evLUStart[buf], s));
         // the first panels of a batch on the fill stream, behind its fill:
         // beside the previous batch's last factorisation steps
-        st = factor_batch(c, o, n, lda, nb, pf, have32[buf], bb, nullptr, overlap ? sf : nullptr);
+        st = factor_batch(c, o, n, lda, nb, pf, have32[buf], bb, nullptr, overlap ? sf : nullptr,
+                          bi > 0 ? c->evTail[(bi - 1) & 1] : nullptr, c->evTail[bi & 1]);
         if (st) return st;
         HIPCHK(hipEventRecord(c->evLU[buf], s));
         HIPCHK(hipStreamWaitEvent(si, c->evLU[buf], 0));
@@ -2971,7 +2984,8 @@ efie_status efie_sweep_many(efie_ctx* c, const efie_sweep_item* items, int count
         const BatchBufs bb = bufs(buf);
         HIPCHK(hipStreamWaitEvent(s, c->evFill[buf], 0));
         HIPCHK(hipEventRecord(c->evLUStart[buf], s));
-        st = factor_batch(c, o, b.np, b.lda, b.nb, pf, true, bb, nebuf(buf), overlap ? sf : nullptr);
+        st = factor_batch(c, o, b.np, b.lda, b.nb, pf, true, bb, nebuf(buf), overlap ? sf : nullptr,
+                          bi > 0 ? c->evTail[(bi - 1) & 1] : nullptr, c->evTail[bi & 1]);
         if (st) return st;
         HIPCHK(hipEventRecord(c->evLU[buf], s));
         HIPCHK(hipStreamWaitEvent(si, c->evLU[buf], 0));

@@ -216,7 +216,16 @@ struct LuWork {
     // (the fill stream), so they can overlap the previous factorisation's
     // last steps on the main and panel streams; null: on the panel stream
     hipStream_t head = nullptr;
+    // with a head stream: the first block's update (and the second block's
+    // panels) also run there, after headWait (the previous factorisation's
+    // tailMark: its last kTailCols columns have begun, the latency-bound
+    // steps these fill); tailMark is recorded on the main stream at that
+    // point of this factorisation (either may be null)
+    hipEvent_t headWait = nullptr, tailMark = nullptr;
 };
+// columns of a factorisation's end that the next one's first update may
+// overlap
+constexpr int kTailCols = 2560;
 size_t lu_split_bytes(int n);
 // ints per batch element of LuWork::swaps
 int lu_swap_stride();

@@ -1692,12 +1692,35 @@ static hipError_t lu_factor_sym(T* A, size_t strideA, int n, int lda, int batch,
         chk(hipStreamWaitEvent(sp, wk.evAhead, 0));
     }
     panel_phase<T>(A, strideA, n, lda, batch, wk, 0, n < WB ? n : WB, 0, sh, pf);
+    // with a head stream the first block's update and the second block's
+    // panels run there too, in order, once the previous factorisation has
+    // reached its last (latency-bound) steps on s
+    int j0 = 0;
+    if (sh != sp && n > WB) {
+        const int jend = WB, jnext = n - jend < WB ? n : jend + WB;
+        if (wk.headWait) chk(hipStreamWaitEvent(sh, wk.headWait, 0));
+        pf->begin(7);
+        hipLaunchKernelGGL(k_sym_u12<T>, dim3((n - jend + 63) / 64, (WB + 31) / 32, batch), dim3(256), 0, sh, A,
+                           strideA, lda, 0, jend, jend, n, wk.nElem);
+        pf->end(7);
+        pf->begin(8);
+        pf->end(8, gemm<T>(A, strideA, lda, batch, jend, n, jend, jnext, 0, WB, sh, 1, wk.nElem));
+        panel_phase<T>(A, strideA, n, lda, batch, wk, jend, jnext, 0, sh, pf);
+        pf->begin(8);
+        pf->end(8, gemm<T>(A, strideA, lda, batch, jend, n, jnext, n, 0, WB, sh, 1, wk.nElem));
+        j0 = WB;
+    }
     if (sh != s) chk(hipEventRecord(wk.evPanel, sh));
-    for (int j0 = 0; j0 < n; j0 += WB) {
+    bool marked = false;
+    for (; j0 < n; j0 += WB) {
         const int wb = n - j0 < WB ? n - j0 : WB;
         const int jend = j0 + wb;
         const int jnext = n - jend < WB ? n : jend + WB;
-        if ((j0 == 0 ? sh : sp) != s) chk(hipStreamWaitEvent(s, wk.evPanel, 0));
+        if ((j0 == 0 || (j0 == WB && sh != sp) ? sh : sp) != s) chk(hipStreamWaitEvent(s, wk.evPanel, 0));
+        if (wk.tailMark && !marked && n - j0 <= kTailCols) {
+            chk(hipEventRecord(wk.tailMark, s));
+            marked = true;
+        }
         if (jend >= n) break;
         pf->begin(7);
         hipLaunchKernelGGL(k_sym_u12<T>, dim3((n - jend + 63) / 64, (wb + 31) / 32, batch), dim3(256), 0, s, A,
@@ -1714,6 +1737,7 @@ static hipError_t lu_factor_sym(T* A, size_t strideA, int n, int lda, int batch,
         pf->begin(8);
         pf->end(8, gemm<T>(A, strideA, lda, batch, jend, n, jnext, n, j0, wb, s, 1, wk.nElem));
     }
+    if (wk.tailMark && !marked) chk(hipEventRecord(wk.tailMark, s));
     return chk.e != hipSuccess ? chk.e : hipGetLastError();
 }
 
