@@ -538,7 +538,7 @@ def trace_states(path, window_gap_ms=0.0, range_ms=None):
 
 TRAFFIC_KEYS = {"k_gemm<float": "k_gemm", "k_gemm<double": "k_gemm_f64", "k_series_fill": "k_series_fill",
                 "k_symv_finish": "k_symv_finish", "k_symv_strips": "k_symv_strips",
-                "k_trsv_fused<float, false>": "k_trsv_fused_upper", "k_trsv_fused<float, true>": "k_trsv_fused_lower"}
+                "k_trsv_fused<float, false": "k_trsv_fused_upper", "k_trsv_fused<float, true": "k_trsv_fused_lower"}
 
 
 def pmc_summary(root, legs):

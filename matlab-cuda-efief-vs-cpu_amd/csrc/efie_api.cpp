@@ -2119,9 +2119,8 @@ efie_status refine_batch(efie_ctx* c, const efie_opts& o, int n, int lda, int nb
     for (int it = 0; it <= o.ir_max_iter; ++it) {
         pf.begin(EFIE_K_OTHER);
         if (symZ)
-            HIPCHK(launch_residual_sym(bb.Z, 2 * zElems, n, lda, V, vStride, w.X,
-                                       w.R, w.P, w.Rn2,
-                                       w.done, nb, it == 0 && o.lu_kernel == EFIE_LU_SYM, si));
+            HIPCHK(launch_residual_sym(bb.Z, 2 * zElems, n, lda, V, vStride, w.X, w.R, w.P, w.Rn2, w.done, nb,
+                                       it == 0 && o.lu_kernel == EFIE_LU_SYM, w.tw->rows > 0, si));
         else
             HIPCHK(launch_residual(bb.Z, 2 * zElems, n, lda, V, vStride, w.X, w.R,
                                    w.Rn2, w.done, nb, si));

@@ -301,10 +301,11 @@ hipError_t launch_residual(const double* A64, size_t strideA, int n, int lda, co
                            int batch, hipStream_t s);
 // The same for a complex symmetric A64 (A64 = A64^T bit-exact), reading
 // only its upper triangle; P holds residual_sym_bytes(n, batch) of partial
-// sums.  xZero: X = 0 (R = V, A64 and X not read).
+// sums.  xZero: X = 0 (R = V, A64 and X not read).  shared: the residual
+// runs beside a factorisation (narrow strips, few registers per wave).
 hipError_t launch_residual_sym(const double* A64, size_t strideA, int n, int lda, const double* V, size_t vStride,
                                const double* X, double* R, double* P, double* rn2, const int* done, int batch,
-                               int xZero, hipStream_t s);
+                               int xZero, bool shared, hipStream_t s);
 size_t residual_sym_bytes(int n, int batch);
 // remaining > 0 (a retry follows): also stop a system whose last reduction
 // factor, kept for the remaining steps, would not reach tol.

@@ -1221,8 +1221,11 @@ __device__ __forceinline__ double lane_read(double v, int j) {
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-template <typename T, bool LOWER>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1))) void k_trsv_fused(const T* A, size_t strideA, int lda, int n, T* X,
+// SHARED: the solve runs beside a factorisation, whose update GEMM holds
+// 128 registers per wave; the FP32 solve then keeps to 128 too (a few
+// spills), so the GEMM keeps three waves on the SIMDs it shares.
+template <typename T, bool LOWER, bool SHARED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && SHARED ? 4 : 1))) void k_trsv_fused(const T* A, size_t strideA, int lda, int n, T* X,
                                                     size_t strideX, size_t xs, int* flags, int* tickets,
                                                     int epoch, const int* skip, int* err) {
     __shared__ cpx<T> u[64][65];
@@ -1830,11 +1833,22 @@ hipError_t lu_trsv(const T* A, size_t strideA, int n, int lda, T* X, size_t stri
     ++tw->epoch;
     hipLaunchKernelGGL(k_zero_int, dim3((batch + 255) / 256), dim3(256), 0, s, tw->tickets, batch);
     const int nb = tw->rows > 0 && tw->rows < nblk ? tw->rows : nblk;
+    if constexpr (std::is_same<T, float>::value) {
+        if (tw->rows > 0) {
+            if (lower)
+                hipLaunchKernelGGL((k_trsv_fused<T, true, true>), dim3(batch, nb), dim3(256), 0, s, A, strideA, lda,
+                                   n, X, strideX, xs, tw->flags, tw->tickets, tw->epoch, skip, tw->err);
+            else
+                hipLaunchKernelGGL((k_trsv_fused<T, false, true>), dim3(batch, nb), dim3(256), 0, s, A, strideA, lda,
+                                   n, X, strideX, xs, tw->flags, tw->tickets, tw->epoch, skip, tw->err);
+            return hipGetLastError();
+        }
+    }
     if (lower)
-        hipLaunchKernelGGL((k_trsv_fused<T, true>), dim3(batch, nb), dim3(256), 0, s, A, strideA, lda, n, X,
+        hipLaunchKernelGGL((k_trsv_fused<T, true, false>), dim3(batch, nb), dim3(256), 0, s, A, strideA, lda, n, X,
                            strideX, xs, tw->flags, tw->tickets, tw->epoch, skip, tw->err);
     else
-        hipLaunchKernelGGL((k_trsv_fused<T, false>), dim3(batch, nb), dim3(256), 0, s, A, strideA, lda, n, X,
+        hipLaunchKernelGGL((k_trsv_fused<T, false, false>), dim3(batch, nb), dim3(256), 0, s, A, strideA, lda, n, X,
                            strideX, xs, tw->flags, tw->tickets, tw->epoch, skip, tw->err);
     return hipGetLastError();
 }

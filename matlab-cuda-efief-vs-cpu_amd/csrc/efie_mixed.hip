@@ -77,14 +77,16 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ A64
 // The kernel runs beside the trailing-update GEMM of the next batch, whose
 // waves take 128 registers each (four per SIMD): a wave here must stay
 // within the fourth wave's 128 registers, or the GEMM keeps only two waves
-// on every SIMD this kernel shares; hence 32-row strips (eight rows per
-// thread).
-constexpr int kSymRows = 8;
-constexpr int kSymStrip = 4 * kSymRows;
+// on every SIMD this kernel shares; hence 32-row strips (ROWS = 8 rows per
+// thread).  A residual with nothing beside it takes 64-row strips
+// (ROWS = 16): half the strips, partials and x reloads.
+constexpr int kSymRowsShared = 8, kSymRowsAlone = 16;
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_symv_strips(
-    const double* __restrict__ A64, size_t strideA, int n, int lda, const double* __restrict__ X,
-    double* __restrict__ Y, double* __restrict__ P, const int* __restrict__ done) {
+template <int ROWS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ROWS == kSymRowsShared ? 4 : 1))) void
+k_symv_strips(const double* __restrict__ A64, size_t strideA, int n, int lda, const double* __restrict__ X,
+              double* __restrict__ Y, double* __restrict__ P, const int* __restrict__ done) {
+    constexpr int kSymRows = ROWS, kSymStrip = 4 * ROWS;
     __shared__ double2 red[4][kSymStrip];
     const int b = blockIdx.x, S = blockIdx.y;
     if (done[b]) return;
@@ -177,7 +179,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 __global__ __launch_bounds__(256) void k_symv_finish(const double* __restrict__ V, size_t vStride, int n,
                                                      const double* __restrict__ P, double* __restrict__ R,
                                                      double* __restrict__ rn2, const int* __restrict__ done,
-                                                     int xZero) {
+                                                     int xZero, int kSymStrip) {
     __shared__ double red[4];
     const int b = blockIdx.y;
     if (done[b]) return;
@@ -303,17 +305,25 @@ hipError_t launch_residual(const double* A64, size_t strideA, int n, int lda, co
 
 hipError_t launch_residual_sym(const double* A64, size_t strideA, int n, int lda, const double* V, size_t vStride,
                                const double* X, double* R, double* P, double* rn2, const int* done, int batch,
-                               int xZero, hipStream_t s) {
-    if (!xZero)
-        hipLaunchKernelGGL(k_symv_strips, dim3(batch, (n + kSymStrip - 1) / kSymStrip), dim3(256), 0, s, A64,
-                           strideA, n, lda, X, R, P, done);
+                               int xZero, bool shared, hipStream_t s) {
+    const int strip = 4 * (shared ? kSymRowsShared : kSymRowsAlone);
+    if (!xZero) {
+        if (shared)
+            hipLaunchKernelGGL(k_symv_strips<kSymRowsShared>, dim3(batch, (n + strip - 1) / strip), dim3(256), 0, s,
+                               A64, strideA, n, lda, X, R, P, done);
+        else
+            hipLaunchKernelGGL(k_symv_strips<kSymRowsAlone>, dim3(batch, (n + strip - 1) / strip), dim3(256), 0, s,
+                               A64, strideA, n, lda, X, R, P, done);
+    }
     hipLaunchKernelGGL(k_symv_finish, dim3((n + 255) / 256, batch), dim3(256), 0, s, V, vStride, n, P, R, rn2, done,
-                       xZero);
+                       xZero, strip);
     return hipGetLastError();
 }
 
+// sized for the narrower (shared) strips
 size_t residual_sym_bytes(int n, int batch) {
-    return sizeof(double) * 2 * (size_t)batch * ((n + kSymStrip - 1) / kSymStrip) * n;
+    const int strip = 4 * kSymRowsShared;
+    return sizeof(double) * 2 * (size_t)batch * ((n + strip - 1) / strip) * n;
 }
 
 hipError_t launch_ir_check(double* rn2, const double* vn2, double tol, int batch, int* done, int* iters,
