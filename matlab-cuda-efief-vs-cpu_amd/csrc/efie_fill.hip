@@ -1680,9 +1680,11 @@ __global__ __launch_bounds__(256) void k_series_coeffs(const double* __restrict_
 //   Z(m,n) = -eta/(4 pi) e^{-jk R0e} z^{-1} sum_j z^j D[j]
 //          = h e^{-jk R0e} sum_j z^j D[j],  h = -alpha/(4 pi k^2) (Freq.hr/hi),
 // written with its mirror (through LDS) to matrix f of the batch (FP64, and
-// FP32 when Z32 is set).
+// FP32 when Z32 is set).  It runs beside the previous batch's factorisation,
+// so it keeps to 128 registers per wave (the update GEMM's share: the GEMM
+// then keeps three waves on the SIMDs it shares).
 template <int NT, bool LOSSY>
-__global__ __launch_bounds__(256) void k_series_fill(const double* __restrict__ C, int Ne,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_series_fill(const double* __restrict__ C, int Ne,
                                                      const Freq* __restrict__ fr, int nf, double* __restrict__ Z,
                                                      size_t ld, size_t zStride, float* __restrict__ Z32,
                                                      size_t ld32, size_t z32Stride, int nblk, int tri) {
