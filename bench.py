@@ -379,9 +379,16 @@ def bench_c4(a):
             for c, (g, V, F, r, pe, ge, sp) in zip(ctxs, probs):
                 c.set_problem(V, F, r)
                 items.append(dict(ctx=c, freqs=fr, patchEdge=pe, groundEdge=ge, V0=1.0, spread=sp, Z0=50.0,
-                                  release=1))
+                                  release=0 if a.c4_reuse else 1))
             res, st = efie.sweep_many(eng, items, W.MU0, W.EPS0, o, stats=True)
             t_all = time.perf_counter() - t0
+            if a.c4_reuse:
+                # the same call again on the same contexts: series kept, so
+                # no builds (what the builds cost inside the pipeline)
+                t1 = time.perf_counter()
+                efie.sweep_many(eng, items, W.MU0, W.EPS0, o)
+                t_re = time.perf_counter() - t1
+                extra["series_kept"] = {"ms": 1e3 * t_re, "value": len(fr) * len(probs) / t_re}
             for z, _ in res:
                 if not np.all(np.isfinite(z)):
                     raise RuntimeError("non-finite Zin in the c4 sweep")
@@ -598,6 +605,8 @@ def main():
     ap.add_argument("--ir-tol", type=float, default=1e-10)
     ap.add_argument("--ir-max-iter", type=int, default=16, help="refinement corrections at most (the library default)")
     ap.add_argument("--batch", type=int, default=0, help="frequencies per batch (0: the library's choice)")
+    ap.add_argument("--c4-reuse", action="store_true",
+                    help="c4 many mode: also time a second call on the same contexts (series already built)")
     ap.add_argument("--c4-mode", choices=["many", "per-geometry"], default="many",
                     help="c4: one efie_sweep_many call (default) or one efie_sweep per geometry")
     ap.add_argument("--c4-max-n", type=int, default=0,
