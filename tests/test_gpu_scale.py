@@ -76,6 +76,31 @@ def test_c2_6k_sweep_vs_oracle_golden(efie_mod, ctx, c2, path):
         assert st["ir_resid_max"] <= 1e-10
 
 
+def test_c2_6k_pipelined_refinement_forms(efie_mod, ctx, c2):
+    """Three pipelined batches of four frequencies: the first two are refined
+    beside the next batch's factorisation (32-row residual strips, solves
+    held to 128 registers), the last alone (64-row strips, uncapped solves),
+    and the next batch's first update overlaps each factorisation's tail.
+    A profiled sweep (one stream, every kernel alone) takes the wide forms
+    throughout.  Zin at the oracle's three frequencies (one in each batch)
+    against the golden solution, and every frequency equal between the two
+    sweeps to the refinement accuracy."""
+    g, V, F, r, pe, ge, spread = c2
+    ctx.set_problem(V, F, r)
+    fz = [float(f) for f in g["zin_freqs"]]
+    fr = np.array([fz[0], 1.7e9, 1.8e9, 1.9e9, 2.3e9, fz[1], 2.5e9, 2.6e9, 3.2e9, 3.3e9, fz[2], 3.5e9])
+    at = [0, 5, 10]
+    o = dict(W.OPTS_C1, precision=efie_mod.PREC_MIXED, near_mode=efie_mod.NEAR_MOMENTS, ir_tol=1e-10, batch=4)
+    zp, _, sp = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0, efie_mod.opts(**o), stats=True)
+    zs, _, ss = ctx.sweep(fr, W.MU0, W.EPS0, pe, ge, 1.0, spread, 50.0, efie_mod.opts(**o, profile=1), stats=True)
+    for st in (sp, ss):
+        assert st["ir_resid_max"] <= 1e-10 and st["n_lu_retry"] == 0 and st["n_fp64_retry"] == 0
+    err = np.abs(zp[at] - g["zin"]) / np.abs(g["zin"])
+    print("pipelined vs oracle", err, "corrections", sp["ir_iters_total"], ss["ir_iters_total"], flush=True)
+    assert np.max(err) <= ZIN_TOL
+    assert np.max(np.abs(zp - zs) / np.abs(zs)) <= 1e-9
+
+
 @pytest.mark.parametrize("near_mode", [0, 1])
 def test_c2_6k_rows_vs_oracle_golden(efie_mod, ctx, c2, near_mode):
     g, V, F, r, pe, ge, spread = c2
